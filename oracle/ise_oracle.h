@@ -1,0 +1,112 @@
+/*
+ * ise_oracle.h -- CPU oracle for the batched canonicity test.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker that the HIP path is compared
+ * against; it is never linked into, loaded by or called from the product
+ * library.  Only tests/, the smoke check and the CPU-baseline leg of the
+ * benchmark may use it.
+ *
+ * It restates, in plain C over uint8 0/1 matrices, the reference's CPU
+ * algorithms:
+ *   - CCanonicityChecker::TestCanonicity   Source/CanonicityChecker.h:255-586
+ *     (non-combined designs: numParts == 1, rank == 2, no group on parts)
+ *   - CCanonicityChecker::checkColOrbit    Source/CanonicityChecker.cpp:191-223
+ *   - CCanonicityChecker::rowToChange      Source/CanonicityChecker.cpp:320-330
+ *   - CCanonicityChecker::constructColIndex Source/CanonicityChecker.cpp:332-342
+ *   - CGroupOrder::next_permutation         k-Factorization/Utils/Include/GroupOrder.h:136-248
+ *   - CGroupOrder::addAutomorphism / updateGroupOrder / udpdateStabLength
+ *                                           GroupOrder.h:56-102, Update_Orbits :30-49
+ *   - a depth-first row-by-row BIBD enumerator in the shape of
+ *     CEnumerator::Enumerate (Source/Enumerator.cpp:540-1024) with its own
+ *     row solver (the reference's InSysSolver is replaced by a direct DFS over
+ *     column-orbit counts)
+ *   - the k-factorization canonizer alldata::cnvCheckNew / canonizator /
+ *     cnvCheckKm1 / kmProcessMatrix (k-Factorization/TripleSys/Source/cnv.cpp,
+ *     kmSupport.cpp)
+ *
+ * Parity pins (see tests/): published design counts and |Aut| histograms,
+ * canonical matrices printed by the reference, brute-force canonical forms and
+ * automorphism-group orders of small matrices.
+ */
+#ifndef ISE_ORACLE_H
+#define ISE_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_ROWS 32
+#define ORC_MAX_COLS 64
+
+/* outInfo bits, same values as t_canonOutInfo (Source/CanonicityChecker.h:27-32) */
+#define ORC_SAVE_ROW_PERMUTATIONS 1u
+#define ORC_SAVE_COL_PERMUTATIONS 2u
+#define ORC_SAVE_ROW_TO_CHANGE    4u
+
+/* Per-candidate result arrays (structure of arrays, n entries each). */
+typedef struct {
+    uint8_t  *verdict;        /* [n]      1 = canonical */
+    uint8_t  *fail_row;       /* [n]      row where the permuted matrix won (255 if canonical) */
+    uint8_t  *fail_col;       /* [n]      first column of the deciding column orbit (255) */
+    uint8_t  *row_to_change;  /* [n]      rowToChange(fail_row) (255 if canonical) */
+    uint64_t *group_order;    /* [n]      |Aut| (row action) when ORC_SAVE_ROW_PERMUTATIONS */
+    uint8_t  *perm_row;       /* [n][32]  row permutation at exit (witness when non-canonical) */
+    uint8_t  *perm_col;       /* [n][64]  column permutation at exit */
+    uint8_t  *orbits;         /* [n][32]  row orbits of the automorphisms found */
+    uint16_t *n_gens;         /* [n]      automorphisms found (generators recorded) */
+    uint8_t  *gens;           /* [n][gen_cap][64] or NULL */
+    uint32_t *counters;       /* [n][4]   perms tried, rows compared, orbits compared, cells examined */
+} orc_results;
+
+/* Canonicity test of a batch of bit-packed candidates.
+ * rows/cells: [n][stride] uint64 words, bit c = column c.  cells[i] is the
+ * column-orbit start mask of row i (orbits induced by rows 0..i-1); when cells
+ * is NULL it is derived from the rows.  meta[c] = depth | ncols<<8 |
+ * lenStab<<16 | outInfo<<24.  Returns 0 on success, <0 on invalid input. */
+int orc_check_batch(uint32_t n, uint32_t stride, const uint64_t *rows,
+                    const uint64_t *cells, const uint32_t *meta,
+                    uint32_t gen_cap, orc_results *res);
+
+/* Single candidate on a row-major uint8 matrix (nrows x ncols, entries 0/1). */
+int orc_test_canonicity(const uint8_t *mat, int nrows, int ncols, int len_stab,
+                        unsigned out_info, orc_results *res, uint32_t gen_cap);
+
+/* Column-orbit start masks of a bit-packed matrix: out[i] = orbits of row i. */
+void orc_cell_masks(const uint64_t *rows, int nrows, int ncols, uint64_t *out);
+
+/* Depth-first enumeration of 2-(v,k,lambda) designs (b, r derived).
+ * Canonical full designs are written to designs[] as v packed row words each
+ * (up to max_designs) with |Aut| in auts[].  stats[0..3] = candidates tested,
+ * canonical partials, fully constructed matrices, designs.  Returns the number
+ * of designs found, or <0 on error.  If cand_rows is non-NULL every tested
+ * candidate with depth in [dmin,dmax] is appended (stride words each, up to
+ * cand_cap; the count goes to stats[4]). */
+long orc_enumerate_bibd(int v, int k, int lambda, uint64_t *designs,
+                        uint64_t *auts, long max_designs, uint64_t *stats,
+                        uint64_t *cand_rows, uint32_t *cand_meta, long cand_cap,
+                        int stride, int dmin, int dmax);
+
+/* Brute-force reference for tiny matrices: the lexicographically largest
+ * matrix over all row permutations (columns re-sorted in decreasing order)
+ * and the number of row permutations that map the matrix to itself. */
+int orc_bruteforce_canon(const uint8_t *mat, int nrows, int ncols,
+                         uint8_t *canon_out, uint64_t *n_row_auts);
+
+/* ---- k-factorizations of K_n (n <= 16): rounds of n/2 pairs ---- */
+/* Canonicity of a partial 1-factorization (nrounds x n bytes, each round the
+ * n players as n/2 ordered pairs).  Returns 1 canonical, 0 not, <0 error.
+ * On non-canonical, *witness receives the index of the first relabelling in
+ * plain order that produced a smaller normal form; n_auts counts
+ * relabellings that reproduced the matrix. */
+int orc_kf_check(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
+                 uint64_t *n_auts, uint64_t *n_tried);
+
+/* Normal form of a relabelled factorization (used by tests). */
+void orc_kf_normalize(uint8_t *mat, int n, int nrounds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,618 @@
+/*
+ * ise_oracle_bibd.c -- CPU oracle: canonicity test of (partial) incidence
+ * matrices and a depth-first BIBD enumerator.
+ *
+ * TEST INFRASTRUCTURE ONLY (see ise_oracle.h).  Works on an unpacked uint8
+ * matrix and walks column orbits one by one, the way the reference's scalar
+ * checker does, so that it is an independent formulation of the bit-mask /
+ * lane-parallel one used by the HIP kernel.
+ *
+ * The permutation generator keeps the reference's 8-bit index arithmetic
+ * (T = unsigned char, ELEMENT_MAX = 255, IDX_MAX = 254; Source/DataTypes.h:186-188),
+ * because the order in which row permutations are visited decides which
+ * witness is reported first and which automorphisms are found.
+ */
+#include "ise_oracle.h"
+#include <string.h>
+#include <stdlib.h>
+
+typedef unsigned char u8;
+#define EMAX ((u8)255)  /* ELEMENT_MAX */
+#define IMAX ((u8)254)  /* IDX_MAX     */
+
+/* Group-order bookkeeping of CGroupOrder (GroupOrder.h:51-107). */
+typedef struct {
+    unsigned short stab;      /* m_nStabLength    */
+    unsigned short stab_aut;  /* m_nStabLengthAut */
+    uint64_t order;           /* m_nGroupOrder    */
+} grp_t;
+
+/* Reverse perm[lo+1 .. hi-1]  (GroupOrder.h:25-28, `revert`). */
+static void reverse_tail(u8 *perm, u8 hi, u8 lo)
+{
+    while (++lo < --hi) { u8 t = perm[lo]; perm[lo] = perm[hi]; perm[hi] = t; }
+}
+
+/* Merge row orbits under an automorphism, starting at position `from`
+ * (GroupOrder.h:30-49).  Orbits are labelled by their smallest element. */
+static void merge_orbits(const u8 *perm, unsigned short len, u8 *orb, u8 from)
+{
+    u8 p = from;
+    do {
+        u8 a = orb[p], c = orb[perm[p]];
+        if (a == c) continue;
+        if (c < a) { u8 t = a; a = c; c = t; }
+        for (unsigned q = len; q--;)
+            if (orb[q] == c) orb[q] = a;
+    } while (++p < len);
+}
+
+/* Multiply |Aut| by the length of the orbit of the current stabiliser point
+ * (GroupOrder.h:63-73). */
+static void grow_group_order(grp_t *g, unsigned short degree, const u8 *orb)
+{
+    uint64_t len = 1;
+    unsigned short p = g->stab_aut;
+    while (++p < degree)
+        if (orb[p] == g->stab_aut) len++;
+    g->order *= len;
+}
+
+/* Record a new automorphism (GroupOrder.h:56-62, :76-79, :88-102). */
+static void record_automorphism(grp_t *g, unsigned short degree, const u8 *perm,
+                                u8 *orb, int row_permut)
+{
+    u8 first_moved = 0;
+    while (first_moved == perm[first_moved]) first_moved++;
+    if (row_permut && g->stab_aut > first_moved)
+        grow_group_order(g, degree, orb);
+    g->stab_aut = first_moved;
+    g->stab = first_moved;
+    merge_orbits(perm, degree, orb, first_moved);
+}
+
+/*
+ * Next row permutation fixing the first `fixed` rows, skipping images at the
+ * stabiliser position that are not orbit leaders (GroupOrder.h:136-248).
+ *   at == EMAX : plain lexicographic successor
+ *   at == IMAX : first call after an automorphism was recorded
+ *   at <  IMAX : the prefix perm[0..at] was refuted; change position `at`
+ * Returns the first position whose image changed, or EMAX when exhausted.
+ */
+static u8 next_row_perm(grp_t *g, u8 *perm, const u8 *orb, u8 n, u8 at, u8 fixed)
+{
+    u8 piv, succ, old;
+
+    if (at == IMAX && perm[g->stab] == n - 1)
+        at = EMAX;
+
+    if (at == IMAX) {
+        /* restart the search just after the stabiliser position */
+        piv = (u8)g->stab;
+        old = perm[piv];
+        at = piv;
+        for (succ = n; --succ > old;)
+            perm[succ] = succ;
+        for (u8 q = succ++; q-- > piv;)
+            perm[q + 1] = q;
+    } else if (at >= IMAX) {
+        succ = piv = n;
+        while (--piv > 0 && perm[piv - 1] >= perm[piv])
+            ;
+        if (piv == fixed)
+            return EMAX;
+        old = perm[--piv];
+        while (perm[--succ] <= old)
+            ;
+    } else {
+        piv = succ = at;
+        old = perm[piv];
+        while (++succ < n && perm[succ] <= old)
+            ;
+        if (succ >= n) {
+            reverse_tail(perm, n, piv);
+            return next_row_perm(g, perm, orb, n, EMAX, 0);
+        }
+    }
+
+    if (g->stab == piv) {
+        /* the image of the stabiliser point must be an orbit leader */
+        int step_back = 0;
+        u8 q = succ, img = perm[succ];
+        if (at >= IMAX) {
+            while (q > piv && orb[perm[q]] != perm[q])
+                q--;
+            if (q != succ) {
+                if (!q)
+                    return EMAX;
+                step_back = (q == piv);
+                img = perm[q--];
+                while (++q < succ)
+                    perm[q] = perm[q + 1];
+            }
+        } else {
+            while (q < n && orb[perm[q]] != perm[q])
+                q++;
+            if (q != succ) {
+                step_back = (q == n);
+                if (step_back) {
+                    if (!piv)
+                        return EMAX;
+                    q = at - 1;
+                    while (++q < succ)
+                        perm[q] = q;
+                } else {
+                    img = perm[q++];
+                    while (--q > succ)
+                        perm[q] = perm[q - 1];
+                }
+            }
+        }
+        perm[succ] = img;
+        if (step_back) {
+            succ = at >= IMAX ? n - 1 : piv;
+            old = perm[--piv];
+            g->stab = piv;
+        }
+    }
+
+    perm[piv] = perm[succ];
+    perm[succ] = old;
+    if (at >= IMAX) {
+        if (g->stab > piv)
+            g->stab = piv;
+        reverse_tail(perm, n, piv);
+    }
+    return piv;
+}
+
+/* Column-orbit start list of row `row` from its start mask. */
+static int orbit_starts(uint64_t mask, int ncols, u8 *starts, u8 *lens)
+{
+    int n = 0;
+    for (int c = 0; c < ncols; c++)
+        if ((mask >> c) & 1) starts[n++] = (u8)c;
+    for (int o = 0; o < n; o++)
+        lens[o] = (u8)((o + 1 < n ? starts[o + 1] : ncols) - starts[o]);
+    return n;
+}
+
+void orc_cell_masks(const uint64_t *rows, int nrows, int ncols, uint64_t *out)
+{
+    /* column c starts an orbit of row i when c == 0 or columns c-1 and c
+     * differ somewhere in rows 0..i-1 (ColOrbitManager.h:115-212 keeps the
+     * same partition as linked lists) */
+    uint64_t full = ncols >= 64 ? ~0ull : ((1ull << ncols) - 1);
+    uint64_t m = 1;
+    for (int i = 0; i < nrows; i++) {
+        out[i] = m & full;
+        uint64_t r = rows[i];
+        m |= (r ^ (r << 1)) & full;
+    }
+}
+
+/* One column orbit of the current row against the permuted row
+ * (CanonicityChecker.cpp:191-223, rank 2).  Returns +1 when the original has
+ * more ones (the permuted matrix is smaller), -1 when fewer (it is larger),
+ * 0 when equal; on 0 the orbit's columns of pcol are stably reordered so that
+ * the columns holding a one in the permuted row come first. */
+static int compare_orbit(int start, int len, const u8 *row, const u8 *prow, u8 *pcol)
+{
+    int ones_orig = 0, ones_perm = 0;
+    u8 ones[ORC_MAX_COLS], zeros[ORC_MAX_COLS];
+    int n1 = 0, n0 = 0;
+    for (int q = 0; q < len; q++) {
+        u8 col = pcol[start + q];
+        ones_orig += row[start + q];
+        if (prow[col]) ones[n1++] = col; else zeros[n0++] = col;
+    }
+    ones_perm = n1;
+    if (ones_orig != ones_perm)
+        return ones_orig > ones_perm ? 1 : -1;
+    for (int q = 0; q < n1; q++) pcol[start + q] = ones[q];
+    for (int q = 0; q < n0; q++) pcol[start + n1 + q] = zeros[q];
+    return 0;
+}
+
+static u8 row_to_change(const u8 *perm, u8 row)
+{
+    u8 best = row;
+    for (int q = row; q >= 0; q--)
+        if (perm[q] > best) best = perm[q];
+    return best;
+}
+
+typedef struct {
+    int nrows, ncols, len_stab;
+    unsigned out_info;
+    const u8 *mat;                 /* nrows x ncols */
+    u8 ostart[ORC_MAX_ROWS + 1][ORC_MAX_COLS];
+    u8 olen[ORC_MAX_ROWS + 1][ORC_MAX_COLS];
+    int norb[ORC_MAX_ROWS + 1];
+} cand_t;
+
+static int run_check(const cand_t *cd, orc_results *res, size_t idx, uint32_t gen_cap)
+{
+    const int d = cd->nrows, b = cd->ncols;
+    const int row_permut = (cd->out_info & ORC_SAVE_ROW_PERMUTATIONS) != 0;
+    u8 perm[ORC_MAX_ROWS], orb[ORC_MAX_ROWS], pcol[ORC_MAX_COLS];
+    grp_t g;
+    uint32_t perms = 0, rows_cmp = 0, orbs_cmp = 0, cells = 0;
+    uint16_t ngen = 0;
+    int verdict = 1;
+    u8 frow = EMAX, fcol = EMAX, rtc = EMAX;
+
+    for (int q = 0; q < d; q++) perm[q] = orb[q] = (u8)q;
+    for (int q = 0; q < b; q++) pcol[q] = (u8)q;
+    g.stab = (unsigned short)(d - 1);
+    g.stab_aut = EMAX;
+    g.order = 1;
+
+    u8 row = EMAX;
+    for (;;) {
+        row = next_row_perm(&g, perm, orb, (u8)d, row, (u8)cd->len_stab);
+        if (row == EMAX)
+            break;
+        perms++;
+        int refuted = 0;
+        for (; row < d; row++) {
+            const u8 *orig = cd->mat + (size_t)row * b;
+            const u8 *prow = cd->mat + (size_t)perm[row] * b;
+            rows_cmp++;
+            int diff = 0, o;
+            for (o = 0; o < cd->norb[row]; o++) {
+                int s = cd->ostart[row][o], len = cd->olen[row][o];
+                orbs_cmp++;
+                cells += (uint32_t)len;
+                if (len > 1)
+                    diff = compare_orbit(s, len, orig, prow, pcol);
+                else
+                    diff = (int)orig[s] - (int)prow[pcol[s]];
+                if (diff) break;
+            }
+            if (diff > 0) { refuted = 1; break; }
+            if (diff < 0) {
+                verdict = 0;
+                frow = row;
+                fcol = cd->ostart[row][o];
+                rtc = row_to_change(perm, row);
+                goto done;
+            }
+        }
+        if (refuted)
+            continue;
+        /* every row matched: perm is an automorphism */
+        if (!row_permut) {
+            /* permutation it induces on the column orbits of the next row
+             * (CanonicityChecker.h:427-450) */
+            if (res->gens && ngen < gen_cap) {
+                u8 *gp = res->gens + ((size_t)idx * gen_cap + ngen) * ORC_MAX_COLS;
+                memset(gp, 0xff, ORC_MAX_COLS);
+                u8 colidx[ORC_MAX_COLS];
+                memset(colidx, 0xff, sizeof colidx);
+                for (int o = 0; o < cd->norb[d]; o++) colidx[cd->ostart[d][o]] = (u8)o;
+                for (int o = 0; o < cd->norb[d]; o++) gp[o] = colidx[pcol[cd->ostart[d][o]]];
+            }
+        } else if (res->gens && ngen < gen_cap) {
+            u8 *gp = res->gens + ((size_t)idx * gen_cap + ngen) * ORC_MAX_COLS;
+            memset(gp, 0xff, ORC_MAX_COLS);
+            memcpy(gp, perm, (size_t)d);
+        }
+        ngen++;
+        record_automorphism(&g, (unsigned short)d, perm, orb, row_permut);
+        row = IMAX;
+    }
+    if (row_permut)
+        grow_group_order(&g, (unsigned short)d, orb);
+
+done:
+    if (res->verdict) res->verdict[idx] = (uint8_t)verdict;
+    if (res->fail_row) res->fail_row[idx] = frow;
+    if (res->fail_col) res->fail_col[idx] = fcol;
+    if (res->row_to_change) res->row_to_change[idx] = rtc;
+    if (res->group_order) res->group_order[idx] = verdict ? g.order : 0;
+    if (res->perm_row) {
+        memset(res->perm_row + idx * ORC_MAX_ROWS, 0xff, ORC_MAX_ROWS);
+        memcpy(res->perm_row + idx * ORC_MAX_ROWS, perm, (size_t)d);
+    }
+    if (res->perm_col) {
+        memset(res->perm_col + idx * ORC_MAX_COLS, 0xff, ORC_MAX_COLS);
+        memcpy(res->perm_col + idx * ORC_MAX_COLS, pcol, (size_t)b);
+    }
+    if (res->orbits) {
+        memset(res->orbits + idx * ORC_MAX_ROWS, 0xff, ORC_MAX_ROWS);
+        memcpy(res->orbits + idx * ORC_MAX_ROWS, orb, (size_t)d);
+    }
+    if (res->n_gens) res->n_gens[idx] = ngen;
+    if (res->counters) {
+        uint32_t *c = res->counters + idx * 4;
+        c[0] = perms; c[1] = rows_cmp; c[2] = orbs_cmp; c[3] = cells;
+    }
+    return verdict;
+}
+
+static void build_orbits(cand_t *cd, const uint64_t *cells_in, const uint64_t *rows)
+{
+    uint64_t masks[ORC_MAX_ROWS + 1];
+    if (cells_in) {
+        for (int i = 0; i < cd->nrows; i++) masks[i] = cells_in[i];
+        uint64_t full = cd->ncols >= 64 ? ~0ull : ((1ull << cd->ncols) - 1);
+        uint64_t r = rows[cd->nrows - 1];
+        masks[cd->nrows] = (masks[cd->nrows - 1] | (r ^ (r << 1))) & full;
+    } else {
+        uint64_t tmp[ORC_MAX_ROWS + 1];
+        uint64_t ext[ORC_MAX_ROWS + 1];
+        memcpy(ext, rows, sizeof(uint64_t) * cd->nrows);
+        ext[cd->nrows] = 0;
+        orc_cell_masks(ext, cd->nrows + 1, cd->ncols, tmp);
+        memcpy(masks, tmp, sizeof(uint64_t) * (cd->nrows + 1));
+    }
+    for (int i = 0; i <= cd->nrows; i++)
+        cd->norb[i] = orbit_starts(masks[i], cd->ncols, cd->ostart[i], cd->olen[i]);
+}
+
+int orc_check_batch(uint32_t n, uint32_t stride, const uint64_t *rows,
+                    const uint64_t *cells, const uint32_t *meta,
+                    uint32_t gen_cap, orc_results *res)
+{
+    static __thread cand_t cd;
+    static __thread u8 mat[ORC_MAX_ROWS * ORC_MAX_COLS];
+    for (uint32_t c = 0; c < n; c++) {
+        uint32_t m = meta[c];
+        int d = m & 0xff, b = (m >> 8) & 0xff;
+        if (d < 1 || d > ORC_MAX_ROWS || d > (int)stride || b < 1 || b > ORC_MAX_COLS)
+            return -1;
+        const uint64_t *rp = rows + (size_t)c * stride;
+        cd.nrows = d; cd.ncols = b;
+        cd.len_stab = (m >> 16) & 0xff;
+        cd.out_info = (m >> 24) & 0xff;
+        for (int i = 0; i < d; i++)
+            for (int q = 0; q < b; q++)
+                mat[i * b + q] = (u8)((rp[i] >> q) & 1);
+        cd.mat = mat;
+        build_orbits(&cd, cells ? cells + (size_t)c * stride : NULL, rp);
+        run_check(&cd, res, c, gen_cap);
+    }
+    return 0;
+}
+
+int orc_test_canonicity(const uint8_t *mat, int nrows, int ncols, int len_stab,
+                        unsigned out_info, orc_results *res, uint32_t gen_cap)
+{
+    if (nrows < 1 || nrows > ORC_MAX_ROWS || ncols < 1 || ncols > ORC_MAX_COLS)
+        return -1;
+    cand_t *cd = (cand_t *)malloc(sizeof(cand_t));
+    uint64_t rows[ORC_MAX_ROWS];
+    for (int i = 0; i < nrows; i++) {
+        rows[i] = 0;
+        for (int q = 0; q < ncols; q++)
+            if (mat[i * ncols + q]) rows[i] |= 1ull << q;
+    }
+    cd->nrows = nrows; cd->ncols = ncols; cd->len_stab = len_stab;
+    cd->out_info = out_info; cd->mat = mat;
+    build_orbits(cd, NULL, rows);
+    int v = run_check(cd, res, 0, gen_cap);
+    free(cd);
+    return v;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Brute force for tiny matrices.                                            */
+
+static int cmp_col_desc(const void *a, const void *b)
+{
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? 1 : (x > y ? -1 : 0);
+}
+
+/* Columns as integers with row 0 as the most significant bit, sorted in
+ * decreasing order, give the largest matrix for a fixed row order. */
+static void canon_for_rows(const uint8_t *mat, int nrows, int ncols, const int *rp, uint64_t *colv)
+{
+    for (int q = 0; q < ncols; q++) {
+        uint64_t x = 0;
+        for (int i = 0; i < nrows; i++)
+            x = (x << 1) | mat[rp[i] * ncols + q];
+        colv[q] = x;
+    }
+    qsort(colv, (size_t)ncols, sizeof(uint64_t), cmp_col_desc);
+}
+
+int orc_bruteforce_canon(const uint8_t *mat, int nrows, int ncols,
+                         uint8_t *canon_out, uint64_t *n_row_auts)
+{
+    if (nrows > 10 || ncols > 64) return -1;
+    int rp[16], c[16];
+    uint64_t best[64], cur[64], self[64];
+    int have = 0;
+    uint64_t auts = 0;
+    for (int i = 0; i < nrows; i++) { rp[i] = i; c[i] = 0; }
+    canon_for_rows(mat, nrows, ncols, rp, self);
+    /* Heap's algorithm over all row orders */
+    int i = 0;
+    for (;;) {
+        canon_for_rows(mat, nrows, ncols, rp, cur);
+        if (!have || memcmp(cur, best, sizeof(uint64_t) * ncols) != 0) {
+            int better = !have;
+            for (int q = 0; q < ncols && !better; q++) {
+                if (cur[q] != best[q]) { better = cur[q] > best[q]; break; }
+            }
+            if (better) { memcpy(best, cur, sizeof(uint64_t) * ncols); have = 1; }
+        }
+        if (memcmp(cur, self, sizeof(uint64_t) * ncols) == 0) auts++;
+        while (i < nrows && c[i] >= i) { c[i] = 0; i++; }
+        if (i >= nrows) break;
+        int t;
+        if (i % 2 == 0) { t = rp[0]; rp[0] = rp[i]; rp[i] = t; }
+        else { t = rp[c[i]]; rp[c[i]] = rp[i]; rp[i] = t; }
+        c[i]++;
+        i = 0;
+    }
+    if (canon_out)
+        for (int r = 0; r < nrows; r++)
+            for (int q = 0; q < ncols; q++)
+                canon_out[r * ncols + q] = (uint8_t)((best[q] >> (nrows - 1 - r)) & 1);
+    if (n_row_auts) *n_row_auts = auts;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Depth-first enumerator (Enumerator.cpp:540-1024 in shape; own row solver) */
+
+typedef struct {
+    int v, b, r, k, lambda;
+    uint64_t rows[ORC_MAX_ROWS];
+    uint64_t *designs; uint64_t *auts; long max_designs; long ndesigns;
+    uint64_t *stats;
+    uint64_t *cand_rows; uint32_t *cand_meta; long cand_cap; int stride, dmin, dmax;
+    u8 mat[ORC_MAX_ROWS * ORC_MAX_COLS];
+} enum_t;
+
+/* orbit data of the row about to be built */
+typedef struct {
+    int n;
+    u8 start[ORC_MAX_COLS], len[ORC_MAX_COLS], weight[ORC_MAX_COLS];
+    uint32_t inrow[ORC_MAX_COLS];  /* bit i: orbit has a one in row i */
+    int lo[ORC_MAX_COLS], hi[ORC_MAX_COLS];
+} orbs_t;
+
+static int check_partial(enum_t *e, int depth, uint64_t *aut)
+{
+    const int b = e->b;
+    for (int i = 0; i < depth; i++)
+        for (int q = 0; q < b; q++)
+            e->mat[i * b + q] = (u8)((e->rows[i] >> q) & 1);
+    if (e->cand_rows && depth >= e->dmin && depth <= e->dmax && (long)e->stats[4] < e->cand_cap) {
+        long c = (long)e->stats[4]++;
+        memset(e->cand_rows + (size_t)c * e->stride, 0, sizeof(uint64_t) * e->stride);
+        memcpy(e->cand_rows + (size_t)c * e->stride, e->rows, sizeof(uint64_t) * depth);
+        unsigned oi = depth == e->v ? (ORC_SAVE_ROW_PERMUTATIONS | ORC_SAVE_ROW_TO_CHANGE) : 0;
+        e->cand_meta[c] = (uint32_t)depth | ((uint32_t)b << 8) | (oi << 24);
+    }
+    uint8_t verdict = 0; uint64_t order = 0;
+    orc_results res;
+    memset(&res, 0, sizeof res);
+    res.verdict = &verdict; res.group_order = &order;
+    unsigned oi = depth == e->v ? (ORC_SAVE_ROW_PERMUTATIONS | ORC_SAVE_ROW_TO_CHANGE) : 0;
+    orc_test_canonicity(e->mat, depth, b, 0, oi, &res, 0);
+    e->stats[0]++;
+    if (aut) *aut = order;
+    return verdict;
+}
+
+static void dfs_row(enum_t *e, int depth);
+
+/* Enumerate orbit counts x[o] for the row `depth` (the reference solves the
+ * same system with CInSysSolver, InSysSolver.h:92-207). */
+static void solve_orbit(enum_t *e, int depth, const orbs_t *ob, int o, int *x,
+                        int sum, int *lam, const int *cap_r, const int *cap_l)
+{
+    if (o == ob->n) {
+        if (sum != e->r) return;
+        for (int i = 0; i < depth; i++) if (lam[i] != e->lambda) return;
+        uint64_t row = 0;
+        for (int p = 0; p < ob->n; p++)
+            if (x[p]) row |= (((x[p] >= 64) ? ~0ull : ((1ull << x[p]) - 1)) << ob->start[p]);
+        e->rows[depth] = row;
+        dfs_row(e, depth + 1);
+        return;
+    }
+    /* remaining capacity after this orbit */
+    for (int val = ob->hi[o]; val >= ob->lo[o]; val--) {
+        int ns = sum + val;
+        if (ns > e->r || ns + cap_r[o + 1] < e->r) continue;
+        int ok = 1;
+        for (int i = 0; i < depth; i++) {
+            int nl = lam[i] + (((ob->inrow[o] >> i) & 1) ? val : 0);
+            if (nl > e->lambda || nl + cap_l[(o + 1) * ORC_MAX_ROWS + i] < e->lambda) { ok = 0; break; }
+        }
+        if (!ok) continue;
+        x[o] = val;
+        for (int i = 0; i < depth; i++) if ((ob->inrow[o] >> i) & 1) lam[i] += val;
+        solve_orbit(e, depth, ob, o + 1, x, ns, lam, cap_r, cap_l);
+        for (int i = 0; i < depth; i++) if ((ob->inrow[o] >> i) & 1) lam[i] -= val;
+    }
+}
+
+static void dfs_row(enum_t *e, int depth)
+{
+    if (depth > 0) {
+        uint64_t aut = 0;
+        int canon = check_partial(e, depth, &aut);
+        if (!canon) return;
+        if (depth == e->v) {
+            e->stats[2]++;
+            if (e->ndesigns < e->max_designs && e->designs) {
+                memcpy(e->designs + (size_t)e->ndesigns * e->v, e->rows, sizeof(uint64_t) * e->v);
+                if (e->auts) e->auts[e->ndesigns] = aut;
+            }
+            e->ndesigns++;
+            e->stats[3]++;
+            return;
+        }
+        e->stats[1]++;
+    }
+    /* orbits of row `depth` */
+    orbs_t ob;
+    uint64_t masks[ORC_MAX_ROWS + 1];
+    uint64_t ext[ORC_MAX_ROWS + 1];
+    memcpy(ext, e->rows, sizeof(uint64_t) * depth);
+    ext[depth] = 0;
+    orc_cell_masks(ext, depth + 1, e->b, masks);
+    ob.n = orbit_starts(masks[depth], e->b, ob.start, ob.len);
+    const int remaining_after = e->v - depth - 1;
+    for (int o = 0; o < ob.n; o++) {
+        int s = ob.start[o], w = 0;
+        uint32_t in = 0;
+        for (int i = 0; i < depth; i++)
+            if ((e->rows[i] >> s) & 1) { w++; in |= 1u << i; }
+        ob.weight[o] = (u8)w;
+        ob.inrow[o] = in;
+        int need = e->k - w;               /* ones still missing in each column */
+        if (need < 0 || need > remaining_after + 1) return;  /* dead */
+        ob.hi[o] = need > 0 ? ob.len[o] : 0;
+        ob.lo[o] = need > remaining_after ? ob.len[o] : 0;
+    }
+    int cap_r[ORC_MAX_COLS + 1];
+    static __thread int cap_l[(ORC_MAX_COLS + 1) * ORC_MAX_ROWS];
+    cap_r[ob.n] = 0;
+    for (int i = 0; i < depth; i++) cap_l[ob.n * ORC_MAX_ROWS + i] = 0;
+    for (int o = ob.n - 1; o >= 0; o--) {
+        cap_r[o] = cap_r[o + 1] + ob.hi[o];
+        for (int i = 0; i < depth; i++)
+            cap_l[o * ORC_MAX_ROWS + i] = cap_l[(o + 1) * ORC_MAX_ROWS + i] +
+                (((ob.inrow[o] >> i) & 1) ? ob.hi[o] : 0);
+    }
+    int x[ORC_MAX_COLS], lam[ORC_MAX_ROWS];
+    memset(lam, 0, sizeof lam);
+    /* cap arrays are indexed by depth-specific content: keep a private copy */
+    int *capl = (int *)malloc(sizeof(int) * (ob.n + 1) * ORC_MAX_ROWS);
+    memcpy(capl, cap_l, sizeof(int) * (ob.n + 1) * ORC_MAX_ROWS);
+    solve_orbit(e, depth, &ob, 0, x, 0, lam, cap_r, capl);
+    free(capl);
+}
+
+long orc_enumerate_bibd(int v, int k, int lambda, uint64_t *designs,
+                        uint64_t *auts, long max_designs, uint64_t *stats,
+                        uint64_t *cand_rows, uint32_t *cand_meta, long cand_cap,
+                        int stride, int dmin, int dmax)
+{
+    if (v < 2 || k < 2 || k > v || lambda < 1) return -1;
+    if ((lambda * (v - 1)) % (k - 1)) return -2;
+    int r = lambda * (v - 1) / (k - 1);
+    if ((v * r) % k) return -2;
+    int b = v * r / k;
+    if (v > ORC_MAX_ROWS || b > ORC_MAX_COLS) return -3;
+    enum_t *e = (enum_t *)calloc(1, sizeof(enum_t));
+    e->v = v; e->b = b; e->r = r; e->k = k; e->lambda = lambda;
+    e->designs = designs; e->auts = auts; e->max_designs = max_designs;
+    uint64_t st[5] = {0, 0, 0, 0, 0};
+    e->stats = st;
+    e->cand_rows = cand_rows; e->cand_meta = cand_meta; e->cand_cap = cand_cap;
+    e->stride = stride; e->dmin = dmin; e->dmax = dmax;
+    dfs_row(e, 0);
+    long nd = e->ndesigns;
+    if (stats) memcpy(stats, st, sizeof st);
+    free(e);
+    return nd;
+}
