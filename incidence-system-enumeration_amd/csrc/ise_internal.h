@@ -1,0 +1,29 @@
+// ise_internal.h -- declarations shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "ise.h"
+
+namespace ise {
+
+// Enqueue the queue-head reset and the canonicity kernel on `stream`.
+hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_block,
+                        uint32_t n, uint32_t stride, const uint64_t *rows,
+                        const uint64_t *cells, const uint32_t *meta,
+                        const ise_results &out, uint32_t *head, uint32_t max_perms,
+                        hipStream_t stream);
+
+// Resident workgroups per CU for the kernel variant.
+hipError_t occupancy_blocks(int group_lanes, uint32_t waves_per_block, int *blocks_per_cu);
+
+// Smallest lane-group size that holds every candidate of a batch.
+inline int fit_group_lanes(uint32_t max_depth, uint32_t max_cols) {
+    const uint32_t need = max_depth > max_cols ? max_depth : max_cols;
+    return need <= 16 ? 16 : need <= 32 ? 32 : 64;
+}
+
+// Geometry of a context (rows per candidate, candidates per submit).
+uint32_t ctx_stride(const ise_ctx *c);
+uint32_t ctx_capacity(const ise_ctx *c);
+
+} // namespace ise

@@ -1,0 +1,325 @@
+// ise_runtime.cpp -- contexts, device buffers and the C ABI of the
+// canonicity kernels (declared in include/ise.h).
+//
+// A context owns one HIP stream and device buffers for `capacity` candidates
+// (inputs and every result array), so that a steady stream of batches causes
+// no allocation after ise_open.  This replaces the reference's per-matrix
+// device-heap checker objects (Source/CanonicityChecker.cu:17-68) and its
+// batches of at most 128 matrices (Source/CudaInfo.h:9).
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include "ise_internal.h"
+
+struct ise_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    uint32_t capacity = 0, stride = 0;
+    // device inputs
+    uint64_t *d_rows = nullptr, *d_cells = nullptr;
+    uint32_t *d_meta = nullptr, *d_head = nullptr;
+    // device results
+    ise_results d{};
+    uint32_t gen_cap_alloc = 0;
+    // host queue filled by ise_assign
+    std::vector<uint64_t> q_rows;
+    std::vector<uint32_t> q_meta;
+    // last submit
+    uint32_t last_n = 0, last_gen_cap = 0;
+    bool pending = false, last_cells = false;
+    ise_launch_cfg cfg{0, 4, 0, 1u << 26};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+    std::string err;
+};
+
+namespace {
+
+int fail(ise_ctx *c, hipError_t e, const char *what) {
+    if (c) {
+        c->err = std::string(what) + ": " + hipGetErrorString(e);
+    }
+    return ISE_ERR_HIP;
+}
+
+#define HIPCHK(ctx, call)                                   \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return fail(ctx, e_, #call);  \
+    } while (0)
+
+template <typename T> hipError_t dalloc(T **p, size_t count) {
+    return hipMalloc((void **)p, count * sizeof(T) + 16);
+}
+
+int alloc_gens(ise_ctx *c, uint32_t gen_cap) {
+    if (gen_cap <= c->gen_cap_alloc) return ISE_OK;
+    if (c->d.gens) HIPCHK(c, hipFree(c->d.gens));
+    c->d.gens = nullptr;
+    HIPCHK(c, dalloc(&c->d.gens, (size_t)c->capacity * gen_cap * ISE_MAX_COLS));
+    c->gen_cap_alloc = gen_cap;
+    return ISE_OK;
+}
+
+// Validate host-side records and pick the lane-group size.
+int scan_meta(const uint32_t *meta, uint32_t n, uint32_t stride, uint32_t *max_d, uint32_t *max_b) {
+    uint32_t md = 0, mb = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t m = meta[i], d = m & 0xff, b = (m >> 8) & 0xff, f = (m >> 16) & 0xff;
+        if (d < 1 || d > ISE_MAX_ROWS || d > stride || b < 1 || b > ISE_MAX_COLS || f >= d)
+            return ISE_ERR_ARG;
+        if (d > md) md = d;
+        if (b > mb) mb = b;
+    }
+    *max_d = md;
+    *max_b = mb;
+    return ISE_OK;
+}
+
+int launch(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const uint64_t *cells,
+           const uint32_t *meta, const ise_results &out, uint32_t *head, hipStream_t s,
+           int group_lanes) {
+    const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
+    uint32_t blocks = c->cfg.blocks;
+    if (!blocks) {
+        int per_cu = 0;
+        HIPCHK(c, ise::occupancy_blocks(group_lanes, wpb, &per_cu));
+        if (per_cu < 1) per_cu = 1;
+        blocks = (uint32_t)per_cu * (uint32_t)c->num_cus;
+        const uint64_t groups_needed = ((uint64_t)n * group_lanes + 64ull * wpb - 1) / (64ull * wpb);
+        if (blocks > groups_needed) blocks = (uint32_t)(groups_needed ? groups_needed : 1);
+    }
+    HIPCHK(c, ise::launch_check(group_lanes, blocks, wpb, n, stride, rows, cells, meta, out, head,
+                                c->cfg.max_perms ? c->cfg.max_perms : (1u << 26), s));
+    return ISE_OK;
+}
+
+} // namespace
+
+namespace ise {
+uint32_t ctx_stride(const ise_ctx *c) { return c->stride; }
+uint32_t ctx_capacity(const ise_ctx *c) { return c->capacity; }
+} // namespace ise
+
+extern "C" {
+
+int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride) {
+    if (!out || capacity == 0 || stride == 0 || stride > ISE_MAX_ROWS) return ISE_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return ISE_ERR_NODEV;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return ISE_ERR_NODEV;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ISE_ERR_NODEV;
+    ise_ctx *c = new ise_ctx();
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    c->capacity = capacity;
+    c->stride = stride;
+    int rc = ISE_OK;
+    do {
+        if (hipSetDevice(device) != hipSuccess) { rc = ISE_ERR_NODEV; break; }
+        const size_t n = capacity;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = dalloc(&c->d_rows, n * stride);
+        if (e == hipSuccess) e = dalloc(&c->d_cells, n * stride);
+        if (e == hipSuccess) e = dalloc(&c->d_meta, n);
+        if (e == hipSuccess) e = dalloc(&c->d_head, 1);
+        if (e == hipSuccess) e = dalloc(&c->d.verdict, n);
+        if (e == hipSuccess) e = dalloc(&c->d.fail_row, n);
+        if (e == hipSuccess) e = dalloc(&c->d.fail_col, n);
+        if (e == hipSuccess) e = dalloc(&c->d.row_to_change, n);
+        if (e == hipSuccess) e = dalloc(&c->d.group_order, n);
+        if (e == hipSuccess) e = dalloc(&c->d.perm_row, n * ISE_MAX_ROWS);
+        if (e == hipSuccess) e = dalloc(&c->d.perm_col, n * ISE_MAX_COLS);
+        if (e == hipSuccess) e = dalloc(&c->d.orbits, n * ISE_MAX_ROWS);
+        if (e == hipSuccess) e = dalloc(&c->d.n_gens, n);
+        if (e == hipSuccess) e = dalloc(&c->d.counters, n * 4);
+        if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+        if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+        if (e != hipSuccess) { rc = fail(c, e, "ise_open"); break; }
+    } while (0);
+    if (rc != ISE_OK) {
+        ise_close(c);
+        return rc;
+    }
+    *out = c;
+    return ISE_OK;
+}
+
+int ise_close(ise_ctx *c) {
+    if (!c) return ISE_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *bufs[] = {c->d_rows, c->d_cells, c->d_meta, c->d_head, c->d.verdict, c->d.fail_row,
+                    c->d.fail_col, c->d.row_to_change, c->d.group_order, c->d.perm_row,
+                    c->d.perm_col, c->d.orbits, c->d.n_gens, c->d.counters, c->d.gens};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ISE_OK;
+}
+
+int ise_set_launch(ise_ctx *c, const ise_launch_cfg *cfg) {
+    if (!c || !cfg) return ISE_ERR_ARG;
+    if (cfg->group_lanes && cfg->group_lanes != 16 && cfg->group_lanes != 32 && cfg->group_lanes != 64)
+        return ISE_ERR_ARG;
+    if (cfg->waves_per_block > 16) return ISE_ERR_ARG;
+    c->cfg = *cfg;
+    return ISE_OK;
+}
+
+int ise_get_launch(const ise_ctx *c, ise_launch_cfg *cfg) {
+    if (!c || !cfg) return ISE_ERR_ARG;
+    *cfg = c->cfg;
+    return ISE_OK;
+}
+
+int ise_assign(ise_ctx *c, const uint64_t *rows, uint32_t depth, uint32_t ncols,
+               uint32_t len_stab, uint32_t out_info) {
+    if (!c || !rows || depth < 1 || depth > c->stride || ncols < 1 || ncols > ISE_MAX_COLS ||
+        len_stab >= depth || out_info > 0xff)
+        return ISE_ERR_ARG;
+    if (c->q_meta.size() >= c->capacity) return ISE_ERR_CAPACITY;
+    const size_t at = c->q_rows.size();
+    c->q_rows.resize(at + c->stride, 0ull);
+    memcpy(&c->q_rows[at], rows, depth * sizeof(uint64_t));
+    c->q_meta.push_back(depth | (ncols << 8) | (len_stab << 16) | (out_info << 24));
+    return ISE_OK;
+}
+
+int ise_num_assigned(const ise_ctx *c, uint32_t *n) {
+    if (!c || !n) return ISE_ERR_ARG;
+    *n = (uint32_t)c->q_meta.size();
+    return ISE_OK;
+}
+
+int ise_submit(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows,
+               const uint64_t *cells, const uint32_t *meta, uint32_t gen_cap) {
+    if (!c) return ISE_ERR_ARG;
+    bool from_queue = false;
+    if (n == 0 && !rows) {
+        from_queue = true;
+        n = (uint32_t)c->q_meta.size();
+        stride = c->stride;
+        rows = c->q_rows.data();
+        meta = c->q_meta.data();
+        cells = nullptr;
+    }
+    if (n > c->capacity) return ISE_ERR_CAPACITY;
+    if (n && (!rows || !meta || stride != c->stride)) return ISE_ERR_ARG;
+    uint32_t md = 1, mb = 1;
+    if (n && scan_meta(meta, n, stride, &md, &mb) != ISE_OK) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (gen_cap) {
+        int rc = alloc_gens(c, gen_cap);
+        if (rc) return rc;
+    }
+    c->last_n = n;
+    c->last_gen_cap = gen_cap;
+    c->last_cells = cells != nullptr;
+    if (n) {
+        HIPCHK(c, hipMemcpyAsync(c->d_rows, rows, (size_t)n * stride * 8, hipMemcpyHostToDevice, c->stream));
+        if (cells)
+            HIPCHK(c, hipMemcpyAsync(c->d_cells, cells, (size_t)n * stride * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_meta, meta, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+        ise_results o = c->d;
+        o.gens = gen_cap ? c->d.gens : nullptr;
+        o.gen_cap = gen_cap;
+        int group = c->cfg.group_lanes ? (int)c->cfg.group_lanes : ise::fit_group_lanes(md, mb);
+        if ((uint32_t)group < md || (uint32_t)group < mb) return ISE_ERR_ARG;
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        int rc = launch(c, n, stride, c->d_rows, cells ? c->d_cells : nullptr, c->d_meta, o,
+                        c->d_head, c->stream, group);
+        if (rc) return rc;
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    }
+    if (from_queue) {
+        c->q_rows.clear();
+        c->q_meta.clear();
+    }
+    c->pending = true;
+    return ISE_OK;
+}
+
+int ise_collect(ise_ctx *c, ise_results *h) {
+    if (!c || !h) return ISE_ERR_ARG;
+    if (!c->pending) return ISE_ERR_STATE;
+    const size_t n = c->last_n;
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n) {
+        if (h->verdict) HIPCHK(c, hipMemcpyAsync(h->verdict, c->d.verdict, n, hipMemcpyDeviceToHost, s));
+        if (h->fail_row) HIPCHK(c, hipMemcpyAsync(h->fail_row, c->d.fail_row, n, hipMemcpyDeviceToHost, s));
+        if (h->fail_col) HIPCHK(c, hipMemcpyAsync(h->fail_col, c->d.fail_col, n, hipMemcpyDeviceToHost, s));
+        if (h->row_to_change)
+            HIPCHK(c, hipMemcpyAsync(h->row_to_change, c->d.row_to_change, n, hipMemcpyDeviceToHost, s));
+        if (h->group_order)
+            HIPCHK(c, hipMemcpyAsync(h->group_order, c->d.group_order, n * 8, hipMemcpyDeviceToHost, s));
+        if (h->perm_row)
+            HIPCHK(c, hipMemcpyAsync(h->perm_row, c->d.perm_row, n * ISE_MAX_ROWS, hipMemcpyDeviceToHost, s));
+        if (h->perm_col)
+            HIPCHK(c, hipMemcpyAsync(h->perm_col, c->d.perm_col, n * ISE_MAX_COLS, hipMemcpyDeviceToHost, s));
+        if (h->orbits)
+            HIPCHK(c, hipMemcpyAsync(h->orbits, c->d.orbits, n * ISE_MAX_ROWS, hipMemcpyDeviceToHost, s));
+        if (h->n_gens) HIPCHK(c, hipMemcpyAsync(h->n_gens, c->d.n_gens, n * 2, hipMemcpyDeviceToHost, s));
+        if (h->counters)
+            HIPCHK(c, hipMemcpyAsync(h->counters, c->d.counters, n * 16, hipMemcpyDeviceToHost, s));
+        if (h->gens && c->last_gen_cap) {
+            if (h->gen_cap != c->last_gen_cap) return ISE_ERR_ARG;
+            HIPCHK(c, hipMemcpyAsync(h->gens, c->d.gens, n * c->last_gen_cap * ISE_MAX_COLS,
+                                     hipMemcpyDeviceToHost, s));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (n) HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    c->pending = false;
+    return ISE_OK;
+}
+
+int ise_test_canonicity(ise_ctx *c, const uint8_t *mat, uint32_t nrows, uint32_t ncols,
+                        uint32_t len_stab, uint32_t out_info, uint32_t gen_cap, ise_results *h) {
+    if (!c || !mat || !h || nrows < 1 || nrows > c->stride || ncols < 1 || ncols > ISE_MAX_COLS ||
+        len_stab >= nrows)
+        return ISE_ERR_ARG;
+    std::vector<uint64_t> rows(c->stride, 0ull);
+    for (uint32_t i = 0; i < nrows; i++)
+        for (uint32_t q = 0; q < ncols; q++)
+            if (mat[i * ncols + q]) rows[i] |= 1ull << q;
+    const uint32_t meta = nrows | (ncols << 8) | (len_stab << 16) | ((out_info & 0xff) << 24);
+    int rc = ise_submit(c, 1, c->stride, rows.data(), nullptr, &meta, gen_cap);
+    if (rc) return rc;
+    return ise_collect(c, h);
+}
+
+int ise_check_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                           const uint64_t *d_cells, const uint32_t *d_meta, const ise_results *d_out,
+                           uint32_t *d_counter, void *stream) {
+    if (!c || !d_out || !d_out->verdict || !d_counter || stride < 1 || stride > ISE_MAX_ROWS)
+        return ISE_ERR_ARG;
+    if (n == 0) return ISE_OK;
+    if (!d_rows || !d_meta) return ISE_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const int group = c->cfg.group_lanes ? (int)c->cfg.group_lanes : 64;
+    HIPCHK(c, hipSetDevice(c->device));
+    return launch(c, n, stride, d_rows, d_cells, d_meta, *d_out, d_counter, s, group);
+}
+
+int ise_sync(ise_ctx *c) {
+    if (!c) return ISE_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISE_OK;
+}
+
+const char *ise_last_error(const ise_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+// Kernel time of the last collected submit, milliseconds (0 if none).
+float ise_last_kernel_ms(const ise_ctx *c) { return c ? c->last_ms : 0.f; }
+
+} // extern "C"

@@ -1,0 +1,383 @@
+// ise_wave.h -- lane-parallel canonicity test of one bit-packed candidate.
+//
+// One group of G lanes (G = 16, 32 or 64; 64/G candidates per wavefront) owns
+// one candidate partial incidence matrix.  Every piece of per-candidate state
+// lives in the group's lanes:
+//   lane l :  rows[l]   original row l as a column bit mask   (l < depth)
+//             cells[l]  column-orbit start mask of row l     (l < depth)
+//             perm[l]   image of row l under the current row permutation
+//             orb[l]    row orbit label (smallest element of the orbit)
+//             pcol[l]   column permutation, position l -> source column (l < ncols)
+// Control flow is uniform inside a group; cross-lane work uses ballots,
+// readlane/bpermute and ds_permute.
+//
+// The search replays the reference's scalar algorithm step for step so that
+// the verdict, the first witness, the automorphisms found, |Aut| and the
+// final permutation state are identical to it:
+//   - the row-permutation walk of CGroupOrder::next_permutation
+//     (k-Factorization/Utils/Include/GroupOrder.h:136-248),
+//   - the row-by-row / orbit-by-orbit comparison of
+//     CCanonicityChecker::TestCanonicity (Source/CanonicityChecker.h:326-477)
+//     and checkColOrbit (Source/CanonicityChecker.cpp:191-223): a row of the
+//     permuted matrix is compared with the original one column orbit at a
+//     time by the number of ones; the first orbit that differs decides, and
+//     the column permutation is stably partitioned (ones first) inside every
+//     orbit that compared equal before it,
+//   - addAutomorphism / updateGroupOrder (GroupOrder.h:56-102, :30-49).
+#pragma once
+#include <stdint.h>
+#include "ise.h"
+
+#ifndef ISE_DEV
+#define ISE_DEV __device__ __forceinline__
+#endif
+
+namespace ise {
+
+// ---- cross-lane primitives (CDNA wave64) ---------------------------------
+ISE_DEV int lane_id() { return (int)__lane_id(); }
+ISE_DEV uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+ISE_DEV int pull(int x, int src_lane) { return __builtin_amdgcn_ds_bpermute(src_lane << 2, x); }
+ISE_DEV int push(int x, int dst_lane) { return __builtin_amdgcn_ds_permute(dst_lane << 2, x); }
+ISE_DEV int popc64(uint64_t x) { return __popcll(x); }
+ISE_DEV int ctz64(uint64_t x) { return __builtin_ctzll(x); }
+ISE_DEV int hib64(uint64_t x) { return 63 - __builtin_clzll(x); }
+ISE_DEV uint64_t lowmask(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1); }
+
+// Group-local views: G lanes starting at `base`.
+template <int G> struct Grp {
+    int ln;     // lane index inside the group
+    int base;   // first wave lane of the group
+    ISE_DEV Grp() { int l = lane_id(); ln = l & (G - 1); base = l - ln; }
+    ISE_DEV uint64_t ballot(bool p) const {
+        uint64_t m = ballot64(p);
+        if constexpr (G == 64) return m;
+        else return (m >> base) & ((1ull << G) - 1);
+    }
+    // value of x held by group lane i (i uniform in the group)
+    ISE_DEV int read(int x, int i) const {
+        if constexpr (G == 64) return __builtin_amdgcn_readlane(x, i);
+        else return pull(x, base + i);
+    }
+    ISE_DEV uint64_t read64(uint64_t x, int i) const {
+        uint32_t lo = (uint32_t)read((int)(uint32_t)x, i);
+        uint32_t hi = (uint32_t)read((int)(uint32_t)(x >> 32), i);
+        return ((uint64_t)hi << 32) | lo;
+    }
+    ISE_DEV int gather(int x, int i) const { return pull(x, base + i); }
+    ISE_DEV int scatter(int x, int i) const { return push(x, base + i); }
+};
+
+constexpr int EMAX = 255;   // ELEMENT_MAX for 8-bit indices
+constexpr int IMAX = 254;   // IDX_MAX
+
+// Row-permutation state of one candidate.
+struct PermState {
+    int stab;        // stabiliser length (m_nStabLength)
+    int stab_aut;    // m_nStabLengthAut
+    uint64_t order;  // |Aut| accumulated along the stabiliser chain
+};
+
+// perm[piv+1 .. n-1] reversed
+template <int G> ISE_DEV int reverse_tail(const Grp<G> &g, int pr, int n, int piv) {
+    int l = g.ln;
+    int src = (l > piv && l < n) ? (n + piv - l) : l;
+    return g.gather(pr, src);
+}
+
+// Next row permutation fixing the first `fixed` rows (GroupOrder.h:136-248).
+// `at`: EMAX plain successor, IMAX after an automorphism, else refuted prefix
+// position.  Returns the first changed position or EMAX.
+template <int G>
+ISE_DEV int next_row_perm(const Grp<G> &g, PermState &st, int &pr, int ob, int n, int at, int fixed) {
+    const int l = g.ln;
+    for (;;) {
+        int piv, succ, old;
+        if (at == IMAX && g.read(pr, st.stab) == n - 1)
+            at = EMAX;
+
+        if (at == IMAX) {
+            piv = st.stab;
+            old = g.read(pr, piv);
+            at = piv;
+            // positions piv+1..old take piv..old-1, positions old+1..n-1 keep their index
+            if (l > piv && l <= old) pr = l - 1;
+            else if (l > old && l < n) pr = l;
+            succ = old + 1;
+        } else if (at == EMAX) {
+            int prev = g.gather(pr, l > 0 ? l - 1 : 0);
+            uint64_t asc = g.ballot(l >= 1 && l < n && prev < pr);
+            piv = asc ? hib64(asc) : 0;
+            if (piv == fixed)
+                return EMAX;
+            piv -= 1;
+            old = g.read(pr, piv);
+            uint64_t big = g.ballot(l > piv && l < n && pr > old);
+            succ = hib64(big);
+        } else {
+            piv = at;
+            old = g.read(pr, piv);
+            uint64_t big = g.ballot(l > piv && l < n && pr > old);
+            if (!big) {
+                pr = reverse_tail(g, pr, n, piv);
+                at = EMAX;
+                fixed = 0;
+                continue;   // the reference recurses with default arguments
+            }
+            succ = ctz64(big);
+        }
+
+        if (st.stab == piv) {
+            // the image of the stabiliser point must be an orbit leader
+            bool step_back = false;
+            int img = g.read(pr, succ);
+            bool leader = g.gather(ob, pr) == pr;
+            if (at >= IMAX) {
+                uint64_t lead = g.ballot(leader && l > piv && l <= succ);
+                int q = lead ? hib64(lead) : piv;
+                if (q != succ) {
+                    if (q == 0)
+                        return EMAX;
+                    step_back = (q == piv);
+                    img = g.read(pr, q);
+                    int src = (l >= q && l < succ) ? l + 1 : l;
+                    pr = g.gather(pr, src);
+                }
+            } else {
+                uint64_t lead = g.ballot(leader && l >= succ && l < n);
+                int q = lead ? ctz64(lead) : n;
+                if (q != succ) {
+                    step_back = (q == n);
+                    if (step_back) {
+                        if (piv == 0)
+                            return EMAX;
+                        if (l >= at && l < succ) pr = l;
+                    } else {
+                        img = g.read(pr, q);
+                        int src = (l > succ && l <= q) ? l - 1 : l;
+                        pr = g.gather(pr, src);
+                    }
+                }
+            }
+            if (l == succ) pr = img;
+            if (step_back) {
+                succ = at >= IMAX ? n - 1 : piv;
+                piv -= 1;
+                old = g.read(pr, piv);
+                st.stab = piv;
+            }
+        }
+
+        int at_succ = g.read(pr, succ);
+        if (l == piv) pr = at_succ;
+        if (l == succ) pr = old;
+        if (at >= IMAX) {
+            if (st.stab > piv)
+                st.stab = piv;
+            pr = reverse_tail(g, pr, n, piv);
+        }
+        return piv;
+    }
+}
+
+// |Aut| *= length of the orbit of the current stabiliser point (GroupOrder.h:63-73)
+template <int G> ISE_DEV void grow_order(const Grp<G> &g, PermState &st, int ob, int n) {
+    if (st.stab_aut >= n) return;
+    uint64_t m = g.ballot(g.ln > st.stab_aut && g.ln < n && ob == st.stab_aut);
+    st.order *= (uint64_t)(1 + popc64(m));
+}
+
+// New automorphism `pr` (GroupOrder.h:56-62, :76-102, Update_Orbits :30-49)
+template <int G>
+ISE_DEV void add_automorphism(const Grp<G> &g, PermState &st, int pr, int &ob, int n, bool row_permut) {
+    const int l = g.ln;
+    uint64_t moved = g.ballot(l < n && pr != l);
+    int first = ctz64(moved);
+    if (row_permut && st.stab_aut > first)
+        grow_order(g, st, ob, n);
+    st.stab_aut = first;
+    st.stab = first;
+    for (int p = first; p < n; p++) {
+        int a = g.read(ob, p);
+        int c = g.read(ob, g.read(pr, p));
+        if (a == c) continue;
+        if (c < a) { int t = a; a = c; c = t; }
+        if (ob == c) ob = a;
+    }
+}
+
+// Outputs of one batch (device pointers, structure of arrays).
+struct OutPtrs {
+    uint8_t *verdict, *fail_row, *fail_col, *row_to_change;
+    uint64_t *group_order;
+    uint8_t *perm_row;   // [ISE_MAX_ROWS] per candidate
+    uint8_t *perm_col;   // [ISE_MAX_COLS] per candidate
+    uint8_t *orbits;     // [ISE_MAX_ROWS] per candidate
+    uint16_t *n_gens;
+    uint8_t *gens;       // [gen_cap][ISE_MAX_COLS] per candidate, may be null
+    uint32_t *counters;  // [4] per candidate, may be null
+    uint32_t gen_cap;
+};
+
+// position of the k-th (0-based) set bit of m (k < popcount(m))
+ISE_DEV int select_bit(uint64_t m, int k) {
+    int lo = 0, hi = 63;            // answer p: popc(m & lowmask(p+1)) == k+1
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (popc64(m & lowmask(mid + 1)) >= k + 1) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Full canonicity test of candidate `ci` by the group g.
+template <int G>
+ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta,
+                             const uint64_t *rows, const uint64_t *cells,
+                             uint32_t stride, const OutPtrs &o, uint32_t max_perms) {
+    const int l = g.ln;
+    const int d = meta & 0xff, b = (meta >> 8) & 0xff;
+    const int fixed = (meta >> 16) & 0xff;
+    const bool row_permut = ((meta >> 24) & ISE_SAVE_ROW_PERMUTATIONS) != 0;
+    const uint64_t bmask = lowmask(b);
+
+    // stage the candidate in the group's lanes (lane l reads word l)
+    const uint64_t *rp = rows + (uint64_t)ci * stride;
+    uint64_t rowv = (l < d) ? rp[l] : 0ull;
+    uint64_t cellv;
+    if (cells) {
+        cellv = (l < d) ? cells[(uint64_t)ci * stride + l] : 0ull;
+    } else {
+        // column c starts an orbit of row l when an earlier row changes value at c
+        uint64_t m = 1;
+        for (int j = 0; j < d; j++) {
+            uint64_t r = g.read64(rowv, j);
+            if (j < l) m |= r ^ (r << 1);
+        }
+        cellv = m & bmask;
+    }
+    // orbits induced by all d rows (indexing the column-orbit generators)
+    const uint64_t last = g.read64(rowv, d - 1);
+    const uint64_t cell_next = (g.read64(cellv, d - 1) | (last ^ (last << 1))) & bmask;
+
+    int pr = l, ob = l, pc = l;
+    PermState st;
+    st.stab = d - 1;
+    st.stab_aut = EMAX;
+    st.order = 1;
+    uint32_t perms = 0, rows_cmp = 0, orbs_cmp = 0, cells_cmp = 0;
+    uint32_t ngen = 0;
+    int verdict = 1, frow = EMAX, fcol = EMAX, rtc = EMAX;
+
+    int row = EMAX;
+    for (;;) {
+        row = next_row_perm(g, st, pr, ob, d, row, fixed);
+        if (row == EMAX)
+            break;
+        if (++perms > max_perms) { verdict = ISE_VERDICT_LIMIT; break; }
+        int outcome = 0, dec = 0;
+        for (; row < d; row++) {
+            const uint64_t R = g.read64(rowv, row);
+            const uint64_t Pw = g.read64(rowv, g.read(pr, row));
+            const uint64_t C = g.read64(cellv, row);
+            const bool act = l < b;
+            const int bit = act ? (int)((Pw >> pc) & 1ull) : 0;
+            const uint64_t P = g.ballot(bit != 0);
+            const uint64_t le = (l == 63) ? ~0ull : ((2ull << l) - 1);
+            const int s = hib64(C & le);
+            const uint64_t above = C & ~le & bmask;
+            const int e = above ? ctz64(above) : b;
+            const uint64_t om = lowmask(e) & ~lowmask(s);
+            const int cP = popc64(P & om), cR = popc64(R & om);
+            const uint64_t bad = g.ballot(act && l == s && cP != cR);
+            int lim = b;
+            rows_cmp++;
+            if (bad) {
+                lim = ctz64(bad);
+                outcome = g.read(cR > cP ? 1 : -1, lim);
+                dec = lim;
+                orbs_cmp += popc64(C & lowmask(lim + 1));
+                cells_cmp += (uint32_t)g.read(e, lim);
+            } else {
+                orbs_cmp += popc64(C & bmask);
+                cells_cmp += (uint32_t)b;
+            }
+            // stable partition (ones first) inside every orbit before `lim`
+            const bool upd = act && s < lim;
+            const int ones_before = popc64(P & om & lowmask(l));
+            const int np = bit ? s + ones_before : s + cP + (l - s - ones_before);
+            pc = g.scatter(pc, upd ? np : l);
+            if (outcome) break;
+        }
+        if (outcome > 0)
+            continue;                 // refuted prefix: advance position `row`
+        if (outcome < 0) {            // the permuted matrix is larger: witness
+            verdict = 0;
+            frow = row;
+            fcol = dec;
+            // rowToChange (CanonicityChecker.cpp:320-330): max(row, perm[0..row])
+            const int inv = g.scatter(l, (l < d) ? pr : l);
+            const uint64_t seen = g.ballot(l < d && inv <= row);
+            const int mx = hib64(seen);
+            rtc = mx > row ? mx : row;
+            break;
+        }
+        // automorphism: record a generator
+        if (o.gens && ngen < o.gen_cap) {
+            uint8_t *gp = o.gens + ((uint64_t)ci * o.gen_cap + ngen) * ISE_MAX_COLS;
+            if (!row_permut) {
+                // permutation induced on the column orbits of the next row
+                // (CanonicityChecker.h:427-450)
+                const int norb = popc64(cell_next);
+                for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
+                    const int x = x0 + l;
+                    const int src = x < norb ? select_bit(cell_next, x) : 0;
+                    const int pcs = g.gather(pc, src < G ? src : 0);
+                    gp[x] = (uint8_t)(x < norb ? popc64(cell_next & lowmask(pcs)) : 0xff);
+                }
+            } else {
+                for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
+                    const int x = x0 + l;
+                    const int v = g.gather(pr, x < G ? x : 0);
+                    gp[x] = (uint8_t)(x < d ? v : 0xff);
+                }
+            }
+        }
+        ngen++;
+        add_automorphism(g, st, pr, ob, d, row_permut);
+        row = IMAX;
+    }
+    if (verdict == 1 && row_permut)
+        grow_order(g, st, ob, d);
+
+    // write the record
+    if (l == 0) {
+        o.verdict[ci] = (uint8_t)verdict;
+        if (o.fail_row) o.fail_row[ci] = (uint8_t)frow;
+        if (o.fail_col) o.fail_col[ci] = (uint8_t)fcol;
+        if (o.row_to_change) o.row_to_change[ci] = (uint8_t)rtc;
+        if (o.group_order) o.group_order[ci] = verdict == 1 ? st.order : 0ull;
+        if (o.n_gens) o.n_gens[ci] = (uint16_t)(ngen > 0xffffu ? 0xffffu : ngen);
+        if (o.counters) {
+            uint32_t *c = o.counters + (uint64_t)ci * 4;
+            c[0] = perms; c[1] = rows_cmp; c[2] = orbs_cmp; c[3] = cells_cmp;
+        }
+    }
+    // every lane takes part in the gathers; lanes past the record skip the store
+    for (int x0 = 0; x0 < ISE_MAX_ROWS; x0 += G) {
+        const int x = x0 + l;
+        const int vp = g.gather(pr, x < G ? x : 0), vo = g.gather(ob, x < G ? x : 0);
+        if (x < ISE_MAX_ROWS) {
+            if (o.perm_row) o.perm_row[(uint64_t)ci * ISE_MAX_ROWS + x] = (uint8_t)(x < d ? vp : 0xff);
+            if (o.orbits) o.orbits[(uint64_t)ci * ISE_MAX_ROWS + x] = (uint8_t)(x < d ? vo : 0xff);
+        }
+    }
+    if (o.perm_col) {
+        for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
+            const int x = x0 + l;
+            const int vc = g.gather(pc, x < G ? x : 0);
+            o.perm_col[(uint64_t)ci * ISE_MAX_COLS + x] = (uint8_t)(x < b ? vc : 0xff);
+        }
+    }
+}
+
+} // namespace ise

@@ -1,0 +1,282 @@
+"""Python view of the MI355X batched canonicity test (libise_amd.so).
+
+The classes mirror the reference's host interface:
+
+* `CanonicityChecker.TestCanonicity` follows
+  `CCanonicityChecker::TestCanonicity(nRowMax, pCanonParam, outInfo, ...)`
+  (Source/CanonicityChecker.h:58) and exposes `permRow()`, `permCol()`,
+  `groupOrder()`, `orbits()` like the reference accessors (:61-70, :115).
+* `GPUCanonChecker.assignChecker` / `LaunchCanonicityTesting` follow
+  `CGPU_CanonChecker` (Source/MatrixCanonChecker.h:248-329).
+* `Context.check_batch` is the batched form used by the enumerator and the
+  benchmark.
+
+All work runs in the HIP library; importing this module without the built
+library, or using it without a gfx950 device, raises instead of falling back
+to any CPU path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from ._abi import (MAX_COLS, MAX_ROWS, SAVE_COL_PERMUTATIONS, SAVE_ROW_PERMUTATIONS,
+                   SAVE_ROW_TO_CHANGE, VERDICT_CANONICAL, VERDICT_INVALID, VERDICT_LIMIT,
+                   VERDICT_NONCANONICAL, ResultsStruct, alloc_results, pack_meta, pack_rows,
+                   results_struct, unpack_rows)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libise_amd.so")
+
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_NODEV = -3
+ERR_CAPACITY = -4
+ERR_STATE = -5
+
+_lib = None
+
+
+class ISEError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"ise status {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """The loaded HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ISEError(ERR_NODEV, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        P, U32, U64, I = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+        L.ise_open.argtypes = [ctypes.POINTER(P), I, U32, U32]
+        L.ise_close.argtypes = [P]
+        L.ise_set_launch.argtypes = [P, P]
+        L.ise_get_launch.argtypes = [P, P]
+        L.ise_assign.argtypes = [P, P, U32, U32, U32, U32]
+        L.ise_num_assigned.argtypes = [P, ctypes.POINTER(U32)]
+        L.ise_submit.argtypes = [P, U32, U32, P, P, P, U32]
+        L.ise_collect.argtypes = [P, ctypes.POINTER(ResultsStruct)]
+        L.ise_test_canonicity.argtypes = [P, P, U32, U32, U32, U32, U32, ctypes.POINTER(ResultsStruct)]
+        L.ise_check_batch_device.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P, P]
+        L.ise_sync.argtypes = [P]
+        L.ise_last_error.argtypes = [P]
+        L.ise_last_error.restype = ctypes.c_char_p
+        L.ise_last_kernel_ms.argtypes = [P]
+        L.ise_last_kernel_ms.restype = ctypes.c_float
+        L.ise_enumerate_bibd.argtypes = [P, I, I, I, P, P, ctypes.c_long, P, P]
+        L.ise_enumerate_bibd.restype = ctypes.c_long
+        L.ise_enumerate_bibd_candidates.argtypes = [P, I, I, I, I, I, U32, P, P, ctypes.c_long,
+                                                    ctypes.POINTER(ctypes.c_long)]
+        L.ise_enumerate_bibd_candidates.restype = ctypes.c_long
+        L.ise_random_extensions.argtypes = [I, I, I, U64, I, I, U64, U32, P, P]
+        _lib = L
+    return _lib
+
+
+class LaunchCfg(ctypes.Structure):
+    _fields_ = [("group_lanes", ctypes.c_uint32), ("waves_per_block", ctypes.c_uint32),
+                ("blocks", ctypes.c_uint32), ("max_perms", ctypes.c_uint32)]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def design_params(v, k, lam):
+    """(v, b, r, k, lambda) of a 2-design; raises if not admissible."""
+    if (lam * (v - 1)) % (k - 1):
+        raise ValueError("r is not an integer")
+    r = lam * (v - 1) // (k - 1)
+    if (v * r) % k:
+        raise ValueError("b is not an integer")
+    return v, v * r // k, r, k, lam
+
+
+class Context:
+    """One device context (stream + buffers) for batches of candidates."""
+
+    def __init__(self, capacity=1 << 16, stride=32, device=0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.ise_open(ctypes.byref(h), device, capacity, stride)
+        if rc != OK:
+            raise ISEError(rc, "ise_open failed (no gfx950 device?)")
+        self.h = h
+        self.capacity = capacity
+        self.stride = stride
+
+    def close(self):
+        if self.h:
+            self._L.ise_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise ISEError(rc, self._L.ise_last_error(self.h).decode(errors="replace"))
+        return rc
+
+    def set_launch(self, group_lanes=0, waves_per_block=4, blocks=0, max_perms=1 << 26):
+        cfg = LaunchCfg(group_lanes, waves_per_block, blocks, max_perms)
+        self._check(self._L.ise_set_launch(self.h, ctypes.byref(cfg)))
+
+    def get_launch(self):
+        cfg = LaunchCfg()
+        self._check(self._L.ise_get_launch(self.h, ctypes.byref(cfg)))
+        return dict(group_lanes=cfg.group_lanes, waves_per_block=cfg.waves_per_block,
+                    blocks=cfg.blocks, max_perms=cfg.max_perms)
+
+    def check_batch(self, rows, meta, cells=None, gen_cap=0, fields=None):
+        """Canonicity of a batch: rows [n][stride] uint64, meta [n] uint32.
+        Returns the result arrays (see include/ise.h)."""
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32)
+        n = len(meta)
+        if rows.shape != (n, self.stride):
+            raise ValueError(f"rows must be [n][{self.stride}]")
+        if cells is not None:
+            cells = np.ascontiguousarray(cells, dtype=np.uint64)
+            if cells.shape != rows.shape:
+                raise ValueError("cells must have the shape of rows")
+        out = {k: [] for k in (fields or ())}
+        res_all = []
+        for off in range(0, n, self.capacity):
+            cnt = min(self.capacity, n - off)
+            self._check(self._L.ise_submit(self.h, cnt, self.stride, _ptr(rows[off:off + cnt]),
+                                           _ptr(None if cells is None else cells[off:off + cnt]),
+                                           _ptr(meta[off:off + cnt]), gen_cap))
+            r = alloc_results(cnt, gen_cap, fields)
+            s = results_struct(r, gen_cap)
+            self._check(self._L.ise_collect(self.h, ctypes.byref(s)))
+            res_all.append(r)
+        if not res_all:
+            return alloc_results(0, gen_cap, fields)
+        out = {k: np.concatenate([r[k] for r in res_all]) for k in res_all[0]}
+        return out
+
+    def last_kernel_ms(self):
+        return float(self._L.ise_last_kernel_ms(self.h))
+
+    def enumerate_bibd(self, v, k, lam, max_designs=1 << 16):
+        """Full enumeration; returns (designs [m][v] uint64, auts [m], stats, level_counts)."""
+        des = np.zeros((max_designs, v), np.uint64)
+        auts = np.zeros(max_designs, np.uint64)
+        stats = np.zeros(5, np.uint64)
+        levels = np.zeros(v + 1, np.uint64)
+        n = self._L.ise_enumerate_bibd(self.h, v, k, lam, _ptr(des), _ptr(auts), max_designs,
+                                       _ptr(stats), _ptr(levels))
+        if n < 0:
+            self._check(int(n))
+        m = min(n, max_designs)
+        return des[:m], auts[:m], dict(candidates=int(stats[0]), canonical_partial=int(stats[1]),
+                                       fully_constructed=int(stats[2]), designs=int(stats[3]),
+                                       kernel_ns=int(stats[4])), levels
+
+    def enumerator_candidates(self, v, k, lam, dmin, dmax, stride=None, cap=1 << 20):
+        """Every candidate the enumerator tests at depths dmin..dmax."""
+        stride = stride or self.stride
+        rows = np.zeros((cap, stride), np.uint64)
+        meta = np.zeros(cap, np.uint32)
+        nout = ctypes.c_long(0)
+        rc = self._L.ise_enumerate_bibd_candidates(self.h, v, k, lam, dmin, dmax, stride, _ptr(rows),
+                                                   _ptr(meta), cap, ctypes.byref(nout))
+        if rc < 0:
+            self._check(int(rc))
+        return rows[:nout.value], meta[:nout.value]
+
+
+def random_extensions(v, k, lam, n, dmin, dmax, seed, stride=32):
+    """Seeded random valid partial matrices (host code, no device needed)."""
+    rows = np.zeros((n, stride), np.uint64)
+    meta = np.zeros(n, np.uint32)
+    rc = lib().ise_random_extensions(v, k, lam, n, dmin, dmax, seed, stride, _ptr(rows), _ptr(meta))
+    if rc != OK:
+        raise ISEError(rc, "ise_random_extensions")
+    return rows, meta
+
+
+class CanonicityChecker:
+    """Reference-shaped single-matrix checker on the GPU."""
+
+    def __init__(self, nRow, nCol, rank=2, ctx=None):
+        if rank != 2:
+            raise ValueError("only 0/1 incidence matrices (rank 2)")
+        self.nRow, self.nCol = nRow, nCol
+        self._ctx = ctx or Context(capacity=64, stride=max(nRow, 1))
+        self._res = None
+        self.stabiliserLengthExt = 0
+
+    def TestCanonicity(self, nRowMax, matrix, outInfo=0, gen_cap=0):
+        mat = np.ascontiguousarray(np.asarray(matrix, dtype=np.uint8)[:nRowMax, :self.nCol])
+        rows = np.zeros((1, self._ctx.stride), np.uint64)
+        rows[0, :nRowMax] = pack_rows(mat)
+        meta = np.array([pack_meta(nRowMax, self.nCol, self.stabiliserLengthExt, outInfo)], np.uint32)
+        self._res = self._ctx.check_batch(rows, meta, gen_cap=gen_cap)
+        self._nrow = nRowMax
+        return bool(self._res["verdict"][0] == VERDICT_CANONICAL)
+
+    def permRow(self):
+        return self._res["perm_row"][0, :self._nrow].copy()
+
+    def permCol(self):
+        return self._res["perm_col"][0, :self.nCol].copy()
+
+    def orbits(self):
+        return self._res["orbits"][0, :self._nrow].copy()
+
+    def groupOrder(self):
+        return int(self._res["group_order"][0])
+
+    def rowToChange(self):
+        return int(self._res["row_to_change"][0])
+
+
+class GPUCanonChecker:
+    """assignChecker / LaunchCanonicityTesting batching like CGPU_CanonChecker."""
+
+    def __init__(self, nRow, nCol, batch=128, ctx=None):
+        self.nRow, self.nCol = nRow, nCol
+        self._ctx = ctx or Context(capacity=batch, stride=nRow)
+        self._L = lib()
+
+    def assignChecker(self, matrix, nRowMax=None, outInfo=SAVE_ROW_PERMUTATIONS | SAVE_ROW_TO_CHANGE):
+        mat = np.asarray(matrix, dtype=np.uint8)
+        d = nRowMax or mat.shape[0]
+        words = np.ascontiguousarray(pack_rows(mat[:d, :self.nCol]))
+        self._ctx._check(self._L.ise_assign(self._ctx.h, _ptr(words), d, self.nCol, 0, outInfo))
+
+    def numAssigned(self):
+        n = ctypes.c_uint32(0)
+        self._ctx._check(self._L.ise_num_assigned(self._ctx.h, ctypes.byref(n)))
+        return n.value
+
+    def LaunchCanonicityTesting(self):
+        n = self.numAssigned()
+        self._ctx._check(self._L.ise_submit(self._ctx.h, 0, self._ctx.stride, None, None, None, 0))
+        r = alloc_results(n)
+        s = results_struct(r)
+        self._ctx._check(self._L.ise_collect(self._ctx.h, ctypes.byref(s)))
+        return r
+
+
+__all__ = [
+    "Context", "CanonicityChecker", "GPUCanonChecker", "ISEError", "random_extensions", "lib",
+    "design_params", "alloc_results", "pack_meta", "pack_rows", "unpack_rows", "MAX_ROWS",
+    "MAX_COLS", "SAVE_ROW_PERMUTATIONS", "SAVE_COL_PERMUTATIONS", "SAVE_ROW_TO_CHANGE",
+    "VERDICT_CANONICAL", "VERDICT_NONCANONICAL", "VERDICT_LIMIT", "VERDICT_INVALID",
+]

@@ -1,0 +1,185 @@
+/*
+ * ise.h -- C ABI of the MI355X batched canonicity test.
+ *
+ * Plain C: pointers, sizes and status codes only.  One context per host
+ * thread (the reference keeps one CGPU_CanonChecker per enumerator thread,
+ * Source/ThreadEnumerator.h:96, each with its own stream,
+ * Source/MatrixCanonChecker.h:264); calls on one context are not thread-safe.
+ *
+ * Wire format of a batch of BIBD-type candidates (all little-endian):
+ *   rows  [n][stride] uint64  bit c of word i = entry (row i, column c)
+ *   cells [n][stride] uint64  column-orbit start mask of row i, i.e. bit c set
+ *                             when c == 0 or columns c-1 and c differ in some
+ *                             row j < i.  May be NULL: it is then derived from
+ *                             the rows (the candidate's columns must be
+ *                             grouped so that orbits are contiguous with ones
+ *                             left-justified, as MakeRow builds them,
+ *                             Source/MatrixCanonChecker.cpp:14-93).
+ *   meta  [n] uint32          depth | ncols << 8 | lenStab << 16 | outInfo << 24
+ * with 1 <= depth <= 32, 1 <= ncols <= 64, depth <= stride.
+ *
+ * Results (structure of arrays; any pointer except verdict may be NULL):
+ *   verdict        [n] uint8   1 canonical, 0 not canonical, 2 search limit hit
+ *   fail_row       [n] uint8   row at which the permuted matrix was larger (255)
+ *   fail_col       [n] uint8   first column of the deciding column orbit (255)
+ *   row_to_change  [n] uint8   CCanonicityChecker::rowToChange of fail_row (255)
+ *   group_order    [n] uint64  |Aut| (row action) when outInfo has
+ *                              ISE_SAVE_ROW_PERMUTATIONS and canonical, else
+ *                              1 (canonical) / 0 (not canonical)
+ *   perm_row       [n][32] uint8  row permutation at exit (witness if verdict 0)
+ *   perm_col       [n][64] uint8  column permutation at exit
+ *   orbits         [n][32] uint8  row orbits under the automorphisms found
+ *   n_gens         [n] uint16  automorphisms found
+ *   gens           [n][gen_cap][64] uint8  per automorphism: the permutation
+ *                              it induces on the column orbits of the next row
+ *                              (partial check) or the row permutation
+ *                              (ISE_SAVE_ROW_PERMUTATIONS); unused bytes 255
+ *   counters       [n][4] uint32  permutations tried, rows compared, column
+ *                              orbits compared, columns examined
+ */
+#ifndef ISE_H
+#define ISE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISE_MAX_ROWS 32
+#define ISE_MAX_COLS 64
+
+/* outInfo flags: same bits as t_canonOutInfo (Source/CanonicityChecker.h:27-32) */
+#define ISE_SAVE_ROW_PERMUTATIONS 1u
+#define ISE_SAVE_COL_PERMUTATIONS 2u
+#define ISE_SAVE_ROW_TO_CHANGE    4u
+
+#define ISE_VERDICT_NONCANONICAL 0
+#define ISE_VERDICT_CANONICAL    1
+#define ISE_VERDICT_LIMIT        2
+#define ISE_VERDICT_INVALID      3   /* record failed validation on the device */
+
+/* status codes */
+#define ISE_OK            0
+#define ISE_ERR_ARG      -1   /* invalid argument or candidate            */
+#define ISE_ERR_HIP      -2   /* a HIP runtime call failed                */
+#define ISE_ERR_NODEV    -3   /* no usable gfx950 device                  */
+#define ISE_ERR_CAPACITY -4   /* more candidates than the context holds   */
+#define ISE_ERR_STATE    -5   /* call out of order (collect before submit) */
+
+typedef struct ise_ctx ise_ctx;
+
+typedef struct {
+    uint8_t  *verdict;
+    uint8_t  *fail_row;
+    uint8_t  *fail_col;
+    uint8_t  *row_to_change;
+    uint64_t *group_order;
+    uint8_t  *perm_row;
+    uint8_t  *perm_col;
+    uint8_t  *orbits;
+    uint16_t *n_gens;
+    uint8_t  *gens;
+    uint32_t *counters;
+    uint32_t  gen_cap;
+} ise_results;
+
+/* Launch configuration of the canonicity kernel. */
+typedef struct {
+    uint32_t group_lanes;     /* lanes per candidate: 16, 32, 64; 0 = smallest that fits */
+    uint32_t waves_per_block; /* 1..16 */
+    uint32_t blocks;          /* persistent grid size; 0 = from occupancy */
+    uint32_t max_perms;       /* search limit per candidate (verdict 2 beyond it) */
+} ise_launch_cfg;
+
+/* Open a context on HIP device `device` able to hold `capacity` candidates of
+ * up to `stride` rows per submit.  Replaces the construction of
+ * CGPU_CanonChecker (Source/MatrixCanonChecker.h:204-290) and the CDTools
+ * createCanonizer (k-Factorization/Utils/Include/CDTools.h:9). */
+int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride);
+
+/* Release the context (CGPU_CanonChecker dtor; CDTools releaseCanonizer,
+ * CDTools.h:10). */
+int ise_close(ise_ctx *ctx);
+
+int ise_set_launch(ise_ctx *ctx, const ise_launch_cfg *cfg);
+int ise_get_launch(const ise_ctx *ctx, ise_launch_cfg *cfg);
+
+/* Queue one matrix for the next batch (CGPU_CanonChecker::assignChecker,
+ * Source/MatrixCanonChecker.h:248-256).  rows: `depth` packed words. */
+int ise_assign(ise_ctx *ctx, const uint64_t *rows, uint32_t depth, uint32_t ncols,
+               uint32_t len_stab, uint32_t out_info);
+/* Number of matrices queued since the last launch. */
+int ise_num_assigned(const ise_ctx *ctx, uint32_t *n);
+
+/* Submit a whole batch from host memory (copied to the context's device
+ * buffers) and launch asynchronously on the context stream
+ * (CGPU_CanonChecker::LaunchCanonicityTesting, MatrixCanonChecker.h:303-329).
+ * n == 0 with rows == NULL launches the matrices queued by ise_assign. */
+int ise_submit(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *rows,
+               const uint64_t *cells, const uint32_t *meta, uint32_t gen_cap);
+
+/* Wait for the last submit and copy its results into host arrays. */
+int ise_collect(ise_ctx *ctx, ise_results *out_host);
+
+/* Synchronous single-matrix test: CCanonicityChecker::TestCanonicity
+ * (Source/CanonicityChecker.h:58) on a row-major uint8 0/1 matrix. */
+int ise_test_canonicity(ise_ctx *ctx, const uint8_t *mat, uint32_t nrows, uint32_t ncols,
+                        uint32_t len_stab, uint32_t out_info, uint32_t gen_cap,
+                        ise_results *out_host);
+
+/* Device-resident batch: every pointer is device memory, results are written
+ * in place; enqueued on `stream` (hipStream_t, NULL = the context stream).
+ * `d_counter` is one uint32 of device memory used as the work-queue head
+ * (cleared by the call).  No allocation happens in this call. */
+int ise_check_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride,
+                           const uint64_t *d_rows, const uint64_t *d_cells,
+                           const uint32_t *d_meta, const ise_results *d_out,
+                           uint32_t *d_counter, void *stream);
+
+/* Wait for all work on the context stream. */
+int ise_sync(ise_ctx *ctx);
+
+/* Kernel time of the last collected ise_submit in milliseconds (HIP events
+ * around the kernel launches on the context stream). */
+float ise_last_kernel_ms(const ise_ctx *ctx);
+
+/* Last HIP error string of the context ("" if none). */
+const char *ise_last_error(const ise_ctx *ctx);
+
+/* ---- host-side candidate producers (stay on the CPU) -------------------- */
+
+/* Level-synchronous enumeration of 2-(v,k,lambda) designs: every level's
+ * candidates are checked as one batch on the GPU (the reference checks them
+ * one by one after every row, Source/Enumerator.cpp:435-501, 350-433).
+ * Canonical designs go to designs[] (v packed words each, in the order of
+ * discovery, up to max_designs) and |Aut| to auts[].
+ * stats[0..3] = candidates tested, canonical partial matrices, fully
+ * constructed matrices, designs; stats[4] = GPU kernel time in ns.
+ * level_counts (may be NULL) receives per depth 1..v the candidates tested
+ * ([v+1] entries).  Returns designs found (>= 0) or a status code. */
+long ise_enumerate_bibd(ise_ctx *ctx, int v, int k, int lambda, uint64_t *designs,
+                        uint64_t *auts, long max_designs, uint64_t *stats,
+                        uint64_t *level_counts);
+
+/* Same enumeration, but every candidate at depth in [dmin,dmax] is also
+ * copied out (stride words each, up to cap; *n_out receives the count):
+ * the "enumerator output" part of the candidate streams. */
+long ise_enumerate_bibd_candidates(ise_ctx *ctx, int v, int k, int lambda,
+                                   int dmin, int dmax, uint32_t stride,
+                                   uint64_t *rows_out, uint32_t *meta_out,
+                                   long cap, long *n_out);
+
+/* Seeded random valid extensions: each new row takes r columns uniformly
+ * among those whose weight is below k and that keep every pairwise row
+ * intersection <= lambda (restart when stuck); columns are then sorted in
+ * decreasing lexicographic order so that orbits are contiguous.  Depth is
+ * uniform in [dmin, dmax].  Writes n candidates (stride words each) and
+ * meta.  Deterministic for a given seed.  Returns ISE_OK or a status. */
+int ise_random_extensions(int v, int k, int lambda, uint64_t n, int dmin, int dmax,
+                          uint64_t seed, uint32_t stride, uint64_t *rows_out,
+                          uint32_t *meta_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -433,9 +433,15 @@ int orc_bruteforce_canon(const uint8_t *mat, int nrows, int ncols,
     for (;;) {
         canon_for_rows(mat, nrows, ncols, rp, cur);
         if (!have || memcmp(cur, best, sizeof(uint64_t) * ncols) != 0) {
+            /* compare the two column-sorted matrices row by row, column 0 first */
             int better = !have;
-            for (int q = 0; q < ncols && !better; q++) {
-                if (cur[q] != best[q]) { better = cur[q] > best[q]; break; }
+            for (int r = 0; r < nrows && have; r++) {
+                int decided = 0;
+                for (int q = 0; q < ncols; q++) {
+                    int x = (int)((cur[q] >> (nrows - 1 - r)) & 1), y = (int)((best[q] >> (nrows - 1 - r)) & 1);
+                    if (x != y) { better = x > y; decided = 1; break; }
+                }
+                if (decided) break;
             }
             if (better) { memcpy(best, cur, sizeof(uint64_t) * ncols); have = 1; }
         }

@@ -1,0 +1,114 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle_ise.so).
+
+The oracle is test infrastructure: only tests, the smoke check and the CPU
+baseline of the benchmark load it.  It is built from oracle/ with make when
+the shared library is missing.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "liboracle_ise.so")
+
+_P = ctypes.c_void_p
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        U32 = ctypes.c_uint32
+        L.orc_check_batch.argtypes = [U32, U32, _P, _P, _P, U32, _P]
+        L.orc_test_canonicity.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, U32]
+        L.orc_cell_masks.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
+        L.orc_enumerate_bibd.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_long,
+                                         _P, _P, _P, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_enumerate_bibd.restype = ctypes.c_long
+        L.orc_bruteforce_canon.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P]
+        if hasattr(L, "orc_kf_check"):
+            L.orc_kf_check.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
+            L.orc_kf_normalize.argtypes = [_P, ctypes.c_int, ctypes.c_int]
+        _lib = Oracle(L)
+    return _lib
+
+
+class Oracle:
+    def __init__(self, L):
+        self.L = L
+
+    def check_batch(self, rows, meta, cells=None, gen_cap=0):
+        import ise_amd._abi as abi
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32)
+        n, stride = rows.shape
+        res = abi.alloc_results(n, gen_cap)
+        s = abi.results_struct(res, gen_cap)
+        cells_p = None
+        if cells is not None:
+            cells = np.ascontiguousarray(cells, dtype=np.uint64)
+            cells_p = cells.ctypes.data
+        rc = self.L.orc_check_batch(n, stride, rows.ctypes.data, cells_p, meta.ctypes.data, gen_cap,
+                                    ctypes.addressof(s))
+        if rc != 0:
+            raise ValueError(f"oracle rejected the batch ({rc})")
+        return res
+
+    def cell_masks(self, rows, nrows, ncols):
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        out = np.zeros(nrows, np.uint64)
+        self.L.orc_cell_masks(rows.ctypes.data, nrows, ncols, out.ctypes.data)
+        return out
+
+    def enumerate(self, v, k, lam, max_designs=1 << 16, capture=None):
+        """DFS enumeration: (designs [m][v], auts [m], stats[5], (cand rows, cand meta))."""
+        des = np.zeros((max_designs, v), np.uint64)
+        auts = np.zeros(max_designs, np.uint64)
+        st = np.zeros(5, np.uint64)
+        crow = cmeta = None
+        cap, stride, dmin, dmax = 0, 0, 0, 0
+        if capture:
+            cap, stride, dmin, dmax = capture
+            crow = np.zeros((cap, stride), np.uint64)
+            cmeta = np.zeros(cap, np.uint32)
+        n = self.L.orc_enumerate_bibd(v, k, lam, des.ctypes.data, auts.ctypes.data, max_designs,
+                                      st.ctypes.data, None if crow is None else crow.ctypes.data,
+                                      None if cmeta is None else cmeta.ctypes.data, cap, stride,
+                                      dmin, dmax)
+        if n < 0:
+            raise ValueError(f"oracle enumerate failed ({n})")
+        m = min(n, max_designs)
+        cands = None
+        if capture:
+            c = int(st[4])
+            cands = (crow[:c], cmeta[:c])
+        return des[:m], auts[:m], st, cands
+
+    def test_canonicity(self, mat, len_stab=0, out_info=0, gen_cap=0):
+        import ise_amd._abi as abi
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        res = abi.alloc_results(1, gen_cap)
+        s = abi.results_struct(res, gen_cap)
+        self.L.orc_test_canonicity(mat.ctypes.data, mat.shape[0], mat.shape[1], len_stab, out_info,
+                                   ctypes.addressof(s), gen_cap)
+        return res
+
+    def bruteforce(self, mat):
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        out = np.zeros_like(mat)
+        auts = np.zeros(1, np.uint64)
+        rc = self.L.orc_bruteforce_canon(mat.ctypes.data, mat.shape[0], mat.shape[1], out.ctypes.data,
+                                         auts.ctypes.data)
+        if rc != 0:
+            raise ValueError("bruteforce: matrix too large")
+        return out, int(auts[0])
