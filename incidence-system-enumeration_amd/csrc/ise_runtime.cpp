@@ -311,6 +311,63 @@ int ise_check_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64
     return launch(c, n, stride, d_rows, d_cells, d_meta, *d_out, d_counter, s, group);
 }
 
+int ise_check_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                                 const uint64_t *d_cells, const uint32_t *d_meta,
+                                 const ise_results *d_out, uint32_t *d_counter, uint32_t repeats,
+                                 float *ms_out) {
+    if (!c || !ms_out || repeats == 0) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<hipEvent_t> ev(repeats + 1, nullptr);
+    int rc = ISE_OK;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = ISE_ERR_HIP;
+    if (rc == ISE_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) rc = ISE_ERR_HIP;
+    for (uint32_t i = 0; i < repeats && rc == ISE_OK; i++) {
+        rc = ise_check_batch_device(c, n, stride, d_rows, d_cells, d_meta, d_out, d_counter, nullptr);
+        if (rc == ISE_OK && hipEventRecord(ev[i + 1], c->stream) != hipSuccess) rc = ISE_ERR_HIP;
+    }
+    if (rc == ISE_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = ISE_ERR_HIP;
+    for (uint32_t i = 0; i < repeats && rc == ISE_OK; i++)
+        if (hipEventElapsedTime(&ms_out[i], ev[i], ev[i + 1]) != hipSuccess) rc = ISE_ERR_HIP;
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc == ISE_ERR_HIP && c->err.empty()) c->err = "timed launch failed";
+    return rc;
+}
+
+int ise_device_alloc(ise_ctx *c, void **d_ptr, uint64_t bytes) {
+    if (!c || !d_ptr || bytes == 0) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMalloc(d_ptr, bytes));
+    HIPCHK(c, hipMemsetAsync(*d_ptr, 0, bytes, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISE_OK;
+}
+
+int ise_device_free(ise_ctx *c, void *d_ptr) {
+    if (!c) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (d_ptr) HIPCHK(c, hipFree(d_ptr));
+    return ISE_OK;
+}
+
+int ise_copy_to_device(ise_ctx *c, void *d_dst, const void *src, uint64_t bytes) {
+    if (!c || !d_dst || !src) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISE_OK;
+}
+
+int ise_copy_to_host(ise_ctx *c, void *dst, const void *d_src, uint64_t bytes) {
+    if (!c || !dst || !d_src) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISE_OK;
+}
+
 int ise_sync(ise_ctx *c) {
     if (!c) return ISE_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -62,6 +62,12 @@ def lib():
         L.ise_collect.argtypes = [P, ctypes.POINTER(ResultsStruct)]
         L.ise_test_canonicity.argtypes = [P, P, U32, U32, U32, U32, U32, ctypes.POINTER(ResultsStruct)]
         L.ise_check_batch_device.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P, P]
+        L.ise_check_batch_device_timed.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P,
+                                                   U32, P]
+        L.ise_device_alloc.argtypes = [P, ctypes.POINTER(P), U64]
+        L.ise_device_free.argtypes = [P, P]
+        L.ise_copy_to_device.argtypes = [P, P, P, U64]
+        L.ise_copy_to_host.argtypes = [P, P, P, U64]
         L.ise_sync.argtypes = [P]
         L.ise_last_error.argtypes = [P]
         L.ise_last_error.restype = ctypes.c_char_p
@@ -169,6 +175,38 @@ class Context:
         out = {k: np.concatenate([r[k] for r in res_all]) for k in res_all[0]}
         return out
 
+    # ---- device-resident batches ------------------------------------------
+    def device_array(self, shape, dtype, host=None):
+        """Zero-filled device buffer (optionally filled from a host array)."""
+        return DeviceArray(self, shape, dtype, host)
+
+    def device_results(self, n, fields, gen_cap=0):
+        """Device result arrays for n candidates; returns (dict, ResultsStruct)."""
+        host_like = alloc_results(n, gen_cap, fields)
+        dev = {k: self.device_array(a.shape, a.dtype) for k, a in host_like.items()}
+        s = ResultsStruct()
+        for k in ("verdict", "fail_row", "fail_col", "row_to_change", "group_order", "perm_row",
+                  "perm_col", "orbits", "n_gens", "gens", "counters"):
+            setattr(s, k, dev[k].ptr if k in dev else None)
+        s.gen_cap = gen_cap if "gens" in dev else 0
+        return dev, s
+
+    def check_batch_device(self, n, stride, d_rows, d_meta, res_struct, d_head, d_cells=None):
+        self._check(self._L.ise_check_batch_device(self.h, n, stride, d_rows.ptr,
+                                                   d_cells.ptr if d_cells is not None else None,
+                                                   d_meta.ptr, ctypes.byref(res_struct), d_head.ptr, None))
+
+    def time_batch_device(self, n, stride, d_rows, d_meta, res_struct, d_head, repeats, d_cells=None):
+        """Launch `repeats` times; per-launch milliseconds from HIP events."""
+        ms = np.zeros(repeats, np.float32)
+        self._check(self._L.ise_check_batch_device_timed(
+            self.h, n, stride, d_rows.ptr, d_cells.ptr if d_cells is not None else None, d_meta.ptr,
+            ctypes.byref(res_struct), d_head.ptr, repeats, ms.ctypes.data))
+        return ms
+
+    def sync(self):
+        self._check(self._L.ise_sync(self.h))
+
     def last_kernel_ms(self):
         return float(self._L.ise_last_kernel_ms(self.h))
 
@@ -198,6 +236,43 @@ class Context:
         if rc < 0:
             self._check(int(rc))
         return rows[:nout.value], meta[:nout.value]
+
+
+class DeviceArray:
+    """A device buffer of a context (freed with the object)."""
+
+    def __init__(self, ctx, shape, dtype, host=None):
+        self.ctx = ctx
+        self.shape = tuple(shape) if np.ndim(shape) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        ctx._check(ctx._L.ise_device_alloc(ctx.h, ctypes.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        if host.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        self.ctx._check(self.ctx._L.ise_copy_to_device(self.ctx.h, self.ptr, host.ctypes.data, self.nbytes))
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(self.ctx._L.ise_copy_to_host(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            self.ctx._L.ise_device_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def random_extensions(v, k, lam, n, dmin, dmax, seed, stride=32):
@@ -275,7 +350,7 @@ class GPUCanonChecker:
 
 
 __all__ = [
-    "Context", "CanonicityChecker", "GPUCanonChecker", "ISEError", "random_extensions", "lib",
+    "Context", "DeviceArray", "ResultsStruct", "CanonicityChecker", "GPUCanonChecker", "ISEError", "random_extensions", "lib",
     "design_params", "alloc_results", "pack_meta", "pack_rows", "unpack_rows", "MAX_ROWS",
     "MAX_COLS", "SAVE_ROW_PERMUTATIONS", "SAVE_COL_PERMUTATIONS", "SAVE_ROW_TO_CHANGE",
     "VERDICT_CANONICAL", "VERDICT_NONCANONICAL", "VERDICT_LIMIT", "VERDICT_INVALID",

@@ -136,6 +136,22 @@ int ise_check_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride,
                            const uint32_t *d_meta, const ise_results *d_out,
                            uint32_t *d_counter, void *stream);
 
+/* Device-resident batch launched `repeats` times back to back on the context
+ * stream; ms_out[i] receives the time of launch i from HIP events recorded on
+ * that stream around it.  Same arguments and results as
+ * ise_check_batch_device.  Synchronous. */
+int ise_check_batch_device_timed(ise_ctx *ctx, uint32_t n, uint32_t stride,
+                                 const uint64_t *d_rows, const uint64_t *d_cells,
+                                 const uint32_t *d_meta, const ise_results *d_out,
+                                 uint32_t *d_counter, uint32_t repeats, float *ms_out);
+
+/* Device memory owned by the caller through the context's device: allocate
+ * (zero-filled), free, and synchronous copies on the context stream. */
+int ise_device_alloc(ise_ctx *ctx, void **d_ptr, uint64_t bytes);
+int ise_device_free(ise_ctx *ctx, void *d_ptr);
+int ise_copy_to_device(ise_ctx *ctx, void *d_dst, const void *src, uint64_t bytes);
+int ise_copy_to_host(ise_ctx *ctx, void *dst, const void *d_src, uint64_t bytes);
+
 /* Wait for all work on the context stream. */
 int ise_sync(ise_ctx *ctx);
 

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import ise_amd
-from _helpers import check_witness, counts_for, design_key, golden, hist, unpack
+from _helpers import check_witness, col_sorted, counts_for, design_key, golden, hist, unpack
 
 pytestmark = pytest.mark.gpu
 
@@ -50,16 +50,17 @@ def test_enumerator_candidates_parity(gpu_ctx, oracle, v, k, lam, group):
 
 
 @pytest.mark.parametrize("with_cells", [False, True])
-@pytest.mark.parametrize("group", [0, 32, 64])
-def test_random_extensions_parity(gpu_ctx, oracle, with_cells, group):
-    print(f"case random 2-(15,3,1) cells={with_cells} group={group}", flush=True)
-    rows, meta = ise_amd.random_extensions(15, 3, 1, 1 << 13, 4, 15, 0x15031 + group)
+@pytest.mark.parametrize("v,k,lam,group", [(15, 3, 1, 0), (15, 3, 1, 64), (13, 3, 1, 32), (9, 3, 1, 16)])
+def test_random_extensions_parity(gpu_ctx, oracle, with_cells, v, k, lam, group):
+    print(f"case random {(v, k, lam)} cells={with_cells} group={group}", flush=True)
+    b = ise_amd.design_params(v, k, lam)[1]
+    rows, meta = ise_amd.random_extensions(v, k, lam, 1 << 13, 4, v, 0x15031 + group)
     cells = None
     if with_cells:
         cells = np.zeros_like(rows)
         for i in range(len(rows)):
             d = int(meta[i]) & 0xff
-            cells[i, :d] = oracle.cell_masks(rows[i, :d], d, 35)
+            cells[i, :d] = oracle.cell_masks(rows[i, :d], d, b)
     gpu_ctx.set_launch(group_lanes=group)
     try:
         g = gpu_ctx.check_batch(rows, meta, cells=cells, gen_cap=4)
@@ -70,7 +71,18 @@ def test_random_extensions_parity(gpu_ctx, oracle, with_cells, group):
     # the reported witness really makes the matrix larger
     for i in np.nonzero(g["verdict"] == 0)[0][:200]:
         d = int(meta[i]) & 0xff
-        check_witness(unpack(rows[i, :d], 35), g["perm_row"][i], int(g["fail_row"][i]))
+        check_witness(unpack(rows[i, :d], b), g["perm_row"][i], int(g["fail_row"][i]))
+
+
+def test_group_too_small_is_rejected(gpu_ctx):
+    rows, meta = ise_amd.random_extensions(15, 3, 1, 16, 4, 15, 3)
+    gpu_ctx.set_launch(group_lanes=32)
+    try:
+        with pytest.raises(ise_amd.ISEError) as e:
+            gpu_ctx.check_batch(rows, meta)
+        assert e.value.code == ise_amd.ERR_ARG
+    finally:
+        gpu_ctx.set_launch()
 
 
 @pytest.mark.parametrize("v,k,lam", [(15, 7, 3), (16, 6, 2), (19, 9, 4)])
@@ -130,7 +142,6 @@ def test_edge_shapes(gpu_ctx, oracle):
     for d, b in [(1, 1), (1, 64), (32, 1), (2, 64), (32, 64), (20, 64), (32, 40), (16, 16), (17, 16)]:
         for _ in range(8):
             mat = (rng.random((d, b)) < 0.5).astype(np.uint8)
-            from _helpers import col_sorted
             mat = col_sorted(mat)
             w = np.zeros(32, np.uint64)
             w[:d] = ise_amd.pack_rows(mat)
@@ -159,8 +170,12 @@ def test_reference_shaped_facades(oracle):
         mat = unpack(d, 26)
         assert chk.TestCanonicity(13, mat, ise_amd.SAVE_ROW_PERMUTATIONS)
         assert chk.groupOrder() == int(a)
-        swapped = mat[[1, 0] + list(range(2, 13))]
-        assert not chk.TestCanonicity(13, swapped, ise_amd.SAVE_ROW_TO_CHANGE)
+        # another row order, columns re-sorted into orbit form
+        other = col_sorted(mat[::-1])
+        same = bool((other == mat).all())
+        assert chk.TestCanonicity(13, other, ise_amd.SAVE_ROW_TO_CHANGE) == same
+        if not same:
+            assert chk.rowToChange() < 13
     gpu = ise_amd.GPUCanonChecker(13, 26, batch=128)
     for d in des:
         gpu.assignChecker(unpack(d, 26))
@@ -171,35 +186,26 @@ def test_reference_shaped_facades(oracle):
 
 
 def test_device_resident_entry_point(gpu_ctx, oracle):
-    """ise_check_batch_device on torch-allocated HBM buffers."""
-    torch = pytest.importorskip("torch")
+    """ise_check_batch_device / _timed on HBM-resident buffers."""
     print("case device entry point", flush=True)
-    rows, meta = ise_amd.random_extensions(15, 3, 1, 4096, 4, 15, 77)
-    dev = torch.device("cuda:0")
-    d_rows = torch.from_numpy(rows.view(np.int64)).to(dev)
-    d_meta = torch.from_numpy(meta.view(np.int32)).to(dev)
+    rows, meta = ise_amd.random_extensions(15, 3, 1, 4096, 4, 15, 77, stride=16)
     n = len(meta)
-    outs = {
-        "verdict": torch.zeros(n, dtype=torch.uint8, device=dev),
-        "fail_row": torch.zeros(n, dtype=torch.uint8, device=dev),
-        "perm_row": torch.zeros((n, 32), dtype=torch.uint8, device=dev),
-        "group_order": torch.zeros(n, dtype=torch.int64, device=dev),
-    }
-    head = torch.zeros(1, dtype=torch.int32, device=dev)
-    s = ise_amd.ResultsStruct()
-    for k, t in outs.items():
-        setattr(s, k, t.data_ptr())
-    L = ise_amd.lib()
-    stream = torch.cuda.current_stream().cuda_stream
-    rc = L.ise_check_batch_device(gpu_ctx.h, n, 32, d_rows.data_ptr(), None, d_meta.data_ptr(),
-                                  ctypes.byref(s), head.data_ptr(), stream)
-    assert rc == 0
-    torch.cuda.synchronize()
-    o = oracle.check_batch(rows, meta)
-    assert np.array_equal(outs["verdict"].cpu().numpy(), o["verdict"])
-    assert np.array_equal(outs["fail_row"].cpu().numpy(), o["fail_row"])
-    assert np.array_equal(outs["perm_row"].cpu().numpy(), o["perm_row"])
-    assert np.array_equal(outs["group_order"].cpu().numpy().view(np.uint64), o["group_order"])
+    d_rows = gpu_ctx.device_array(rows.shape, np.uint64, rows)
+    d_meta = gpu_ctx.device_array(meta.shape, np.uint32, meta)
+    d_head = gpu_ctx.device_array((1,), np.uint32)
+    fields = ("verdict", "fail_row", "perm_row", "group_order", "counters")
+    dev, s = gpu_ctx.device_results(n, fields)
+    gpu_ctx.check_batch_device(n, 16, d_rows, d_meta, s, d_head)
+    gpu_ctx.sync()
+    rows32 = np.zeros((n, 32), np.uint64)
+    rows32[:, :16] = rows
+    o = oracle.check_batch(rows32, meta)
+    for f in fields:
+        assert np.array_equal(dev[f].to_host(), o[f]), f
+    ms = gpu_ctx.time_batch_device(n, 16, d_rows, d_meta, s, d_head, 3)
+    assert (ms > 0).all()
+    for f in fields:
+        assert np.array_equal(dev[f].to_host(), o[f]), f
 
 
 def test_full_size_stress_batch_properties(gpu_ctx, oracle):
