@@ -28,12 +28,27 @@ SEED = 0x15031
 RESULT_BYTES = 1 + 1 + 1 + 8 + 32   # verdict, fail_row, row_to_change, |Aut|, witness row perm
 
 
-def build_batch(ctx, ise_amd, n_random):
-    rows_e, meta_e = ctx.enumerator_candidates(15, 3, 1, 4, 15, stride=STRIDE, cap=1 << 20)
+def build_batch(n_random, path):
+    """Enumerator partials (the enumerator checks each level on the GPU) plus
+    random extensions, written to `path`.  Runs in a child process so that the
+    measured process launches nothing but the timed batches."""
+    import ise_amd
+    with ise_amd.Context(capacity=1 << 16, stride=32) as ctx:
+        rows_e, meta_e = ctx.enumerator_candidates(15, 3, 1, 4, 15, stride=STRIDE, cap=1 << 20)
     rows_r, meta_r = ise_amd.random_extensions(15, 3, 1, n_random, 4, 15, SEED, stride=STRIDE)
-    rows = np.concatenate([rows_e, rows_r])
-    meta = np.concatenate([meta_e, meta_r])
-    return rows, meta, len(meta_e)
+    np.savez(path, rows=np.concatenate([rows_e, rows_r]), meta=np.concatenate([meta_e, meta_r]),
+             n_enum=np.array([len(meta_e)]))
+
+
+def load_batch(n_random):
+    import subprocess
+    import tempfile
+    path = os.path.join(tempfile.gettempdir(), f"ise_bench_15_3_1_{n_random}_{SEED:x}.npz")
+    if not os.path.exists(path):
+        subprocess.run([sys.executable, os.path.abspath(__file__), "--build-batch", path,
+                        "--random", str(n_random)], check=True)
+    z = np.load(path)
+    return z["rows"], z["meta"], int(z["n_enum"][0])
 
 
 def cpu_baseline(rows, meta, seconds=15.0):
@@ -44,17 +59,30 @@ def cpu_baseline(rows, meta, seconds=15.0):
     orc = _oracle.load()
     step = 4096
     done, t0 = 0, time.perf_counter()
-    # interleaved sample: every 4th candidate, so both parts of the stream appear,
-    # taken in order until the time budget is spent
-    idx = np.arange(0, len(meta), 4)
+    # the batch in a seeded random order (so both parts of the stream are in
+    # the sample) until the time budget is spent
+    idx = np.random.default_rng(0).permutation(len(meta))
     while done < len(idx) and time.perf_counter() - t0 < seconds:
         sel = idx[done:done + step]
         orc.check_batch(np.ascontiguousarray(rows[sel]), np.ascontiguousarray(meta[sel]))
         done += len(sel)
     dt = time.perf_counter() - t0
     return dict(value=done / dt, unit="candidates/s", cores=1, kind="port",
-                sample=f"first {done} of every 4th candidate of the same batch, oracle single thread, "
-                       f"{dt:.1f} s")
+                sample=f"{done} candidates of the same batch (seeded random order), oracle single "
+                       f"thread, {dt:.1f} s")
+
+
+def profiled_launch_figures():
+    """HBM bytes and instruction counts of one launch of this workload, as
+    measured by rocprofv3 PMC runs and committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", "check_kernel64_2-15-3-1_stream_traffic.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                out[k.strip()] = float(v)
+    return out
 
 
 def main():
@@ -66,17 +94,24 @@ def main():
     ap.add_argument("--group-lanes", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--build-batch", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="also run one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is a design "
                                    "note (candidates are independent), not a measured path"}))
         sys.exit(2)
 
+    if args.build_batch:
+        build_batch(args.random, args.build_batch)
+        return
+
+    t_setup = time.time()
+    rows, meta, n_enum = load_batch(args.random)
     import ise_amd
 
-    ctx = ise_amd.Context(capacity=1 << 16, stride=32)
-    t_setup = time.time()
-    rows, meta, n_enum = build_batch(ctx, ise_amd, args.random)
+    ctx = ise_amd.Context(capacity=1 << 10, stride=32)
     n = len(meta)
     depth = (meta & 0xff).astype(np.int64)
     print(f"bench: batch of {n} candidates ({n_enum} enumerator, {n - n_enum} random) built in "
@@ -85,8 +120,8 @@ def main():
     # inputs resident in HBM before the timed region
     d_rows = ctx.device_array(rows.shape, np.uint64, rows)
     d_meta = ctx.device_array(meta.shape, np.uint32, meta)
-    d_head = ctx.device_array((1,), np.uint32)
-    fields = ("verdict", "fail_row", "row_to_change", "group_order", "perm_row", "counters")
+    d_head = ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
+    fields = ("verdict", "fail_row", "row_to_change", "group_order", "perm_row")
     out, res = ctx.device_results(n, fields)
     ctx.set_launch(group_lanes=args.group_lanes, waves_per_block=args.waves_per_block)
 
@@ -99,7 +134,12 @@ def main():
     value = n / (ms / 1e3)
 
     verdict = out["verdict"].to_host()
-    counters = out["counters"].to_host()
+    # statistics from one extra, untimed launch with the counters compiled in
+    cout, cres = ctx.device_results(n, fields + ("counters",))
+    ctx.check_batch_device(n, STRIDE, d_rows, d_meta, cres, d_head)
+    counters = cout["counters"].to_host()
+    if not np.array_equal(cout["verdict"].to_host(), verdict):
+        raise RuntimeError("verdicts differ between the counted and the timed kernel")
     if (verdict > 1).any():
         raise RuntimeError("search limit hit or invalid record in the bench batch")
     rej = verdict == 0
@@ -109,6 +149,7 @@ def main():
     hbm_bytes = float((depth * 8 + 4).sum() + n * RESULT_BYTES)
     achieved_gbs = hbm_bytes / (ms / 1e3) / 1e9
     rows_compared = float(counters[:, 1].sum())
+    prof = profiled_launch_figures()
 
     line = {
         "metric": METRIC,
@@ -132,8 +173,14 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved_gbs / HBM_PEAK_GBS,
+            "traffic": (prof["fetch_bytes"] + prof["write_bytes"]) if prof else None,
         },
+        # the binding resource: instruction issue (rocprofv3 SQ counters of one launch)
+        "issue": ({"valu_frac": prof["valu_insts"] / prof["cu_cycles"],
+                   "salu_frac": prof["salu_insts"] / prof["cu_cycles"],
+                   "peak": "1 wave64 VALU + 1 SALU instruction per CU per cycle",
+                   "source": "profiles/check_kernel64_2-15-3-1_stream.md"} if prof else None),
         "canonical": int((verdict == 1).sum()),
         "noncanonical": int(rej.sum()),
         "mean_perms_to_reject_plain": perms_to_reject,
@@ -141,9 +188,15 @@ def main():
         "wall_s_timed": wall,
     }
     line["ms_per_step_min"] = float(np.min(per_step_ms))
+    if args.copy_bytes:
+        src = ctx.device_array((args.copy_bytes,), np.uint8)
+        dst = ctx.device_array((args.copy_bytes,), np.uint8)
+        ctx._check(ctx._L.ise_copy_on_device(ctx.h, dst.ptr, src.ptr, args.copy_bytes))
+        src.free()
+        dst.free()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(rows, meta)
-    for buf in list(out.values()) + [d_rows, d_meta, d_head]:
+    for buf in list(out.values()) + list(cout.values()) + [d_rows, d_meta, d_head]:
         buf.free()
     ctx.close()
     print(json.dumps(line), flush=True)

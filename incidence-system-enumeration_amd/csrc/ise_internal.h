@@ -14,7 +14,10 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
                         hipStream_t stream);
 
 // Resident workgroups per CU for the kernel variant.
-hipError_t occupancy_blocks(int group_lanes, uint32_t waves_per_block, int *blocks_per_cu);
+hipError_t occupancy_blocks(int group_lanes, uint32_t waves_per_block, uint32_t stride, int *blocks_per_cu);
+
+// Dynamic LDS of one workgroup (orbit geometry of every group's candidate).
+size_t lds_bytes(int group_lanes, uint32_t waves_per_block, uint32_t stride);
 
 // Smallest lane-group size that holds every candidate of a batch.
 inline int fit_group_lanes(uint32_t max_depth, uint32_t max_cols) {

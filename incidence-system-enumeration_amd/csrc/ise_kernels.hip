@@ -12,31 +12,65 @@
 
 namespace ise {
 
-__global__ void queue_reset_kernel(uint32_t *head) { *head = 0u; }
+// The candidate range is cut into one contiguous slice per XCD (workgroups
+// are dispatched to the XCDs round-robin, so workgroup w runs on XCD w % 8).
+// Each slice has its own queue head on a separate 128-byte line; a group
+// drains its home slice first and then takes from the others in turn, so
+// neighbouring result records are written from one L2 and the queue atomics
+// stay local until the tail of the batch.
+constexpr int NUM_XCD = 8;
+constexpr int HEAD_STRIDE = 32;   // words between queue heads
 
+__global__ void queue_reset_kernel(uint32_t *head) {
+    if (threadIdx.x < NUM_XCD) head[threadIdx.x * HEAD_STRIDE] = 0u;
+}
+
+// Next candidate index for the group (uniform), or n when the batch is drained.
 template <int G>
+__device__ __forceinline__ uint32_t next_candidate(const Grp<G> &g, uint32_t n, uint32_t *head, int &slice,
+                                                   int home) {
+    while (slice < home + NUM_XCD) {
+        const int x = slice % NUM_XCD;
+        const uint32_t lo = (uint32_t)(((uint64_t)n * x) / NUM_XCD);
+        const uint32_t hi = (uint32_t)(((uint64_t)n * (x + 1)) / NUM_XCD);
+        int taken = 0;
+        if (g.ln == 0)
+            taken = (int)atomicAdd(head + x * HEAD_STRIDE, 1u);
+        const uint32_t k = (uint32_t)g.read(taken, 0);
+        if (lo + k < hi)
+            return lo + k;
+        slice++;
+    }
+    return n;
+}
+
+template <int G, bool COUNT>
 __global__ __launch_bounds__(1024) void check_kernel(uint32_t n, uint32_t stride,
                                                     const uint64_t *__restrict__ rows,
                                                     const uint64_t *__restrict__ cells,
                                                     const uint32_t *__restrict__ meta,
-                                                    OutPtrs o, uint32_t *head, uint32_t max_perms) {
+                                                    OutPtrs o, uint32_t *head, uint32_t max_perms,
+                                                    uint32_t rows_cap) {
+    extern __shared__ uint32_t lds_geo[];
     Grp<G> g;
+    // this group's orbit-geometry scratch: rows_cap rows of G words
+    uint32_t *geo = lds_geo + ((threadIdx.x >> 6) * (64 / G) + (g.base / G)) * rows_cap * G;
+    const int home = (int)(blockIdx.x % NUM_XCD);
+    int slice = home;
     for (;;) {
-        int taken = 0;
-        if (g.ln == 0)
-            taken = (int)atomicAdd(head, 1u);
-        const uint32_t ci = (uint32_t)g.read(taken, 0);
+        const uint32_t ci = next_candidate(g, n, head, slice, home);
         if (ci >= n)
             break;
         const uint32_t m = meta[ci];
         const int d = m & 0xff, b = (m >> 8) & 0xff, fixed = (m >> 16) & 0xff;
         const bool ok = d >= 1 && d <= G && d <= ISE_MAX_ROWS && (uint32_t)d <= stride &&
+                        (uint32_t)d <= rows_cap &&
                         b >= 1 && b <= G && fixed < d;
         if (!ok) {
             if (g.ln == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
             continue;
         }
-        check_candidate<G>(g, ci, m, rows, cells, stride, o, max_perms);
+        check_candidate<G, COUNT>(g, ci, m, rows, cells, stride, o, max_perms, geo);
     }
 }
 
@@ -50,35 +84,52 @@ static OutPtrs to_out(const ise_results &r) {
     return o;
 }
 
-template <int G> static const void *kernel_ptr() { return (const void *)&check_kernel<G>; }
+template <int G> static const void *kernel_ptr() { return (const void *)&check_kernel<G, true>; }
+
+// Rows of orbit geometry kept per candidate in LDS.
+static uint32_t rows_cap_for(int group_lanes, uint32_t stride) {
+    uint32_t cap = stride < (uint32_t)ISE_MAX_ROWS ? stride : (uint32_t)ISE_MAX_ROWS;
+    return cap < (uint32_t)group_lanes ? cap : (uint32_t)group_lanes;
+}
+
+size_t lds_bytes(int group_lanes, uint32_t waves_per_block, uint32_t stride) {
+    return (size_t)waves_per_block * 64u * rows_cap_for(group_lanes, stride) * sizeof(uint32_t);
+}
 
 hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_block,
                         uint32_t n, uint32_t stride, const uint64_t *rows,
                         const uint64_t *cells, const uint32_t *meta,
                         const ise_results &out, uint32_t *head, uint32_t max_perms,
                         hipStream_t stream) {
-    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, stream, head);
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, stream, head);
     const dim3 grid(blocks), block(64 * waves_per_block);
     const OutPtrs o = to_out(out);
+    const uint32_t cap = rows_cap_for(group_lanes, stride);
+    const size_t lds = lds_bytes(group_lanes, waves_per_block, stride);
+    // statistics counters are compiled in only when the caller asks for them
+    const bool count = out.counters != nullptr;
+#define ISE_LAUNCH(GL)                                                                                 \
+    if (count)                                                                                         \
+        hipLaunchKernelGGL((check_kernel<GL, true>), grid, block, lds, stream, n, stride, rows, cells,  \
+                           meta, o, head, max_perms, cap);                                             \
+    else                                                                                               \
+        hipLaunchKernelGGL((check_kernel<GL, false>), grid, block, lds, stream, n, stride, rows, cells, \
+                           meta, o, head, max_perms, cap);
     switch (group_lanes) {
-    case 16:
-        hipLaunchKernelGGL(check_kernel<16>, grid, block, 0, stream, n, stride, rows, cells, meta, o, head, max_perms);
-        break;
-    case 32:
-        hipLaunchKernelGGL(check_kernel<32>, grid, block, 0, stream, n, stride, rows, cells, meta, o, head, max_perms);
-        break;
-    case 64:
-        hipLaunchKernelGGL(check_kernel<64>, grid, block, 0, stream, n, stride, rows, cells, meta, o, head, max_perms);
-        break;
+    case 16: ISE_LAUNCH(16) break;
+    case 32: ISE_LAUNCH(32) break;
+    case 64: ISE_LAUNCH(64) break;
     default:
         return hipErrorInvalidValue;
     }
+#undef ISE_LAUNCH
     return hipGetLastError();
 }
 
-hipError_t occupancy_blocks(int group_lanes, uint32_t waves_per_block, int *blocks_per_cu) {
+hipError_t occupancy_blocks(int group_lanes, uint32_t waves_per_block, uint32_t stride, int *blocks_per_cu) {
     const void *f = group_lanes == 16 ? kernel_ptr<16>() : group_lanes == 32 ? kernel_ptr<32>() : kernel_ptr<64>();
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, (int)(64 * waves_per_block), 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, (int)(64 * waves_per_block),
+                                                        lds_bytes(group_lanes, waves_per_block, stride));
 }
 
 } // namespace ise

@@ -86,7 +86,7 @@ int launch(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const 
     uint32_t blocks = c->cfg.blocks;
     if (!blocks) {
         int per_cu = 0;
-        HIPCHK(c, ise::occupancy_blocks(group_lanes, wpb, &per_cu));
+        HIPCHK(c, ise::occupancy_blocks(group_lanes, wpb, stride, &per_cu));
         if (per_cu < 1) per_cu = 1;
         blocks = (uint32_t)per_cu * (uint32_t)c->num_cus;
         const uint64_t groups_needed = ((uint64_t)n * group_lanes + 64ull * wpb - 1) / (64ull * wpb);
@@ -127,7 +127,7 @@ int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride) {
         if (e == hipSuccess) e = dalloc(&c->d_rows, n * stride);
         if (e == hipSuccess) e = dalloc(&c->d_cells, n * stride);
         if (e == hipSuccess) e = dalloc(&c->d_meta, n);
-        if (e == hipSuccess) e = dalloc(&c->d_head, 1);
+        if (e == hipSuccess) e = dalloc(&c->d_head, ISE_QUEUE_WORDS);
         if (e == hipSuccess) e = dalloc(&c->d.verdict, n);
         if (e == hipSuccess) e = dalloc(&c->d.fail_row, n);
         if (e == hipSuccess) e = dalloc(&c->d.fail_col, n);
@@ -364,6 +364,14 @@ int ise_copy_to_host(ise_ctx *c, void *dst, const void *d_src, uint64_t bytes) {
     if (!c || !dst || !d_src) return ISE_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISE_OK;
+}
+
+int ise_copy_on_device(ise_ctx *c, void *d_dst, const void *d_src, uint64_t bytes) {
+    if (!c || !d_dst || !d_src) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ISE_OK;
 }

@@ -230,10 +230,17 @@ ISE_DEV int select_bit(uint64_t m, int k) {
 }
 
 // Full canonicity test of candidate `ci` by the group g.
-template <int G>
+//
+// `geo` is the group's LDS scratch of d*G words: word row*G+l holds, for the
+// column orbit of row `row` that contains column l, its first column, its
+// length and the number of ones of the original row inside it.  It is filled
+// once per candidate so that the comparison loop does not recompute orbit
+// boundaries.
+template <int G, bool COUNT>
 ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta,
                              const uint64_t *rows, const uint64_t *cells,
-                             uint32_t stride, const OutPtrs &o, uint32_t max_perms) {
+                             uint32_t stride, const OutPtrs &o, uint32_t max_perms,
+                             uint32_t *geo) {
     const int l = g.ln;
     const int d = meta & 0xff, b = (meta >> 8) & 0xff;
     const int fixed = (meta >> 16) & 0xff;
@@ -245,7 +252,8 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta,
     uint64_t rowv = (l < d) ? rp[l] : 0ull;
     uint64_t cellv;
     if (cells) {
-        cellv = (l < d) ? cells[(uint64_t)ci * stride + l] : 0ull;
+        // column 0 always starts an orbit; columns past ncols do not exist
+        cellv = (l < d) ? ((cells[(uint64_t)ci * stride + l] | 1ull) & bmask) : 0ull;
     } else {
         // column c starts an orbit of row l when an earlier row changes value at c
         uint64_t m = 1;
@@ -258,6 +266,20 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta,
     // orbits induced by all d rows (indexing the column-orbit generators)
     const uint64_t last = g.read64(rowv, d - 1);
     const uint64_t cell_next = (g.read64(cellv, d - 1) | (last ^ (last << 1))) & bmask;
+
+    // per-row orbit geometry of this lane's column
+    const uint64_t le = (l == 63) ? ~0ull : ((2ull << l) - 1);   // columns 0..l
+    const uint64_t below_l = lowmask(l);                          // columns 0..l-1
+    for (int r = 0; r < d; r++) {
+        const uint64_t C = g.read64(cellv, r), R = g.read64(rowv, r);
+        int s0 = hib64(C & le);
+        const uint64_t above = C & ~le & bmask;
+        const int e0 = above ? ctz64(above) : b;
+        int len = e0 - s0;
+        int cR0 = popc64(R & (lowmask(e0) & ~lowmask(s0)));
+        if (l >= b) { s0 = l; len = 1; cR0 = 0; }
+        geo[r * G + l] = (uint32_t)s0 | ((uint32_t)len << 8) | ((uint32_t)cR0 << 16);
+    }
 
     int pr = l, ob = l, pc = l;
     PermState st;
@@ -276,36 +298,41 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta,
         if (++perms > max_perms) { verdict = ISE_VERDICT_LIMIT; break; }
         int outcome = 0, dec = 0;
         for (; row < d; row++) {
-            const uint64_t R = g.read64(rowv, row);
+            const uint32_t gi = geo[row * G + l];
+            const int s0 = gi & 0xff, len = (gi >> 8) & 0xff, cR = (gi >> 16) & 0xff;
+            const uint64_t om = (~0ull >> (64 - len)) << s0;
             const uint64_t Pw = g.read64(rowv, g.read(pr, row));
             const uint64_t C = g.read64(cellv, row);
             const bool act = l < b;
             const int bit = act ? (int)((Pw >> pc) & 1ull) : 0;
             const uint64_t P = g.ballot(bit != 0);
-            const uint64_t le = (l == 63) ? ~0ull : ((2ull << l) - 1);
-            const int s = hib64(C & le);
-            const uint64_t above = C & ~le & bmask;
-            const int e = above ? ctz64(above) : b;
-            const uint64_t om = lowmask(e) & ~lowmask(s);
-            const int cP = popc64(P & om), cR = popc64(R & om);
-            const uint64_t bad = g.ballot(act && l == s && cP != cR);
+            const int cP = popc64(P & om);
+            // orbit heads whose counts differ (C marks the heads)
+            const uint64_t bad = g.ballot(cP != cR) & C;
             int lim = b;
-            rows_cmp++;
             if (bad) {
                 lim = ctz64(bad);
-                outcome = g.read(cR > cP ? 1 : -1, lim);
+                const uint64_t more = g.ballot(cR > cP);
+                outcome = ((more >> lim) & 1ull) ? 1 : -1;
                 dec = lim;
-                orbs_cmp += popc64(C & lowmask(lim + 1));
-                cells_cmp += (uint32_t)g.read(e, lim);
-            } else {
-                orbs_cmp += popc64(C & bmask);
-                cells_cmp += (uint32_t)b;
+            }
+            if constexpr (COUNT) {
+                rows_cmp++;
+                if (bad) {
+                    orbs_cmp += popc64(C & lowmask(lim + 1));
+                    const uint64_t after = C & ~lowmask(lim + 1);
+                    cells_cmp += (uint32_t)(after ? ctz64(after) : b);
+                } else {
+                    orbs_cmp += popc64(C);
+                    cells_cmp += (uint32_t)b;
+                }
             }
             // stable partition (ones first) inside every orbit before `lim`
-            const bool upd = act && s < lim;
-            const int ones_before = popc64(P & om & lowmask(l));
-            const int np = bit ? s + ones_before : s + cP + (l - s - ones_before);
-            pc = g.scatter(pc, upd ? np : l);
+            const int ones_before = popc64(P & om & below_l);
+            const int np = bit ? s0 + ones_before : cP + l - ones_before;
+            const int dst = (act && s0 < lim) ? np : l;
+            if (g.ballot(dst != l))
+                pc = g.scatter(pc, dst);
             if (outcome) break;
         }
         if (outcome > 0)
@@ -357,7 +384,7 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta,
         if (o.row_to_change) o.row_to_change[ci] = (uint8_t)rtc;
         if (o.group_order) o.group_order[ci] = verdict == 1 ? st.order : 0ull;
         if (o.n_gens) o.n_gens[ci] = (uint16_t)(ngen > 0xffffu ? 0xffffu : ngen);
-        if (o.counters) {
+        if (COUNT && o.counters) {
             uint32_t *c = o.counters + (uint64_t)ci * 4;
             c[0] = perms; c[1] = rows_cmp; c[2] = orbs_cmp; c[3] = cells_cmp;
         }

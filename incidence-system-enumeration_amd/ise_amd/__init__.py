@@ -35,6 +35,8 @@ ERR_NODEV = -3
 ERR_CAPACITY = -4
 ERR_STATE = -5
 
+QUEUE_WORDS = 256   # ISE_QUEUE_WORDS: device words behind a work-queue argument
+
 _lib = None
 
 
@@ -68,6 +70,7 @@ def lib():
         L.ise_device_free.argtypes = [P, P]
         L.ise_copy_to_device.argtypes = [P, P, P, U64]
         L.ise_copy_to_host.argtypes = [P, P, P, U64]
+        L.ise_copy_on_device.argtypes = [P, P, P, U64]
         L.ise_sync.argtypes = [P]
         L.ise_last_error.argtypes = [P]
         L.ise_last_error.restype = ctypes.c_char_p
@@ -350,7 +353,7 @@ class GPUCanonChecker:
 
 
 __all__ = [
-    "Context", "DeviceArray", "ResultsStruct", "CanonicityChecker", "GPUCanonChecker", "ISEError", "random_extensions", "lib",
+    "QUEUE_WORDS", "Context", "DeviceArray", "ResultsStruct", "CanonicityChecker", "GPUCanonChecker", "ISEError", "random_extensions", "lib",
     "design_params", "alloc_results", "pack_meta", "pack_rows", "unpack_rows", "MAX_ROWS",
     "MAX_COLS", "SAVE_ROW_PERMUTATIONS", "SAVE_COL_PERMUTATIONS", "SAVE_ROW_TO_CHANGE",
     "VERDICT_CANONICAL", "VERDICT_NONCANONICAL", "VERDICT_LIMIT", "VERDICT_INVALID",

@@ -127,10 +127,14 @@ int ise_test_canonicity(ise_ctx *ctx, const uint8_t *mat, uint32_t nrows, uint32
                         uint32_t len_stab, uint32_t out_info, uint32_t gen_cap,
                         ise_results *out_host);
 
+/* Words of device memory behind the work-queue argument of the device entry
+ * points: one queue head per XCD, each on its own 128-byte line. */
+#define ISE_QUEUE_WORDS 256
+
 /* Device-resident batch: every pointer is device memory, results are written
  * in place; enqueued on `stream` (hipStream_t, NULL = the context stream).
- * `d_counter` is one uint32 of device memory used as the work-queue head
- * (cleared by the call).  No allocation happens in this call. */
+ * `d_counter` is ISE_QUEUE_WORDS uint32 of device memory used as the work
+ * queue (cleared by the call).  No allocation happens in this call. */
 int ise_check_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride,
                            const uint64_t *d_rows, const uint64_t *d_cells,
                            const uint32_t *d_meta, const ise_results *d_out,
@@ -151,6 +155,7 @@ int ise_device_alloc(ise_ctx *ctx, void **d_ptr, uint64_t bytes);
 int ise_device_free(ise_ctx *ctx, void *d_ptr);
 int ise_copy_to_device(ise_ctx *ctx, void *d_dst, const void *src, uint64_t bytes);
 int ise_copy_to_host(ise_ctx *ctx, void *dst, const void *d_src, uint64_t bytes);
+int ise_copy_on_device(ise_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes);
 
 /* Wait for all work on the context stream. */
 int ise_sync(ise_ctx *ctx);

@@ -335,8 +335,9 @@ static void build_orbits(cand_t *cd, const uint64_t *cells_in, const uint64_t *r
 {
     uint64_t masks[ORC_MAX_ROWS + 1];
     if (cells_in) {
-        for (int i = 0; i < cd->nrows; i++) masks[i] = cells_in[i];
         uint64_t full = cd->ncols >= 64 ? ~0ull : ((1ull << cd->ncols) - 1);
+        /* column 0 always starts an orbit; columns past ncols do not exist */
+        for (int i = 0; i < cd->nrows; i++) masks[i] = (cells_in[i] | 1ull) & full;
         uint64_t r = rows[cd->nrows - 1];
         masks[cd->nrows] = (masks[cd->nrows - 1] | (r ^ (r << 1))) & full;
     } else {

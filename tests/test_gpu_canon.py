@@ -192,7 +192,7 @@ def test_device_resident_entry_point(gpu_ctx, oracle):
     n = len(meta)
     d_rows = gpu_ctx.device_array(rows.shape, np.uint64, rows)
     d_meta = gpu_ctx.device_array(meta.shape, np.uint32, meta)
-    d_head = gpu_ctx.device_array((1,), np.uint32)
+    d_head = gpu_ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
     fields = ("verdict", "fail_row", "perm_row", "group_order", "counters")
     dev, s = gpu_ctx.device_results(n, fields)
     gpu_ctx.check_batch_device(n, 16, d_rows, d_meta, s, d_head)
