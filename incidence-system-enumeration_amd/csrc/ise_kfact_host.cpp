@@ -1,0 +1,163 @@
+// ise_kfact_host.cpp -- host side of the 1-factorization path: the
+// pair-slot arrangement tables and the seeded random candidate producer.
+#include <stdint.h>
+#include <string.h>
+#include <algorithm>
+#include <thread>
+#include <vector>
+#include "ise_kfact.h"
+
+namespace ise {
+
+static uint32_t factorial(uint32_t k) {
+    uint32_t f = 1;
+    for (uint32_t i = 2; i <= k; i++) f *= i;
+    return f;
+}
+
+size_t kf_table_words() {
+    size_t w = 0;
+    for (uint32_t k = 0; k <= KF_MAX_SLOTS; k++) w += factorial(k);
+    return w;
+}
+
+// Fuchs' countdown QuickPerm (the order canonizator visits the arrangements
+// of the pair slots in, k-Factorization/TripleSys/Source/cnv.cpp:283-315).
+void kf_build_tables(uint64_t *words, uint32_t *offset, uint32_t *count) {
+    uint32_t at = 0;
+    for (uint32_t k = 0; k <= KF_MAX_SLOTS; k++) {
+        offset[k] = at;
+        count[k] = factorial(k);
+        uint8_t a[KF_MAX_SLOTS + 1] = {0}, p[KF_MAX_SLOTS + 1] = {0};
+        for (uint32_t i = 0; i < k; i++) { a[i] = (uint8_t)(2 * i); p[i] = (uint8_t)i; }
+        p[k] = (uint8_t)k;
+        auto emit = [&]() {
+            uint64_t w = 0;
+            for (uint32_t i = 0; i < k; i++) w |= (uint64_t)a[i] << (8 * i);
+            words[at++] = w;
+        };
+        emit();
+        uint32_t i = 1;
+        while (i < k) {
+            p[i]--;
+            const uint32_t j = (i % 2) ? p[i] : 0;
+            std::swap(a[i], a[j]);
+            emit();
+            i = 0;
+            while (!p[++i]) p[i] = (uint8_t)i;
+        }
+    }
+}
+
+namespace {
+
+inline uint64_t mix(uint64_t &s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// One random partial factorization of up to nr rounds of n players as pair
+// lists; returns the rounds reached.  A stuck round is redrawn (64 tries), a
+// matrix whose round cannot be placed is restarted (64 restarts); when every
+// restart fails the deepest matrix met is kept, so the call always ends.
+int random_rounds(int n, int nr, uint64_t &rng, uint8_t rounds[][KF_MAX_PLAYERS]) {
+    uint8_t cur[KF_MAX_ROUNDS][KF_MAX_PLAYERS];
+    int best = 0;
+    for (int restart = 0; restart < 64 && best < nr; restart++) {
+        uint32_t used[KF_MAX_PLAYERS] = {0};
+        int r = 0;
+        for (; r < nr; r++) {
+            bool placed = false;
+            for (int tries = 0; tries < 64 && !placed; tries++) {
+                uint32_t freep = (1u << n) - 1;
+                int k = 0;
+                bool stuck = false;
+                while (freep) {
+                    const int x = __builtin_ctz(freep);
+                    freep &= ~(1u << x);
+                    const uint32_t cand = freep & ~used[x];
+                    if (!cand) { stuck = true; break; }
+                    int pick = (int)(mix(rng) % (uint64_t)__builtin_popcount(cand));
+                    uint32_t m = cand;
+                    while (pick--) m &= m - 1;
+                    const int y = __builtin_ctz(m);
+                    freep &= ~(1u << y);
+                    cur[r][k++] = (uint8_t)x;
+                    cur[r][k++] = (uint8_t)y;
+                }
+                if (!stuck) {
+                    for (int q = 0; q < n; q += 2) {
+                        used[cur[r][q]] |= 1u << cur[r][q + 1];
+                        used[cur[r][q + 1]] |= 1u << cur[r][q];
+                    }
+                    placed = true;
+                }
+            }
+            if (!placed) break;
+        }
+        if (r > best) {
+            best = r;
+            memcpy(rounds, cur, sizeof(cur[0]) * r);
+        }
+    }
+    return best;
+}
+
+// Relabel so that round 0 becomes (0 1)(2 3)..., then normal form.
+void normalize(int n, int nr, uint8_t rounds[][KF_MAX_PLAYERS], uint64_t *out) {
+    uint8_t lab[KF_MAX_PLAYERS];
+    for (int q = 0; q < n; q++) lab[rounds[0][q]] = (uint8_t)q;
+    struct Row { uint8_t v[KF_MAX_PLAYERS]; };
+    std::vector<Row> rows(nr);
+    for (int r = 0; r < nr; r++) {
+        uint8_t partner[KF_MAX_PLAYERS];
+        for (int q = 0; q < n; q += 2) {
+            const uint8_t x = lab[rounds[r][q]], y = lab[rounds[r][q + 1]];
+            partner[x] = y;
+            partner[y] = x;
+        }
+        int k = 0;
+        for (int v = 0; v < n; v++)
+            if (partner[v] > v) { rows[r].v[k++] = (uint8_t)v; rows[r].v[k++] = partner[v]; }
+    }
+    std::sort(rows.begin(), rows.end(), [](const Row &a, const Row &b) { return a.v[1] < b.v[1]; });
+    for (int r = 0; r < nr; r++) {
+        uint64_t w = 0;
+        for (int q = 0; q < n; q++) w |= (uint64_t)rows[r].v[q] << (4 * q);
+        out[r] = w;
+    }
+}
+
+} // namespace
+} // namespace ise
+
+extern "C" int ise_kf_random(int players, int rmin, int rmax, uint64_t count, uint64_t seed, uint32_t stride,
+                             uint64_t *rounds_out, uint32_t *meta_out) {
+    using namespace ise;
+    if (players < 4 || players > (int)KF_MAX_PLAYERS || (players & 1) || rmin < 1 || rmax < rmin ||
+        rmax > players - 1 || rmax > (int)KF_MAX_ROUNDS || (uint32_t)rmax > stride || !rounds_out || !meta_out)
+        return ISE_ERR_ARG;
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt == 0 ? 1 : (nt > 16 ? 16 : nt);
+    if (count < 1024) nt = 1;
+    auto work = [&](unsigned t) {
+        uint8_t rounds[KF_MAX_ROUNDS][KF_MAX_PLAYERS];
+        for (uint64_t i = t; i < count; i += nt) {
+            uint64_t rng = seed ^ (0xa0761d6478bd642full * (i + 1));
+            mix(rng);
+            const int target = rmin + (int)(mix(rng) % (uint64_t)(rmax - rmin + 1));
+            const int nr = random_rounds(players, target, rng, rounds);
+            uint64_t *dst = rounds_out + i * stride;
+            memset(dst, 0, stride * sizeof(uint64_t));
+            normalize(players, nr, rounds, dst);
+            meta_out[i] = (uint32_t)nr | ((uint32_t)players << 8);
+        }
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+    return ISE_OK;
+}

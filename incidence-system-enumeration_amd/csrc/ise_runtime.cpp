@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 #include "ise_internal.h"
+#include "ise_kfact.h"
 
 struct ise_ctx {
     int device = 0;
@@ -31,6 +32,10 @@ struct ise_ctx {
     uint32_t last_n = 0, last_gen_cap = 0;
     bool pending = false, last_cells = false;
     ise_launch_cfg cfg{0, 4, 0, 1u << 26};
+    // 1-factorization path: QuickPerm tables and per-candidate results
+    uint64_t *d_kf_tab = nullptr;
+    ise::KfTables kf_tab{};
+    uint64_t *d_kf_witness = nullptr, *d_kf_auts = nullptr, *d_kf_tried = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
     std::string err;
@@ -138,6 +143,17 @@ int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride) {
         if (e == hipSuccess) e = dalloc(&c->d.orbits, n * ISE_MAX_ROWS);
         if (e == hipSuccess) e = dalloc(&c->d.n_gens, n);
         if (e == hipSuccess) e = dalloc(&c->d.counters, n * 4);
+        if (e == hipSuccess) e = dalloc(&c->d_kf_witness, n);
+        if (e == hipSuccess) e = dalloc(&c->d_kf_auts, n);
+        if (e == hipSuccess) e = dalloc(&c->d_kf_tried, n);
+        if (e == hipSuccess) {
+            std::vector<uint64_t> words(ise::kf_table_words());
+            ise::kf_build_tables(words.data(), c->kf_tab.offset, c->kf_tab.count);
+            e = dalloc(&c->d_kf_tab, words.size());
+            if (e == hipSuccess)
+                e = hipMemcpy(c->d_kf_tab, words.data(), words.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+            c->kf_tab.base = c->d_kf_tab;
+        }
         if (e == hipSuccess) e = hipEventCreate(&c->ev0);
         if (e == hipSuccess) e = hipEventCreate(&c->ev1);
         if (e != hipSuccess) { rc = fail(c, e, "ise_open"); break; }
@@ -156,7 +172,8 @@ int ise_close(ise_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_rows, c->d_cells, c->d_meta, c->d_head, c->d.verdict, c->d.fail_row,
                     c->d.fail_col, c->d.row_to_change, c->d.group_order, c->d.perm_row,
-                    c->d.perm_col, c->d.orbits, c->d.n_gens, c->d.counters, c->d.gens};
+                    c->d.perm_col, c->d.orbits, c->d.n_gens, c->d.counters, c->d.gens,
+                    c->d_kf_tab, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -372,6 +389,100 @@ int ise_copy_on_device(ise_ctx *c, void *d_dst, const void *d_src, uint64_t byte
     if (!c || !d_dst || !d_src) return ISE_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISE_OK;
+}
+
+static ise::KfOut kf_out(const ise_kf_results &r) {
+    ise::KfOut o;
+    o.verdict = r.verdict; o.witness = r.witness; o.n_auts = r.n_auts; o.n_tried = r.n_tried;
+    return o;
+}
+
+static int kf_launch_on(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                        const uint32_t *d_meta, const ise_kf_results &d_out, uint32_t *d_queue,
+                        hipStream_t s) {
+    uint32_t blocks = c->cfg.blocks;
+    if (!blocks) {
+        int per_cu = 0;
+        HIPCHK(c, ise::kf_occupancy(&per_cu));
+        blocks = (uint32_t)(per_cu < 1 ? 1 : per_cu) * (uint32_t)c->num_cus;
+        if (blocks > n) blocks = n ? n : 1;
+    }
+    HIPCHK(c, ise::kf_launch(blocks, n, stride, d_rounds, d_meta, kf_out(d_out), d_queue, c->kf_tab, s));
+    return ISE_OK;
+}
+
+int ise_kf_check_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                              const uint32_t *d_meta, const ise_kf_results *d_out, uint32_t *d_queue,
+                              void *stream) {
+    if (!c || !d_out || !d_out->verdict || !d_queue || stride < 1 || stride > ISE_MAX_ROWS)
+        return ISE_ERR_ARG;
+    if (n == 0) return ISE_OK;
+    if (!d_rounds || !d_meta) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    return kf_launch_on(c, n, stride, d_rounds, d_meta, *d_out, d_queue,
+                        stream ? (hipStream_t)stream : c->stream);
+}
+
+int ise_kf_check_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                                    const uint32_t *d_meta, const ise_kf_results *d_out,
+                                    uint32_t *d_queue, uint32_t repeats, float *ms_out) {
+    if (!c || !ms_out || repeats == 0) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<hipEvent_t> ev(repeats + 1, nullptr);
+    int rc = ISE_OK;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = ISE_ERR_HIP;
+    if (rc == ISE_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) rc = ISE_ERR_HIP;
+    for (uint32_t i = 0; i < repeats && rc == ISE_OK; i++) {
+        rc = ise_kf_check_batch_device(c, n, stride, d_rounds, d_meta, d_out, d_queue, nullptr);
+        if (rc == ISE_OK && hipEventRecord(ev[i + 1], c->stream) != hipSuccess) rc = ISE_ERR_HIP;
+    }
+    if (rc == ISE_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = ISE_ERR_HIP;
+    for (uint32_t i = 0; i < repeats && rc == ISE_OK; i++)
+        if (hipEventElapsedTime(&ms_out[i], ev[i], ev[i + 1]) != hipSuccess) rc = ISE_ERR_HIP;
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc == ISE_ERR_HIP && c->err.empty()) c->err = "timed launch failed";
+    return rc;
+}
+
+int ise_kf_check(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
+                 ise_kf_results *h) {
+    if (!c || !h || !h->verdict || (n && (!rounds || !meta)) || stride < 1 || stride > c->stride)
+        return ISE_ERR_ARG;
+    if (n > c->capacity) return ISE_ERR_CAPACITY;
+    if (n == 0) return ISE_OK;
+    // host-side validation: edge-disjoint perfect matchings of the stated size
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t nr = meta[i] & 0xff, np = (meta[i] >> 8) & 0xff;
+        if (np < 4 || np > ise::KF_MAX_PLAYERS || (np & 1) || nr < 1 || nr > np - 1 || nr > stride)
+            return ISE_ERR_ARG;
+        uint32_t used[ise::KF_MAX_PLAYERS] = {0};
+        for (uint32_t r = 0; r < nr; r++) {
+            const uint64_t w = rounds[(uint64_t)i * stride + r];
+            uint32_t seen = 0;
+            for (uint32_t p = 0; p < np; p += 2) {
+                const uint32_t x = (uint32_t)(w >> (4 * p)) & 0xf, y = (uint32_t)(w >> (4 * p + 4)) & 0xf;
+                if (x >= np || y >= np || x == y || (seen >> x & 1) || (seen >> y & 1) || (used[x] >> y & 1))
+                    return ISE_ERR_ARG;
+                seen |= (1u << x) | (1u << y);
+                used[x] |= 1u << y;
+                used[y] |= 1u << x;
+            }
+        }
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->d_rows, rounds, (size_t)n * stride * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_meta, meta, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    ise_kf_results d{c->d.verdict, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried};
+    int rc = kf_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->d_head, c->stream);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(h->verdict, d.verdict, n, hipMemcpyDeviceToHost, c->stream));
+    if (h->witness) HIPCHK(c, hipMemcpyAsync(h->witness, d.witness, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (h->n_auts) HIPCHK(c, hipMemcpyAsync(h->n_auts, d.n_auts, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (h->n_tried) HIPCHK(c, hipMemcpyAsync(h->n_tried, d.n_tried, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ISE_OK;
 }

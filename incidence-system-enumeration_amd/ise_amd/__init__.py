@@ -82,8 +82,53 @@ def lib():
                                                     ctypes.POINTER(ctypes.c_long)]
         L.ise_enumerate_bibd_candidates.restype = ctypes.c_long
         L.ise_random_extensions.argtypes = [I, I, I, U64, I, I, U64, U32, P, P]
+        L.ise_kf_check.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
+        L.ise_kf_check_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P, P]
+        L.ise_kf_check_batch_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P,
+                                                      U32, P]
+        L.ise_kf_random.argtypes = [I, I, I, U64, U64, U32, P, P]
         _lib = L
     return _lib
+
+
+class KfResultsStruct(ctypes.Structure):
+    _fields_ = [("verdict", ctypes.c_void_p), ("witness", ctypes.c_void_p), ("n_auts", ctypes.c_void_p),
+                ("n_tried", ctypes.c_void_p)]
+
+
+KF_FIELDS = ("verdict", "witness", "n_auts", "n_tried")
+
+
+def kf_alloc(n):
+    return {"verdict": np.zeros(n, np.uint8), "witness": np.zeros(n, np.uint64),
+            "n_auts": np.zeros(n, np.uint64), "n_tried": np.zeros(n, np.uint64)}
+
+
+def kf_struct(arrays, device=False):
+    s = KfResultsStruct()
+    for k in KF_FIELDS:
+        a = arrays.get(k)
+        setattr(s, k, None if a is None else (a.ptr if device else a.ctypes.data))
+    return s
+
+
+def kf_pack(rounds_list, stride=16):
+    """List of matrices (nrounds x n players) -> (rounds [m][stride] uint64, meta [m])."""
+    m = len(rounds_list)
+    rows = np.zeros((m, stride), np.uint64)
+    meta = np.zeros(m, np.uint32)
+    for i, mat in enumerate(rounds_list):
+        mat = np.asarray(mat, dtype=np.uint64)
+        nr, n = mat.shape
+        shifts = (np.arange(n, dtype=np.uint64) * np.uint64(4))
+        rows[i, :nr] = np.bitwise_or.reduce(mat << shifts, axis=1)
+        meta[i] = nr | (n << 8)
+    return rows, meta
+
+
+def kf_unpack(words, n):
+    words = np.asarray(words, dtype=np.uint64)
+    return ((words[:, None] >> (np.arange(n, dtype=np.uint64) * np.uint64(4))) & np.uint64(15)).astype(np.uint8)
 
 
 class LaunchCfg(ctypes.Structure):
@@ -210,6 +255,36 @@ class Context:
     def sync(self):
         self._check(self._L.ise_sync(self.h))
 
+    # ---- 1-factorizations -------------------------------------------------
+    def kf_check(self, rounds, meta):
+        """Canonicity of partial 1-factorizations (host arrays in and out)."""
+        rounds = np.ascontiguousarray(rounds, dtype=np.uint64)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32)
+        n, stride = rounds.shape
+        outs = []
+        for off in range(0, n, self.capacity):
+            cnt = min(self.capacity, n - off)
+            r = kf_alloc(cnt)
+            s = kf_struct(r)
+            self._check(self._L.ise_kf_check(self.h, cnt, stride, rounds[off:off + cnt].ctypes.data,
+                                             meta[off:off + cnt].ctypes.data, ctypes.byref(s)))
+            outs.append(r)
+        if not outs:
+            return kf_alloc(0)
+        return {k: np.concatenate([o[k] for o in outs]) for k in KF_FIELDS}
+
+    def kf_device_results(self, n):
+        dev = {"verdict": self.device_array((n,), np.uint8), "witness": self.device_array((n,), np.uint64),
+               "n_auts": self.device_array((n,), np.uint64), "n_tried": self.device_array((n,), np.uint64)}
+        return dev, kf_struct(dev, device=True)
+
+    def kf_time_batch_device(self, n, stride, d_rounds, d_meta, res_struct, d_head, repeats):
+        ms = np.zeros(repeats, np.float32)
+        self._check(self._L.ise_kf_check_batch_device_timed(self.h, n, stride, d_rounds.ptr, d_meta.ptr,
+                                                            ctypes.byref(res_struct), d_head.ptr, repeats,
+                                                            ms.ctypes.data))
+        return ms
+
     def last_kernel_ms(self):
         return float(self._L.ise_last_kernel_ms(self.h))
 
@@ -288,6 +363,16 @@ def random_extensions(v, k, lam, n, dmin, dmax, seed, stride=32):
     return rows, meta
 
 
+def kf_random(players, rmin, rmax, count, seed, stride=16):
+    """Seeded random partial 1-factorizations of K_players in normal form."""
+    rows = np.zeros((count, stride), np.uint64)
+    meta = np.zeros(count, np.uint32)
+    rc = lib().ise_kf_random(players, rmin, rmax, count, seed, stride, rows.ctypes.data, meta.ctypes.data)
+    if rc != OK:
+        raise ISEError(rc, "ise_kf_random")
+    return rows, meta
+
+
 class CanonicityChecker:
     """Reference-shaped single-matrix checker on the GPU."""
 
@@ -353,7 +438,7 @@ class GPUCanonChecker:
 
 
 __all__ = [
-    "QUEUE_WORDS", "Context", "DeviceArray", "ResultsStruct", "CanonicityChecker", "GPUCanonChecker", "ISEError", "random_extensions", "lib",
+    "QUEUE_WORDS", "kf_random", "kf_pack", "kf_unpack", "kf_alloc", "KF_FIELDS", "Context", "DeviceArray", "ResultsStruct", "CanonicityChecker", "GPUCanonChecker", "ISEError", "random_extensions", "lib",
     "design_params", "alloc_results", "pack_meta", "pack_rows", "unpack_rows", "MAX_ROWS",
     "MAX_COLS", "SAVE_ROW_PERMUTATIONS", "SAVE_COL_PERMUTATIONS", "SAVE_ROW_TO_CHANGE",
     "VERDICT_CANONICAL", "VERDICT_NONCANONICAL", "VERDICT_LIMIT", "VERDICT_INVALID",

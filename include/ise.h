@@ -167,6 +167,49 @@ float ise_last_kernel_ms(const ise_ctx *ctx);
 /* Last HIP error string of the context ("" if none). */
 const char *ise_last_error(const ise_ctx *ctx);
 
+/* ---- 1-factorizations of K_n (groups of two players) --------------------
+ * Wire format: rounds[cand][stride] uint64, nibble p of word i = player at
+ * position p of round i (the n/2 pairs of the round in order); meta[cand] =
+ * nrounds | n << 8, with n even, 4 <= n <= 16, 1 <= nrounds <= min(n-1, 15).
+ * The rounds must be edge-disjoint perfect matchings.  Canonical = no
+ * relabelling of the players (with any round moved first) gives a smaller
+ * normal form (pairs sorted, pairs by first player, rounds by the partner of
+ * player 0), compared round by round -- alldata::canonizator / cnvCheckKm1 /
+ * kmProcessMatrix (k-Factorization/TripleSys/Source/cnv.cpp:65-351,
+ * kmSupport.cpp:275-313). */
+typedef struct {
+    uint8_t  *verdict;  /* [n] 1 canonical, 0 not, 3 invalid record            */
+    uint64_t *witness;  /* [n] plain-order index of the first smaller relabelling (~0 if none) */
+    uint64_t *n_auts;   /* [n] relabellings that reproduce the matrix, counted up to the
+                           decision (|Aut| when canonical)                       */
+    uint64_t *n_tried;  /* [n] relabellings evaluated up to the decision          */
+} ise_kf_results;
+
+/* Host arrays in and out, synchronous (cnvCheckNew of one matrix per record,
+ * cnv.cpp:211-270).  Up to the context capacity per call. */
+int ise_kf_check(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *rounds,
+                 const uint32_t *meta, ise_kf_results *out_host);
+
+/* Device-resident batch on `stream` (NULL = context stream); `d_queue` is
+ * ISE_QUEUE_WORDS words of device memory.  No allocation. */
+int ise_kf_check_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                              const uint32_t *d_meta, const ise_kf_results *d_out,
+                              uint32_t *d_queue, void *stream);
+/* Same, launched `repeats` times on the context stream; ms_out[i] = time of
+ * launch i from HIP events.  Synchronous. */
+int ise_kf_check_batch_device_timed(ise_ctx *ctx, uint32_t n, uint32_t stride,
+                                    const uint64_t *d_rounds, const uint32_t *d_meta,
+                                    const ise_kf_results *d_out, uint32_t *d_queue,
+                                    uint32_t repeats, float *ms_out);
+
+/* Seeded random partial 1-factorizations of K_players with nrounds uniform in
+ * [rmin, rmax]: each round is a uniformly drawn perfect matching avoiding the
+ * edges already used (the smallest free player takes a uniform free partner;
+ * a stuck round is redrawn, a stuck matrix restarted), then relabelled so that
+ * round 0 is (0 1)(2 3)... and put in normal form.  Deterministic per seed. */
+int ise_kf_random(int players, int rmin, int rmax, uint64_t count, uint64_t seed,
+                  uint32_t stride, uint64_t *rounds_out, uint32_t *meta_out);
+
 /* ---- host-side candidate producers (stay on the CPU) -------------------- */
 
 /* Level-synchronous enumeration of 2-(v,k,lambda) designs: every level's

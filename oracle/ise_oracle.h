@@ -103,8 +103,17 @@ int orc_bruteforce_canon(const uint8_t *mat, int nrows, int ncols,
 int orc_kf_check(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
                  uint64_t *n_auts, uint64_t *n_tried);
 
-/* Normal form of a relabelled factorization (used by tests). */
+/* Normal form of a factorization under the identity labelling (pairs
+ * sorted, pairs by first element, rows by partner of player 0). */
 void orc_kf_normalize(uint8_t *mat, int n, int nrounds);
+
+/* Brute force over all n! player relabellings (n <= 8): canonical or not,
+ * and the number of relabellings that reproduce the matrix. */
+int orc_kf_bruteforce(const uint8_t *mat, int n, int nrounds, uint64_t *n_auts);
+
+/* Fuchs' countdown QuickPerm order of the pair slots (k! rows of k bytes,
+ * each byte the base label 2*slot); returns k!. */
+long orc_kf_quickperm(int k, uint8_t *out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,160 @@
+/*
+ * ise_oracle_kfact.c -- CPU oracle: canonicity of partial 1-factorizations of
+ * K_n (groups of size 2, complete graph).
+ *
+ * TEST INFRASTRUCTURE ONLY (see ise_oracle.h).
+ *
+ * Restates alldata::canonizator / cnvCheckTgNew / cnvCheckKm / cnvCheckKm1
+ * (k-Factorization/TripleSys/Source/cnv.cpp:65-351) with the generic normal
+ * form kmProcessMatrix / kmSortMatrixForReorderedPlayers / kmSortGroups /
+ * kmSortGroupsByFirstValue / kmSortRowsBy2ndValue
+ * (k-Factorization/TripleSys/Source/kmSupport.cpp:43-166, 220-313), for the
+ * default switches USE_EQUAL == 0 (TripleSys.h:79) and early exit on the first
+ * smaller matrix (m_doNotExitEarlyIfNotCanonical == false, TripleSys.h:387).
+ *
+ * A matrix is nrounds rows of n players; each row lists the n/2 pairs of one
+ * round.  The relabellings tried, in order:
+ *   for every arrangement of the n/2 pair slots (Fuchs' countdown QuickPerm
+ *   order, cnv.cpp:283-315), for every choice of orientation of each pair
+ *   (odometer over {0,1}^(n/2), last pair fastest, cnv.cpp:193-208), for every
+ *   day d (0..nrounds-1, cnv.cpp:91-163): the relabelling that sends the pair
+ *   in slot i of round d to the labels (a[i], a[i]+1) in that orientation.
+ * Each relabelled matrix is normalised (pairs sorted, pairs of a row sorted by
+ * their first element, rows sorted by the partner of player 0) and compared
+ * row by row with the original: smaller -> not canonical (stop), equal ->
+ * automorphism.
+ */
+#include "ise_oracle.h"
+#include <string.h>
+#include <stdlib.h>
+
+typedef unsigned char u8;
+#define KF_MAX_N 16
+
+/* Fuchs' countdown QuickPerm: writes all k! arrangements of 0..k-1 (scaled by
+ * 2: the base label of each pair slot) in visiting order; returns k!. */
+static long quickperm_table(int k, u8 *out)
+{
+    u8 a[KF_MAX_N / 2 + 1], p[KF_MAX_N / 2 + 1];
+    long cnt = 0;
+    for (int i = 0; i < k; i++) { a[i] = (u8)(2 * i); p[i] = (u8)i; }
+    p[k] = (u8)k;
+    memcpy(out + cnt++ * k, a, (size_t)k);
+    int i = 1;
+    while (i < k) {
+        p[i]--;
+        int j = (i % 2) ? p[i] : 0;
+        u8 t = a[i]; a[i] = a[j]; a[j] = t;
+        memcpy(out + cnt++ * k, a, (size_t)k);
+        i = 0;
+        while (!p[++i])
+            p[i] = (u8)i;
+    }
+    return cnt;
+}
+
+/* normal form of matrix mi (nr x n) relabelled by tr, into mo */
+static void normal_form(const u8 *mi, int nr, int n, const u8 *tr, u8 *mo)
+{
+    u8 tmp[KF_MAX_N * KF_MAX_N];
+    const int npairs = n / 2;
+    for (int r = 0; r < nr; r++) {
+        u8 byfirst[KF_MAX_N][2];
+        u8 present[KF_MAX_N];
+        memset(present, 0, sizeof present);
+        for (int q = 0; q < npairs; q++) {
+            u8 x = tr[mi[r * n + 2 * q]], y = tr[mi[r * n + 2 * q + 1]];
+            if (x > y) { u8 t = x; x = y; y = t; }
+            byfirst[x][0] = x; byfirst[x][1] = y; present[x] = 1;
+        }
+        int k = 0;
+        for (int v = 0; v < n; v++)
+            if (present[v]) { tmp[r * n + k++] = byfirst[v][0]; tmp[r * n + k++] = byfirst[v][1]; }
+    }
+    /* rows by their second entry (partner of player 0) */
+    int k = 0;
+    for (int key = 0; key < n; key++)
+        for (int r = 0; r < nr; r++)
+            if (tmp[r * n + 1] == key) { memcpy(mo + k * n, tmp + r * n, (size_t)n); k++; }
+}
+
+void orc_kf_normalize(uint8_t *mat, int n, int nrounds)
+{
+    u8 id[KF_MAX_N], out[KF_MAX_N * KF_MAX_N];
+    for (int i = 0; i < n; i++) id[i] = (u8)i;
+    normal_form(mat, nrounds, n, id, out);
+    memcpy(mat, out, (size_t)(n * nrounds));
+}
+
+int orc_kf_check(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
+                 uint64_t *n_auts, uint64_t *n_tried)
+{
+    if (n < 4 || n > KF_MAX_N || (n & 1) || nrounds < 1 || nrounds > n - 1) return -1;
+    const int npairs = n / 2;
+    long nperm = 1;
+    for (int i = 2; i <= npairs; i++) nperm *= i;
+    u8 *qp = (u8 *)malloc((size_t)nperm * npairs);
+    quickperm_table(npairs, qp);
+    u8 trmk[KF_MAX_N], ttr[KF_MAX_N], mo[KF_MAX_N * KF_MAX_N];
+    uint64_t auts = 0, tried = 0, idx = 0;
+    int verdict = 1;
+    for (long h = 0; h < nperm && verdict; h++) {
+        const u8 *a = qp + h * npairs;
+        for (unsigned t = 0; t < (1u << npairs) && verdict; t++) {
+            for (int i = 0; i < npairs; i++) {
+                /* orientation bit of slot i: the odometer's last digit is the fastest */
+                int flip = (t >> (npairs - 1 - i)) & 1;
+                trmk[2 * i] = (u8)(a[i] + flip);
+                trmk[2 * i + 1] = (u8)(a[i] + 1 - flip);
+            }
+            for (int day = 0; day < nrounds; day++, idx++) {
+                const u8 *rd = mat + day * n;
+                for (int pos = 0; pos < n; pos++) ttr[rd[pos]] = trmk[pos];
+                normal_form(mat, nrounds, n, ttr, mo);
+                tried++;
+                int cmp = memcmp(mo, mat, (size_t)(n * nrounds));
+                /* row-wise early exit gives the same sign as a full compare */
+                if (cmp == 0) auts++;
+                else if (cmp < 0) {
+                    verdict = 0;
+                    if (witness) *witness = idx;
+                    break;
+                }
+            }
+        }
+    }
+    free(qp);
+    if (n_auts) *n_auts = auts;
+    if (n_tried) *n_tried = tried;
+    return verdict;
+}
+
+/* Brute force for tiny n: canonical <=> no relabelling by any permutation of
+ * the players gives a smaller normal form; also counts the permutations that
+ * reproduce the matrix. */
+int orc_kf_bruteforce(const uint8_t *mat, int n, int nrounds, uint64_t *n_auts)
+{
+    if (n > 8) return -1;
+    u8 tr[KF_MAX_N], mo[KF_MAX_N * KF_MAX_N];
+    int c[KF_MAX_N], i = 0, canon = 1;
+    uint64_t auts = 0;
+    for (int q = 0; q < n; q++) { tr[q] = (u8)q; c[q] = 0; }
+    for (;;) {
+        normal_form(mat, nrounds, n, tr, mo);
+        int cmp = memcmp(mo, mat, (size_t)(n * nrounds));
+        if (cmp < 0) canon = 0;
+        if (cmp == 0) auts++;
+        while (i < n && c[i] >= i) { c[i] = 0; i++; }
+        if (i >= n) break;
+        u8 t;
+        if (i % 2 == 0) { t = tr[0]; tr[0] = tr[i]; tr[i] = t; }
+        else { t = tr[c[i]]; tr[c[i]] = tr[i]; tr[i] = t; }
+        c[i]++;
+        i = 0;
+    }
+    if (n_auts) *n_auts = auts;
+    return canon;
+}
+
+/* the QuickPerm visiting order (for tests) */
+long orc_kf_quickperm(int k, uint8_t *out) { return quickperm_table(k, out); }

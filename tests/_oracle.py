@@ -36,9 +36,11 @@ def load():
                                          _P, _P, _P, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_enumerate_bibd.restype = ctypes.c_long
         L.orc_bruteforce_canon.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P]
-        if hasattr(L, "orc_kf_check"):
-            L.orc_kf_check.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
-            L.orc_kf_normalize.argtypes = [_P, ctypes.c_int, ctypes.c_int]
+        L.orc_kf_check.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
+        L.orc_kf_normalize.argtypes = [_P, ctypes.c_int, ctypes.c_int]
+        L.orc_kf_bruteforce.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
+        L.orc_kf_quickperm.argtypes = [ctypes.c_int, _P]
+        L.orc_kf_quickperm.restype = ctypes.c_long
         _lib = Oracle(L)
     return _lib
 
@@ -102,6 +104,33 @@ class Oracle:
         self.L.orc_test_canonicity(mat.ctypes.data, mat.shape[0], mat.shape[1], len_stab, out_info,
                                    ctypes.addressof(s), gen_cap)
         return res
+
+    def kf_check(self, mat):
+        """(verdict, witness index, automorphisms counted, relabellings tried)."""
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        w, a, t = (np.zeros(1, np.uint64) for _ in range(3))
+        v = self.L.orc_kf_check(mat.ctypes.data, mat.shape[1], mat.shape[0], w.ctypes.data, a.ctypes.data,
+                                t.ctypes.data)
+        if v < 0:
+            raise ValueError("oracle rejected the factorization")
+        return v, int(w[0]) if v == 0 else (1 << 64) - 1, int(a[0]), int(t[0])
+
+    def kf_bruteforce(self, mat):
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        a = np.zeros(1, np.uint64)
+        v = self.L.orc_kf_bruteforce(mat.ctypes.data, mat.shape[1], mat.shape[0], a.ctypes.data)
+        return v, int(a[0])
+
+    def kf_normalize(self, mat):
+        mat = np.ascontiguousarray(mat, dtype=np.uint8).copy()
+        self.L.orc_kf_normalize(mat.ctypes.data, mat.shape[1], mat.shape[0])
+        return mat
+
+    def kf_quickperm(self, k):
+        import math
+        out = np.zeros((math.factorial(k), k), np.uint8)
+        self.L.orc_kf_quickperm(k, out.ctypes.data)
+        return out
 
     def bruteforce(self, mat):
         mat = np.ascontiguousarray(mat, dtype=np.uint8)

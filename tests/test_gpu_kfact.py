@@ -1,0 +1,106 @@
+"""GPU parity of the 1-factorization canonicity kernel against the oracle:
+verdict, first witness in plain order, automorphisms counted and relabellings
+tried, on random partial factorizations of K_12 (3-10 rounds), the reference's
+printed canonical factorizations and their prefixes, and brute-forced K_8."""
+import numpy as np
+import pytest
+
+import ise_amd
+from _helpers import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_batch(oracle, rows, meta, n):
+    out = {k: [] for k in ise_amd.KF_FIELDS}
+    for i in range(len(meta)):
+        nr = int(meta[i]) & 0xff
+        v, w, a, t = oracle.kf_check(ise_amd.kf_unpack(rows[i, :nr], n))
+        for k, x in zip(ise_amd.KF_FIELDS, (v, w, a, t)):
+            out[k].append(x)
+    return {k: np.array(v, np.uint64 if k != "verdict" else np.uint8) for k, v in out.items()}
+
+
+def assert_same(g, o, what):
+    for k in ise_amd.KF_FIELDS:
+        if not np.array_equal(g[k], o[k]):
+            i = int(np.nonzero(g[k] != o[k])[0][0])
+            raise AssertionError(f"{what}: {k} differs at {i}: gpu={g[k][i]} oracle={o[k][i]}")
+
+
+def test_random_k12_partials(gpu_ctx, oracle):
+    print("case kfact random K12 3-10 rounds", flush=True)
+    rows, meta = ise_amd.kf_random(12, 3, 10, 4096, 0x120311)
+    g = gpu_ctx.kf_check(rows, meta)
+    o = oracle_batch(oracle, rows, meta, 12)
+    assert_same(g, o, "K12 random")
+
+
+def test_printed_canonical_factorizations_and_prefixes(gpu_ctx, oracle):
+    print("case kfact printed K10 + U12 + prefixes", flush=True)
+    kf = golden("kfact.json")
+    mats = [(m["rounds"], m["aut"]) for m in kf["k10"]["matrices"]]
+    rows, meta = ise_amd.kf_pack([m for m, _ in mats])
+    g = gpu_ctx.kf_check(rows, meta)
+    assert (g["verdict"] == 1).all()
+    assert (g["n_auts"] == np.array([a for _, a in mats], np.uint64)).all()
+    assert (g["n_tried"] == 9 * 120 * 32).all()
+    u12 = np.array(kf["u12"]["matrices"][0]["rounds"], np.uint8)
+    prefixes = [u12[:k] for k in range(1, 12)] + [np.array(m, np.uint8)[:k] for m, _ in mats[:40] for k in (3, 5, 7)]
+    rows, meta = ise_amd.kf_pack(prefixes)
+    g = gpu_ctx.kf_check(rows, meta)
+    o = {k: [] for k in ise_amd.KF_FIELDS}
+    for p in prefixes:
+        for k, x in zip(ise_amd.KF_FIELDS, oracle.kf_check(p)):
+            o[k].append(x)
+    o = {k: np.array(v, np.uint64 if k != "verdict" else np.uint8) for k, v in o.items()}
+    assert_same(g, o, "prefixes")
+    assert int(g["n_auts"][10]) == 660 and g["verdict"][10] == 1
+
+
+def test_k8_against_bruteforce(gpu_ctx, oracle):
+    print("case kfact K8 brute force", flush=True)
+    rows, meta = ise_amd.kf_random(8, 1, 7, 300, 88)
+    g = gpu_ctx.kf_check(rows, meta)
+    for i in range(300):
+        nr = int(meta[i]) & 0xff
+        mat = ise_amd.kf_unpack(rows[i, :nr], 8)
+        bv, bauts = oracle.kf_bruteforce(mat)
+        assert g["verdict"][i] == bv
+        if bv:
+            assert int(g["n_auts"][i]) == bauts
+    o = oracle_batch(oracle, rows, meta, 8)
+    assert_same(g, o, "K8")
+
+
+def test_mixed_sizes_and_invalid(gpu_ctx, oracle):
+    print("case kfact mixed sizes", flush=True)
+    parts = [ise_amd.kf_random(n, 1, min(n - 1, 6), 200, n) for n in (4, 6, 10, 14, 16)]
+    rows = np.concatenate([p[0] for p in parts])
+    meta = np.concatenate([p[1] for p in parts])
+    g = gpu_ctx.kf_check(rows, meta)
+    o = {k: [] for k in ise_amd.KF_FIELDS}
+    for i in range(len(meta)):
+        nr, n = int(meta[i]) & 0xff, (int(meta[i]) >> 8) & 0xff
+        for k, x in zip(ise_amd.KF_FIELDS, oracle.kf_check(ise_amd.kf_unpack(rows[i, :nr], n))):
+            o[k].append(x)
+    o = {k: np.array(v, np.uint64 if k != "verdict" else np.uint8) for k, v in o.items()}
+    assert_same(g, o, "mixed")
+    bad = rows[:1].copy()
+    bad[0, 1] = bad[0, 0]                       # repeats the edges of round 0
+    with pytest.raises(ise_amd.ISEError):
+        gpu_ctx.kf_check(bad, meta[:1])
+
+
+def test_device_timed_path(gpu_ctx, oracle):
+    print("case kfact device timed", flush=True)
+    rows, meta = ise_amd.kf_random(12, 3, 10, 2048, 5)
+    n = len(meta)
+    d_rows = gpu_ctx.device_array(rows.shape, np.uint64, rows)
+    d_meta = gpu_ctx.device_array(meta.shape, np.uint32, meta)
+    d_head = gpu_ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
+    dev, s = gpu_ctx.kf_device_results(n)
+    ms = gpu_ctx.kf_time_batch_device(n, 16, d_rows, d_meta, s, d_head, 2)
+    assert (ms > 0).all()
+    g = {k: dev[k].to_host() for k in ise_amd.KF_FIELDS}
+    assert_same(g, oracle_batch(oracle, rows, meta, 12), "device")

@@ -1,0 +1,105 @@
+"""The 1-factorization oracle pinned against the reference's printed canonical
+factorizations (K_10: 396 with |Aut(M)|; the uniform K_12 one with |Aut| 660),
+brute force over all relabellings for n <= 8, and the published QuickPerm
+visiting order."""
+import itertools
+
+import numpy as np
+import pytest
+
+import ise_amd
+from _helpers import golden
+
+
+def test_k10_printed_factorizations(oracle):
+    block = golden("kfact.json")["k10"]
+    assert len(block["matrices"]) == 396
+    for m in block["matrices"]:
+        v, w, auts, tried = oracle.kf_check(m["rounds"])
+        assert v == 1 and auts == m["aut"]
+        assert tried == 9 * 120 * 32
+
+
+def test_u12_uniform_factorization(oracle):
+    m = golden("kfact.json")["u12"]["matrices"][0]
+    v, w, auts, tried = oracle.kf_check(m["rounds"])
+    assert v == 1 and auts == 660 == m["aut"]
+
+
+def test_quickperm_order_matches_published_sequence(oracle):
+    # Fuchs' countdown QuickPerm lists ABC, BAC, CAB, ACB, BCA, CBA
+    seq = oracle.kf_quickperm(3) // 2
+    assert ["".join("ABC"[x] for x in row) for row in seq] == ["ABC", "BAC", "CAB", "ACB", "BCA", "CBA"]
+    for k in range(2, 8):
+        t = oracle.kf_quickperm(k)
+        assert len({tuple(r) for r in t}) == len(t)      # every arrangement exactly once
+        assert (np.sort(t, axis=1) == 2 * np.arange(k)).all()
+
+
+def _random_factorization(rng, n, nr):
+    while True:
+        used = set()
+        rounds = []
+        ok = True
+        for _ in range(nr):
+            for _ in range(200):
+                free = list(range(n))
+                pairs = []
+                while free:
+                    x = free.pop(0)
+                    cand = [y for y in free if (min(x, y), max(x, y)) not in used]
+                    if not cand:
+                        break
+                    y = cand[rng.integers(len(cand))]
+                    free.remove(y)
+                    pairs.append((x, y))
+                if len(pairs) == n // 2:
+                    break
+            else:
+                ok = False
+                break
+            if len(pairs) != n // 2:
+                ok = False
+                break
+            for x, y in pairs:
+                used.add((min(x, y), max(x, y)))
+            rounds.append([v for p in pairs for v in p])
+        if ok:
+            return np.array(rounds, np.uint8)
+
+
+@pytest.mark.parametrize("n", [4, 6, 8])
+def test_matches_bruteforce(oracle, n):
+    """Canonical <=> no player relabelling gives a smaller normal form, and the
+    automorphisms counted equal those found by brute force over all n!."""
+    rng = np.random.default_rng(n)
+    seen_canonical = 0
+    for _ in range(40 if n < 8 else 12):
+        nr = int(rng.integers(1, n))
+        raw = _random_factorization(rng, n, nr)
+        # relabel so that round 0 is the identity pairing, then normal form
+        lab = np.empty(n, np.uint8)
+        lab[raw[0]] = np.arange(n)
+        mat = oracle.kf_normalize(lab[raw])
+        v, w, auts, tried = oracle.kf_check(mat)
+        bv, bauts = oracle.kf_bruteforce(mat)
+        assert v == bv
+        if v:
+            seen_canonical += 1
+            assert auts == bauts
+    assert seen_canonical > 0
+
+
+def test_random_k12_generator_is_valid_and_normal(oracle):
+    rows, meta = ise_amd.kf_random(12, 3, 10, 500, 7)
+    for i in range(500):
+        nr = int(meta[i]) & 0xff
+        mat = ise_amd.kf_unpack(rows[i, :nr], 12)
+        assert (mat[0] == np.arange(12)).all()
+        edges = set()
+        for r in mat:
+            assert sorted(r) == list(range(12))
+            for x, y in zip(r[::2], r[1::2]):
+                assert x < y and (x, y) not in edges
+                edges.add((x, y))
+        assert (oracle.kf_normalize(mat) == mat).all()
