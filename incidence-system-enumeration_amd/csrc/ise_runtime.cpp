@@ -9,7 +9,9 @@
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 #include "ise_internal.h"
 #include "ise_kfact.h"
@@ -37,6 +39,10 @@ struct ise_ctx {
     ise::KfTables kf_tab{};
     uint64_t *d_kf_witness = nullptr, *d_kf_auts = nullptr, *d_kf_tried = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // executable graphs of ise_submit launches, keyed by batch shape
+    bool use_graphs = false;
+    std::map<std::tuple<uint32_t, uint32_t, int, bool, uint32_t, uint32_t, uint32_t, uint32_t>, hipGraphExec_t> graphs;
+    uint64_t graph_replays = 0;
     float last_ms = 0.f;
     std::string err;
 };
@@ -84,22 +90,40 @@ int scan_meta(const uint32_t *meta, uint32_t n, uint32_t stride, uint32_t *max_d
     return ISE_OK;
 }
 
-int launch(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const uint64_t *cells,
-           const uint32_t *meta, const ise_results &out, uint32_t *head, hipStream_t s,
-           int group_lanes) {
+// Persistent grid size: resident workgroups per CU times CUs, capped by the work.
+int grid_blocks(ise_ctx *c, uint32_t n, uint32_t stride, int group_lanes, uint32_t *blocks) {
     const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
-    uint32_t blocks = c->cfg.blocks;
-    if (!blocks) {
+    uint32_t b = c->cfg.blocks;
+    if (!b) {
         int per_cu = 0;
         HIPCHK(c, ise::occupancy_blocks(group_lanes, wpb, stride, &per_cu));
         if (per_cu < 1) per_cu = 1;
-        blocks = (uint32_t)per_cu * (uint32_t)c->num_cus;
+        b = (uint32_t)per_cu * (uint32_t)c->num_cus;
         const uint64_t groups_needed = ((uint64_t)n * group_lanes + 64ull * wpb - 1) / (64ull * wpb);
-        if (blocks > groups_needed) blocks = (uint32_t)(groups_needed ? groups_needed : 1);
+        if (b > groups_needed) b = (uint32_t)(groups_needed ? groups_needed : 1);
     }
+    *blocks = b;
+    return ISE_OK;
+}
+
+// Enqueue queue reset + kernel; `blocks` from grid_blocks (no runtime queries
+// here, so the sequence can be captured into a graph).
+int launch_with(ise_ctx *c, uint32_t blocks, uint32_t n, uint32_t stride, const uint64_t *rows,
+                const uint64_t *cells, const uint32_t *meta, const ise_results &out, uint32_t *head,
+                hipStream_t s, int group_lanes) {
+    const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
     HIPCHK(c, ise::launch_check(group_lanes, blocks, wpb, n, stride, rows, cells, meta, out, head,
                                 c->cfg.max_perms ? c->cfg.max_perms : (1u << 26), s));
     return ISE_OK;
+}
+
+int launch(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const uint64_t *cells,
+           const uint32_t *meta, const ise_results &out, uint32_t *head, hipStream_t s,
+           int group_lanes) {
+    uint32_t blocks = 0;
+    int rc = grid_blocks(c, n, stride, group_lanes, &blocks);
+    if (rc) return rc;
+    return launch_with(c, blocks, n, stride, rows, cells, meta, out, head, s, group_lanes);
 }
 
 } // namespace
@@ -176,6 +200,7 @@ int ise_close(ise_ctx *c) {
                     c->d_kf_tab, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
+    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -252,9 +277,35 @@ int ise_submit(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows,
         int group = c->cfg.group_lanes ? (int)c->cfg.group_lanes : ise::fit_group_lanes(md, mb);
         if ((uint32_t)group < md || (uint32_t)group < mb) return ISE_ERR_ARG;
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        int rc = launch(c, n, stride, c->d_rows, cells ? c->d_cells : nullptr, c->d_meta, o,
-                        c->d_head, c->stream, group);
-        if (rc) return rc;
+        if (c->use_graphs) {
+            const auto key = std::make_tuple(n, stride, group, cells != nullptr, gen_cap,
+                                             c->cfg.waves_per_block, c->cfg.blocks, c->cfg.max_perms);
+            auto it = c->graphs.find(key);
+            if (it == c->graphs.end()) {
+                // capture once: every pointer is a context buffer, so replays see new data
+                hipGraph_t g = nullptr;
+                hipGraphExec_t ge = nullptr;
+                uint32_t blocks = 0;
+                int rc = grid_blocks(c, n, stride, group, &blocks);
+                if (rc) return rc;
+                HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                rc = launch_with(c, blocks, n, stride, c->d_rows, cells ? c->d_cells : nullptr, c->d_meta,
+                                 o, c->d_head, c->stream, group);
+                hipError_t e = hipStreamEndCapture(c->stream, &g);
+                if (rc) return rc;
+                if (e != hipSuccess) return fail(c, e, "hipStreamEndCapture");
+                e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) return fail(c, e, "hipGraphInstantiate");
+                it = c->graphs.emplace(key, ge).first;
+            }
+            HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+            c->graph_replays++;
+        } else {
+            int rc = launch(c, n, stride, c->d_rows, cells ? c->d_cells : nullptr, c->d_meta, o,
+                            c->d_head, c->stream, group);
+            if (rc) return rc;
+        }
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     }
     if (from_queue) {
@@ -484,6 +535,57 @@ int ise_kf_check(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds
     if (h->n_auts) HIPCHK(c, hipMemcpyAsync(h->n_auts, d.n_auts, n * 8, hipMemcpyDeviceToHost, c->stream));
     if (h->n_tried) HIPCHK(c, hipMemcpyAsync(h->n_tried, d.n_tried, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISE_OK;
+}
+
+int ise_use_graphs(ise_ctx *c, int on) {
+    if (!c) return ISE_ERR_ARG;
+    c->use_graphs = on != 0;
+    return ISE_OK;
+}
+
+int ise_graph_stats(const ise_ctx *c, uint32_t *graphs, uint64_t *replays) {
+    if (!c) return ISE_ERR_ARG;
+    if (graphs) *graphs = (uint32_t)c->graphs.size();
+    if (replays) *replays = c->graph_replays;
+    return ISE_OK;
+}
+
+int ise_autotune(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rows, const uint64_t *d_cells,
+                 const uint32_t *d_meta, const ise_results *d_out, uint32_t *d_counter, uint32_t max_depth,
+                 uint32_t max_cols, uint32_t repeats, float *table, ise_launch_cfg *best) {
+    if (!c || !table || !d_out || n == 0 || repeats == 0 || max_depth < 1 || max_cols < 1 ||
+        max_depth > stride || max_depth > ISE_MAX_ROWS || max_cols > ISE_MAX_COLS)
+        return ISE_ERR_ARG;
+    const uint32_t groups[3] = {16, 32, 64}, waves[4] = {1, 2, 4, 8};
+    const ise_launch_cfg saved = c->cfg;
+    ise_launch_cfg win = saved;
+    float win_ms = -1.f;
+    std::vector<float> ms(repeats);
+    for (int gi = 0; gi < 3; gi++) {
+        for (int wi = 0; wi < 4; wi++) {
+            table[gi * 4 + wi] = -1.f;
+            if (groups[gi] < max_depth || groups[gi] < max_cols) continue;
+            c->cfg.group_lanes = groups[gi];
+            c->cfg.waves_per_block = waves[wi];
+            c->cfg.blocks = 0;
+            int rc = ise_check_batch_device_timed(c, n, stride, d_rows, d_cells, d_meta, d_out, d_counter,
+                                                  repeats, ms.data());
+            if (rc) {
+                c->cfg = saved;
+                return rc;
+            }
+            float m = ms[0];
+            for (float x : ms) m = x < m ? x : m;     // best of the repeats
+            table[gi * 4 + wi] = m;
+            if (win_ms < 0 || m < win_ms) {
+                win_ms = m;
+                win = c->cfg;
+            }
+        }
+    }
+    c->cfg = win;
+    if (best) *best = win;
     return ISE_OK;
 }
 

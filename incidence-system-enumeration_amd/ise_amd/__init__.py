@@ -71,6 +71,9 @@ def lib():
         L.ise_copy_to_device.argtypes = [P, P, P, U64]
         L.ise_copy_to_host.argtypes = [P, P, P, U64]
         L.ise_copy_on_device.argtypes = [P, P, P, U64]
+        L.ise_use_graphs.argtypes = [P, I]
+        L.ise_graph_stats.argtypes = [P, ctypes.POINTER(U32), ctypes.POINTER(U64)]
+        L.ise_autotune.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P, U32, U32, U32, P, P]
         L.ise_sync.argtypes = [P]
         L.ise_last_error.argtypes = [P]
         L.ise_last_error.restype = ctypes.c_char_p
@@ -251,6 +254,25 @@ class Context:
             self.h, n, stride, d_rows.ptr, d_cells.ptr if d_cells is not None else None, d_meta.ptr,
             ctypes.byref(res_struct), d_head.ptr, repeats, ms.ctypes.data))
         return ms
+
+    def use_graphs(self, on=True):
+        self._check(self._L.ise_use_graphs(self.h, 1 if on else 0))
+
+    def graph_stats(self):
+        g, r = ctypes.c_uint32(0), ctypes.c_uint64(0)
+        self._check(self._L.ise_graph_stats(self.h, ctypes.byref(g), ctypes.byref(r)))
+        return g.value, r.value
+
+    def autotune(self, n, stride, d_rows, d_meta, res_struct, d_head, max_depth, max_cols, repeats=2,
+                 d_cells=None):
+        """Sweep lane-group size x waves per workgroup; keeps the fastest.
+        Returns (best launch config, table[16/32/64][1/2/4/8 waves] in ms)."""
+        table = np.zeros((3, 4), np.float32)
+        best = LaunchCfg()
+        self._check(self._L.ise_autotune(self.h, n, stride, d_rows.ptr, d_cells.ptr if d_cells else None,
+                                         d_meta.ptr, ctypes.byref(res_struct), d_head.ptr, max_depth, max_cols,
+                                         repeats, table.ctypes.data, ctypes.byref(best)))
+        return dict(group_lanes=best.group_lanes, waves_per_block=best.waves_per_block), table
 
     def sync(self):
         self._check(self._L.ise_sync(self.h))

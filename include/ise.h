@@ -157,6 +157,27 @@ int ise_copy_to_device(ise_ctx *ctx, void *d_dst, const void *src, uint64_t byte
 int ise_copy_to_host(ise_ctx *ctx, void *dst, const void *d_src, uint64_t bytes);
 int ise_copy_on_device(ise_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes);
 
+/* Capture every launch of ise_submit as a hipGraph (queue reset + kernel),
+ * one executable graph per batch shape (n, stride, group size, inputs and
+ * outputs), instantiated once and replayed afterwards; the enumerator then
+ * replays one graph per level and chunk size.  on = 0 launches directly. */
+int ise_use_graphs(ise_ctx *ctx, int on);
+/* Number of executable graphs instantiated so far and replays of them. */
+int ise_graph_stats(const ise_ctx *ctx, uint32_t *graphs, uint64_t *replays);
+
+/* Autotuner: times the device-resident batch (same arguments as
+ * ise_check_batch_device) for every lane-group size that holds the batch
+ * (16/32/64) and every workgroup size of 1, 2, 4, 8 waves (which sets the
+ * LDS tile of orbit geometry per workgroup), `repeats` launches each, keeps
+ * the fastest in the context's launch configuration and writes the
+ * measurements (ms per launch, -1 if not run) to table[3][4] (rows: 16, 32,
+ * 64 lanes; columns: 1, 2, 4, 8 waves).  max_depth/max_cols describe the
+ * batch (the caller knows them; device data is not inspected). */
+int ise_autotune(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                 const uint64_t *d_cells, const uint32_t *d_meta, const ise_results *d_out,
+                 uint32_t *d_counter, uint32_t max_depth, uint32_t max_cols, uint32_t repeats,
+                 float *table, ise_launch_cfg *best);
+
 /* Wait for all work on the context stream. */
 int ise_sync(ise_ctx *ctx);
 
