@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--group-lanes", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--order", choices=["plain", "hunt"], default="plain")
+    ap.add_argument("--hunt-budget", type=int, default=64)
     ap.add_argument("--build-batch", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run one device-to-device copy of this many bytes (counter calibration)")
@@ -124,6 +126,8 @@ def main():
     fields = ("verdict", "fail_row", "row_to_change", "group_order", "perm_row")
     out, res = ctx.device_results(n, fields)
     ctx.set_launch(group_lanes=args.group_lanes, waves_per_block=args.waves_per_block)
+    if args.order == "hunt":
+        ctx.set_order(ise_amd.ORDER_HUNT, args.hunt_budget)
 
     if args.warmup:
         ctx.time_batch_device(n, STRIDE, d_rows, d_meta, res, d_head, args.warmup)

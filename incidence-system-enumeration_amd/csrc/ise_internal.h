@@ -11,6 +11,7 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
                         uint32_t n, uint32_t stride, const uint64_t *rows,
                         const uint64_t *cells, const uint32_t *meta,
                         const ise_results &out, uint32_t *head, uint32_t max_perms,
+                        uint32_t hunt_budget, const uint8_t *order, uint32_t order_dim,
                         hipStream_t stream);
 
 // Resident workgroups per CU for the kernel variant.

@@ -44,13 +44,14 @@ __device__ __forceinline__ uint32_t next_candidate(const Grp<G> &g, uint32_t n, 
     return n;
 }
 
-template <int G, bool COUNT>
+template <int G, bool COUNT, bool HUNT>
 __global__ __launch_bounds__(1024) void check_kernel(uint32_t n, uint32_t stride,
                                                     const uint64_t *__restrict__ rows,
                                                     const uint64_t *__restrict__ cells,
                                                     const uint32_t *__restrict__ meta,
                                                     OutPtrs o, uint32_t *head, uint32_t max_perms,
-                                                    uint32_t rows_cap) {
+                                                    uint32_t rows_cap, const uint8_t *__restrict__ order,
+                                                    int od, uint32_t hunt_budget) {
     extern __shared__ uint32_t lds_geo[];
     Grp<G> g;
     // this group's orbit-geometry scratch: rows_cap rows of G words
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(1024) void check_kernel(uint32_t n, uint32_t stride
             if (g.ln == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
             continue;
         }
-        check_candidate<G, COUNT>(g, ci, m, rows, cells, stride, o, max_perms, geo);
+        check_candidate<G, COUNT, HUNT>(g, ci, m, rows, cells, stride, o, max_perms, geo, order, od, hunt_budget);
     }
 }
 
@@ -84,7 +85,7 @@ static OutPtrs to_out(const ise_results &r) {
     return o;
 }
 
-template <int G> static const void *kernel_ptr() { return (const void *)&check_kernel<G, true>; }
+template <int G> static const void *kernel_ptr() { return (const void *)&check_kernel<G, true, false>; }
 
 // Rows of orbit geometry kept per candidate in LDS.
 static uint32_t rows_cap_for(int group_lanes, uint32_t stride) {
@@ -100,21 +101,28 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
                         uint32_t n, uint32_t stride, const uint64_t *rows,
                         const uint64_t *cells, const uint32_t *meta,
                         const ise_results &out, uint32_t *head, uint32_t max_perms,
+                        uint32_t hunt_budget, const uint8_t *order, uint32_t order_dim,
                         hipStream_t stream) {
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, stream, head);
     const dim3 grid(blocks), block(64 * waves_per_block);
     const OutPtrs o = to_out(out);
     const uint32_t cap = rows_cap_for(group_lanes, stride);
     const size_t lds = lds_bytes(group_lanes, waves_per_block, stride);
-    // statistics counters are compiled in only when the caller asks for them
+    // statistics counters and the witness hunt are compiled in only when asked for
     const bool count = out.counters != nullptr;
-#define ISE_LAUNCH(GL)                                                                                 \
-    if (count)                                                                                         \
-        hipLaunchKernelGGL((check_kernel<GL, true>), grid, block, lds, stream, n, stride, rows, cells,  \
-                           meta, o, head, max_perms, cap);                                             \
-    else                                                                                               \
-        hipLaunchKernelGGL((check_kernel<GL, false>), grid, block, lds, stream, n, stride, rows, cells, \
-                           meta, o, head, max_perms, cap);
+    const bool hunt = hunt_budget > 0;
+    const int od = (int)order_dim;
+#define ISE_LAUNCH_V(GL, C, H)                                                                          \
+    hipLaunchKernelGGL((check_kernel<GL, C, H>), grid, block, lds, stream, n, stride, rows, cells, meta, o, \
+                       head, max_perms, cap, order, od, hunt_budget)
+#define ISE_LAUNCH(GL)                                          \
+    if (count) {                                                \
+        if (hunt) ISE_LAUNCH_V(GL, true, true);                 \
+        else ISE_LAUNCH_V(GL, true, false);                     \
+    } else {                                                    \
+        if (hunt) ISE_LAUNCH_V(GL, false, true);                \
+        else ISE_LAUNCH_V(GL, false, false);                    \
+    }
     switch (group_lanes) {
     case 16: ISE_LAUNCH(16) break;
     case 32: ISE_LAUNCH(32) break;
@@ -123,6 +131,7 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
         return hipErrorInvalidValue;
     }
 #undef ISE_LAUNCH
+#undef ISE_LAUNCH_V
     return hipGetLastError();
 }
 

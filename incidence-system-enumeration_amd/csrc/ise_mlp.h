@@ -1,0 +1,32 @@
+// ise_mlp.h -- kernels of the learned row-order scorer (ise_mlp.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ise {
+
+constexpr int MLP_IN = 16, MLP_HID = 128, MLP_OUT_PAD = 16;
+
+// Packed parameter layout (floats): W1[16][128], b1[128], W2[128][128], b2[128],
+// W3[128][16] (column 0 used), b3[16] (entry 0 used).
+constexpr long MLP_W1 = 0, MLP_B1 = MLP_W1 + MLP_IN * MLP_HID, MLP_W2 = MLP_B1 + MLP_HID,
+               MLP_B2 = MLP_W2 + MLP_HID * MLP_HID, MLP_W3 = MLP_B2 + MLP_HID,
+               MLP_B3 = MLP_W3 + MLP_HID * MLP_OUT_PAD, MLP_PARAMS = MLP_B3 + MLP_OUT_PAD;
+
+hipError_t gemm_bf16(int M, int N, int K, const uint16_t *A, long sam, long sak, const uint16_t *B, long sbk,
+                     long sbn, const float *bias, int relu, float *Cf, uint16_t *Cb, long ldc, int accumulate,
+                     hipStream_t s);
+hipError_t bce(int M, const float *z, long ldz, const float *y, const float *w, float inv_wsum, uint16_t *dz,
+               float *loss_terms, hipStream_t s);
+hipError_t relu_back(long count, const float *dh, const float *z, uint16_t *dz, hipStream_t s);
+hipError_t colsum(int M, int N, const uint16_t *x, float *out, hipStream_t s);
+hipError_t sum(long count, const float *x, float *out, hipStream_t s);
+hipError_t adam(long count, float *w, const float *g, float *m1, float *m2, uint16_t *wb, float lr, float b1,
+                float b2, float eps, float c1, float c2, hipStream_t s);
+hipError_t to_bf16(long count, const float *x, uint16_t *y, hipStream_t s);
+hipError_t features(uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta, uint32_t od,
+                    uint16_t *X, hipStream_t s);
+hipError_t order_from_scores(uint32_t n, const uint32_t *meta, uint32_t od, const float *score, long score_ld,
+                             uint8_t *order, hipStream_t s);
+
+} // namespace ise

@@ -34,6 +34,10 @@ struct ise_ctx {
     uint32_t last_n = 0, last_gen_cap = 0;
     bool pending = false, last_cells = false;
     ise_launch_cfg cfg{0, 4, 0, 1u << 26};
+    // search order (ise_set_order)
+    int order_mode = ISE_ORDER_PLAIN;
+    uint32_t hunt_budget = 0, order_dim = 0;
+    const uint8_t *d_order = nullptr;
     // 1-factorization path: QuickPerm tables and per-candidate results
     uint64_t *d_kf_tab = nullptr;
     ise::KfTables kf_tab{};
@@ -41,7 +45,9 @@ struct ise_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // executable graphs of ise_submit launches, keyed by batch shape
     bool use_graphs = false;
-    std::map<std::tuple<uint32_t, uint32_t, int, bool, uint32_t, uint32_t, uint32_t, uint32_t>, hipGraphExec_t> graphs;
+    using GraphKey = std::tuple<uint32_t, uint32_t, int, bool, uint32_t, uint32_t, uint32_t, uint32_t, int,
+                                uint32_t, uintptr_t, uint32_t>;
+    std::map<GraphKey, hipGraphExec_t> graphs;
     uint64_t graph_replays = 0;
     float last_ms = 0.f;
     std::string err;
@@ -112,8 +118,11 @@ int launch_with(ise_ctx *c, uint32_t blocks, uint32_t n, uint32_t stride, const 
                 const uint64_t *cells, const uint32_t *meta, const ise_results &out, uint32_t *head,
                 hipStream_t s, int group_lanes) {
     const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
+    const uint32_t budget = c->order_mode == ISE_ORDER_PLAIN ? 0u : c->hunt_budget;
+    const uint8_t *order = c->order_mode == ISE_ORDER_LEARNED ? c->d_order : nullptr;
     HIPCHK(c, ise::launch_check(group_lanes, blocks, wpb, n, stride, rows, cells, meta, out, head,
-                                c->cfg.max_perms ? c->cfg.max_perms : (1u << 26), s));
+                                c->cfg.max_perms ? c->cfg.max_perms : (1u << 26), budget, order,
+                                c->order_dim, s));
     return ISE_OK;
 }
 
@@ -278,8 +287,9 @@ int ise_submit(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows,
         if ((uint32_t)group < md || (uint32_t)group < mb) return ISE_ERR_ARG;
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         if (c->use_graphs) {
-            const auto key = std::make_tuple(n, stride, group, cells != nullptr, gen_cap,
-                                             c->cfg.waves_per_block, c->cfg.blocks, c->cfg.max_perms);
+            const ise_ctx::GraphKey key{n, stride, group, cells != nullptr, gen_cap, c->cfg.waves_per_block,
+                                        c->cfg.blocks, c->cfg.max_perms, c->order_mode, c->hunt_budget,
+                                        (uintptr_t)c->d_order, c->order_dim};
             auto it = c->graphs.find(key);
             if (it == c->graphs.end()) {
                 // capture once: every pointer is a context buffer, so replays see new data
@@ -535,6 +545,17 @@ int ise_kf_check(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds
     if (h->n_auts) HIPCHK(c, hipMemcpyAsync(h->n_auts, d.n_auts, n * 8, hipMemcpyDeviceToHost, c->stream));
     if (h->n_tried) HIPCHK(c, hipMemcpyAsync(h->n_tried, d.n_tried, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ISE_OK;
+}
+
+int ise_set_order(ise_ctx *c, int mode, uint32_t hunt_budget, const uint8_t *d_order, uint32_t order_dim) {
+    if (!c || mode < ISE_ORDER_PLAIN || mode > ISE_ORDER_LEARNED) return ISE_ERR_ARG;
+    if (mode != ISE_ORDER_PLAIN && hunt_budget == 0) return ISE_ERR_ARG;
+    if (mode == ISE_ORDER_LEARNED && (!d_order || order_dim < 1 || order_dim > ISE_MAX_ROWS)) return ISE_ERR_ARG;
+    c->order_mode = mode;
+    c->hunt_budget = hunt_budget;
+    c->d_order = mode == ISE_ORDER_LEARNED ? d_order : nullptr;
+    c->order_dim = mode == ISE_ORDER_LEARNED ? order_dim : 0;
     return ISE_OK;
 }
 

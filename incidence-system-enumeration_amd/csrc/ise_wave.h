@@ -229,164 +229,289 @@ ISE_DEV int select_bit(uint64_t m, int k) {
     return lo;
 }
 
-// Full canonicity test of candidate `ci` by the group g.
-//
-// `geo` is the group's LDS scratch of d*G words: word row*G+l holds, for the
-// column orbit of row `row` that contains column l, its first column, its
-// length and the number of ones of the original row inside it.  It is filled
-// once per candidate so that the comparison loop does not recompute orbit
-// boundaries.
-template <int G, bool COUNT>
-ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta,
-                             const uint64_t *rows, const uint64_t *cells,
-                             uint32_t stride, const OutPtrs &o, uint32_t max_perms,
-                             uint32_t *geo) {
-    const int l = g.ln;
-    const int d = meta & 0xff, b = (meta >> 8) & 0xff;
-    const int fixed = (meta >> 16) & 0xff;
-    const bool row_permut = ((meta >> 24) & ISE_SAVE_ROW_PERMUTATIONS) != 0;
-    const uint64_t bmask = lowmask(b);
+// One candidate staged in a group's lanes.
+struct Staged {
+    int d, b, fixed;
+    bool row_permut;
+    uint64_t bmask, rowv, cellv, cell_next, below_l;
+};
 
-    // stage the candidate in the group's lanes (lane l reads word l)
+// Load candidate `ci` into the lanes and fill `geo` (the group's LDS scratch
+// of d*G words): word row*G+l holds, for the column orbit of row `row` that
+// contains column l, its first column, its length and the number of ones of
+// the original row inside it, so that row comparisons do not recompute
+// orbit boundaries.
+template <int G>
+ISE_DEV Staged stage(const Grp<G> &g, uint32_t ci, uint32_t meta, const uint64_t *rows,
+                     const uint64_t *cells, uint32_t stride, uint32_t *geo) {
+    const int l = g.ln;
+    Staged c;
+    c.d = meta & 0xff;
+    c.b = (meta >> 8) & 0xff;
+    c.fixed = (meta >> 16) & 0xff;
+    c.row_permut = ((meta >> 24) & ISE_SAVE_ROW_PERMUTATIONS) != 0;
+    c.bmask = lowmask(c.b);
+    const int d = c.d, b = c.b;
+    // lane l reads word l (coalesced)
     const uint64_t *rp = rows + (uint64_t)ci * stride;
-    uint64_t rowv = (l < d) ? rp[l] : 0ull;
-    uint64_t cellv;
+    c.rowv = (l < d) ? rp[l] : 0ull;
     if (cells) {
         // column 0 always starts an orbit; columns past ncols do not exist
-        cellv = (l < d) ? ((cells[(uint64_t)ci * stride + l] | 1ull) & bmask) : 0ull;
+        c.cellv = (l < d) ? ((cells[(uint64_t)ci * stride + l] | 1ull) & c.bmask) : 0ull;
     } else {
         // column c starts an orbit of row l when an earlier row changes value at c
         uint64_t m = 1;
         for (int j = 0; j < d; j++) {
-            uint64_t r = g.read64(rowv, j);
+            uint64_t r = g.read64(c.rowv, j);
             if (j < l) m |= r ^ (r << 1);
         }
-        cellv = m & bmask;
+        c.cellv = m & c.bmask;
     }
     // orbits induced by all d rows (indexing the column-orbit generators)
-    const uint64_t last = g.read64(rowv, d - 1);
-    const uint64_t cell_next = (g.read64(cellv, d - 1) | (last ^ (last << 1))) & bmask;
-
-    // per-row orbit geometry of this lane's column
+    const uint64_t last = g.read64(c.rowv, d - 1);
+    c.cell_next = (g.read64(c.cellv, d - 1) | (last ^ (last << 1))) & c.bmask;
     const uint64_t le = (l == 63) ? ~0ull : ((2ull << l) - 1);   // columns 0..l
-    const uint64_t below_l = lowmask(l);                          // columns 0..l-1
+    c.below_l = lowmask(l);                                        // columns 0..l-1
     for (int r = 0; r < d; r++) {
-        const uint64_t C = g.read64(cellv, r), R = g.read64(rowv, r);
+        const uint64_t C = g.read64(c.cellv, r), R = g.read64(c.rowv, r);
         int s0 = hib64(C & le);
-        const uint64_t above = C & ~le & bmask;
+        const uint64_t above = C & ~le & c.bmask;
         const int e0 = above ? ctz64(above) : b;
         int len = e0 - s0;
         int cR0 = popc64(R & (lowmask(e0) & ~lowmask(s0)));
         if (l >= b) { s0 = l; len = 1; cR0 = 0; }
         geo[r * G + l] = (uint32_t)s0 | ((uint32_t)len << 8) | ((uint32_t)cR0 << 16);
     }
+    return c;
+}
 
-    int pr = l, ob = l, pc = l;
+// Compare original row `row` with row `prow` placed at position `row`
+// (checkColOrbit over every orbit at once): +1 the permuted row is smaller,
+// -1 larger, 0 equal.  The column permutation is stably partitioned inside
+// every orbit before the deciding one; *dec receives the deciding orbit's
+// first column.
+template <int G, bool COUNT>
+ISE_DEV int compare_row(const Grp<G> &g, const Staged &c, const uint32_t *geo, int row, int prow, int &pc,
+                        int *dec, uint32_t &orbs_cmp, uint32_t &cells_cmp) {
+    const int l = g.ln;
+    const uint32_t gi = geo[row * G + l];
+    const int s0 = gi & 0xff, len = (gi >> 8) & 0xff, cR = (gi >> 16) & 0xff;
+    const uint64_t om = (~0ull >> (64 - len)) << s0;
+    const uint64_t Pw = g.read64(c.rowv, prow);
+    const uint64_t C = g.read64(c.cellv, row);
+    const bool act = l < c.b;
+    const int bit = act ? (int)((Pw >> pc) & 1ull) : 0;
+    const uint64_t P = g.ballot(bit != 0);
+    const int cP = popc64(P & om);
+    // orbit heads whose counts differ (C marks the heads)
+    const uint64_t bad = g.ballot(cP != cR) & C;
+    int lim = c.b, outcome = 0;
+    if (bad) {
+        lim = ctz64(bad);
+        const uint64_t more = g.ballot(cR > cP);
+        outcome = ((more >> lim) & 1ull) ? 1 : -1;
+        *dec = lim;
+    }
+    if constexpr (COUNT) {
+        if (bad) {
+            orbs_cmp += popc64(C & lowmask(lim + 1));
+            const uint64_t after = C & ~lowmask(lim + 1);
+            cells_cmp += (uint32_t)(after ? ctz64(after) : c.b);
+        } else {
+            orbs_cmp += popc64(C);
+            cells_cmp += (uint32_t)c.b;
+        }
+    }
+    // stable partition (ones first) inside every orbit before `lim`
+    const int ones_before = popc64(P & om & c.below_l);
+    const int np = bit ? s0 + ones_before : cP + l - ones_before;
+    const int dst = (act && s0 < lim) ? np : l;
+    if (g.ballot(dst != l))
+        pc = g.scatter(pc, dst);
+    return outcome;
+}
+
+// Result of one candidate before it is written.
+struct Verdict {
+    int verdict, frow, fcol, rtc;
+    uint64_t order;
+    uint32_t perms, rows_cmp, orbs_cmp, cells_cmp, ngen;
+};
+
+// rowToChange (CanonicityChecker.cpp:320-330): max(row, perm[0..row])
+template <int G> ISE_DEV int row_to_change(const Grp<G> &g, int pr, int d, int row) {
+    const int l = g.ln;
+    const int inv = g.scatter(l, (l < d) ? pr : l);
+    const uint64_t seen = g.ballot(l < d && inv <= row);
+    const int mx = hib64(seen);
+    return mx > row ? mx : row;
+}
+
+// The reference's search (plain order) on a staged candidate.
+template <int G, bool COUNT>
+ISE_DEV void plain_search(const Grp<G> &g, const Staged &c, const uint32_t *geo, uint32_t ci,
+                          const OutPtrs &o, uint32_t max_perms, int &pr, int &ob, int &pc, Verdict &v) {
+    const int l = g.ln, d = c.d, b = c.b;
     PermState st;
     st.stab = d - 1;
     st.stab_aut = EMAX;
     st.order = 1;
-    uint32_t perms = 0, rows_cmp = 0, orbs_cmp = 0, cells_cmp = 0;
-    uint32_t ngen = 0;
-    int verdict = 1, frow = EMAX, fcol = EMAX, rtc = EMAX;
-
     int row = EMAX;
     for (;;) {
-        row = next_row_perm(g, st, pr, ob, d, row, fixed);
+        row = next_row_perm(g, st, pr, ob, d, row, c.fixed);
         if (row == EMAX)
             break;
-        if (++perms > max_perms) { verdict = ISE_VERDICT_LIMIT; break; }
+        if (++v.perms > max_perms) { v.verdict = ISE_VERDICT_LIMIT; break; }
         int outcome = 0, dec = 0;
         for (; row < d; row++) {
-            const uint32_t gi = geo[row * G + l];
-            const int s0 = gi & 0xff, len = (gi >> 8) & 0xff, cR = (gi >> 16) & 0xff;
-            const uint64_t om = (~0ull >> (64 - len)) << s0;
-            const uint64_t Pw = g.read64(rowv, g.read(pr, row));
-            const uint64_t C = g.read64(cellv, row);
-            const bool act = l < b;
-            const int bit = act ? (int)((Pw >> pc) & 1ull) : 0;
-            const uint64_t P = g.ballot(bit != 0);
-            const int cP = popc64(P & om);
-            // orbit heads whose counts differ (C marks the heads)
-            const uint64_t bad = g.ballot(cP != cR) & C;
-            int lim = b;
-            if (bad) {
-                lim = ctz64(bad);
-                const uint64_t more = g.ballot(cR > cP);
-                outcome = ((more >> lim) & 1ull) ? 1 : -1;
-                dec = lim;
-            }
-            if constexpr (COUNT) {
-                rows_cmp++;
-                if (bad) {
-                    orbs_cmp += popc64(C & lowmask(lim + 1));
-                    const uint64_t after = C & ~lowmask(lim + 1);
-                    cells_cmp += (uint32_t)(after ? ctz64(after) : b);
-                } else {
-                    orbs_cmp += popc64(C);
-                    cells_cmp += (uint32_t)b;
-                }
-            }
-            // stable partition (ones first) inside every orbit before `lim`
-            const int ones_before = popc64(P & om & below_l);
-            const int np = bit ? s0 + ones_before : cP + l - ones_before;
-            const int dst = (act && s0 < lim) ? np : l;
-            if (g.ballot(dst != l))
-                pc = g.scatter(pc, dst);
+            if constexpr (COUNT) v.rows_cmp++;
+            outcome = compare_row<G, COUNT>(g, c, geo, row, g.read(pr, row), pc, &dec, v.orbs_cmp, v.cells_cmp);
             if (outcome) break;
         }
         if (outcome > 0)
             continue;                 // refuted prefix: advance position `row`
         if (outcome < 0) {            // the permuted matrix is larger: witness
-            verdict = 0;
-            frow = row;
-            fcol = dec;
-            // rowToChange (CanonicityChecker.cpp:320-330): max(row, perm[0..row])
-            const int inv = g.scatter(l, (l < d) ? pr : l);
-            const uint64_t seen = g.ballot(l < d && inv <= row);
-            const int mx = hib64(seen);
-            rtc = mx > row ? mx : row;
+            v.verdict = 0;
+            v.frow = row;
+            v.fcol = dec;
+            v.rtc = row_to_change(g, pr, d, row);
             break;
         }
         // automorphism: record a generator
-        if (o.gens && ngen < o.gen_cap) {
-            uint8_t *gp = o.gens + ((uint64_t)ci * o.gen_cap + ngen) * ISE_MAX_COLS;
-            if (!row_permut) {
+        if (o.gens && v.ngen < o.gen_cap) {
+            uint8_t *gp = o.gens + ((uint64_t)ci * o.gen_cap + v.ngen) * ISE_MAX_COLS;
+            if (!c.row_permut) {
                 // permutation induced on the column orbits of the next row
                 // (CanonicityChecker.h:427-450)
-                const int norb = popc64(cell_next);
+                const int norb = popc64(c.cell_next);
                 for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
                     const int x = x0 + l;
-                    const int src = x < norb ? select_bit(cell_next, x) : 0;
+                    const int src = x < norb ? select_bit(c.cell_next, x) : 0;
                     const int pcs = g.gather(pc, src < G ? src : 0);
-                    gp[x] = (uint8_t)(x < norb ? popc64(cell_next & lowmask(pcs)) : 0xff);
+                    gp[x] = (uint8_t)(x < norb ? popc64(c.cell_next & lowmask(pcs)) : 0xff);
                 }
             } else {
                 for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
                     const int x = x0 + l;
-                    const int v = g.gather(pr, x < G ? x : 0);
-                    gp[x] = (uint8_t)(x < d ? v : 0xff);
+                    const int vv = g.gather(pr, x < G ? x : 0);
+                    gp[x] = (uint8_t)(x < d ? vv : 0xff);
                 }
             }
         }
-        ngen++;
-        add_automorphism(g, st, pr, ob, d, row_permut);
+        v.ngen++;
+        add_automorphism(g, st, pr, ob, d, c.row_permut);
         row = IMAX;
     }
-    if (verdict == 1 && row_permut)
+    if (v.verdict == 1 && c.row_permut)
         grow_order(g, st, ob, d);
+    v.order = st.order;
+    (void)b;
+}
+
+// Witness hunt in a proposed order: depth-first over row prefixes, position i
+// trying rows in the order order[i*od + 0..d-1] (ascending when `order` is
+// null), with the same comparison and column state as the plain search but
+// no automorphism pruning; at most `budget` rows compared.  Extending the
+// identity prefix by its own row before the first comparison is equal by
+// construction and not compared.
+// Returns true with a witness in pr (prefix, then the unused rows in
+// increasing order) and v.frow / v.fcol / v.rtc set.
+template <int G, bool COUNT>
+ISE_DEV bool hunt(const Grp<G> &g, const Staged &c, const uint32_t *geo, const uint8_t *order, int od,
+                  uint32_t budget, int &pr, int &pc, Verdict &v) {
+    const int l = g.ln, d = c.d;
+    int cursor = 0;            // lane i: next index into order row i
+    uint32_t used = 0;         // rows placed at positions 0..i-1
+    int i = 0;
+    int idlen = 0;             // positions 0..idlen-1 hold the identity
+    uint32_t nodes = 0;
+    for (;;) {
+        if (i == d) {          // the whole matrix matched: backtrack
+            i--;
+            used &= ~(1u << g.read(pr, i));
+            if (idlen > i) idlen = i;
+            continue;
+        }
+        int k = g.read(cursor, i);
+        int j = -1;
+        while (k < d) {
+            const int cand = order ? (int)order[i * od + k] : k;
+            k++;
+            if (!((used >> cand) & 1)) { j = cand; break; }
+        }
+        if (l == i) cursor = k;
+        if (j < 0) {           // depth exhausted
+            if (i == 0) return false;
+            i--;
+            used &= ~(1u << g.read(pr, i));
+            if (idlen > i) idlen = i;
+            continue;
+        }
+        if (l == i) pr = j;
+        if (j == i && idlen == i && nodes == 0) {
+            // identity prefix extended by its own row before any comparison:
+            // the column permutation is still the identity, the rows are
+            // equal and comparing would move no column
+            used |= 1u << j;
+            idlen = ++i;
+            if (l == i) cursor = 0;
+            continue;
+        }
+        if (nodes == budget) return false;
+        nodes++;
+        v.perms++;
+        if constexpr (COUNT) v.rows_cmp++;
+        int dec = 0;
+        const int outcome = compare_row<G, COUNT>(g, c, geo, i, j, pc, &dec, v.orbs_cmp, v.cells_cmp);
+        if (outcome < 0) {
+            // complete the permutation with the unused rows in increasing order
+            const uint32_t u = ~(used | (1u << j)) & (uint32_t)lowmask(d);
+            if (l > i && l < d) pr = select_bit((uint64_t)u, l - i - 1);
+            v.verdict = 0;
+            v.frow = i;
+            v.fcol = dec;
+            v.rtc = row_to_change(g, pr, d, i);
+            return true;
+        }
+        if (outcome > 0) continue;
+        used |= 1u << j;
+        i++;
+        if (l == i) cursor = 0;
+    }
+}
+
+// Full canonicity test of candidate `ci` by the group g.  With HUNT, a
+// budgeted witness hunt in the proposed order runs first; the plain search
+// decides whatever the hunt leaves open (canonical candidates, |Aut|).
+template <int G, bool COUNT, bool HUNT>
+ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const uint64_t *rows,
+                             const uint64_t *cells, uint32_t stride, const OutPtrs &o, uint32_t max_perms,
+                             uint32_t *geo, const uint8_t *order, int od, uint32_t hunt_budget) {
+    const int l = g.ln;
+    const Staged c = stage(g, ci, meta, rows, cells, stride, geo);
+    const int d = c.d, b = c.b;
+    int pr = l, ob = l, pc = l;
+    Verdict v{1, EMAX, EMAX, EMAX, 1ull, 0u, 0u, 0u, 0u, 0u};
+    bool found = false;
+    if constexpr (HUNT) {
+        found = hunt<G, COUNT>(g, c, geo, order ? order + (uint64_t)ci * od * od : nullptr, od, hunt_budget, pr,
+                               pc, v);
+        if (!found) { pr = l; pc = l; }
+    }
+    if (!found)
+        plain_search<G, COUNT>(g, c, geo, ci, o, max_perms, pr, ob, pc, v);
 
     // write the record
     if (l == 0) {
-        o.verdict[ci] = (uint8_t)verdict;
-        if (o.fail_row) o.fail_row[ci] = (uint8_t)frow;
-        if (o.fail_col) o.fail_col[ci] = (uint8_t)fcol;
-        if (o.row_to_change) o.row_to_change[ci] = (uint8_t)rtc;
-        if (o.group_order) o.group_order[ci] = verdict == 1 ? st.order : 0ull;
-        if (o.n_gens) o.n_gens[ci] = (uint16_t)(ngen > 0xffffu ? 0xffffu : ngen);
+        o.verdict[ci] = (uint8_t)v.verdict;
+        if (o.fail_row) o.fail_row[ci] = (uint8_t)v.frow;
+        if (o.fail_col) o.fail_col[ci] = (uint8_t)v.fcol;
+        if (o.row_to_change) o.row_to_change[ci] = (uint8_t)v.rtc;
+        if (o.group_order) o.group_order[ci] = v.verdict == 1 ? v.order : 0ull;
+        if (o.n_gens) o.n_gens[ci] = (uint16_t)(v.ngen > 0xffffu ? 0xffffu : v.ngen);
         if (COUNT && o.counters) {
-            uint32_t *c = o.counters + (uint64_t)ci * 4;
-            c[0] = perms; c[1] = rows_cmp; c[2] = orbs_cmp; c[3] = cells_cmp;
+            uint32_t *cc = o.counters + (uint64_t)ci * 4;
+            cc[0] = v.perms; cc[1] = v.rows_cmp; cc[2] = v.orbs_cmp; cc[3] = v.cells_cmp;
         }
     }
     // every lane takes part in the gathers; lanes past the record skip the store

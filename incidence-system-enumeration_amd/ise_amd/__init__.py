@@ -35,6 +35,10 @@ ERR_NODEV = -3
 ERR_CAPACITY = -4
 ERR_STATE = -5
 
+ORDER_PLAIN = 0     # the reference's search alone
+ORDER_HUNT = 1      # budgeted witness hunt in increasing row order, then the plain search
+ORDER_LEARNED = 2   # budgeted witness hunt in a per-candidate order table, then the plain search
+
 QUEUE_WORDS = 256   # ISE_QUEUE_WORDS: device words behind a work-queue argument
 
 _lib = None
@@ -71,6 +75,7 @@ def lib():
         L.ise_copy_to_device.argtypes = [P, P, P, U64]
         L.ise_copy_to_host.argtypes = [P, P, P, U64]
         L.ise_copy_on_device.argtypes = [P, P, P, U64]
+        L.ise_set_order.argtypes = [P, I, U32, P, U32]
         L.ise_use_graphs.argtypes = [P, I]
         L.ise_graph_stats.argtypes = [P, ctypes.POINTER(U32), ctypes.POINTER(U64)]
         L.ise_autotune.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P, U32, U32, U32, P, P]
@@ -254,6 +259,11 @@ class Context:
             self.h, n, stride, d_rows.ptr, d_cells.ptr if d_cells is not None else None, d_meta.ptr,
             ctypes.byref(res_struct), d_head.ptr, repeats, ms.ctypes.data))
         return ms
+
+    def set_order(self, mode=ORDER_PLAIN, hunt_budget=0, d_order=None, order_dim=0):
+        """Search order of the following launches (verdicts never depend on it)."""
+        ptr = None if d_order is None else d_order.ptr
+        self._check(self._L.ise_set_order(self.h, mode, hunt_budget, ptr, order_dim))
 
     def use_graphs(self, on=True):
         self._check(self._L.ise_use_graphs(self.h, 1 if on else 0))
@@ -460,7 +470,7 @@ class GPUCanonChecker:
 
 
 __all__ = [
-    "QUEUE_WORDS", "kf_random", "kf_pack", "kf_unpack", "kf_alloc", "KF_FIELDS", "Context", "DeviceArray", "ResultsStruct", "CanonicityChecker", "GPUCanonChecker", "ISEError", "random_extensions", "lib",
+    "QUEUE_WORDS", "ORDER_PLAIN", "ORDER_HUNT", "ORDER_LEARNED", "kf_random", "kf_pack", "kf_unpack", "kf_alloc", "KF_FIELDS", "Context", "DeviceArray", "ResultsStruct", "CanonicityChecker", "GPUCanonChecker", "ISEError", "random_extensions", "lib",
     "design_params", "alloc_results", "pack_meta", "pack_rows", "unpack_rows", "MAX_ROWS",
     "MAX_COLS", "SAVE_ROW_PERMUTATIONS", "SAVE_COL_PERMUTATIONS", "SAVE_ROW_TO_CHANGE",
     "VERDICT_CANONICAL", "VERDICT_NONCANONICAL", "VERDICT_LIMIT", "VERDICT_INVALID",

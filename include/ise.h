@@ -91,6 +91,26 @@ typedef struct {
     uint32_t max_perms;       /* search limit per candidate (verdict 2 beyond it) */
 } ise_launch_cfg;
 
+/* Search order of the canonicity test (the verdict never depends on it):
+ *  ISE_ORDER_PLAIN    the reference's search alone (its first witness);
+ *  ISE_ORDER_HUNT     first a witness hunt of at most `hunt_budget` row
+ *                     comparisons, depth-first over row prefixes in increasing
+ *                     row order without automorphism pruning, then the plain
+ *                     search for the candidates it leaves open;
+ *  ISE_ORDER_LEARNED  the same with the rows of each position tried in the
+ *                     order given per candidate by `d_order`
+ *                     ([n][order_dim][order_dim] bytes, row i of a candidate
+ *                     lists the rows to try at position i), e.g. ranked by the
+ *                     learned scorer (ise_mlp_order).
+ * A hunt witness is reported in place of the plain one (verdict, fail_row,
+ * rowToChange and the witness permutation refer to it); canonical verdicts,
+ * |Aut| and generators always come from the plain search. */
+#define ISE_ORDER_PLAIN   0
+#define ISE_ORDER_HUNT    1
+#define ISE_ORDER_LEARNED 2
+int ise_set_order(ise_ctx *ctx, int mode, uint32_t hunt_budget, const uint8_t *d_order,
+                  uint32_t order_dim);
+
 /* Open a context on HIP device `device` able to hold `capacity` candidates of
  * up to `stride` rows per submit.  Replaces the construction of
  * CGPU_CanonChecker (Source/MatrixCanonChecker.h:204-290) and the CDTools
@@ -187,6 +207,56 @@ float ise_last_kernel_ms(const ise_ctx *ctx);
 
 /* Last HIP error string of the context ("" if none). */
 const char *ise_last_error(const ise_ctx *ctx);
+
+/* ---- learned row order ---------------------------------------------------
+ * A small MLP scores (candidate, position i, row j) pairs from 16 features of
+ * row j placed at position i under the identity prefix (the search's own
+ * orbit-by-orbit comparison of row j with row i, intersections, positions);
+ * the rows of each position are then tried by decreasing score in the
+ * witness hunt (ISE_ORDER_LEARNED).  Network: 16 -> 128 -> 128 -> 1 with ReLU,
+ * bf16 weights and FP32 accumulation on MFMA; trained on the GPU (binary
+ * cross-entropy, Adam) by the library's own forward and backward kernels.
+ * Parameters are exchanged as ISE_MLP_PARAMS floats packed as W1[16][128],
+ * b1[128], W2[128][128], b2[128], W3[128][16] (column 0 used), b3[16]. */
+#define ISE_MLP_FEATURES 16
+#define ISE_MLP_PARAMS 20752
+
+typedef struct ise_mlp ise_mlp;
+
+/* Model on the context's device with workspace for up to max_rows pair rows
+ * per call (rounded up to 16); weights initialised from `seed`. */
+int ise_mlp_create(ise_ctx *ctx, ise_mlp **out, uint32_t max_rows, uint64_t seed);
+int ise_mlp_destroy(ise_mlp *m);
+int ise_mlp_get_params(const ise_mlp *m, float *host_params);
+int ise_mlp_set_params(ise_mlp *m, const float *host_params);
+
+/* Pair features of a device batch: d_X[(c*od + i)*od + j][16] bf16, zero for
+ * i or j >= depth.  od in 1..32, every depth <= od. */
+int ise_mlp_features(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                     const uint32_t *d_meta, uint32_t od, uint16_t *d_X);
+
+/* Scores (logits) of M pair rows (M a multiple of 16, <= max_rows):
+ * d_score[m] = network(d_X[m]). */
+int ise_mlp_forward(ise_mlp *m, const uint16_t *d_X, uint32_t M, float *d_score);
+
+/* Gradient of the weighted mean binary cross-entropy of the logits against
+ * labels d_y with weights d_w (all device, M rows): writes the gradient
+ * (ISE_MLP_PARAMS floats, packed like the parameters) to host_grad and the
+ * loss to *loss.  Does not change the parameters. */
+int ise_mlp_gradient(ise_mlp *m, const uint16_t *d_X, const float *d_y, const float *d_w, uint32_t M,
+                     float *host_grad, float *loss);
+
+/* One Adam step (lr, beta1 0.9, beta2 0.999, eps 1e-8) on the same loss;
+ * *loss receives the loss before the step. */
+int ise_mlp_train_step(ise_mlp *m, const uint16_t *d_X, const float *d_y, const float *d_w, uint32_t M,
+                       float lr, float *loss);
+
+/* Features + scores + ordering for a device batch: d_order[c][i][0..od-1]
+ * lists the rows of candidate c by decreasing score for position i (ties by
+ * increasing row), the input of ISE_ORDER_LEARNED.  Works in chunks that fit
+ * the model's workspace. */
+int ise_mlp_order(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows, const uint32_t *d_meta,
+                  uint32_t od, uint8_t *d_order);
 
 /* ---- 1-factorizations of K_n (groups of two players) --------------------
  * Wire format: rounds[cand][stride] uint64, nibble p of word i = player at

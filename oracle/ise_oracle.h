@@ -94,6 +94,21 @@ long orc_enumerate_bibd(int v, int k, int lambda, uint64_t *designs,
 int orc_bruteforce_canon(const uint8_t *mat, int nrows, int ncols,
                          uint8_t *canon_out, uint64_t *n_row_auts);
 
+/* Witness hunt: depth-first search over row prefixes where position i tries
+ * rows in the order order[i*d + 0..d-1]; same comparison rule and column
+ * state as the plain search; at most `budget` children compared.  Returns 1
+ * (witness in perm_out, first larger row in *fail_row) or 0. */
+int orc_hunt(const uint8_t *mat, int d, int b, const uint8_t *order, uint32_t budget,
+             uint8_t *perm_out, uint8_t *fail_row, uint32_t *nodes);
+
+/* Batch test with a witness hunt first (order NULL = increasing row order;
+ * else [n][order_dim][order_dim] bytes), then the plain search for the
+ * candidates the hunt leaves open.  Same result layout as orc_check_batch;
+ * counters add the hunt's comparisons to the plain search's. */
+int orc_check_batch_hunt(uint32_t n, uint32_t stride, const uint64_t *rows, const uint64_t *cells,
+                         const uint32_t *meta, const uint8_t *order, uint32_t order_dim,
+                         uint32_t budget, uint32_t gen_cap, orc_results *res);
+
 /* ---- k-factorizations of K_n (n <= 16): rounds of n/2 pairs ---- */
 /* Canonicity of a partial 1-factorization (nrounds x n bytes, each round the
  * n players as n/2 ordered pairs).  Returns 1 canonical, 0 not, <0 error.

@@ -231,13 +231,23 @@ typedef struct {
     int norb[ORC_MAX_ROWS + 1];
 } cand_t;
 
+static int run_check_from(const cand_t *cd, orc_results *res, size_t idx, uint32_t gen_cap,
+                          const uint32_t *init);
+
 static int run_check(const cand_t *cd, orc_results *res, size_t idx, uint32_t gen_cap)
+{
+    return run_check_from(cd, res, idx, gen_cap, NULL);
+}
+
+static int run_check_from(const cand_t *cd, orc_results *res, size_t idx, uint32_t gen_cap,
+                          const uint32_t *init)
 {
     const int d = cd->nrows, b = cd->ncols;
     const int row_permut = (cd->out_info & ORC_SAVE_ROW_PERMUTATIONS) != 0;
     u8 perm[ORC_MAX_ROWS], orb[ORC_MAX_ROWS], pcol[ORC_MAX_COLS];
     grp_t g;
-    uint32_t perms = 0, rows_cmp = 0, orbs_cmp = 0, cells = 0;
+    uint32_t perms = init ? init[0] : 0, rows_cmp = init ? init[1] : 0;
+    uint32_t orbs_cmp = init ? init[2] : 0, cells = init ? init[3] : 0;
     uint16_t ngen = 0;
     int verdict = 1;
     u8 frow = EMAX, fcol = EMAX, rtc = EMAX;
@@ -622,4 +632,164 @@ long orc_enumerate_bibd(int v, int k, int lambda, uint64_t *designs,
     if (stats) memcpy(stats, st, sizeof st);
     free(e);
     return nd;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Witness hunt in a proposed order (the learned-order search).              */
+/*
+ * Depth-first search over row prefixes in which position i tries the rows in
+ * the order order[i][0..d-1] (rows already placed are skipped).  A row is
+ * compared with the same orbit-by-orbit rule and column-permutation state as
+ * the plain search (compare_orbit); "smaller" prunes the child, "larger" is a
+ * witness (stop), "equal" descends.  At most `budget` children are compared.
+ * No automorphism pruning: the hunt only looks for a witness; canonicity is
+ * proved by the plain search.  Returns 1 with the witness in perm_out (prefix
+ * perm[0..fail_row], then the unused rows in increasing order) or 0 when the
+ * budget or the tree is exhausted; *nodes receives the children compared.
+ */
+static int hunt_core(const cand_t *cd, const uint8_t *order, uint32_t budget, u8 *perm_out,
+                     u8 *pcol_out, u8 *fail_row, u8 *fail_col, uint32_t *cnt)
+{
+    const int d = cd->nrows, b = cd->ncols;
+    const u8 *mat = cd->mat;
+    u8 perm[ORC_MAX_ROWS], pcol[ORC_MAX_COLS];
+    int cursor[ORC_MAX_ROWS + 1];
+    uint32_t used = 0, n = 0;
+    for (int q = 0; q < b; q++) pcol[q] = (u8)q;
+    int i = 0, found = 0, idlen = 0;      /* perm[0..idlen-1] is the identity */
+    u8 dec = 0;
+    cursor[0] = 0;
+    for (;;) {
+        if (i == d) {                      /* the whole matrix matched: backtrack */
+            i--;
+            used &= ~(1u << perm[i]);
+            if (idlen > i) idlen = i;
+            continue;
+        }
+        while (cursor[i] < d && ((used >> (order ? order[i * d + cursor[i]] : cursor[i])) & 1))
+            cursor[i]++;
+        if (cursor[i] == d) {
+            if (i == 0) break;
+            i--;
+            used &= ~(1u << perm[i]);
+            if (idlen > i) idlen = i;
+            continue;
+        }
+        const u8 j = order ? order[i * d + cursor[i]] : (u8)cursor[i];
+        cursor[i]++;
+        perm[i] = j;
+        if (j == i && idlen == i && n == 0) {
+            /* identity prefix extended by its own row before any comparison
+             * (the column permutation is still the identity): equal, and the
+             * comparison would leave the columns in place */
+            used |= 1u << j;
+            idlen = ++i;
+            cursor[i] = 0;
+            continue;
+        }
+        if (n == budget) break;
+        n++;
+        cnt[0]++; cnt[1]++;
+        const u8 *orig = mat + (size_t)i * b, *prow = mat + (size_t)j * b;
+        int diff = 0, o;
+        for (o = 0; o < cd->norb[i]; o++) {
+            int s = cd->ostart[i][o], len = cd->olen[i][o];
+            cnt[2]++;
+            cnt[3] += (uint32_t)len;
+            diff = len > 1 ? compare_orbit(s, len, orig, prow, pcol) : (int)orig[s] - (int)prow[pcol[s]];
+            if (diff) break;
+        }
+        if (diff < 0) { found = 1; dec = cd->ostart[i][o]; break; }
+        if (diff > 0) continue;
+        used |= 1u << j;
+        i++;
+        cursor[i] = 0;
+    }
+    if (found) {
+        uint32_t u = used | (1u << perm[i]);
+        int k = i + 1;
+        for (int r = 0; r < d; r++)
+            if (!((u >> r) & 1)) perm[k++] = (u8)r;
+        if (perm_out) memcpy(perm_out, perm, (size_t)d);
+        if (pcol_out) memcpy(pcol_out, pcol, (size_t)b);
+        if (fail_row) *fail_row = (u8)i;
+        if (fail_col) *fail_col = dec;
+    }
+    return found;
+}
+
+int orc_hunt(const uint8_t *mat, int d, int b, const uint8_t *order, uint32_t budget,
+             uint8_t *perm_out, uint8_t *fail_row, uint32_t *nodes)
+{
+    if (d < 1 || d > ORC_MAX_ROWS || b < 1 || b > ORC_MAX_COLS) return -1;
+    cand_t *cd = (cand_t *)malloc(sizeof(cand_t));
+    uint64_t rows[ORC_MAX_ROWS];
+    for (int i = 0; i < d; i++) {
+        rows[i] = 0;
+        for (int q = 0; q < b; q++)
+            if (mat[i * b + q]) rows[i] |= 1ull << q;
+    }
+    cd->nrows = d; cd->ncols = b; cd->len_stab = 0; cd->out_info = 0; cd->mat = mat;
+    build_orbits(cd, NULL, rows);
+    uint32_t cnt[4] = {0, 0, 0, 0};
+    int found = hunt_core(cd, order, budget, perm_out, NULL, fail_row, NULL, cnt);
+    if (nodes) *nodes = cnt[0];
+    free(cd);
+    return found;
+}
+
+int orc_check_batch_hunt(uint32_t n, uint32_t stride, const uint64_t *rows, const uint64_t *cells,
+                         const uint32_t *meta, const uint8_t *order, uint32_t order_dim,
+                         uint32_t budget, uint32_t gen_cap, orc_results *res)
+{
+    static __thread cand_t cd;
+    static __thread u8 mat[ORC_MAX_ROWS * ORC_MAX_COLS];
+    for (uint32_t c = 0; c < n; c++) {
+        uint32_t m = meta[c];
+        int d = m & 0xff, b = (m >> 8) & 0xff;
+        if (d < 1 || d > ORC_MAX_ROWS || d > (int)stride || b < 1 || b > ORC_MAX_COLS)
+            return -1;
+        if (order && d > (int)order_dim) return -1;
+        const uint64_t *rp = rows + (size_t)c * stride;
+        cd.nrows = d; cd.ncols = b;
+        cd.len_stab = (m >> 16) & 0xff;
+        cd.out_info = (m >> 24) & 0xff;
+        for (int i = 0; i < d; i++)
+            for (int q = 0; q < b; q++)
+                mat[i * b + q] = (u8)((rp[i] >> q) & 1);
+        cd.mat = mat;
+        build_orbits(&cd, cells ? cells + (size_t)c * stride : NULL, rp);
+        /* the order table of a candidate is order_dim x order_dim; compact it to d x d */
+        u8 ord[ORC_MAX_ROWS * ORC_MAX_ROWS];
+        if (order)
+            for (int i = 0; i < d; i++)
+                for (int k = 0; k < d; k++)
+                    ord[i * d + k] = order[(size_t)c * order_dim * order_dim + (size_t)i * order_dim + k];
+        uint32_t cnt[4] = {0, 0, 0, 0};
+        u8 perm[ORC_MAX_ROWS], pcol[ORC_MAX_COLS], frow = 255, fcol = 255;
+        if (hunt_core(&cd, order ? ord : NULL, budget, perm, pcol, &frow, &fcol, cnt)) {
+            if (res->verdict) res->verdict[c] = 0;
+            if (res->fail_row) res->fail_row[c] = frow;
+            if (res->fail_col) res->fail_col[c] = fcol;
+            if (res->row_to_change) res->row_to_change[c] = row_to_change(perm, frow);
+            if (res->group_order) res->group_order[c] = 0;
+            if (res->perm_row) {
+                memset(res->perm_row + c * ORC_MAX_ROWS, 0xff, ORC_MAX_ROWS);
+                memcpy(res->perm_row + c * ORC_MAX_ROWS, perm, (size_t)d);
+            }
+            if (res->perm_col) {
+                memset(res->perm_col + c * ORC_MAX_COLS, 0xff, ORC_MAX_COLS);
+                memcpy(res->perm_col + c * ORC_MAX_COLS, pcol, (size_t)b);
+            }
+            if (res->orbits) {
+                memset(res->orbits + c * ORC_MAX_ROWS, 0xff, ORC_MAX_ROWS);
+                for (int i = 0; i < d; i++) res->orbits[c * ORC_MAX_ROWS + i] = (u8)i;
+            }
+            if (res->n_gens) res->n_gens[c] = 0;
+            if (res->counters) memcpy(res->counters + c * 4, cnt, sizeof cnt);
+        } else {
+            run_check_from(&cd, res, c, gen_cap, cnt);
+        }
+    }
+    return 0;
 }

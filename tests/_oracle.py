@@ -36,6 +36,8 @@ def load():
                                          _P, _P, _P, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_enumerate_bibd.restype = ctypes.c_long
         L.orc_bruteforce_canon.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P]
+        L.orc_check_batch_hunt.argtypes = [U32, U32, _P, _P, _P, _P, U32, U32, U32, _P]
+        L.orc_hunt.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, U32, _P, _P, _P]
         L.orc_kf_check.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.orc_kf_normalize.argtypes = [_P, ctypes.c_int, ctypes.c_int]
         L.orc_kf_bruteforce.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
@@ -64,6 +66,24 @@ class Oracle:
                                     ctypes.addressof(s))
         if rc != 0:
             raise ValueError(f"oracle rejected the batch ({rc})")
+        return res
+
+    def check_batch_hunt(self, rows, meta, order=None, order_dim=0, budget=64, gen_cap=0):
+        """Witness hunt first (increasing row order, or order [n][dim][dim]), then plain."""
+        import ise_amd._abi as abi
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32)
+        n, stride = rows.shape
+        res = abi.alloc_results(n, gen_cap)
+        s = abi.results_struct(res, gen_cap)
+        optr = None
+        if order is not None:
+            order = np.ascontiguousarray(order, dtype=np.uint8)
+            optr = order.ctypes.data
+        rc = self.L.orc_check_batch_hunt(n, stride, rows.ctypes.data, None, meta.ctypes.data, optr, order_dim,
+                                         budget, gen_cap, ctypes.addressof(s))
+        if rc != 0:
+            raise ValueError("oracle rejected the batch")
         return res
 
     def cell_masks(self, rows, nrows, ncols):

@@ -156,3 +156,30 @@ def test_cell_masks_given_or_derived_agree(oracle):
     c = oracle.check_batch(rows, meta, cells=cells)
     for f in a:
         assert np.array_equal(a[f], c[f]), f
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_hunt_never_changes_a_verdict(oracle, seed):
+    """A witness hunt in any row order (random here) followed by the plain
+    search gives exactly the plain verdicts and, for canonical candidates, the
+    same |Aut|; every hunt witness makes its matrix larger."""
+    import ise_amd
+    r1, m1 = ise_amd.random_extensions(13, 3, 1, 1500, 3, 13, seed)
+    _, _, _, (r2, m2) = oracle.enumerate(9, 3, 2, capture=(1 << 13, 32, 3, 9))
+    rows, meta = np.concatenate([r1, r2]), np.concatenate([m1, m2])
+    rng = np.random.default_rng(seed)
+    dim = 16
+    order = np.zeros((len(meta), dim, dim), np.uint8)
+    for c in range(len(meta)):
+        d = int(meta[c]) & 0xff
+        for i in range(d):
+            order[c, i, :d] = rng.permutation(d)
+    plain = oracle.check_batch(rows, meta)
+    for kw in (dict(order=order, order_dim=dim, budget=40), dict(budget=3)):
+        h = oracle.check_batch_hunt(rows, meta, **kw)
+        assert np.array_equal(h["verdict"], plain["verdict"])
+        canon = plain["verdict"] == 1
+        assert np.array_equal(h["group_order"][canon], plain["group_order"][canon])
+        for i in np.nonzero(h["verdict"] == 0)[0][:200]:
+            d, b = int(meta[i]) & 0xff, (int(meta[i]) >> 8) & 0xff
+            check_witness(unpack(rows[i, :d], b), h["perm_row"][i], int(h["fail_row"][i]))
