@@ -177,8 +177,32 @@ struct EnumSink {
     uint32_t cand_stride = 0;
 };
 
+inline uint64_t splitmix(uint64_t &s);
+
+// Keep `beam` of the canonical candidates of a level (beam 0 keeps all):
+// partial Fisher-Yates over their indices driven by splitmix64, the chosen
+// indices then restored to level order.  The generator state carries over.
+void sample_beam(std::vector<uint64_t> &keep, uint32_t stride, uint32_t beam, uint64_t &rng) {
+    const size_t n = keep.size() / stride;
+    if (beam == 0 || n <= beam) return;
+    std::vector<uint32_t> idx(n);
+    for (size_t i = 0; i < n; i++) idx[i] = (uint32_t)i;
+    for (uint32_t t = 0; t < beam; t++) {
+        const size_t j = t + (size_t)(splitmix(rng) % (uint64_t)(n - t));
+        std::swap(idx[t], idx[j]);
+    }
+    idx.resize(beam);
+    std::sort(idx.begin(), idx.end());
+    std::vector<uint64_t> out((size_t)beam * stride);
+    for (uint32_t t = 0; t < beam; t++)
+        memcpy(&out[(size_t)t * stride], &keep[(size_t)idx[t] * stride], stride * sizeof(uint64_t));
+    keep.swap(out);
+}
+
 long run_enumeration(ise_ctx *ctx, const Design &ds, EnumSink &sink, uint64_t *stats,
-                     uint64_t *level_counts, uint32_t stride, uint32_t capacity) {
+                     uint64_t *level_counts, uint32_t stride, uint32_t capacity, uint32_t beam = 0,
+                     uint64_t seed = 0) {
+    uint64_t rng = seed;
     uint64_t st[5] = {0, 0, 0, 0, 0};
     double kernel_ms = 0.0;
     if (level_counts)
@@ -238,6 +262,7 @@ long run_enumeration(ise_ctx *ctx, const Design &ds, EnumSink &sink, uint64_t *s
                 }
             }
         }
+        sample_beam(keep, stride, beam, rng);
         if (depth < ds.v) extend_level(ds, keep, depth, stride, next);
         else next.clear();
         level.swap(next);
@@ -367,6 +392,27 @@ long ise_enumerate_bibd_candidates(ise_ctx *ctx, int v, int k, int lambda, int d
     long rc = run_enumeration(ctx, ds, sink, nullptr, nullptr, cstride, ise::ctx_capacity(ctx));
     *n_out = sink.ncand;
     return rc;
+}
+
+long ise_enumerate_bibd_sampled(ise_ctx *ctx, int v, int k, int lambda, uint32_t beam, uint64_t seed, int dmin,
+                                int dmax, uint32_t stride, uint64_t *rows_out, uint32_t *meta_out, long cap,
+                                long *n_out, uint64_t *level_counts) {
+    Design ds;
+    if (!ctx || !derive(v, k, lambda, &ds) || !rows_out || !meta_out || !n_out || beam == 0 ||
+        stride < (uint32_t)dmax || dmin < 1 || dmax < dmin)
+        return ISE_ERR_ARG;
+    const uint32_t cstride = ise::ctx_stride(ctx);
+    if ((int)cstride < v) return ISE_ERR_ARG;
+    EnumSink sink;
+    sink.dmin = dmin;
+    sink.dmax = dmax;
+    sink.cand_rows = rows_out;
+    sink.cand_meta = meta_out;
+    sink.cand_cap = cap;
+    sink.cand_stride = stride;
+    long rc = run_enumeration(ctx, ds, sink, nullptr, level_counts, cstride, ise::ctx_capacity(ctx), beam, seed);
+    *n_out = sink.ncand;
+    return rc < 0 ? rc : sink.ncand;
 }
 
 int ise_random_extensions(int v, int k, int lambda, uint64_t n, int dmin, int dmax, uint64_t seed,

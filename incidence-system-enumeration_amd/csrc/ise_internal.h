@@ -29,5 +29,10 @@ inline int fit_group_lanes(uint32_t max_depth, uint32_t max_cols) {
 // Geometry of a context (rows per candidate, candidates per submit).
 uint32_t ctx_stride(const ise_ctx *c);
 uint32_t ctx_capacity(const ise_ctx *c);
+hipStream_t ctx_stream(const ise_ctx *c);
+int ctx_device(const ise_ctx *c);
+int ctx_num_cus(const ise_ctx *c);
+void ctx_set_error(ise_ctx *c, const char *msg);
+void ctx_get_order(const ise_ctx *c, int *mode, uint32_t *budget, const uint8_t **order, uint32_t *dim);
 
 } // namespace ise

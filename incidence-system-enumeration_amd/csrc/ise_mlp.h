@@ -26,6 +26,17 @@ hipError_t adam(long count, float *w, const float *g, float *m1, float *m2, uint
 hipError_t to_bf16(long count, const float *x, uint16_t *y, hipStream_t s);
 hipError_t features(uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta, uint32_t od,
                     uint16_t *X, hipStream_t s);
+// Weights as MFMA A fragments (FRAG words of uint16) for learned_order.
+constexpr long MLP_FRAG_WORDS = 80L * 64 * 4;
+hipError_t frag_pack(const uint16_t *wb, uint16_t *frag, hipStream_t s);
+// Fused features -> MLP -> per-position order, one wave per candidate.
+// LDS of one workgroup: the weight fragments (40 KiB) plus the orbit
+// geometry of each wave's candidate (8 KiB per wave).
+size_t learned_order_lds(uint32_t waves_per_block);
+hipError_t learned_order(uint32_t blocks, uint32_t waves_per_block, uint32_t n, uint32_t stride,
+                         const uint64_t *rows, const uint32_t *meta, uint32_t od, const uint16_t *frag,
+                         const float *w, uint8_t *order, float *score, hipStream_t s);
+hipError_t learned_order_occupancy(uint32_t waves_per_block, int *blocks_per_cu);
 hipError_t order_from_scores(uint32_t n, const uint32_t *meta, uint32_t od, const float *score, long score_ld,
                              uint8_t *order, hipStream_t s);
 

@@ -189,9 +189,67 @@ hipError_t to_bf16(long count, const float *x, uint16_t *y, hipStream_t s) {
 //
 // For candidate c, position i and row j (i, j < d), 16 features describing
 // row j placed at position i under the identity prefix: the orbit-by-orbit
-// comparison of row j with row i (the same rule as the search), intersections
-// and positions.  Written as bf16 to X[(c*od + i)*od + j][0..15]; pairs with
-// i or j >= d are zero.
+// comparison of row j with row i (the same rule as the search), row
+// intersections, column-orbit sizes, positions and depth.  All values are
+// small multiples of powers of two, exact in bf16.
+
+// Wave-uniform statistics of one (i, j) pair.
+struct PairStats {
+    int out;        // compare_row outcome: -1 row j larger at position i, 1 smaller, 0 equal
+    int dec;        // deciding orbit's first column (b when equal)
+    int dsize;      // deciding orbit's size (0 when equal)
+    int ddiff;      // ones of row j minus ones of row i inside it
+    int inter_i;    // |R_j & R_i|
+    int inter_n;    // |R_j & R_{d-1}|
+    int first;      // ones of row j in the first column orbit of position i
+    int norb;       // column orbits at position i
+};
+
+ISE_DEV PairStats pair_stats(const Grp<64> &g, const Staged &c, const uint32_t *geo, int i, int j) {
+    PairStats s;
+    const int b = c.b, d = c.d;
+    int pc = g.ln, dec = b;
+    uint32_t dummy0 = 0, dummy1 = 0;
+    s.out = compare_row<64, false>(g, c, geo, i, j, pc, &dec, dummy0, dummy1);
+    const uint64_t Ri = g.read64(c.rowv, i), Ci = g.read64(c.cellv, i);
+    const uint64_t Rj = g.read64(c.rowv, j), Rn = g.read64(c.rowv, d - 1);
+    s.dec = s.out ? dec : b;
+    const uint64_t after = Ci & ~lowmask(s.dec + 1);
+    const int e = after ? ctz64(after) : b;
+    const uint64_t om = s.out ? (lowmask(e) & ~lowmask(s.dec)) : 0ull;
+    s.dsize = popc64(om);
+    s.ddiff = popc64(Rj & om) - popc64(Ri & om);
+    s.inter_i = popc64(Rj & Ri);
+    s.inter_n = popc64(Rj & Rn);
+    const uint64_t heads = Ci & ~1ull;
+    s.first = popc64(Rj & lowmask(heads ? ctz64(heads) : b));
+    s.norb = popc64(Ci);
+    return s;
+}
+
+ISE_DEV float pair_feature(int id, const PairStats &s, int i, int j, int d) {
+    switch (id) {
+    case 0: return s.out < 0 ? 1.f : (s.out > 0 ? -1.f : 0.f);
+    case 1: return (float)s.dec * (1.f / 64.f);
+    case 2: return (float)s.ddiff * 0.125f;
+    case 3: return (float)i * (1.f / 32.f);
+    case 4: return (float)j * (1.f / 32.f);
+    case 5: return (float)(j - i) * (1.f / 32.f);
+    case 6: return (float)d * (1.f / 32.f);
+    case 7: return j == i ? 1.f : 0.f;
+    case 8: return j == d - 1 ? 1.f : 0.f;
+    case 9: return (float)s.inter_i * 0.0625f;
+    case 10: return (float)s.norb * (1.f / 64.f);
+    case 11: return (float)s.first * 0.0625f;
+    case 12: return j < i ? 1.f : 0.f;
+    case 13: return (float)s.inter_n * 0.0625f;
+    case 14: return (float)s.dsize * (1.f / 64.f);
+    default: return 1.f;
+    }
+}
+
+// Features of every pair to X[(c*od + i)*od + j][16] (pairs with i or j >= d
+// are zero); one wave per candidate.
 __global__ __launch_bounds__(256) void features_kernel(uint32_t n, uint32_t stride, const uint64_t *__restrict__ rows,
                                                        const uint32_t *__restrict__ meta, uint32_t od,
                                                        uint16_t *__restrict__ X) {
@@ -210,43 +268,12 @@ __global__ __launch_bounds__(256) void features_kernel(uint32_t n, uint32_t stri
         return;
     }
     const Staged c = stage(g, ci, m, rows, nullptr, stride, geo);
-    uint32_t dummy0 = 0, dummy1 = 0;
-    const float inv32 = 1.f / 32.f;
     for (int i = 0; i < (int)od; i++) {
-        const uint64_t Ri = i < d ? g.read64(c.rowv, i) : 0ull;
-        const uint64_t Ci = i < d ? g.read64(c.cellv, i) : 0ull;
         for (int j = 0; j < (int)od; j++) {
             float f = 0.f;
             if (i < d && j < d) {
-                int pc = l, dec = b;
-                const int out = compare_row<64, false>(g, c, geo, i, j, pc, &dec, dummy0, dummy1);
-                const uint64_t Rj = g.read64(c.rowv, j);
-                const uint64_t Rn = g.read64(c.rowv, d - 1);
-                // ones of row j and row i inside the deciding orbit
-                const uint64_t after = Ci & ~lowmask(dec + 1);
-                const int e = after ? ctz64(after) : b;
-                const uint64_t om = (dec < b) ? (lowmask(e) & ~lowmask(dec)) : 0ull;
-                const uint64_t first = Ci & ~1ull;
-                const int e0 = first ? ctz64(first) : b;
-                switch (l) {
-                case 0: f = out < 0 ? 1.f : (out > 0 ? -1.f : 0.f); break;
-                case 1: f = (float)(out ? dec : b) / 64.f; break;
-                case 2: f = (float)(popc64(Rj & om) - popc64(Ri & om)) * 0.125f; break;
-                case 3: f = i * inv32; break;
-                case 4: f = j * inv32; break;
-                case 5: f = (j - i) * inv32; break;
-                case 6: f = d * inv32; break;
-                case 7: f = j == i ? 1.f : 0.f; break;
-                case 8: f = j == d - 1 ? 1.f : 0.f; break;
-                case 9: f = popc64(Rj & Ri) * 0.0625f; break;
-                case 10: f = popc64(Ci) / 64.f; break;
-                case 11: f = popc64(Rj & lowmask(e0)) * 0.0625f; break;
-                case 12: f = j < i ? 1.f : 0.f; break;
-                case 13: f = popc64(Rj & Rn) * 0.0625f; break;
-                case 14: f = b / 64.f; break;
-                case 15: f = 1.f; break;
-                default: break;
-                }
+                const PairStats s = pair_stats(g, c, geo, i, j);
+                f = pair_feature(l & 15, s, i, j, d);
             }
             if (l < 16) xc[((uint64_t)i * od + j) * 16 + l] = f2bf(f);
         }
@@ -260,6 +287,164 @@ hipError_t features(uint32_t n, uint32_t stride, const uint64_t *rows, const uin
     hipLaunchKernelGGL(features_kernel, dim3((n + wpb - 1) / wpb), dim3(64 * wpb), lds, s, n, stride, rows, meta, od,
                        X);
     return hipGetLastError();
+}
+
+// ---- fused scoring: features -> MLP -> per-position row order -------------
+//
+// The products are computed transposed, H^T = W^T X^T, with the weights as
+// the A operand: the C fragment of one layer (n = pair row = l % 16, m = unit
+// 4*(l/16) + t) is then exactly the B fragment of the next (k = unit, n =
+// row), so activations stay in registers from the features to the score.
+// The weights are pre-arranged as A fragments (frag_pack_kernel) and staged
+// in LDS once per workgroup: tile (mt, kt) of a layer holds, for lane l,
+// W[kt*16 + 4*(l/16) + t][mt*16 + l % 16], t = 0..3.
+
+constexpr int FRAG_L1 = 0, FRAG_L2 = 8, FRAG_L3 = 8 + 64, FRAG_TILES = 8 + 64 + 8;
+
+__global__ void frag_pack_kernel(const uint16_t *__restrict__ wb, uint16_t *__restrict__ frag) {
+    const int tile = blockIdx.x, l = threadIdx.x;   // 80 blocks of 64 lanes
+    const int q = l >> 4, r = l & 15;
+    for (int t = 0; t < 4; t++) {
+        uint16_t v;
+        if (tile < FRAG_L2) {                         // W1 [16][128]: mt = tile, kt = 0
+            v = wb[MLP_W1 + (4 * q + t) * MLP_HID + tile * 16 + r];
+        } else if (tile < FRAG_L3) {                  // W2 [128][128]
+            const int mt = (tile - FRAG_L2) / 8, kt = (tile - FRAG_L2) % 8;
+            v = wb[MLP_W2 + (kt * 16 + 4 * q + t) * MLP_HID + mt * 16 + r];
+        } else {                                      // W3 [128][16]: mt = 0
+            const int kt = tile - FRAG_L3;
+            v = wb[MLP_W3 + (kt * 16 + 4 * q + t) * MLP_OUT_PAD + r];
+        }
+        frag[((size_t)tile * 64 + l) * 4 + t] = v;
+    }
+}
+
+hipError_t frag_pack(const uint16_t *wb, uint16_t *frag, hipStream_t s) {
+    hipLaunchKernelGGL(frag_pack_kernel, dim3(FRAG_TILES), dim3(64), 0, s, wb, frag);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ bf16x4 relu_bias_bf16(f32x4 acc, const float *bias) {
+    bf16x4 h;
+    for (int t = 0; t < 4; t++) {
+        const float v = acc[t] + bias[t];
+        h[t] = (short)f2bf(v > 0.f ? v : 0.f);
+    }
+    return h;
+}
+
+// Scores of one tile of 16 pair rows given as a B fragment; lane l < 16
+// returns the score of row l of the tile.
+__device__ __forceinline__ float mlp_tile(bf16x4 xb, const bf16x4 *__restrict__ frag, const float *__restrict__ w,
+                                          int l) {
+    const int q = l >> 4;
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    bf16x4 h1[8], h2[8];
+    for (int mt = 0; mt < 8; mt++) {
+        const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag[(FRAG_L1 + mt) * 64 + l], xb, zero, 0, 0, 0);
+        h1[mt] = relu_bias_bf16(acc, w + MLP_B1 + mt * 16 + 4 * q);
+    }
+    for (int mt = 0; mt < 8; mt++) {
+        f32x4 acc = zero;
+        for (int kt = 0; kt < 8; kt++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag[(FRAG_L2 + mt * 8 + kt) * 64 + l], h1[kt], acc, 0, 0,
+                                                            0);
+        h2[mt] = relu_bias_bf16(acc, w + MLP_B2 + mt * 16 + 4 * q);
+    }
+    f32x4 acc = zero;
+    for (int kt = 0; kt < 8; kt++)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag[(FRAG_L3 + kt) * 64 + l], h2[kt], acc, 0, 0, 0);
+    return acc[0] + w[MLP_B3];
+}
+
+// One wave per candidate (grid-stride).  For every position i < d: features
+// of the pairs (i, j), j < d, in tiles of 16, the MLP, then the rows j < d
+// sorted by decreasing score (ties by increasing j) into order[c][i][0..d-1];
+// entries j >= d, and positions i >= d, hold the identity.  Optionally the
+// scores (score[c][i][j], 0 outside d x d).
+__global__ __launch_bounds__(256) void learned_order_kernel(uint32_t n, uint32_t stride,
+                                                            const uint64_t *__restrict__ rows,
+                                                            const uint32_t *__restrict__ meta, uint32_t od,
+                                                            const uint16_t *__restrict__ frag_g,
+                                                            const float *__restrict__ w, uint8_t *__restrict__ order,
+                                                            float *__restrict__ score) {
+    extern __shared__ uint32_t lds[];
+    bf16x4 *frag = (bf16x4 *)lds;                                       // FRAG_TILES * 64 * 8 bytes
+    uint32_t *geo = lds + FRAG_TILES * 64 * 2 + (threadIdx.x >> 6) * ISE_MAX_ROWS * 64;
+    {
+        const uint2 *src = (const uint2 *)frag_g;
+        uint2 *dst = (uint2 *)frag;
+        for (int t = threadIdx.x; t < FRAG_TILES * 64; t += blockDim.x) dst[t] = src[t];
+    }
+    __syncthreads();
+    Grp<64> g;
+    const int l = g.ln, q = l >> 4, r16 = l & 15;
+    const uint32_t wpb = blockDim.x >> 6;
+    for (uint32_t ci = blockIdx.x * wpb + (threadIdx.x >> 6); ci < n; ci += gridDim.x * wpb) {
+        const uint32_t m = meta[ci];
+        const int d = m & 0xff, b = (m >> 8) & 0xff;
+        uint8_t *oc = order + (uint64_t)ci * od * od;
+        float *sc = score ? score + (uint64_t)ci * od * od : nullptr;
+        const bool ok = d >= 1 && d <= (int)od && d <= ISE_MAX_ROWS && (uint32_t)d <= stride && b >= 1 && b <= 64;
+        // identity outside the d x d block (written once; the block is written below)
+        const uint32_t dd = ok ? (uint32_t)d : 0u;
+        for (uint32_t t = l; t < od * od; t += 64) {
+            if (t / od < dd && t % od < dd) continue;
+            oc[t] = (uint8_t)(t % od);
+            if (sc) sc[t] = 0.f;
+        }
+        if (!ok) continue;
+        const Staged c = stage(g, ci, m, rows, nullptr, stride, geo);
+        for (int i = 0; i < d; i++) {
+            float mine = 0.f;   // lane j < d: score of row j at position i
+            for (int jt = 0; jt * 16 < d; jt++) {
+                bf16x4 xb = {0, 0, 0, 0};
+                for (int jj = 0; jj < 16; jj++) {
+                    const int j = jt * 16 + jj;
+                    if (j >= d) break;
+                    const PairStats s = pair_stats(g, c, geo, i, j);
+                    if (r16 == jj)
+                        for (int t = 0; t < 4; t++) xb[t] = (short)f2bf(pair_feature(4 * q + t, s, i, j, d));
+                }
+                // an opaque zero offset keeps the weight fragments in LDS
+                // (loaded per tile) instead of hoisted into registers
+                int zero;
+                asm volatile("s_mov_b32 %0, 0" : "=s"(zero));
+                const float s16 = mlp_tile(xb, frag + zero, w, l);
+                // move the tile's scores (lanes 0..15) to lanes jt*16 .. jt*16+15
+                const float moved = __shfl(s16, (l - jt * 16) & 63);
+                if (l >= jt * 16 && l < jt * 16 + 16) mine = moved;
+            }
+            // stable descending rank among rows 0..d-1
+            int rank = 0;
+            for (int k = 0; k < d; k++) {
+                const float sk = __shfl(mine, k);
+                rank += (sk > mine) || (sk == mine && k < l);
+            }
+            if (l < d) {
+                oc[i * od + rank] = (uint8_t)l;
+                if (sc) sc[i * od + l] = mine;
+            }
+        }
+    }
+}
+
+size_t learned_order_lds(uint32_t wpb) {
+    return (size_t)FRAG_TILES * 64 * 8 + (size_t)wpb * ISE_MAX_ROWS * 64 * sizeof(uint32_t);
+}
+
+hipError_t learned_order(uint32_t blocks, uint32_t wpb, uint32_t n, uint32_t stride, const uint64_t *rows,
+                         const uint32_t *meta, uint32_t od, const uint16_t *frag, const float *w, uint8_t *order,
+                         float *score, hipStream_t s) {
+    if (wpb < 1 || wpb > 4) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(learned_order_kernel, dim3(blocks), dim3(64 * wpb), learned_order_lds(wpb), s, n, stride, rows,
+                       meta, od, frag, w, order, score);
+    return hipGetLastError();
+}
+
+hipError_t learned_order_occupancy(uint32_t wpb, int *blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void *)&learned_order_kernel,
+                                                        (int)(64 * wpb), learned_order_lds(wpb));
 }
 
 // Scores [n][od][od] -> order [n][od][od]: row i of a candidate lists the rows

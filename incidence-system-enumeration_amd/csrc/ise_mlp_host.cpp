@@ -1,6 +1,7 @@
 // ise_mlp_host.cpp -- the learned row-order model: parameters, Adam state,
 // activation workspace and the forward / backward / training / ordering
-// sequences built from the kernels of ise_mlp.hip.
+// sequences built from the kernels of ise_mlp.hip, plus the witness-log
+// labels it is trained on.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -12,18 +13,18 @@ using namespace ise;
 
 struct ise_mlp {
     ise_ctx *ctx = nullptr;
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr;   // the context's stream
     uint32_t max_rows = 0;
     int step = 0;
     float *w = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;   // [MLP_PARAMS]
     uint16_t *wb = nullptr;                                           // bf16 copy [MLP_PARAMS]
-    // workspace
+    uint16_t *frag = nullptr;                                         // A fragments [MLP_FRAG_WORDS]
+    // workspace of the unfused forward/backward
     float *h1f = nullptr, *h2f = nullptr, *z3 = nullptr, *dh2 = nullptr, *dh1 = nullptr, *lossv = nullptr;
     uint16_t *h1b = nullptr, *h2b = nullptr, *dz3 = nullptr, *dz2 = nullptr, *dz1 = nullptr;
     float *scalar = nullptr;
-    // ordering workspace
-    uint16_t *X = nullptr;
-    float *score = nullptr;
+    int order_blocks = 0;
+    uint32_t order_wpb = 4;
 };
 
 namespace {
@@ -37,7 +38,7 @@ int chk(hipError_t e) { return e == hipSuccess ? ISE_OK : ISE_ERR_HIP; }
 
 template <typename T> hipError_t dalloc(T **p, size_t count) { return hipMalloc((void **)p, count * sizeof(T) + 64); }
 
-// He-uniform initial weights from a fixed generator (splitmix64).
+// He-uniform initial weights from splitmix64(seed); biases zero.
 void init_params(std::vector<float> &p, uint64_t seed) {
     uint64_t st = seed;
     auto next = [&]() {
@@ -64,7 +65,7 @@ int forward(ise_mlp *m, const uint16_t *X, uint32_t M) {
     return ISE_OK;
 }
 
-// gradient of the loss into m->g; loss into m->scalar[0]
+// gradient of the loss into m->g (same packing as the parameters); loss into m->scalar[0]
 int backward(ise_mlp *m, const uint16_t *X, const float *y, const float *wt, uint32_t M, float inv_wsum) {
     const uint16_t *W2 = m->wb + MLP_W2, *W3 = m->wb + MLP_W3;
     float *g = m->g;
@@ -91,12 +92,41 @@ int backward(ise_mlp *m, const uint16_t *X, const float *y, const float *wt, uin
     return ISE_OK;
 }
 
-float weight_sum(ise_mlp *m, const float *wt, uint32_t M) {
+int weight_sum(ise_mlp *m, const float *wt, uint32_t M, float *ws) {
+    CK(sum(M, wt, m->scalar + 1, m->s));
+    CK(hipMemcpyAsync(ws, m->scalar + 1, sizeof(float), hipMemcpyDeviceToHost, m->s));
+    CK(hipStreamSynchronize(m->s));
+    return ISE_OK;
+}
+
+int gradient(ise_mlp *m, const uint16_t *X, const float *y, const float *wt, uint32_t M) {
+    if (!m || !X || !y || !wt || M == 0 || M % 16 || M > m->max_rows) return ISE_ERR_ARG;
     float ws = 0.f;
-    if (sum(M, wt, m->scalar + 1, m->s) != hipSuccess) return -1.f;
-    if (hipMemcpyAsync(&ws, m->scalar + 1, sizeof(float), hipMemcpyDeviceToHost, m->s) != hipSuccess) return -1.f;
-    if (hipStreamSynchronize(m->s) != hipSuccess) return -1.f;
-    return ws;
+    int rc = weight_sum(m, wt, M, &ws);
+    if (rc) return rc;
+    if (!(ws > 0.f)) return ISE_ERR_ARG;
+    rc = forward(m, X, M);
+    if (!rc) rc = backward(m, X, y, wt, M, 1.f / ws);
+    return rc;
+}
+
+int adam_step(ise_mlp *m, float lr) {
+    m->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f / (1.f - powf(b1, (float)m->step)), c2 = 1.f / (1.f - powf(b2, (float)m->step));
+    CK(adam(MLP_PARAMS, m->w, m->g, m->m1, m->m2, m->wb, lr, b1, b2, 1e-8f, c1, c2, m->s));
+    CK(frag_pack(m->wb, m->frag, m->s));
+    return ISE_OK;
+}
+
+int order_launch(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows, const uint32_t *d_meta, uint32_t od,
+                 uint8_t *d_order, float *d_score) {
+    uint32_t blocks = (uint32_t)m->order_blocks;
+    const uint32_t need = (n + m->order_wpb - 1) / m->order_wpb;
+    if (blocks > need) blocks = need;
+    if (blocks == 0) blocks = 1;
+    CK(learned_order(blocks, m->order_wpb, n, stride, d_rows, d_meta, od, m->frag, m->w, d_order, d_score, m->s));
+    return ISE_OK;
 }
 
 } // namespace
@@ -104,18 +134,28 @@ float weight_sum(ise_mlp *m, const float *wt, uint32_t M) {
 extern "C" {
 
 int ise_mlp_create(ise_ctx *ctx, ise_mlp **out, uint32_t max_rows, uint64_t seed) {
-    if (!ctx || !out || max_rows == 0) return ISE_ERR_ARG;
+    if (!ctx || !out || max_rows == 0 || max_rows > (1u << 26)) return ISE_ERR_ARG;
     *out = nullptr;
+    if (hipSetDevice(ctx_device(ctx)) != hipSuccess) return ISE_ERR_HIP;
     ise_mlp *m = new ise_mlp();
     m->ctx = ctx;
+    m->s = ctx_stream(ctx);
     m->max_rows = (max_rows + 15) / 16 * 16;
     const size_t R = m->max_rows;
-    hipError_t e = hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking);
+    // as many waves per workgroup (up to 4) as the per-workgroup LDS limit allows
+    int max_lds = 0;
+    hipError_t e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx_device(ctx));
+    m->order_wpb = 4;
+    while (m->order_wpb > 1 && learned_order_lds(m->order_wpb) > (size_t)max_lds) m->order_wpb--;
+    int occ = 0;
+    if (e == hipSuccess) e = learned_order_occupancy(m->order_wpb, &occ);
+    m->order_blocks = (occ > 0 ? occ : 1) * (ctx_num_cus(ctx) > 0 ? ctx_num_cus(ctx) : 1);
     if (e == hipSuccess) e = dalloc(&m->w, MLP_PARAMS);
     if (e == hipSuccess) e = dalloc(&m->g, MLP_PARAMS);
     if (e == hipSuccess) e = dalloc(&m->m1, MLP_PARAMS);
     if (e == hipSuccess) e = dalloc(&m->m2, MLP_PARAMS);
     if (e == hipSuccess) e = dalloc(&m->wb, MLP_PARAMS);
+    if (e == hipSuccess) e = dalloc(&m->frag, MLP_FRAG_WORDS);
     if (e == hipSuccess) e = dalloc(&m->h1f, R * MLP_HID);
     if (e == hipSuccess) e = dalloc(&m->h2f, R * MLP_HID);
     if (e == hipSuccess) e = dalloc(&m->dh1, R * MLP_HID);
@@ -128,8 +168,6 @@ int ise_mlp_create(ise_ctx *ctx, ise_mlp **out, uint32_t max_rows, uint64_t seed
     if (e == hipSuccess) e = dalloc(&m->dz3, R * MLP_OUT_PAD);
     if (e == hipSuccess) e = dalloc(&m->lossv, R);
     if (e == hipSuccess) e = dalloc(&m->scalar, 16);
-    if (e == hipSuccess) e = dalloc(&m->X, R * MLP_IN);
-    if (e == hipSuccess) e = dalloc(&m->score, R);
     if (e != hipSuccess) {
         ise_mlp_destroy(m);
         return ISE_ERR_HIP;
@@ -148,11 +186,10 @@ int ise_mlp_create(ise_ctx *ctx, ise_mlp **out, uint32_t max_rows, uint64_t seed
 int ise_mlp_destroy(ise_mlp *m) {
     if (!m) return ISE_ERR_ARG;
     if (m->s) (void)hipStreamSynchronize(m->s);
-    void *bufs[] = {m->w, m->g, m->m1, m->m2, m->wb, m->h1f, m->h2f, m->dh1, m->dh2, m->h1b, m->h2b,
-                    m->dz2, m->dz1, m->z3, m->dz3, m->lossv, m->scalar, m->X, m->score};
+    void *bufs[] = {m->w, m->g, m->m1, m->m2, m->wb, m->frag, m->h1f, m->h2f, m->dh1, m->dh2,
+                    m->h1b, m->h2b, m->dz2, m->dz1, m->z3, m->dz3, m->lossv, m->scalar};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    if (m->s) (void)hipStreamDestroy(m->s);
     delete m;
     return ISE_OK;
 }
@@ -163,6 +200,7 @@ int ise_mlp_set_params(ise_mlp *m, const float *p) {
     CK(hipMemsetAsync(m->m1, 0, MLP_PARAMS * sizeof(float), m->s));
     CK(hipMemsetAsync(m->m2, 0, MLP_PARAMS * sizeof(float), m->s));
     CK(to_bf16(MLP_PARAMS, m->w, m->wb, m->s));
+    CK(frag_pack(m->wb, m->frag, m->s));
     CK(hipStreamSynchronize(m->s));
     m->step = 0;
     return ISE_OK;
@@ -179,9 +217,9 @@ int ise_mlp_features(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *
                      uint32_t od, uint16_t *d_X) {
     if (!ctx || !d_rows || !d_meta || !d_X || od < 1 || od > ISE_MAX_ROWS || stride < 1) return ISE_ERR_ARG;
     if (n == 0) return ISE_OK;
-    hipStream_t s = nullptr;
-    CK(features(n, stride, d_rows, d_meta, od, d_X, s));
-    CK(hipStreamSynchronize(s));
+    CK(hipSetDevice(ctx_device(ctx)));
+    CK(features(n, stride, d_rows, d_meta, od, d_X, ctx_stream(ctx)));
+    CK(hipStreamSynchronize(ctx_stream(ctx)));
     return ISE_OK;
 }
 
@@ -197,11 +235,7 @@ int ise_mlp_forward(ise_mlp *m, const uint16_t *X, uint32_t M, float *d_score) {
 
 int ise_mlp_gradient(ise_mlp *m, const uint16_t *X, const float *y, const float *wt, uint32_t M, float *host_grad,
                      float *loss) {
-    if (!m || !X || !y || !wt || M == 0 || M % 16 || M > m->max_rows) return ISE_ERR_ARG;
-    const float ws = weight_sum(m, wt, M);
-    if (!(ws > 0.f)) return ISE_ERR_ARG;
-    int rc = forward(m, X, M);
-    if (!rc) rc = backward(m, X, y, wt, M, 1.f / ws);
+    int rc = gradient(m, X, y, wt, M);
     if (rc) return rc;
     if (host_grad) CK(hipMemcpyAsync(host_grad, m->g, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, m->s));
     float l = 0.f;
@@ -213,35 +247,97 @@ int ise_mlp_gradient(ise_mlp *m, const uint16_t *X, const float *y, const float 
 
 int ise_mlp_train_step(ise_mlp *m, const uint16_t *X, const float *y, const float *wt, uint32_t M, float lr,
                        float *loss) {
-    float l = 0.f;
-    int rc = ise_mlp_gradient(m, X, y, wt, M, nullptr, &l);
+    if (!(lr > 0.f)) return ISE_ERR_ARG;
+    int rc = gradient(m, X, y, wt, M);
+    if (!rc) rc = adam_step(m, lr);
     if (rc) return rc;
-    m->step++;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.f / (1.f - powf(b1, (float)m->step)), c2 = 1.f / (1.f - powf(b2, (float)m->step));
-    CK(adam(MLP_PARAMS, m->w, m->g, m->m1, m->m2, m->wb, lr, b1, b2, 1e-8f, c1, c2, m->s));
+    float l = 0.f;
+    CK(hipMemcpyAsync(&l, m->scalar, sizeof(float), hipMemcpyDeviceToHost, m->s));
     CK(hipStreamSynchronize(m->s));
     if (loss) *loss = l;
     return ISE_OK;
 }
 
-int ise_mlp_order(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows, const uint32_t *d_meta,
-                  uint32_t od, uint8_t *d_order) {
-    if (!m || !d_rows || !d_meta || !d_order || od < 1 || od > ISE_MAX_ROWS) return ISE_ERR_ARG;
-    const uint32_t pairs = od * od;
-    const uint32_t per = m->max_rows / pairs;   // candidates per chunk
-    if (per == 0 || (pairs % 16 && per * pairs % 16)) return ISE_ERR_ARG;
-    for (uint32_t c0 = 0; c0 < n; c0 += per) {
-        uint32_t cnt = n - c0 < per ? n - c0 : per;
-        uint32_t M = cnt * pairs;
-        const uint32_t Mp = (M + 15) / 16 * 16;
-        if (Mp != M) CK(hipMemsetAsync(m->X + (size_t)M * MLP_IN, 0, (size_t)(Mp - M) * MLP_IN * 2, m->s));
-        CK(features(cnt, stride, d_rows + (size_t)c0 * stride, d_meta + c0, od, m->X, m->s));
-        int rc = forward(m, m->X, Mp);
+int ise_mlp_fit(ise_mlp *m, const uint16_t *X, const float *y, const float *wt, uint64_t M_total, uint32_t batch,
+                uint32_t steps, float lr, float *loss_curve) {
+    if (!m || !X || !y || !wt || batch == 0 || batch % 16 || batch > m->max_rows || M_total < batch || !(lr > 0.f))
+        return ISE_ERR_ARG;
+    const uint64_t nb = M_total / batch;
+    for (uint32_t s = 0; s < steps; s++) {
+        const uint64_t off = (uint64_t)(s % nb) * batch;
+        int rc = gradient(m, X + off * MLP_IN, y + off, wt + off, batch);
+        if (rc == ISE_ERR_ARG) continue;   // a batch without weight: nothing to learn from it
+        if (!rc) rc = adam_step(m, lr);
         if (rc) return rc;
-        CK(order_from_scores(cnt, d_meta + c0, od, m->z3, MLP_OUT_PAD, d_order + (size_t)c0 * pairs, m->s));
+        if (loss_curve) CK(hipMemcpyAsync(&loss_curve[s], m->scalar, sizeof(float), hipMemcpyDeviceToHost, m->s));
     }
     CK(hipStreamSynchronize(m->s));
+    return ISE_OK;
+}
+
+int ise_mlp_order(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows, const uint32_t *d_meta,
+                  uint32_t od, uint8_t *d_order, float *d_score) {
+    if (!m || !d_rows || !d_meta || !d_order || od < 1 || od > ISE_MAX_ROWS || stride < 1) return ISE_ERR_ARG;
+    if (n == 0) return ISE_OK;
+    int rc = order_launch(m, n, stride, d_rows, d_meta, od, d_order, d_score);
+    if (rc) return rc;
+    CK(hipStreamSynchronize(m->s));
+    return ISE_OK;
+}
+
+int ise_mlp_check_batch_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                                     const uint32_t *d_meta, uint32_t od, uint8_t *d_order, uint32_t hunt_budget,
+                                     const ise_results *d_out, uint32_t *d_counter, uint32_t repeats, float *ms_out,
+                                     float *order_ms_out) {
+    if (!m || !d_rows || !d_meta || !d_order || !d_out || !ms_out || repeats == 0 || hunt_budget == 0 || od < 1 ||
+        od > ISE_MAX_ROWS)
+        return ISE_ERR_ARG;
+    ise_ctx *c = m->ctx;
+    int pmode;
+    uint32_t pbudget, pdim;
+    const uint8_t *porder;
+    ctx_get_order(c, &pmode, &pbudget, &porder, &pdim);
+    int rc = ise_set_order(c, ISE_ORDER_LEARNED, hunt_budget, d_order, od);
+    if (rc) return rc;
+    std::vector<hipEvent_t> ev(2 * repeats + 1, nullptr);
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = ISE_ERR_HIP;
+    if (!rc && hipEventRecord(ev[0], m->s) != hipSuccess) rc = ISE_ERR_HIP;
+    for (uint32_t i = 0; i < repeats && !rc; i++) {
+        rc = order_launch(m, n, stride, d_rows, d_meta, od, d_order, nullptr);
+        if (!rc && hipEventRecord(ev[2 * i + 1], m->s) != hipSuccess) rc = ISE_ERR_HIP;
+        if (!rc) rc = ise_check_batch_device(c, n, stride, d_rows, nullptr, d_meta, d_out, d_counter, nullptr);
+        if (!rc && hipEventRecord(ev[2 * i + 2], m->s) != hipSuccess) rc = ISE_ERR_HIP;
+    }
+    if (!rc && hipStreamSynchronize(m->s) != hipSuccess) rc = ISE_ERR_HIP;
+    for (uint32_t i = 0; i < repeats && !rc; i++) {
+        if (hipEventElapsedTime(&ms_out[i], ev[2 * i], ev[2 * i + 2]) != hipSuccess) rc = ISE_ERR_HIP;
+        if (!rc && order_ms_out && hipEventElapsedTime(&order_ms_out[i], ev[2 * i], ev[2 * i + 1]) != hipSuccess)
+            rc = ISE_ERR_HIP;
+    }
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    const int rc2 = ise_set_order(c, pmode, pbudget, porder, pdim);
+    return rc ? rc : rc2;
+}
+
+int ise_mlp_witness_labels(uint32_t n, uint32_t od, const uint32_t *meta, const uint8_t *verdict,
+                           const uint8_t *fail_row, const uint8_t *perm_row, float *y, float *wt) {
+    if (!meta || !verdict || !fail_row || !perm_row || !y || !wt || od < 1 || od > ISE_MAX_ROWS) return ISE_ERR_ARG;
+    for (uint32_t c = 0; c < n; c++) {
+        const uint32_t d = meta[c] & 0xff;
+        const uint32_t f = fail_row[c];
+        float *yc = y + (size_t)c * od * od, *wc = wt + (size_t)c * od * od;
+        memset(yc, 0, sizeof(float) * od * od);
+        memset(wc, 0, sizeof(float) * od * od);
+        if (verdict[c] != ISE_VERDICT_NONCANONICAL || d > od || f >= d) continue;
+        const uint8_t *w = perm_row + (size_t)c * ISE_MAX_ROWS;
+        for (uint32_t i = 0; i <= f; i++)
+            for (uint32_t j = 0; j < d; j++) {
+                yc[i * od + j] = w[i] == j ? 1.f : 0.f;
+                wc[i * od + j] = 1.f;
+            }
+    }
     return ISE_OK;
 }
 

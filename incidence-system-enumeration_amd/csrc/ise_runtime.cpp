@@ -140,6 +140,16 @@ int launch(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const 
 namespace ise {
 uint32_t ctx_stride(const ise_ctx *c) { return c->stride; }
 uint32_t ctx_capacity(const ise_ctx *c) { return c->capacity; }
+hipStream_t ctx_stream(const ise_ctx *c) { return c->stream; }
+int ctx_device(const ise_ctx *c) { return c->device; }
+int ctx_num_cus(const ise_ctx *c) { return c->num_cus; }
+void ctx_set_error(ise_ctx *c, const char *msg) { c->err = msg; }
+void ctx_get_order(const ise_ctx *c, int *mode, uint32_t *budget, const uint8_t **order, uint32_t *dim) {
+    *mode = c->order_mode;
+    *budget = c->hunt_budget;
+    *order = c->d_order;
+    *dim = c->order_dim;
+}
 } // namespace ise
 
 extern "C" {

@@ -437,7 +437,8 @@ ISE_DEV bool hunt(const Grp<G> &g, const Staged &c, const uint32_t *geo, const u
         while (k < d) {
             const int cand = order ? (int)order[i * od + k] : k;
             k++;
-            if (!((used >> cand) & 1)) { j = cand; break; }
+            // entries that name no row of this candidate are skipped
+            if (cand < d && !((used >> cand) & 1)) { j = cand; break; }
         }
         if (l == i) cursor = k;
         if (j < 0) {           // depth exhausted
@@ -494,8 +495,10 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const 
     Verdict v{1, EMAX, EMAX, EMAX, 1ull, 0u, 0u, 0u, 0u, 0u};
     bool found = false;
     if constexpr (HUNT) {
-        found = hunt<G, COUNT>(g, c, geo, order ? order + (uint64_t)ci * od * od : nullptr, od, hunt_budget, pr,
-                               pc, v);
+        // an order table narrower than the candidate cannot guide it: plain search only
+        if (!order || d <= od)
+            found = hunt<G, COUNT>(g, c, geo, order ? order + (uint64_t)ci * od * od : nullptr, od, hunt_budget,
+                                   pr, pc, v);
         if (!found) { pr = l; pc = l; }
     }
     if (!found)

@@ -89,7 +89,24 @@ def lib():
         L.ise_enumerate_bibd_candidates.argtypes = [P, I, I, I, I, I, U32, P, P, ctypes.c_long,
                                                     ctypes.POINTER(ctypes.c_long)]
         L.ise_enumerate_bibd_candidates.restype = ctypes.c_long
+        L.ise_enumerate_bibd_sampled.argtypes = [P, I, I, I, U32, U64, I, I, U32, P, P, ctypes.c_long,
+                                                 ctypes.POINTER(ctypes.c_long), P]
+        L.ise_enumerate_bibd_sampled.restype = ctypes.c_long
         L.ise_random_extensions.argtypes = [I, I, I, U64, I, I, U64, U32, P, P]
+        F = ctypes.c_float
+        L.ise_mlp_create.argtypes = [P, ctypes.POINTER(P), U32, U64]
+        L.ise_mlp_destroy.argtypes = [P]
+        L.ise_mlp_get_params.argtypes = [P, P]
+        L.ise_mlp_set_params.argtypes = [P, P]
+        L.ise_mlp_features.argtypes = [P, U32, U32, P, P, U32, P]
+        L.ise_mlp_forward.argtypes = [P, P, U32, P]
+        L.ise_mlp_gradient.argtypes = [P, P, P, P, U32, P, ctypes.POINTER(F)]
+        L.ise_mlp_train_step.argtypes = [P, P, P, P, U32, F, ctypes.POINTER(F)]
+        L.ise_mlp_fit.argtypes = [P, P, P, P, U64, U32, U32, F, P]
+        L.ise_mlp_order.argtypes = [P, U32, U32, P, P, U32, P, P]
+        L.ise_mlp_check_batch_device_timed.argtypes = [P, U32, U32, P, P, U32, P, U32,
+                                                       ctypes.POINTER(ResultsStruct), P, U32, P, P]
+        L.ise_mlp_witness_labels.argtypes = [U32, U32, P, P, P, P, P, P]
         L.ise_kf_check.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
         L.ise_kf_check_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P, P]
         L.ise_kf_check_batch_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P,
@@ -346,6 +363,119 @@ class Context:
         if rc < 0:
             self._check(int(rc))
         return rows[:nout.value], meta[:nout.value]
+
+
+    def enumerate_sampled(self, v, k, lam, beam, seed, dmin, dmax, stride=None, cap=1 << 21):
+        """Beam-sampled level enumeration (ise_enumerate_bibd_sampled):
+        (cand rows, cand meta, candidates tested per depth)."""
+        stride = stride or self.stride
+        rows = np.zeros((cap, stride), np.uint64)
+        meta = np.zeros(cap, np.uint32)
+        lc = np.zeros(v + 1, np.uint64)
+        nout = ctypes.c_long(0)
+        rc = self._L.ise_enumerate_bibd_sampled(self.h, v, k, lam, beam, seed, dmin, dmax, stride, _ptr(rows),
+                                                _ptr(meta), cap, ctypes.byref(nout), _ptr(lc))
+        if rc < 0:
+            self._check(int(rc))
+        return rows[:nout.value], meta[:nout.value], lc
+
+
+MLP_FEATURES = 16
+MLP_PARAMS = 20752
+
+
+class RowOrderModel:
+    """The learned row-order scorer (include/ise.h, ise_mlp_*): a
+    16 -> 128 -> 128 -> 1 ReLU network with bf16 weights and FP32
+    accumulation on MFMA, trained on the GPU from plain-search witness logs."""
+
+    def __init__(self, ctx, max_rows=1 << 16, seed=1):
+        self.ctx = ctx
+        self._L = ctx._L
+        h = ctypes.c_void_p()
+        ctx._check(self._L.ise_mlp_create(ctx.h, ctypes.byref(h), max_rows, seed))
+        self.h = h.value
+        self.max_rows = (max_rows + 15) // 16 * 16
+
+    def close(self):
+        if self.h:
+            self._L.ise_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        self.ctx._check(self._L.ise_mlp_get_params(self.h, p.ctypes.data))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.shape != (MLP_PARAMS,):
+            raise ValueError(f"expected {MLP_PARAMS} parameters")
+        self.ctx._check(self._L.ise_mlp_set_params(self.h, p.ctypes.data))
+
+    def features(self, n, stride, d_rows, d_meta, od, d_X):
+        self.ctx._check(self._L.ise_mlp_features(self.ctx.h, n, stride, d_rows.ptr, d_meta.ptr, od, d_X.ptr))
+
+    def forward(self, d_X, M, d_score):
+        self.ctx._check(self._L.ise_mlp_forward(self.h, d_X.ptr, M, d_score.ptr))
+
+    def gradient(self, d_X, d_y, d_w, M):
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = ctypes.c_float(0)
+        self.ctx._check(self._L.ise_mlp_gradient(self.h, d_X.ptr, d_y.ptr, d_w.ptr, M, g.ctypes.data,
+                                                 ctypes.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, d_X, d_y, d_w, M, lr):
+        loss = ctypes.c_float(0)
+        self.ctx._check(self._L.ise_mlp_train_step(self.h, d_X.ptr, d_y.ptr, d_w.ptr, M, lr, ctypes.byref(loss)))
+        return loss.value
+
+    def fit(self, d_X, d_y, d_w, M_total, batch, steps, lr):
+        curve = np.zeros(steps, np.float32)
+        self.ctx._check(self._L.ise_mlp_fit(self.h, d_X.ptr, d_y.ptr, d_w.ptr, M_total, batch, steps, lr,
+                                            curve.ctypes.data))
+        return curve
+
+    def order(self, n, stride, d_rows, d_meta, od, d_order, d_score=None):
+        self.ctx._check(self._L.ise_mlp_order(self.h, n, stride, d_rows.ptr, d_meta.ptr, od, d_order.ptr,
+                                              None if d_score is None else d_score.ptr))
+
+    def time_check(self, n, stride, d_rows, d_meta, od, d_order, hunt_budget, res_struct, d_head, repeats):
+        """Per repeat: ordering kernel + learned-order check; (total ms, ordering ms)."""
+        ms = np.zeros(repeats, np.float32)
+        oms = np.zeros(repeats, np.float32)
+        self.ctx._check(self._L.ise_mlp_check_batch_device_timed(
+            self.h, n, stride, d_rows.ptr, d_meta.ptr, od, d_order.ptr, hunt_budget, ctypes.byref(res_struct),
+            d_head.ptr, repeats, ms.ctypes.data, oms.ctypes.data))
+        return ms, oms
+
+
+def witness_labels(meta, verdict, fail_row, perm_row, od):
+    """Training labels from a plain-search witness log (ise_mlp_witness_labels):
+    y, w of shape [n][od][od]."""
+    meta = np.ascontiguousarray(meta, dtype=np.uint32)
+    verdict = np.ascontiguousarray(verdict, dtype=np.uint8)
+    fail_row = np.ascontiguousarray(fail_row, dtype=np.uint8)
+    perm_row = np.ascontiguousarray(perm_row, dtype=np.uint8)
+    n = len(meta)
+    if perm_row.shape != (n, MAX_ROWS):
+        raise ValueError("perm_row must be [n][32]")
+    y = np.zeros((n, od, od), np.float32)
+    w = np.zeros((n, od, od), np.float32)
+    rc = lib().ise_mlp_witness_labels(n, od, _ptr(meta), _ptr(verdict), _ptr(fail_row), _ptr(perm_row),
+                                      _ptr(y), _ptr(w))
+    if rc != OK:
+        raise ISEError(rc, "ise_mlp_witness_labels")
+    return y, w
 
 
 class DeviceArray:

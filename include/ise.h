@@ -251,12 +251,35 @@ int ise_mlp_gradient(ise_mlp *m, const uint16_t *d_X, const float *d_y, const fl
 int ise_mlp_train_step(ise_mlp *m, const uint16_t *d_X, const float *d_y, const float *d_w, uint32_t M,
                        float lr, float *loss);
 
-/* Features + scores + ordering for a device batch: d_order[c][i][0..od-1]
- * lists the rows of candidate c by decreasing score for position i (ties by
- * increasing row), the input of ISE_ORDER_LEARNED.  Works in chunks that fit
- * the model's workspace. */
+/* ise_mlp_train_step repeated `steps` times over consecutive batches of
+ * `batch` rows (wrapping around M_total; batches with zero total weight are
+ * skipped); loss_curve[s] (optional, host) receives each step's loss. */
+int ise_mlp_fit(ise_mlp *m, const uint16_t *d_X, const float *d_y, const float *d_w, uint64_t M_total,
+                uint32_t batch, uint32_t steps, float lr, float *loss_curve);
+
+/* Row order for a device batch, features, network and sort fused in one
+ * kernel: d_order[c][i][0..d-1] lists the rows of candidate c by decreasing
+ * score for position i (ties by increasing row); entries past the depth, and
+ * positions i >= depth, hold the identity.  This is the input of
+ * ISE_ORDER_LEARNED.  d_score (optional) receives the scores [c][i][j]. */
 int ise_mlp_order(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows, const uint32_t *d_meta,
-                  uint32_t od, uint8_t *d_order);
+                  uint32_t od, uint8_t *d_order, float *d_score);
+
+/* Timed learned-order checking: per repeat, the ordering kernel then the
+ * canonicity kernel in ISE_ORDER_LEARNED mode with `hunt_budget`, on the
+ * context's stream; ms_out[r] = both, order_ms_out[r] (optional) = the
+ * ordering alone.  The context's previous order mode is restored. */
+int ise_mlp_check_batch_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                                     const uint32_t *d_meta, uint32_t od, uint8_t *d_order, uint32_t hunt_budget,
+                                     const ise_results *d_out, uint32_t *d_counter, uint32_t repeats, float *ms_out,
+                                     float *order_ms_out);
+
+/* Witness-log labels (host arrays): for every non-canonical candidate and
+ * every position i <= fail_row, pair (i, j), j < depth, gets label 1 when the
+ * plain search's witness put row j at position i, else 0, and weight 1; all
+ * other pairs weight 0.  y and w are [n][od][od]. */
+int ise_mlp_witness_labels(uint32_t n, uint32_t od, const uint32_t *meta, const uint8_t *verdict,
+                           const uint8_t *fail_row, const uint8_t *perm_row, float *y, float *w);
 
 /* ---- 1-factorizations of K_n (groups of two players) --------------------
  * Wire format: rounds[cand][stride] uint64, nibble p of word i = player at
@@ -323,6 +346,19 @@ long ise_enumerate_bibd_candidates(ise_ctx *ctx, int v, int k, int lambda,
                                    int dmin, int dmax, uint32_t stride,
                                    uint64_t *rows_out, uint32_t *meta_out,
                                    long cap, long *n_out);
+
+/* Beam-sampled enumeration for parameter sets too large to enumerate in
+ * full: level by level as above, but when more than `beam` (>= 1) candidates
+ * of a level are canonical only `beam` of them are extended, chosen by a
+ * partial Fisher-Yates shuffle of their indices with splitmix64 from `seed`
+ * (t = 0..beam-1: swap idx[t] with idx[t + next() % (count - t)]) and kept in
+ * level order; the generator state carries over between levels.  Candidates
+ * at depth in [dmin,dmax] are copied out as above; level_counts (optional,
+ * [v+1]) receives the candidates tested per depth.  Returns the number of
+ * candidates written or a status code. */
+long ise_enumerate_bibd_sampled(ise_ctx *ctx, int v, int k, int lambda, uint32_t beam, uint64_t seed,
+                                int dmin, int dmax, uint32_t stride, uint64_t *rows_out,
+                                uint32_t *meta_out, long cap, long *n_out, uint64_t *level_counts);
 
 /* Seeded random valid extensions: each new row takes r columns uniformly
  * among those whose weight is below k and that keep every pairwise row

@@ -88,6 +88,16 @@ long orc_enumerate_bibd(int v, int k, int lambda, uint64_t *designs,
                         uint64_t *cand_rows, uint32_t *cand_meta, long cand_cap,
                         int stride, int dmin, int dmax);
 
+/* Beam-sampled level-by-level enumeration: every child of the kept canonical
+ * candidates of the previous level is tested (and written out when its depth
+ * is in [dmin,dmax], up to cand_cap); at most `beam` canonical candidates per
+ * level are kept (seeded partial Fisher-Yates with splitmix64, indices then
+ * sorted; beam 0 keeps all).  level_counts[d] = candidates at depth d.
+ * Returns the number of candidates written, <0 on error. */
+long orc_enumerate_sampled(int v, int k, int lambda, uint32_t beam, uint64_t seed, int dmin, int dmax,
+                           int stride, uint64_t *cand_rows, uint32_t *cand_meta, long cand_cap,
+                           uint64_t *level_counts);
+
 /* Brute-force reference for tiny matrices: the lexicographically largest
  * matrix over all row permutations (columns re-sorted in decreasing order)
  * and the number of row permutations that map the matrix to itself. */
@@ -108,6 +118,19 @@ int orc_hunt(const uint8_t *mat, int d, int b, const uint8_t *order, uint32_t bu
 int orc_check_batch_hunt(uint32_t n, uint32_t stride, const uint64_t *rows, const uint64_t *cells,
                          const uint32_t *meta, const uint8_t *order, uint32_t order_dim,
                          uint32_t budget, uint32_t gen_cap, orc_results *res);
+
+/* ---- learned row-order scorer (ise_oracle_mlp.c) ---- */
+/* 16 features of every (position i, row j) pair: X[(c*od+i)*od+j][16],
+ * zero outside depth x depth. */
+int orc_mlp_features(uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta,
+                     uint32_t od, float *X);
+/* Logits of M feature rows (params packed as the library's ISE_MLP_PARAMS);
+ * bf16_rounding != 0 rounds at the GPU path's points. */
+int orc_mlp_forward(const float *params, const float *X, uint32_t M, int bf16_rounding, double *z);
+/* Weighted mean binary cross-entropy of the logits against y and its
+ * gradient (packed like the parameters). */
+int orc_mlp_gradient(const float *params, const float *X, const float *y, const float *w, uint32_t M,
+                     int bf16_rounding, double *grad, double *loss);
 
 /* ---- k-factorizations of K_n (n <= 16): rounds of n/2 pairs ---- */
 /* Canonicity of a partial 1-factorization (nrounds x n bytes, each round the

@@ -483,6 +483,9 @@ typedef struct {
     uint64_t *stats;
     uint64_t *cand_rows; uint32_t *cand_meta; long cand_cap; int stride, dmin, dmax;
     u8 mat[ORC_MAX_ROWS * ORC_MAX_COLS];
+    /* when set, solutions of the row system are appended here instead of
+     * being recursed into (level-by-level enumeration) */
+    uint64_t *collect; long ncollect, collect_cap;
 } enum_t;
 
 /* orbit data of the row about to be built */
@@ -531,6 +534,11 @@ static void solve_orbit(enum_t *e, int depth, const orbs_t *ob, int o, int *x,
         for (int p = 0; p < ob->n; p++)
             if (x[p]) row |= (((x[p] >= 64) ? ~0ull : ((1ull << x[p]) - 1)) << ob->start[p]);
         e->rows[depth] = row;
+        if (e->collect) {
+            if (e->ncollect < e->collect_cap) e->collect[e->ncollect] = row;
+            e->ncollect++;
+            return;
+        }
         dfs_row(e, depth + 1);
         return;
     }
@@ -551,6 +559,8 @@ static void solve_orbit(enum_t *e, int depth, const orbs_t *ob, int o, int *x,
     }
 }
 
+static void expand(enum_t *e, int depth);
+
 static void dfs_row(enum_t *e, int depth)
 {
     if (depth > 0) {
@@ -569,6 +579,12 @@ static void dfs_row(enum_t *e, int depth)
         }
         e->stats[1]++;
     }
+    expand(e, depth);
+}
+
+/* Solve the system of row `depth` over the column orbits of rows 0..depth-1. */
+static void expand(enum_t *e, int depth)
+{
     /* orbits of row `depth` */
     orbs_t ob;
     uint64_t masks[ORC_MAX_ROWS + 1];
@@ -635,6 +651,109 @@ long orc_enumerate_bibd(int v, int k, int lambda, uint64_t *designs,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Beam-sampled level-by-level enumeration.                                  */
+/*
+ * Level d holds the children (row-system solutions, in solver order) of the
+ * canonical candidates kept at level d-1; every child is tested and those in
+ * [dmin, dmax] are written out.  When more than `beam` candidates of a level
+ * are canonical, `beam` of them are kept: a partial Fisher-Yates shuffle of
+ * their indices driven by splitmix64 from `seed` (t = 0..beam-1: swap idx[t]
+ * with idx[t + next() % (count - t)]), the chosen indices then sorted.  The
+ * generator state carries over from level to level.  beam == 0 keeps all.
+ */
+static uint64_t splitmix_next(uint64_t *s)
+{
+    uint64_t z = (*s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+static int cmp_u32(const void *a, const void *b)
+{
+    uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+long orc_enumerate_sampled(int v, int k, int lambda, uint32_t beam, uint64_t seed, int dmin, int dmax,
+                           int stride, uint64_t *cand_rows, uint32_t *cand_meta, long cand_cap,
+                           uint64_t *level_counts)
+{
+    if (v < 2 || k < 2 || k > v || lambda < 1 || stride < v) return -1;
+    if ((lambda * (v - 1)) % (k - 1)) return -2;
+    int r = lambda * (v - 1) / (k - 1);
+    if ((v * r) % k) return -2;
+    int b = v * r / k;
+    if (v > ORC_MAX_ROWS || b > ORC_MAX_COLS) return -3;
+    enum_t *e = (enum_t *)calloc(1, sizeof(enum_t));
+    e->v = v; e->b = b; e->r = r; e->k = k; e->lambda = lambda;
+    uint64_t st[5] = {0, 0, 0, 0, 0};
+    e->stats = st;
+    size_t cap_par = 1024, npar = 1;
+    uint64_t *par = (uint64_t *)calloc(cap_par * ORC_MAX_ROWS, sizeof(uint64_t));
+    size_t cap_kid = 1 << 16;
+    uint64_t *kid = (uint64_t *)malloc(cap_kid * ORC_MAX_ROWS * sizeof(uint64_t));
+    uint64_t rng = seed, sols[4096];
+    long nout = 0;
+    for (int depth = 0; depth < v && npar; depth++) {
+        /* children of every kept parent, in parent order */
+        size_t nkid = 0;
+        for (size_t p = 0; p < npar; p++) {
+            memcpy(e->rows, par + p * ORC_MAX_ROWS, sizeof(uint64_t) * ORC_MAX_ROWS);
+            e->collect = sols; e->ncollect = 0; e->collect_cap = 4096;
+            expand(e, depth);
+            e->collect = NULL;
+            if (e->ncollect > 4096) { free(par); free(kid); free(e); return -4; }
+            for (long q = 0; q < e->ncollect; q++) {
+                if (nkid == cap_kid) {
+                    cap_kid *= 2;
+                    kid = (uint64_t *)realloc(kid, cap_kid * ORC_MAX_ROWS * sizeof(uint64_t));
+                }
+                uint64_t *dst = kid + nkid * ORC_MAX_ROWS;
+                memcpy(dst, par + p * ORC_MAX_ROWS, sizeof(uint64_t) * ORC_MAX_ROWS);
+                dst[depth] = sols[q];
+                nkid++;
+            }
+        }
+        const int nd = depth + 1;
+        if (level_counts) level_counts[nd] = nkid;
+        /* test every child; keep the canonical ones */
+        uint32_t *canon = (uint32_t *)malloc((nkid + 1) * sizeof(uint32_t));
+        size_t ncan = 0;
+        for (size_t c = 0; c < nkid; c++) {
+            const uint64_t *src = kid + c * ORC_MAX_ROWS;
+            if (cand_rows && nd >= dmin && nd <= dmax && nout < cand_cap) {
+                memset(cand_rows + (size_t)nout * stride, 0, sizeof(uint64_t) * stride);
+                memcpy(cand_rows + (size_t)nout * stride, src, sizeof(uint64_t) * nd);
+                unsigned oi = nd == v ? (ORC_SAVE_ROW_PERMUTATIONS | ORC_SAVE_ROW_TO_CHANGE) : 0;
+                cand_meta[nout] = (uint32_t)nd | ((uint32_t)b << 8) | (oi << 24);
+                nout++;
+            }
+            memcpy(e->rows, src, sizeof(uint64_t) * ORC_MAX_ROWS);
+            if (check_partial(e, nd, NULL)) canon[ncan++] = (uint32_t)c;
+        }
+        if (beam && ncan > beam) {
+            for (uint32_t t = 0; t < beam; t++) {
+                size_t j = t + (size_t)(splitmix_next(&rng) % (uint64_t)(ncan - t));
+                uint32_t x = canon[t]; canon[t] = canon[j]; canon[j] = x;
+            }
+            ncan = beam;
+            qsort(canon, ncan, sizeof(uint32_t), cmp_u32);
+        }
+        if (ncan > cap_par) {
+            cap_par = ncan;
+            par = (uint64_t *)realloc(par, cap_par * ORC_MAX_ROWS * sizeof(uint64_t));
+        }
+        for (size_t q = 0; q < ncan; q++)
+            memcpy(par + q * ORC_MAX_ROWS, kid + (size_t)canon[q] * ORC_MAX_ROWS, sizeof(uint64_t) * ORC_MAX_ROWS);
+        npar = nd < v ? ncan : 0;
+        free(canon);
+    }
+    free(par); free(kid); free(e);
+    return nout;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Witness hunt in a proposed order (the learned-order search).              */
 /*
  * Depth-first search over row prefixes in which position i tries the rows in
@@ -666,7 +785,10 @@ static int hunt_core(const cand_t *cd, const uint8_t *order, uint32_t budget, u8
             if (idlen > i) idlen = i;
             continue;
         }
-        while (cursor[i] < d && ((used >> (order ? order[i * d + cursor[i]] : cursor[i])) & 1))
+        /* entries naming no row of this candidate (>= d) are skipped */
+        while (cursor[i] < d && (order ? (order[i * d + cursor[i]] >= d ||
+                                          ((used >> order[i * d + cursor[i]]) & 1))
+                                       : ((used >> cursor[i]) & 1)))
             cursor[i]++;
         if (cursor[i] == d) {
             if (i == 0) break;
@@ -749,7 +871,8 @@ int orc_check_batch_hunt(uint32_t n, uint32_t stride, const uint64_t *rows, cons
         int d = m & 0xff, b = (m >> 8) & 0xff;
         if (d < 1 || d > ORC_MAX_ROWS || d > (int)stride || b < 1 || b > ORC_MAX_COLS)
             return -1;
-        if (order && d > (int)order_dim) return -1;
+        /* a table narrower than the candidate cannot guide it: plain search only */
+        const int use_hunt = !order || d <= (int)order_dim;
         const uint64_t *rp = rows + (size_t)c * stride;
         cd.nrows = d; cd.ncols = b;
         cd.len_stab = (m >> 16) & 0xff;
@@ -761,13 +884,13 @@ int orc_check_batch_hunt(uint32_t n, uint32_t stride, const uint64_t *rows, cons
         build_orbits(&cd, cells ? cells + (size_t)c * stride : NULL, rp);
         /* the order table of a candidate is order_dim x order_dim; compact it to d x d */
         u8 ord[ORC_MAX_ROWS * ORC_MAX_ROWS];
-        if (order)
+        if (order && use_hunt)
             for (int i = 0; i < d; i++)
                 for (int k = 0; k < d; k++)
                     ord[i * d + k] = order[(size_t)c * order_dim * order_dim + (size_t)i * order_dim + k];
         uint32_t cnt[4] = {0, 0, 0, 0};
         u8 perm[ORC_MAX_ROWS], pcol[ORC_MAX_COLS], frow = 255, fcol = 255;
-        if (hunt_core(&cd, order ? ord : NULL, budget, perm, pcol, &frow, &fcol, cnt)) {
+        if (use_hunt && hunt_core(&cd, order ? ord : NULL, budget, perm, pcol, &frow, &fcol, cnt)) {
             if (res->verdict) res->verdict[c] = 0;
             if (res->fail_row) res->fail_row[c] = frow;
             if (res->fail_col) res->fail_col[c] = fcol;

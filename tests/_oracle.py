@@ -35,6 +35,10 @@ def load():
         L.orc_enumerate_bibd.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_long,
                                          _P, _P, _P, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_enumerate_bibd.restype = ctypes.c_long
+        L.orc_enumerate_sampled.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                            ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P,
+                                            ctypes.c_long, _P]
+        L.orc_enumerate_sampled.restype = ctypes.c_long
         L.orc_bruteforce_canon.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P]
         L.orc_check_batch_hunt.argtypes = [U32, U32, _P, _P, _P, _P, U32, U32, U32, _P]
         L.orc_hunt.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, U32, _P, _P, _P]
@@ -43,6 +47,9 @@ def load():
         L.orc_kf_bruteforce.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
         L.orc_kf_quickperm.argtypes = [ctypes.c_int, _P]
         L.orc_kf_quickperm.restype = ctypes.c_long
+        L.orc_mlp_features.argtypes = [U32, U32, _P, _P, U32, _P]
+        L.orc_mlp_forward.argtypes = [_P, _P, U32, ctypes.c_int, _P]
+        L.orc_mlp_gradient.argtypes = [_P, _P, _P, _P, U32, ctypes.c_int, _P, _P]
         _lib = Oracle(L)
     return _lib
 
@@ -115,6 +122,46 @@ class Oracle:
             c = int(st[4])
             cands = (crow[:c], cmeta[:c])
         return des[:m], auts[:m], st, cands
+
+    def enumerate_sampled(self, v, k, lam, beam, seed, dmin, dmax, stride=32, cap=1 << 20):
+        """Beam-sampled level enumeration: (cand rows, cand meta, level counts)."""
+        crow = np.zeros((cap, stride), np.uint64)
+        cmeta = np.zeros(cap, np.uint32)
+        lc = np.zeros(33, np.uint64)
+        n = self.L.orc_enumerate_sampled(v, k, lam, beam, ctypes.c_uint64(seed), dmin, dmax, stride,
+                                         crow.ctypes.data, cmeta.ctypes.data, cap, lc.ctypes.data)
+        if n < 0:
+            raise ValueError(f"oracle sampled enumeration failed ({n})")
+        return crow[:n], cmeta[:n], lc[:v + 1]
+
+    # ---- learned row-order scorer ----
+    def mlp_features(self, rows, meta, od):
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32)
+        n, stride = rows.shape
+        X = np.zeros((n, od, od, 16), np.float32)
+        self.L.orc_mlp_features(n, stride, rows.ctypes.data, meta.ctypes.data, od, X.ctypes.data)
+        return X
+
+    def mlp_forward(self, params, X, bf16=True):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 16)
+        z = np.zeros(len(X), np.float64)
+        self.L.orc_mlp_forward(params.ctypes.data, X.ctypes.data, len(X), int(bf16), z.ctypes.data)
+        return z
+
+    def mlp_gradient(self, params, X, y, w, bf16=True):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 16)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        g = np.zeros(len(params), np.float64)
+        loss = np.zeros(1, np.float64)
+        rc = self.L.orc_mlp_gradient(params.ctypes.data, X.ctypes.data, y.ctypes.data, w.ctypes.data, len(X),
+                                     int(bf16), g.ctypes.data, loss.ctypes.data)
+        if rc:
+            raise ValueError("zero total weight")
+        return g, float(loss[0])
 
     def test_canonicity(self, mat, len_stab=0, out_info=0, gen_cap=0):
         import ise_amd._abi as abi
