@@ -1,13 +1,26 @@
-"""Throughput of the batched canonicity test on one MI355X.
+"""Throughput of the batched canonicity test on one MI355X, plain and learned
+row order.
 
-Workload (one "step" = one pass of the HIP canonicity kernel over one batch
-already resident in HBM): the 2-(15,3,1) candidate stream, i.e. every partial
-matrix the row-by-row enumerator tests at depths 4..15 plus a stress batch of
-2^20 seeded random valid extensions (depth 4..15), rows bit-packed as uint64
-words.  Each candidate gets its verdict, witness row permutation, failing row,
+Workload (one "step" = one pass of the hot path over one batch already
+resident in HBM): the 2-(25,4,1) candidate stream at row depths 8..16 --
+every candidate a beam-sampled run of the row-by-row enumerator tests there
+(beam 64, seed 0x25041) followed by 2^20 seeded random valid extensions
+(seed 0x25041) -- rows bit-packed as uint64 words (50 columns).  Each
+candidate gets its verdict, witness row permutation, failing row,
 rowToChange and |Aut|.
 
-Prints one JSON line (metric, value, roofline, cpu_baseline, ...).
+  * plain order: one launch of the canonicity kernel per step;
+  * learned order: per step, the fused scoring kernel (pair features ->
+    16x128x128x1 bf16 MFMA network -> per-position row order) and then the
+    canonicity kernel with the witness hunt in that order.  Verdicts are
+    checked equal to the plain order's.
+
+`value` is the plain-order rate (the library's default order and, as
+measured, the faster one); the learned-order rate, both mean
+permutations-to-rejection, the 1-factorization kernel on K_12 and a full
+enumeration's designs/s are reported beside it.
+
+Prints one JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -23,19 +36,22 @@ sys.path.insert(0, os.path.join(ROOT, "incidence-system-enumeration_amd"))
 
 METRIC = "candidates canon-checked/sec on 1 MI355X, learned-order vs plain; bit-exact"
 HBM_PEAK_GBS = 8000.0
-STRIDE = 16            # v = 15 rows fit in 16 words
-SEED = 0x15031
+STRIDE = 16            # depth <= 16 rows per candidate
+SEED = 0x25041
+BEAM = 64
+DMIN, DMAX = 8, 16
 RESULT_BYTES = 1 + 1 + 1 + 8 + 32   # verdict, fail_row, row_to_change, |Aut|, witness row perm
+PROFILE = os.path.join(ROOT, "profiles", "check_kernel64_2-25-4-1_stream_traffic.txt")
 
 
 def build_batch(n_random, path):
-    """Enumerator partials (the enumerator checks each level on the GPU) plus
-    random extensions, written to `path`.  Runs in a child process so that the
-    measured process launches nothing but the timed batches."""
+    """Beam-sampled enumerator candidates (each level checked on the GPU)
+    plus random extensions, written to `path`.  Runs in a child process so
+    that the measured process launches nothing but the timed batches."""
     import ise_amd
     with ise_amd.Context(capacity=1 << 16, stride=32) as ctx:
-        rows_e, meta_e = ctx.enumerator_candidates(15, 3, 1, 4, 15, stride=STRIDE, cap=1 << 20)
-    rows_r, meta_r = ise_amd.random_extensions(15, 3, 1, n_random, 4, 15, SEED, stride=STRIDE)
+        rows_e, meta_e, _ = ctx.enumerate_sampled(25, 4, 1, BEAM, SEED, DMIN, DMAX, stride=STRIDE, cap=1 << 21)
+    rows_r, meta_r = ise_amd.random_extensions(25, 4, 1, n_random, DMIN, DMAX, SEED, stride=STRIDE)
     np.savez(path, rows=np.concatenate([rows_e, rows_r]), meta=np.concatenate([meta_e, meta_r]),
              n_enum=np.array([len(meta_e)]))
 
@@ -43,7 +59,7 @@ def build_batch(n_random, path):
 def load_batch(n_random):
     import subprocess
     import tempfile
-    path = os.path.join(tempfile.gettempdir(), f"ise_bench_15_3_1_{n_random}_{SEED:x}.npz")
+    path = os.path.join(tempfile.gettempdir(), f"ise_bench_25_4_1_{n_random}_{SEED:x}_b{BEAM}.npz")
     if not os.path.exists(path):
         subprocess.run([sys.executable, os.path.abspath(__file__), "--build-batch", path,
                         "--random", str(n_random)], check=True)
@@ -53,14 +69,12 @@ def load_batch(n_random):
 
 def cpu_baseline(rows, meta, seconds=15.0):
     """The oracle (a scalar port of the reference's checker), one thread, on a
-    bounded leading sample of the same batch."""
+    bounded sample of the same batch in seeded random order."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle
     orc = _oracle.load()
-    step = 4096
+    step = 2048
     done, t0 = 0, time.perf_counter()
-    # the batch in a seeded random order (so both parts of the stream are in
-    # the sample) until the time budget is spent
     idx = np.random.default_rng(0).permutation(len(meta))
     while done < len(idx) and time.perf_counter() - t0 < seconds:
         sel = idx[done:done + step]
@@ -73,15 +87,52 @@ def cpu_baseline(rows, meta, seconds=15.0):
 
 
 def profiled_launch_figures():
-    """HBM bytes and instruction counts of one launch of this workload, as
-    measured by rocprofv3 PMC runs and committed under profiles/."""
-    path = os.path.join(ROOT, "profiles", "check_kernel64_2-15-3-1_stream_traffic.txt")
+    """HBM bytes and instruction counts of one plain launch of this workload,
+    as measured by rocprofv3 PMC runs and committed under profiles/."""
     out = {}
-    if os.path.exists(path):
-        for line in open(path):
+    if os.path.exists(PROFILE):
+        for line in open(PROFILE):
             if ":" in line and not line.startswith("#"):
                 k, v = line.split(":", 1)
                 out[k.strip()] = float(v)
+    return out
+
+
+def kfact_rate(ctx, steps, warmup):
+    """1-factorizations of K_12: 2^18 seeded partial factorizations of 3-10
+    rounds, plain relabelling order, one launch per step."""
+    import ise_amd
+    rows, meta = ise_amd.kf_random(12, 3, 10, 1 << 18, 0x12, stride=16)
+    n = len(meta)
+    d_rows = ctx.device_array(rows.shape, np.uint64, rows)
+    d_meta = ctx.device_array(meta.shape, np.uint32, meta)
+    d_head = ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
+    dev, res = ctx.kf_device_results(n)
+    if warmup:
+        ctx.kf_time_batch_device(n, 16, d_rows, d_meta, res, d_head, warmup)
+    ms = float(np.mean(ctx.kf_time_batch_device(n, 16, d_rows, d_meta, res, d_head, steps)))
+    verdict = dev["verdict"].to_host()
+    tried = dev["n_tried"].to_host()
+    for b in list(dev.values()) + [d_rows, d_meta, d_head]:
+        b.free()
+    return {"workload": "K_12 partial 1-factorizations, 3-10 rounds, 2^18 seeded (seed 0x12), plain order",
+            "candidates": n, "ms_per_step": ms, "candidates_per_s": n / (ms / 1e3),
+            "canonical": int((verdict == 1).sum()), "relabellings_per_s": float(tried.sum()) / (ms / 1e3)}
+
+
+def enumeration_rate():
+    """Full enumeration of 2-(13,3,1) and 2-(15,3,1) (host row solver + GPU
+    canonicity per level): designs and candidates per second, wall clock."""
+    import ise_amd
+    out = {}
+    with ise_amd.Context(capacity=1 << 16, stride=32) as ctx:
+        for v, k, lam in ((13, 3, 1), (15, 3, 1)):
+            ctx.enumerate_bibd(v, k, lam)          # warm-up (graphs, allocations)
+            t0 = time.perf_counter()
+            des, _, st, _ = ctx.enumerate_bibd(v, k, lam)
+            dt = time.perf_counter() - t0
+            out[f"2-({v},{k},{lam})"] = {"designs": len(des), "seconds": dt, "designs_per_s": len(des) / dt,
+                                         "candidates_per_s": st["candidates"] / dt}
     return out
 
 
@@ -93,9 +144,10 @@ def main():
     ap.add_argument("--random", type=int, default=1 << 20)
     ap.add_argument("--group-lanes", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=4)
+    ap.add_argument("--hunt-budget", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--order", choices=["plain", "hunt"], default="plain")
-    ap.add_argument("--hunt-budget", type=int, default=64)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the K_12 and enumeration figures")
     ap.add_argument("--build-batch", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run one device-to-device copy of this many bytes (counter calibration)")
@@ -112,6 +164,7 @@ def main():
     t_setup = time.time()
     rows, meta, n_enum = load_batch(args.random)
     import ise_amd
+    from ise_amd import learned
 
     ctx = ise_amd.Context(capacity=1 << 10, stride=32)
     n = len(meta)
@@ -126,9 +179,8 @@ def main():
     fields = ("verdict", "fail_row", "row_to_change", "group_order", "perm_row")
     out, res = ctx.device_results(n, fields)
     ctx.set_launch(group_lanes=args.group_lanes, waves_per_block=args.waves_per_block)
-    if args.order == "hunt":
-        ctx.set_order(ise_amd.ORDER_HUNT, args.hunt_budget)
 
+    # ---- plain order: the timed steps
     if args.warmup:
         ctx.time_batch_device(n, STRIDE, d_rows, d_meta, res, d_head, args.warmup)
     t0 = time.perf_counter()
@@ -136,10 +188,11 @@ def main():
     wall = time.perf_counter() - t0
     ms = float(np.mean(per_step_ms))
     value = n / (ms / 1e3)
-
     verdict = out["verdict"].to_host()
+    print(f"bench: plain {ms:.2f} ms/step", file=sys.stderr, flush=True)
+
     # statistics from one extra, untimed launch with the counters compiled in
-    cout, cres = ctx.device_results(n, fields + ("counters",))
+    cout, cres = ctx.device_results(n, ("verdict", "counters"))
     ctx.check_batch_device(n, STRIDE, d_rows, d_meta, cres, d_head)
     counters = cout["counters"].to_host()
     if not np.array_equal(cout["verdict"].to_host(), verdict):
@@ -147,12 +200,39 @@ def main():
     if (verdict > 1).any():
         raise RuntimeError("search limit hit or invalid record in the bench batch")
     rej = verdict == 0
-    perms_to_reject = float(counters[rej, 0].mean()) if rej.any() else 0.0
 
-    # algorithmic HBM bytes of one launch: candidate rows + meta in, result record out
+    # ---- learned order: scoring kernel + hunt-then-plain kernel per step
+    learned_line = None
+    if not args.no_learned:
+        model = learned.load(ctx)
+        d_order = ctx.device_array((n, STRIDE, STRIDE), np.uint8)
+        if args.warmup:
+            model.time_check(n, STRIDE, d_rows, d_meta, STRIDE, d_order, args.hunt_budget, res, d_head,
+                             args.warmup)
+        lms, oms = model.time_check(n, STRIDE, d_rows, d_meta, STRIDE, d_order, args.hunt_budget, res, d_head,
+                                    args.steps)
+        if not np.array_equal(out["verdict"].to_host(), verdict):
+            raise RuntimeError("the learned order changed a verdict")
+        model.time_check(n, STRIDE, d_rows, d_meta, STRIDE, d_order, args.hunt_budget, cres, d_head, 1)
+        lcount = cout["counters"].to_host()
+        if not np.array_equal(cout["verdict"].to_host(), verdict):
+            raise RuntimeError("the learned order changed a verdict (counted launch)")
+        lmean = float(np.mean(lms))
+        learned_line = {
+            "value": n / (lmean / 1e3), "ms_per_step": lmean, "scoring_ms_per_step": float(np.mean(oms)),
+            "hunt_budget": args.hunt_budget, "weights": "incidence-system-enumeration_amd/ise_amd/"
+                                                        "row_order_weights.npy",
+            "mean_perms_to_reject": float(lcount[rej, 0].mean()),
+            "mean_rows_to_reject": float(lcount[rej, 1].mean()),
+            "verdicts_equal_to_plain": True,
+        }
+        print(f"bench: learned {lmean:.2f} ms/step (scoring {np.mean(oms):.2f})", file=sys.stderr, flush=True)
+        d_order.free()
+        model.close()
+
+    # algorithmic HBM bytes of one plain launch: candidate rows + meta in, result record out
     hbm_bytes = float((depth * 8 + 4).sum() + n * RESULT_BYTES)
     achieved_gbs = hbm_bytes / (ms / 1e3) / 1e9
-    rows_compared = float(counters[:, 1].sum())
     prof = profiled_launch_figures()
 
     line = {
@@ -167,10 +247,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
-        "data": "synthetic (seeded enumerator partials + random valid extensions)",
+        "data": "synthetic (seeded beam-sampled enumerator candidates + random valid extensions)",
         "config": {
-            "workload": "2-(15,3,1) candidate stream: enumerator partials at depth 4-15 + 2^20 seeded "
-                        "random valid extensions (depth 4-15), plain order",
+            "workload": "2-(25,4,1) candidate stream at depth 8-16: beam-sampled enumerator candidates "
+                        "(beam 64) + 2^20 seeded random valid extensions; value = plain order",
             "candidates": n, "enumerator_candidates": n_enum, "random_candidates": n - n_enum,
             "seed": SEED, "order": "plain", "group_lanes": ctx.get_launch()["group_lanes"] or 64,
             "waves_per_block": args.waves_per_block,
@@ -180,29 +260,35 @@ def main():
             "frac": achieved_gbs / HBM_PEAK_GBS,
             "traffic": (prof["fetch_bytes"] + prof["write_bytes"]) if prof else None,
         },
-        # the binding resource: instruction issue (rocprofv3 SQ counters of one launch)
+        # the binding resource: instruction issue (rocprofv3 SQ counters of one plain launch)
         "issue": ({"valu_frac": prof["valu_insts"] / prof["cu_cycles"],
                    "salu_frac": prof["salu_insts"] / prof["cu_cycles"],
                    "peak": "1 wave64 VALU + 1 SALU instruction per CU per cycle",
-                   "source": "profiles/check_kernel64_2-15-3-1_stream.md"} if prof else None),
+                   "source": "profiles/check_kernel64_2-25-4-1_stream.md"} if prof else None),
         "canonical": int((verdict == 1).sum()),
         "noncanonical": int(rej.sum()),
-        "mean_perms_to_reject_plain": perms_to_reject,
-        "rows_compared_per_s": rows_compared / (ms / 1e3),
+        "mean_perms_to_reject_plain": float(counters[rej, 0].mean()) if rej.any() else 0.0,
+        "mean_rows_to_reject_plain": float(counters[rej, 1].mean()) if rej.any() else 0.0,
+        "learned": learned_line,
+        "learned_over_plain": (learned_line["value"] / value) if learned_line else None,
+        "rows_compared_per_s": float(counters[:, 1].sum()) / (ms / 1e3),
         "wall_s_timed": wall,
+        "ms_per_step_min": float(np.min(per_step_ms)),
     }
-    line["ms_per_step_min"] = float(np.min(per_step_ms))
     if args.copy_bytes:
         src = ctx.device_array((args.copy_bytes,), np.uint8)
         dst = ctx.device_array((args.copy_bytes,), np.uint8)
         ctx._check(ctx._L.ise_copy_on_device(ctx.h, dst.ptr, src.ptr, args.copy_bytes))
         src.free()
         dst.free()
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(rows, meta)
     for buf in list(out.values()) + list(cout.values()) + [d_rows, d_meta, d_head]:
         buf.free()
+    if not args.no_extra:
+        line["kfact_k12"] = kfact_rate(ctx, max(3, args.steps // 4), 1)
+        line["enumeration"] = enumeration_rate()
     ctx.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(rows, meta)
     print(json.dumps(line), flush=True)
 
 

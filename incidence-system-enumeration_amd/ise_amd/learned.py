@@ -96,3 +96,91 @@ def load(ctx, path=WEIGHTS):
         raise ValueError("stored parameters have the wrong size")
     model.params = p
     return model
+
+
+def c5_stream(ctx, n_random=1 << 20, beam=64, seed=0x25041, dmin=8, dmax=16):
+    """The 2-(25,4,1) stream at depths 8-16: beam-sampled enumerator output
+    followed by seeded random valid extensions."""
+    er, em, _ = ctx.enumerate_sampled(25, 4, 1, beam, seed, dmin, dmax, stride=dmax, cap=1 << 21)
+    rr, rm = random_extensions(25, 4, 1, n_random, dmin, dmax, seed, stride=dmax)
+    return np.concatenate([er, rr]), np.concatenate([em, rm]), len(em)
+
+
+def compare_orders(ctx, model, rows, meta, od, budgets=(8, 32, 128), repeats=3):
+    """Plain vs learned order on one device batch: verdicts (must agree),
+    mean permutations and rows compared before rejection (from a counting
+    launch), and kernel times (from launches without counters)."""
+    n, stride = rows.shape
+    d_rows = ctx.device_array(rows.shape, np.uint64, rows)
+    d_meta = ctx.device_array(meta.shape, np.uint32, meta)
+    d_head = ctx.device_array((256,), np.uint32)
+    tdev, tres = ctx.device_results(n, ("verdict",))
+    cdev, cres = ctx.device_results(n, ("verdict", "counters"))
+    ms_plain = ctx.time_batch_device(n, stride, d_rows, d_meta, tres, d_head, repeats)
+    ctx.time_batch_device(n, stride, d_rows, d_meta, cres, d_head, 1)
+    verdict = cdev["verdict"].to_host()
+    counters = cdev["counters"].to_host()
+    if not np.array_equal(tdev["verdict"].to_host(), verdict):
+        raise AssertionError("counting changed a verdict")
+    rej = verdict == 0
+    out = {"n": int(n), "noncanonical": int(rej.sum()), "plain_ms": float(np.min(ms_plain)),
+           "plain_mean_perms_to_reject": float(counters[rej, 0].mean()),
+           "plain_mean_rows_to_reject": float(counters[rej, 1].mean()),
+           "learned": []}
+    d_order = ctx.device_array((n, od, od), np.uint8)
+    for budget in budgets:
+        ms, oms = model.time_check(n, stride, d_rows, d_meta, od, d_order, budget, tres, d_head, repeats)
+        model.time_check(n, stride, d_rows, d_meta, od, d_order, budget, cres, d_head, 1)
+        got_v, got_c = cdev["verdict"].to_host(), cdev["counters"].to_host()
+        if not (np.array_equal(got_v, verdict) and np.array_equal(tdev["verdict"].to_host(), verdict)):
+            raise AssertionError("learned order changed a verdict")
+        out["learned"].append({
+            "budget": int(budget), "ms": float(np.min(ms)), "order_ms": float(np.min(oms)),
+            "mean_perms_to_reject": float(got_c[rej, 0].mean()),
+            "mean_rows_to_reject": float(got_c[rej, 1].mean()),
+            "rows_total_ratio": float(counters[:, 1].sum() / max(got_c[:, 1].sum(), 1))})
+    return out
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import time
+    from . import Context
+    ap = argparse.ArgumentParser(description="train the row-order network on plain witness logs and "
+                                             "compare learned with plain order on 2-(25,4,1)")
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--report", default=None)
+    ap.add_argument("--per-set", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--eval-random", type=int, default=1 << 16)
+    ap.add_argument("--beam", type=int, default=64)
+    args = ap.parse_args(argv)
+    ctx = Context(capacity=1 << 18, stride=32)
+    t0 = time.time()
+    model = RowOrderModel(ctx, max_rows=args.batch, seed=1)
+    stream = training_stream(ctx, per_set=args.per_set, seed=1)
+    d_X, d_y, d_w, M = pair_dataset(ctx, model, *stream, od=19, seed=2)
+    t1 = time.time()
+    curve = model.fit(d_X, d_y, d_w, M, args.batch, args.steps, args.lr)
+    t2 = time.time()
+    np.save(args.out, model.params)
+    print(f"pairs {M}, loss {curve[:50].mean():.4f} -> {curve[-50:].mean():.4f}; data {t1 - t0:.1f}s, "
+          f"training {t2 - t1:.1f}s", flush=True)
+    rows, meta, n_enum = c5_stream(ctx, n_random=args.eval_random, beam=args.beam)
+    report = {"training": {"pairs": int(M), "steps": args.steps, "batch": args.batch, "lr": args.lr,
+                           "loss_first50": float(curve[:50].mean()), "loss_last50": float(curve[-50:].mean()),
+                           "seconds": t2 - t1}}
+    report["enumerator"] = compare_orders(ctx, model, rows[:n_enum], meta[:n_enum], 16)
+    print(json.dumps(report["enumerator"]), flush=True)
+    report["random"] = compare_orders(ctx, model, rows[n_enum:], meta[n_enum:], 16)
+    print(json.dumps(report["random"]), flush=True)
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(report, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
