@@ -107,6 +107,9 @@ def lib():
         L.ise_mlp_check_batch_device_timed.argtypes = [P, U32, U32, P, P, U32, P, U32,
                                                        ctypes.POINTER(ResultsStruct), P, U32, P, P]
         L.ise_mlp_witness_labels.argtypes = [U32, U32, P, P, P, P, P, P]
+        L.ise_group_closure.argtypes = [P, U32, U32, P, U64, ctypes.POINTER(U64)]
+        L.ise_adjust_generators.argtypes = [P, U32, U32, P, U32, P]
+        L.ise_solution_orbits.argtypes = [P, P, U64, U32, P, U32, P, P]
         L.ise_kf_check.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
         L.ise_kf_check_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P, P]
         L.ise_kf_check_batch_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P,
@@ -247,6 +250,18 @@ class Context:
             return alloc_results(0, gen_cap, fields)
         out = {k: np.concatenate([r[k] for r in res_all]) for k in res_all[0]}
         return out
+
+    def solution_orbits(self, group, sols):
+        """Canonical (largest) image of every solution under the group, on the
+        GPU, and the index of the first listed solution equal to it
+        (CPermutStorage::findSolutionIndex)."""
+        group = np.ascontiguousarray(group, dtype=np.uint8)
+        sols = np.ascontiguousarray(sols, dtype=np.uint8)
+        canon = np.zeros_like(sols)
+        rep = np.zeros(len(sols), np.uint32)
+        self._check(self._L.ise_solution_orbits(self.h, _ptr(group), len(group), group.shape[1], _ptr(sols),
+                                                len(sols), _ptr(canon), _ptr(rep)))
+        return canon, rep
 
     # ---- device-resident batches ------------------------------------------
     def device_array(self, shape, dtype, host=None):
@@ -457,6 +472,32 @@ class RowOrderModel:
             self.h, n, stride, d_rows.ptr, d_meta.ptr, od, d_order.ptr, hunt_budget, ctypes.byref(res_struct),
             d_head.ptr, repeats, ms.ctypes.data, oms.ctypes.data))
         return ms, oms
+
+
+def group_closure(gens, cap=1 << 20):
+    """All elements of the group generated by gens [n][len] (uint8
+    permutations), identity first (CPermutStorage::constructGroup)."""
+    gens = np.ascontiguousarray(gens, dtype=np.uint8)
+    n, length = gens.shape
+    out = np.zeros((cap, length), np.uint8)
+    order = ctypes.c_uint64(0)
+    rc = lib().ise_group_closure(_ptr(gens) if n else None, n, length, _ptr(out), cap, ctypes.byref(order))
+    if rc != OK:
+        raise ISEError(rc, f"ise_group_closure (order >= {order.value})")
+    return out[:order.value]
+
+
+def adjust_generators(gens, idx):
+    """Generators restricted to the column orbits idx (CPermutStorage::adjustGenerators)."""
+    gens = np.ascontiguousarray(gens, dtype=np.uint8)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    n, length = gens.shape
+    out = np.zeros((n, len(idx)), np.uint8)
+    rc = lib().ise_adjust_generators(_ptr(gens) if n else None, n, length, _ptr(idx), len(idx),
+                                     _ptr(out) if n else None)
+    if rc != OK:
+        raise ISEError(rc, "ise_adjust_generators")
+    return out
 
 
 def witness_labels(meta, verdict, fail_row, perm_row, od):

@@ -324,6 +324,33 @@ int ise_kf_check_batch_device_timed(ise_ctx *ctx, uint32_t n, uint32_t stride,
 int ise_kf_random(int players, int rmin, int rmax, uint64_t count, uint64_t seed,
                   uint32_t stride, uint64_t *rounds_out, uint32_t *meta_out);
 
+/* ---- the automorphism group after the test ------------------------------
+ * Generators returned by the test (ise_results.gens: column-orbit
+ * permutations of the next row, or row permutations with
+ * ISE_SAVE_ROW_PERMUTATIONS) as byte permutations of length len. */
+
+/* Closure of the generators (CPermutStorage::constructGroup,
+ * Source/PermutStorage.h:183-225): every group element, identity first, into
+ * elements[cap][len]; *order_out = |G|.  ISE_ERR_CAPACITY (with *order_out
+ * >= cap + 1) when the group has more than cap elements. */
+int ise_group_closure(const uint8_t *gens, uint32_t n_gens, uint32_t len, uint8_t *elements, uint64_t cap,
+                      uint64_t *order_out);
+
+/* Generators restricted to the column orbits idx[0..len_idx-1]
+ * (CPermutStorage::adjustGenerators, Source/PermutStorage.h:155-181):
+ * out[g][j] = position in idx of gens[g][idx[j]].  ISE_ERR_ARG if the subset
+ * is not invariant. */
+int ise_adjust_generators(const uint8_t *gens, uint32_t n_gens, uint32_t len, const uint32_t *idx,
+                          uint32_t len_idx, uint8_t *out);
+
+/* Orbits of row solutions under a group (CPermutStorage::findSolutionIndex,
+ * Source/PermutStorage.h:281-325), on the GPU: canon_out[s] = the
+ * lexicographically largest image of solution s (image[i] = sol[g[i]] over
+ * every element g); rep_out[s] (optional) = the first listed solution equal
+ * to that image, or 0xffffffff if none is listed.  len <= 64. */
+int ise_solution_orbits(ise_ctx *ctx, const uint8_t *group, uint64_t order, uint32_t len, const uint8_t *sols,
+                        uint32_t n_sol, uint8_t *canon_out, uint32_t *rep_out);
+
 /* ---- host-side candidate producers (stay on the CPU) -------------------- */
 
 /* Level-synchronous enumeration of 2-(v,k,lambda) designs: every level's

@@ -679,7 +679,7 @@ long orc_enumerate_sampled(int v, int k, int lambda, uint32_t beam, uint64_t see
                            int stride, uint64_t *cand_rows, uint32_t *cand_meta, long cand_cap,
                            uint64_t *level_counts)
 {
-    if (v < 2 || k < 2 || k > v || lambda < 1 || stride < v) return -1;
+    if (v < 2 || k < 2 || k > v || lambda < 1 || dmin < 1 || dmax < dmin || stride < dmax) return -1;
     if ((lambda * (v - 1)) % (k - 1)) return -2;
     int r = lambda * (v - 1) / (k - 1);
     if ((v * r) % k) return -2;

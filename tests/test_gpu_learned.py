@@ -193,3 +193,58 @@ def test_sampled_enumerator_matches_oracle(gpu_ctx, oracle, v, k, lam, beam, see
     assert np.array_equal(meta, m2)
     assert np.array_equal(rows, r2)
     assert np.array_equal(lc[1:], lc2[1:v + 1])
+
+
+def _hunt_success(oracle, rows, meta, tables, budget, idx):
+    """Fraction of candidates idx whose witness the hunt finds within budget
+    (tables None = increasing row order)."""
+    found = 0
+    for c in idx:
+        d, b = int(meta[c]) & 0xff, (int(meta[c]) >> 8) & 0xff
+        mat = np.ascontiguousarray(unpack(rows[c, :d], b))
+        o = None if tables is None else np.ascontiguousarray(tables[c, :d, :d])
+        perm = np.zeros(32, np.uint8)
+        fr = np.zeros(1, np.uint8)
+        nodes = np.zeros(1, np.uint32)
+        found += oracle.L.orc_hunt(mat.ctypes.data, d, b, None if o is None else o.ctypes.data, budget,
+                                   perm.ctypes.data, fr.ctypes.data, nodes.ctypes.data)
+    return found / len(idx)
+
+
+def test_trained_network_gain(gpu_ctx, oracle):
+    """What the trained network gains: against an untrained network of the
+    same shape, and against the method without it (plain order; increasing
+    row order for the hunt), on 2-(25,4,1) candidates it never saw."""
+    print("case trained row-order network vs untrained / plain on 2-(25,4,1)", flush=True)
+    from ise_amd import learned
+    rows, meta, _ = gpu_ctx.enumerate_sampled(25, 4, 1, 16, 5, 8, 16, stride=16, cap=1 << 16)
+    r2, m2 = ise_amd.random_extensions(25, 4, 1, 2000, 8, 16, 9, stride=16)
+    rows, meta = np.concatenate([rows, r2]), np.concatenate([meta, m2])
+    n, od, budget = len(meta), 16, 8
+    d_rows = gpu_ctx.device_array(rows.shape, np.uint64, rows)
+    d_meta = gpu_ctx.device_array(meta.shape, np.uint32, meta)
+    d_head = gpu_ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
+    dev, res = gpu_ctx.device_results(n, ("verdict", "counters"))
+    gpu_ctx.check_batch_device(n, 16, d_rows, d_meta, res, d_head)
+    verdict, plain = dev["verdict"].to_host(), dev["counters"].to_host()
+    rej = np.nonzero(verdict == 0)[0]
+    assert len(rej) > 10000
+    out = {}
+    for name, model in (("trained", learned.load(gpu_ctx)), ("untrained", ise_amd.RowOrderModel(gpu_ctx, 4096, 3))):
+        d_order = gpu_ctx.device_array((n, od, od), np.uint8)
+        model.time_check(n, 16, d_rows, d_meta, od, d_order, budget, res, d_head, 1)
+        assert np.array_equal(dev["verdict"].to_host(), verdict), name
+        out[name] = (dev["counters"].to_host(), d_order.to_host())
+        model.close()
+    sample = rej[:: max(1, len(rej) // 3000)]
+    succ = {name: _hunt_success(oracle, rows, meta, out[name][1], budget, sample) for name in out}
+    succ["increasing"] = _hunt_success(oracle, rows, meta, None, budget, sample)
+    perms = {"plain": plain[rej, 0].mean(), **{k: v[0][rej, 0].mean() for k, v in out.items()}}
+    print(f"hunt success (budget {budget}): " + ", ".join(f"{k} {v:.3f}" for k, v in succ.items()), flush=True)
+    print("mean permutations before rejection: " + ", ".join(f"{k} {v:.2f}" for k, v in perms.items()), flush=True)
+    # training teaches the network where witnesses are ...
+    assert succ["trained"] > succ["untrained"] + 0.3
+    assert succ["trained"] >= succ["increasing"] - 0.02
+    # ... but the plain order already rejects those candidates as fast: the
+    # learned order is no faster than plain here, and must not be much slower
+    assert perms["trained"] <= 1.1 * perms["plain"]
