@@ -86,6 +86,47 @@ def cpu_baseline(rows, meta, seconds=15.0):
                        f"thread, {dt:.1f} s")
 
 
+def cpu_all_cores(rows, meta, seconds=15.0):
+    """The same oracle on every host core (one thread per core, chunks of the
+    same seeded random order; the oracle releases the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+    orc = _oracle.load()
+    cores = min(os.cpu_count() or 1, 16)
+    step = 1024
+    idx = np.random.default_rng(0).permutation(len(meta))
+    chunks = [idx[i:i + step] for i in range(0, len(idx), step)]
+    t0 = time.perf_counter()
+    done = 0
+
+    def work(sel):
+        if time.perf_counter() - t0 > seconds:
+            return 0
+        orc.check_batch(np.ascontiguousarray(rows[sel]), np.ascontiguousarray(meta[sel]))
+        return len(sel)
+
+    with ThreadPoolExecutor(cores) as ex:
+        for k in ex.map(work, chunks):
+            done += k
+    dt = time.perf_counter() - t0
+    return dict(value=done / dt, unit="candidates/s", cores=cores, kind="port",
+                sample=f"{done} candidates of the same batch, {cores} threads, {dt:.1f} s")
+
+
+def host_buffer_rate(rows, meta):
+    """Whole batch through the host-buffer entry points (ise_submit /
+    ise_collect: copies to and from the device included)."""
+    import ise_amd
+    with ise_amd.Context(capacity=1 << 18, stride=STRIDE) as ctx:
+        ctx.check_batch(rows[:4096], meta[:4096])
+        t0 = time.perf_counter()
+        ctx.check_batch(rows, meta, fields=("verdict", "fail_row", "row_to_change", "group_order", "perm_row"))
+        dt = time.perf_counter() - t0
+    return {"candidates_per_s": len(meta) / dt, "seconds": dt,
+            "note": "host arrays in and out, chunks of 2^18 candidates, PCIe copies included"}
+
+
 def profiled_launch_figures():
     """HBM bytes and instruction counts of one plain launch of this workload,
     as measured by rocprofv3 PMC runs and committed under profiles/."""
@@ -286,9 +327,11 @@ def main():
     if not args.no_extra:
         line["kfact_k12"] = kfact_rate(ctx, max(3, args.steps // 4), 1)
         line["enumeration"] = enumeration_rate()
+        line["host_buffers"] = host_buffer_rate(rows, meta)
     ctx.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(rows, meta)
+        line["cpu_all_cores"] = cpu_all_cores(rows, meta)
     print(json.dumps(line), flush=True)
 
 
