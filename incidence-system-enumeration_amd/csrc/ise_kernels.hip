@@ -44,8 +44,19 @@ __device__ __forceinline__ uint32_t next_candidate(const Grp<G> &g, uint32_t n, 
     return n;
 }
 
+// Waves per SIMD the register allocation of the canonicity kernel must leave
+// room for (0: the compiler's choice).
+#ifndef ISE_CHECK_WAVES_PER_EU
+#define ISE_CHECK_WAVES_PER_EU 0
+#endif
+#if ISE_CHECK_WAVES_PER_EU > 0
+#define ISE_CHECK_OCCUPANCY __attribute__((amdgpu_waves_per_eu(ISE_CHECK_WAVES_PER_EU)))
+#else
+#define ISE_CHECK_OCCUPANCY
+#endif
+
 template <int G, bool COUNT, bool HUNT>
-__global__ __launch_bounds__(1024) void check_kernel(uint32_t n, uint32_t stride,
+__global__ __launch_bounds__(1024) ISE_CHECK_OCCUPANCY void check_kernel(uint32_t n, uint32_t stride,
                                                     const uint64_t *__restrict__ rows,
                                                     const uint64_t *__restrict__ cells,
                                                     const uint32_t *__restrict__ meta,

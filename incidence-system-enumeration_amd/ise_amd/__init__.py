@@ -110,6 +110,8 @@ def lib():
         L.ise_group_closure.argtypes = [P, U32, U32, P, U64, ctypes.POINTER(U64)]
         L.ise_adjust_generators.argtypes = [P, U32, U32, P, U32, P]
         L.ise_solution_orbits.argtypes = [P, P, U64, U32, P, U32, P, P]
+        L.ise_format_matrix.argtypes = [P, I, I, U64, U64, I, U64, P, U64, ctypes.POINTER(U64)]
+        L.ise_format_aut_table.argtypes = [I, I, I, I, I, U32, P, P, P, P, U64, ctypes.POINTER(U64)]
         L.ise_kf_check.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
         L.ise_kf_check_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P, P]
         L.ise_kf_check_batch_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P,
@@ -498,6 +500,34 @@ def adjust_generators(gens, idx):
     if rc != OK:
         raise ISEError(rc, "ise_adjust_generators")
     return out
+
+
+def _format(call):
+    need = ctypes.c_uint64(0)
+    call(None, 0, ctypes.byref(need))
+    buf = ctypes.create_string_buffer(need.value + 1)
+    rc = call(buf, need.value + 1, ctypes.byref(need))
+    if rc != OK:
+        raise ISEError(rc, "format")
+    return buf.value.decode()
+
+
+def format_matrix(rows, ncols, number=0, sub=0, canon=1, aut=1):
+    """Text of one matrix as the reference prints it (packed rows)."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint64)
+    return _format(lambda b, c, n: lib().ise_format_matrix(_ptr(rows), len(rows), ncols, number, sub, canon, aut,
+                                                           b, c, n))
+
+
+def format_aut_table(v, k, lam, designs, auts, orbits):
+    """The |Aut(D)| summary table of an enumeration (reference layout)."""
+    _, b, r, _, _ = design_params(v, k, lam)
+    designs = np.ascontiguousarray(designs, dtype=np.uint64)
+    auts = np.ascontiguousarray(auts, dtype=np.uint64)
+    orbits = np.ascontiguousarray(orbits, dtype=np.uint8)
+    n = len(auts)
+    return _format(lambda bf, c, ln: lib().ise_format_aut_table(v, b, r, k, lam, n, _ptr(designs), _ptr(auts),
+                                                                _ptr(orbits), bf, c, ln))
 
 
 def witness_labels(meta, verdict, fail_row, perm_row, od):

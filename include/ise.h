@@ -351,6 +351,27 @@ int ise_adjust_generators(const uint8_t *gens, uint32_t n_gens, uint32_t len, co
 int ise_solution_orbits(ise_ctx *ctx, const uint8_t *group, uint64_t order, uint32_t len, const uint8_t *sols,
                         uint32_t n_sol, uint8_t *canon_out, uint32_t *rep_out);
 
+/* ---- text output of results (host) --------------------------------------
+ * Both write a NUL-terminated string into buf (cap bytes); *len_out receives
+ * the length needed (without the NUL), and ISE_ERR_CAPACITY is returned when
+ * it does not fit. */
+
+/* One matrix as the reference prints it (CMatrixData::printOut,
+ * Source/Matrix.cpp:43-155): "\nMatrix # %3llu[.sub][  *** Non-canonical ***]
+ * |Aut(M)| = %6llu" then one line of 0/1 per row; number 0 prints the rows
+ * only. */
+int ise_format_matrix(const uint64_t *rows, int nrows, int ncols, uint64_t number, uint64_t sub, int canon,
+                      uint64_t aut, char *buf, uint64_t cap, uint64_t *len_out);
+
+/* The enumeration summary: "BIBD(v, b, r, k, lambda)" and the |Aut(D)| table
+ * (CGroupsInfo::printGroupInfo, Source/GroupsInfo.cpp:96-150): per group
+ * order the numbers of designs, of simple designs (no replicated blocks), of
+ * designs transitive on the rows and of both, then the totals.  designs
+ * [n][v] packed rows, auts [n], orbits [n][32] row orbits from the test
+ * (transitive = every row in orbit 0). */
+int ise_format_aut_table(int v, int b, int r, int k, int lambda, uint32_t n, const uint64_t *designs,
+                         const uint64_t *auts, const uint8_t *orbits, char *buf, uint64_t cap, uint64_t *len_out);
+
 /* ---- host-side candidate producers (stay on the CPU) -------------------- */
 
 /* Level-synchronous enumeration of 2-(v,k,lambda) designs: every level's

@@ -9,6 +9,8 @@ JSON fixtures next to this script:
                      Results/BIBDs/Partial Results/25_4_1_tmp.txt.
 * printed_bibds.json canonical BIBD matrices printed with |Aut(M)| in
                      Results/Quasi-Combined_BIBDs/V = <v>/*.txt.
+* format_golden.json verbatim |Aut(D)| tables and printed matrix blocks
+                     (expected output of the text formatters).
 * kfact_k10.json     the 396 canonical 1-factorizations of K_10 with |Aut(M)|,
                      k-Factorization/Results/Complete_graphs/10/10x9x2/K_OrderedMatrices.txt,
                      and the uniform K_12 factorization of
@@ -146,6 +148,38 @@ def kfact():
     return res
 
 
+def text_blocks():
+    """Verbatim output lines (expected text for the output formatters): the
+    "BIBD(...)" header through the "Total:" line of the result files written
+    in the current table layout, and the first printed matrix blocks of a
+    Quasi-Combined result file."""
+    tables = {}
+    for name in ["V =   7/7_3_1.txt"] + [f"V =   6/6_3_{l}.txt" for l in (2, 4, 6, 8, 10, 12)]:
+        path = os.path.join(REF, "Results/BIBDs", name)
+        if not os.path.exists(path):
+            continue
+        lines = read(path).splitlines()
+        start = next(i for i, t in enumerate(lines) if t.startswith("BIBD("))
+        end = next(i for i, t in enumerate(lines) if t.strip().startswith("Total:"))
+        key = os.path.basename(name)[:-4]
+        tables[key] = dict(source=os.path.relpath(path, REF), lines=lines[start:end + 1])
+    mpath = os.path.join(REF, "Results/Quasi-Combined_BIBDs/V =   9/9_3_2.txt")
+    blocks = []
+    if os.path.exists(mpath):
+        lines = read(mpath).splitlines()
+        i = 0
+        while i < len(lines) and len(blocks) < 12:
+            if lines[i].startswith("Matrix #"):
+                j = i + 1
+                while j < len(lines) and lines[j] and set(lines[j]) <= set("01"):
+                    j += 1
+                blocks.append(lines[i:j])
+                i = j
+            else:
+                i += 1
+    return dict(tables=tables, matrices=dict(source=os.path.relpath(mpath, REF), blocks=blocks))
+
+
 def main():
     counts = bibd_counts()
     with open(os.path.join(OUT, "bibd_counts.json"), "w") as f:
@@ -156,6 +190,9 @@ def main():
     kf = kfact()
     with open(os.path.join(OUT, "kfact.json"), "w") as f:
         json.dump(kf, f, indent=0)
+    blocks = text_blocks()
+    with open(os.path.join(OUT, "format_golden.json"), "w") as f:
+        json.dump(blocks, f, indent=0)
     print(len(counts), "count tables;", sum(len(p["matrices"]) for p in printed), "printed BIBDs;",
           {k: len(v["matrices"]) for k, v in kf.items()}, "k-fact matrices")
 
