@@ -162,8 +162,9 @@ def kfact_rate(ctx, steps, warmup):
 
 
 def enumeration_rate():
-    """Full enumeration of 2-(13,3,1) and 2-(15,3,1) (host row solver + GPU
-    canonicity per level): designs and candidates per second, wall clock."""
+    """Full enumeration of 2-(13,3,1) and 2-(15,3,1): host row solver + GPU
+    canonicity per level, and the device-resident enumeration (GPU row
+    solver, canonicity and compaction); designs per second, wall clock."""
     import ise_amd
     out = {}
     with ise_amd.Context(capacity=1 << 16, stride=32) as ctx:
@@ -172,8 +173,16 @@ def enumeration_rate():
             t0 = time.perf_counter()
             des, _, st, _ = ctx.enumerate_bibd(v, k, lam)
             dt = time.perf_counter() - t0
+            ctx.enumerate_bibd_device(v, k, lam)
+            t0 = time.perf_counter()
+            des2, _, _, _ = ctx.enumerate_bibd_device(v, k, lam)
+            dt2 = time.perf_counter() - t0
+            if not np.array_equal(des, des2):
+                raise RuntimeError("device enumeration differs from the host-solver enumeration")
             out[f"2-({v},{k},{lam})"] = {"designs": len(des), "seconds": dt, "designs_per_s": len(des) / dt,
-                                         "candidates_per_s": st["candidates"] / dt}
+                                         "candidates_per_s": st["candidates"] / dt,
+                                         "device_resident_seconds": dt2,
+                                         "device_resident_designs_per_s": len(des) / dt2}
     return out
 
 

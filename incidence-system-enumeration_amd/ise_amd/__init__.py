@@ -110,6 +110,9 @@ def lib():
         L.ise_group_closure.argtypes = [P, U32, U32, P, U64, ctypes.POINTER(U64)]
         L.ise_adjust_generators.argtypes = [P, U32, U32, P, U32, P]
         L.ise_solution_orbits.argtypes = [P, P, U64, U32, P, U32, P, P]
+        L.ise_solve_rows_device.argtypes = [P, I, I, I, U32, U32, P, I, P, P, P, U64, ctypes.POINTER(U64)]
+        L.ise_enumerate_bibd_device.argtypes = [P, I, I, I, P, P, ctypes.c_long, P, P]
+        L.ise_enumerate_bibd_device.restype = ctypes.c_long
         L.ise_format_matrix.argtypes = [P, I, I, U64, U64, I, U64, P, U64, ctypes.POINTER(U64)]
         L.ise_format_aut_table.argtypes = [I, I, I, I, I, U32, P, P, P, P, U64, ctypes.POINTER(U64)]
         L.ise_kf_check.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
@@ -381,6 +384,36 @@ class Context:
             self._check(int(rc))
         return rows[:nout.value], meta[:nout.value]
 
+
+    def enumerate_bibd_device(self, v, k, lam, max_designs=1 << 16):
+        """Full enumeration with row extension, canonicity and compaction all on
+        the device; same returns as enumerate_bibd."""
+        des = np.zeros((max_designs, v), np.uint64)
+        auts = np.zeros(max_designs, np.uint64)
+        stats = np.zeros(5, np.uint64)
+        levels = np.zeros(v + 1, np.uint64)
+        n = self._L.ise_enumerate_bibd_device(self.h, v, k, lam, _ptr(des), _ptr(auts), max_designs,
+                                              _ptr(stats), _ptr(levels))
+        if n < 0:
+            self._check(int(n))
+        m = min(n, max_designs)
+        return des[:m], auts[:m], dict(candidates=int(stats[0]), canonical_partial=int(stats[1]),
+                                       fully_constructed=int(stats[2]), designs=int(stats[3])), levels
+
+    def solve_rows_device(self, v, k, lam, parents, depth, cap=1 << 20):
+        """Children of host parents [n][stride] by the GPU row solver:
+        (children [m][stride], solutions per parent)."""
+        parents = np.ascontiguousarray(parents, dtype=np.uint64)
+        n, stride = parents.shape
+        d_par = self.device_array(parents.shape, np.uint64, parents)
+        d_cnt = self.device_array((max(n, 1),), np.uint32)
+        d_off = self.device_array((max(n, 1),), np.uint32)
+        d_kids = self.device_array((cap, stride), np.uint64)
+        total = ctypes.c_uint64(0)
+        self._check(self._L.ise_solve_rows_device(self.h, v, k, lam, n, stride, d_par.ptr, depth, d_cnt.ptr,
+                                                  d_off.ptr, d_kids.ptr, cap, ctypes.byref(total)))
+        kids = d_kids.to_host()[:total.value]
+        return kids, d_cnt.to_host()[:n]
 
     def enumerate_sampled(self, v, k, lam, beam, seed, dmin, dmax, stride=None, cap=1 << 21):
         """Beam-sampled level enumeration (ise_enumerate_bibd_sampled):

@@ -395,6 +395,28 @@ long ise_enumerate_bibd_candidates(ise_ctx *ctx, int v, int k, int lambda,
                                    uint64_t *rows_out, uint32_t *meta_out,
                                    long cap, long *n_out);
 
+/* Row extension on the GPU (the reference's CInSysSolver / CRowSolution,
+ * Source/InSysSolver.h:92-286): for n parents of `depth` rows (device,
+ * [n][stride] words), every next row whose column-orbit counts sum to r,
+ * meet every earlier row in lambda columns and leave every column able to
+ * reach k ones; ones left-justified inside each orbit, counts tried from the
+ * largest down (the order of the host solver).  d_counts/d_offsets [n]
+ * (device) receive the solutions per parent and their exclusive prefix sum;
+ * the children (parent rows + the new row, zero-padded to stride) go to
+ * d_children [children_cap][stride] in parent order.  *n_children = total;
+ * ISE_ERR_CAPACITY (with the total) when it exceeds children_cap. */
+int ise_solve_rows_device(ise_ctx *ctx, int v, int k, int lambda, uint32_t n, uint32_t stride,
+                          const uint64_t *d_parents, int depth, uint32_t *d_counts, uint32_t *d_offsets,
+                          uint64_t *d_children, uint64_t children_cap, uint64_t *n_children);
+
+/* ise_enumerate_bibd with every step on the device: row extension
+ * (ise_solve_rows_device kernels), canonicity, and an order-preserving
+ * compaction of the canonical children per level; the host reads only the
+ * level sizes and the final designs.  Same outputs as ise_enumerate_bibd. */
+long ise_enumerate_bibd_device(ise_ctx *ctx, int v, int k, int lambda, uint64_t *designs,
+                               uint64_t *auts, long max_designs, uint64_t *stats,
+                               uint64_t *level_counts);
+
 /* Beam-sampled enumeration for parameter sets too large to enumerate in
  * full: level by level as above, but when more than `beam` (>= 1) candidates
  * of a level are canonical only `beam` of them are extended, chosen by a

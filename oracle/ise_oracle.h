@@ -98,6 +98,11 @@ long orc_enumerate_sampled(int v, int k, int lambda, uint32_t beam, uint64_t see
                            int stride, uint64_t *cand_rows, uint32_t *cand_meta, long cand_cap,
                            uint64_t *level_counts);
 
+/* Solutions of the next-row system of one partial matrix (rows[0..depth-1]):
+ * the rows the enumerator would try next, in its order (up to cap written).
+ * Returns the number of solutions or <0 on error. */
+long orc_row_solutions(int v, int k, int lambda, const uint64_t *rows, int depth, uint64_t *out, long cap);
+
 /* Brute-force reference for tiny matrices: the lexicographically largest
  * matrix over all row permutations (columns re-sorted in decreasing order)
  * and the number of row permutations that map the matrix to itself. */

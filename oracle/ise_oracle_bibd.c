@@ -650,6 +650,28 @@ long orc_enumerate_bibd(int v, int k, int lambda, uint64_t *designs,
     return nd;
 }
 
+/* Solutions of the next-row system of one partial matrix (rows[0..depth-1]),
+ * in solver order (up to cap written); returns the number of solutions. */
+long orc_row_solutions(int v, int k, int lambda, const uint64_t *rows, int depth, uint64_t *out, long cap)
+{
+    if (v < 2 || k < 2 || k > v || lambda < 1 || depth < 0 || depth >= v) return -1;
+    if ((lambda * (v - 1)) % (k - 1)) return -2;
+    int r = lambda * (v - 1) / (k - 1);
+    if ((v * r) % k) return -2;
+    int b = v * r / k;
+    if (v > ORC_MAX_ROWS || b > ORC_MAX_COLS) return -3;
+    enum_t *e = (enum_t *)calloc(1, sizeof(enum_t));
+    e->v = v; e->b = b; e->r = r; e->k = k; e->lambda = lambda;
+    uint64_t st[5] = {0, 0, 0, 0, 0};
+    e->stats = st;
+    memcpy(e->rows, rows, sizeof(uint64_t) * depth);
+    e->collect = out; e->ncollect = 0; e->collect_cap = cap;
+    expand(e, depth);
+    long n = e->ncollect;
+    free(e);
+    return n;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Beam-sampled level-by-level enumeration.                                  */
 /*

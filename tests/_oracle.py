@@ -39,6 +39,9 @@ def load():
                                             ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P,
                                             ctypes.c_long, _P]
         L.orc_enumerate_sampled.restype = ctypes.c_long
+        L.orc_row_solutions.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, _P,
+                                        ctypes.c_long]
+        L.orc_row_solutions.restype = ctypes.c_long
         L.orc_bruteforce_canon.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P]
         L.orc_check_batch_hunt.argtypes = [U32, U32, _P, _P, _P, _P, U32, U32, U32, _P]
         L.orc_hunt.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, U32, _P, _P, _P]
@@ -162,6 +165,14 @@ class Oracle:
         if rc:
             raise ValueError("zero total weight")
         return g, float(loss[0])
+
+    def row_solutions(self, v, k, lam, rows, depth, cap=1 << 16):
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        out = np.zeros(cap, np.uint64)
+        n = self.L.orc_row_solutions(v, k, lam, rows.ctypes.data, depth, out.ctypes.data, cap)
+        if n < 0:
+            raise ValueError(f"oracle row solutions failed ({n})")
+        return out[:min(n, cap)]
 
     def test_canonicity(self, mat, len_stab=0, out_info=0, gen_cap=0):
         import ise_amd._abi as abi
