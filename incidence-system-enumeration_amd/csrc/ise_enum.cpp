@@ -208,6 +208,12 @@ long run_enumeration(ise_ctx *ctx, const Design &ds, EnumSink &sink, uint64_t *s
     if (level_counts)
         for (int i = 0; i <= ds.v; i++) level_counts[i] = 0;
     std::vector<uint64_t> level, next, keep;
+    // only verdicts and |Aut| are read back: no column permutations
+    struct ColumnsOff {
+        ise_ctx *c;
+        bool prev;
+        ~ColumnsOff() { ise::ctx_report_columns(c, prev); }
+    } columns_off{ctx, ise::ctx_report_columns(ctx, false)};
     // depth 0 -> candidates of depth 1
     std::vector<uint64_t> root(stride, 0ull);
     extend_level(ds, root, 0, stride, level);

@@ -15,10 +15,16 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
                         hipStream_t stream);
 
 // Resident workgroups per CU for the kernel variant.
-hipError_t occupancy_blocks(int group_lanes, uint32_t waves_per_block, uint32_t stride, int *blocks_per_cu);
+hipError_t occupancy_blocks(int group_lanes, uint32_t waves_per_block, uint32_t stride, bool sets,
+                            int *blocks_per_cu);
 
-// Dynamic LDS of one workgroup (orbit geometry of every group's candidate).
-size_t lds_bytes(int group_lanes, uint32_t waves_per_block, uint32_t stride);
+// Dynamic LDS of one workgroup (orbit geometry, or column sets, of every
+// group's candidate).
+size_t lds_bytes(int group_lanes, uint32_t waves_per_block, uint32_t stride, bool sets);
+
+// Whether a launch with these outputs runs the column-set search (no column
+// permutation reported, no witness hunt).
+bool uses_sets(const ise_results &out, uint32_t hunt_budget);
 
 // Smallest lane-group size that holds every candidate of a batch.
 inline int fit_group_lanes(uint32_t max_depth, uint32_t max_cols) {
@@ -33,6 +39,8 @@ hipStream_t ctx_stream(const ise_ctx *c);
 int ctx_device(const ise_ctx *c);
 int ctx_num_cus(const ise_ctx *c);
 void ctx_set_error(ise_ctx *c, const char *msg);
+// Set whether ise_submit reports column permutations; returns the previous setting.
+bool ctx_report_columns(ise_ctx *c, bool on);
 void ctx_get_order(const ise_ctx *c, int *mode, uint32_t *budget, const uint8_t **order, uint32_t *dim);
 
 } // namespace ise

@@ -55,7 +55,14 @@ __device__ __forceinline__ uint32_t next_candidate(const Grp<G> &g, uint32_t n, 
 #define ISE_CHECK_OCCUPANCY
 #endif
 
-template <int G, bool COUNT, bool HUNT>
+// 32-bit words of LDS scratch per lane group: orbit geometry (one word per
+// row and lane) or, for the column-set search, one 64-bit set per row and
+// lane plus one count byte per row and lane.
+template <int G, bool SETS> __host__ __device__ constexpr uint32_t group_scratch_words(uint32_t rows_cap) {
+    return SETS ? rows_cap * G * 2u + rows_cap * G / 4u : rows_cap * G;
+}
+
+template <int G, bool COUNT, bool HUNT, bool SETS>
 __global__ __launch_bounds__(1024) ISE_CHECK_OCCUPANCY void check_kernel(uint32_t n, uint32_t stride,
                                                     const uint64_t *__restrict__ rows,
                                                     const uint64_t *__restrict__ cells,
@@ -66,7 +73,7 @@ __global__ __launch_bounds__(1024) ISE_CHECK_OCCUPANCY void check_kernel(uint32_
     extern __shared__ uint32_t lds_geo[];
     Grp<G> g;
     // this group's orbit-geometry scratch: rows_cap rows of G words
-    uint32_t *geo = lds_geo + ((threadIdx.x >> 6) * (64 / G) + (g.base / G)) * rows_cap * G;
+    uint32_t *geo = lds_geo + ((threadIdx.x >> 6) * (64 / G) + (g.base / G)) * group_scratch_words<G, SETS>(rows_cap);
     const int home = (int)(blockIdx.x % NUM_XCD);
     int slice = home;
     for (;;) {
@@ -82,7 +89,8 @@ __global__ __launch_bounds__(1024) ISE_CHECK_OCCUPANCY void check_kernel(uint32_
             if (g.ln == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
             continue;
         }
-        check_candidate<G, COUNT, HUNT>(g, ci, m, rows, cells, stride, o, max_perms, geo, order, od, hunt_budget);
+        check_candidate<G, COUNT, HUNT, SETS>(g, ci, m, rows, cells, stride, o, max_perms, geo, order, od,
+                                              hunt_budget);
     }
 }
 
@@ -96,7 +104,9 @@ static OutPtrs to_out(const ise_results &r) {
     return o;
 }
 
-template <int G> static const void *kernel_ptr() { return (const void *)&check_kernel<G, true, false>; }
+template <int G, bool SETS> static const void *kernel_ptr() {
+    return (const void *)&check_kernel<G, true, false, SETS>;
+}
 
 // Rows of orbit geometry kept per candidate in LDS.
 static uint32_t rows_cap_for(int group_lanes, uint32_t stride) {
@@ -104,8 +114,20 @@ static uint32_t rows_cap_for(int group_lanes, uint32_t stride) {
     return cap < (uint32_t)group_lanes ? cap : (uint32_t)group_lanes;
 }
 
-size_t lds_bytes(int group_lanes, uint32_t waves_per_block, uint32_t stride) {
-    return (size_t)waves_per_block * 64u * rows_cap_for(group_lanes, stride) * sizeof(uint32_t);
+size_t lds_bytes(int group_lanes, uint32_t waves_per_block, uint32_t stride, bool sets) {
+    const uint32_t cap = rows_cap_for(group_lanes, stride);
+    const uint32_t groups = waves_per_block * (64u / (uint32_t)group_lanes);
+    uint32_t words = 0;
+    switch (group_lanes) {
+    case 16: words = sets ? group_scratch_words<16, true>(cap) : group_scratch_words<16, false>(cap); break;
+    case 32: words = sets ? group_scratch_words<32, true>(cap) : group_scratch_words<32, false>(cap); break;
+    default: words = sets ? group_scratch_words<64, true>(cap) : group_scratch_words<64, false>(cap); break;
+    }
+    return (size_t)groups * words * sizeof(uint32_t);
+}
+
+bool uses_sets(const ise_results &out, uint32_t hunt_budget) {
+    return out.perm_col == nullptr && hunt_budget == 0;
 }
 
 hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_block,
@@ -118,21 +140,25 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
     const dim3 grid(blocks), block(64 * waves_per_block);
     const OutPtrs o = to_out(out);
     const uint32_t cap = rows_cap_for(group_lanes, stride);
-    const size_t lds = lds_bytes(group_lanes, waves_per_block, stride);
-    // statistics counters and the witness hunt are compiled in only when asked for
+    // statistics counters and the witness hunt are compiled in only when asked for;
+    // the column-set search runs when no column permutation is reported
     const bool count = out.counters != nullptr;
     const bool hunt = hunt_budget > 0;
+    const bool sets = uses_sets(out, hunt_budget);
+    const size_t lds = lds_bytes(group_lanes, waves_per_block, stride, sets);
     const int od = (int)order_dim;
-#define ISE_LAUNCH_V(GL, C, H)                                                                          \
-    hipLaunchKernelGGL((check_kernel<GL, C, H>), grid, block, lds, stream, n, stride, rows, cells, meta, o, \
+#define ISE_LAUNCH_V(GL, C, H, S)                                                                        \
+    hipLaunchKernelGGL((check_kernel<GL, C, H, S>), grid, block, lds, stream, n, stride, rows, cells, meta, o, \
                        head, max_perms, cap, order, od, hunt_budget)
 #define ISE_LAUNCH(GL)                                          \
     if (count) {                                                \
-        if (hunt) ISE_LAUNCH_V(GL, true, true);                 \
-        else ISE_LAUNCH_V(GL, true, false);                     \
+        if (hunt) ISE_LAUNCH_V(GL, true, true, false);          \
+        else if (sets) ISE_LAUNCH_V(GL, true, false, true);     \
+        else ISE_LAUNCH_V(GL, true, false, false);              \
     } else {                                                    \
-        if (hunt) ISE_LAUNCH_V(GL, false, true);                \
-        else ISE_LAUNCH_V(GL, false, false);                    \
+        if (hunt) ISE_LAUNCH_V(GL, false, true, false);         \
+        else if (sets) ISE_LAUNCH_V(GL, false, false, true);    \
+        else ISE_LAUNCH_V(GL, false, false, false);             \
     }
     switch (group_lanes) {
     case 16: ISE_LAUNCH(16) break;
@@ -146,10 +172,14 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
     return hipGetLastError();
 }
 
-hipError_t occupancy_blocks(int group_lanes, uint32_t waves_per_block, uint32_t stride, int *blocks_per_cu) {
-    const void *f = group_lanes == 16 ? kernel_ptr<16>() : group_lanes == 32 ? kernel_ptr<32>() : kernel_ptr<64>();
+hipError_t occupancy_blocks(int group_lanes, uint32_t waves_per_block, uint32_t stride, bool sets,
+                            int *blocks_per_cu) {
+    const void *f = sets ? (group_lanes == 16 ? kernel_ptr<16, true>() : group_lanes == 32 ? kernel_ptr<32, true>()
+                                                                                          : kernel_ptr<64, true>())
+                         : (group_lanes == 16 ? kernel_ptr<16, false>() : group_lanes == 32 ? kernel_ptr<32, false>()
+                                                                                           : kernel_ptr<64, false>());
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, (int)(64 * waves_per_block),
-                                                        lds_bytes(group_lanes, waves_per_block, stride));
+                                                        lds_bytes(group_lanes, waves_per_block, stride, sets));
 }
 
 } // namespace ise

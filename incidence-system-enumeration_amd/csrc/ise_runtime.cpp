@@ -33,6 +33,8 @@ struct ise_ctx {
     // last submit
     uint32_t last_n = 0, last_gen_cap = 0;
     bool pending = false, last_cells = false;
+    // ise_submit reports column permutations (ise_report_columns)
+    bool report_columns = true, last_columns = true;
     ise_launch_cfg cfg{0, 4, 0, 1u << 26};
     // search order (ise_set_order)
     int order_mode = ISE_ORDER_PLAIN;
@@ -46,7 +48,7 @@ struct ise_ctx {
     // executable graphs of ise_submit launches, keyed by batch shape
     bool use_graphs = false;
     using GraphKey = std::tuple<uint32_t, uint32_t, int, bool, uint32_t, uint32_t, uint32_t, uint32_t, int,
-                                uint32_t, uintptr_t, uint32_t>;
+                                uint32_t, uintptr_t, uint32_t, bool>;
     std::map<GraphKey, hipGraphExec_t> graphs;
     uint64_t graph_replays = 0;
     float last_ms = 0.f;
@@ -97,12 +99,14 @@ int scan_meta(const uint32_t *meta, uint32_t n, uint32_t stride, uint32_t *max_d
 }
 
 // Persistent grid size: resident workgroups per CU times CUs, capped by the work.
-int grid_blocks(ise_ctx *c, uint32_t n, uint32_t stride, int group_lanes, uint32_t *blocks) {
+int grid_blocks(ise_ctx *c, uint32_t n, uint32_t stride, int group_lanes, const ise_results &out,
+                uint32_t *blocks) {
     const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
+    const uint32_t budget = c->order_mode == ISE_ORDER_PLAIN ? 0u : c->hunt_budget;
     uint32_t b = c->cfg.blocks;
     if (!b) {
         int per_cu = 0;
-        HIPCHK(c, ise::occupancy_blocks(group_lanes, wpb, stride, &per_cu));
+        HIPCHK(c, ise::occupancy_blocks(group_lanes, wpb, stride, ise::uses_sets(out, budget), &per_cu));
         if (per_cu < 1) per_cu = 1;
         b = (uint32_t)per_cu * (uint32_t)c->num_cus;
         const uint64_t groups_needed = ((uint64_t)n * group_lanes + 64ull * wpb - 1) / (64ull * wpb);
@@ -130,7 +134,7 @@ int launch(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const 
            const uint32_t *meta, const ise_results &out, uint32_t *head, hipStream_t s,
            int group_lanes) {
     uint32_t blocks = 0;
-    int rc = grid_blocks(c, n, stride, group_lanes, &blocks);
+    int rc = grid_blocks(c, n, stride, group_lanes, out, &blocks);
     if (rc) return rc;
     return launch_with(c, blocks, n, stride, rows, cells, meta, out, head, s, group_lanes);
 }
@@ -144,6 +148,11 @@ hipStream_t ctx_stream(const ise_ctx *c) { return c->stream; }
 int ctx_device(const ise_ctx *c) { return c->device; }
 int ctx_num_cus(const ise_ctx *c) { return c->num_cus; }
 void ctx_set_error(ise_ctx *c, const char *msg) { c->err = msg; }
+bool ctx_report_columns(ise_ctx *c, bool on) {
+    const bool prev = c->report_columns;
+    c->report_columns = on;
+    return prev;
+}
 void ctx_get_order(const ise_ctx *c, int *mode, uint32_t *budget, const uint8_t **order, uint32_t *dim) {
     *mode = c->order_mode;
     *budget = c->hunt_budget;
@@ -285,6 +294,7 @@ int ise_submit(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows,
     c->last_n = n;
     c->last_gen_cap = gen_cap;
     c->last_cells = cells != nullptr;
+    c->last_columns = c->report_columns;
     if (n) {
         HIPCHK(c, hipMemcpyAsync(c->d_rows, rows, (size_t)n * stride * 8, hipMemcpyHostToDevice, c->stream));
         if (cells)
@@ -293,20 +303,21 @@ int ise_submit(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows,
         ise_results o = c->d;
         o.gens = gen_cap ? c->d.gens : nullptr;
         o.gen_cap = gen_cap;
+        if (!c->report_columns) o.perm_col = nullptr;
         int group = c->cfg.group_lanes ? (int)c->cfg.group_lanes : ise::fit_group_lanes(md, mb);
         if ((uint32_t)group < md || (uint32_t)group < mb) return ISE_ERR_ARG;
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         if (c->use_graphs) {
             const ise_ctx::GraphKey key{n, stride, group, cells != nullptr, gen_cap, c->cfg.waves_per_block,
                                         c->cfg.blocks, c->cfg.max_perms, c->order_mode, c->hunt_budget,
-                                        (uintptr_t)c->d_order, c->order_dim};
+                                        (uintptr_t)c->d_order, c->order_dim, c->report_columns};
             auto it = c->graphs.find(key);
             if (it == c->graphs.end()) {
                 // capture once: every pointer is a context buffer, so replays see new data
                 hipGraph_t g = nullptr;
                 hipGraphExec_t ge = nullptr;
                 uint32_t blocks = 0;
-                int rc = grid_blocks(c, n, stride, group, &blocks);
+                int rc = grid_blocks(c, n, stride, group, o, &blocks);
                 if (rc) return rc;
                 HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 rc = launch_with(c, blocks, n, stride, c->d_rows, cells ? c->d_cells : nullptr, c->d_meta,
@@ -342,6 +353,10 @@ int ise_collect(ise_ctx *c, ise_results *h) {
     const size_t n = c->last_n;
     hipStream_t s = c->stream;
     HIPCHK(c, hipSetDevice(c->device));
+    if (n && h->perm_col && !c->last_columns) {
+        c->err = "column permutations were not reported for this batch (ise_report_columns)";
+        return ISE_ERR_STATE;
+    }
     if (n) {
         if (h->verdict) HIPCHK(c, hipMemcpyAsync(h->verdict, c->d.verdict, n, hipMemcpyDeviceToHost, s));
         if (h->fail_row) HIPCHK(c, hipMemcpyAsync(h->fail_row, c->d.fail_row, n, hipMemcpyDeviceToHost, s));
@@ -566,6 +581,12 @@ int ise_set_order(ise_ctx *c, int mode, uint32_t hunt_budget, const uint8_t *d_o
     c->hunt_budget = hunt_budget;
     c->d_order = mode == ISE_ORDER_LEARNED ? d_order : nullptr;
     c->order_dim = mode == ISE_ORDER_LEARNED ? order_dim : 0;
+    return ISE_OK;
+}
+
+int ise_report_columns(ise_ctx *c, int on) {
+    if (!c) return ISE_ERR_ARG;
+    c->report_columns = on != 0;
     return ISE_OK;
 }
 

@@ -236,14 +236,11 @@ struct Staged {
     uint64_t bmask, rowv, cellv, cell_next, below_l;
 };
 
-// Load candidate `ci` into the lanes and fill `geo` (the group's LDS scratch
-// of d*G words): word row*G+l holds, for the column orbit of row `row` that
-// contains column l, its first column, its length and the number of ones of
-// the original row inside it, so that row comparisons do not recompute
-// orbit boundaries.
+// Load candidate `ci` into the lanes (rows, cell masks of every row and of
+// the row after the last).
 template <int G>
-ISE_DEV Staged stage(const Grp<G> &g, uint32_t ci, uint32_t meta, const uint64_t *rows,
-                     const uint64_t *cells, uint32_t stride, uint32_t *geo) {
+ISE_DEV Staged load_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const uint64_t *rows,
+                              const uint64_t *cells, uint32_t stride) {
     const int l = g.ln;
     Staged c;
     c.d = meta & 0xff;
@@ -270,8 +267,19 @@ ISE_DEV Staged stage(const Grp<G> &g, uint32_t ci, uint32_t meta, const uint64_t
     // orbits induced by all d rows (indexing the column-orbit generators)
     const uint64_t last = g.read64(c.rowv, d - 1);
     c.cell_next = (g.read64(c.cellv, d - 1) | (last ^ (last << 1))) & c.bmask;
-    const uint64_t le = (l == 63) ? ~0ull : ((2ull << l) - 1);   // columns 0..l
     c.below_l = lowmask(l);                                        // columns 0..l-1
+    (void)b;
+    return c;
+}
+
+// Fill `geo` (the group's LDS scratch of d*G words): word row*G+l holds, for
+// the column orbit of row `row` that contains column l, its first column, its
+// length and the number of ones of the original row inside it, so that row
+// comparisons do not recompute orbit boundaries.
+template <int G>
+ISE_DEV void write_geo(const Grp<G> &g, const Staged &c, uint32_t *geo) {
+    const int l = g.ln, d = c.d, b = c.b;
+    const uint64_t le = (l == 63) ? ~0ull : ((2ull << l) - 1);   // columns 0..l
     for (int r = 0; r < d; r++) {
         const uint64_t C = g.read64(c.cellv, r), R = g.read64(c.rowv, r);
         int s0 = hib64(C & le);
@@ -282,6 +290,13 @@ ISE_DEV Staged stage(const Grp<G> &g, uint32_t ci, uint32_t meta, const uint64_t
         if (l >= b) { s0 = l; len = 1; cR0 = 0; }
         geo[r * G + l] = (uint32_t)s0 | ((uint32_t)len << 8) | ((uint32_t)cR0 << 16);
     }
+}
+
+template <int G>
+ISE_DEV Staged stage(const Grp<G> &g, uint32_t ci, uint32_t meta, const uint64_t *rows,
+                     const uint64_t *cells, uint32_t stride, uint32_t *geo) {
+    const Staged c = load_candidate(g, ci, meta, rows, cells, stride);
+    write_geo(g, c, geo);
     return c;
 }
 
@@ -408,6 +423,159 @@ ISE_DEV void plain_search(const Grp<G> &g, const Staged &c, const uint32_t *geo,
     (void)b;
 }
 
+// Column-set state of the search (used when the candidate is in orbit form,
+// see stage_sets).  The column permutation of the reference only matters
+// through the set of source columns it places in each column orbit: a row
+// compares orbit by orbit by the number of ones, which does not depend on the
+// order inside an orbit, and the ones-first partition of an orbit that
+// compared equal splits its set into the sets of the two orbits of the next
+// row.  So lane l keeps S = the set of source columns placed in the orbit of
+// level `row` that contains position l, one 64-bit word, and
+//   count of the permuted row in that orbit = popc(R[perm[row]] & S),
+//   descending to row+1:  S &= (bit l of original row `row`) ? R : ~R.
+// Level-i sets are kept on a per-candidate LDS stack (word i*G + l) so that
+// the walk can return to any earlier position.  What this drops is the order
+// inside an orbit, i.e. the column permutation itself: kernels that have to
+// report it run the replayed search.
+//
+// Orbit form, which makes the above exact: inside every column orbit of row i
+// the ones of row i come first, and the orbits of row i+1 are those of row i
+// split after these ones.  Rows written as the reference's MakeRow writes them
+// (Source/MatrixCanonChecker.cpp:14-93) have it.
+template <int G>
+ISE_DEV bool stage_sets(const Grp<G> &g, const Staged &c, const uint64_t *cells, uint64_t *stk, uint8_t *cnt) {
+    const int l = g.ln, d = c.d, b = c.b;
+    const uint64_t le = (l == 63) ? ~0ull : ((2ull << l) - 1);
+    bool ok = true;
+    for (int r = 0; r < d; r++) {
+        const uint64_t C = g.read64(c.cellv, r), R = g.read64(c.rowv, r);
+        const int s0 = hib64(C & le);
+        const uint64_t above = C & ~le & c.bmask;
+        const int e0 = above ? ctz64(above) : b;
+        uint64_t om = lowmask(e0) & ~lowmask(s0);
+        int cR0 = popc64(R & om);
+        if (l >= b) { om = 0; cR0 = 0; }
+        stk[r * G + l] = om;
+        cnt[r * G + l] = (uint8_t)cR0;
+        // ones of row r first inside the orbit
+        ok = ok && (R & om) == (l < b ? (lowmask(s0 + cR0) & ~lowmask(s0)) : 0ull);
+        // orbits of row r+1 = orbits of row r split after those ones (given cell masks only;
+        // derived masks are built that way)
+        if (cells && r + 1 < d)
+            ok = ok && g.read64(c.cellv, r + 1) == ((C | (R ^ (R << 1))) & c.bmask);
+    }
+    return g.ballot(!ok) == 0;
+}
+
+// The reference's search (plain order) on the column-set state: the same
+// walk, verdict, witness row permutation, fail row/column, rowToChange,
+// automorphisms, generators, |Aut|, orbits and counters as plain_search.
+// A row refuted at a position other than the stabiliser point moves on to
+// the next larger unused row in place (next_row_perm's `at` branch, which
+// keeps the tail sorted), without leaving the comparison loop.
+template <int G, bool COUNT>
+ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const uint8_t *cnt, uint32_t ci,
+                        const OutPtrs &o, uint32_t max_perms, int &pr, int &ob, Verdict &v) {
+    const int l = g.ln, d = c.d;
+    PermState st;
+    st.stab = d - 1;
+    st.stab_aut = EMAX;
+    st.order = 1;
+    int row = EMAX;
+    for (;;) {
+        row = next_row_perm(g, st, pr, ob, d, row, c.fixed);
+        if (row == EMAX)
+            break;
+        if (++v.perms > max_perms) { v.verdict = ISE_VERDICT_LIMIT; break; }
+        uint64_t S = stk[row * G + l];
+        int cR = cnt[row * G + l];
+        uint64_t C = g.read64(c.cellv, row);
+        int outcome = 0, dec = 0;
+        bool limit = false;
+        for (;;) {
+            if constexpr (COUNT) v.rows_cmp++;
+            const uint64_t Rp = g.read64(c.rowv, g.read(pr, row));
+            const int cP = popc64(Rp & S);
+            const uint64_t bad = g.ballot(cP != cR) & C;
+            outcome = 0;
+            if (!bad) {
+                if constexpr (COUNT) { v.orbs_cmp += popc64(C); v.cells_cmp += (uint32_t)c.b; }
+                const uint64_t Ro = g.read64(c.rowv, row);
+                S &= ((Ro >> l) & 1ull) ? Rp : ~Rp;
+                if (++row == d)
+                    break;                    // every row matched: automorphism
+                stk[row * G + l] = S;
+                cR = cnt[row * G + l];
+                C = g.read64(c.cellv, row);
+                continue;
+            }
+            const int lim = ctz64(bad);
+            const uint64_t more = g.ballot(cR > cP);
+            outcome = ((more >> lim) & 1ull) ? 1 : -1;
+            dec = lim;
+            if constexpr (COUNT) {
+                v.orbs_cmp += popc64(C & lowmask(lim + 1));
+                const uint64_t after = C & ~lowmask(lim + 1);
+                v.cells_cmp += (uint32_t)(after ? ctz64(after) : c.b);
+            }
+            if (outcome < 0 || row == st.stab)
+                break;
+            // refuted: next larger unused row at this position, if any
+            const int old = g.read(pr, row);
+            const uint64_t big = g.ballot(l > row && l < d && pr > old);
+            if (!big)
+                break;
+            const int succ = ctz64(big);
+            const int nw = g.read(pr, succ);
+            if (l == row) pr = nw;
+            if (l == succ) pr = old;
+            if (++v.perms > max_perms) { limit = true; break; }
+        }
+        if (limit) { v.verdict = ISE_VERDICT_LIMIT; break; }
+        if (outcome > 0)
+            continue;                 // refuted prefix: the walk advances position `row`
+        if (outcome < 0) {            // the permuted matrix is larger: witness
+            v.verdict = 0;
+            v.frow = row;
+            v.fcol = dec;
+            v.rtc = row_to_change(g, pr, d, row);
+            break;
+        }
+        // automorphism: S holds the level-d sets
+        if (o.gens && v.ngen < o.gen_cap) {
+            uint8_t *gp = o.gens + ((uint64_t)ci * o.gen_cap + v.ngen) * ISE_MAX_COLS;
+            if (!c.row_permut) {
+                // orbit of the next row whose first column is the smallest source column
+                // placed in orbit x (CanonicityChecker.h:427-450; columns of one orbit
+                // keep their relative order under the ones-first partitions)
+                const int norb = popc64(c.cell_next);
+                for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
+                    const int x = x0 + l;
+                    const int src = x < norb ? select_bit(c.cell_next, x) : 0;
+                    const int sl = src < G ? src : 0;
+                    const uint32_t lo = (uint32_t)g.gather((int)(uint32_t)S, sl);
+                    const uint32_t hi = (uint32_t)g.gather((int)(uint32_t)(S >> 32), sl);
+                    const uint64_t Sx = ((uint64_t)hi << 32) | lo;
+                    const int pcs = Sx ? ctz64(Sx) : 0;
+                    gp[x] = (uint8_t)(x < norb ? popc64(c.cell_next & lowmask(pcs)) : 0xff);
+                }
+            } else {
+                for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
+                    const int x = x0 + l;
+                    const int vv = g.gather(pr, x < G ? x : 0);
+                    gp[x] = (uint8_t)(x < d ? vv : 0xff);
+                }
+            }
+        }
+        v.ngen++;
+        add_automorphism(g, st, pr, ob, d, c.row_permut);
+        row = IMAX;
+    }
+    if (v.verdict == 1 && c.row_permut)
+        grow_order(g, st, ob, d);
+    v.order = st.order;
+}
+
 // Witness hunt in a proposed order: depth-first over row prefixes, position i
 // trying rows in the order order[i*od + 0..d-1] (ascending when `order` is
 // null), with the same comparison and column state as the plain search but
@@ -483,17 +651,33 @@ ISE_DEV bool hunt(const Grp<G> &g, const Staged &c, const uint32_t *geo, const u
 
 // Full canonicity test of candidate `ci` by the group g.  With HUNT, a
 // budgeted witness hunt in the proposed order runs first; the plain search
-// decides whatever the hunt leaves open (canonical candidates, |Aut|).
-template <int G, bool COUNT, bool HUNT>
+// decides whatever the hunt leaves open (canonical candidates, |Aut|).  With
+// SETS (no column permutation reported), a candidate in orbit form runs the
+// column-set search, any other the replayed one.  `lds` is the group's
+// scratch: d*G orbit-geometry words, or in SETS mode d*G set words followed by
+// d*G count bytes.
+template <int G, bool COUNT, bool HUNT, bool SETS>
 ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const uint64_t *rows,
                              const uint64_t *cells, uint32_t stride, const OutPtrs &o, uint32_t max_perms,
-                             uint32_t *geo, const uint8_t *order, int od, uint32_t hunt_budget) {
+                             uint32_t *lds, const uint8_t *order, int od, uint32_t hunt_budget) {
+    static_assert(!(HUNT && SETS), "the witness hunt runs on the replayed search");
     const int l = g.ln;
-    const Staged c = stage(g, ci, meta, rows, cells, stride, geo);
+    const Staged c = load_candidate(g, ci, meta, rows, cells, stride);
     const int d = c.d, b = c.b;
     int pr = l, ob = l, pc = l;
     Verdict v{1, EMAX, EMAX, EMAX, 1ull, 0u, 0u, 0u, 0u, 0u};
     bool found = false;
+    uint32_t *geo = lds;
+    if constexpr (SETS) {
+        uint64_t *stk = reinterpret_cast<uint64_t *>(lds);
+        uint8_t *cnt = reinterpret_cast<uint8_t *>(stk + d * G);
+        if (stage_sets(g, c, cells, stk, cnt)) {
+            set_search<G, COUNT>(g, c, stk, cnt, ci, o, max_perms, pr, ob, v);
+            found = true;
+        }
+    }
+    if (!found)
+        write_geo(g, c, geo);
     if constexpr (HUNT) {
         // an order table narrower than the candidate cannot guide it: plain search only
         if (!order || d <= od)

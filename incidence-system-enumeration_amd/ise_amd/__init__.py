@@ -77,6 +77,7 @@ def lib():
         L.ise_copy_on_device.argtypes = [P, P, P, U64]
         L.ise_set_order.argtypes = [P, I, U32, P, U32]
         L.ise_use_graphs.argtypes = [P, I]
+        L.ise_report_columns.argtypes = [P, I]
         L.ise_graph_stats.argtypes = [P, ctypes.POINTER(U32), ctypes.POINTER(U64)]
         L.ise_autotune.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P, U32, U32, U32, P, P]
         L.ise_sync.argtypes = [P]
@@ -240,7 +241,8 @@ class Context:
             cells = np.ascontiguousarray(cells, dtype=np.uint64)
             if cells.shape != rows.shape:
                 raise ValueError("cells must have the shape of rows")
-        out = {k: [] for k in (fields or ())}
+        # without a column permutation to report the kernel runs the column-set search
+        self._check(self._L.ise_report_columns(self.h, int(fields is None or "perm_col" in fields)))
         res_all = []
         for off in range(0, n, self.capacity):
             cnt = min(self.capacity, n - off)

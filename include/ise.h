@@ -182,6 +182,14 @@ int ise_copy_on_device(ise_ctx *ctx, void *d_dst, const void *d_src, uint64_t by
  * outputs), instantiated once and replayed afterwards; the enumerator then
  * replays one graph per level and chunk size.  on = 0 launches directly. */
 int ise_use_graphs(ise_ctx *ctx, int on);
+
+/* Whether ise_submit reports the column permutation (perm_col; the
+ * reference's t_saveColPermutations output, CanonicityChecker.h:27-32).
+ * Default on.  With it off the kernel runs the column-set form of the same
+ * search (every other result identical) and ise_collect refuses a perm_col
+ * array with ISE_ERR_STATE.  The device-resident entry points decide the same
+ * way from whether d_out->perm_col is null. */
+int ise_report_columns(ise_ctx *ctx, int on);
 /* Number of executable graphs instantiated so far and replays of them. */
 int ise_graph_stats(const ise_ctx *ctx, uint32_t *graphs, uint64_t *replays);
 
