@@ -64,6 +64,18 @@ template <int G> struct Grp {
         uint32_t hi = (uint32_t)read((int)(uint32_t)(x >> 32), i);
         return ((uint64_t)hi << 32) | lo;
     }
+    // A value the whole group holds (in a full wave: pinned to a scalar
+    // register, so that loops driven by it compile to scalar branches).
+    ISE_DEV int uni(int x) const {
+        if constexpr (G == 64) return __builtin_amdgcn_readfirstlane(x);
+        else return x;
+    }
+    ISE_DEV uint64_t uni64(uint64_t x) const {
+        if constexpr (G == 64)
+            return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32)) << 32) |
+                   (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+        else return x;
+    }
     ISE_DEV int gather(int x, int i) const { return pull(x, base + i); }
     ISE_DEV int scatter(int x, int i) const { return push(x, base + i); }
 };
@@ -483,7 +495,7 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
     st.order = 1;
     int row = EMAX;
     for (;;) {
-        row = next_row_perm(g, st, pr, ob, d, row, c.fixed);
+        row = g.uni(next_row_perm(g, st, pr, ob, d, row, c.fixed));
         if (row == EMAX)
             break;
         if (++v.perms > max_perms) { v.verdict = ISE_VERDICT_LIMIT; break; }
@@ -493,6 +505,10 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
         int outcome = 0, dec = 0;
         bool limit = false;
         for (;;) {
+            row = g.uni(row);
+            C = g.uni64(C);
+            st.stab = g.uni(st.stab);
+            v.perms = (uint32_t)g.uni((int)v.perms);
             if constexpr (COUNT) v.rows_cmp++;
             const uint64_t Rp = g.read64(c.rowv, g.read(pr, row));
             const int cP = popc64(Rp & S);
