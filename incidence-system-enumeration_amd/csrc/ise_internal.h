@@ -12,7 +12,14 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
                         const uint64_t *cells, const uint32_t *meta,
                         const ise_results &out, uint32_t *head, uint32_t max_perms,
                         uint32_t hunt_budget, const uint8_t *order, uint32_t order_dim,
+                        uint32_t *defer, const uint32_t *defer_budgets, uint32_t *sched,
                         hipStream_t stream);
+
+// Words of the schedule buffer beyond the n list entries (depth counters).
+constexpr uint32_t SCHED_EXTRA_WORDS = 128;
+
+// Passes of a deferred launch: up to two budgets below the limit, then the limit.
+constexpr int DEFER_BUDGETS = 2;
 
 // Resident workgroups per CU for the kernel variant.
 hipError_t occupancy_blocks(int group_lanes, uint32_t waves_per_block, uint32_t stride, bool sets,

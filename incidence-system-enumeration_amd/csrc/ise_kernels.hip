@@ -20,12 +20,26 @@ namespace ise {
 // stay local until the tail of the batch.
 constexpr int NUM_XCD = 8;
 constexpr int HEAD_STRIDE = 32;   // words between queue heads
+// Long searches are deferred: a first pass runs every candidate with a small
+// permutation budget and lists those that exceed it; a later pass runs the
+// listed ones from the start with a larger budget (listing again those that
+// exceed it), and the last pass with the full budget.  The long searches of
+// a batch thus start together instead of wherever they sat in the queue.
+// Words of the queue buffer that hold, per pass p >= 1, the number of
+// candidates it listed and the queue head of the pass that reads that list.
+constexpr int DEFER_PASSES = 3;
+__host__ __device__ constexpr int defer_count_word(int p) { return 16 + 64 * (p - 1); }
+__host__ __device__ constexpr int defer_head_word(int p) { return 48 + 64 * (p - 1); }
 
 __global__ void queue_reset_kernel(uint32_t *head) {
-    if (threadIdx.x < NUM_XCD) head[threadIdx.x * HEAD_STRIDE] = 0u;
+    const int t = (int)threadIdx.x;
+    if (t < NUM_XCD) head[t * HEAD_STRIDE] = 0u;
+    if (t >= NUM_XCD && t < NUM_XCD + DEFER_PASSES - 1) {
+        head[defer_count_word(t - NUM_XCD + 1)] = 0u;
+        head[defer_head_word(t - NUM_XCD + 1)] = 0u;
+    }
 }
 
-// Next candidate index for the group (uniform), or n when the batch is drained.
 template <int G>
 __device__ __forceinline__ uint32_t next_candidate(const Grp<G> &g, uint32_t n, uint32_t *head, int &slice,
                                                    int home) {
@@ -42,6 +56,84 @@ __device__ __forceinline__ uint32_t next_candidate(const Grp<G> &g, uint32_t n, 
         slice++;
     }
     return n;
+}
+
+// The same queue over a schedule (a list of candidate indices, deepest rows
+// first): slice x holds list positions x, x+8, x+16, ... so that every XCD
+// starts on the front of the schedule.  Returns a list position or n.
+template <int G>
+__device__ __forceinline__ uint32_t next_scheduled(const Grp<G> &g, uint32_t n, uint32_t *head, int &slice,
+                                                   int home) {
+    while (slice < home + NUM_XCD) {
+        const int x = slice % NUM_XCD;
+        int taken = 0;
+        if (g.ln == 0)
+            taken = (int)atomicAdd(head + x * HEAD_STRIDE, 1u);
+        const uint64_t p = (uint64_t)(uint32_t)g.read(taken, 0) * NUM_XCD + (uint64_t)x;
+        if (p < n)
+            return (uint32_t)p;
+        slice++;
+    }
+    return n;
+}
+
+// ---- schedule: candidate indices ordered by decreasing depth ---------------
+// A search's length grows with the number of rows to permute, so deeper
+// candidates are started first (longest-processing-time-first on the depth).
+// aux words [0, 33): candidates per depth; [64, 97): fill cursors.
+constexpr int SCHED_AUX_WORDS = 128;
+
+__global__ void sched_reset_kernel(uint32_t *aux) {
+    for (int t = (int)threadIdx.x; t < SCHED_AUX_WORDS; t += (int)blockDim.x) aux[t] = 0u;
+}
+
+__global__ __launch_bounds__(256) void sched_count_kernel(uint32_t n, const uint32_t *__restrict__ meta,
+                                                          uint32_t *aux) {
+    __shared__ uint32_t h[33];
+    if (threadIdx.x < 33) h[threadIdx.x] = 0u;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t d = meta[i] & 0xffu;
+        atomicAdd(&h[d < 32u ? d : 32u], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 33 && h[threadIdx.x]) atomicAdd(&aux[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void sched_fill_kernel(uint32_t n, const uint32_t *__restrict__ meta,
+                                                         uint32_t *aux, uint32_t *list) {
+    __shared__ uint32_t base[33];
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int d = 32; d >= 0; d--) { base[d] = acc; acc += aux[d]; }
+    }
+    __syncthreads();
+    const int lane = (int)__lane_id();
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        if (i >= n) continue;
+        const uint32_t m = meta[i] & 0xffu;
+        const int d = (int)(m < 32u ? m : 32u);
+        // one cursor atomic per distinct depth in the wave: every lane keeps
+        // the mask of lanes sharing its depth, the lowest of them adds their
+        // number, and each lane pulls the old cursor from that lane (a
+        // per-lane source index, valid wherever the compiler places it)
+        uint64_t same = 0;
+        for (;;) {
+            const int lead = __builtin_amdgcn_readfirstlane(d);
+            const uint64_t m = __builtin_amdgcn_ballot_w64(d == lead);
+            if (d == lead) {
+                same = m;
+                break;
+            }
+        }
+        const int first = __builtin_ctzll(same);
+        uint32_t at = 0;
+        if (lane == first) at = atomicAdd(&aux[64 + d], (uint32_t)__popcll(same));
+        at = (uint32_t)__builtin_amdgcn_ds_bpermute(first << 2, (int)at);
+        const uint64_t below = lane ? (same & ((1ull << lane) - 1)) : 0ull;
+        list[base[d] + at + (uint32_t)__popcll(below)] = i;
+    }
 }
 
 // Waves per SIMD the register allocation of the canonicity kernel must leave
@@ -69,15 +161,35 @@ __global__ __launch_bounds__(1024) ISE_CHECK_OCCUPANCY void check_kernel(uint32_
                                                     const uint32_t *__restrict__ meta,
                                                     OutPtrs o, uint32_t *head, uint32_t max_perms,
                                                     uint32_t rows_cap, const uint8_t *__restrict__ order,
-                                                    int od, uint32_t hunt_budget) {
+                                                    int od, uint32_t hunt_budget,
+                                                    const uint32_t *__restrict__ in_list, uint32_t *out_list,
+                                                    uint32_t budget, int pass,
+                                                    const uint32_t *__restrict__ sched) {
     extern __shared__ uint32_t lds_geo[];
     Grp<G> g;
     // this group's orbit-geometry scratch: rows_cap rows of G words
     uint32_t *geo = lds_geo + ((threadIdx.x >> 6) * (64 / G) + (g.base / G)) * group_scratch_words<G, SETS>(rows_cap);
     const int home = (int)(blockIdx.x % NUM_XCD);
     int slice = home;
+    // pass 0 or 1: the whole batch from the queue; pass p >= 2: the list of
+    // pass p-1.  With out_list, searches past `budget` are listed for the
+    // next pass instead of answered.
+    const uint32_t listed = in_list ? head[defer_count_word(pass - 1)] : 0u;
     for (;;) {
-        const uint32_t ci = next_candidate(g, n, head, slice, home);
+        uint32_t ci;
+        if (in_list) {
+            int taken = 0;
+            if (g.ln == 0) taken = (int)atomicAdd(head + defer_head_word(pass - 1), 1u);
+            const uint32_t k = (uint32_t)g.read(taken, 0);
+            if (k >= listed)
+                break;
+            ci = in_list[k];
+        } else if (sched) {
+            const uint32_t p = next_scheduled(g, n, head, slice, home);
+            ci = p < n ? sched[p] : n;
+        } else {
+            ci = next_candidate(g, n, head, slice, home);
+        }
         if (ci >= n)
             break;
         const uint32_t m = meta[ci];
@@ -89,8 +201,9 @@ __global__ __launch_bounds__(1024) ISE_CHECK_OCCUPANCY void check_kernel(uint32_
             if (g.ln == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
             continue;
         }
-        check_candidate<G, COUNT, HUNT, SETS>(g, ci, m, rows, cells, stride, o, max_perms, geo, order, od,
-                                              hunt_budget);
+        check_candidate<G, COUNT, HUNT, SETS>(g, ci, m, rows, cells, stride, o, budget, geo, order, od,
+                                              hunt_budget, out_list,
+                                              out_list ? head + defer_count_word(pass) : nullptr);
     }
 }
 
@@ -135,8 +248,17 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
                         const uint64_t *cells, const uint32_t *meta,
                         const ise_results &out, uint32_t *head, uint32_t max_perms,
                         uint32_t hunt_budget, const uint8_t *order, uint32_t order_dim,
+                        uint32_t *defer, const uint32_t *defer_budgets, uint32_t *sched,
                         hipStream_t stream) {
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, stream, head);
+    if (sched) {
+        // schedule list (n words) followed by its counters
+        uint32_t *aux = sched + n;
+        const uint32_t sb = (uint32_t)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+        hipLaunchKernelGGL(sched_reset_kernel, dim3(1), dim3(128), 0, stream, aux);
+        hipLaunchKernelGGL(sched_count_kernel, dim3(sb), dim3(256), 0, stream, n, meta, aux);
+        hipLaunchKernelGGL(sched_fill_kernel, dim3(sb), dim3(256), 0, stream, n, meta, aux, sched);
+    }
     const dim3 grid(blocks), block(64 * waves_per_block);
     const OutPtrs o = to_out(out);
     const uint32_t cap = rows_cap_for(group_lanes, stride);
@@ -147,9 +269,31 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
     const bool sets = uses_sets(out, hunt_budget);
     const size_t lds = lds_bytes(group_lanes, waves_per_block, stride, sets);
     const int od = (int)order_dim;
-#define ISE_LAUNCH_V(GL, C, H, S)                                                                        \
+    // passes: the budgets below the limit, in increasing order, then the limit;
+    // pass p lists into half (p-1)&1 of `defer` (2n words) and pass p+1 reads it
+    uint32_t budgets[DEFER_PASSES];
+    int passes = 0;
+    if (defer && defer_budgets)
+        for (int i = 0; i < DEFER_PASSES - 1; i++) {
+            const uint32_t b = defer_budgets[i];
+            if (b > 0 && b < max_perms && (passes == 0 || b > budgets[passes - 1])) budgets[passes++] = b;
+        }
+    budgets[passes++] = max_perms;
+#define ISE_LAUNCH_P(GL, C, H, S, P, IN, OUT, B)                                                         \
     hipLaunchKernelGGL((check_kernel<GL, C, H, S>), grid, block, lds, stream, n, stride, rows, cells, meta, o, \
-                       head, max_perms, cap, order, od, hunt_budget)
+                       head, max_perms, cap, order, od, hunt_budget, IN, OUT, B, P, sched)
+#define ISE_LAUNCH_V(GL, C, H, S)                                                                        \
+    do {                                                                                                 \
+        if (passes == 1) {                                                                               \
+            ISE_LAUNCH_P(GL, C, H, S, 0, nullptr, nullptr, max_perms);                                   \
+        } else {                                                                                         \
+            for (int p = 1; p <= passes; p++) {                                                          \
+                const uint32_t *in = p > 1 ? defer + (size_t)((p - 2) & 1) * n : nullptr;                \
+                uint32_t *outl = p < passes ? defer + (size_t)((p - 1) & 1) * n : nullptr;               \
+                ISE_LAUNCH_P(GL, C, H, S, p, in, outl, budgets[p - 1]);                                  \
+            }                                                                                            \
+        }                                                                                                \
+    } while (0)
 #define ISE_LAUNCH(GL)                                          \
     if (count) {                                                \
         if (hunt) ISE_LAUNCH_V(GL, true, true, false);          \
@@ -169,6 +313,7 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
     }
 #undef ISE_LAUNCH
 #undef ISE_LAUNCH_V
+#undef ISE_LAUNCH_P
     return hipGetLastError();
 }
 

@@ -35,6 +35,14 @@ struct ise_ctx {
     bool pending = false, last_cells = false;
     // ise_submit reports column permutations (ise_report_columns)
     bool report_columns = true, last_columns = true;
+    // deferred long searches (ise_set_defer): first-pass budget and the list
+    uint32_t defer_perms[ise::DEFER_BUDGETS] = {ISE_DEFAULT_DEFER_PERMS, ISE_DEFAULT_DEFER_PERMS2};
+    uint32_t *d_defer = nullptr;
+    size_t defer_cap = 0;
+    // deepest candidates first (ise_set_schedule): list of n + SCHED_EXTRA_WORDS words
+    int schedule = ISE_SCHEDULE_DEEPEST_FIRST;
+    uint32_t *d_sched = nullptr;
+    size_t sched_cap = 0;
     ise_launch_cfg cfg{0, 4, 0, 1u << 26};
     // search order (ise_set_order)
     int order_mode = ISE_ORDER_PLAIN;
@@ -124,9 +132,36 @@ int launch_with(ise_ctx *c, uint32_t blocks, uint32_t n, uint32_t stride, const 
     const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
     const uint32_t budget = c->order_mode == ISE_ORDER_PLAIN ? 0u : c->hunt_budget;
     const uint8_t *order = c->order_mode == ISE_ORDER_LEARNED ? c->d_order : nullptr;
+    uint32_t *defer = (c->defer_perms[0] && n <= c->defer_cap) ? c->d_defer : nullptr;
+    uint32_t *sched = (c->schedule == ISE_SCHEDULE_DEEPEST_FIRST && n <= c->sched_cap) ? c->d_sched : nullptr;
     HIPCHK(c, ise::launch_check(group_lanes, blocks, wpb, n, stride, rows, cells, meta, out, head,
                                 c->cfg.max_perms ? c->cfg.max_perms : (1u << 26), budget, order,
-                                c->order_dim, s));
+                                c->order_dim, defer, c->defer_perms, sched, s));
+    return ISE_OK;
+}
+
+// Schedule list for batches of up to n, grown outside any launch.
+int ensure_sched(ise_ctx *c, size_t n) {
+    if (c->schedule != ISE_SCHEDULE_DEEPEST_FIRST || n <= c->sched_cap) return ISE_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->d_sched) HIPCHK(c, hipFree(c->d_sched));
+    c->d_sched = nullptr;
+    c->sched_cap = 0;
+    HIPCHK(c, hipMalloc(&c->d_sched, (n + ise::SCHED_EXTRA_WORDS) * sizeof(uint32_t)));
+    c->sched_cap = n;
+    return ISE_OK;
+}
+
+// Lists of deferred candidates (two halves of n words) for batches of up to
+// n, grown outside any launch.
+int ensure_defer(ise_ctx *c, size_t n) {
+    if (!c->defer_perms[0] || n <= c->defer_cap) return ISE_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->d_defer) HIPCHK(c, hipFree(c->d_defer));
+    c->d_defer = nullptr;
+    c->defer_cap = 0;
+    HIPCHK(c, hipMalloc(&c->d_defer, 2 * n * sizeof(uint32_t)));
+    c->defer_cap = n;
     return ISE_OK;
 }
 
@@ -185,6 +220,10 @@ int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride) {
         if (e == hipSuccess) e = dalloc(&c->d_cells, n * stride);
         if (e == hipSuccess) e = dalloc(&c->d_meta, n);
         if (e == hipSuccess) e = dalloc(&c->d_head, ISE_QUEUE_WORDS);
+        if (e == hipSuccess) e = dalloc(&c->d_defer, 2 * n);
+        if (e == hipSuccess) c->defer_cap = n;
+        if (e == hipSuccess) e = dalloc(&c->d_sched, n + ise::SCHED_EXTRA_WORDS);
+        if (e == hipSuccess) c->sched_cap = n;
         if (e == hipSuccess) e = dalloc(&c->d.verdict, n);
         if (e == hipSuccess) e = dalloc(&c->d.fail_row, n);
         if (e == hipSuccess) e = dalloc(&c->d.fail_col, n);
@@ -225,7 +264,7 @@ int ise_close(ise_ctx *c) {
     void *bufs[] = {c->d_rows, c->d_cells, c->d_meta, c->d_head, c->d.verdict, c->d.fail_row,
                     c->d.fail_col, c->d.row_to_change, c->d.group_order, c->d.perm_row,
                     c->d.perm_col, c->d.orbits, c->d.n_gens, c->d.counters, c->d.gens,
-                    c->d_kf_tab, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried};
+                    c->d_kf_tab, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried, c->d_defer, c->d_sched};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -411,6 +450,9 @@ int ise_check_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     const int group = c->cfg.group_lanes ? (int)c->cfg.group_lanes : 64;
     HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_defer(c, n);
+    if (rc == ISE_OK) rc = ensure_sched(c, n);
+    if (rc) return rc;
     return launch(c, n, stride, d_rows, d_cells, d_meta, *d_out, d_counter, s, group);
 }
 
@@ -582,6 +624,20 @@ int ise_set_order(ise_ctx *c, int mode, uint32_t hunt_budget, const uint8_t *d_o
     c->d_order = mode == ISE_ORDER_LEARNED ? d_order : nullptr;
     c->order_dim = mode == ISE_ORDER_LEARNED ? order_dim : 0;
     return ISE_OK;
+}
+
+int ise_set_defer(ise_ctx *c, uint32_t first_pass_perms, uint32_t second_pass_perms) {
+    if (!c) return ISE_ERR_ARG;
+    if (second_pass_perms && second_pass_perms <= first_pass_perms) return ISE_ERR_ARG;
+    c->defer_perms[0] = first_pass_perms;
+    c->defer_perms[1] = first_pass_perms ? second_pass_perms : 0u;
+    return ensure_defer(c, c->capacity);
+}
+
+int ise_set_schedule(ise_ctx *c, int schedule) {
+    if (!c || (schedule != ISE_SCHEDULE_QUEUE && schedule != ISE_SCHEDULE_DEEPEST_FIRST)) return ISE_ERR_ARG;
+    c->schedule = schedule;
+    return ensure_sched(c, c->capacity);
 }
 
 int ise_report_columns(ise_ctx *c, int on) {

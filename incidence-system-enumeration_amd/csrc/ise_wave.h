@@ -672,10 +672,15 @@ ISE_DEV bool hunt(const Grp<G> &g, const Staged &c, const uint32_t *geo, const u
 // column-set search, any other the replayed one.  `lds` is the group's
 // scratch: d*G orbit-geometry words, or in SETS mode d*G set words followed by
 // d*G count bytes.
+//
+// With `defer` set, a candidate whose search runs past max_perms is appended
+// to that list (count in *defer_count) and no record is written: it is
+// checked again from the start by a later launch.
 template <int G, bool COUNT, bool HUNT, bool SETS>
 ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const uint64_t *rows,
                              const uint64_t *cells, uint32_t stride, const OutPtrs &o, uint32_t max_perms,
-                             uint32_t *lds, const uint8_t *order, int od, uint32_t hunt_budget) {
+                             uint32_t *lds, const uint8_t *order, int od, uint32_t hunt_budget,
+                             uint32_t *defer, uint32_t *defer_count) {
     static_assert(!(HUNT && SETS), "the witness hunt runs on the replayed search");
     const int l = g.ln;
     const Staged c = load_candidate(g, ci, meta, rows, cells, stride);
@@ -704,6 +709,10 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const 
     if (!found)
         plain_search<G, COUNT>(g, c, geo, ci, o, max_perms, pr, ob, pc, v);
 
+    if (defer && v.verdict == ISE_VERDICT_LIMIT) {
+        if (l == 0) defer[atomicAdd(defer_count, 1u)] = ci;
+        return;
+    }
     // write the record
     if (l == 0) {
         o.verdict[ci] = (uint8_t)v.verdict;

@@ -34,6 +34,8 @@ ERR_HIP = -2
 ERR_NODEV = -3
 ERR_CAPACITY = -4
 ERR_STATE = -5
+DEFAULT_DEFER_PERMS = 16
+DEFAULT_DEFER_PERMS2 = 0
 
 ORDER_PLAIN = 0     # the reference's search alone
 ORDER_HUNT = 1      # budgeted witness hunt in increasing row order, then the plain search
@@ -78,6 +80,8 @@ def lib():
         L.ise_set_order.argtypes = [P, I, U32, P, U32]
         L.ise_use_graphs.argtypes = [P, I]
         L.ise_report_columns.argtypes = [P, I]
+        L.ise_set_defer.argtypes = [P, U32, U32]
+        L.ise_set_schedule.argtypes = [P, I]
         L.ise_graph_stats.argtypes = [P, ctypes.POINTER(U32), ctypes.POINTER(U64)]
         L.ise_autotune.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P, U32, U32, U32, P, P]
         L.ise_sync.argtypes = [P]
@@ -222,6 +226,15 @@ class Context:
     def set_launch(self, group_lanes=0, waves_per_block=4, blocks=0, max_perms=1 << 26):
         cfg = LaunchCfg(group_lanes, waves_per_block, blocks, max_perms)
         self._check(self._L.ise_set_launch(self.h, ctypes.byref(cfg)))
+
+    def set_defer(self, first_pass_perms=DEFAULT_DEFER_PERMS, second_pass_perms=DEFAULT_DEFER_PERMS2):
+        """Launch in passes that defer searches longer than first_pass_perms,
+        then second_pass_perms (0: one launch in queue order)."""
+        self._check(self._L.ise_set_defer(self.h, first_pass_perms, second_pass_perms))
+
+    def set_schedule(self, deepest_first=True):
+        """Start the candidates of a batch deepest first (default) or in index order."""
+        self._check(self._L.ise_set_schedule(self.h, 1 if deepest_first else 0))
 
     def get_launch(self):
         cfg = LaunchCfg()

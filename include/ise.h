@@ -83,6 +83,10 @@ typedef struct {
     uint32_t  gen_cap;
 } ise_results;
 
+/* Permutation budgets of the first and second pass by default (ise_set_defer). */
+#define ISE_DEFAULT_DEFER_PERMS  16u
+#define ISE_DEFAULT_DEFER_PERMS2 0u
+
 /* Launch configuration of the canonicity kernel. */
 typedef struct {
     uint32_t group_lanes;     /* lanes per candidate: 16, 32, 64; 0 = smallest that fits */
@@ -190,6 +194,28 @@ int ise_use_graphs(ise_ctx *ctx, int on);
  * array with ISE_ERR_STATE.  The device-resident entry points decide the same
  * way from whether d_out->perm_col is null. */
 int ise_report_columns(ise_ctx *ctx, int on);
+
+/* Long searches together or in queue order.  With first_pass_perms > 0 a
+ * batch runs in passes: every candidate with that permutation budget; then,
+ * from the start, the candidates that ran past it, with budget
+ * second_pass_perms (0: no limit but the context's max_perms); then, if there
+ * was a second budget, those that ran past it, with no limit.  0 runs one
+ * launch in queue order.  Results are identical either way.  Defaults:
+ * ISE_DEFAULT_DEFER_PERMS, ISE_DEFAULT_DEFER_PERMS2.  The lists of deferred
+ * candidates are a context buffer: batches on one context must not run
+ * concurrently, and the first device-resident batch larger than the context
+ * capacity grows it (hipMalloc).  second_pass_perms must exceed
+ * first_pass_perms when set. */
+int ise_set_defer(ise_ctx *ctx, uint32_t first_pass_perms, uint32_t second_pass_perms);
+
+/* Order in which the candidates of a batch are started (results do not
+ * depend on it): ISE_SCHEDULE_QUEUE in index order, ISE_SCHEDULE_DEEPEST_FIRST
+ * (default) by decreasing depth, so that the long searches of the deepest
+ * partial matrices do not start at the end of the batch.  The schedule list
+ * is a context buffer, like the deferred lists of ise_set_defer. */
+#define ISE_SCHEDULE_QUEUE          0
+#define ISE_SCHEDULE_DEEPEST_FIRST  1
+int ise_set_schedule(ise_ctx *ctx, int schedule);
 /* Number of executable graphs instantiated so far and replays of them. */
 int ise_graph_stats(const ise_ctx *ctx, uint32_t *graphs, uint64_t *replays);
 

@@ -156,3 +156,47 @@ def test_device_path_without_columns(gpu_ctx, oracle):
         assert np.array_equal(dev[f].to_host(), o[f]), f
     for b in list(dev.values()) + [d_rows, d_meta, d_head]:
         b.free()
+
+
+@pytest.mark.parametrize("first,second", [(0, 0), (1, 0), (1, 8), (7, 64), (64, 4096)])
+@pytest.mark.parametrize("columns", [False, True])
+def test_deferred_long_searches_same_results(gpu_ctx, oracle, first, second, columns):
+    """Launches in passes (searches past `first`, then `second` permutations
+    deferred and rerun from the start) give the oracle's results, with and
+    without the column permutation."""
+    print(f"case deferral first={first} second={second} columns={columns}", flush=True)
+    _, _, _, (r1, m1) = oracle.enumerate(16, 6, 2, capture=(1 << 14, 32, 1, 16))
+    r2, m2 = ise_amd.random_extensions(19, 9, 4, 4096, 5, 19, 0xDEF)
+    r3, m3 = oracle.enumerate_sampled(25, 4, 1, 4, 0xDEF, 8, 16, stride=32, cap=1 << 15)[:2]
+    rows, meta = np.concatenate([r1, r2, r3]), np.concatenate([m1, m2, m3])
+    gpu_ctx.set_defer(first, second)
+    try:
+        fields = FIELDS + (("perm_col",) if columns else ())
+        g = gpu_ctx.check_batch(rows, meta, gen_cap=4, fields=fields)
+    finally:
+        gpu_ctx.set_defer()
+    o = oracle.check_batch(rows, meta, gen_cap=4)
+    if columns:
+        assert np.array_equal(g.pop("perm_col"), o["perm_col"])
+    assert_same(g, o, 4, f"deferral {first}")
+
+
+@pytest.mark.parametrize("deepest_first", [True, False])
+def test_schedule_after_smaller_batches(gpu_ctx, oracle, deepest_first):
+    """Deepest-first schedules over consecutive batches of growing size and
+    shuffled depths (every wave of the schedule kernels sees several depths)
+    leave no candidate out and give the oracle's results."""
+    print(f"case schedule deepest_first={deepest_first}", flush=True)
+    gpu_ctx.set_schedule(deepest_first)
+    try:
+        for v, k, lam, n in ((7, 3, 1, 40), (9, 3, 2, 700), (16, 6, 2, 5000)):
+            _, _, _, (rows, meta) = oracle.enumerate(v, k, lam, capture=(1 << 16, 32, 1, v))
+            r2, m2 = ise_amd.random_extensions(v, k, lam, n, 2, v, 0x5C + n)
+            rows, meta = np.concatenate([rows, r2]), np.concatenate([meta, m2])
+            idx = np.random.default_rng(n).permutation(len(meta))
+            rows, meta = rows[idx], meta[idx]
+            g = gpu_ctx.check_batch(rows, meta, gen_cap=4, fields=FIELDS)
+            o = oracle.check_batch(rows, meta, gen_cap=4)
+            assert_same(g, o, 4, f"schedule {(v, k, lam)}")
+    finally:
+        gpu_ctx.set_schedule(True)
