@@ -479,12 +479,62 @@ ISE_DEV bool stage_sets(const Grp<G> &g, const Staged &c, const uint64_t *cells,
     return g.ballot(!ok) == 0;
 }
 
+// Generator of an automorphism found by the column-set search: the
+// permutation it induces on the column orbits of the next row
+// (CanonicityChecker.h:427-450) -- the orbit whose first column is the
+// smallest source column placed in orbit x (columns of one orbit keep their
+// relative order under the ones-first partitions) -- or, for a full matrix
+// with t_saveRowPermutations, the row permutation.  S holds the level-d sets.
+template <int G>
+ISE_DEV void set_generator(const Grp<G> &g, const Staged &c, uint64_t S, int pr, uint32_t ci, const OutPtrs &o,
+                           uint32_t ngen) {
+    const int l = g.ln, d = c.d;
+    if (!o.gens || ngen >= o.gen_cap)
+        return;
+    uint8_t *gp = o.gens + ((uint64_t)ci * o.gen_cap + ngen) * ISE_MAX_COLS;
+    if (!c.row_permut) {
+        const int norb = popc64(c.cell_next);
+        for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
+            const int x = x0 + l;
+            const int src = x < norb ? select_bit(c.cell_next, x) : 0;
+            const int sl = src < G ? src : 0;
+            const uint32_t lo = (uint32_t)g.gather((int)(uint32_t)S, sl);
+            const uint32_t hi = (uint32_t)g.gather((int)(uint32_t)(S >> 32), sl);
+            const uint64_t Sx = ((uint64_t)hi << 32) | lo;
+            const int pcs = Sx ? ctz64(Sx) : 0;
+            gp[x] = (uint8_t)(x < norb ? popc64(c.cell_next & lowmask(pcs)) : 0xff);
+        }
+    } else {
+        for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
+            const int x = x0 + l;
+            const int vv = g.gather(pr, x < G ? x : 0);
+            gp[x] = (uint8_t)(x < d ? vv : 0xff);
+        }
+    }
+}
+
+// Positions i+1.. of the permutation: the rows of `rest` in increasing order
+// (the walk keeps the tail after the last changed position sorted).
+template <int G> ISE_DEV int fill_tail(const Grp<G> &g, int pr, int i, int d, uint32_t rest) {
+    const int l = g.ln;
+    return (l > i && l < d) ? select_bit((uint64_t)rest, l - i - 1) : pr;
+}
+
 // The reference's search (plain order) on the column-set state: the same
 // walk, verdict, witness row permutation, fail row/column, rowToChange,
 // automorphisms, generators, |Aut|, orbits and counters as plain_search.
-// A row refuted at a position other than the stabiliser point moves on to
-// the next larger unused row in place (next_row_perm's `at` branch, which
-// keeps the tail sorted), without leaving the comparison loop.
+//
+// Until the first automorphism the row orbits are trivial, so the walk of
+// next_row_perm (GroupOrder.h:136-248) is a plain depth-first search over
+// row prefixes in increasing row order: a row refuted at position i is
+// followed by the next larger row not used before i; when there is none the
+// search backs up to the deepest position that has one (the walk's
+// reversal and lexicographic successor), which is also where the walk puts
+// its stabiliser point.  That phase runs on a scalar mask U of the rows not
+// used before the current position, with positions after it implied (the
+// unused rows in increasing order), and writes out the full permutation only
+// when it stops.  From the first automorphism on, the replayed walk takes
+// over with the permutation, stabiliser point and orbits the walk would have.
 template <int G, bool COUNT>
 ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const uint8_t *cnt, uint32_t ci,
                         const OutPtrs &o, uint32_t max_perms, int &pr, int &ob, Verdict &v) {
@@ -493,13 +543,143 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
     st.stab = d - 1;
     st.stab_aut = EMAX;
     st.order = 1;
+    uint64_t S = 0;
     int row = EMAX;
+    bool aut = false;
+
+    // ---- depth-first phase -------------------------------------------------
+    {
+        int i = d - 1;                         // start: the identity, backed up from its last row
+        uint32_t U = 1u << (d - 1);            // rows not used before position i (perm[i] included)
+        uint32_t left = max_perms;             // permutations still allowed
+        bool first = true;
+        for (;;) {
+            // back up to the deepest position with a larger unused row
+            int p = g.uni(i), nxt = -1;
+            U = (uint32_t)g.uni((int)U);
+            for (;;) {
+                const uint32_t above = U & (0xfffffffeu << g.read(pr, p));
+                if (above) { nxt = ctz64(above); break; }
+                if (p == 0) break;
+                p--;
+                U |= 1u << g.read(pr, p);
+            }
+            if (nxt < 0 || p < (first ? c.fixed : 0)) {
+                // exhausted: the walk ends on the descending permutation
+                // (on the identity when it could not move at all)
+                if (!first && l < d) pr = d - 1 - l;
+                break;
+            }
+            first = false;
+            if (st.stab > p) st.stab = p;
+            i = g.uni(p);
+            int cur = g.uni(nxt);
+            if (l == i) pr = cur;
+            S = stk[i * G + l];
+            int cR = cnt[i * G + l];
+            uint64_t C = g.uni64(g.read64(c.cellv, i));
+            bool fresh = true;                 // cur makes a new permutation (not a descent)
+            int why = 0;                       // 1 witness, 2 automorphism, 3 limit
+            for (;;) {
+                if (fresh) {
+                    if (left == 0) { why = 3; break; }
+                    left--;
+                }
+                // rows tried at position i until one is not refuted: only cur changes
+                uint64_t bad, more;
+                for (;;) {
+                    const uint64_t Rp = g.read64(c.rowv, cur);
+                    const int cP = popc64(Rp & S);
+                    bad = g.ballot(cP != cR) & C;
+                    more = g.ballot(cR > cP);
+                    if constexpr (COUNT) {
+                        v.rows_cmp++;
+                        if (bad) {
+                            const int lim = ctz64(bad);
+                            v.orbs_cmp += popc64(C & lowmask(lim + 1));
+                            const uint64_t after = C & ~lowmask(lim + 1);
+                            v.cells_cmp += (uint32_t)(after ? ctz64(after) : c.b);
+                        } else {
+                            v.orbs_cmp += popc64(C);
+                            v.cells_cmp += (uint32_t)c.b;
+                        }
+                    }
+                    if (!bad || !(more & bad & (0ull - bad)))
+                        break;                 // equal, or larger (a witness)
+                    const uint32_t above = U & (0xfffffffeu << cur);
+                    if (!above || left == 0)
+                        break;                 // no larger unused row, or out of permutations
+                    left--;
+                    cur = ctz64(above);
+                }
+                if (l == i) pr = cur;
+                if (!bad) {
+                    // equal: descend to the next position
+                    const uint64_t Ro = g.read64(c.rowv, i);
+                    const uint64_t Rp = g.read64(c.rowv, cur);
+                    S &= ((Ro >> l) & 1ull) ? Rp : ~Rp;
+                    U &= ~(1u << cur);
+                    if (++i == d) { why = 2; break; }
+                    cur = ctz64(U);
+                    if (l == i) pr = cur;
+                    stk[i * G + l] = S;
+                    cR = cnt[i * G + l];
+                    C = g.uni64(g.read64(c.cellv, i));
+                    i = g.uni(i);
+                    cur = g.uni(cur);
+                    U = (uint32_t)g.uni((int)U);
+                    fresh = false;
+                    continue;
+                }
+                if (!(more & bad & (0ull - bad))) {
+                    // the permuted matrix is larger: witness
+                    pr = fill_tail(g, pr, i, d, U & ~(1u << cur));
+                    v.verdict = 0;
+                    v.frow = i;
+                    v.fcol = ctz64(bad);
+                    v.rtc = row_to_change(g, pr, d, i);
+                    why = 1;
+                    break;
+                }
+                const uint32_t above = U & (0xfffffffeu << cur);
+                if (above) {
+                    // out of permutations on the next larger row
+                    cur = ctz64(above);
+                    if (l == i) pr = cur;
+                    why = 3;
+                }
+                break;                         // no larger unused row: back up
+            }
+            if (why == 3) {
+                pr = fill_tail(g, pr, i, d, U & ~(1u << cur));
+                v.verdict = ISE_VERDICT_LIMIT;
+            }
+            if (why == 2) aut = true;
+            if (why) break;
+        }
+        v.perms = max_perms - left;
+    }
+    if (!aut) {
+        if (v.verdict == 1 && c.row_permut)
+            grow_order(g, st, ob, d);
+        v.order = st.order;
+        return;
+    }
+
+    // ---- replayed walk from the first automorphism on ------------------------
     for (;;) {
+        if (aut) {
+            set_generator(g, c, S, pr, ci, o, v.ngen);
+            v.ngen++;
+            add_automorphism(g, st, pr, ob, d, c.row_permut);
+            row = IMAX;
+            aut = false;
+        }
         row = g.uni(next_row_perm(g, st, pr, ob, d, row, c.fixed));
         if (row == EMAX)
             break;
         if (++v.perms > max_perms) { v.verdict = ISE_VERDICT_LIMIT; break; }
-        uint64_t S = stk[row * G + l];
+        S = stk[row * G + l];
         int cR = cnt[row * G + l];
         uint64_t C = g.read64(c.cellv, row);
         int outcome = 0, dec = 0;
@@ -557,35 +737,7 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
             v.rtc = row_to_change(g, pr, d, row);
             break;
         }
-        // automorphism: S holds the level-d sets
-        if (o.gens && v.ngen < o.gen_cap) {
-            uint8_t *gp = o.gens + ((uint64_t)ci * o.gen_cap + v.ngen) * ISE_MAX_COLS;
-            if (!c.row_permut) {
-                // orbit of the next row whose first column is the smallest source column
-                // placed in orbit x (CanonicityChecker.h:427-450; columns of one orbit
-                // keep their relative order under the ones-first partitions)
-                const int norb = popc64(c.cell_next);
-                for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
-                    const int x = x0 + l;
-                    const int src = x < norb ? select_bit(c.cell_next, x) : 0;
-                    const int sl = src < G ? src : 0;
-                    const uint32_t lo = (uint32_t)g.gather((int)(uint32_t)S, sl);
-                    const uint32_t hi = (uint32_t)g.gather((int)(uint32_t)(S >> 32), sl);
-                    const uint64_t Sx = ((uint64_t)hi << 32) | lo;
-                    const int pcs = Sx ? ctz64(Sx) : 0;
-                    gp[x] = (uint8_t)(x < norb ? popc64(c.cell_next & lowmask(pcs)) : 0xff);
-                }
-            } else {
-                for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
-                    const int x = x0 + l;
-                    const int vv = g.gather(pr, x < G ? x : 0);
-                    gp[x] = (uint8_t)(x < d ? vv : 0xff);
-                }
-            }
-        }
-        v.ngen++;
-        add_automorphism(g, st, pr, ob, d, c.row_permut);
-        row = IMAX;
+        aut = true;                   // automorphism: S holds the level-d sets
     }
     if (v.verdict == 1 && c.row_permut)
         grow_order(g, st, ob, d);
