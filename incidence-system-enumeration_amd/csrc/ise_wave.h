@@ -520,6 +520,50 @@ template <int G> ISE_DEV int fill_tail(const Grp<G> &g, int pr, int i, int d, ui
     return (l > i && l < d) ? select_bit((uint64_t)rest, l - i - 1) : pr;
 }
 
+// Rows tried at one position of the depth-first phase: compares row `cur`
+// and, while it is refuted, moves on to the next larger unused row (the bits
+// of U above cur).  Stops on an equal row (bad = 0), a larger one (a witness)
+// or when no larger unused row is left; with CHECKED also when `left`
+// permutations have been used.  Returns the number of rows moved on to;
+// bad and more are the last comparison's ballots.
+template <int G, bool COUNT, bool CHECKED>
+ISE_DEV uint32_t sibling_run(const Grp<G> &g, const Staged &c, uint64_t S, int cR, uint64_t C, uint32_t U,
+                             uint32_t left, int &cur, uint64_t &bad, uint64_t &more, Verdict &v) {
+    uint32_t k = 0;
+    for (;;) {
+        const uint64_t Rp = g.read64(c.rowv, cur);
+        const int cP = popc64(Rp & S);
+        bad = g.ballot(cP != cR) & C;
+        more = g.ballot(cR > cP);
+        if constexpr (COUNT) {
+            v.rows_cmp++;
+            if (bad) {
+                const int lim = ctz64(bad);
+                v.orbs_cmp += popc64(C & lowmask(lim + 1));
+                const uint64_t after = C & ~lowmask(lim + 1);
+                v.cells_cmp += (uint32_t)(after ? ctz64(after) : c.b);
+            } else {
+                v.orbs_cmp += popc64(C);
+                v.cells_cmp += (uint32_t)c.b;
+            }
+        }
+        if (!bad)
+            break;                             // equal
+        if (!((more >> ctz64(bad)) & 1ull))
+            break;                             // larger: a witness
+        const uint32_t above = U & (0xfffffffeu << cur);
+        if (!above)
+            break;                             // no larger unused row
+        if constexpr (CHECKED) {
+            if (k == left)
+                break;                         // out of permutations
+        }
+        k++;
+        cur = ctz64(above);
+    }
+    return k;
+}
+
 // The reference's search (plain order) on the column-set state: the same
 // walk, verdict, witness row permutation, fail row/column, rowToChange,
 // automorphisms, generators, |Aut|, orbits and counters as plain_search.
@@ -587,31 +631,10 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
                 }
                 // rows tried at position i until one is not refuted: only cur changes
                 uint64_t bad, more;
-                for (;;) {
-                    const uint64_t Rp = g.read64(c.rowv, cur);
-                    const int cP = popc64(Rp & S);
-                    bad = g.ballot(cP != cR) & C;
-                    more = g.ballot(cR > cP);
-                    if constexpr (COUNT) {
-                        v.rows_cmp++;
-                        if (bad) {
-                            const int lim = ctz64(bad);
-                            v.orbs_cmp += popc64(C & lowmask(lim + 1));
-                            const uint64_t after = C & ~lowmask(lim + 1);
-                            v.cells_cmp += (uint32_t)(after ? ctz64(after) : c.b);
-                        } else {
-                            v.orbs_cmp += popc64(C);
-                            v.cells_cmp += (uint32_t)c.b;
-                        }
-                    }
-                    if (!bad || !(more & bad & (0ull - bad)))
-                        break;                 // equal, or larger (a witness)
-                    const uint32_t above = U & (0xfffffffeu << cur);
-                    if (!above || left == 0)
-                        break;                 // no larger unused row, or out of permutations
-                    left--;
-                    cur = ctz64(above);
-                }
+                if (left >= 32u)               // at most 31 more rows at one position
+                    left -= sibling_run<G, COUNT, false>(g, c, S, cR, C, U, left, cur, bad, more, v);
+                else
+                    left -= sibling_run<G, COUNT, true>(g, c, S, cR, C, U, left, cur, bad, more, v);
                 if (l == i) pr = cur;
                 if (!bad) {
                     // equal: descend to the next position
@@ -631,7 +654,7 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
                     fresh = false;
                     continue;
                 }
-                if (!(more & bad & (0ull - bad))) {
+                if (!((more >> ctz64(bad)) & 1ull)) {
                     // the permuted matrix is larger: witness
                     pr = fill_tail(g, pr, i, d, U & ~(1u << cur));
                     v.verdict = 0;

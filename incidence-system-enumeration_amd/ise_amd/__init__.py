@@ -34,7 +34,7 @@ ERR_HIP = -2
 ERR_NODEV = -3
 ERR_CAPACITY = -4
 ERR_STATE = -5
-DEFAULT_DEFER_PERMS = 16
+DEFAULT_DEFER_PERMS = 0
 DEFAULT_DEFER_PERMS2 = 0
 
 ORDER_PLAIN = 0     # the reference's search alone

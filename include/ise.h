@@ -84,7 +84,7 @@ typedef struct {
 } ise_results;
 
 /* Permutation budgets of the first and second pass by default (ise_set_defer). */
-#define ISE_DEFAULT_DEFER_PERMS  16u
+#define ISE_DEFAULT_DEFER_PERMS  0u
 #define ISE_DEFAULT_DEFER_PERMS2 0u
 
 /* Launch configuration of the canonicity kernel. */
@@ -199,13 +199,12 @@ int ise_report_columns(ise_ctx *ctx, int on);
  * batch runs in passes: every candidate with that permutation budget; then,
  * from the start, the candidates that ran past it, with budget
  * second_pass_perms (0: no limit but the context's max_perms); then, if there
- * was a second budget, those that ran past it, with no limit.  0 runs one
- * launch in queue order.  Results are identical either way.  Defaults:
- * ISE_DEFAULT_DEFER_PERMS, ISE_DEFAULT_DEFER_PERMS2.  The lists of deferred
- * candidates are a context buffer: batches on one context must not run
- * concurrently, and the first device-resident batch larger than the context
- * capacity grows it (hipMalloc).  second_pass_perms must exceed
- * first_pass_perms when set. */
+ * was a second budget, those that ran past it, with no limit.  0 (the
+ * default, ISE_DEFAULT_DEFER_PERMS) runs one launch.  Results are identical
+ * either way.  The lists of deferred candidates are a context buffer: batches
+ * on one context must not run concurrently, and the first device-resident
+ * batch larger than the context capacity grows it (hipMalloc).
+ * second_pass_perms must exceed first_pass_perms when set. */
 int ise_set_defer(ise_ctx *ctx, uint32_t first_pass_perms, uint32_t second_pass_perms);
 
 /* Order in which the candidates of a batch are started (results do not
