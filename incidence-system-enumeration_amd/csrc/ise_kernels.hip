@@ -102,37 +102,29 @@ __global__ __launch_bounds__(256) void sched_count_kernel(uint32_t n, const uint
 
 __global__ __launch_bounds__(256) void sched_fill_kernel(uint32_t n, const uint32_t *__restrict__ meta,
                                                          uint32_t *aux, uint32_t *list) {
-    __shared__ uint32_t base[33];
+    // per tile of 256 candidates: rank inside the block by LDS atomics, then
+    // one global cursor atomic per depth present in the tile
+    __shared__ uint32_t base[33], tile_cnt[33], tile_off[33];
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
         for (int d = 32; d >= 0; d--) { base[d] = acc; acc += aux[d]; }
     }
-    __syncthreads();
-    const int lane = (int)__lane_id();
     for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        if (threadIdx.x < 33) tile_cnt[threadIdx.x] = 0u;
+        __syncthreads();
         const uint32_t i = i0 + threadIdx.x;
-        if (i >= n) continue;
-        const uint32_t m = meta[i] & 0xffu;
-        const int d = (int)(m < 32u ? m : 32u);
-        // one cursor atomic per distinct depth in the wave: every lane keeps
-        // the mask of lanes sharing its depth, the lowest of them adds their
-        // number, and each lane pulls the old cursor from that lane (a
-        // per-lane source index, valid wherever the compiler places it)
-        uint64_t same = 0;
-        for (;;) {
-            const int lead = __builtin_amdgcn_readfirstlane(d);
-            const uint64_t m = __builtin_amdgcn_ballot_w64(d == lead);
-            if (d == lead) {
-                same = m;
-                break;
-            }
+        uint32_t d = 0, rank = 0;
+        if (i < n) {
+            const uint32_t m = meta[i] & 0xffu;
+            d = m < 32u ? m : 32u;
+            rank = atomicAdd(&tile_cnt[d], 1u);
         }
-        const int first = __builtin_ctzll(same);
-        uint32_t at = 0;
-        if (lane == first) at = atomicAdd(&aux[64 + d], (uint32_t)__popcll(same));
-        at = (uint32_t)__builtin_amdgcn_ds_bpermute(first << 2, (int)at);
-        const uint64_t below = lane ? (same & ((1ull << lane) - 1)) : 0ull;
-        list[base[d] + at + (uint32_t)__popcll(below)] = i;
+        __syncthreads();
+        if (threadIdx.x < 33 && tile_cnt[threadIdx.x])
+            tile_off[threadIdx.x] = atomicAdd(&aux[64 + threadIdx.x], tile_cnt[threadIdx.x]);
+        __syncthreads();
+        if (i < n) list[base[d] + tile_off[d] + rank] = i;
+        __syncthreads();
     }
 }
 
