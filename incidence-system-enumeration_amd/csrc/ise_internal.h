@@ -13,7 +13,19 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
                         const ise_results &out, uint32_t *head, uint32_t max_perms,
                         uint32_t hunt_budget, const uint8_t *order, uint32_t order_dim,
                         uint32_t *defer, const uint32_t *defer_budgets, uint32_t *sched,
-                        hipStream_t stream);
+                        hipStream_t stream, int pass_lo = 0, int pass_hi = 0);
+
+// One pass of a gated launch on HBM-resident inputs (the learned order's
+// batching): pass 1 runs the plain search on every candidate with a budget of
+// `gate` permutations and lists those that exceed it; pass 2 runs the listed
+// ones from the start, with the witness hunt in `order` when hunt_budget > 0.
+// *list receives the list (its count: defer_count_ptr(head, 1)).
+int launch_gated_pass(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta,
+                      const ise_results &out, uint32_t *head, uint32_t gate, int pass, uint32_t hunt_budget,
+                      const uint8_t *order, uint32_t order_dim, const uint32_t **list);
+
+// Word of the queue buffer that counts the candidates pass `pass` deferred.
+uint32_t *defer_count_ptr(uint32_t *head, int pass);
 
 // Words of the schedule buffer beyond the n list entries (depth counters).
 constexpr uint32_t SCHED_EXTRA_WORDS = 128;

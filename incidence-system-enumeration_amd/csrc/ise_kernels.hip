@@ -232,8 +232,11 @@ size_t lds_bytes(int group_lanes, uint32_t waves_per_block, uint32_t stride, boo
 }
 
 bool uses_sets(const ise_results &out, uint32_t hunt_budget) {
-    return out.perm_col == nullptr && hunt_budget == 0;
+    (void)hunt_budget;
+    return out.perm_col == nullptr;
 }
+
+uint32_t *defer_count_ptr(uint32_t *head, int pass) { return head + defer_count_word(pass); }
 
 hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_block,
                         uint32_t n, uint32_t stride, const uint64_t *rows,
@@ -241,9 +244,13 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
                         const ise_results &out, uint32_t *head, uint32_t max_perms,
                         uint32_t hunt_budget, const uint8_t *order, uint32_t order_dim,
                         uint32_t *defer, const uint32_t *defer_budgets, uint32_t *sched,
-                        hipStream_t stream) {
-    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, stream, head);
-    if (sched) {
+                        hipStream_t stream, int pass_lo, int pass_hi) {
+    // passes pass_lo..pass_hi of the launch (1-based; 0, 0: all); the queue
+    // reset and the schedule belong to the first
+    const bool from_start = pass_lo <= 1;
+    if (from_start)
+        hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, stream, head);
+    if (sched && from_start) {
         // schedule list (n words) followed by its counters
         uint32_t *aux = sched + n;
         const uint32_t sb = (uint32_t)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
@@ -280,6 +287,7 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
             ISE_LAUNCH_P(GL, C, H, S, 0, nullptr, nullptr, max_perms);                                   \
         } else {                                                                                         \
             for (int p = 1; p <= passes; p++) {                                                          \
+                if ((pass_lo && p < pass_lo) || (pass_hi && p > pass_hi)) continue;                      \
                 const uint32_t *in = p > 1 ? defer + (size_t)((p - 2) & 1) * n : nullptr;                \
                 uint32_t *outl = p < passes ? defer + (size_t)((p - 1) & 1) * n : nullptr;               \
                 ISE_LAUNCH_P(GL, C, H, S, p, in, outl, budgets[p - 1]);                                  \
@@ -288,11 +296,13 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
     } while (0)
 #define ISE_LAUNCH(GL)                                          \
     if (count) {                                                \
-        if (hunt) ISE_LAUNCH_V(GL, true, true, false);          \
+        if (hunt && sets) ISE_LAUNCH_V(GL, true, true, true);   \
+        else if (hunt) ISE_LAUNCH_V(GL, true, true, false);     \
         else if (sets) ISE_LAUNCH_V(GL, true, false, true);     \
         else ISE_LAUNCH_V(GL, true, false, false);              \
     } else {                                                    \
-        if (hunt) ISE_LAUNCH_V(GL, false, true, false);         \
+        if (hunt && sets) ISE_LAUNCH_V(GL, false, true, true);  \
+        else if (hunt) ISE_LAUNCH_V(GL, false, true, false);    \
         else if (sets) ISE_LAUNCH_V(GL, false, false, true);    \
         else ISE_LAUNCH_V(GL, false, false, false);             \
     }

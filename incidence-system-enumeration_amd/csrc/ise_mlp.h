@@ -33,9 +33,11 @@ hipError_t frag_pack(const uint16_t *wb, uint16_t *frag, hipStream_t s);
 // LDS of one workgroup: the weight fragments (40 KiB) plus the orbit
 // geometry of each wave's candidate (8 KiB per wave).
 size_t learned_order_lds(uint32_t waves_per_block);
+// With `list`, only the candidates list[0 .. *list_count) are scored.
 hipError_t learned_order(uint32_t blocks, uint32_t waves_per_block, uint32_t n, uint32_t stride,
                          const uint64_t *rows, const uint32_t *meta, uint32_t od, const uint16_t *frag,
-                         const float *w, uint8_t *order, float *score, hipStream_t s);
+                         const float *w, uint8_t *order, float *score, hipStream_t s,
+                         const uint32_t *list = nullptr, const uint32_t *list_count = nullptr);
 hipError_t learned_order_occupancy(uint32_t waves_per_block, int *blocks_per_cu);
 hipError_t order_from_scores(uint32_t n, const uint32_t *meta, uint32_t od, const float *score, long score_ld,
                              uint8_t *order, hipStream_t s);

@@ -367,7 +367,9 @@ __global__ __launch_bounds__(256) void learned_order_kernel(uint32_t n, uint32_t
                                                             const uint32_t *__restrict__ meta, uint32_t od,
                                                             const uint16_t *__restrict__ frag_g,
                                                             const float *__restrict__ w, uint8_t *__restrict__ order,
-                                                            float *__restrict__ score) {
+                                                            float *__restrict__ score,
+                                                            const uint32_t *__restrict__ list,
+                                                            const uint32_t *__restrict__ list_count) {
     extern __shared__ uint32_t lds[];
     bf16x4 *frag = (bf16x4 *)lds;                                       // FRAG_TILES * 64 * 8 bytes
     uint32_t *geo = lds + FRAG_TILES * 64 * 2 + (threadIdx.x >> 6) * ISE_MAX_ROWS * 64;
@@ -380,7 +382,11 @@ __global__ __launch_bounds__(256) void learned_order_kernel(uint32_t n, uint32_t
     Grp<64> g;
     const int l = g.ln, q = l >> 4, r16 = l & 15;
     const uint32_t wpb = blockDim.x >> 6;
-    for (uint32_t ci = blockIdx.x * wpb + (threadIdx.x >> 6); ci < n; ci += gridDim.x * wpb) {
+    // every candidate, or only those of `list` (list_count of them)
+    const uint32_t total = list ? *list_count : n;
+    for (uint32_t k = blockIdx.x * wpb + (threadIdx.x >> 6); k < total; k += gridDim.x * wpb) {
+        const uint32_t ci = list ? list[k] : k;
+        if (ci >= n) continue;
         const uint32_t m = meta[ci];
         const int d = m & 0xff, b = (m >> 8) & 0xff;
         uint8_t *oc = order + (uint64_t)ci * od * od;
@@ -435,10 +441,10 @@ size_t learned_order_lds(uint32_t wpb) {
 
 hipError_t learned_order(uint32_t blocks, uint32_t wpb, uint32_t n, uint32_t stride, const uint64_t *rows,
                          const uint32_t *meta, uint32_t od, const uint16_t *frag, const float *w, uint8_t *order,
-                         float *score, hipStream_t s) {
+                         float *score, hipStream_t s, const uint32_t *list, const uint32_t *list_count) {
     if (wpb < 1 || wpb > 4) return hipErrorInvalidValue;
     hipLaunchKernelGGL(learned_order_kernel, dim3(blocks), dim3(64 * wpb), learned_order_lds(wpb), s, n, stride, rows,
-                       meta, od, frag, w, order, score);
+                       meta, od, frag, w, order, score, list, list_count);
     return hipGetLastError();
 }
 

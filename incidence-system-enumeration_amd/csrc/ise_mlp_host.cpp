@@ -25,6 +25,9 @@ struct ise_mlp {
     float *scalar = nullptr;
     int order_blocks = 0;
     uint32_t order_wpb = 4;
+    // learned-order batches: candidates decided within this many plain
+    // permutations are not scored (ise_mlp_set_gate; 0 scores all)
+    uint32_t gate_perms = ISE_DEFAULT_LEARNED_GATE;
 };
 
 namespace {
@@ -120,12 +123,14 @@ int adam_step(ise_mlp *m, float lr) {
 }
 
 int order_launch(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows, const uint32_t *d_meta, uint32_t od,
-                 uint8_t *d_order, float *d_score) {
+                 uint8_t *d_order, float *d_score, const uint32_t *list = nullptr,
+                 const uint32_t *list_count = nullptr) {
     uint32_t blocks = (uint32_t)m->order_blocks;
     const uint32_t need = (n + m->order_wpb - 1) / m->order_wpb;
     if (blocks > need) blocks = need;
     if (blocks == 0) blocks = 1;
-    CK(learned_order(blocks, m->order_wpb, n, stride, d_rows, d_meta, od, m->frag, m->w, d_order, d_score, m->s));
+    CK(learned_order(blocks, m->order_wpb, n, stride, d_rows, d_meta, od, m->frag, m->w, d_order, d_score, m->s,
+                     list, list_count));
     return ISE_OK;
 }
 
@@ -304,9 +309,22 @@ int ise_mlp_check_batch_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, co
         if (hipEventCreate(&e) != hipSuccess) rc = ISE_ERR_HIP;
     if (!rc && hipEventRecord(ev[0], m->s) != hipSuccess) rc = ISE_ERR_HIP;
     for (uint32_t i = 0; i < repeats && !rc; i++) {
-        rc = order_launch(m, n, stride, d_rows, d_meta, od, d_order, nullptr);
-        if (!rc && hipEventRecord(ev[2 * i + 1], m->s) != hipSuccess) rc = ISE_ERR_HIP;
-        if (!rc) rc = ise_check_batch_device(c, n, stride, d_rows, nullptr, d_meta, d_out, d_counter, nullptr);
+        if (m->gate_perms) {
+            // plain search with a small budget for all; the network orders only
+            // the candidates it leaves open, which then run hunt + plain search
+            const uint32_t *list = nullptr;
+            rc = launch_gated_pass(c, n, stride, d_rows, d_meta, *d_out, d_counter, m->gate_perms, 1, 0, nullptr,
+                                   od, &list);
+            if (!rc) rc = order_launch(m, n, stride, d_rows, d_meta, od, d_order, nullptr, list,
+                                       defer_count_ptr(d_counter, 1));
+            if (!rc && hipEventRecord(ev[2 * i + 1], m->s) != hipSuccess) rc = ISE_ERR_HIP;
+            if (!rc) rc = launch_gated_pass(c, n, stride, d_rows, d_meta, *d_out, d_counter, m->gate_perms, 2,
+                                            hunt_budget, d_order, od, nullptr);
+        } else {
+            rc = order_launch(m, n, stride, d_rows, d_meta, od, d_order, nullptr);
+            if (!rc && hipEventRecord(ev[2 * i + 1], m->s) != hipSuccess) rc = ISE_ERR_HIP;
+            if (!rc) rc = ise_check_batch_device(c, n, stride, d_rows, nullptr, d_meta, d_out, d_counter, nullptr);
+        }
         if (!rc && hipEventRecord(ev[2 * i + 2], m->s) != hipSuccess) rc = ISE_ERR_HIP;
     }
     if (!rc && hipStreamSynchronize(m->s) != hipSuccess) rc = ISE_ERR_HIP;
@@ -319,6 +337,12 @@ int ise_mlp_check_batch_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, co
         if (e) (void)hipEventDestroy(e);
     const int rc2 = ise_set_order(c, pmode, pbudget, porder, pdim);
     return rc ? rc : rc2;
+}
+
+int ise_mlp_set_gate(ise_mlp *m, uint32_t gate_perms) {
+    if (!m) return ISE_ERR_ARG;
+    m->gate_perms = gate_perms;
+    return ISE_OK;
 }
 
 int ise_mlp_witness_labels(uint32_t n, uint32_t od, const uint32_t *meta, const uint8_t *verdict,

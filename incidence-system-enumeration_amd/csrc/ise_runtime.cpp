@@ -177,6 +177,33 @@ int launch(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const 
 } // namespace
 
 namespace ise {
+int launch_gated_pass(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta,
+                      const ise_results &out, uint32_t *head, uint32_t gate, int pass, uint32_t hunt_budget,
+                      const uint8_t *order, uint32_t order_dim, const uint32_t **list) {
+    if (!c || gate == 0 || (pass != 1 && pass != 2) || !head || !out.verdict) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    // the lists live in the deferral buffer even when deferral is off
+    const uint32_t saved = c->defer_perms[0];
+    c->defer_perms[0] = gate;
+    int rc = ensure_defer(c, n);
+    c->defer_perms[0] = saved;
+    if (rc == ISE_OK) rc = ensure_sched(c, n);
+    if (rc) return rc;
+    const int group = c->cfg.group_lanes ? (int)c->cfg.group_lanes : 64;
+    uint32_t blocks = 0;
+    rc = grid_blocks(c, n, stride, group, out, &blocks);
+    if (rc) return rc;
+    const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
+    const uint32_t budgets[DEFER_BUDGETS] = {gate, 0u};
+    uint32_t *sched = c->schedule == ISE_SCHEDULE_DEEPEST_FIRST ? c->d_sched : nullptr;
+    HIPCHK(c, launch_check(group, blocks, wpb, n, stride, rows, nullptr, meta, out, head,
+                           c->cfg.max_perms ? c->cfg.max_perms : (1u << 26), pass == 2 ? hunt_budget : 0u,
+                           pass == 2 ? order : nullptr, order_dim, c->d_defer, budgets, sched, c->stream, pass,
+                           pass));
+    if (list) *list = c->d_defer;
+    return ISE_OK;
+}
+
 uint32_t ctx_stride(const ise_ctx *c) { return c->stride; }
 uint32_t ctx_capacity(const ise_ctx *c) { return c->capacity; }
 hipStream_t ctx_stream(const ise_ctx *c) { return c->stream; }

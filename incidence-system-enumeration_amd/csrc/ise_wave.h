@@ -595,7 +595,7 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
     {
         int i = d - 1;                         // start: the identity, backed up from its last row
         uint32_t U = 1u << (d - 1);            // rows not used before position i (perm[i] included)
-        uint32_t left = max_perms;             // permutations still allowed
+        uint32_t left = max_perms > v.perms ? max_perms - v.perms : 0u;   // permutations still allowed
         bool first = true;
         for (;;) {
             // back up to the deepest position with a larger unused row
@@ -680,7 +680,7 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
             if (why == 2) aut = true;
             if (why) break;
         }
-        v.perms = max_perms - left;
+        v.perms = max_perms - left;            // the hunt's and this phase's
     }
     if (!aut) {
         if (v.verdict == 1 && c.row_permut)
@@ -840,6 +840,93 @@ ISE_DEV bool hunt(const Grp<G> &g, const Staged &c, const uint32_t *geo, const u
     }
 }
 
+// The witness hunt on the column-set state: the same depth-first order,
+// budget, counters and witness as `hunt`, comparisons as in set_search.  The
+// set stack holds the identity sets on entry (as stage_sets leaves it) and
+// the sets of the hunt's prefixes on return.
+template <int G, bool COUNT>
+ISE_DEV bool hunt_sets(const Grp<G> &g, const Staged &c, uint64_t *stk, const uint8_t *cnt, const uint8_t *order,
+                       int od, uint32_t budget, int &pr, Verdict &v) {
+    const int l = g.ln, d = c.d;
+    int cursor = 0;            // lane i: next index into order row i
+    uint32_t used = 0;         // rows placed at positions 0..i-1
+    int i = 0;
+    int idlen = 0;             // positions 0..idlen-1 hold the identity
+    uint32_t nodes = 0;
+    for (;;) {
+        i = g.uni(i);
+        if (i == d) {          // the whole matrix matched: backtrack
+            i--;
+            used &= ~(1u << g.read(pr, i));
+            if (idlen > i) idlen = i;
+            continue;
+        }
+        int k = g.read(cursor, i);
+        int j = -1;
+        while (k < d) {
+            const int cand = order ? (int)order[i * od + k] : k;
+            k++;
+            // entries that name no row of this candidate are skipped
+            if (cand < d && !((used >> cand) & 1)) { j = cand; break; }
+        }
+        if (l == i) cursor = k;
+        if (j < 0) {           // depth exhausted
+            if (i == 0) return false;
+            i--;
+            used &= ~(1u << g.read(pr, i));
+            if (idlen > i) idlen = i;
+            continue;
+        }
+        j = g.uni(j);
+        if (l == i) pr = j;
+        if (j == i && idlen == i && nodes == 0) {
+            // identity prefix extended by its own row before any comparison
+            // (the set stack already holds the identity sets)
+            used |= 1u << j;
+            idlen = ++i;
+            if (l == i) cursor = 0;
+            continue;
+        }
+        if (nodes == budget) return false;
+        nodes++;
+        v.perms++;
+        const uint64_t S = stk[i * G + l];
+        const int cR = cnt[i * G + l];
+        const uint64_t C = g.uni64(g.read64(c.cellv, i));
+        const uint64_t Rp = g.read64(c.rowv, j);
+        const int cP = popc64(Rp & S);
+        const uint64_t bad = g.ballot(cP != cR) & C;
+        const uint64_t more = g.ballot(cR > cP);
+        if constexpr (COUNT) v.rows_cmp++;
+        if (bad) {
+            const int lim = ctz64(bad);
+            if constexpr (COUNT) {
+                v.orbs_cmp += popc64(C & lowmask(lim + 1));
+                const uint64_t after = C & ~lowmask(lim + 1);
+                v.cells_cmp += (uint32_t)(after ? ctz64(after) : c.b);
+            }
+            if ((more >> lim) & 1ull)
+                continue;      // refuted: next row at this position
+            // larger: witness; the unused rows follow in increasing order
+            const uint32_t u = ~(used | (1u << j)) & (uint32_t)lowmask(d);
+            if (l > i && l < d) pr = select_bit((uint64_t)u, l - i - 1);
+            v.verdict = 0;
+            v.frow = i;
+            v.fcol = lim;
+            v.rtc = row_to_change(g, pr, d, i);
+            return true;
+        }
+        if constexpr (COUNT) { v.orbs_cmp += popc64(C); v.cells_cmp += (uint32_t)c.b; }
+        if (i + 1 < d) {
+            const uint64_t Ro = g.read64(c.rowv, i);
+            stk[(i + 1) * G + l] = S & (((Ro >> l) & 1ull) ? Rp : ~Rp);
+        }
+        used |= 1u << j;
+        i++;
+        if (l == i) cursor = 0;
+    }
+}
+
 // Full canonicity test of candidate `ci` by the group g.  With HUNT, a
 // budgeted witness hunt in the proposed order runs first; the plain search
 // decides whatever the hunt leaves open (canonical candidates, |Aut|).  With
@@ -856,7 +943,6 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const 
                              const uint64_t *cells, uint32_t stride, const OutPtrs &o, uint32_t max_perms,
                              uint32_t *lds, const uint8_t *order, int od, uint32_t hunt_budget,
                              uint32_t *defer, uint32_t *defer_count) {
-    static_assert(!(HUNT && SETS), "the witness hunt runs on the replayed search");
     const int l = g.ln;
     const Staged c = load_candidate(g, ci, meta, rows, cells, stride);
     const int d = c.d, b = c.b;
@@ -868,21 +954,37 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const 
         uint64_t *stk = reinterpret_cast<uint64_t *>(lds);
         uint8_t *cnt = reinterpret_cast<uint8_t *>(stk + d * G);
         if (stage_sets(g, c, cells, stk, cnt)) {
-            set_search<G, COUNT>(g, c, stk, cnt, ci, o, max_perms, pr, ob, v);
+            bool hfound = false;
+            if constexpr (HUNT) {
+                // an order table narrower than the candidate cannot guide it: plain search only
+                if (!order || d <= od) {
+                    hfound = hunt_sets<G, COUNT>(g, c, stk, cnt, order ? order + (uint64_t)ci * od * od : nullptr,
+                                                 od, hunt_budget, pr, v);
+                    if (!hfound) {
+                        pr = l;
+                        stage_sets(g, c, cells, stk, cnt);   // identity sets again
+                    }
+                }
+            }
+            if (!hfound)
+                set_search<G, COUNT>(g, c, stk, cnt, ci, o, max_perms, pr, ob, v);
             found = true;
         }
     }
-    if (!found)
+    if (!found) {
+        // replayed search (column permutation reported, or not in orbit form)
         write_geo(g, c, geo);
-    if constexpr (HUNT) {
-        // an order table narrower than the candidate cannot guide it: plain search only
-        if (!order || d <= od)
-            found = hunt<G, COUNT>(g, c, geo, order ? order + (uint64_t)ci * od * od : nullptr, od, hunt_budget,
-                                   pr, pc, v);
-        if (!found) { pr = l; pc = l; }
+        bool hfound = false;
+        if constexpr (HUNT) {
+            // an order table narrower than the candidate cannot guide it: plain search only
+            if (!order || d <= od)
+                hfound = hunt<G, COUNT>(g, c, geo, order ? order + (uint64_t)ci * od * od : nullptr, od,
+                                        hunt_budget, pr, pc, v);
+            if (!hfound) { pr = l; pc = l; }
+        }
+        if (!hfound)
+            plain_search<G, COUNT>(g, c, geo, ci, o, max_perms, pr, ob, pc, v);
     }
-    if (!found)
-        plain_search<G, COUNT>(g, c, geo, ci, o, max_perms, pr, ob, pc, v);
 
     if (defer && v.verdict == ISE_VERDICT_LIMIT) {
         if (l == 0) defer[atomicAdd(defer_count, 1u)] = ci;

@@ -36,6 +36,7 @@ ERR_CAPACITY = -4
 ERR_STATE = -5
 DEFAULT_DEFER_PERMS = 0
 DEFAULT_DEFER_PERMS2 = 0
+DEFAULT_LEARNED_GATE = 16
 
 ORDER_PLAIN = 0     # the reference's search alone
 ORDER_HUNT = 1      # budgeted witness hunt in increasing row order, then the plain search
@@ -112,6 +113,7 @@ def lib():
         L.ise_mlp_check_batch_device_timed.argtypes = [P, U32, U32, P, P, U32, P, U32,
                                                        ctypes.POINTER(ResultsStruct), P, U32, P, P]
         L.ise_mlp_witness_labels.argtypes = [U32, U32, P, P, P, P, P, P]
+        L.ise_mlp_set_gate.argtypes = [P, U32]
         L.ise_group_closure.argtypes = [P, U32, U32, P, U64, ctypes.POINTER(U64)]
         L.ise_adjust_generators.argtypes = [P, U32, U32, P, U32, P]
         L.ise_solution_orbits.argtypes = [P, P, U64, U32, P, U32, P, P]
@@ -514,8 +516,14 @@ class RowOrderModel:
         self.ctx._check(self._L.ise_mlp_order(self.h, n, stride, d_rows.ptr, d_meta.ptr, od, d_order.ptr,
                                               None if d_score is None else d_score.ptr))
 
+    def set_gate(self, gate_perms=DEFAULT_LEARNED_GATE):
+        """Order only the candidates the plain search leaves open after
+        gate_perms permutations (0: order every candidate)."""
+        self.ctx._check(self._L.ise_mlp_set_gate(self.h, gate_perms))
+
     def time_check(self, n, stride, d_rows, d_meta, od, d_order, hunt_budget, res_struct, d_head, repeats):
-        """Per repeat: ordering kernel + learned-order check; (total ms, ordering ms)."""
+        """Per repeat: gated plain pass + ordering + learned-order check;
+        (total ms, ms up to the end of the ordering)."""
         ms = np.zeros(repeats, np.float32)
         oms = np.zeros(repeats, np.float32)
         self.ctx._check(self._L.ise_mlp_check_batch_device_timed(

@@ -298,14 +298,25 @@ int ise_mlp_fit(ise_mlp *m, const uint16_t *d_X, const float *d_y, const float *
 int ise_mlp_order(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows, const uint32_t *d_meta,
                   uint32_t od, uint8_t *d_order, float *d_score);
 
-/* Timed learned-order checking: per repeat, the ordering kernel then the
- * canonicity kernel in ISE_ORDER_LEARNED mode with `hunt_budget`, on the
- * context's stream; ms_out[r] = both, order_ms_out[r] (optional) = the
- * ordering alone.  The context's previous order mode is restored. */
+/* Timed learned-order checking on the context's stream; ms_out[r] = the
+ * whole repeat, order_ms_out[r] (optional) = up to the end of the ordering.
+ * With a gate (ise_mlp_set_gate, default ISE_DEFAULT_LEARNED_GATE) a repeat
+ * is: the plain search on every candidate with a budget of `gate`
+ * permutations (the verdict of those it decides is final), the ordering
+ * kernel on the candidates it leaves open only, then on those the witness
+ * hunt in the learned order (`hunt_budget` comparisons) and the plain search
+ * for what the hunt leaves open.  Without a gate: the ordering kernel on every
+ * candidate, then the canonicity kernel in ISE_ORDER_LEARNED mode.  The
+ * context's previous order mode is restored. */
 int ise_mlp_check_batch_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows,
                                      const uint32_t *d_meta, uint32_t od, uint8_t *d_order, uint32_t hunt_budget,
                                      const ise_results *d_out, uint32_t *d_counter, uint32_t repeats, float *ms_out,
                                      float *order_ms_out);
+
+/* Permutations of plain search after which a candidate of a learned-order
+ * batch is ordered by the network (0: every candidate is). */
+#define ISE_DEFAULT_LEARNED_GATE 16u
+int ise_mlp_set_gate(ise_mlp *m, uint32_t gate_perms);
 
 /* Witness-log labels (host arrays): for every non-canonical candidate and
  * every position i <= fail_row, pair (i, j), j < depth, gets label 1 when the

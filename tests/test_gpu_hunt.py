@@ -76,3 +76,32 @@ def test_order_mode_arguments_checked(gpu_ctx):
     with pytest.raises(ise_amd.ISEError):
         gpu_ctx.set_order(ise_amd.ORDER_LEARNED, 16)
     gpu_ctx.set_order(ise_amd.ORDER_PLAIN)
+
+
+SET_FIELDS = tuple(f for f in FIELDS if f != "perm_col")
+
+
+@pytest.mark.parametrize("v,k,lam,budget", [(13, 3, 1, 64), (9, 3, 2, 8), (15, 7, 3, 32), (16, 6, 2, 4)])
+def test_hunt_on_column_sets_parity(gpu_ctx, oracle, v, k, lam, budget):
+    """The hunt in column-set form (no column permutation asked for): every
+    other field equal to the oracle's hunt-then-plain restatement."""
+    print(f"case hunt column sets {(v, k, lam)} budget {budget}", flush=True)
+    _, _, _, (rows, meta) = oracle.enumerate(v, k, lam, capture=(1 << 15, 32, 1, v))
+    r2, m2 = ise_amd.random_extensions(v, k, lam, 2000, 2, v, 0x4E + budget)
+    rows, meta = np.concatenate([rows, r2]), np.concatenate([meta, m2])
+    dim = 32
+    order = random_orders(meta, dim, budget)
+    d_order = gpu_ctx.device_array(order.shape, np.uint8, order)
+    gpu_ctx.set_order(ise_amd.ORDER_LEARNED, budget, d_order, dim)
+    try:
+        g = gpu_ctx.check_batch(rows, meta, fields=SET_FIELDS)
+    finally:
+        gpu_ctx.set_order(ise_amd.ORDER_PLAIN)
+    o = oracle.check_batch_hunt(rows, meta, order=order, order_dim=dim, budget=budget)
+    for f in SET_FIELDS:
+        a, b = g[f], o[f]
+        eq = (a == b) if a.ndim == 1 else (a == b).all(axis=tuple(range(1, a.ndim)))
+        if not eq.all():
+            i = int(np.nonzero(~eq)[0][0])
+            raise AssertionError(f"{(v, k, lam)}: {f} differs at {i}: gpu={a[i]} oracle={b[i]}")
+    d_order.free()

@@ -232,6 +232,7 @@ def test_trained_network_gain(gpu_ctx, oracle):
     out = {}
     for name, model in (("trained", learned.load(gpu_ctx)), ("untrained", ise_amd.RowOrderModel(gpu_ctx, 4096, 3))):
         d_order = gpu_ctx.device_array((n, od, od), np.uint8)
+        model.set_gate(0)            # order every candidate: the whole stream is scored
         model.time_check(n, 16, d_rows, d_meta, od, d_order, budget, res, d_head, 1)
         assert np.array_equal(dev["verdict"].to_host(), verdict), name
         out[name] = (dev["counters"].to_host(), d_order.to_host())
@@ -248,3 +249,44 @@ def test_trained_network_gain(gpu_ctx, oracle):
     # ... but the plain order already rejects those candidates as fast: the
     # learned order is no faster than plain here, and must not be much slower
     assert perms["trained"] <= 1.1 * perms["plain"]
+
+
+@pytest.mark.parametrize("gate", [1, 16])
+def test_gated_learned_batch(gpu_ctx, oracle, model, gate):
+    """Learned-order batches with a gate: candidates the plain search decides
+    within `gate` permutations keep the plain results; the others get the
+    oracle's hunt-then-plain results in the order the network proposed for
+    them.  Verdicts equal the plain search's throughout."""
+    print(f"case gated learned batch gate {gate}", flush=True)
+    rng = np.random.default_rng(9)
+    model.params = random_params(rng)
+    r1, m1 = mixed_batch(oracle, 300)
+    r2, m2 = oracle.enumerate_sampled(25, 4, 1, 2, 0x61, 8, 16, stride=32, cap=1 << 14)[:2]
+    rows, meta = np.concatenate([r1, r2]), np.concatenate([m1, m2])
+    od, budget = 32, 24
+    n = len(meta)
+    d_rows = gpu_ctx.device_array(rows.shape, np.uint64, rows)
+    d_meta = gpu_ctx.device_array(meta.shape, np.uint32, meta)
+    d_head = gpu_ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
+    d_order = gpu_ctx.device_array((n, od, od), np.uint8)
+    fields = ("verdict", "fail_row", "fail_col", "row_to_change", "group_order", "perm_row", "orbits", "counters")
+    dev, res = gpu_ctx.device_results(n, fields)
+    model.set_gate(gate)
+    try:
+        model.time_check(n, rows.shape[1], d_rows, d_meta, od, d_order, budget, res, d_head, 1)
+    finally:
+        model.set_gate()
+    g = {f: dev[f].to_host() for f in fields}
+    order = d_order.to_host()
+    plain = oracle.check_batch(rows, meta)
+    assert np.array_equal(g["verdict"], plain["verdict"])
+    gated = plain["counters"][:, 0] > gate          # left open after `gate` permutations
+    assert gated.any() and (~gated).any()
+    for f in fields:
+        assert np.array_equal(g[f][~gated], plain[f][~gated]), f
+    idx = np.nonzero(gated)[0]
+    hunt = oracle.check_batch_hunt(rows[idx], meta[idx], order=order[idx], order_dim=od, budget=budget)
+    for f in fields:
+        assert np.array_equal(g[f][idx], hunt[f]), f
+    for b in list(dev.values()) + [d_rows, d_meta, d_head, d_order]:
+        b.free()
