@@ -528,10 +528,10 @@ template <int G> ISE_DEV int fill_tail(const Grp<G> &g, int pr, int i, int d, ui
 // bad and more are the last comparison's ballots.
 template <int G, bool COUNT, bool CHECKED>
 ISE_DEV uint32_t sibling_run(const Grp<G> &g, const Staged &c, uint64_t S, int cR, uint64_t C, uint32_t U,
-                             uint32_t left, int &cur, uint64_t &bad, uint64_t &more, Verdict &v) {
+                             uint32_t left, int &cur, uint64_t &bad, uint64_t &more, uint64_t &Rp, Verdict &v) {
     uint32_t k = 0;
     for (;;) {
-        const uint64_t Rp = g.read64(c.rowv, cur);
+        Rp = g.read64(c.rowv, cur);
         const int cP = popc64(Rp & S);
         bad = g.ballot(cP != cR) & C;
         more = g.ballot(cR > cP);
@@ -601,12 +601,14 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
             // back up to the deepest position with a larger unused row
             int p = g.uni(i), nxt = -1;
             U = (uint32_t)g.uni((int)U);
+            int e = g.read(pr, p);
             for (;;) {
-                const uint32_t above = U & (0xfffffffeu << g.read(pr, p));
+                const uint32_t above = U & (0xfffffffeu << e);
                 if (above) { nxt = ctz64(above); break; }
                 if (p == 0) break;
                 p--;
-                U |= 1u << g.read(pr, p);
+                e = g.read(pr, p);
+                U |= 1u << e;
             }
             if (nxt < 0 || p < (first ? c.fixed : 0)) {
                 // exhausted: the walk ends on the descending permutation
@@ -630,16 +632,15 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
                     left--;
                 }
                 // rows tried at position i until one is not refuted: only cur changes
-                uint64_t bad, more;
+                uint64_t bad, more, Rp;
                 if (left >= 32u)               // at most 31 more rows at one position
-                    left -= sibling_run<G, COUNT, false>(g, c, S, cR, C, U, left, cur, bad, more, v);
+                    left -= sibling_run<G, COUNT, false>(g, c, S, cR, C, U, left, cur, bad, more, Rp, v);
                 else
-                    left -= sibling_run<G, COUNT, true>(g, c, S, cR, C, U, left, cur, bad, more, v);
+                    left -= sibling_run<G, COUNT, true>(g, c, S, cR, C, U, left, cur, bad, more, Rp, v);
                 if (l == i) pr = cur;
                 if (!bad) {
-                    // equal: descend to the next position
+                    // equal: descend to the next position (Rp: the row just compared)
                     const uint64_t Ro = g.read64(c.rowv, i);
-                    const uint64_t Rp = g.read64(c.rowv, cur);
                     S &= ((Ro >> l) & 1ull) ? Rp : ~Rp;
                     U &= ~(1u << cur);
                     if (++i == d) { why = 2; break; }
