@@ -9,7 +9,7 @@
 // of exactly lambda, and leave every column able to reach k ones by the last
 // row.  The reference solves this system with CInSysSolver / CRowSolution
 // (Source/InSysSolver.h:92-286, Source/RowSolution.h:216-323); here one GPU
-// thread per parent runs the same depth-first search over the orbits, with
+// wave per parent runs the same depth-first search over the orbits, with
 // the counts tried from the largest down, so the solutions come out in the
 // order of the host solver (ise_enum.cpp) and of the oracle.
 //
@@ -34,117 +34,126 @@ struct SolverDesign {
 
 __device__ __forceinline__ uint64_t low_bits_d(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1); }
 
-// Solutions of the row system of one parent (rows[0..depth-1]); writes the
-// solution rows to out[0..] when out is non-null.  Returns the number found.
-__device__ uint32_t solve_parent(const uint64_t *rows, int depth, const SolverDesign ds, uint64_t *out) {
-    uint8_t start[ISE_MAX_COLS], hi[ISE_MAX_COLS], lo[ISE_MAX_COLS];
-    uint32_t rowmask[ISE_MAX_COLS];
-    uint8_t cap_r[ISE_MAX_COLS + 1];
-    uint8_t cap_l[ISE_MAX_COLS + 1][ISE_MAX_ROWS];
-    int8_t cur[ISE_MAX_COLS];
-    uint8_t lam[ISE_MAX_ROWS];
+// Solutions of the row system of one parent (rows[0..depth-1]), one wave
+// per parent: lane i < depth holds earlier row i and its intersection so far
+// (lam), lane c < b is column c, lane o < norb holds orbit o's first column,
+// end, count range and current count.
+// The lambda conditions of every earlier row are checked at once (a ballot);
+// the ones row i can still collect after orbit o are the ones of row i in
+// the columns after it that still need a one -- a popcount.  Solutions come
+// out in the thread version's order.  Writes the children of the parent
+// (parent rows, then the solution row, stride words each) when `children`
+// is non-null and returns the number of solutions (every lane).
+__device__ uint32_t solve_parent_wave(const uint64_t *rows, int depth, uint32_t stride, const SolverDesign ds,
+                                      uint64_t *children) {
+    const int l = (int)__lane_id();
     const uint64_t bmask = low_bits_d(ds.b);
-    uint64_t starts = 1;
-    for (int i = 0; i < depth; i++) starts |= rows[i] ^ (rows[i] << 1);
+    const uint64_t prow = l < depth ? rows[l] : 0ull;          // lane i: row i
+    // column orbits: column c starts one when an earlier row changes value at c
+    uint64_t starts = 1, w_col = 0;
+    for (int i = 0; i < depth; i++) {
+        const uint64_t ri = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(prow >> 32), i) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)prow, i);
+        starts |= ri ^ (ri << 1);
+        w_col += (ri >> l) & 1ull;                            // lane c: ones of column c so far
+    }
     starts &= bmask;
-    const int after = ds.v - depth - 1;   // rows still to come after this one
-    int norb = 0;
-    for (uint64_t m = starts; m; m &= m - 1) {
-        const int s = __builtin_ctzll(m);
+    const int norb = __popcll(starts);
+    const int after = ds.v - depth - 1;                       // rows still to come after this one
+    const int need = ds.k - (int)w_col;
+    // a column that can no longer reach k ones (or must take more than the rows left)
+    if (__builtin_amdgcn_ballot_w64(l < ds.b && (need < 0 || need > after + 1)))
+        return 0;
+    // columns that may take a one in this row, and those that must
+    const uint64_t HI = __builtin_amdgcn_ballot_w64(l < ds.b && need > 0);
+    const uint64_t LO = __builtin_amdgcn_ballot_w64(l < ds.b && need > after);
+    // lane o: orbit o
+    int o_start = 0, o_end = 0, o_hi = 0, o_lo = 0;
+    if (l < norb) {
+        uint64_t m = starts;
+        for (int t = 0; t < l; t++) m &= m - 1;
+        o_start = __builtin_ctzll(m);
         const uint64_t rest = m & (m - 1);
-        const int e = rest ? __builtin_ctzll(rest) : ds.b;
-        uint32_t rm = 0;
-        int w = 0;
-        for (int i = 0; i < depth; i++)
-            if ((rows[i] >> s) & 1) {
-                w++;
-                rm |= 1u << i;
-            }
-        const int need = ds.k - w;
-        if (need < 0 || need > after + 1) return 0;   // dead parent
-        start[norb] = (uint8_t)s;
-        rowmask[norb] = rm;
-        hi[norb] = (uint8_t)(need > 0 ? e - s : 0);
-        lo[norb] = (uint8_t)(need > after ? e - s : 0);
-        norb++;
+        o_end = rest ? __builtin_ctzll(rest) : ds.b;
+        o_hi = ((HI >> o_start) & 1ull) ? o_end - o_start : 0;
+        o_lo = ((LO >> o_start) & 1ull) ? o_end - o_start : 0;
     }
-    // capacities of the orbits from o on, overall and per earlier row
-    cap_r[norb] = 0;
-    for (int i = 0; i < depth; i++) cap_l[norb][i] = 0;
-    for (int o = norb - 1; o >= 0; o--) {
-        cap_r[o] = (uint8_t)(cap_r[o + 1] + hi[o]);
-        for (int i = 0; i < depth; i++)
-            cap_l[o][i] = (uint8_t)(cap_l[o + 1][i] + (((rowmask[o] >> i) & 1) ? hi[o] : 0));
-    }
-    for (int i = 0; i < depth; i++) lam[i] = 0;
+    // lane c: its orbit index and first column (to write a solution row)
+    const uint64_t le = l == 63 ? ~0ull : ((2ull << l) - 1);
+    const int c_orb = __popcll(starts & le) - 1;
+    const int c_start = 63 - __builtin_clzll(starts & le);
+    int lam = 0;                                              // lane i < depth
+    int cur = 0;                                              // lane o: count applied (hi + 1: none)
     uint32_t found = 0;
     int sum = 0, o = 0;
-    cur[0] = (int8_t)(hi[0] + 1);   // hi + 1: no count applied at this orbit
+    if (l == 0) cur = __builtin_amdgcn_readlane(o_hi, 0) + 1;
     for (;;) {
+        o = __builtin_amdgcn_readfirstlane(o);
+        sum = __builtin_amdgcn_readfirstlane(sum);
         if (o == norb) {
-            bool ok = sum == ds.r;
-            for (int i = 0; i < depth && ok; i++) ok = lam[i] == ds.lambda;
-            if (ok) {
-                if (out) {
-                    uint64_t row = 0;
-                    for (int p = 0; p < norb; p++) row |= low_bits_d(cur[p]) << start[p];
-                    out[found] = row;
+            const bool all = sum == ds.r &&
+                             !__builtin_amdgcn_ballot_w64(l < depth && lam != ds.lambda);
+            if (all) {
+                if (children) {
+                    const int cc = __builtin_amdgcn_ds_bpermute((c_orb < 0 ? 0 : c_orb) << 2, cur);
+                    const uint64_t row = __builtin_amdgcn_ballot_w64(l < ds.b && l - c_start < cc);
+                    uint64_t *ch = children + (uint64_t)found * stride;
+                    if (l < (int)stride) ch[l] = l < depth ? prow : (l == depth ? row : 0ull);
                 }
                 found++;
             }
             if (--o < 0) break;
             continue;
         }
-        if (cur[o] <= hi[o]) {            // undo the count applied at this orbit
-            sum -= cur[o];
-            for (int i = 0; i < depth; i++)
-                if ((rowmask[o] >> i) & 1) lam[i] -= cur[o];
+        const int hi = __builtin_amdgcn_readlane(o_hi, o), lo = __builtin_amdgcn_readlane(o_lo, o);
+        const int s0 = __builtin_amdgcn_readlane(o_start, o), e0 = __builtin_amdgcn_readlane(o_end, o);
+        const int in_row = (int)((prow >> s0) & 1ull);       // lane i: orbit o inside row i
+        int x = __builtin_amdgcn_readlane(cur, o);
+        if (x <= hi) {                                        // undo the count applied at this orbit
+            sum -= x;
+            lam -= in_row * x;
         }
-        int x = cur[o] - 1;
-        for (; x >= lo[o]; x--) {
+        const uint64_t later = bmask & ~low_bits_d(e0);       // columns after orbit o
+        const int cap_r = __popcll(HI & later);
+        const int cap_l = __popcll(prow & HI & later);
+        for (x = x - 1; x >= lo; x--) {
             const int ns = sum + x;
-            if (ns > ds.r || ns + cap_r[o + 1] < ds.r) continue;
-            bool ok = true;
-            for (int i = 0; i < depth && ok; i++) {
-                const int nl = lam[i] + (((rowmask[o] >> i) & 1) ? x : 0);
-                ok = nl <= ds.lambda && nl + cap_l[o + 1][i] >= ds.lambda;
-            }
-            if (ok) break;
+            if (ns > ds.r || ns + cap_r < ds.r) continue;
+            const int nl = lam + in_row * x;
+            if (!__builtin_amdgcn_ballot_w64(l < depth && (nl > ds.lambda || nl + cap_l < ds.lambda)))
+                break;
         }
-        if (x < lo[o]) {                  // orbit exhausted: back to the previous one
-            cur[o] = (int8_t)(hi[o] + 1);
+        if (x < lo) {                                         // orbit exhausted: back to the previous one
+            if (l == o) cur = hi + 1;
             if (--o < 0) break;
             continue;
         }
-        cur[o] = (int8_t)x;
+        if (l == o) cur = x;
         sum += x;
-        for (int i = 0; i < depth; i++)
-            if ((rowmask[o] >> i) & 1) lam[i] += x;
-        if (++o < norb) cur[o] = (int8_t)(hi[o] + 1);
+        lam += in_row * x;
+        if (++o < norb && l == o) cur = __builtin_amdgcn_readlane(o_hi, o) + 1;
     }
     return found;
 }
 
-__global__ void solve_count_kernel(uint32_t n, uint32_t stride, const uint64_t *__restrict__ parents, int depth,
-                                   SolverDesign ds, uint32_t *__restrict__ counts) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void solve_count_kernel(uint32_t n, uint32_t stride,
+                                                          const uint64_t *__restrict__ parents, int depth,
+                                                          SolverDesign ds, uint32_t *__restrict__ counts) {
+    const uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (p >= n) return;
-    counts[p] = solve_parent(parents + (uint64_t)p * stride, depth, ds, nullptr);
+    const uint32_t cnt = solve_parent_wave(parents + (uint64_t)p * stride, depth, stride, ds, nullptr);
+    if (__lane_id() == 0) counts[p] = cnt;
 }
 
 // children[offsets[p] + t] = parent p with solution t as row `depth`
-__global__ void solve_write_kernel(uint32_t n, uint32_t stride, const uint64_t *__restrict__ parents, int depth,
-                                   SolverDesign ds, const uint32_t *__restrict__ offsets,
-                                   uint64_t *__restrict__ sol_rows, uint64_t *__restrict__ children) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void solve_write_kernel(uint32_t n, uint32_t stride,
+                                                          const uint64_t *__restrict__ parents, int depth,
+                                                          SolverDesign ds, const uint32_t *__restrict__ offsets,
+                                                          uint64_t *__restrict__ children) {
+    const uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (p >= n) return;
-    const uint64_t *par = parents + (uint64_t)p * stride;
-    const uint32_t off = offsets[p];
-    const uint32_t cnt = solve_parent(par, depth, ds, sol_rows + off);
-    for (uint32_t t = 0; t < cnt; t++) {
-        uint64_t *c = children + (uint64_t)(off + t) * stride;
-        for (int i = 0; i < (int)stride; i++) c[i] = i < depth ? par[i] : (i == depth ? sol_rows[off + t] : 0ull);
-    }
+    solve_parent_wave(parents + (uint64_t)p * stride, depth, stride, ds,
+                      children + (uint64_t)offsets[p] * stride);
 }
 
 __global__ void fill_meta_kernel(uint32_t n, uint32_t m, uint32_t *__restrict__ meta) {
@@ -205,6 +214,8 @@ struct Scanner {
 };
 
 inline unsigned blocks256(size_t n) { return (unsigned)((n + 255) / 256); }
+// one wave per parent, 4 per workgroup
+inline unsigned wave_blocks(size_t n) { return (unsigned)((n + 3) / 4); }
 
 } // namespace ise
 
@@ -228,7 +239,7 @@ int ise_solve_rows_device(ise_ctx *ctx, int v, int k, int lambda, uint32_t n, ui
     if (hipSetDevice(ctx_device(ctx)) != hipSuccess) return ISE_ERR_HIP;
     hipStream_t s = ctx_stream(ctx);
     Scanner scan;
-    hipLaunchKernelGGL(solve_count_kernel, dim3(blocks256(n)), dim3(256), 0, s, n, stride, d_parents, depth, ds,
+    hipLaunchKernelGGL(solve_count_kernel, dim3(wave_blocks(n)), dim3(256), 0, s, n, stride, d_parents, depth, ds,
                        d_counts);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = scan.exclusive(d_counts, d_offsets, n, s);
@@ -241,13 +252,9 @@ int ise_solve_rows_device(ise_ctx *ctx, int v, int k, int lambda, uint32_t n, ui
     *n_children = total;
     if (total == 0) return ISE_OK;
     if (!d_children || total > children_cap) return ISE_ERR_CAPACITY;
-    DevBuf<uint64_t> sol;
-    e = sol.reserve(total);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(solve_write_kernel, dim3(blocks256(n)), dim3(256), 0, s, n, stride, d_parents, depth, ds,
-                           d_offsets, sol.p, d_children);
-        e = hipGetLastError();
-    }
+    hipLaunchKernelGGL(solve_write_kernel, dim3(wave_blocks(n)), dim3(256), 0, s, n, stride, d_parents, depth, ds,
+                       d_offsets, d_children);
+    e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e == hipSuccess ? ISE_OK : ISE_ERR_HIP;
 }
@@ -263,7 +270,7 @@ long ise_enumerate_bibd_device(ise_ctx *ctx, int v, int k, int lambda, uint64_t 
     const uint32_t stride = (uint32_t)v;   // rows of one candidate, packed
     if (hipSetDevice(ctx_device(ctx)) != hipSuccess) return ISE_ERR_HIP;
     hipStream_t s = ctx_stream(ctx);
-    DevBuf<uint64_t> par, kids, order_all, order_kept, sol;
+    DevBuf<uint64_t> par, kids, order_all, order_kept;
     DevBuf<uint32_t> counts, offs, meta, flag, pos, head, bad;
     DevBuf<uint8_t> verdict;
     Scanner scan;
@@ -292,7 +299,7 @@ long ise_enumerate_bibd_device(ise_ctx *ctx, int v, int k, int lambda, uint64_t 
     for (int depth = 0; depth < v && npar > 0; depth++) {
         // ---- children of every kept parent
         if ((e = counts.reserve(npar)) != hipSuccess || (e = offs.reserve(npar)) != hipSuccess) break;
-        hipLaunchKernelGGL(solve_count_kernel, dim3(blocks256(npar)), dim3(256), 0, s, (uint32_t)npar, stride, par.p,
+        hipLaunchKernelGGL(solve_count_kernel, dim3(wave_blocks(npar)), dim3(256), 0, s, (uint32_t)npar, stride, par.p,
                            depth, ds, counts.p);
         if ((e = hipGetLastError()) != hipSuccess) break;
         if ((e = scan.exclusive(counts.p, offs.p, (uint32_t)npar, s)) != hipSuccess) break;
@@ -305,9 +312,9 @@ long ise_enumerate_bibd_device(ise_ctx *ctx, int v, int k, int lambda, uint64_t 
         if (level_counts) level_counts[nd] = nkid;
         if (nkid == 0) break;
         if (nkid > 0xffffffffull) { result = ISE_ERR_CAPACITY; break; }
-        if ((e = kids.reserve(nkid * stride)) != hipSuccess || (e = sol.reserve(nkid)) != hipSuccess) break;
-        hipLaunchKernelGGL(solve_write_kernel, dim3(blocks256(npar)), dim3(256), 0, s, (uint32_t)npar, stride, par.p,
-                           depth, ds, offs.p, sol.p, kids.p);
+        if ((e = kids.reserve(nkid * stride)) != hipSuccess) break;
+        hipLaunchKernelGGL(solve_write_kernel, dim3(wave_blocks(npar)), dim3(256), 0, s, (uint32_t)npar, stride,
+                           par.p, depth, ds, offs.p, kids.p);
         if ((e = hipGetLastError()) != hipSuccess) break;
         // ---- canonicity of every child
         const uint32_t out_info = nd == v ? (ISE_SAVE_ROW_PERMUTATIONS | ISE_SAVE_ROW_TO_CHANGE) : 0u;
