@@ -164,6 +164,49 @@ def kfact_rate(ctx, steps, warmup):
             "canonical": int((verdict == 1).sum()), "relabellings_per_s": float(tried.sum()) / (ms / 1e3)}
 
 
+def stream_rates(ctx, steps, warmup):
+    """The other candidate-stream configurations, plain order, HBM-resident:
+    2-(15,3,1) -- every candidate the enumerator tests at depth 4-15 plus 2^20
+    seeded random valid extensions (seed 0x15031); and the uneven batch -- a
+    seeded 1:1:1 mix of 2-(15,7,3), 2-(16,6,2) and 2-(19,9,4) random valid
+    extensions (2^18 each) at depth 5..v, one persistent launch."""
+    import ise_amd
+    out = {}
+    fields = ("verdict", "fail_row", "row_to_change", "group_order", "perm_row")
+    r1, m1 = ctx.enumerator_candidates(15, 3, 1, 4, 15, stride=16)
+    r2, m2 = ise_amd.random_extensions(15, 3, 1, 1 << 20, 4, 15, 0x15031, stride=16)
+    parts = [("2-(15,3,1) enumerator candidates (depth 4-15) + 2^20 random extensions",
+              np.concatenate([r1, r2]), np.concatenate([m1, m2]), 16)]
+    mix_r, mix_m = [], []
+    for i, (v, k, lam) in enumerate(((15, 7, 3), (16, 6, 2), (19, 9, 4))):
+        r, m = ise_amd.random_extensions(v, k, lam, 1 << 18, 5, v, 0xC3 + i, stride=32)
+        mix_r.append(r)
+        mix_m.append(m)
+    mr, mm = np.concatenate(mix_r), np.concatenate(mix_m)
+    perm = np.random.default_rng(0xC3).permutation(len(mm))
+    parts.append(("1:1:1 mix of 2-(15,7,3), 2-(16,6,2), 2-(19,9,4) random extensions, 2^18 each, depth 5..v",
+                  mr[perm], mm[perm], 32))
+    for name, rows, meta, stride in parts:
+        n = len(meta)
+        d_rows = ctx.device_array(rows.shape, np.uint64, rows)
+        d_meta = ctx.device_array(meta.shape, np.uint32, meta)
+        d_head = ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
+        dev, res = ctx.device_results(n, fields)
+        ctx.set_launch(group_lanes=0)
+        if warmup:
+            ctx.time_batch_device(n, stride, d_rows, d_meta, res, d_head, warmup)
+        ms = float(np.mean(ctx.time_batch_device(n, stride, d_rows, d_meta, res, d_head, steps)))
+        verdict = dev["verdict"].to_host()
+        if (verdict > 1).any():
+            raise RuntimeError(f"search limit hit or invalid record in the {name} batch")
+        out[name] = {"candidates": n, "ms_per_step": ms, "candidates_per_s": n / (ms / 1e3),
+                     "canonical": int((verdict == 1).sum())}
+        for b in list(dev.values()) + [d_rows, d_meta, d_head]:
+            b.free()
+    ctx.set_launch(group_lanes=64)
+    return out
+
+
 def enumeration_rate():
     """Full enumeration of 2-(13,3,1) and 2-(15,3,1): host row solver + GPU
     canonicity per level, and the device-resident enumeration (GPU row
@@ -344,6 +387,7 @@ def main():
         buf.free()
     if not args.no_extra:
         line["kfact_k12"] = kfact_rate(ctx, max(3, args.steps // 4), 1)
+        line["streams"] = stream_rates(ctx, max(3, args.steps // 4), 1)
         line["enumeration"] = enumeration_rate()
         line["host_buffers"] = host_buffer_rate(rows, meta)
     ctx.close()
