@@ -144,7 +144,9 @@ def profiled_launch_figures():
 
 def kfact_rate(ctx, steps, warmup):
     """1-factorizations of K_12: 2^18 seeded partial factorizations of 3-10
-    rounds, plain relabelling order, one launch per step."""
+    rounds, plain relabelling order, one split launch per step (first pass of
+    512 relabellings, then up to 256 ranges per undecided candidate); the
+    one-wave-per-candidate launch is timed beside it."""
     import ise_amd
     rows, meta = ise_amd.kf_random(12, 3, 10, 1 << 18, 0x12, stride=16)
     n = len(meta)
@@ -152,16 +154,24 @@ def kfact_rate(ctx, steps, warmup):
     d_meta = ctx.device_array(meta.shape, np.uint32, meta)
     d_head = ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
     dev, res = ctx.kf_device_results(n)
-    if warmup:
-        ctx.kf_time_batch_device(n, 16, d_rows, d_meta, res, d_head, warmup)
-    ms = float(np.mean(ctx.kf_time_batch_device(n, 16, d_rows, d_meta, res, d_head, steps)))
+    timed = {}
+    for name, split in (("one_pass", (0, 1)), ("split", (ise_amd.DEFAULT_KF_FIRST_CHUNKS,
+                                                         ise_amd.DEFAULT_KF_MAX_RANGES))):
+        ctx.kf_set_split(*split)
+        if warmup:
+            ctx.kf_time_batch_device(n, 16, d_rows, d_meta, res, d_head, warmup)
+        timed[name] = float(np.mean(ctx.kf_time_batch_device(n, 16, d_rows, d_meta, res, d_head, steps)))
+    ctx.kf_set_split()
+    ms = timed["split"]
     verdict = dev["verdict"].to_host()
     tried = dev["n_tried"].to_host()
     for b in list(dev.values()) + [d_rows, d_meta, d_head]:
         b.free()
     return {"workload": "K_12 partial 1-factorizations, 3-10 rounds, 2^18 seeded (seed 0x12), plain order",
             "candidates": n, "ms_per_step": ms, "candidates_per_s": n / (ms / 1e3),
-            "canonical": int((verdict == 1).sum()), "relabellings_per_s": float(tried.sum()) / (ms / 1e3)}
+            "canonical": int((verdict == 1).sum()), "relabellings_per_s": float(tried.sum()) / (ms / 1e3),
+            "split": {"first_chunks": ise_amd.DEFAULT_KF_FIRST_CHUNKS, "max_ranges": ise_amd.DEFAULT_KF_MAX_RANGES},
+            "one_pass_ms_per_step": timed["one_pass"]}
 
 
 def stream_rates(ctx, steps, warmup):

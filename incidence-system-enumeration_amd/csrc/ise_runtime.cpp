@@ -52,6 +52,12 @@ struct ise_ctx {
     uint64_t *d_kf_tab = nullptr;
     ise::KfTables kf_tab{};
     uint64_t *d_kf_witness = nullptr, *d_kf_auts = nullptr, *d_kf_tried = nullptr;
+    // split launch (ise_kf_set_split): list, first-pass counts, published
+    // witnesses for kf_cap candidates, and the range results
+    uint32_t kf_first_chunks = ISE_DEFAULT_KF_FIRST_CHUNKS, kf_max_ranges = ISE_DEFAULT_KF_MAX_RANGES;
+    uint32_t *d_kf_list = nullptr, *d_kf_best = nullptr;
+    uint64_t *d_kf_part = nullptr, *d_kf_ranges = nullptr;
+    size_t kf_cap = 0, kf_range_words = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // executable graphs of ise_submit launches, keyed by batch shape
     bool use_graphs = false;
@@ -165,6 +171,29 @@ int ensure_defer(ise_ctx *c, size_t n) {
     return ISE_OK;
 }
 
+// Range results of the split 1-factorization launch: at least this many
+// words whatever the capacity, so that small batches get many ranges each.
+constexpr size_t KF_MIN_RANGE_WORDS = size_t(1) << 17;
+
+int ensure_kf_split(ise_ctx *c, size_t n) {
+    if (!c->kf_first_chunks || n <= c->kf_cap) return ISE_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    void *bufs[] = {c->d_kf_list, c->d_kf_best, c->d_kf_part, c->d_kf_ranges};
+    for (void *p : bufs)
+        if (p) HIPCHK(c, hipFree(p));
+    c->d_kf_list = c->d_kf_best = nullptr;
+    c->d_kf_part = c->d_kf_ranges = nullptr;
+    c->kf_cap = c->kf_range_words = 0;
+    const size_t words = n > KF_MIN_RANGE_WORDS ? n : KF_MIN_RANGE_WORDS;
+    HIPCHK(c, hipMalloc(&c->d_kf_list, n * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&c->d_kf_best, n * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&c->d_kf_part, n * sizeof(uint64_t)));
+    HIPCHK(c, hipMalloc(&c->d_kf_ranges, words * sizeof(uint64_t)));
+    c->kf_cap = n;
+    c->kf_range_words = words;
+    return ISE_OK;
+}
+
 int launch(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const uint64_t *cells,
            const uint32_t *meta, const ise_results &out, uint32_t *head, hipStream_t s,
            int group_lanes) {
@@ -264,6 +293,14 @@ int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride) {
         if (e == hipSuccess) e = dalloc(&c->d_kf_witness, n);
         if (e == hipSuccess) e = dalloc(&c->d_kf_auts, n);
         if (e == hipSuccess) e = dalloc(&c->d_kf_tried, n);
+        if (e == hipSuccess) e = dalloc(&c->d_kf_list, n);
+        if (e == hipSuccess) e = dalloc(&c->d_kf_best, n);
+        if (e == hipSuccess) e = dalloc(&c->d_kf_part, n);
+        if (e == hipSuccess) e = dalloc(&c->d_kf_ranges, n > KF_MIN_RANGE_WORDS ? n : KF_MIN_RANGE_WORDS);
+        if (e == hipSuccess) {
+            c->kf_cap = n;
+            c->kf_range_words = n > KF_MIN_RANGE_WORDS ? n : KF_MIN_RANGE_WORDS;
+        }
         if (e == hipSuccess) {
             std::vector<uint64_t> words(ise::kf_table_words());
             ise::kf_build_tables(words.data(), c->kf_tab.offset, c->kf_tab.count);
@@ -291,7 +328,8 @@ int ise_close(ise_ctx *c) {
     void *bufs[] = {c->d_rows, c->d_cells, c->d_meta, c->d_head, c->d.verdict, c->d.fail_row,
                     c->d.fail_col, c->d.row_to_change, c->d.group_order, c->d.perm_row,
                     c->d.perm_col, c->d.orbits, c->d.n_gens, c->d.counters, c->d.gens,
-                    c->d_kf_tab, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried, c->d_defer, c->d_sched};
+                    c->d_kf_tab, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried, c->d_defer, c->d_sched,
+                    c->d_kf_list, c->d_kf_best, c->d_kf_part, c->d_kf_ranges};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -557,14 +595,23 @@ static ise::KfOut kf_out(const ise_kf_results &r) {
 static int kf_launch_on(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
                         const uint32_t *d_meta, const ise_kf_results &d_out, uint32_t *d_queue,
                         hipStream_t s) {
-    uint32_t blocks = c->cfg.blocks;
-    if (!blocks) {
+    // one wave per workgroup; the range pass fills the chip whatever n is
+    uint32_t range_blocks = c->cfg.blocks;
+    if (!range_blocks) {
         int per_cu = 0;
         HIPCHK(c, ise::kf_occupancy(&per_cu));
-        blocks = (uint32_t)(per_cu < 1 ? 1 : per_cu) * (uint32_t)c->num_cus;
-        if (blocks > n) blocks = n ? n : 1;
+        range_blocks = (uint32_t)(per_cu < 1 ? 1 : per_cu) * (uint32_t)c->num_cus;
     }
-    HIPCHK(c, ise::kf_launch(blocks, n, stride, d_rounds, d_meta, kf_out(d_out), d_queue, c->kf_tab, s));
+    const uint32_t blocks = c->cfg.blocks ? c->cfg.blocks : (range_blocks > n ? (n ? n : 1) : range_blocks);
+    const uint32_t limit = c->kf_first_chunks * 64u;
+    if (limit) {
+        const int rc = ensure_kf_split(c, n);
+        if (rc) return rc;
+    }
+    const ise::KfSplit sp{c->d_kf_list, c->d_kf_part, c->d_kf_best, c->d_kf_ranges, (uint32_t)c->kf_range_words,
+                          c->kf_max_ranges};
+    HIPCHK(c, ise::kf_launch(blocks, range_blocks, n, stride, d_rounds, d_meta, kf_out(d_out), d_queue,
+                             c->kf_tab, limit, sp, s));
     return ISE_OK;
 }
 
@@ -665,6 +712,14 @@ int ise_set_schedule(ise_ctx *c, int schedule) {
     if (!c || (schedule != ISE_SCHEDULE_QUEUE && schedule != ISE_SCHEDULE_DEEPEST_FIRST)) return ISE_ERR_ARG;
     c->schedule = schedule;
     return ensure_sched(c, c->capacity);
+}
+
+int ise_kf_set_split(ise_ctx *c, uint32_t first_chunks, uint32_t max_ranges) {
+    if (!c || first_chunks > (1u << 20) || (first_chunks && (max_ranges < 1 || max_ranges > 65536)))
+        return ISE_ERR_ARG;
+    c->kf_first_chunks = first_chunks;
+    c->kf_max_ranges = first_chunks ? max_ranges : c->kf_max_ranges;
+    return ensure_kf_split(c, c->capacity);
 }
 
 int ise_report_columns(ise_ctx *c, int on) {

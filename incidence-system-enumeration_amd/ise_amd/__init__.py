@@ -37,6 +37,8 @@ ERR_STATE = -5
 DEFAULT_DEFER_PERMS = 0
 DEFAULT_DEFER_PERMS2 = 0
 DEFAULT_LEARNED_GATE = 16
+DEFAULT_KF_FIRST_CHUNKS = 8   # ISE_DEFAULT_KF_FIRST_CHUNKS
+DEFAULT_KF_MAX_RANGES = 256   # ISE_DEFAULT_KF_MAX_RANGES
 
 ORDER_PLAIN = 0     # the reference's search alone
 ORDER_HUNT = 1      # budgeted witness hunt in increasing row order, then the plain search
@@ -83,6 +85,7 @@ def lib():
         L.ise_report_columns.argtypes = [P, I]
         L.ise_set_defer.argtypes = [P, U32, U32]
         L.ise_set_schedule.argtypes = [P, I]
+        L.ise_kf_set_split.argtypes = [P, U32, U32]
         L.ise_graph_stats.argtypes = [P, ctypes.POINTER(U32), ctypes.POINTER(U64)]
         L.ise_autotune.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P, U32, U32, U32, P, P]
         L.ise_sync.argtypes = [P]
@@ -342,6 +345,12 @@ class Context:
         self._check(self._L.ise_sync(self.h))
 
     # ---- 1-factorizations -------------------------------------------------
+    def kf_set_split(self, first_chunks=DEFAULT_KF_FIRST_CHUNKS, max_ranges=DEFAULT_KF_MAX_RANGES):
+        """1-factorization launches: first pass of 64*first_chunks relabellings,
+        the rest of undecided candidates in up to max_ranges ranges each
+        (first_chunks=0: one wave per candidate to its decision)."""
+        self._check(self._L.ise_kf_set_split(self.h, first_chunks, max_ranges))
+
     def kf_check(self, rounds, meta):
         """Canonicity of partial 1-factorizations (host arrays in and out)."""
         rounds = np.ascontiguousarray(rounds, dtype=np.uint64)

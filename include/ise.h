@@ -349,7 +349,11 @@ int ise_kf_check(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *roun
                  const uint32_t *meta, ise_kf_results *out_host);
 
 /* Device-resident batch on `stream` (NULL = context stream); `d_queue` is
- * ISE_QUEUE_WORDS words of device memory.  No allocation. */
+ * ISE_QUEUE_WORDS words of device memory.  With the split launch on (the
+ * default, ise_kf_set_split) the call uses context buffers sized for the
+ * context capacity: a larger batch grows them on its first call (hipMalloc),
+ * and calls on one context must not overlap in time.  With the split off
+ * nothing is allocated and concurrent calls need only different d_queue. */
 int ise_kf_check_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
                               const uint32_t *d_meta, const ise_kf_results *d_out,
                               uint32_t *d_queue, void *stream);
@@ -359,6 +363,18 @@ int ise_kf_check_batch_device_timed(ise_ctx *ctx, uint32_t n, uint32_t stride,
                                     const uint64_t *d_rounds, const uint32_t *d_meta,
                                     const ise_kf_results *d_out, uint32_t *d_queue,
                                     uint32_t repeats, float *ms_out);
+
+/* Split launch of the 1-factorization test.  A canonical candidate must try
+ * every relabelling (46080 x rounds for K_12), which on one wave would be the
+ * batch's tail.  With first_chunks > 0 every candidate first runs up to
+ * 64 x first_chunks relabellings; the rest of each candidate still undecided
+ * is cut into up to max_ranges contiguous ranges run by independent waves and
+ * combined in plain order (first witness, automorphisms before it).  Results
+ * are identical to first_chunks = 0, one wave per candidate to its decision.
+ * No counterpart in the reference, whose test is one sequential loop. */
+#define ISE_DEFAULT_KF_FIRST_CHUNKS 8u
+#define ISE_DEFAULT_KF_MAX_RANGES 256u
+int ise_kf_set_split(ise_ctx *ctx, uint32_t first_chunks, uint32_t max_ranges);
 
 /* Seeded random partial 1-factorizations of K_players with nrounds uniform in
  * [rmin, rmax]: each round is a uniformly drawn perfect matching avoiding the

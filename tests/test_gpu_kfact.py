@@ -104,3 +104,77 @@ def test_device_timed_path(gpu_ctx, oracle):
     assert (ms > 0).all()
     g = {k: dev[k].to_host() for k in ise_amd.KF_FIELDS}
     assert_same(g, oracle_batch(oracle, rows, meta, 12), "device")
+
+
+_SPLIT_CASE = {}
+
+
+def split_case(oracle):
+    """Random K_12 partials (a few run tens of thousands of relabellings before
+    their witness), the printed K_10 factorizations (all canonical), the
+    uniform K_12 factorization and its prefixes, and small K_14/K_16 partials."""
+    if not _SPLIT_CASE:
+        kf = golden("kfact.json")
+        r1, m1 = ise_amd.kf_random(12, 3, 10, 1 << 14, 0x12)
+        r2, m2 = ise_amd.kf_pack([m["rounds"] for m in kf["k10"]["matrices"][:64]])
+        u12 = np.array(kf["u12"]["matrices"][0]["rounds"], np.uint8)
+        r3, m3 = ise_amd.kf_pack([u12[:k] for k in range(1, 12)])
+        r4, m4 = ise_amd.kf_random(14, 1, 3, 64, 0x14)
+        r5, m5 = ise_amd.kf_random(16, 1, 2, 16, 0x16)
+        rows = np.concatenate([r1, r2, r3, r4, r5])
+        meta = np.concatenate([m1, m2, m3, m4, m5])
+        o = {k: [] for k in ise_amd.KF_FIELDS}
+        for i in range(len(meta)):
+            nr, n = int(meta[i]) & 0xff, (int(meta[i]) >> 8) & 0xff
+            for k, x in zip(ise_amd.KF_FIELDS, oracle.kf_check(ise_amd.kf_unpack(rows[i, :nr], n))):
+                o[k].append(x)
+        o = {k: np.array(v, np.uint64 if k != "verdict" else np.uint8) for k, v in o.items()}
+        _SPLIT_CASE.update(rows=rows, meta=meta, o=o)
+    return _SPLIT_CASE["rows"], _SPLIT_CASE["meta"], _SPLIT_CASE["o"]
+
+
+@pytest.mark.parametrize("first,ranges", [(0, 1), (1, 1), (1, 3), (2, 256), (8, 256), (1, 65536)])
+def test_split_launch_same_results(gpu_ctx, oracle, first, ranges):
+    """First pass + ranges + combine give the single scan's results: the
+    first witness, the automorphisms before it and the relabellings tried,
+    with one range per candidate, a few, and more ranges than chunks."""
+    print(f"case kfact split first={first} ranges={ranges}", flush=True)
+    rows, meta, o = split_case(oracle)
+    gpu_ctx.kf_set_split(first, ranges)
+    try:
+        g = gpu_ctx.kf_check(rows, meta)
+        g1 = gpu_ctx.kf_check(rows[-40:], meta[-40:])
+    finally:
+        gpu_ctx.kf_set_split()
+    assert_same(g, o, f"split {first}/{ranges}")
+    assert_same(g1, {k: v[-40:] for k, v in o.items()}, f"split {first}/{ranges} small batch")
+
+
+@pytest.mark.parametrize("variant", ["rounds_permuted", "pairs_shuffled", "pairs_flipped"])
+def test_rounds_not_in_normal_form(gpu_ctx, oracle, variant):
+    """The test compares each relabelled round in normal form with the round
+    as given, byte for byte.  Given rounds out of order (round 0 no longer
+    (0 1)(2 3)...), with pairs out of order, or with pairs written larger
+    player first, must give the oracle's results too."""
+    print(f"case kfact not normal form {variant}", flush=True)
+    rows, meta = ise_amd.kf_random(12, 2, 8, 3000, 0x5EED)
+    rng = np.random.default_rng(len(variant))
+    mats = []
+    for i in range(len(meta)):
+        nr = int(meta[i]) & 0xff
+        mat = ise_amd.kf_unpack(rows[i, :nr], 12).copy()
+        if variant == "rounds_permuted":
+            mat = mat[rng.permutation(nr)]
+        else:
+            for r in range(nr):
+                pairs = mat[r].reshape(-1, 2)
+                if variant == "pairs_shuffled":
+                    pairs = pairs[rng.permutation(len(pairs))]
+                else:
+                    flip = rng.random(len(pairs)) < 0.3
+                    pairs[flip] = pairs[flip][:, ::-1]
+                mat[r] = pairs.reshape(-1)
+        mats.append(mat)
+    rows, meta = ise_amd.kf_pack(mats)
+    g = gpu_ctx.kf_check(rows, meta)
+    assert_same(g, oracle_batch(oracle, rows, meta, 12), variant)
