@@ -10,11 +10,17 @@ constexpr uint32_t KF_MAX_PLAYERS = 16;   // labels fit in 4 bits
 constexpr uint32_t KF_MAX_ROUNDS = 15;
 constexpr uint32_t KF_MAX_SLOTS = 8;      // pairs per round
 
+// Byte selector of v_perm_b32 that yields 0x00.
+constexpr uint32_t KF_PERM_ZERO = 0x0c;
+
 // QuickPerm tables of the pair-slot arrangements for every slot count 2..8:
 // entry h of table k is a uint64 whose byte i is the base label (2*slot) put
-// in slot i by the h-th arrangement.
+// in slot i by the h-th arrangement.  The inverse table (same offsets) holds
+// in byte j the slot whose pair receives labels 2j, 2j+1 (KF_PERM_ZERO for
+// j >= k).
 struct KfTables {
     const uint64_t *base;
+    const uint64_t *inv;
     uint32_t offset[KF_MAX_SLOTS + 1];
     uint32_t count[KF_MAX_SLOTS + 1];
 };
@@ -24,24 +30,35 @@ struct KfOut {
     uint64_t *witness, *n_auts, *n_tried;
 };
 
-// Queue words of a split launch, each on its own 128-byte line: head[0] the
-// first pass's candidate queue, head[KF_HEAD_WORD] the number of candidates
-// listed for the ranges, head[2 * KF_HEAD_WORD] the range queue.
-constexpr uint32_t KF_HEAD_WORD = 32;
+// Split launch: after the first pass the relabellings of the candidates still
+// undecided are taken in windows [W_p, W_p+1), W_0 = the first-pass limit and
+// W_p+1 = KF_WINDOW_GROWTH * W_p, each window cut into ranges over many
+// waves; a candidate decided in a window leaves the list.
+constexpr uint32_t KF_WINDOW_GROWTH = 16;
+constexpr uint32_t KF_MAX_WINDOWS = 6;   // from a limit of 64: 64 * 16^6 > 2^28 > every index space
 
-// Context buffers of a split launch: list, part and best hold one entry per
-// listed candidate (candidate index, automorphisms of the first pass, least
-// witness published so far); ranges holds range_words results.
+// Queue words, each on its own 128-byte line: head[0] the candidate queue of
+// a one-pass launch, head[KF_HEAD_WORD * (1 + p)] the length of the list of
+// candidates entering window p.
+constexpr uint32_t KF_HEAD_WORD = 32;
+static_assert(KF_HEAD_WORD * (KF_MAX_WINDOWS + 1) < ISE_QUEUE_WORDS, "queue words of a split launch");
+
+// Context buffers of a split launch.  Window p reads list[p & 1] / part[p & 1]
+// (candidate index, automorphisms counted before the window) and appends the
+// candidates it leaves undecided to the other pair; best holds per entry the
+// least witness published in the current window; ranges holds range_words
+// range results.
 struct KfSplit {
-    uint32_t *list;
-    uint64_t *part;
+    uint32_t *list[2];
+    uint64_t *part[2];
     uint32_t *best;
     uint64_t *ranges;
     uint32_t range_words, max_ranges;
 };
 
 // Host side: build the tables (Fuchs' countdown QuickPerm order,
-// k-Factorization/TripleSys/Source/cnv.cpp:283-315).
+// k-Factorization/TripleSys/Source/cnv.cpp:283-315); kf_table_words() words,
+// the arrangements in the first half, their inverses in the second.
 size_t kf_table_words();
 void kf_build_tables(uint64_t *words, uint32_t *offset, uint32_t *count);
 

@@ -19,19 +19,24 @@
 // Split launch.  Most candidates are decided within a chunk or two, but a
 // canonical one must try every relabelling (46080 x rounds for K_12), which
 // on one wave is the whole batch's tail.  With a first-pass limit the batch
-// runs in three kernels:
+// runs as:
 //   1. kf_check_kernel: every candidate up to `limit` relabellings; those
 //      still undecided are listed with their automorphism count so far;
-//   2. kf_range_kernel: the rest of each listed candidate's index space is
-//      cut into G contiguous ranges taken by independent waves; each range
-//      records its first smaller index and the automorphisms before it, and
-//      publishes the index (atomicMin) so that ranges lying wholly after it
-//      stop early;
-//   3. kf_combine_kernel: per listed candidate, the ranges in index order up
-//      to the first that holds a smaller relabelling.
+//   2. per window [W_p, W_p+1) of relabelling indices (W_0 = limit, each
+//      window 16 times further out than the last):
+//      kf_range_kernel cuts the window of each listed candidate into G
+//      contiguous ranges taken by independent waves; each range records its
+//      first smaller index and the automorphisms before it, and publishes the
+//      index (atomicMin) so that ranges lying wholly after it stop early;
+//      kf_combine_kernel, one wave per listed candidate, takes the ranges in
+//      index order up to the first that holds a smaller relabelling, writes
+//      the results of the candidates decided and lists the others for the
+//      next window.
 // The results are those of the single scan: the first range with a witness
 // holds the least one, every range before it was scanned in full, and a
-// range that stopped early lies after a range with a witness.
+// range that stopped early lies after a range with a witness.  Windows keep
+// the work of a candidate whose witness lies just past the limit near that
+// witness instead of spreading it over the whole index space.
 #include <hip/hip_runtime.h>
 #include "ise_kfact.h"
 
@@ -59,14 +64,13 @@ __device__ __forceinline__ uint32_t kf_total(const KfTables &qp, uint32_t np, ui
 struct KfLds {
     uint32_t plo[KF_MAX_ROUNDS], phi[KF_MAX_ROUNDS];   // partner of each player, per round
     uint32_t rlo[KF_MAX_ROUNDS], rhi[KF_MAX_ROUNDS];   // the rounds as given (player per position)
-    uint64_t key[KF_MAX_ROUNDS];                        // the rounds as comparison keys
-    uint32_t fast, skip0;
+    uint64_t key[KF_MAX_ROUNDS];                        // the rounds as comparison keys (general form)
+    uint64_t meet[KF_MAX_PLAYERS];   // meet[x] entry y: the round pairing x with y, 15 if none
 };
 
 // A 16-entry table of 4-bit values held in two 32-bit words, entry i at bits
-// 4i of lo:hi.  tget reads entry i; tset sets entry i (still zero) to v; rset
-// does the same in the reversed layout, entry 0 the most significant nibble
-// of hi.
+// 4i of lo:hi.  tget reads entry i; tset sets entry i (still zero) to v; tfind
+// returns the least i whose entry is z (z must occur among entries < 16).
 __device__ __forceinline__ uint32_t tget(uint32_t lo, uint32_t hi, uint32_t i) {
     return __builtin_amdgcn_ubfe((i & 8u) ? hi : lo, i << 2, 4);
 }
@@ -74,60 +78,155 @@ __device__ __forceinline__ void tset(uint32_t &lo, uint32_t &hi, uint32_t i, uin
     const uint32_t m = v << ((i & 7u) << 2);
     if (i & 8u) hi |= m; else lo |= m;
 }
-__device__ __forceinline__ void rset(uint32_t &lo, uint32_t &hi, uint32_t i, uint32_t v) {
-    const uint32_t m = v << ((~i & 7u) << 2);
-    if (i & 8u) lo |= m; else hi |= m;
+__device__ __forceinline__ uint32_t tfind(uint32_t lo, uint32_t hi, uint32_t z) {
+    // a zero nibble of w ^ z*0x11111111 marks z; the lowest flagged nibble is
+    // exact (a borrow only flags nibbles above a true zero)
+    const uint32_t rep = z * 0x11111111u;
+    uint32_t x = lo ^ rep;
+    uint32_t m = (x - 0x11111111u) & ~x & 0x88888888u;
+    if (m) return (uint32_t)__builtin_ctz(m) >> 2;
+    x = hi ^ rep;
+    m = (x - 0x11111111u) & ~x & 0x88888888u;
+    return 8u + ((uint32_t)__builtin_ctz(m) >> 2);
 }
 
-// Comparison keys.  Two rounds in normal form (pairs (a, b) with a < b,
-// pairs by first element) compare exactly as their partner arrays p[0], p[1],
-// ... compare lexicographically: the positions before the next pair's first
-// element are covered by equal earlier pairs, and at that element the arrays
-// hold the partners the pair comparison looks at.  When every given round is
-// in this form (`fast`) a round's key is its partner array, p[0] most
-// significant; otherwise the key is the round's players in given order and the
-// relabelled round is spelled out pair by pair after normalising.  Round 0 of
-// every relabelled matrix is its day round, relabelled to (0 1)(2 3)...; when
-// the given round 0 is that, it compares equal and is skipped (`skip0`).
+// Two forms of the comparison.
+//
+// Label order ("fast", every given round in normal form: pairs (a, b) with
+// a < b, pairs by first element).  Two rounds in normal form compare exactly
+// as their partner arrays p[0], p[1], ... compare lexicographically: the
+// positions before the next pair's first element are covered by equal earlier
+// pairs, and at that element the arrays hold the partners the pair comparison
+// looks at.  The relabelled matrix's rounds are ordered by the label of
+// player 0's partner, p[0]; walking the labels L = 1, 2, ... with y the player
+// relabelled to L, the round meet[p0][y] (p0 the player relabelled to 0) is
+// the next round of the relabelled matrix and its key is L.  So a round is
+// decided by comparing L with the given round's p[0], and only on equality by
+// its partner array from position 1; the relabelling is needed only as the
+// inverse map label -> player, which is a byte gather of the day round's
+// pairs by the inverse slot arrangement (v_perm_b32) with the pairs of
+// flipped slots nibble-swapped.
+//
+// General form (some given round not in normal form): the key is the round's
+// players in given order, and the relabelled round is spelled out pair by
+// pair after normalising -- the comparison the reference makes byte by byte.
+//
+// Round 0 of every relabelled matrix is its day round, relabelled to
+// (0 1)(2 3)...; when the given round 0 is that, it compares equal and is
+// skipped (`skip0`).  When in addition the rounds are in normal form and in
+// order of player 0's partner (`ident`), relabelling 0 (day 0, identity
+// arrangement, no pair flipped) is the identity and reproduces the matrix: it
+// is counted as an automorphism without the comparison, which would otherwise
+// walk every round of the first chunk of every candidate.
+
+struct KfUni {   // per-candidate values held in scalar registers
+    uint32_t nr, n, np;
+    uint32_t fast, skip0, ident;
+    uint32_t glo, ghi;               // entry i: partner of player 0 in round i as given
+};
+
+__device__ __forceinline__ uint32_t kf_uni(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
 
 // Lanes < nr decode their round (`row`, loaded by the caller) into the LDS
-// tables.  Ends with a barrier.
-__device__ __forceinline__ void kf_stage(KfLds &s, uint64_t row, uint32_t nr, uint32_t n, uint32_t l) {
+// tables; returns the candidate's uniform values.  Ends with a barrier.
+__device__ __forceinline__ KfUni kf_stage(KfLds &s, uint64_t row, uint32_t nr, uint32_t n, uint32_t l) {
+    const uint32_t rlo = (uint32_t)row, rhi = (uint32_t)(row >> 32);
     bool normal = true;
-    uint32_t plo = 0, phi = 0, klo = 0, khi = 0;
-    uint64_t raw = 0;
     if (l < nr) {
+        uint32_t plo = 0, phi = 0, prev = 0;
         for (uint32_t p = 0; p < n; p += 2) {
-            const uint32_t x = nib(row, p), y = nib(row, p + 1);
+            const uint32_t w = p < 8 ? rlo : rhi;
+            const uint32_t x = __builtin_amdgcn_ubfe(w, p << 2, 4), y = __builtin_amdgcn_ubfe(w, (p << 2) + 4, 4);
             tset(plo, phi, x, y);
             tset(plo, phi, y, x);
-            rset(klo, khi, x, y);
-            rset(klo, khi, y, x);
-            normal = normal && x < y && (p == 0 || x > nib(row, p - 2));
+            normal = normal && x < y && (p == 0 || x > prev);
+            prev = x;
         }
-        for (uint32_t p = 0; p < n; p++) raw = (raw << 4) | nib(row, p);
-    }
-    const bool fast = __builtin_amdgcn_ballot_w64(l < nr && !normal) == 0;
-    if (l < nr) {
         s.plo[l] = plo;
         s.phi[l] = phi;
-        s.rlo[l] = (uint32_t)row;
-        s.rhi[l] = (uint32_t)(row >> 32);
-        s.key[l] = fast ? (((uint64_t)khi << 32) | klo) : raw;
+        s.rlo[l] = rlo;
+        s.rhi[l] = rhi;
     }
-    if (l == 0) {
-        uint64_t id = 0;
-        for (uint32_t p = 0; p < n; p++) id = (id << 4) | p;
-        s.fast = fast;
-        s.skip0 = raw == id;
+    const uint32_t fast = __builtin_amdgcn_ballot_w64(l < nr && !normal) == 0;
+    // round 0 as given is (0 1)(2 3)...: player p at position p
+    const uint64_t id = 0xfedcba9876543210ull & (n >= 16 ? ~0ull : ((1ull << (4 * n)) - 1ull));
+    const uint32_t skip0 = (uint32_t)(__builtin_amdgcn_ballot_w64(l == 0 && row == id) != 0);
+    if (!fast && l < nr) {
+        uint64_t raw = 0;   // the players in given order, first most significant
+        for (uint32_t p = 0; p < n; p++) raw = (raw << 4) | nib(row, p);
+        s.key[l] = raw;
     }
     __syncthreads();
+    KfUni u{nr, n, n / 2, fast, skip0, 0u, 0u, 0u};
+    if (fast) {
+        if (l < n) {
+            uint32_t mlo = 0, mhi = 0;   // 15 - round, so that absent pairs read 15 after inverting
+            for (uint32_t k = 0; k < nr; k++) tset(mlo, mhi, tget(s.plo[k], s.phi[k], l), 15u - k);
+            s.meet[l] = ~(((uint64_t)mhi << 32) | mlo);
+        }
+        uint32_t glo = 0, ghi = 0;
+        bool sorted = true;   // rounds in order of player 0's partner
+        uint32_t prev = 0;
+        for (uint32_t k = 0; k < nr; k++) {
+            const uint32_t g = kf_uni(s.plo[k]) & 0xfu;
+            tset(glo, ghi, k, g);
+            sorted = sorted && (k == 0 || g > prev);
+            prev = g;
+        }
+        u.glo = glo;
+        u.ghi = ghi;
+        u.ident = skip0 && sorted;
+    }
+    __syncthreads();
+    return u;
 }
 
-// Relabelling r against the matrix: -1 smaller, 0 equal, 1 larger.  A is the
-// slot arrangement of r (tab[r / nr >> np]).
-__device__ __forceinline__ int kf_compare(const KfLds &s, uint64_t A, uint32_t r, uint32_t nr, uint32_t n,
-                                          uint32_t np, bool fast, uint32_t skip0) {
+// Label-order comparison of relabelling r (I = inverse slot arrangement).
+__device__ __forceinline__ int kf_compare_fast(const KfLds &s, const KfUni &u, uint64_t I, uint32_t r) {
+    const uint32_t day = r % u.nr, q = r / u.nr;
+    const uint32_t t = q & ((1u << u.np) - 1u);
+    const uint32_t dlo = s.rlo[day], dhi = s.rhi[day];
+    const uint32_t slo = (uint32_t)I, shi = (uint32_t)(I >> 32);
+    // orientation of slot i (bit np-1-i of t) as byte i of f: 0xff = flipped
+    const uint32_t tr = __builtin_bitreverse32(t) >> (32u - u.np);
+    const uint32_t flo = ((tr & 0xfu) * 0x00204081u & 0x01010101u) * 0xffu;
+    const uint32_t fhi = (((tr >> 4) & 0xfu) * 0x00204081u & 0x01010101u) * 0xffu;
+    // byte j: the pair that receives labels 2j, 2j+1, and whether it is flipped
+    const uint32_t plo = __builtin_amdgcn_perm(dhi, dlo, slo), phi = __builtin_amdgcn_perm(dhi, dlo, shi);
+    const uint32_t mlo = __builtin_amdgcn_perm(fhi, flo, slo), mhi = __builtin_amdgcn_perm(fhi, flo, shi);
+    const uint32_t wlo = ((plo >> 4) & 0x0f0f0f0fu) | ((plo << 4) & 0xf0f0f0f0u);
+    const uint32_t whi = ((phi >> 4) & 0x0f0f0f0fu) | ((phi << 4) & 0xf0f0f0f0u);
+    const uint32_t vlo = (wlo & mlo) | (plo & ~mlo), vhi = (whi & mhi) | (phi & ~mhi);   // label -> player
+    const uint64_t mt = s.meet[vlo & 0xfu];
+    const uint32_t elo = (uint32_t)mt, ehi = (uint32_t)(mt >> 32);
+    uint32_t i = 0;
+    for (uint32_t L = 1; L < u.n; L++) {
+        const uint32_t k = tget(elo, ehi, tget(vlo, vhi, L));
+        if (k == 15u) continue;
+        if (i == 0 && u.skip0) {
+            i = 1;
+            continue;
+        }
+        const uint32_t g = tget(u.glo, u.ghi, i);
+        if (L != g) return L < g ? -1 : 1;
+        const uint32_t klo = s.plo[k], khi = s.phi[k];
+        const uint32_t qlo = s.plo[i], qhi = s.phi[i];
+        for (uint32_t v = 1; v < u.n; v++) {
+            const uint32_t gv = tget(qlo, qhi, v);                       // given: partner of v
+            const uint32_t z = tget(klo, khi, tget(vlo, vhi, v));        // player whose label is v's partner
+            if (z != tget(vlo, vhi, gv)) {
+                const uint32_t lz = tfind(vlo, vhi, z);
+                return lz < gv ? -1 : 1;
+            }
+        }
+        i++;
+    }
+    return 0;
+}
+
+// General-form comparison of relabelling r (A = slot arrangement).
+__device__ __forceinline__ int kf_compare_general(const KfLds &s, const KfUni &u, uint64_t A, uint32_t r) {
+    const uint32_t nr = u.nr, n = u.n, np = u.np;
     const uint32_t day = r % nr, q = r / nr;
     const uint32_t t = q & ((1u << np) - 1u);
     const uint32_t dlo = s.rlo[day], dhi = s.rhi[day];
@@ -150,7 +249,7 @@ __device__ __forceinline__ int kf_compare(const KfLds &s, uint64_t A, uint32_t r
         tset(kylo, kyhi, key, k);
     }
     uint32_t i = 0;
-    if (skip0) {
+    if (u.skip0) {
         present &= present - 1;
         i = 1;
     }
@@ -158,28 +257,14 @@ __device__ __forceinline__ int kf_compare(const KfLds &s, uint64_t A, uint32_t r
         const uint32_t key = __builtin_ctz(present);
         present &= present - 1;
         const uint32_t k = tget(kylo, kyhi, key);
-        uint64_t enc;
-        if (fast) {
-            const uint32_t rlo = s.rlo[k], rhi = s.rhi[k];
-            uint32_t elo = 0, ehi = 0;   // relabelled partner array, reversed
-            for (uint32_t j = 0; j < np; j++) {
-                const uint32_t w = j < 4 ? rlo : rhi;
-                const uint32_t x = tget(tlo, thi, __builtin_amdgcn_ubfe(w, j << 3, 4));
-                const uint32_t y = tget(tlo, thi, __builtin_amdgcn_ubfe(w, (j << 3) + 4, 4));
-                rset(elo, ehi, x, y);
-                rset(elo, ehi, y, x);
-            }
-            enc = ((uint64_t)ehi << 32) | elo;
-        } else {
-            const uint32_t klo = s.plo[k], khi = s.phi[k];
-            uint32_t qlo = 0, qhi = 0;   // relabelled partner array
-            for (uint32_t x = 0; x < n; x++)
-                tset(qlo, qhi, tget(tlo, thi, x), tget(tlo, thi, tget(klo, khi, x)));
-            enc = 0;   // pairs (v, partner) for v ascending, v < partner
-            for (uint32_t v = 0; v < n; v++) {
-                const uint32_t w = tget(qlo, qhi, v);
-                if (w > v) enc = (enc << 8) | (v << 4) | w;
-            }
+        const uint32_t klo = s.plo[k], khi = s.phi[k];
+        uint32_t qlo = 0, qhi = 0;   // relabelled partner array
+        for (uint32_t x = 0; x < n; x++)
+            tset(qlo, qhi, tget(tlo, thi, x), tget(tlo, thi, tget(klo, khi, x)));
+        uint64_t enc = 0;   // pairs (v, partner) for v ascending, v < partner
+        for (uint32_t v = 0; v < n; v++) {
+            const uint32_t w = tget(qlo, qhi, v);
+            if (w > v) enc = (enc << 8) | (v << 4) | w;
         }
         const uint64_t e = s.key[i];
         if (enc != e) return enc < e ? -1 : 1;
@@ -192,14 +277,18 @@ __device__ __forceinline__ int kf_compare(const KfLds &s, uint64_t A, uint32_t r
 // `stop`, every few chunks the scan reads the least witness published for the
 // candidate and returns KF_PRUNED once its next chunk starts past it.  The
 // arrangement of the next chunk is loaded while the current one is compared.
-__device__ __forceinline__ uint32_t kf_scan(const KfLds &s, const uint64_t *tab, uint32_t lo, uint32_t hi,
-                                            uint32_t nr, uint32_t n, uint32_t np, const uint32_t *stop,
-                                            uint64_t &auts, uint32_t l) {
-    const bool fast = s.fast != 0;
-    const uint32_t skip0 = s.skip0;
-    const uint32_t shift = np;
+// A0: this lane's entry of the inverse table for relabelling lo + l, loaded
+// by the caller before staging (reloaded from the forward table when the
+// candidate takes the general form).
+__device__ __forceinline__ uint32_t kf_scan(const KfLds &s, const KfUni &u, const KfTables &qp, uint32_t lo,
+                                            uint32_t hi, const uint32_t *stop, uint64_t &auts, uint32_t l,
+                                            uint64_t A0) {
+    const uint32_t nr = u.nr, np = u.np;
+    const bool fast = u.fast != 0;
+    const uint32_t ident = u.ident;
+    const uint64_t *tab = (fast ? qp.inv : qp.base) + qp.offset[np];
     uint32_t since = 0;
-    uint64_t A = lo + l < hi ? tab[(lo + l) / nr >> shift] : 0;
+    uint64_t A = fast ? A0 : (lo + l < hi ? tab[(lo + l) / nr >> np] : 0);
     for (uint32_t base = lo; base < hi; base += 64) {
         if (stop && ++since == KF_STOP_EVERY) {
             since = 0;
@@ -208,8 +297,10 @@ __device__ __forceinline__ uint32_t kf_scan(const KfLds &s, const uint64_t *tab,
         }
         const uint32_t r = base + l;
         const uint32_t rn = r + 64;
-        const uint64_t An = rn < hi ? tab[rn / nr >> shift] : 0;
-        const int icmp = r < hi ? kf_compare(s, A, r, nr, n, np, fast, skip0) : 1;
+        const uint64_t An = rn < hi ? tab[rn / nr >> np] : 0;
+        int icmp = 1;
+        if (r == 0 && ident) icmp = 0;
+        else if (r < hi) icmp = fast ? kf_compare_fast(s, u, A, r) : kf_compare_general(s, u, A, r);
         const uint64_t smaller = __builtin_amdgcn_ballot_w64(icmp < 0);
         const uint64_t equal = __builtin_amdgcn_ballot_w64(r < hi && icmp == 0);
         if (smaller) {
@@ -233,7 +324,7 @@ __device__ __forceinline__ void kf_write(const KfOut &o, uint32_t ci, uint32_t w
 }
 
 __global__ void kf_queue_reset_kernel(uint32_t *head) {
-    if (threadIdx.x < 3) head[KF_HEAD_WORD * threadIdx.x] = 0u;
+    if (threadIdx.x <= KF_MAX_WINDOWS + 1) head[KF_HEAD_WORD * threadIdx.x] = 0u;   // in bounds: see KF_HEAD_WORD
 }
 
 __device__ __forceinline__ bool kf_meta_ok(uint32_t m, uint32_t stride) {
@@ -270,16 +361,17 @@ __global__ __launch_bounds__(64) void kf_check_kernel(uint32_t ncand, uint32_t s
             if (l == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
         } else {
             const uint32_t nr = m & 0xff, n = (m >> 8) & 0xff, np = n / 2;
-            kf_stage(s, row, nr, n, l);
             const uint32_t total = kf_total(qp, np, nr);
             const uint32_t hi = (limit && limit < total) ? limit : total;
+            const uint64_t A0 = l < hi ? qp.inv[qp.offset[np] + (l / nr >> np)] : 0;
+            const KfUni u = kf_stage(s, row, nr, n, l);
             uint64_t auts = 0;
-            const uint32_t witness = kf_scan(s, qp.base + qp.offset[np], 0, hi, nr, n, np, nullptr, auts, l);
+            const uint32_t witness = kf_scan(s, u, qp, 0, hi, nullptr, auts, l, A0);
             if (l == 0) {
                 if (witness == KF_NONE && hi < total) {
                     const uint32_t j = atomicAdd(&head[KF_HEAD_WORD], 1u);
-                    sp.list[j] = ci;
-                    sp.part[j] = auts;
+                    sp.list[0][j] = ci;
+                    sp.part[0][j] = auts;
                     sp.best[j] = KF_NONE;
                 } else {
                     kf_write(o, ci, witness, auts, total);
@@ -303,41 +395,38 @@ __global__ __launch_bounds__(64) void kf_check_kernel(uint32_t ncand, uint32_t s
     }
 }
 
-// Pass 2: ranges of the listed candidates' remaining relabellings.
+// Window p: ranges of the listed candidates' relabellings in [wlo, whi).
 __global__ __launch_bounds__(64) void kf_range_kernel(uint32_t stride, const uint64_t *__restrict__ rounds,
-                                                      const uint32_t *__restrict__ meta, uint32_t *head,
-                                                      KfTables qp, uint32_t limit, KfSplit sp) {
+                                                      const uint32_t *__restrict__ meta, const uint32_t *head,
+                                                      KfTables qp, KfSplit sp, uint32_t p, uint32_t wlo,
+                                                      uint32_t whi) {
     __shared__ KfLds s;
-    __shared__ uint32_t s_item;
     const uint32_t l = threadIdx.x;
-    const uint32_t listed = head[KF_HEAD_WORD];
+    const uint32_t listed = head[KF_HEAD_WORD * (1 + p)];
+    const uint32_t *list = sp.list[p & 1];
     const uint32_t G = kf_ranges(listed, sp.range_words, sp.max_ranges);
     const uint32_t items = listed * G;
-    for (uint32_t it = 0;; it++) {
-        uint32_t item = blockIdx.x;
-        if (it) {
-            if (l == 0) s_item = atomicAdd(&head[2 * KF_HEAD_WORD], 1u) + gridDim.x;
-            __syncthreads();
-            item = s_item;
-            __syncthreads();
-        }
-        if (item >= items) break;
+    // Items by a fixed grid stride: the ranges of one candidate are of one
+    // size and lie on consecutive items, so they spread over the waves.  (One
+    // queue word taken by every wave per item would serialise the items.)
+    for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
         const uint32_t j = item / G, g = item - j * G;
-        const uint32_t ci = sp.list[j];
+        const uint32_t ci = list[j];
         const uint32_t m = meta[ci];
         const uint32_t nr = m & 0xff, n = (m >> 8) & 0xff, np = n / 2;
         const uint32_t total = kf_total(qp, np, nr);
-        const uint32_t chunks = (total - limit + 63u) / 64u;
-        const uint32_t per = (chunks + G - 1u) / G;
-        const uint64_t lo64 = (uint64_t)limit + (uint64_t)g * per * 64u;
+        const uint32_t end = whi < total ? whi : total;   // listed: wlo < total
+        const uint32_t per = ((end - wlo + 63u) / 64u + G - 1u) / G;
+        const uint64_t lo64 = (uint64_t)wlo + (uint64_t)g * per * 64u;
         uint32_t witness = KF_NONE;
         uint64_t auts = 0;
-        if (lo64 < total) {
+        if (lo64 < end) {
             const uint32_t lo = (uint32_t)lo64;
-            const uint32_t hi = total - lo > per * 64u ? lo + per * 64u : total;
+            const uint32_t hi = end - lo > per * 64u ? lo + per * 64u : end;
             const uint32_t rl = stride < KF_MAX_ROUNDS ? stride : KF_MAX_ROUNDS;
-            kf_stage(s, l < rl ? rounds[(uint64_t)ci * stride + l] : 0ull, nr, n, l);
-            witness = kf_scan(s, qp.base + qp.offset[np], lo, hi, nr, n, np, &sp.best[j], auts, l);
+            const uint64_t A0 = lo + l < hi ? qp.inv[qp.offset[np] + ((lo + l) / nr >> np)] : 0;
+            const KfUni u = kf_stage(s, l < rl ? rounds[(uint64_t)ci * stride + l] : 0ull, nr, n, l);
+            witness = kf_scan(s, u, qp, lo, hi, &sp.best[j], auts, l, A0);
             if (l == 0 && witness < KF_PRUNED) atomicMin(&sp.best[j], witness);
         }
         if (l == 0) sp.ranges[(uint64_t)j * G + g] = (auts << 32) | witness;
@@ -345,26 +434,53 @@ __global__ __launch_bounds__(64) void kf_range_kernel(uint32_t stride, const uin
     }
 }
 
-// Pass 3: per listed candidate, the ranges in index order.
-__global__ void kf_combine_kernel(const uint32_t *__restrict__ meta, const uint32_t *head, KfTables qp,
-                                  KfSplit sp, KfOut o) {
-    const uint32_t listed = head[KF_HEAD_WORD];
+__device__ __forceinline__ uint64_t kf_wave_sum(uint64_t x) {
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, d, 64);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), d, 64);
+        x += ((uint64_t)hi << 32) | lo;
+    }
+    return x;
+}
+
+// Window p: per listed candidate (one wave each), the ranges in index order.
+__global__ __launch_bounds__(64) void kf_combine_kernel(const uint32_t *__restrict__ meta, uint32_t *head,
+                                                        KfTables qp, KfSplit sp, KfOut o, uint32_t p,
+                                                        uint32_t whi) {
+    const uint32_t l = threadIdx.x;
+    const uint32_t listed = head[KF_HEAD_WORD * (1 + p)];
     const uint32_t G = kf_ranges(listed, sp.range_words, sp.max_ranges);
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < listed; j += gridDim.x * blockDim.x) {
-        const uint32_t ci = sp.list[j];
-        const uint32_t m = meta[ci];
-        const uint32_t nr = m & 0xff, np = ((m >> 8) & 0xff) / 2;
-        uint64_t auts = sp.part[j];
+    const uint32_t *list = sp.list[p & 1];
+    const uint64_t *part = sp.part[p & 1];
+    for (uint32_t j = blockIdx.x; j < listed; j += gridDim.x) {
+        uint64_t auts = 0;
         uint32_t witness = KF_NONE;
-        for (uint32_t g = 0; g < G; g++) {
-            const uint64_t e = sp.ranges[(uint64_t)j * G + g];
-            auts += e >> 32;
-            if ((uint32_t)e != KF_NONE) {
-                witness = (uint32_t)e;   // a real index: see the ordering argument above
+        for (uint32_t g0 = 0; g0 < G; g0 += 64) {
+            const uint64_t e = g0 + l < G ? sp.ranges[(uint64_t)j * G + g0 + l] : (uint64_t)KF_NONE;
+            const uint32_t w = (uint32_t)e;
+            const uint64_t hit = __builtin_amdgcn_ballot_w64(w != KF_NONE);
+            const uint32_t upto = hit ? (uint32_t)__builtin_ctzll(hit) : 63u;
+            auts += kf_wave_sum(l <= upto ? (e >> 32) : 0ull);
+            if (hit) {
+                // a real index: see the ordering argument at the top
+                witness = (uint32_t)__builtin_amdgcn_readlane((int)w, (int)upto);
                 break;
             }
         }
-        kf_write(o, ci, witness, auts, kf_total(qp, np, nr));
+        if (l == 0) {
+            const uint32_t ci = list[j];
+            const uint32_t m = meta[ci];
+            const uint32_t total = kf_total(qp, ((m >> 8) & 0xff) / 2, m & 0xff);
+            auts += part[j];
+            if (witness != KF_NONE || whi >= total) {
+                kf_write(o, ci, witness, auts, total);
+            } else {
+                const uint32_t k = atomicAdd(&head[KF_HEAD_WORD * (2 + p)], 1u);
+                sp.list[(p + 1) & 1][k] = ci;
+                sp.part[(p + 1) & 1][k] = auts;
+                sp.best[k] = KF_NONE;
+            }
+        }
     }
 }
 
@@ -374,10 +490,15 @@ hipError_t kf_launch(uint32_t blocks, uint32_t range_blocks, uint32_t ncand, uin
     hipLaunchKernelGGL(kf_queue_reset_kernel, dim3(1), dim3(64), 0, stream, head);
     hipLaunchKernelGGL(kf_check_kernel, dim3(blocks), dim3(64), 0, stream, ncand, stride, rounds, meta, o,
                        head, qp, limit, sp);
-    if (limit) {
-        hipLaunchKernelGGL(kf_range_kernel, dim3(range_blocks), dim3(64), 0, stream, stride, rounds, meta, head, qp,
-                           limit, sp);
-        hipLaunchKernelGGL(kf_combine_kernel, dim3(64), dim3(256), 0, stream, meta, head, qp, sp, o);
+    // windows up to the largest index space (K_16, 15 rounds: 40320 * 256 * 15 < 2^28)
+    uint64_t wlo = limit;
+    for (uint32_t p = 0; limit && p < KF_MAX_WINDOWS && wlo < (1ull << 28); p++) {
+        const uint64_t whi = wlo * KF_WINDOW_GROWTH;
+        const uint32_t w1 = whi > 0xffffffffull ? 0xffffffffu : (uint32_t)whi;
+        hipLaunchKernelGGL(kf_range_kernel, dim3(range_blocks), dim3(64), 0, stream, stride, rounds, meta,
+                           (const uint32_t *)head, qp, sp, p, (uint32_t)wlo, w1);
+        hipLaunchKernelGGL(kf_combine_kernel, dim3(1024), dim3(64), 0, stream, meta, head, qp, sp, o, p, w1);
+        wlo = whi;
     }
     return hipGetLastError();
 }

@@ -18,12 +18,13 @@ static uint32_t factorial(uint32_t k) {
 size_t kf_table_words() {
     size_t w = 0;
     for (uint32_t k = 0; k <= KF_MAX_SLOTS; k++) w += factorial(k);
-    return w;
+    return 2 * w;   // arrangements, then their inverses
 }
 
 // Fuchs' countdown QuickPerm (the order canonizator visits the arrangements
 // of the pair slots in, k-Factorization/TripleSys/Source/cnv.cpp:283-315).
 void kf_build_tables(uint64_t *words, uint32_t *offset, uint32_t *count) {
+    const size_t half = kf_table_words() / 2;
     uint32_t at = 0;
     for (uint32_t k = 0; k <= KF_MAX_SLOTS; k++) {
         offset[k] = at;
@@ -32,9 +33,16 @@ void kf_build_tables(uint64_t *words, uint32_t *offset, uint32_t *count) {
         for (uint32_t i = 0; i < k; i++) { a[i] = (uint8_t)(2 * i); p[i] = (uint8_t)i; }
         p[k] = (uint8_t)k;
         auto emit = [&]() {
-            uint64_t w = 0;
+            uint64_t w = 0, v = 0;
             for (uint32_t i = 0; i < k; i++) w |= (uint64_t)a[i] << (8 * i);
-            words[at++] = w;
+            for (uint32_t j = 0; j < KF_MAX_SLOTS; j++) v |= (uint64_t)KF_PERM_ZERO << (8 * j);
+            for (uint32_t i = 0; i < k; i++) {
+                const uint32_t j = a[i] / 2;
+                v = (v & ~(0xffull << (8 * j))) | ((uint64_t)i << (8 * j));
+            }
+            words[at] = w;
+            words[half + at] = v;
+            at++;
         };
         emit();
         uint32_t i = 1;

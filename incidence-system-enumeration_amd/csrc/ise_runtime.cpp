@@ -52,8 +52,8 @@ struct ise_ctx {
     uint64_t *d_kf_tab = nullptr;
     ise::KfTables kf_tab{};
     uint64_t *d_kf_witness = nullptr, *d_kf_auts = nullptr, *d_kf_tried = nullptr;
-    // split launch (ise_kf_set_split): list, first-pass counts, published
-    // witnesses for kf_cap candidates, and the range results
+    // split launch (ise_kf_set_split): two lists of kf_cap entries (candidate,
+    // automorphisms before the window), published witnesses, range results
     uint32_t kf_first_chunks = ISE_DEFAULT_KF_FIRST_CHUNKS, kf_max_ranges = ISE_DEFAULT_KF_MAX_RANGES;
     uint32_t *d_kf_list = nullptr, *d_kf_best = nullptr;
     uint64_t *d_kf_part = nullptr, *d_kf_ranges = nullptr;
@@ -185,9 +185,9 @@ int ensure_kf_split(ise_ctx *c, size_t n) {
     c->d_kf_part = c->d_kf_ranges = nullptr;
     c->kf_cap = c->kf_range_words = 0;
     const size_t words = n > KF_MIN_RANGE_WORDS ? n : KF_MIN_RANGE_WORDS;
-    HIPCHK(c, hipMalloc(&c->d_kf_list, n * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&c->d_kf_list, 2 * n * sizeof(uint32_t)));
     HIPCHK(c, hipMalloc(&c->d_kf_best, n * sizeof(uint32_t)));
-    HIPCHK(c, hipMalloc(&c->d_kf_part, n * sizeof(uint64_t)));
+    HIPCHK(c, hipMalloc(&c->d_kf_part, 2 * n * sizeof(uint64_t)));
     HIPCHK(c, hipMalloc(&c->d_kf_ranges, words * sizeof(uint64_t)));
     c->kf_cap = n;
     c->kf_range_words = words;
@@ -293,9 +293,9 @@ int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride) {
         if (e == hipSuccess) e = dalloc(&c->d_kf_witness, n);
         if (e == hipSuccess) e = dalloc(&c->d_kf_auts, n);
         if (e == hipSuccess) e = dalloc(&c->d_kf_tried, n);
-        if (e == hipSuccess) e = dalloc(&c->d_kf_list, n);
+        if (e == hipSuccess) e = dalloc(&c->d_kf_list, 2 * n);
         if (e == hipSuccess) e = dalloc(&c->d_kf_best, n);
-        if (e == hipSuccess) e = dalloc(&c->d_kf_part, n);
+        if (e == hipSuccess) e = dalloc(&c->d_kf_part, 2 * n);
         if (e == hipSuccess) e = dalloc(&c->d_kf_ranges, n > KF_MIN_RANGE_WORDS ? n : KF_MIN_RANGE_WORDS);
         if (e == hipSuccess) {
             c->kf_cap = n;
@@ -308,6 +308,7 @@ int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride) {
             if (e == hipSuccess)
                 e = hipMemcpy(c->d_kf_tab, words.data(), words.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
             c->kf_tab.base = c->d_kf_tab;
+            c->kf_tab.inv = c->d_kf_tab + words.size() / 2;
         }
         if (e == hipSuccess) e = hipEventCreate(&c->ev0);
         if (e == hipSuccess) e = hipEventCreate(&c->ev1);
@@ -608,8 +609,10 @@ static int kf_launch_on(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t 
         const int rc = ensure_kf_split(c, n);
         if (rc) return rc;
     }
-    const ise::KfSplit sp{c->d_kf_list, c->d_kf_part, c->d_kf_best, c->d_kf_ranges, (uint32_t)c->kf_range_words,
-                          c->kf_max_ranges};
+    const size_t cap = c->kf_cap;
+    const ise::KfSplit sp{{c->d_kf_list, c->d_kf_list ? c->d_kf_list + cap : nullptr},
+                          {c->d_kf_part, c->d_kf_part ? c->d_kf_part + cap : nullptr},
+                          c->d_kf_best, c->d_kf_ranges, (uint32_t)c->kf_range_words, c->kf_max_ranges};
     HIPCHK(c, ise::kf_launch(blocks, range_blocks, n, stride, d_rounds, d_meta, kf_out(d_out), d_queue,
                              c->kf_tab, limit, sp, s));
     return ISE_OK;
