@@ -256,6 +256,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the K_12 and enumeration figures")
+    ap.add_argument("--only-kfact", action="store_true", help="run only the K_12 figure (profiling)")
     ap.add_argument("--build-batch", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run one device-to-device copy of this many bytes (counter calibration)")
@@ -267,6 +268,11 @@ def main():
 
     if args.build_batch:
         build_batch(args.random, args.build_batch)
+        return
+    if args.only_kfact:
+        import ise_amd
+        with ise_amd.Context(capacity=1 << 18, stride=16) as kctx:
+            print(json.dumps({"kfact_k12": kfact_rate(kctx, args.steps, args.warmup)}), flush=True)
         return
 
     t_setup = time.time()
