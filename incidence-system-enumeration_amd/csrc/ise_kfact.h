@@ -68,6 +68,10 @@ void kf_build_tables(uint64_t *words, uint32_t *offset, uint32_t *count);
 hipError_t kf_launch(uint32_t blocks, uint32_t range_blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
                      const uint32_t *meta, const KfOut &o, uint32_t *head, const KfTables &qp,
                      uint32_t limit, const KfSplit &sp, hipStream_t stream);
+// Two-row canonizer of perfect 1-factorizations (cnvCheck2P1F): one wave per
+// candidate, candidates by grid stride.
+hipError_t kf2_launch(uint32_t blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
+                      const uint32_t *meta, const KfOut &o, hipStream_t stream);
 hipError_t kf_occupancy(int *blocks_per_cu);
 
 } // namespace ise

@@ -484,6 +484,133 @@ __global__ __launch_bounds__(64) void kf_combine_kernel(const uint32_t *__restri
     }
 }
 
+// ---- two-row canonizer of perfect 1-factorizations ------------------------
+//
+// alldata::cnvCheck2P1F (k-Factorization/TripleSys/Source/cnvSupport.cpp:3-95)
+// with create2P1FTr (p1fSupport.cpp:3-38) and kmProcessMatrix2p1f
+// (kmSupport.cpp:519-598): in a perfect 1-factorization every pair of rounds
+// is one Hamiltonian cycle, so a relabelling that could give a matrix no
+// larger than the given one must send some ordered pair of rounds (r0, r1)
+// onto rounds 0, 1; walking the two cycles side by side from a start player k
+// and from player 0 fixes it.  Relabelling index r = ((pair * 2 + swap) * n +
+// k) with pair = last * (last - 1) / 2 + ind over last = 1.., ind < last, and
+// (r0, r1) = (ind, last) or, swapped, (last, ind).  Rounds r0, r1 become rounds
+// 0, 1 and are not compared; the relabelled round with partner-of-0 label v
+// (v = 3..nr) is compared with round v - 1 as given, in order of v, a missing
+// v making the relabelling not smaller.  Fewer than 3 rounds: every
+// relabelling is equal.  Rounds must be in normal form and every pair of them
+// one Hamiltonian cycle; otherwise the record is invalid.
+
+// Relabelling r against the matrix: -1 smaller, 0 equal, 1 not smaller.
+__device__ __forceinline__ int kf2_compare(const KfLds &s, const KfUni &u, uint32_t r, uint32_t z0lo,
+                                           uint32_t z0hi, uint32_t z1lo, uint32_t z1hi) {
+    const uint32_t n = u.n, nr = u.nr;
+    const uint32_t k = r % n, q = r / n;
+    const uint32_t pi = q >> 1;
+    uint32_t last = 1;
+    while ((last + 1) * last / 2 <= pi) last++;
+    const uint32_t ind = pi - last * (last - 1) / 2;
+    const uint32_t r0 = (q & 1) ? last : ind, r1 = (q & 1) ? ind : last;
+    const uint32_t alo = s.plo[r0], ahi = s.phi[r0], blo = s.plo[r1], bhi = s.phi[r1];
+    // label -> player: walk round r0 / r1 from k against round 0 / 1 from player 0
+    uint32_t vlo = 0, vhi = 0, v1 = k, it = 0;
+    for (uint32_t st = 0; st < n; st += 2) {
+        tset(vlo, vhi, it, v1);
+        const uint32_t v2 = tget(alo, ahi, v1);
+        it = tget(z0lo, z0hi, it);
+        tset(vlo, vhi, it, v2);
+        it = tget(z1lo, z1hi, it);
+        v1 = tget(blo, bhi, v2);
+    }
+    if (nr < 3) return 0;
+    const uint64_t mt = s.meet[k];
+    const uint32_t elo = (uint32_t)mt, ehi = (uint32_t)(mt >> 32);
+    for (uint32_t p = 2; p < nr; p++) {
+        const uint32_t L = p + 1;
+        const uint32_t m = tget(elo, ehi, tget(vlo, vhi, L));
+        if (m == 15u) return 1;
+        const uint32_t g = tget(u.glo, u.ghi, p);
+        if (L != g) return L < g ? -1 : 1;
+        const uint32_t klo = s.plo[m], khi = s.phi[m];
+        const uint32_t qlo = s.plo[p], qhi = s.phi[p];
+        for (uint32_t v = 1; v < n; v++) {
+            const uint32_t gv = tget(qlo, qhi, v);                  // given: partner of v
+            const uint32_t z = tget(klo, khi, tget(vlo, vhi, v));   // player whose label is v's partner
+            if (z != tget(vlo, vhi, gv)) return tfind(vlo, vhi, z) < gv ? -1 : 1;
+        }
+    }
+    return 0;
+}
+
+__global__ __launch_bounds__(64) void kf2_check_kernel(uint32_t ncand, uint32_t stride,
+                                                       const uint64_t *__restrict__ rounds,
+                                                       const uint32_t *__restrict__ meta, KfOut o) {
+    __shared__ KfLds s;
+    const uint32_t l = threadIdx.x;
+    const uint32_t rl = stride < KF_MAX_ROUNDS ? stride : KF_MAX_ROUNDS;
+    // at most 15 * 14 * 16 relabellings per candidate: a fixed grid stride
+    for (uint32_t ci = blockIdx.x; ci < ncand; ci += gridDim.x) {
+        const uint32_t m = meta[ci];
+        if (!kf_meta_ok(m, stride)) {
+            if (l == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
+            continue;
+        }
+        const uint32_t nr = m & 0xff, n = (m >> 8) & 0xff;
+        const KfUni u = kf_stage(s, l < rl ? rounds[(uint64_t)ci * stride + l] : 0ull, nr, n, l);
+        // every pair of rounds one Hamiltonian cycle
+        bool bad = !u.fast;
+        const uint32_t npairs = nr * (nr - 1) / 2;
+        for (uint32_t b0 = 0; b0 < npairs && !bad; b0 += 64) {
+            bool ham = true;
+            const uint32_t pi = b0 + l;
+            if (pi < npairs) {
+                uint32_t b = 1;
+                while ((b + 1) * b / 2 <= pi) b++;
+                const uint32_t a = pi - b * (b - 1) / 2;
+                const uint32_t alo = s.plo[a], ahi = s.phi[a], blo = s.plo[b], bhi = s.phi[b];
+                uint32_t v = 0, len = 0;
+                do {
+                    v = tget(blo, bhi, tget(alo, ahi, v));
+                    len += 2;
+                } while (v != 0 && len <= n);
+                ham = len == n;
+            }
+            bad = __builtin_amdgcn_ballot_w64(!ham) != 0;
+        }
+        if (bad) {
+            if (l == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
+            __syncthreads();
+            continue;
+        }
+        const uint32_t z0lo = kf_uni(s.plo[0]), z0hi = kf_uni(s.phi[0]);
+        const uint32_t z1lo = nr > 1 ? kf_uni(s.plo[1]) : 0u, z1hi = nr > 1 ? kf_uni(s.phi[1]) : 0u;
+        const uint32_t total = nr * (nr - 1) * n;
+        uint64_t auts = 0;
+        uint32_t witness = KF_NONE;
+        for (uint32_t base = 0; base < total; base += 64) {
+            const uint32_t r = base + l;
+            const int icmp = r < total ? kf2_compare(s, u, r, z0lo, z0hi, z1lo, z1hi) : 1;
+            const uint64_t smaller = __builtin_amdgcn_ballot_w64(icmp < 0);
+            const uint64_t equal = __builtin_amdgcn_ballot_w64(r < total && icmp == 0);
+            if (smaller) {
+                const uint32_t first = __builtin_ctzll(smaller);
+                auts += __popcll(equal & ((1ull << first) - 1ull));
+                witness = base + first;
+                break;
+            }
+            auts += __popcll(equal);
+        }
+        if (l == 0) kf_write(o, ci, witness, auts, total);
+        __syncthreads();
+    }
+}
+
+hipError_t kf2_launch(uint32_t blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
+                      const uint32_t *meta, const KfOut &o, hipStream_t stream) {
+    hipLaunchKernelGGL(kf2_check_kernel, dim3(blocks), dim3(64), 0, stream, ncand, stride, rounds, meta, o);
+    return hipGetLastError();
+}
+
 hipError_t kf_launch(uint32_t blocks, uint32_t range_blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
                      const uint32_t *meta, const KfOut &o, uint32_t *head, const KfTables &qp,
                      uint32_t limit, const KfSplit &sp, hipStream_t stream) {

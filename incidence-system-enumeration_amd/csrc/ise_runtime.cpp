@@ -653,36 +653,105 @@ int ise_kf_check_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride, con
     return rc;
 }
 
-int ise_kf_check(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
-                 ise_kf_results *h) {
+static int kf2_launch_on(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                         const uint32_t *d_meta, const ise_kf_results &d_out, hipStream_t s) {
+    uint32_t blocks = c->cfg.blocks;
+    if (!blocks) {
+        int per_cu = 0;
+        HIPCHK(c, ise::kf_occupancy(&per_cu));
+        blocks = (uint32_t)(per_cu < 1 ? 1 : per_cu) * (uint32_t)c->num_cus;
+        if (blocks > n) blocks = n ? n : 1;
+    }
+    HIPCHK(c, ise::kf2_launch(blocks, n, stride, d_rounds, d_meta, kf_out(d_out), s));
+    return ISE_OK;
+}
+
+int ise_kf_check_2p1f_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                                   const uint32_t *d_meta, const ise_kf_results *d_out, void *stream) {
+    if (!c || !d_out || !d_out->verdict || stride < 1 || stride > ISE_MAX_ROWS) return ISE_ERR_ARG;
+    if (n == 0) return ISE_OK;
+    if (!d_rounds || !d_meta) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    return kf2_launch_on(c, n, stride, d_rounds, d_meta, *d_out, stream ? (hipStream_t)stream : c->stream);
+}
+
+int ise_kf_check_2p1f_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                                         const uint32_t *d_meta, const ise_kf_results *d_out,
+                                         uint32_t repeats, float *ms_out) {
+    if (!c || !ms_out || repeats == 0) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<hipEvent_t> ev(repeats + 1, nullptr);
+    int rc = ISE_OK;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = ISE_ERR_HIP;
+    if (rc == ISE_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) rc = ISE_ERR_HIP;
+    for (uint32_t i = 0; i < repeats && rc == ISE_OK; i++) {
+        rc = ise_kf_check_2p1f_batch_device(c, n, stride, d_rounds, d_meta, d_out, nullptr);
+        if (rc == ISE_OK && hipEventRecord(ev[i + 1], c->stream) != hipSuccess) rc = ISE_ERR_HIP;
+    }
+    if (rc == ISE_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = ISE_ERR_HIP;
+    for (uint32_t i = 0; i < repeats && rc == ISE_OK; i++)
+        if (hipEventElapsedTime(&ms_out[i], ev[i], ev[i + 1]) != hipSuccess) rc = ISE_ERR_HIP;
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc == ISE_ERR_HIP && c->err.empty()) c->err = "timed launch failed";
+    return rc;
+}
+
+// Host records of 1-factorizations: edge-disjoint perfect matchings of the
+// stated size; for the two-row canonizer also rounds in normal form (pairs
+// (a, b) with a < b, pairs by first player) and every pair of rounds one
+// Hamiltonian cycle.
+static bool kf_records_ok(uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta, bool p1f) {
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t nr = meta[i] & 0xff, np = (meta[i] >> 8) & 0xff;
+        if (np < 4 || np > ise::KF_MAX_PLAYERS || (np & 1) || nr < 1 || nr > np - 1 || nr > stride)
+            return false;
+        uint32_t used[ise::KF_MAX_PLAYERS] = {0};
+        uint8_t part[ise::KF_MAX_ROUNDS][ise::KF_MAX_PLAYERS];
+        for (uint32_t r = 0; r < nr; r++) {
+            const uint64_t w = rounds[(uint64_t)i * stride + r];
+            uint32_t seen = 0, prev = 0;
+            for (uint32_t p = 0; p < np; p += 2) {
+                const uint32_t x = (uint32_t)(w >> (4 * p)) & 0xf, y = (uint32_t)(w >> (4 * p + 4)) & 0xf;
+                if (x >= np || y >= np || x == y || (seen >> x & 1) || (seen >> y & 1) || (used[x] >> y & 1))
+                    return false;
+                if (p1f && (x > y || (p && x < prev))) return false;
+                prev = x;
+                seen |= (1u << x) | (1u << y);
+                used[x] |= 1u << y;
+                used[y] |= 1u << x;
+                part[r][x] = (uint8_t)y;
+                part[r][y] = (uint8_t)x;
+            }
+        }
+        if (p1f)
+            for (uint32_t a = 0; a < nr; a++)
+                for (uint32_t b = a + 1; b < nr; b++) {
+                    uint32_t v = 0, len = 0;
+                    do {
+                        v = part[b][part[a][v]];
+                        len += 2;
+                    } while (v != 0 && len <= np);
+                    if (len != np) return false;
+                }
+    }
+    return true;
+}
+
+static int kf_host_run(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
+                       ise_kf_results *h, bool p1f) {
     if (!c || !h || !h->verdict || (n && (!rounds || !meta)) || stride < 1 || stride > c->stride)
         return ISE_ERR_ARG;
     if (n > c->capacity) return ISE_ERR_CAPACITY;
     if (n == 0) return ISE_OK;
-    // host-side validation: edge-disjoint perfect matchings of the stated size
-    for (uint32_t i = 0; i < n; i++) {
-        const uint32_t nr = meta[i] & 0xff, np = (meta[i] >> 8) & 0xff;
-        if (np < 4 || np > ise::KF_MAX_PLAYERS || (np & 1) || nr < 1 || nr > np - 1 || nr > stride)
-            return ISE_ERR_ARG;
-        uint32_t used[ise::KF_MAX_PLAYERS] = {0};
-        for (uint32_t r = 0; r < nr; r++) {
-            const uint64_t w = rounds[(uint64_t)i * stride + r];
-            uint32_t seen = 0;
-            for (uint32_t p = 0; p < np; p += 2) {
-                const uint32_t x = (uint32_t)(w >> (4 * p)) & 0xf, y = (uint32_t)(w >> (4 * p + 4)) & 0xf;
-                if (x >= np || y >= np || x == y || (seen >> x & 1) || (seen >> y & 1) || (used[x] >> y & 1))
-                    return ISE_ERR_ARG;
-                seen |= (1u << x) | (1u << y);
-                used[x] |= 1u << y;
-                used[y] |= 1u << x;
-            }
-        }
-    }
+    if (!kf_records_ok(n, stride, rounds, meta, p1f)) return ISE_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpyAsync(c->d_rows, rounds, (size_t)n * stride * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_meta, meta, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
     ise_kf_results d{c->d.verdict, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried};
-    int rc = kf_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->d_head, c->stream);
+    const int rc = p1f ? kf2_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->stream)
+                       : kf_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->d_head, c->stream);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(h->verdict, d.verdict, n, hipMemcpyDeviceToHost, c->stream));
     if (h->witness) HIPCHK(c, hipMemcpyAsync(h->witness, d.witness, n * 8, hipMemcpyDeviceToHost, c->stream));
@@ -690,6 +759,16 @@ int ise_kf_check(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds
     if (h->n_tried) HIPCHK(c, hipMemcpyAsync(h->n_tried, d.n_tried, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ISE_OK;
+}
+
+int ise_kf_check(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
+                 ise_kf_results *h) {
+    return kf_host_run(c, n, stride, rounds, meta, h, false);
+}
+
+int ise_kf_check_2p1f(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
+                      ise_kf_results *h) {
+    return kf_host_run(c, n, stride, rounds, meta, h, true);
 }
 
 int ise_set_order(ise_ctx *c, int mode, uint32_t hunt_budget, const uint8_t *d_order, uint32_t order_dim) {

@@ -130,6 +130,10 @@ def lib():
         L.ise_kf_check_batch_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P,
                                                       U32, P]
         L.ise_kf_random.argtypes = [I, I, I, U64, U64, U32, P, P]
+        L.ise_kf_check_2p1f.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
+        L.ise_kf_check_2p1f_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P]
+        L.ise_kf_check_2p1f_batch_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct),
+                                                           U32, P]
         _lib = L
     return _lib
 
@@ -351,8 +355,10 @@ class Context:
         (first_chunks=0: one wave per candidate to its decision)."""
         self._check(self._L.ise_kf_set_split(self.h, first_chunks, max_ranges))
 
-    def kf_check(self, rounds, meta):
-        """Canonicity of partial 1-factorizations (host arrays in and out)."""
+    def kf_check(self, rounds, meta, two_row_p1f=False):
+        """Canonicity of partial 1-factorizations (host arrays in and out);
+        two_row_p1f: the two-row canonizer of perfect 1-factorizations."""
+        fn = self._L.ise_kf_check_2p1f if two_row_p1f else self._L.ise_kf_check
         rounds = np.ascontiguousarray(rounds, dtype=np.uint64)
         meta = np.ascontiguousarray(meta, dtype=np.uint32)
         n, stride = rounds.shape
@@ -361,8 +367,8 @@ class Context:
             cnt = min(self.capacity, n - off)
             r = kf_alloc(cnt)
             s = kf_struct(r)
-            self._check(self._L.ise_kf_check(self.h, cnt, stride, rounds[off:off + cnt].ctypes.data,
-                                             meta[off:off + cnt].ctypes.data, ctypes.byref(s)))
+            self._check(fn(self.h, cnt, stride, rounds[off:off + cnt].ctypes.data,
+                           meta[off:off + cnt].ctypes.data, ctypes.byref(s)))
             outs.append(r)
         if not outs:
             return kf_alloc(0)
@@ -378,6 +384,17 @@ class Context:
         self._check(self._L.ise_kf_check_batch_device_timed(self.h, n, stride, d_rounds.ptr, d_meta.ptr,
                                                             ctypes.byref(res_struct), d_head.ptr, repeats,
                                                             ms.ctypes.data))
+        return ms
+
+    def kf_check_2p1f_device(self, n, stride, d_rounds, d_meta, res_struct):
+        self._check(self._L.ise_kf_check_2p1f_batch_device(self.h, n, stride, d_rounds.ptr, d_meta.ptr,
+                                                           ctypes.byref(res_struct), None))
+
+    def kf_time_2p1f_device(self, n, stride, d_rounds, d_meta, res_struct, repeats):
+        ms = np.zeros(repeats, np.float32)
+        self._check(self._L.ise_kf_check_2p1f_batch_device_timed(self.h, n, stride, d_rounds.ptr, d_meta.ptr,
+                                                                 ctypes.byref(res_struct), repeats,
+                                                                 ms.ctypes.data))
         return ms
 
     def last_kernel_ms(self):

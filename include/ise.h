@@ -364,6 +364,30 @@ int ise_kf_check_batch_device_timed(ise_ctx *ctx, uint32_t n, uint32_t stride,
                                     const ise_kf_results *d_out, uint32_t *d_queue,
                                     uint32_t repeats, float *ms_out);
 
+/* Two-row canonizer of perfect 1-factorizations (every pair of rounds one
+ * Hamiltonian cycle): alldata::cnvCheck2P1F with create2P1FTr and
+ * kmProcessMatrix2p1f (k-Factorization/TripleSys/Source/cnvSupport.cpp:3-95,
+ * p1fSupport.cpp:3-38, kmSupport.cpp:519-598), the check the reference selects
+ * for P1F runs (allData.cpp:123-145).  Same records and result fields as
+ * ise_kf_check; the relabellings tried are those sending an ordered pair of
+ * rounds onto rounds 0, 1, in the reference's order: index
+ * ((pair * 2 + swap) * n + start player), pair = last * (last - 1) / 2 + ind
+ * for rounds ind < last, swap choosing (ind, last) or (last, ind) as the
+ * rounds sent to 0 and 1.  Rounds sent to 0, 1 are not compared; the round
+ * whose relabelled partner of player 0 is v (v = 3..nrounds) is compared with
+ * round v - 1 as given.  Rounds must be in normal form (pairs (a, b) with
+ * a < b, pairs by first player) and every pair of them one Hamiltonian cycle:
+ * host arrays are rejected with ISE_ERR_ARG otherwise, device records get
+ * ISE_VERDICT_INVALID.  Host path: synchronous, up to the context capacity;
+ * device path: no allocation. */
+int ise_kf_check_2p1f(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *rounds,
+                      const uint32_t *meta, ise_kf_results *out_host);
+int ise_kf_check_2p1f_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                                   const uint32_t *d_meta, const ise_kf_results *d_out, void *stream);
+int ise_kf_check_2p1f_batch_device_timed(ise_ctx *ctx, uint32_t n, uint32_t stride,
+                                         const uint64_t *d_rounds, const uint32_t *d_meta,
+                                         const ise_kf_results *d_out, uint32_t repeats, float *ms_out);
+
 /* Split launch of the 1-factorization test.  A canonical candidate must try
  * every relabelling (46080 x rounds for K_12), which on one wave would be the
  * batch's tail.  With first_chunks > 0 every candidate first runs up to

@@ -153,6 +153,10 @@ void orc_kf_normalize(uint8_t *mat, int n, int nrounds);
 /* Brute force over all n! player relabellings (n <= 8): canonical or not,
  * and the number of relabellings that reproduce the matrix. */
 int orc_kf_bruteforce(const uint8_t *mat, int n, int nrounds, uint64_t *n_auts);
+/* cnvCheck2P1F: the two-row canonizer of perfect 1-factorizations (same
+ * results fields as orc_kf_check; witness = index in its own order). */
+int orc_kf_check_2p1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
+                      uint64_t *n_auts, uint64_t *n_tried);
 
 /* Fuchs' countdown QuickPerm order of the pair slots (k! rows of k bytes,
  * each byte the base label 2*slot); returns k!. */

@@ -156,5 +156,106 @@ int orc_kf_bruteforce(const uint8_t *mat, int n, int nrounds, uint64_t *n_auts)
     return canon;
 }
 
+/*
+ * Two-row canonizer for perfect 1-factorizations: alldata::cnvCheck2P1F
+ * (k-Factorization/TripleSys/Source/cnvSupport.cpp:3-95) with create2P1FTr
+ * (p1fSupport.cpp:3-38) and kmProcessMatrix2p1f / kmProcessOneNot1stRow2
+ * (kmSupport.cpp:357-381, 519-598), for the defaults of the P1F runs
+ * (AutSaveTestedTrs 0, no row precalculation, early exit on the first
+ * smaller matrix).
+ *
+ * Relabellings, in order: for iRowLast = 1..nr-1, indRow = 0..iRowLast-1,
+ * swap = 0, 1 ((r0, r1) = (indRow, iRowLast), then (iRowLast, indRow)), start
+ * player k = 0..n-1: walk the cycle of rounds r0, r1 from k and the cycle of
+ * rounds 0, 1 from player 0 side by side, sending the players of the first to
+ * the players of the second.  Rounds r0, r1 become rounds 0, 1 and are not
+ * compared.  Every other round is relabelled, normalised (pairs sorted, pairs
+ * by first element) and placed by its second entry v (the partner of player
+ * 0): rounds with v = 3..nr are compared, in order of v, with rounds v-1 of the
+ * matrix as given (memcmp); a missing v in 3..nr makes the relabelling not
+ * smaller (return 2 in the reference).  Fewer than 3 rounds: every relabelling
+ * is equal.  |Aut| = relabellings equal before the decision (each is a distinct
+ * player permutation).  Returns 1 canonical, 0 not, -1 bad arguments or a pair
+ * of rounds that is not one Hamiltonian cycle (not a P1F).
+ */
+static int kf_is_p1f_pair(const u8 *pa, const u8 *pb, int n)
+{
+    int v = 0, len = 0;
+    do { v = pb[pa[v]]; len += 2; } while (v != 0 && len <= n);
+    return len == n;
+}
+
+int orc_kf_check_2p1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
+                      uint64_t *n_auts, uint64_t *n_tried)
+{
+    if (n < 4 || n > KF_MAX_N || (n & 1) || nrounds < 1 || nrounds > n - 1) return -1;
+    u8 part[KF_MAX_N][KF_MAX_N];
+    for (int r = 0; r < nrounds; r++)
+        for (int q = 0; q < n; q += 2) {
+            const u8 x = mat[r * n + q], y = mat[r * n + q + 1];
+            part[r][x] = y;
+            part[r][y] = x;
+        }
+    for (int a = 0; a < nrounds; a++)
+        for (int b = a + 1; b < nrounds; b++)
+            if (!kf_is_p1f_pair(part[a], part[b], n)) return -1;
+    uint64_t auts = 0, tried = 0, idx = 0;
+    int verdict = 1;
+    if (witness) *witness = ~0ull;
+    for (int last = 1; last < nrounds && verdict; last++)
+        for (int ind = 0; ind < last && verdict; ind++)
+            for (int swap = 0; swap < 2 && verdict; swap++) {
+                const int r0 = swap ? last : ind, r1 = swap ? ind : last;
+                for (int k = 0; k < n; k++, idx++) {
+                    u8 tr[KF_MAX_N];
+                    int v1 = k, it = 0;   /* create2P1FTr */
+                    for (int s = 0; s < n; s += 2) {
+                        tr[v1] = (u8)it;
+                        const int v2 = part[r0][v1];
+                        it = part[0][it];
+                        tr[v2] = (u8)it;
+                        it = part[1][it];
+                        v1 = part[r1][v2];
+                    }
+                    tried++;
+                    int cmp = 0;
+                    if (nrounds >= 3) {
+                        u8 byv[KF_MAX_N][KF_MAX_N];
+                        int have[KF_MAX_N] = {0};
+                        for (int r = 0; r < nrounds; r++) {
+                            if (r == r0 || r == r1) continue;
+                            u8 row[KF_MAX_N], pres[KF_MAX_N] = {0}, sec[KF_MAX_N];
+                            for (int q = 0; q < n; q += 2) {
+                                u8 x = tr[mat[r * n + q]], y = tr[mat[r * n + q + 1]];
+                                if (x > y) { u8 t = x; x = y; y = t; }
+                                pres[x] = 1;
+                                sec[x] = y;
+                            }
+                            int m = 0;
+                            for (int v = 0; v < n; v++)
+                                if (pres[v]) { row[m++] = (u8)v; row[m++] = sec[v]; }
+                            memcpy(byv[row[1]], row, (size_t)n);
+                            have[row[1]] = 1;
+                        }
+                        cmp = 0;
+                        for (int v = 3; v <= nrounds && cmp == 0; v++) {
+                            if (!have[v]) { cmp = 2; break; }
+                            const int c = memcmp(byv[v], mat + (v - 1) * n, (size_t)n);
+                            cmp = c < 0 ? -1 : (c > 0 ? 1 : 0);
+                        }
+                    }
+                    if (cmp == 0) auts++;
+                    else if (cmp < 0) {
+                        verdict = 0;
+                        if (witness) *witness = idx;
+                        break;
+                    }
+                }
+            }
+    if (n_auts) *n_auts = auts;
+    if (n_tried) *n_tried = tried;
+    return verdict;
+}
+
 /* the QuickPerm visiting order (for tests) */
 long orc_kf_quickperm(int k, uint8_t *out) { return quickperm_table(k, out); }

@@ -60,3 +60,49 @@ def check_witness(mat, perm_row, fail_row):
     assert lex_cmp(permuted, mat) > 0
     first = np.nonzero((permuted != mat).any(axis=1))[0][0]
     assert first == fail_row
+
+
+def p1f_relabel(mat, r0, r1, k):
+    """Relabel a perfect 1-factorization so that rounds r0, r1 become its rounds
+    0, 1 (the cycle of r0, r1 walked from player k against the cycle of rounds
+    0, 1 walked from player 0), then put it in normal form (pairs sorted, pairs
+    by first player, rounds by the partner of player 0)."""
+    mat = np.asarray(mat, dtype=np.int64)
+    nr, n = mat.shape
+    part = np.zeros((nr, n), np.int64)
+    for r in range(nr):
+        part[r, mat[r, 0::2]] = mat[r, 1::2]
+        part[r, mat[r, 1::2]] = mat[r, 0::2]
+    tr = np.full(n, -1, np.int64)
+    v1, it = k, 0
+    for _ in range(n // 2):
+        tr[v1] = it
+        v2 = part[r0, v1]
+        it = part[0, it]
+        tr[v2] = it
+        it = part[1, it]
+        v1 = part[r1, v2]
+    assert (np.sort(tr) == np.arange(n)).all()
+    rows = []
+    for r in range(nr):
+        pairs = np.sort(tr[mat[r].reshape(-1, 2)], axis=1)
+        pairs = pairs[np.argsort(pairs[:, 0])]
+        rows.append(pairs.reshape(-1))
+    rows.sort(key=lambda row: row[1])
+    return np.array(rows, np.uint8)
+
+
+def p1f_candidates(block, count, seed, rmin=3):
+    """Prefixes (rmin.. rounds) of the printed perfect 1-factorizations of one
+    block relabelled through a random pair of rounds and start player: rounds 0
+    and 1 stay the standard pair, the partner of player 0 in round r is r + 1."""
+    rng = np.random.default_rng(seed)
+    mats = [np.array(m["rounds"], np.uint8) for m in block["matrices"]]
+    out = []
+    for _ in range(count):
+        m = mats[int(rng.integers(len(mats)))]
+        nr, n = m.shape
+        r0, r1 = rng.choice(nr, 2, replace=False)
+        full = p1f_relabel(m, int(r0), int(r1), int(rng.integers(n)))
+        out.append(full[:int(rng.integers(rmin, nr + 1))])
+    return out

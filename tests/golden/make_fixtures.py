@@ -13,8 +13,10 @@ JSON fixtures next to this script:
                      (expected output of the text formatters).
 * kfact_k10.json     the 396 canonical 1-factorizations of K_10 with |Aut(M)|,
                      k-Factorization/Results/Complete_graphs/10/10x9x2/K_OrderedMatrices.txt,
-                     and the uniform K_12 factorization of
-                     .../12/12x11x2_66/U_OrderedMatrices.txt.
+                     the uniform K_12 factorization of
+                     .../12/12x11x2_66/U_OrderedMatrices.txt, and the perfect
+                     1-factorizations of K_10 and K_14 with |Aut(M)|,
+                     .../10/10x9x2/P_OrderedMatrices.txt, .../14/14x13x2/P_OrderedMatrices.txt.
 
 Run: python tests/golden/make_fixtures.py  (needs the reference checkout).
 """
@@ -140,6 +142,12 @@ def kfact():
         mats = parse_kfact(read(p10), 10)
         res["k10"] = dict(n=10, source=os.path.relpath(p10, REF),
                           matrices=[dict(rounds=m, aut=a) for m, a in mats])
+    for key, n, rel in (("p10", 10, "10/10x9x2/P_OrderedMatrices.txt"), ("p14", 14, "14/14x13x2/P_OrderedMatrices.txt")):
+        path = os.path.join(REF, "k-Factorization/Results/Complete_graphs", rel)
+        if os.path.exists(path):
+            mats = parse_kfact(read(path), n)
+            res[key] = dict(n=n, source=os.path.relpath(path, REF),
+                            matrices=[dict(rounds=m, aut=a) for m, a in mats])
     p12 = glob.glob(os.path.join(REF, "k-Factorization/Results/Complete_graphs/12/12x11x2_66/U_OrderedMatrices.txt"))
     if p12:
         mats = parse_kfact(read(p12[0]), 12)

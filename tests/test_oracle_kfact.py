@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import ise_amd
-from _helpers import golden
+from _helpers import golden, p1f_candidates, p1f_relabel
 
 
 def test_k10_printed_factorizations(oracle):
@@ -103,3 +103,71 @@ def test_random_k12_generator_is_valid_and_normal(oracle):
                 assert x < y and (x, y) not in edges
                 edges.add((x, y))
         assert (oracle.kf_normalize(mat) == mat).all()
+
+
+# ---- two-row canonizer of perfect 1-factorizations (cnvCheck2P1F) ----------
+
+def _p1f_decode(idx, nr, n):
+    """Relabelling index of the two-row order -> (round sent to 0, round sent to 1, start player)."""
+    k, q = idx % n, idx // n
+    swap, pi = q & 1, q >> 1
+    last = 1
+    while (last + 1) * last // 2 <= pi:
+        last += 1
+    ind = pi - last * (last - 1) // 2
+    return (last, ind, k) if swap else (ind, last, k)
+
+
+@pytest.mark.parametrize("key,tried", [("p10", 9 * 8 * 10), ("p14", 13 * 12 * 14)])
+def test_p1f_printed_factorizations(oracle, key, tried):
+    """The reference's printed perfect 1-factorizations of K_10 (1) and K_14
+    (23) are canonical for the two-row canonizer with their printed |Aut(M)|,
+    after trying every ordered pair of rounds from every start player."""
+    block = golden("kfact.json")[key]
+    assert len(block["matrices"]) == (1 if key == "p10" else 23)
+    for m in block["matrices"]:
+        v, w, auts, t = oracle.kf_check_2p1f(m["rounds"])
+        assert v == 1 and auts == m["aut"] and t == tried
+
+
+def test_p1f_agrees_with_the_general_canonizer(oracle):
+    """On prefixes of relabelled perfect 1-factorizations of K_10 (rounds 0, 1
+    the standard pair, round r pairing 0 with r + 1) the two-row canonizer and
+    the general one give the same verdict and, when canonical, the same |Aut|;
+    the two-row witness relabelling gives a lexicographically smaller normal
+    form (checked with an independent relabelling)."""
+    cands = p1f_candidates(golden("kfact.json")["p10"], 400, 0x2F1)
+    canon = 0
+    for mat in cands:
+        v, w, auts, t = oracle.kf_check_2p1f(mat)
+        gv, _, gauts, _ = oracle.kf_check(mat)
+        assert v == gv
+        if v:
+            canon += 1
+            assert auts == gauts
+        else:
+            nr, n = mat.shape
+            r0, r1, k = _p1f_decode(w, nr, n)
+            rel = p1f_relabel(mat, r0, r1, k)
+            assert bytes(rel.reshape(-1)) < bytes(mat.reshape(-1))
+            assert t == w + 1
+    assert 0 < canon < len(cands)
+
+
+def test_p1f_two_rounds_and_rejection(oracle):
+    """Two rounds (one Hamiltonian cycle): all 2n relabellings are
+    automorphisms (the dihedral group of the cycle); a factorization with a
+    pair of rounds that is not one cycle is rejected."""
+    p = np.array(golden("kfact.json")["p14"]["matrices"][0]["rounds"], np.uint8)
+    v, w, auts, t = oracle.kf_check_2p1f(p[:2])
+    assert v == 1 and auts == t == 2 * 14
+    v, w, auts, t = oracle.kf_check_2p1f(p[:1])
+    assert v == 1 and t == 0
+    not_p1f = [np.array(m["rounds"], np.uint8) for m in golden("kfact.json")["k10"]["matrices"]]
+    bad = 0
+    for m in not_p1f:
+        try:
+            oracle.kf_check_2p1f(m)
+        except ValueError:
+            bad += 1
+    assert bad == len(not_p1f) - 1   # the one perfect factorization among the 396
