@@ -174,6 +174,35 @@ def kfact_rate(ctx, steps, warmup):
             "one_pass_ms_per_step": timed["one_pass"]}
 
 
+def kfact_p1f_rate(ctx, steps, warmup):
+    """Perfect 1-factorizations of K_14, two-row canonizer (cnvCheck2P1F):
+    2^18 seeded relabelled prefixes (2-13 rounds) of the 23 perfect
+    1-factorizations of K_14 the reference prints (tests/golden/kfact.json),
+    one launch per step."""
+    import ise_amd
+    with open(os.path.join(ROOT, "tests", "golden", "kfact.json")) as f:
+        block = json.load(f)["p14"]
+    src_rows, src_meta = ise_amd.kf_pack([m["rounds"] for m in block["matrices"]])
+    rows, meta = ise_amd.kf_relabel_p1f(src_rows, src_meta, 1 << 18, 2, 0x14F1, stride=16)
+    n = len(meta)
+    d_rows = ctx.device_array(rows.shape, np.uint64, rows)
+    d_meta = ctx.device_array(meta.shape, np.uint32, meta)
+    dev, res = ctx.kf_device_results(n)
+    if warmup:
+        ctx.kf_time_2p1f_device(n, 16, d_rows, d_meta, res, warmup)
+    ms = float(np.mean(ctx.kf_time_2p1f_device(n, 16, d_rows, d_meta, res, steps)))
+    verdict = dev["verdict"].to_host()
+    tried = dev["n_tried"].to_host()
+    for b in list(dev.values()) + [d_rows, d_meta]:
+        b.free()
+    if (verdict > 1).any():
+        raise RuntimeError("invalid record in the perfect 1-factorization batch")
+    return {"workload": "K_14 perfect 1-factorizations, two-row canonizer: 2^18 relabelled prefixes "
+                        "(2-13 rounds, seed 0x14F1) of the 23 printed ones",
+            "candidates": n, "ms_per_step": ms, "candidates_per_s": n / (ms / 1e3),
+            "canonical": int((verdict == 1).sum()), "relabellings_per_s": float(tried.sum()) / (ms / 1e3)}
+
+
 def stream_rates(ctx, steps, warmup):
     """The other candidate-stream configurations, plain order, HBM-resident:
     2-(15,3,1) -- every candidate the enumerator tests at depth 4-15 plus 2^20
@@ -272,7 +301,8 @@ def main():
     if args.only_kfact:
         import ise_amd
         with ise_amd.Context(capacity=1 << 18, stride=16) as kctx:
-            print(json.dumps({"kfact_k12": kfact_rate(kctx, args.steps, args.warmup)}), flush=True)
+            print(json.dumps({"kfact_k12": kfact_rate(kctx, args.steps, args.warmup),
+                              "kfact_p1f_k14": kfact_p1f_rate(kctx, args.steps, args.warmup)}), flush=True)
         return
 
     t_setup = time.time()
@@ -403,6 +433,7 @@ def main():
         buf.free()
     if not args.no_extra:
         line["kfact_k12"] = kfact_rate(ctx, max(3, args.steps // 4), 1)
+        line["kfact_p1f_k14"] = kfact_p1f_rate(ctx, max(3, args.steps // 4), 1)
         line["streams"] = stream_rates(ctx, max(3, args.steps // 4), 1)
         line["enumeration"] = enumeration_rate()
         line["host_buffers"] = host_buffer_rate(rows, meta)

@@ -169,3 +169,82 @@ extern "C" int ise_kf_random(int players, int rmin, int rmax, uint64_t count, ui
     for (auto &x : th) x.join();
     return ISE_OK;
 }
+
+// Seeded relabelled prefixes of perfect 1-factorizations: for candidate i a
+// source matrix, an ordered pair of its rounds (r0, r1) and a start player k
+// are drawn; the cycle of rounds r0, r1 from k is sent onto the cycle of the
+// source's rounds 0, 1 from player 0, the matrix is put in normal form and its
+// first nr rounds (nr uniform in [rmin, rounds of the source]) are kept.  With
+// sources whose rounds 0, 1 are the standard pair and whose round r pairs 0
+// with r + 1 (as the reference prints them), every candidate has that form too.
+extern "C" int ise_kf_relabel_p1f(const uint64_t *src, const uint32_t *src_meta, uint32_t nsrc, uint32_t src_stride,
+                                  uint64_t count, int rmin, uint64_t seed, uint32_t stride, uint64_t *rounds_out,
+                                  uint32_t *meta_out) {
+    using namespace ise;
+    if (!src || !src_meta || nsrc == 0 || src_stride < 1 || rmin < 1 || !rounds_out || !meta_out) return ISE_ERR_ARG;
+    for (uint32_t s = 0; s < nsrc; s++) {
+        const int nr = (int)(src_meta[s] & 0xff), n = (int)((src_meta[s] >> 8) & 0xff);
+        if (n < 4 || n > (int)KF_MAX_PLAYERS || (n & 1) || nr < 2 || nr > n - 1 || nr < rmin ||
+            (uint32_t)nr > src_stride || (uint32_t)nr > stride)
+            return ISE_ERR_ARG;
+    }
+    for (uint64_t i = 0; i < count; i++) {
+        uint64_t rng = seed ^ (0xe7037ed1a0b428dbull * (i + 1));
+        mix(rng);
+        const uint32_t s = (uint32_t)(mix(rng) % nsrc);
+        const int nr = (int)(src_meta[s] & 0xff), n = (int)((src_meta[s] >> 8) & 0xff);
+        uint8_t part[KF_MAX_ROUNDS][KF_MAX_PLAYERS];
+        for (int r = 0; r < nr; r++) {
+            const uint64_t w = src[(uint64_t)s * src_stride + r];
+            uint32_t cover = 0;
+            for (int q = 0; q < n; q += 2) {
+                const uint8_t x = (uint8_t)((w >> (4 * q)) & 0xf), y = (uint8_t)((w >> (4 * q + 4)) & 0xf);
+                if (x >= n || y >= n) return ISE_ERR_ARG;
+                part[r][x] = y;
+                part[r][y] = x;
+                cover |= (1u << x) | (1u << y);
+            }
+            if (cover != (1u << n) - 1) return ISE_ERR_ARG;   // not a perfect matching
+        }
+        const int r0 = (int)(mix(rng) % (uint64_t)nr);
+        int r1 = (int)(mix(rng) % (uint64_t)(nr - 1));
+        if (r1 >= r0) r1++;
+        int v1 = (int)(mix(rng) % (uint64_t)n), it = 0;
+        uint8_t tr[KF_MAX_PLAYERS];
+        uint32_t seen = 0;
+        for (int st = 0; st < n; st += 2) {   // the walk of create2P1FTr
+            tr[v1] = (uint8_t)it;
+            const int v2 = part[r0][v1];
+            it = part[0][it];
+            tr[v2] = (uint8_t)it;
+            it = part[1][it];
+            seen |= (1u << v1) | (1u << v2);
+            v1 = part[r1][v2];
+        }
+        if (seen != (1u << n) - 1) return ISE_ERR_ARG;   // rounds r0, r1 not one cycle
+        // normal form of the relabelled matrix: pairs sorted, pairs by first
+        // player, rounds by the partner of player 0
+        struct Row { uint8_t v[KF_MAX_PLAYERS]; };
+        Row rows[KF_MAX_ROUNDS];
+        for (int r = 0; r < nr; r++) {
+            uint8_t q[KF_MAX_PLAYERS];
+            for (int x = 0; x < n; x++) q[tr[x]] = tr[part[r][x]];
+            int k = 0;
+            for (int v = 0; v < n; v++)
+                if (q[v] > v) { rows[r].v[k++] = (uint8_t)v; rows[r].v[k++] = q[v]; }
+        }
+        std::sort(rows, rows + nr, [](const Row &x, const Row &y) { return x.v[1] < y.v[1]; });
+        uint64_t full[KF_MAX_ROUNDS];
+        for (int r = 0; r < nr; r++) {
+            uint64_t w = 0;
+            for (int q = 0; q < n; q++) w |= (uint64_t)rows[r].v[q] << (4 * q);
+            full[r] = w;
+        }
+        const int keep = rmin + (int)(mix(rng) % (uint64_t)(nr - rmin + 1));
+        uint64_t *dst = rounds_out + i * stride;
+        memset(dst, 0, stride * sizeof(uint64_t));
+        for (int r = 0; r < keep; r++) dst[r] = full[r];
+        meta_out[i] = (uint32_t)keep | ((uint32_t)n << 8);
+    }
+    return ISE_OK;
+}

@@ -131,6 +131,7 @@ def lib():
                                                       U32, P]
         L.ise_kf_random.argtypes = [I, I, I, U64, U64, U32, P, P]
         L.ise_kf_check_2p1f.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
+        L.ise_kf_relabel_p1f.argtypes = [P, P, U32, U32, U64, I, U64, U32, P, P]
         L.ise_kf_check_2p1f_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P]
         L.ise_kf_check_2p1f_batch_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct),
                                                            U32, P]
@@ -688,6 +689,19 @@ def kf_random(players, rmin, rmax, count, seed, stride=16):
     return rows, meta
 
 
+def kf_relabel_p1f(src_rows, src_meta, count, rmin, seed, stride=16):
+    """Seeded relabelled prefixes of perfect 1-factorizations (host code)."""
+    src_rows = np.ascontiguousarray(src_rows, dtype=np.uint64)
+    src_meta = np.ascontiguousarray(src_meta, dtype=np.uint32)
+    rows = np.zeros((count, stride), np.uint64)
+    meta = np.zeros(count, np.uint32)
+    rc = lib().ise_kf_relabel_p1f(src_rows.ctypes.data, src_meta.ctypes.data, len(src_meta), src_rows.shape[1],
+                                  count, rmin, seed, stride, rows.ctypes.data, meta.ctypes.data)
+    if rc != OK:
+        raise ISEError(rc, "ise_kf_relabel_p1f")
+    return rows, meta
+
+
 class CanonicityChecker:
     """Reference-shaped single-matrix checker on the GPU."""
 
@@ -753,7 +767,7 @@ class GPUCanonChecker:
 
 
 __all__ = [
-    "QUEUE_WORDS", "ORDER_PLAIN", "ORDER_HUNT", "ORDER_LEARNED", "kf_random", "kf_pack", "kf_unpack", "kf_alloc", "KF_FIELDS", "Context", "DeviceArray", "ResultsStruct", "CanonicityChecker", "GPUCanonChecker", "ISEError", "random_extensions", "lib",
+    "QUEUE_WORDS", "ORDER_PLAIN", "ORDER_HUNT", "ORDER_LEARNED", "kf_random", "kf_relabel_p1f", "kf_pack", "kf_unpack", "kf_alloc", "KF_FIELDS", "Context", "DeviceArray", "ResultsStruct", "CanonicityChecker", "GPUCanonChecker", "ISEError", "random_extensions", "lib",
     "design_params", "alloc_results", "pack_meta", "pack_rows", "unpack_rows", "MAX_ROWS",
     "MAX_COLS", "SAVE_ROW_PERMUTATIONS", "SAVE_COL_PERMUTATIONS", "SAVE_ROW_TO_CHANGE",
     "VERDICT_CANONICAL", "VERDICT_NONCANONICAL", "VERDICT_LIMIT", "VERDICT_INVALID",

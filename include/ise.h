@@ -388,6 +388,17 @@ int ise_kf_check_2p1f_batch_device_timed(ise_ctx *ctx, uint32_t n, uint32_t stri
                                          const uint64_t *d_rounds, const uint32_t *d_meta,
                                          const ise_kf_results *d_out, uint32_t repeats, float *ms_out);
 
+/* Seeded relabelled prefixes of perfect 1-factorizations (a candidate stream
+ * for the two-row canonizer): for candidate i a source matrix, an ordered pair
+ * of its rounds and a start player are drawn, the pair is sent onto the
+ * source's rounds 0, 1 as create2P1FTr does, the result is put in normal form
+ * and its first nrounds rounds are kept (nrounds uniform in [rmin, rounds of
+ * the source]).  Sources: nsrc records of src_stride words with src_meta as
+ * for ise_kf_check.  Deterministic per seed. */
+int ise_kf_relabel_p1f(const uint64_t *src, const uint32_t *src_meta, uint32_t nsrc, uint32_t src_stride,
+                       uint64_t count, int rmin, uint64_t seed, uint32_t stride, uint64_t *rounds_out,
+                       uint32_t *meta_out);
+
 /* Split launch of the 1-factorization test.  A canonical candidate must try
  * every relabelling (46080 x rounds for K_12), which on one wave would be the
  * batch's tail.  With first_chunks > 0 every candidate first runs up to

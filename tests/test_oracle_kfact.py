@@ -171,3 +171,25 @@ def test_p1f_two_rounds_and_rejection(oracle):
         except ValueError:
             bad += 1
     assert bad == len(not_p1f) - 1   # the one perfect factorization among the 396
+
+
+def test_p1f_relabel_generator(oracle):
+    """ise_kf_relabel_p1f: every candidate is a perfect partial factorization in
+    normal form with the sources' rounds 0, 1 and round r pairing 0 with r + 1,
+    and a relabelling of a prefix-closed part of a printed matrix."""
+    block = golden("kfact.json")["p14"]
+    src_rows, src_meta = ise_amd.kf_pack([m["rounds"] for m in block["matrices"]])
+    rows, meta = ise_amd.kf_relabel_p1f(src_rows, src_meta, 600, 2, 0x5EED)
+    rows2, meta2 = ise_amd.kf_relabel_p1f(src_rows, src_meta, 600, 2, 0x5EED)
+    assert np.array_equal(rows, rows2) and np.array_equal(meta, meta2)
+    std = np.array(block["matrices"][0]["rounds"], np.uint8)[:2]
+    canon = 0
+    for i in range(len(meta)):
+        nr = int(meta[i]) & 0xff
+        assert 2 <= nr <= 13 and (int(meta[i]) >> 8) == 14
+        mat = ise_amd.kf_unpack(rows[i, :nr], 14)
+        assert (mat[:2] == std).all()
+        assert (mat[:, 1] == np.arange(1, nr + 1)).all()
+        assert (oracle.kf_normalize(mat) == mat).all()
+        canon += oracle.kf_check_2p1f(mat)[0]   # raises unless perfect
+    assert 0 < canon < len(meta)
