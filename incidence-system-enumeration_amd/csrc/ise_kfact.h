@@ -41,6 +41,9 @@ constexpr uint32_t KF_MAX_WINDOWS = 6;   // from a limit of 64: 64 * 16^6 > 2^28
 // a one-pass launch, head[KF_HEAD_WORD * (1 + p)] the length of the list of
 // candidates entering window p.
 constexpr uint32_t KF_HEAD_WORD = 32;
+// head[KF_QUAD_WORD]: the number of candidates the first look lists for the
+// first pass (in list[1], which window 0 refills only after the first pass).
+constexpr uint32_t KF_QUAD_WORD = 16;
 static_assert(KF_HEAD_WORD * (KF_MAX_WINDOWS + 1) < ISE_QUEUE_WORDS, "queue words of a split launch");
 
 // Context buffers of a split launch.  Window p reads list[p & 1] / part[p & 1]

@@ -325,6 +325,7 @@ __device__ __forceinline__ void kf_write(const KfOut &o, uint32_t ci, uint32_t w
 
 __global__ void kf_queue_reset_kernel(uint32_t *head) {
     if (threadIdx.x <= KF_MAX_WINDOWS + 1) head[KF_HEAD_WORD * threadIdx.x] = 0u;   // in bounds: see KF_HEAD_WORD
+    if (threadIdx.x == 0) head[KF_QUAD_WORD] = 0u;
 }
 
 __device__ __forceinline__ bool kf_meta_ok(uint32_t m, uint32_t stride) {
@@ -333,27 +334,129 @@ __device__ __forceinline__ bool kf_meta_ok(uint32_t m, uint32_t stride) {
            nr <= KF_MAX_ROUNDS;
 }
 
+// Quarter-wave staging for kf_quad_kernel: the 16 lanes of group q stage
+// candidate q's tables into s (lane gl of the group takes round gl and player
+// gl); the per-candidate values stay per lane (uniform within the group).
+// Every lane of the wave calls it (it holds the barriers); `ok` marks groups
+// with a valid candidate.
+__device__ __forceinline__ KfUni kf_stage_q(KfLds &s, uint64_t row, uint32_t nr, uint32_t n, uint32_t gl,
+                                            uint32_t q, bool ok) {
+    const uint32_t rlo = (uint32_t)row, rhi = (uint32_t)(row >> 32);
+    bool normal = true;
+    if (ok && gl < nr) {
+        uint32_t plo = 0, phi = 0, prev = 0;
+        for (uint32_t p = 0; p < n; p += 2) {
+            const uint32_t w = p < 8 ? rlo : rhi;
+            const uint32_t x = __builtin_amdgcn_ubfe(w, p << 2, 4), y = __builtin_amdgcn_ubfe(w, (p << 2) + 4, 4);
+            tset(plo, phi, x, y);
+            tset(plo, phi, y, x);
+            normal = normal && x < y && (p == 0 || x > prev);
+            prev = x;
+        }
+        s.plo[gl] = plo;
+        s.phi[gl] = phi;
+        s.rlo[gl] = rlo;
+        s.rhi[gl] = rhi;
+    }
+    const uint32_t sh = 16u * q;
+    const uint32_t fast = ((__builtin_amdgcn_ballot_w64(ok && gl < nr && !normal) >> sh) & 0xffffull) == 0;
+    const uint64_t id = 0xfedcba9876543210ull & (n >= 16 ? ~0ull : ((1ull << (4 * n)) - 1ull));
+    const uint32_t skip0 = (uint32_t)((__builtin_amdgcn_ballot_w64(ok && gl == 0 && row == id) >> sh) & 1ull);
+    __syncthreads();
+    KfUni u{nr, n, n / 2, fast, skip0, 0u, 0u, 0u};
+    if (ok && fast) {
+        if (gl < n) {
+            uint32_t mlo = 0, mhi = 0;
+            for (uint32_t k = 0; k < nr; k++) tset(mlo, mhi, tget(s.plo[k], s.phi[k], gl), 15u - k);
+            s.meet[gl] = ~(((uint64_t)mhi << 32) | mlo);
+        }
+        uint32_t glo = 0, ghi = 0, prev = 0;
+        bool sorted = true;
+        for (uint32_t k = 0; k < nr; k++) {
+            const uint32_t g = s.plo[k] & 0xfu;
+            tset(glo, ghi, k, g);
+            sorted = sorted && (k == 0 || g > prev);
+            prev = g;
+        }
+        u.glo = glo;
+        u.ghi = ghi;
+        u.ident = skip0 && sorted;
+    }
+    __syncthreads();
+    return u;
+}
+
+// First look, four candidates per wave: the 16 lanes of a group compare the
+// first 16 relabellings of their candidate (one in a hundred of the random K_12
+// partials needs more than two).  A candidate decided there (a smaller
+// relabelling, or all of them tried) gets its results; the others, and those
+// whose rounds are not in normal form, are listed for the first pass, which
+// runs them from relabelling 0 again.
+__global__ __launch_bounds__(64) void kf_quad_kernel(uint32_t ncand, uint32_t stride,
+                                                     const uint64_t *__restrict__ rounds,
+                                                     const uint32_t *__restrict__ meta, KfOut o, uint32_t *head,
+                                                     KfTables qp, KfSplit sp) {
+    __shared__ KfLds s4[4];
+    const uint32_t l = threadIdx.x, q = l >> 4, gl = l & 15u;
+    const uint32_t rl = stride < KF_MAX_ROUNDS ? stride : KF_MAX_ROUNDS;
+    KfLds &s = s4[q];
+    for (uint32_t base = blockIdx.x * 4u; base < ncand; base += gridDim.x * 4u) {
+        const uint32_t ci = base + q;
+        const bool active = ci < ncand;
+        const uint32_t m = active ? meta[ci] : 0u;
+        const bool ok = active && kf_meta_ok(m, stride);
+        if (active && !ok && gl == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
+        const uint32_t nr = ok ? (m & 0xff) : 0u, n = ok ? ((m >> 8) & 0xff) : 0u, np = n / 2;
+        const uint32_t total = ok ? kf_total(qp, np, nr) : 0u;
+        const uint64_t row = (ok && gl < rl) ? rounds[(uint64_t)ci * stride + gl] : 0ull;
+        const uint64_t A = gl < total ? qp.inv[qp.offset[np] + (gl / nr >> np)] : 0ull;
+        const KfUni u = kf_stage_q(s, row, nr, n, gl, q, ok);
+        int icmp = 1;
+        if (gl < total && u.fast) icmp = (gl == 0 && u.ident) ? 0 : kf_compare_fast(s, u, A, gl);
+        const uint32_t sh = 16u * q;
+        const uint32_t sm = (uint32_t)((__builtin_amdgcn_ballot_w64(icmp < 0) >> sh) & 0xffffull);
+        const uint32_t eq = (uint32_t)((__builtin_amdgcn_ballot_w64(gl < total && icmp == 0) >> sh) & 0xffffull);
+        if (ok && gl == 0) {
+            if (u.fast && sm) {
+                const uint32_t first = (uint32_t)__builtin_ctz(sm);
+                kf_write(o, ci, first, (uint64_t)__builtin_popcount(eq & ((1u << first) - 1u)), total);
+            } else if (u.fast && total <= 16u) {
+                kf_write(o, ci, KF_NONE, (uint64_t)__builtin_popcount(eq), total);
+            } else {
+                const uint32_t j = atomicAdd(&head[KF_QUAD_WORD], 1u);
+                sp.list[1][j] = ci;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Pass 1 (or the whole test when limit == 0).
+// With in_list, the candidates are those the first look listed (head[KF_QUAD_WORD]).
 __global__ __launch_bounds__(64) void kf_check_kernel(uint32_t ncand, uint32_t stride,
                                                       const uint64_t *__restrict__ rounds,
                                                       const uint32_t *__restrict__ meta, KfOut o,
-                                                      uint32_t *head, KfTables qp, uint32_t limit, KfSplit sp) {
+                                                      uint32_t *head, KfTables qp, uint32_t limit, KfSplit sp,
+                                                      const uint32_t *in_list) {
     __shared__ KfLds s;
     __shared__ uint32_t s_ci;
     const uint32_t l = threadIdx.x;
     const uint32_t rl = stride < KF_MAX_ROUNDS ? stride : KF_MAX_ROUNDS;   // round words a lane may load
+    const uint32_t N = in_list ? head[KF_QUAD_WORD] : ncand;
     // With a limit every candidate's time is bounded, so the candidates go by
     // a fixed grid stride and the next one is loaded while this one runs (one
     // queue word taken by every wave of the chip would serialise the short
     // tests).  Without one, the first candidate by position, the later ones
     // from the queue.
-    uint32_t ci = blockIdx.x;
-    uint32_t m = ci < ncand ? meta[ci] : 0u;
-    uint64_t row = (ci < ncand && l < rl) ? rounds[(uint64_t)ci * stride + l] : 0ull;
-    while (ci < ncand) {
-        uint32_t cn = ci + gridDim.x, mn = 0;
+    uint32_t i = blockIdx.x;
+    uint32_t ci = i < N ? (in_list ? in_list[i] : i) : 0u;
+    uint32_t m = i < N ? meta[ci] : 0u;
+    uint64_t row = (i < N && l < rl) ? rounds[(uint64_t)ci * stride + l] : 0ull;
+    while (i < N) {
+        uint32_t in = i + gridDim.x, cn = 0, mn = 0;
         uint64_t rown = 0;
-        if (limit && cn < ncand) {
+        if (limit && in < N) {
+            cn = in_list ? in_list[in] : in;
             mn = meta[cn];
             if (l < rl) rown = rounds[(uint64_t)cn * stride + l];
         }
@@ -382,13 +485,14 @@ __global__ __launch_bounds__(64) void kf_check_kernel(uint32_t ncand, uint32_t s
         if (!limit) {
             if (l == 0) s_ci = atomicAdd(head, 1u) + gridDim.x;
             __syncthreads();
-            cn = s_ci;
+            in = cn = s_ci;
             __syncthreads();
             if (cn < ncand) {
                 mn = meta[cn];
                 if (l < rl) rown = rounds[(uint64_t)cn * stride + l];
             }
         }
+        i = in;
         ci = cn;
         m = mn;
         row = rown;
@@ -615,8 +719,13 @@ hipError_t kf_launch(uint32_t blocks, uint32_t range_blocks, uint32_t ncand, uin
                      const uint32_t *meta, const KfOut &o, uint32_t *head, const KfTables &qp,
                      uint32_t limit, const KfSplit &sp, hipStream_t stream) {
     hipLaunchKernelGGL(kf_queue_reset_kernel, dim3(1), dim3(64), 0, stream, head);
+    if (limit) {
+        const uint32_t qblocks = (ncand + 3u) / 4u < range_blocks ? (ncand + 3u) / 4u : range_blocks;
+        hipLaunchKernelGGL(kf_quad_kernel, dim3(qblocks ? qblocks : 1u), dim3(64), 0, stream, ncand, stride, rounds,
+                           meta, o, head, qp, sp);
+    }
     hipLaunchKernelGGL(kf_check_kernel, dim3(blocks), dim3(64), 0, stream, ncand, stride, rounds, meta, o,
-                       head, qp, limit, sp);
+                       head, qp, limit, sp, limit ? (const uint32_t *)sp.list[1] : nullptr);
     // windows up to the largest index space (K_16, 15 rounds: 40320 * 256 * 15 < 2^28)
     uint64_t wlo = limit;
     for (uint32_t p = 0; limit && p < KF_MAX_WINDOWS && wlo < (1ull << 28); p++) {
