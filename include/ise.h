@@ -401,10 +401,11 @@ int ise_kf_relabel_p1f(const uint64_t *src, const uint32_t *src_meta, uint32_t n
 
 /* Split launch of the 1-factorization test.  A canonical candidate must try
  * every relabelling (46080 x rounds for K_12), which on one wave would be the
- * batch's tail.  With first_chunks > 0 every candidate first runs up to
- * 64 x first_chunks relabellings; the rest of each candidate still undecided
- * is cut into up to max_ranges contiguous ranges run by independent waves and
- * combined in plain order (first witness, automorphisms before it).  Results
+ * batch's tail.  With first_chunks > 0 every candidate first gets a look at
+ * its first 16 relabellings (four candidates per wave), those still open run
+ * up to 64 x first_chunks relabellings, and the rest of each candidate still
+ * undecided is cut into up to max_ranges contiguous ranges run by independent
+ * waves and combined in plain order (first witness, automorphisms before it).  Results
  * are identical to first_chunks = 0, one wave per candidate to its decision.
  * No counterpart in the reference, whose test is one sequential loop. */
 #define ISE_DEFAULT_KF_FIRST_CHUNKS 8u
