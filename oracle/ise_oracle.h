@@ -157,6 +157,10 @@ int orc_kf_bruteforce(const uint8_t *mat, int n, int nrounds, uint64_t *n_auts);
  * results fields as orc_kf_check; witness = index in its own order). */
 int orc_kf_check_2p1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
                       uint64_t *n_auts, uint64_t *n_tried);
+/* cnvCheck2U1F: the two-row canonizer of uniform 1-factorizations (a pair of
+ * rounds of another cycle type than rounds 0, 1: not canonical, witness ~0). */
+int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
+                      uint64_t *n_auts, uint64_t *n_tried);
 
 /* Fuchs' countdown QuickPerm order of the pair slots (k! rows of k bytes,
  * each byte the base label 2*slot); returns k!. */

@@ -257,5 +257,160 @@ int orc_kf_check_2p1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
     return verdict;
 }
 
+/*
+ * Two-row canonizer of uniform 1-factorizations: alldata::cnvCheck2U1F
+ * (k-Factorization/TripleSys/Source/cnvSupport2.cpp:3-142) with
+ * getCyclesAndPathFromNeighbors and sortCycles (p1fCheck.cpp:65-76, 153-228),
+ * CycleMapping::InitCycleMapping / ProceedToNextMapping (cycles.cpp:111-211),
+ * createU1FTr (TrCycles.cpp:3-50) and the full comparison kmProcessMatrix
+ * (kmSupport.cpp:275-313), for the U1F runs' defaults (complete graph,
+ * AutSaveTestedTrs 0, Any2RowsConvertToFirst2 1, early exit).
+ *
+ * Cycles of a pair of rounds (a, b): from the smallest player not yet met,
+ * alternately the a-partner and the b-partner until the start returns; the
+ * path lists each step as a pair (x, a(x)), (a(x), b(a(x))), ... (2L entries
+ * for a cycle of L players).  Cycles are bubble-sorted by length (stable).
+ * Pairs of rounds in order (last = 1.., ind < last, (ind, last) then
+ * (last, ind)); a pair whose sorted cycle lengths differ from those of rounds
+ * 0, 1 makes the matrix not canonical at once (no relabelling is counted for
+ * it).  Otherwise every mapping of its cycles onto the cycles of rounds 0, 1
+ * is tried: cycles of one length permuted among themselves (lexicographic
+ * next permutation, first group of equal lengths fastest), a start offset per
+ * cycle (L/2 choices, every other player; mixed radix, first cycle fastest),
+ * a direction per cycle (binary, last cycle fastest); directions change
+ * fastest, then offsets, then permutations.  Relabelling index = ordered pair
+ * index * T + position in that order, T = relabellings per pair.
+ */
+#define KF_MAXCYC 4
+
+typedef struct { int n, len[KF_MAXCYC], start[KF_MAXCYC]; u8 path[2 * KF_MAX_N]; } kf_cycles;
+
+static void kf_pair_cycles(const u8 *pa, const u8 *pb, int n, kf_cycles *c)
+{
+    int used[KF_MAX_N] = {0}, ip = 0;
+    c->n = 0;
+    for (int k0 = 0; k0 < n; k0++) {
+        if (used[k0]) continue;
+        c->start[c->n] = ip / 2;
+        int k = k0, len = 0;
+        for (;;) {
+            const int k1 = pa[k], k2 = pb[k1];
+            c->path[ip++] = (u8)k; c->path[ip++] = (u8)k1;
+            c->path[ip++] = (u8)k1; c->path[ip++] = (u8)k2;
+            used[k] = used[k1] = 1;
+            len += 2;
+            if (k2 == k0) break;
+            k = k2;
+        }
+        c->len[c->n++] = len;
+    }
+    for (int j = 1; j < c->n; j++)          /* sortCycles: bubble sort by length */
+        for (int i = 0; i < c->n - j; i++)
+            if (c->len[i] > c->len[i + 1]) {
+                int t = c->len[i]; c->len[i] = c->len[i + 1]; c->len[i + 1] = t;
+                t = c->start[i]; c->start[i] = c->start[i + 1]; c->start[i + 1] = t;
+            }
+}
+
+/* next lexicographic permutation of p[0..m); 0 when p was the last */
+static int kf_next_perm(int *p, int m)
+{
+    int i = m - 2;
+    while (i >= 0 && p[i] > p[i + 1]) i--;
+    if (i < 0) return 0;
+    int j = m - 1;
+    while (p[j] < p[i]) j--;
+    int t = p[i]; p[i] = p[j]; p[j] = t;
+    for (int a = i + 1, b = m - 1; a < b; a++, b--) { t = p[a]; p[a] = p[b]; p[b] = t; }
+    return 1;
+}
+
+int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
+                      uint64_t *n_auts, uint64_t *n_tried)
+{
+    if (n < 4 || n > KF_MAX_N || (n & 1) || nrounds < 1 || nrounds > n - 1) return -1;
+    u8 part[KF_MAX_N][KF_MAX_N];
+    for (int r = 0; r < nrounds; r++)
+        for (int q = 0; q < n; q += 2) {
+            const u8 x = mat[r * n + q], y = mat[r * n + q + 1];
+            part[r][x] = y;
+            part[r][y] = x;
+        }
+    uint64_t auts = 0, tried = 0, idx = 0;
+    int verdict = 1;
+    if (witness) *witness = ~0ull;
+    if (nrounds >= 2) {
+        kf_cycles ref;
+        kf_pair_cycles(part[0], part[1], n, &ref);
+        if (ref.n > KF_MAXCYC) return -1;
+        uint64_t T = 1;   /* relabellings per pair */
+        for (int c = 0; c < ref.n; c++) T *= (uint64_t)ref.len[c];
+        for (int c = 0, g = 0; c < ref.n; c = g) {
+            for (g = c; g < ref.n && ref.len[g] == ref.len[c]; g++) {}
+            for (int f = 2; f <= g - c; f++) T *= (uint64_t)f;
+        }
+        for (int last = 1; last < nrounds && verdict; last++)
+            for (int ind = 0; ind < last && verdict; ind++)
+                for (int swap = 0; swap < 2 && verdict; swap++) {
+                    const int r0 = swap ? last : ind, r1 = swap ? ind : last;
+                    kf_cycles cy;
+                    kf_pair_cycles(part[r0], part[r1], n, &cy);
+                    int same = cy.n == ref.n;
+                    for (int c = 0; c < ref.n && same; c++) same = cy.len[c] == ref.len[c];
+                    if (!same) { verdict = 0; break; }
+                    int dir[KF_MAXCYC] = {0}, off[KF_MAXCYC] = {0}, perm[KF_MAXCYC], sout[KF_MAXCYC];
+                    for (int c = 0; c < cy.n; c++) { perm[c] = c; sout[c] = cy.start[c]; }
+                    for (uint64_t t = 0; t < T; t++, idx++) {
+                        if (t) {   /* ProceedToNextMapping */
+                            int i = cy.n;
+                            while (i-- && dir[i]) dir[i] = 0;
+                            if (i >= 0) dir[i] = 1;
+                            else {
+                                int c = 0;
+                                while (c < cy.n && off[c] + 1 == cy.len[c] / 2) off[c++] = 0;
+                                if (c < cy.n) off[c]++;
+                                else {
+                                    for (int g0 = 0, g1; g0 < cy.n; g0 = g1) {
+                                        for (g1 = g0; g1 < cy.n && cy.len[g1] == cy.len[g0]; g1++) {}
+                                        if (kf_next_perm(perm + g0, g1 - g0)) {
+                                            for (int j = g0; j < g1; j++) sout[j] = cy.start[g0 + perm[j] - g0];
+                                            break;
+                                        }
+                                        for (int j = g0; j < g1; j++) perm[j] = j;
+                                    }
+                                }
+                            }
+                        }
+                        u8 tr[KF_MAX_N];   /* createU1FTr */
+                        for (int c = 0; c < cy.n; c++) {
+                            const int L2 = 2 * cy.len[c];
+                            int ip = dir[c] ? 4 * off[c] + 1 : 4 * off[c];
+                            const int step = dir[c] ? -1 : 1;
+                            const u8 *fp = cy.path + 2 * sout[c];
+                            const u8 *fp01 = ref.path + 2 * ref.start[c];
+                            for (int j = 0; j < L2; j++, ip += step) {
+                                if (ip >= L2) ip -= L2;
+                                else if (ip < 0) ip += L2;
+                                tr[fp[ip]] = fp01[j];
+                            }
+                        }
+                        u8 mo[KF_MAX_N * KF_MAX_N];
+                        normal_form(mat, nrounds, n, tr, mo);
+                        tried++;
+                        const int cmp = memcmp(mo, mat, (size_t)(n * nrounds));
+                        if (cmp == 0) auts++;
+                        else if (cmp < 0) {
+                            verdict = 0;
+                            if (witness) *witness = idx;
+                            break;
+                        }
+                    }
+                }
+    }
+    if (n_auts) *n_auts = auts;
+    if (n_tried) *n_tried = tried;
+    return verdict;
+}
+
 /* the QuickPerm visiting order (for tests) */
 long orc_kf_quickperm(int k, uint8_t *out) { return quickperm_table(k, out); }

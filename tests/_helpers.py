@@ -106,3 +106,63 @@ def p1f_candidates(block, count, seed, rmin=3):
         full = p1f_relabel(m, int(r0), int(r1), int(rng.integers(n)))
         out.append(full[:int(rng.integers(rmin, nr + 1))])
     return out
+
+
+def _cycles(pa, pb):
+    """Cycles of two rounds (partner arrays): from the smallest player not yet
+    met, alternately the pa- and pb-partner; players in order."""
+    n, seen, out = len(pa), set(), []
+    for k0 in range(n):
+        if k0 in seen:
+            continue
+        seq, x = [k0], k0
+        while True:
+            y = int(pa[x])
+            seq.append(y)
+            z = int(pb[y])
+            if z == k0:
+                break
+            seq.append(z)
+            x = z
+        seen.update(seq)
+        out.append(seq)
+    return sorted(out, key=len)
+
+
+def u1f_candidates(block, count, seed, rmin=3):
+    """Prefixes of the printed uniform 1-factorizations of one block
+    relabelled so that a random ordered pair of rounds goes onto rounds 0, 1
+    (its cycles onto theirs: equal lengths in a random assignment, random start
+    and direction), in normal form."""
+    rng = np.random.default_rng(seed)
+    mats = [np.array(m["rounds"], np.int64) for m in block["matrices"]]
+    out = []
+    for _ in range(count):
+        mat = mats[int(rng.integers(len(mats)))]
+        nr, n = mat.shape
+        part = np.zeros((nr, n), np.int64)
+        for r in range(nr):
+            part[r, mat[r, 0::2]] = mat[r, 1::2]
+            part[r, mat[r, 1::2]] = mat[r, 0::2]
+        r0, r1 = (int(x) for x in rng.choice(nr, 2, replace=False))
+        ref, cyc = _cycles(part[0], part[1]), _cycles(part[r0], part[r1])
+        assert [len(c) for c in ref] == [len(c) for c in cyc]
+        tr = np.full(n, -1, np.int64)
+        lens = sorted(set(len(c) for c in ref))
+        for L in lens:
+            P = [c for c in ref if len(c) == L]
+            Q = [cyc[i] for i in rng.permutation([i for i, c in enumerate(cyc) if len(c) == L])]
+            for p, q in zip(P, Q):
+                d = 2 * int(rng.integers(L // 2))
+                back = bool(rng.integers(2))
+                for m in range(L):
+                    tr[q[(d + 1 - m) % L] if back else q[(d + m) % L]] = p[m]
+        assert (np.sort(tr) == np.arange(n)).all()
+        rows = []
+        for r in range(nr):
+            pairs = np.sort(tr[mat[r].reshape(-1, 2)], axis=1)
+            rows.append(pairs[np.argsort(pairs[:, 0])].reshape(-1))
+        rows.sort(key=lambda row: row[1])
+        full = np.array(rows, np.uint8)
+        out.append(full[:int(rng.integers(rmin, nr + 1))])
+    return out

@@ -47,6 +47,7 @@ def load():
         L.orc_hunt.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, U32, _P, _P, _P]
         L.orc_kf_check.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.orc_kf_check_2p1f.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
+        L.orc_kf_check_2u1f.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.orc_kf_normalize.argtypes = [_P, ctypes.c_int, ctypes.c_int]
         L.orc_kf_bruteforce.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
         L.orc_kf_quickperm.argtypes = [ctypes.c_int, _P]
@@ -203,6 +204,16 @@ class Oracle:
         if v < 0:
             raise ValueError("oracle rejected the factorization (not a perfect 1-factorization)")
         return v, int(w[0]) if v == 0 else (1 << 64) - 1, int(a[0]), int(t[0])
+
+    def kf_check_2u1f(self, mat):
+        """Two-row U1F canonizer: (verdict, witness index, automorphisms, tried)."""
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        w, a, t = (np.zeros(1, np.uint64) for _ in range(3))
+        v = self.L.orc_kf_check_2u1f(mat.ctypes.data, mat.shape[1], mat.shape[0], w.ctypes.data, a.ctypes.data,
+                                     t.ctypes.data)
+        if v < 0:
+            raise ValueError("oracle rejected the factorization")
+        return v, int(w[0]), int(a[0]), int(t[0])
 
     def kf_bruteforce(self, mat):
         mat = np.ascontiguousarray(mat, dtype=np.uint8)

@@ -16,7 +16,9 @@ JSON fixtures next to this script:
                      the uniform K_12 factorization of
                      .../12/12x11x2_66/U_OrderedMatrices.txt, and the perfect
                      1-factorizations of K_10 and K_14 with |Aut(M)|,
-                     .../10/10x9x2/P_OrderedMatrices.txt, .../14/14x13x2/P_OrderedMatrices.txt.
+                     .../10/10x9x2/P_OrderedMatrices.txt, .../14/14x13x2/P_OrderedMatrices.txt,
+                     and the uniform ones of K_8 (cycles 4:4) and K_10 (4:6),
+                     .../8/8x7x2_44/U_OrderedMatrices.txt, .../10/10x9x2_46/U_OrderedMatrices.txt.
 
 Run: python tests/golden/make_fixtures.py  (needs the reference checkout).
 """
@@ -142,7 +144,8 @@ def kfact():
         mats = parse_kfact(read(p10), 10)
         res["k10"] = dict(n=10, source=os.path.relpath(p10, REF),
                           matrices=[dict(rounds=m, aut=a) for m, a in mats])
-    for key, n, rel in (("p10", 10, "10/10x9x2/P_OrderedMatrices.txt"), ("p14", 14, "14/14x13x2/P_OrderedMatrices.txt")):
+    for key, n, rel in (("p10", 10, "10/10x9x2/P_OrderedMatrices.txt"), ("p14", 14, "14/14x13x2/P_OrderedMatrices.txt"),
+                        ("u8", 8, "8/8x7x2_44/U_OrderedMatrices.txt"), ("u10", 10, "10/10x9x2_46/U_OrderedMatrices.txt")):
         path = os.path.join(REF, "k-Factorization/Results/Complete_graphs", rel)
         if os.path.exists(path):
             mats = parse_kfact(read(path), n)

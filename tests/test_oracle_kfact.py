@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import ise_amd
-from _helpers import golden, p1f_candidates, p1f_relabel
+from _helpers import golden, p1f_candidates, p1f_relabel, u1f_candidates
 
 
 def test_k10_printed_factorizations(oracle):
@@ -193,3 +193,49 @@ def test_p1f_relabel_generator(oracle):
         assert (oracle.kf_normalize(mat) == mat).all()
         canon += oracle.kf_check_2p1f(mat)[0]   # raises unless perfect
     assert 0 < canon < len(meta)
+
+
+# ---- two-row canonizer of uniform 1-factorizations (cnvCheck2U1F) ----------
+
+@pytest.mark.parametrize("key,cycles,tried", [("u8", (4, 4), 7 * 6 * 2 * 4 * 4), ("u10", (4, 6), 9 * 8 * 4 * 6),
+                                              ("u12", (6, 6), 11 * 10 * 2 * 6 * 6), ("p10", (10,), 9 * 8 * 10)])
+def test_u1f_printed_factorizations(oracle, key, cycles, tried):
+    """The printed uniform 1-factorizations of K_8 (cycles 4:4), K_10 (4:6) and
+    K_12 (6:6), and the perfect one of K_10 (one 10-cycle), are canonical for
+    the two-row canonizer with their printed |Aut(M)|, after every mapping of
+    every ordered pair of rounds onto rounds 0, 1 (per pair: the cycle
+    lengths' product times the permutations of equal lengths)."""
+    for m in golden("kfact.json")[key]["matrices"]:
+        v, w, auts, t = oracle.kf_check_2u1f(m["rounds"])
+        assert v == 1 and auts == m["aut"] and t == tried
+
+
+@pytest.mark.parametrize("key", ["u8", "u10"])
+def test_u1f_agrees_with_the_general_canonizer(oracle, key):
+    """On relabelled prefixes of the printed uniform factorizations the two-row
+    canonizer and the general one agree on the verdict and, when canonical, on
+    |Aut|; on perfect ones it gives the two-row P1F canonizer's |Aut|."""
+    cands = u1f_candidates(golden("kfact.json")[key], 300, 0x2A + len(key))
+    canon = 0
+    for mat in cands:
+        v, w, auts, t = oracle.kf_check_2u1f(mat)
+        gv, _, gauts, _ = oracle.kf_check(mat)
+        assert v == gv
+        if v:
+            canon += 1
+            assert auts == gauts
+        else:
+            assert w == t - 1
+    # every relabelling of the K_8 one that keeps its cycle type reproduces it
+    assert canon > 0 and (key == "u8" or canon < len(cands))
+    for mat in p1f_candidates(golden("kfact.json")["p10"], 100, 0x2B):
+        a, b = oracle.kf_check_2u1f(mat), oracle.kf_check_2p1f(mat)
+        assert a[0] == b[0] and (a[0] == 0 or a[2] == b[2])
+
+
+def test_u1f_other_cycle_type_is_not_canonical(oracle):
+    """A pair of rounds whose cycle lengths differ from those of rounds 0, 1
+    makes the matrix not canonical at once, with no witness relabelling."""
+    m = np.array(golden("kfact.json")["k10"]["matrices"][0]["rounds"], np.uint8)
+    v, w, auts, t = oracle.kf_check_2u1f(m)
+    assert v == 0 and w == (1 << 64) - 1
