@@ -75,6 +75,10 @@ hipError_t kf_launch(uint32_t blocks, uint32_t range_blocks, uint32_t ncand, uin
 // candidate, candidates by grid stride.
 hipError_t kf2_launch(uint32_t blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
                       const uint32_t *meta, const KfOut &o, hipStream_t stream);
+// Two-row canonizer of uniform 1-factorizations (cnvCheck2U1F): one wave per
+// candidate, candidates by grid stride.
+hipError_t kf_u1f_launch(uint32_t blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
+                         const uint32_t *meta, const KfOut &o, hipStream_t stream);
 hipError_t kf_occupancy(int *blocks_per_cu);
 
 } // namespace ise

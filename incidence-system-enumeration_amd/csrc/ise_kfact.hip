@@ -181,6 +181,8 @@ __device__ __forceinline__ KfUni kf_stage(KfLds &s, uint64_t row, uint32_t nr, u
     return u;
 }
 
+__device__ __forceinline__ int kf_compare_labels(const KfLds &s, const KfUni &u, uint32_t vlo, uint32_t vhi);
+
 // Label-order comparison of relabelling r (I = inverse slot arrangement).
 __device__ __forceinline__ int kf_compare_fast(const KfLds &s, const KfUni &u, uint64_t I, uint32_t r) {
     const uint32_t day = r % u.nr, q = r / u.nr;
@@ -197,6 +199,12 @@ __device__ __forceinline__ int kf_compare_fast(const KfLds &s, const KfUni &u, u
     const uint32_t wlo = ((plo >> 4) & 0x0f0f0f0fu) | ((plo << 4) & 0xf0f0f0f0u);
     const uint32_t whi = ((phi >> 4) & 0x0f0f0f0fu) | ((phi << 4) & 0xf0f0f0f0u);
     const uint32_t vlo = (wlo & mlo) | (plo & ~mlo), vhi = (whi & mhi) | (phi & ~mhi);   // label -> player
+    return kf_compare_labels(s, u, vlo, vhi);
+}
+
+// The label-order walk for a relabelling given as its inverse map label ->
+// player (vlo:vhi): -1 smaller, 0 equal, 1 larger.
+__device__ __forceinline__ int kf_compare_labels(const KfLds &s, const KfUni &u, uint32_t vlo, uint32_t vhi) {
     const uint64_t mt = s.meet[vlo & 0xfu];
     const uint32_t elo = (uint32_t)mt, ehi = (uint32_t)(mt >> 32);
     uint32_t i = 0;
@@ -707,6 +715,220 @@ __global__ __launch_bounds__(64) void kf2_check_kernel(uint32_t ncand, uint32_t 
         if (l == 0) kf_write(o, ci, witness, auts, total);
         __syncthreads();
     }
+}
+
+// ---- two-row canonizer of uniform 1-factorizations ------------------------
+//
+// alldata::cnvCheck2U1F (k-Factorization/TripleSys/Source/cnvSupport2.cpp:3-142)
+// with getCyclesAndPathFromNeighbors / sortCycles (p1fCheck.cpp:65-76,
+// 153-228), CycleMapping (cycles.cpp:111-211) and createU1FTr
+// (TrCycles.cpp:3-50).  The cycles of a pair of rounds (r0, r1) start at the
+// smallest player not yet met and go r0-partner first; they are stably sorted
+// by length.  A relabelling maps the cycles of (r0, r1) onto those of rounds
+// 0, 1: cycles of one length in some order (permutations of each group of
+// equal lengths, lexicographic rank, first group fastest), a start offset d
+// per cycle (L/2 choices: players 2d of the cycle), a direction per cycle;
+// cycle c of rounds 0, 1 with players P[0..L) receives Q[(2d + m) mod L] or,
+// reversed, Q[(2d + 1 - m) mod L] at label P[m].  Index within a pair:
+// (permutation rank * V + offsets) * 2^C + directions, with directions as a
+// binary number whose last cycle is the lowest bit and offsets as a mixed-radix
+// number whose first cycle is the lowest digit (V = product of L/2).  (The
+// reference's enumeration keeps the start order of a group that wraps while a
+// later group advances; with at most four cycles of at least four players no
+// two groups of n <= 16 both have two cycles or more, so ranks map one to one.)
+// A pair of another cycle type than rounds 0, 1 ends the test: not canonical,
+// no witness, the relabellings before it tried.  Rounds must be in normal form.
+
+struct KfU1f {
+    uint32_t ncyc, T, cut;            // cycles, relabellings per pair, relabellings before a mismatched pair
+    uint32_t start01, len01;          // rounds 0, 1: start players / half lengths, 4 bits per cycle
+};
+
+// Cycles of rounds a, b (partner tables alo:ahi, blo:bhi): starts (4 bits per
+// cycle, sorted order) and half lengths likewise; returns the cycle count.
+__device__ __forceinline__ uint32_t kf_cycles(uint32_t alo, uint32_t ahi, uint32_t blo, uint32_t bhi, uint32_t n,
+                                              uint32_t &starts, uint32_t &halves) {
+    uint32_t seen = 0, nc = 0;
+    starts = halves = 0;
+    for (uint32_t k0 = 0; k0 < n; k0++) {
+        if ((seen >> k0) & 1u) continue;
+        uint32_t x = k0, len = 0;
+        do {
+            const uint32_t y = tget(alo, ahi, x);
+            seen |= (1u << x) | (1u << y);
+            len += 2;
+            x = tget(blo, bhi, y);
+        } while (x != k0);
+        // insert after every cycle no longer than this one (stable sort by length)
+        uint32_t pos = nc;
+        while (pos > 0 && ((halves >> (4 * (pos - 1))) & 0xfu) > len / 2) pos--;
+        const uint32_t lowm = (1u << (4 * pos)) - 1u;
+        starts = (starts & lowm) | ((starts & ~lowm) << 4) | (k0 << (4 * pos));
+        halves = (halves & lowm) | ((halves & ~lowm) << 4) | ((len / 2) << (4 * pos));
+        nc++;
+    }
+    return nc;
+}
+
+__device__ __forceinline__ int kf_u1f_compare(const KfLds &s, const KfUni &u, const KfU1f &w, uint32_t r,
+                                              uint32_t z0lo, uint32_t z0hi, uint32_t z1lo, uint32_t z1hi) {
+    const uint32_t n = u.n, C = w.ncyc;
+    const uint32_t po = r / w.T, t = r - po * w.T;
+    const uint32_t pi = po >> 1;
+    uint32_t last = 1;
+    while ((last + 1) * last / 2 <= pi) last++;
+    const uint32_t ind = pi - last * (last - 1) / 2;
+    const uint32_t r0 = (po & 1) ? last : ind, r1 = (po & 1) ? ind : last;
+    const uint32_t alo = s.plo[r0], ahi = s.phi[r0], blo = s.plo[r1], bhi = s.phi[r1];
+    uint32_t starts, halves;
+    (void)kf_cycles(alo, ahi, blo, bhi, n, starts, halves);
+    // decode: directions (low bits), offsets, permutation ranks
+    const uint32_t dirs = t & ((1u << C) - 1u);
+    uint32_t rest = t >> C, V = 1;
+    for (uint32_t c = 0; c < C; c++) V *= (w.len01 >> (4 * c)) & 0xfu;
+    uint32_t var = rest % V, rank = rest / V;
+    uint32_t sout = starts;
+    for (uint32_t g0 = 0; g0 < C;) {
+        const uint32_t h = (w.len01 >> (4 * g0)) & 0xfu;
+        uint32_t g1 = g0 + 1;
+        while (g1 < C && ((w.len01 >> (4 * g1)) & 0xfu) == h) g1++;
+        const uint32_t m = g1 - g0;
+        uint32_t fact = 1;
+        for (uint32_t f = 2; f <= m; f++) fact *= f;
+        uint32_t rk = rank % fact;
+        rank /= fact;
+        // lexicographic unrank of rk over the group's cycles
+        uint32_t avail = (1u << m) - 1u;
+        for (uint32_t j = 0; j < m; j++) {
+            fact /= (m - j);
+            uint32_t pick = rk / fact;
+            rk %= fact;
+            uint32_t bits = avail;
+            while (pick--) bits &= bits - 1u;
+            const uint32_t e = (uint32_t)__builtin_ctz(bits);
+            avail &= ~(1u << e);
+            const uint32_t st = (starts >> (4 * (g0 + e))) & 0xfu;
+            sout = (sout & ~(0xfu << (4 * (g0 + j)))) | (st << (4 * (g0 + j)));
+        }
+        g0 = g1;
+    }
+    // label -> player
+    uint32_t vlo = 0, vhi = 0;
+    for (uint32_t c = 0; c < C; c++) {
+        const uint32_t h = (w.len01 >> (4 * c)) & 0xfu, L = 2 * h;
+        const uint32_t d = var % h;
+        var /= h;
+        const bool back = (dirs >> (C - 1 - c)) & 1u;
+        // Q from its start to index k0 = 2d (+1 reversed)
+        uint32_t q = (sout >> (4 * c)) & 0xfu, k = 0;
+        const uint32_t k0 = 2 * d + (back ? 1u : 0u);
+        for (; k < k0; k++) q = (k & 1u) ? tget(blo, bhi, q) : tget(alo, ahi, q);
+        uint32_t p = (w.start01 >> (4 * c)) & 0xfu;
+        for (uint32_t m = 0; m < L; m++) {
+            tset(vlo, vhi, p, q);
+            p = (m & 1u) ? tget(z1lo, z1hi, p) : tget(z0lo, z0hi, p);
+            if (back) {
+                // from Q[k] to Q[k-1]: the r0 edge when k is odd
+                q = (k & 1u) ? tget(alo, ahi, q) : tget(blo, bhi, q);
+                k = k ? k - 1 : L - 1;
+            } else {
+                q = (k & 1u) ? tget(blo, bhi, q) : tget(alo, ahi, q);
+                k = k + 1 == L ? 0 : k + 1;
+            }
+        }
+    }
+    return kf_compare_labels(s, u, vlo, vhi);
+}
+
+__global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t stride,
+                                                    const uint64_t *__restrict__ rounds,
+                                                    const uint32_t *__restrict__ meta, KfOut o) {
+    __shared__ KfLds s;
+    __shared__ uint32_t s_bad;
+    const uint32_t l = threadIdx.x;
+    const uint32_t rl = stride < KF_MAX_ROUNDS ? stride : KF_MAX_ROUNDS;
+    for (uint32_t ci = blockIdx.x; ci < ncand; ci += gridDim.x) {
+        const uint32_t m = meta[ci];
+        if (!kf_meta_ok(m, stride)) {
+            if (l == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
+            continue;
+        }
+        const uint32_t nr = m & 0xff, n = (m >> 8) & 0xff;
+        const KfUni u = kf_stage(s, l < rl ? rounds[(uint64_t)ci * stride + l] : 0ull, nr, n, l);
+        if (!u.fast) {
+            if (l == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
+            __syncthreads();
+            continue;
+        }
+        KfU1f w{0u, 1u, 0u, 0u, 0u};
+        uint32_t total = 0;
+        const uint32_t z0lo = kf_uni(s.plo[0]), z0hi = kf_uni(s.phi[0]);
+        const uint32_t z1lo = nr > 1 ? kf_uni(s.plo[1]) : 0u, z1hi = nr > 1 ? kf_uni(s.phi[1]) : 0u;
+        if (nr > 1) {
+            uint32_t st01, h01;
+            w.ncyc = kf_uni(kf_cycles(z0lo, z0hi, z1lo, z1hi, n, st01, h01));
+            w.start01 = kf_uni(st01);
+            w.len01 = kf_uni(h01);
+            uint32_t T = 1;
+            for (uint32_t c = 0, g = 0; c < w.ncyc; c = g) {
+                const uint32_t h = (w.len01 >> (4 * c)) & 0xfu;
+                for (g = c; g < w.ncyc && ((w.len01 >> (4 * g)) & 0xfu) == h; g++) T *= 2 * h;
+                for (uint32_t f = 2; f <= g - c; f++) T *= f;
+            }
+            w.T = T;
+            // the first unordered pair (in order) of another cycle type
+            if (l == 0) s_bad = 0xffffffffu;
+            __syncthreads();
+            const uint32_t npairs = nr * (nr - 1) / 2;
+            for (uint32_t b0 = 0; b0 < npairs; b0 += 64) {
+                const uint32_t pi = b0 + l;
+                if (pi < npairs) {
+                    uint32_t b = 1;
+                    while ((b + 1) * b / 2 <= pi) b++;
+                    const uint32_t a = pi - b * (b - 1) / 2;
+                    uint32_t st, h;
+                    const uint32_t nc = kf_cycles(s.plo[a], s.phi[a], s.plo[b], s.phi[b], n, st, h);
+                    if (nc != w.ncyc || h != w.len01) atomicMin(&s_bad, pi);
+                }
+            }
+            __syncthreads();
+            const uint32_t bad = kf_uni(s_bad);
+            total = nr * (nr - 1) * T;
+            w.cut = bad == 0xffffffffu ? total : 2u * bad * T;
+        }
+        uint64_t auts = 0;
+        uint32_t witness = KF_NONE;
+        for (uint32_t base = 0; base < w.cut; base += 64) {
+            const uint32_t r = base + l;
+            const int icmp = r < w.cut ? kf_u1f_compare(s, u, w, r, z0lo, z0hi, z1lo, z1hi) : 1;
+            const uint64_t smaller = __builtin_amdgcn_ballot_w64(icmp < 0);
+            const uint64_t equal = __builtin_amdgcn_ballot_w64(r < w.cut && icmp == 0);
+            if (smaller) {
+                const uint32_t first = __builtin_ctzll(smaller);
+                auts += __popcll(equal & ((1ull << first) - 1ull));
+                witness = base + first;
+                break;
+            }
+            auts += __popcll(equal);
+        }
+        if (l == 0) {
+            if (witness == KF_NONE && w.cut < total) {   // a pair of another cycle type
+                o.verdict[ci] = ISE_VERDICT_NONCANONICAL;
+                if (o.witness) o.witness[ci] = ~0ull;
+                if (o.n_auts) o.n_auts[ci] = auts;
+                if (o.n_tried) o.n_tried[ci] = w.cut;
+            } else {
+                kf_write(o, ci, witness, auts, total);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t kf_u1f_launch(uint32_t blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
+                         const uint32_t *meta, const KfOut &o, hipStream_t stream) {
+    hipLaunchKernelGGL(kf_u1f_kernel, dim3(blocks), dim3(64), 0, stream, ncand, stride, rounds, meta, o);
+    return hipGetLastError();
 }
 
 hipError_t kf2_launch(uint32_t blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,

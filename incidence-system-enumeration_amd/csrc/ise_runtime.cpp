@@ -653,8 +653,11 @@ int ise_kf_check_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride, con
     return rc;
 }
 
+// The two-row canonizers: perfect (p1f) or uniform 1-factorizations.
+enum KfMode { KF_GENERAL = 0, KF_2P1F = 1, KF_2U1F = 2 };
+
 static int kf2_launch_on(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
-                         const uint32_t *d_meta, const ise_kf_results &d_out, hipStream_t s) {
+                         const uint32_t *d_meta, const ise_kf_results &d_out, hipStream_t s, int mode) {
     uint32_t blocks = c->cfg.blocks;
     if (!blocks) {
         int per_cu = 0;
@@ -662,7 +665,10 @@ static int kf2_launch_on(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t
         blocks = (uint32_t)(per_cu < 1 ? 1 : per_cu) * (uint32_t)c->num_cus;
         if (blocks > n) blocks = n ? n : 1;
     }
-    HIPCHK(c, ise::kf2_launch(blocks, n, stride, d_rounds, d_meta, kf_out(d_out), s));
+    if (mode == KF_2U1F)
+        HIPCHK(c, ise::kf_u1f_launch(blocks, n, stride, d_rounds, d_meta, kf_out(d_out), s));
+    else
+        HIPCHK(c, ise::kf2_launch(blocks, n, stride, d_rounds, d_meta, kf_out(d_out), s));
     return ISE_OK;
 }
 
@@ -672,7 +678,16 @@ int ise_kf_check_2p1f_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, cons
     if (n == 0) return ISE_OK;
     if (!d_rounds || !d_meta) return ISE_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
-    return kf2_launch_on(c, n, stride, d_rounds, d_meta, *d_out, stream ? (hipStream_t)stream : c->stream);
+    return kf2_launch_on(c, n, stride, d_rounds, d_meta, *d_out, stream ? (hipStream_t)stream : c->stream, KF_2P1F);
+}
+
+int ise_kf_check_2u1f_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                                   const uint32_t *d_meta, const ise_kf_results *d_out, void *stream) {
+    if (!c || !d_out || !d_out->verdict || stride < 1 || stride > ISE_MAX_ROWS) return ISE_ERR_ARG;
+    if (n == 0) return ISE_OK;
+    if (!d_rounds || !d_meta) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    return kf2_launch_on(c, n, stride, d_rounds, d_meta, *d_out, stream ? (hipStream_t)stream : c->stream, KF_2U1F);
 }
 
 int ise_kf_check_2p1f_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
@@ -702,7 +717,8 @@ int ise_kf_check_2p1f_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride
 // stated size; for the two-row canonizer also rounds in normal form (pairs
 // (a, b) with a < b, pairs by first player) and every pair of rounds one
 // Hamiltonian cycle.
-static bool kf_records_ok(uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta, bool p1f) {
+static bool kf_records_ok(uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta, int mode) {
+    const bool p1f = mode == KF_2P1F, normal = mode != KF_GENERAL;
     for (uint32_t i = 0; i < n; i++) {
         const uint32_t nr = meta[i] & 0xff, np = (meta[i] >> 8) & 0xff;
         if (np < 4 || np > ise::KF_MAX_PLAYERS || (np & 1) || nr < 1 || nr > np - 1 || nr > stride)
@@ -716,7 +732,7 @@ static bool kf_records_ok(uint32_t n, uint32_t stride, const uint64_t *rounds, c
                 const uint32_t x = (uint32_t)(w >> (4 * p)) & 0xf, y = (uint32_t)(w >> (4 * p + 4)) & 0xf;
                 if (x >= np || y >= np || x == y || (seen >> x & 1) || (seen >> y & 1) || (used[x] >> y & 1))
                     return false;
-                if (p1f && (x > y || (p && x < prev))) return false;
+                if (normal && (x > y || (p && x < prev))) return false;
                 prev = x;
                 seen |= (1u << x) | (1u << y);
                 used[x] |= 1u << y;
@@ -740,18 +756,18 @@ static bool kf_records_ok(uint32_t n, uint32_t stride, const uint64_t *rounds, c
 }
 
 static int kf_host_run(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
-                       ise_kf_results *h, bool p1f) {
+                       ise_kf_results *h, int mode) {
     if (!c || !h || !h->verdict || (n && (!rounds || !meta)) || stride < 1 || stride > c->stride)
         return ISE_ERR_ARG;
     if (n > c->capacity) return ISE_ERR_CAPACITY;
     if (n == 0) return ISE_OK;
-    if (!kf_records_ok(n, stride, rounds, meta, p1f)) return ISE_ERR_ARG;
+    if (!kf_records_ok(n, stride, rounds, meta, mode)) return ISE_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpyAsync(c->d_rows, rounds, (size_t)n * stride * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_meta, meta, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
     ise_kf_results d{c->d.verdict, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried};
-    const int rc = p1f ? kf2_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->stream)
-                       : kf_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->d_head, c->stream);
+    const int rc = mode != KF_GENERAL ? kf2_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->stream, mode)
+                                      : kf_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->d_head, c->stream);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(h->verdict, d.verdict, n, hipMemcpyDeviceToHost, c->stream));
     if (h->witness) HIPCHK(c, hipMemcpyAsync(h->witness, d.witness, n * 8, hipMemcpyDeviceToHost, c->stream));
@@ -763,12 +779,17 @@ static int kf_host_run(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *
 
 int ise_kf_check(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
                  ise_kf_results *h) {
-    return kf_host_run(c, n, stride, rounds, meta, h, false);
+    return kf_host_run(c, n, stride, rounds, meta, h, KF_GENERAL);
 }
 
 int ise_kf_check_2p1f(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
                       ise_kf_results *h) {
-    return kf_host_run(c, n, stride, rounds, meta, h, true);
+    return kf_host_run(c, n, stride, rounds, meta, h, KF_2P1F);
+}
+
+int ise_kf_check_2u1f(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
+                      ise_kf_results *h) {
+    return kf_host_run(c, n, stride, rounds, meta, h, KF_2U1F);
 }
 
 int ise_set_order(ise_ctx *c, int mode, uint32_t hunt_budget, const uint8_t *d_order, uint32_t order_dim) {

@@ -132,6 +132,8 @@ def lib():
         L.ise_kf_random.argtypes = [I, I, I, U64, U64, U32, P, P]
         L.ise_kf_check_2p1f.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
         L.ise_kf_relabel_p1f.argtypes = [P, P, U32, U32, U64, I, U64, U32, P, P]
+        L.ise_kf_check_2u1f.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
+        L.ise_kf_check_2u1f_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P]
         L.ise_kf_check_2p1f_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P]
         L.ise_kf_check_2p1f_batch_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct),
                                                            U32, P]
@@ -356,10 +358,12 @@ class Context:
         (first_chunks=0: one wave per candidate to its decision)."""
         self._check(self._L.ise_kf_set_split(self.h, first_chunks, max_ranges))
 
-    def kf_check(self, rounds, meta, two_row_p1f=False):
+    def kf_check(self, rounds, meta, two_row_p1f=False, two_row_u1f=False):
         """Canonicity of partial 1-factorizations (host arrays in and out);
-        two_row_p1f: the two-row canonizer of perfect 1-factorizations."""
-        fn = self._L.ise_kf_check_2p1f if two_row_p1f else self._L.ise_kf_check
+        two_row_p1f / two_row_u1f: the two-row canonizers of perfect / uniform
+        1-factorizations."""
+        fn = (self._L.ise_kf_check_2p1f if two_row_p1f else
+              self._L.ise_kf_check_2u1f if two_row_u1f else self._L.ise_kf_check)
         rounds = np.ascontiguousarray(rounds, dtype=np.uint64)
         meta = np.ascontiguousarray(meta, dtype=np.uint32)
         n, stride = rounds.shape
@@ -389,6 +393,10 @@ class Context:
 
     def kf_check_2p1f_device(self, n, stride, d_rounds, d_meta, res_struct):
         self._check(self._L.ise_kf_check_2p1f_batch_device(self.h, n, stride, d_rounds.ptr, d_meta.ptr,
+                                                           ctypes.byref(res_struct), None))
+
+    def kf_check_2u1f_device(self, n, stride, d_rounds, d_meta, res_struct):
+        self._check(self._L.ise_kf_check_2u1f_batch_device(self.h, n, stride, d_rounds.ptr, d_meta.ptr,
                                                            ctypes.byref(res_struct), None))
 
     def kf_time_2p1f_device(self, n, stride, d_rounds, d_meta, res_struct, repeats):

@@ -388,6 +388,27 @@ int ise_kf_check_2p1f_batch_device_timed(ise_ctx *ctx, uint32_t n, uint32_t stri
                                          const uint64_t *d_rounds, const uint32_t *d_meta,
                                          const ise_kf_results *d_out, uint32_t repeats, float *ms_out);
 
+/* Two-row canonizer of uniform 1-factorizations: alldata::cnvCheck2U1F
+ * (k-Factorization/TripleSys/Source/cnvSupport2.cpp:3-142) with the cycle
+ * tables of getCyclesAndPathFromNeighbors (p1fCheck.cpp:153-228), the mapping
+ * order of CycleMapping (cycles.cpp:111-211) and createU1FTr
+ * (TrCycles.cpp:3-50); selected for U1F runs at allData.cpp:123-145.  Every
+ * ordered pair of rounds whose cycles have the lengths of rounds 0, 1's is
+ * sent onto rounds 0, 1 by every mapping of its cycles onto theirs (cycles of
+ * one length permuted, a start offset and a direction per cycle); the
+ * relabelled matrix is normalised and compared in full.  Index: ordered pair *
+ * T + ((permutation rank * V + offsets) * 2^cycles + directions), T the
+ * relabellings per pair, V the product of half cycle lengths (directions: last
+ * cycle lowest bit; offsets: first cycle lowest digit; ranks: lexicographic,
+ * first group of equal lengths lowest).  A pair of rounds with other cycle
+ * lengths makes the matrix not canonical with witness ~0 and n_tried the
+ * relabellings before it.  Rounds must be in normal form (ISE_ERR_ARG on host
+ * arrays, ISE_VERDICT_INVALID on the device).  Device path: no allocation. */
+int ise_kf_check_2u1f(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *rounds,
+                      const uint32_t *meta, ise_kf_results *out_host);
+int ise_kf_check_2u1f_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                                   const uint32_t *d_meta, const ise_kf_results *d_out, void *stream);
+
 /* Seeded relabelled prefixes of perfect 1-factorizations (a candidate stream
  * for the two-row canonizer): for candidate i a source matrix, an ordered pair
  * of its rounds and a start player are drawn, the pair is sent onto the
