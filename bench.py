@@ -174,31 +174,36 @@ def kfact_rate(ctx, steps, warmup):
             "one_pass_ms_per_step": timed["one_pass"]}
 
 
-def kfact_p1f_rate(ctx, steps, warmup):
-    """Perfect 1-factorizations of K_14, two-row canonizer (cnvCheck2P1F):
-    2^18 seeded relabelled prefixes (2-13 rounds) of the 23 perfect
-    1-factorizations of K_14 the reference prints (tests/golden/kfact.json),
-    one launch per step."""
+def kfact_p1f_rate(ctx, steps, warmup, uniform=False):
+    """Two-row canonizers, one launch per step: perfect 1-factorizations of
+    K_14 (cnvCheck2P1F): 2^18 seeded relabelled prefixes (2-13 rounds) of the
+    23 perfect 1-factorizations of K_14 the reference prints; uniform=True:
+    the uniform one of K_12 (cnvCheck2U1F, cycles 6:6), 2^18 relabelled
+    prefixes (2-11 rounds).  Sources from tests/golden/kfact.json."""
     import ise_amd
     with open(os.path.join(ROOT, "tests", "golden", "kfact.json")) as f:
-        block = json.load(f)["p14"]
+        block = json.load(f)["u12" if uniform else "p14"]
     src_rows, src_meta = ise_amd.kf_pack([m["rounds"] for m in block["matrices"]])
-    rows, meta = ise_amd.kf_relabel_p1f(src_rows, src_meta, 1 << 18, 2, 0x14F1, stride=16)
+    rows, meta = ise_amd.kf_relabel_p1f(src_rows, src_meta, 1 << 18, 2, 0x12F1 if uniform else 0x14F1, stride=16,
+                                        uniform=uniform)
     n = len(meta)
     d_rows = ctx.device_array(rows.shape, np.uint64, rows)
     d_meta = ctx.device_array(meta.shape, np.uint32, meta)
     dev, res = ctx.kf_device_results(n)
     if warmup:
-        ctx.kf_time_2p1f_device(n, 16, d_rows, d_meta, res, warmup)
-    ms = float(np.mean(ctx.kf_time_2p1f_device(n, 16, d_rows, d_meta, res, steps)))
+        ctx.kf_time_2p1f_device(n, 16, d_rows, d_meta, res, warmup, uniform=uniform)
+    ms = float(np.mean(ctx.kf_time_2p1f_device(n, 16, d_rows, d_meta, res, steps, uniform=uniform)))
     verdict = dev["verdict"].to_host()
     tried = dev["n_tried"].to_host()
     for b in list(dev.values()) + [d_rows, d_meta]:
         b.free()
     if (verdict > 1).any():
-        raise RuntimeError("invalid record in the perfect 1-factorization batch")
-    return {"workload": "K_14 perfect 1-factorizations, two-row canonizer: 2^18 relabelled prefixes "
-                        "(2-13 rounds, seed 0x14F1) of the 23 printed ones",
+        raise RuntimeError("invalid record in the two-row canonizer batch")
+    workload = ("K_12 uniform 1-factorization (cycles 6:6), two-row canonizer: 2^18 relabelled prefixes "
+                "(2-11 rounds, seed 0x12F1) of the printed one" if uniform else
+                "K_14 perfect 1-factorizations, two-row canonizer: 2^18 relabelled prefixes "
+                "(2-13 rounds, seed 0x14F1) of the 23 printed ones")
+    return {"workload": workload,
             "candidates": n, "ms_per_step": ms, "candidates_per_s": n / (ms / 1e3),
             "canonical": int((verdict == 1).sum()), "relabellings_per_s": float(tried.sum()) / (ms / 1e3)}
 
@@ -302,7 +307,9 @@ def main():
         import ise_amd
         with ise_amd.Context(capacity=1 << 18, stride=16) as kctx:
             print(json.dumps({"kfact_k12": kfact_rate(kctx, args.steps, args.warmup),
-                              "kfact_p1f_k14": kfact_p1f_rate(kctx, args.steps, args.warmup)}), flush=True)
+                              "kfact_p1f_k14": kfact_p1f_rate(kctx, args.steps, args.warmup),
+                              "kfact_u1f_k12": kfact_p1f_rate(kctx, args.steps, args.warmup, uniform=True)}),
+                  flush=True)
         return
 
     t_setup = time.time()
@@ -434,6 +441,7 @@ def main():
     if not args.no_extra:
         line["kfact_k12"] = kfact_rate(ctx, max(3, args.steps // 4), 1)
         line["kfact_p1f_k14"] = kfact_p1f_rate(ctx, max(3, args.steps // 4), 1)
+        line["kfact_u1f_k12"] = kfact_p1f_rate(ctx, max(3, args.steps // 4), 1, uniform=True)
         line["streams"] = stream_rates(ctx, max(3, args.steps // 4), 1)
         line["enumeration"] = enumeration_rate()
         line["host_buffers"] = host_buffer_rate(rows, meta)

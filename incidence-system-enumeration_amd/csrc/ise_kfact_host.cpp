@@ -248,3 +248,103 @@ extern "C" int ise_kf_relabel_p1f(const uint64_t *src, const uint32_t *src_meta,
     }
     return ISE_OK;
 }
+
+// Seeded relabelled prefixes of uniform 1-factorizations: as
+// ise_kf_relabel_p1f, with the drawn pair of rounds sent onto the source's
+// rounds 0, 1 cycle by cycle (cycles of one length in a random assignment, a
+// random start of the right parity and a random direction per cycle).  Every
+// pair of rounds of a source must have the cycle lengths of its rounds 0, 1.
+extern "C" int ise_kf_relabel_u1f(const uint64_t *src, const uint32_t *src_meta, uint32_t nsrc, uint32_t src_stride,
+                                  uint64_t count, int rmin, uint64_t seed, uint32_t stride, uint64_t *rounds_out,
+                                  uint32_t *meta_out) {
+    using namespace ise;
+    if (!src || !src_meta || nsrc == 0 || src_stride < 1 || rmin < 1 || !rounds_out || !meta_out) return ISE_ERR_ARG;
+    for (uint32_t s = 0; s < nsrc; s++) {
+        const int nr = (int)(src_meta[s] & 0xff), n = (int)((src_meta[s] >> 8) & 0xff);
+        if (n < 4 || n > (int)KF_MAX_PLAYERS || (n & 1) || nr < 2 || nr > n - 1 || nr < rmin ||
+            (uint32_t)nr > src_stride || (uint32_t)nr > stride)
+            return ISE_ERR_ARG;
+    }
+    // cycles of rounds a, b: players in order from the smallest one not yet met, a-partner first
+    struct Cyc { int len; uint8_t p[KF_MAX_PLAYERS]; };
+    auto cycles = [](const uint8_t *pa, const uint8_t *pb, int n, Cyc *out) {
+        uint32_t seen = 0;
+        int nc = 0;
+        for (int k0 = 0; k0 < n; k0++) {
+            if ((seen >> k0) & 1u) continue;
+            Cyc c{0, {0}};
+            int x = k0;
+            do {
+                const int y = pa[x];
+                c.p[c.len++] = (uint8_t)x;
+                c.p[c.len++] = (uint8_t)y;
+                seen |= (1u << x) | (1u << y);
+                x = pb[y];
+            } while (x != k0);
+            int pos = nc;   // stable by length
+            while (pos > 0 && out[pos - 1].len > c.len) { out[pos] = out[pos - 1]; pos--; }
+            out[pos] = c;
+            nc++;
+        }
+        return nc;
+    };
+    for (uint64_t i = 0; i < count; i++) {
+        uint64_t rng = seed ^ (0xc2b2ae3d27d4eb4full * (i + 1));
+        mix(rng);
+        const uint32_t s = (uint32_t)(mix(rng) % nsrc);
+        const int nr = (int)(src_meta[s] & 0xff), n = (int)((src_meta[s] >> 8) & 0xff);
+        uint8_t part[KF_MAX_ROUNDS][KF_MAX_PLAYERS];
+        for (int r = 0; r < nr; r++) {
+            const uint64_t w = src[(uint64_t)s * src_stride + r];
+            uint32_t cover = 0;
+            for (int q = 0; q < n; q += 2) {
+                const uint8_t x = (uint8_t)((w >> (4 * q)) & 0xf), y = (uint8_t)((w >> (4 * q + 4)) & 0xf);
+                if (x >= n || y >= n) return ISE_ERR_ARG;
+                part[r][x] = y;
+                part[r][y] = x;
+                cover |= (1u << x) | (1u << y);
+            }
+            if (cover != (1u << n) - 1) return ISE_ERR_ARG;
+        }
+        const int r0 = (int)(mix(rng) % (uint64_t)nr);
+        int r1 = (int)(mix(rng) % (uint64_t)(nr - 1));
+        if (r1 >= r0) r1++;
+        Cyc ref[KF_MAX_PLAYERS / 4 + 1], cyc[KF_MAX_PLAYERS / 4 + 1];
+        const int nc = cycles(part[0], part[1], n, ref);
+        if (cycles(part[r0], part[r1], n, cyc) != nc) return ISE_ERR_ARG;
+        for (int c = 0; c < nc; c++)
+            if (cyc[c].len != ref[c].len) return ISE_ERR_ARG;
+        // shuffle the cycles inside each group of equal length
+        for (int g0 = 0, g1; g0 < nc; g0 = g1) {
+            for (g1 = g0; g1 < nc && cyc[g1].len == cyc[g0].len; g1++) {}
+            for (int j = g1 - 1; j > g0; j--) std::swap(cyc[j], cyc[g0 + (int)(mix(rng) % (uint64_t)(j - g0 + 1))]);
+        }
+        uint8_t tr[KF_MAX_PLAYERS];
+        for (int c = 0; c < nc; c++) {
+            const int L = ref[c].len;
+            const int d = 2 * (int)(mix(rng) % (uint64_t)(L / 2));
+            const bool back = mix(rng) & 1u;
+            for (int m = 0; m < L; m++) tr[cyc[c].p[back ? ((d + 1 - m) % L + L) % L : (d + m) % L]] = ref[c].p[m];
+        }
+        struct Row { uint8_t v[KF_MAX_PLAYERS]; };
+        Row rows[KF_MAX_ROUNDS];
+        for (int r = 0; r < nr; r++) {
+            uint8_t q[KF_MAX_PLAYERS];
+            for (int x = 0; x < n; x++) q[tr[x]] = tr[part[r][x]];
+            int k = 0;
+            for (int v = 0; v < n; v++)
+                if (q[v] > v) { rows[r].v[k++] = (uint8_t)v; rows[r].v[k++] = q[v]; }
+        }
+        std::sort(rows, rows + nr, [](const Row &x, const Row &y) { return x.v[1] < y.v[1]; });
+        const int keep = rmin + (int)(mix(rng) % (uint64_t)(nr - rmin + 1));
+        uint64_t *dst = rounds_out + i * stride;
+        memset(dst, 0, stride * sizeof(uint64_t));
+        for (int r = 0; r < keep; r++) {
+            uint64_t w = 0;
+            for (int q = 0; q < n; q++) w |= (uint64_t)rows[r].v[q] << (4 * q);
+            dst[r] = w;
+        }
+        meta_out[i] = (uint32_t)keep | ((uint32_t)n << 8);
+    }
+    return ISE_OK;
+}

@@ -690,9 +690,8 @@ int ise_kf_check_2u1f_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, cons
     return kf2_launch_on(c, n, stride, d_rounds, d_meta, *d_out, stream ? (hipStream_t)stream : c->stream, KF_2U1F);
 }
 
-int ise_kf_check_2p1f_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
-                                         const uint32_t *d_meta, const ise_kf_results *d_out,
-                                         uint32_t repeats, float *ms_out) {
+// `repeats` launches of `launch_one` on the context stream, each timed with HIP events.
+extern "C++" template <typename F> static int kf_timed(ise_ctx *c, uint32_t repeats, float *ms_out, F launch_one) {
     if (!c || !ms_out || repeats == 0) return ISE_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     std::vector<hipEvent_t> ev(repeats + 1, nullptr);
@@ -701,7 +700,7 @@ int ise_kf_check_2p1f_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride
         if (hipEventCreate(&e) != hipSuccess) rc = ISE_ERR_HIP;
     if (rc == ISE_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) rc = ISE_ERR_HIP;
     for (uint32_t i = 0; i < repeats && rc == ISE_OK; i++) {
-        rc = ise_kf_check_2p1f_batch_device(c, n, stride, d_rounds, d_meta, d_out, nullptr);
+        rc = launch_one();
         if (rc == ISE_OK && hipEventRecord(ev[i + 1], c->stream) != hipSuccess) rc = ISE_ERR_HIP;
     }
     if (rc == ISE_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = ISE_ERR_HIP;
@@ -711,6 +710,22 @@ int ise_kf_check_2p1f_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride
         if (e) (void)hipEventDestroy(e);
     if (rc == ISE_ERR_HIP && c->err.empty()) c->err = "timed launch failed";
     return rc;
+}
+
+int ise_kf_check_2p1f_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                                         const uint32_t *d_meta, const ise_kf_results *d_out,
+                                         uint32_t repeats, float *ms_out) {
+    return kf_timed(c, repeats, ms_out, [&] {
+        return ise_kf_check_2p1f_batch_device(c, n, stride, d_rounds, d_meta, d_out, nullptr);
+    });
+}
+
+int ise_kf_check_2u1f_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                                         const uint32_t *d_meta, const ise_kf_results *d_out,
+                                         uint32_t repeats, float *ms_out) {
+    return kf_timed(c, repeats, ms_out, [&] {
+        return ise_kf_check_2u1f_batch_device(c, n, stride, d_rounds, d_meta, d_out, nullptr);
+    });
 }
 
 // Host records of 1-factorizations: edge-disjoint perfect matchings of the

@@ -132,8 +132,11 @@ def lib():
         L.ise_kf_random.argtypes = [I, I, I, U64, U64, U32, P, P]
         L.ise_kf_check_2p1f.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
         L.ise_kf_relabel_p1f.argtypes = [P, P, U32, U32, U64, I, U64, U32, P, P]
+        L.ise_kf_relabel_u1f.argtypes = [P, P, U32, U32, U64, I, U64, U32, P, P]
         L.ise_kf_check_2u1f.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
         L.ise_kf_check_2u1f_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P]
+        L.ise_kf_check_2u1f_batch_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct),
+                                                           U32, P]
         L.ise_kf_check_2p1f_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P]
         L.ise_kf_check_2p1f_batch_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct),
                                                            U32, P]
@@ -399,11 +402,13 @@ class Context:
         self._check(self._L.ise_kf_check_2u1f_batch_device(self.h, n, stride, d_rounds.ptr, d_meta.ptr,
                                                            ctypes.byref(res_struct), None))
 
-    def kf_time_2p1f_device(self, n, stride, d_rounds, d_meta, res_struct, repeats):
+    def kf_time_2p1f_device(self, n, stride, d_rounds, d_meta, res_struct, repeats, uniform=False):
+        """Timed launches of the two-row canonizer (uniform=True: the U1F one)."""
         ms = np.zeros(repeats, np.float32)
-        self._check(self._L.ise_kf_check_2p1f_batch_device_timed(self.h, n, stride, d_rounds.ptr, d_meta.ptr,
-                                                                 ctypes.byref(res_struct), repeats,
-                                                                 ms.ctypes.data))
+        fn = (self._L.ise_kf_check_2u1f_batch_device_timed if uniform
+              else self._L.ise_kf_check_2p1f_batch_device_timed)
+        self._check(fn(self.h, n, stride, d_rounds.ptr, d_meta.ptr, ctypes.byref(res_struct), repeats,
+                       ms.ctypes.data))
         return ms
 
     def last_kernel_ms(self):
@@ -697,16 +702,18 @@ def kf_random(players, rmin, rmax, count, seed, stride=16):
     return rows, meta
 
 
-def kf_relabel_p1f(src_rows, src_meta, count, rmin, seed, stride=16):
-    """Seeded relabelled prefixes of perfect 1-factorizations (host code)."""
+def kf_relabel_p1f(src_rows, src_meta, count, rmin, seed, stride=16, uniform=False):
+    """Seeded relabelled prefixes of perfect (uniform=True: uniform)
+    1-factorizations (host code)."""
     src_rows = np.ascontiguousarray(src_rows, dtype=np.uint64)
     src_meta = np.ascontiguousarray(src_meta, dtype=np.uint32)
     rows = np.zeros((count, stride), np.uint64)
     meta = np.zeros(count, np.uint32)
-    rc = lib().ise_kf_relabel_p1f(src_rows.ctypes.data, src_meta.ctypes.data, len(src_meta), src_rows.shape[1],
-                                  count, rmin, seed, stride, rows.ctypes.data, meta.ctypes.data)
+    fn = lib().ise_kf_relabel_u1f if uniform else lib().ise_kf_relabel_p1f
+    rc = fn(src_rows.ctypes.data, src_meta.ctypes.data, len(src_meta), src_rows.shape[1],
+            count, rmin, seed, stride, rows.ctypes.data, meta.ctypes.data)
     if rc != OK:
-        raise ISEError(rc, "ise_kf_relabel_p1f")
+        raise ISEError(rc, "ise_kf_relabel_u1f" if uniform else "ise_kf_relabel_p1f")
     return rows, meta
 
 

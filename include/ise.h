@@ -408,6 +408,9 @@ int ise_kf_check_2u1f(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t 
                       const uint32_t *meta, ise_kf_results *out_host);
 int ise_kf_check_2u1f_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
                                    const uint32_t *d_meta, const ise_kf_results *d_out, void *stream);
+int ise_kf_check_2u1f_batch_device_timed(ise_ctx *ctx, uint32_t n, uint32_t stride,
+                                         const uint64_t *d_rounds, const uint32_t *d_meta,
+                                         const ise_kf_results *d_out, uint32_t repeats, float *ms_out);
 
 /* Seeded relabelled prefixes of perfect 1-factorizations (a candidate stream
  * for the two-row canonizer): for candidate i a source matrix, an ordered pair
@@ -417,6 +420,14 @@ int ise_kf_check_2u1f_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride, co
  * the source]).  Sources: nsrc records of src_stride words with src_meta as
  * for ise_kf_check.  Deterministic per seed. */
 int ise_kf_relabel_p1f(const uint64_t *src, const uint32_t *src_meta, uint32_t nsrc, uint32_t src_stride,
+                       uint64_t count, int rmin, uint64_t seed, uint32_t stride, uint64_t *rounds_out,
+                       uint32_t *meta_out);
+
+/* The same for uniform 1-factorizations: the drawn pair of rounds goes onto
+ * the source's rounds 0, 1 cycle by cycle (cycles of one length in a random
+ * assignment, a random start and direction per cycle).  Every pair of rounds
+ * of a source must have the cycle lengths of its rounds 0, 1. */
+int ise_kf_relabel_u1f(const uint64_t *src, const uint32_t *src_meta, uint32_t nsrc, uint32_t src_stride,
                        uint64_t count, int rmin, uint64_t seed, uint32_t stride, uint64_t *rounds_out,
                        uint32_t *meta_out);
 

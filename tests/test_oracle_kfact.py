@@ -239,3 +239,22 @@ def test_u1f_other_cycle_type_is_not_canonical(oracle):
     m = np.array(golden("kfact.json")["k10"]["matrices"][0]["rounds"], np.uint8)
     v, w, auts, t = oracle.kf_check_2u1f(m)
     assert v == 0 and w == (1 << 64) - 1
+
+
+def test_u1f_relabel_generator(oracle):
+    """ise_kf_relabel_p1f(uniform=True): candidates in normal form with the
+    source's rounds 0, 1, every pair of rounds of their cycle type, some
+    canonical and some not."""
+    block = golden("kfact.json")["u12"]
+    src_rows, src_meta = ise_amd.kf_pack([m["rounds"] for m in block["matrices"]])
+    rows, meta = ise_amd.kf_relabel_p1f(src_rows, src_meta, 400, 2, 0x7A, uniform=True)
+    std = np.array(block["matrices"][0]["rounds"], np.uint8)[:2]
+    verdicts = []
+    for i in range(len(meta)):
+        nr = int(meta[i]) & 0xff
+        mat = ise_amd.kf_unpack(rows[i, :nr], 12)
+        assert (mat[:2] == std).all() and (oracle.kf_normalize(mat) == mat).all()
+        v, w, auts, t = oracle.kf_check_2u1f(mat)
+        assert v == 1 or w != (1 << 64) - 1      # no pair of another cycle type
+        verdicts.append(v)
+    assert 0 < sum(verdicts) < len(verdicts)
