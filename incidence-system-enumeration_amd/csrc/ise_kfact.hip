@@ -181,7 +181,8 @@ __device__ __forceinline__ KfUni kf_stage(KfLds &s, uint64_t row, uint32_t nr, u
     return u;
 }
 
-__device__ __forceinline__ int kf_compare_labels(const KfLds &s, const KfUni &u, uint32_t vlo, uint32_t vhi);
+__device__ __forceinline__ int kf_compare_labels(const KfLds &s, const KfUni &u, uint32_t vlo, uint32_t vhi,
+                                                 uint32_t same0 = 15u, uint32_t same1 = 15u);
 
 // Label-order comparison of relabelling r (I = inverse slot arrangement).
 __device__ __forceinline__ int kf_compare_fast(const KfLds &s, const KfUni &u, uint64_t I, uint32_t r) {
@@ -203,16 +204,19 @@ __device__ __forceinline__ int kf_compare_fast(const KfLds &s, const KfUni &u, u
 }
 
 // The label-order walk for a relabelling given as its inverse map label ->
-// player (vlo:vhi): -1 smaller, 0 equal, 1 larger.
-__device__ __forceinline__ int kf_compare_labels(const KfLds &s, const KfUni &u, uint32_t vlo, uint32_t vhi) {
+// player (vlo:vhi): -1 smaller, 0 equal, 1 larger.  Rounds same0 / same1 are
+// known to become the given rounds 0 / 1 exactly (two-row relabellings): they
+// are not compared when they land in those positions.
+__device__ __forceinline__ int kf_compare_labels(const KfLds &s, const KfUni &u, uint32_t vlo, uint32_t vhi,
+                                                 uint32_t same0, uint32_t same1) {
     const uint64_t mt = s.meet[vlo & 0xfu];
     const uint32_t elo = (uint32_t)mt, ehi = (uint32_t)(mt >> 32);
     uint32_t i = 0;
     for (uint32_t L = 1; L < u.n; L++) {
         const uint32_t k = tget(elo, ehi, tget(vlo, vhi, L));
         if (k == 15u) continue;
-        if (i == 0 && u.skip0) {
-            i = 1;
+        if ((i == 0 && (u.skip0 || k == same0)) || (i == 1 && k == same1)) {
+            i++;
             continue;
         }
         const uint32_t g = tget(u.glo, u.ghi, i);
@@ -819,25 +823,20 @@ __device__ __forceinline__ int kf_u1f_compare(const KfLds &s, const KfUni &u, co
         const uint32_t d = var % h;
         var /= h;
         const bool back = (dirs >> (C - 1 - c)) & 1u;
-        // Q from its start to index k0 = 2d (+1 reversed)
-        uint32_t q = (sout >> (4 * c)) & 0xfu, k = 0;
+        // Q from its start to index 2d (+1 reversed).  Either way the walk
+        // then takes an r0 edge, an r1 edge, ...: forward from the even index
+        // 2d, and backward from 2d + 1, whose edge to 2d is an r0 edge.
+        uint32_t q = (sout >> (4 * c)) & 0xfu;
         const uint32_t k0 = 2 * d + (back ? 1u : 0u);
-        for (; k < k0; k++) q = (k & 1u) ? tget(blo, bhi, q) : tget(alo, ahi, q);
+        for (uint32_t k = 0; k < k0; k++) q = (k & 1u) ? tget(blo, bhi, q) : tget(alo, ahi, q);
         uint32_t p = (w.start01 >> (4 * c)) & 0xfu;
         for (uint32_t m = 0; m < L; m++) {
             tset(vlo, vhi, p, q);
             p = (m & 1u) ? tget(z1lo, z1hi, p) : tget(z0lo, z0hi, p);
-            if (back) {
-                // from Q[k] to Q[k-1]: the r0 edge when k is odd
-                q = (k & 1u) ? tget(alo, ahi, q) : tget(blo, bhi, q);
-                k = k ? k - 1 : L - 1;
-            } else {
-                q = (k & 1u) ? tget(blo, bhi, q) : tget(alo, ahi, q);
-                k = k + 1 == L ? 0 : k + 1;
-            }
+            q = (m & 1u) ? tget(blo, bhi, q) : tget(alo, ahi, q);
         }
     }
-    return kf_compare_labels(s, u, vlo, vhi);
+    return kf_compare_labels(s, u, vlo, vhi, r0, r1);
 }
 
 __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t stride,
