@@ -178,3 +178,21 @@ def test_rounds_not_in_normal_form(gpu_ctx, oracle, variant):
     rows, meta = ise_amd.kf_pack(mats)
     g = gpu_ctx.kf_check(rows, meta)
     assert_same(g, oracle_batch(oracle, rows, meta, 12), variant)
+
+
+def test_split_buffers_grow_past_capacity(oracle):
+    """A device batch larger than the context capacity grows the split launch's
+    buffers on its first call and gives the oracle's results."""
+    print("case kfact split buffers grow", flush=True)
+    rows, meta = ise_amd.kf_random(12, 3, 10, 5000, 0x6B)
+    with ise_amd.Context(capacity=1024, stride=16) as ctx:
+        n = len(meta)
+        d_rows = ctx.device_array(rows.shape, np.uint64, rows)
+        d_meta = ctx.device_array(meta.shape, np.uint32, meta)
+        d_head = ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
+        dev, s = ctx.kf_device_results(n)
+        ctx.kf_time_batch_device(n, 16, d_rows, d_meta, s, d_head, 1)
+        g = {k: dev[k].to_host() for k in ise_amd.KF_FIELDS}
+        for b in list(dev.values()) + [d_rows, d_meta, d_head]:
+            b.free()
+    assert_same(g, oracle_batch(oracle, rows, meta, 12), "grown buffers")
