@@ -48,13 +48,20 @@ constexpr uint32_t KF_STOP_EVERY = 4;         // chunks between reads of the pub
 
 __device__ __forceinline__ uint32_t nib(uint64_t w, uint32_t i) { return (uint32_t)(w >> (4 * i)) & 0xfu; }
 
-// Ranges per listed candidate: as many as the range words allow, at most
-// max_ranges.  Both the range and the combine kernel derive it from the list
-// length, so they agree.
-__device__ __forceinline__ uint32_t kf_ranges(uint32_t listed, uint32_t range_words, uint32_t max_ranges) {
+// Ranges per listed candidate in a window of `span` relabellings: as many as
+// the range words allow, at most max_ranges, and at least KF_MIN_RANGE_CHUNKS
+// chunks each (a range stages its candidate once, which costs about a chunk).
+// Both the range and the combine kernel derive it from the same values, so
+// they agree.
+constexpr uint32_t KF_MIN_RANGE_CHUNKS = 4;
+__device__ __forceinline__ uint32_t kf_ranges(uint32_t listed, uint32_t range_words, uint32_t max_ranges,
+                                              uint32_t span) {
     if (listed == 0) return 1u;
-    const uint32_t g = range_words / listed;
-    return g < 1u ? 1u : (g > max_ranges ? max_ranges : g);
+    uint32_t g = range_words / listed;
+    const uint32_t by_size = span / (64u * KF_MIN_RANGE_CHUNKS);
+    if (g > max_ranges) g = max_ranges;
+    if (g > by_size) g = by_size;
+    return g < 1u ? 1u : g;
 }
 
 __device__ __forceinline__ uint32_t kf_total(const KfTables &qp, uint32_t np, uint32_t nr) {
@@ -520,7 +527,7 @@ __global__ __launch_bounds__(64) void kf_range_kernel(uint32_t stride, const uin
     const uint32_t l = threadIdx.x;
     const uint32_t listed = head[KF_HEAD_WORD * (1 + p)];
     const uint32_t *list = sp.list[p & 1];
-    const uint32_t G = kf_ranges(listed, sp.range_words, sp.max_ranges);
+    const uint32_t G = kf_ranges(listed, sp.range_words, sp.max_ranges, whi - wlo);
     const uint32_t items = listed * G;
     // Items by a fixed grid stride: the ranges of one candidate are of one
     // size and lie on consecutive items, so they spread over the waves.  (One
@@ -562,10 +569,10 @@ __device__ __forceinline__ uint64_t kf_wave_sum(uint64_t x) {
 // Window p: per listed candidate (one wave each), the ranges in index order.
 __global__ __launch_bounds__(64) void kf_combine_kernel(const uint32_t *__restrict__ meta, uint32_t *head,
                                                         KfTables qp, KfSplit sp, KfOut o, uint32_t p,
-                                                        uint32_t whi) {
+                                                        uint32_t wlo, uint32_t whi) {
     const uint32_t l = threadIdx.x;
     const uint32_t listed = head[KF_HEAD_WORD * (1 + p)];
-    const uint32_t G = kf_ranges(listed, sp.range_words, sp.max_ranges);
+    const uint32_t G = kf_ranges(listed, sp.range_words, sp.max_ranges, whi - wlo);
     const uint32_t *list = sp.list[p & 1];
     const uint64_t *part = sp.part[p & 1];
     for (uint32_t j = blockIdx.x; j < listed; j += gridDim.x) {
@@ -954,7 +961,8 @@ hipError_t kf_launch(uint32_t blocks, uint32_t range_blocks, uint32_t ncand, uin
         const uint32_t w1 = whi > 0xffffffffull ? 0xffffffffu : (uint32_t)whi;
         hipLaunchKernelGGL(kf_range_kernel, dim3(range_blocks), dim3(64), 0, stream, stride, rounds, meta,
                            (const uint32_t *)head, qp, sp, p, (uint32_t)wlo, w1);
-        hipLaunchKernelGGL(kf_combine_kernel, dim3(1024), dim3(64), 0, stream, meta, head, qp, sp, o, p, w1);
+        hipLaunchKernelGGL(kf_combine_kernel, dim3(1024), dim3(64), 0, stream, meta, head, qp, sp, o, p,
+                           (uint32_t)wlo, w1);
         wlo = whi;
     }
     return hipGetLastError();
