@@ -139,7 +139,9 @@ int orc_mlp_gradient(const float *params, const float *X, const float *y, const 
 
 /* ---- k-factorizations of K_n (n <= 16): rounds of n/2 pairs ---- */
 /* Canonicity of a partial 1-factorization (nrounds x n bytes, each round the
- * n players as n/2 ordered pairs).  Returns 1 canonical, 0 not, <0 error.
+ * n players as n/2 ordered pairs).  Returns 1 canonical, 0 not, <0 error
+ * (bad sizes, a round that is not a perfect matching of 0..n-1, or a pair in
+ * two rounds; the same holds for every orc_kf_* canonizer below).
  * On non-canonical, *witness receives the index of the first relabelling in
  * plain order that produced a smaller normal form; n_auts counts
  * relabellings that reproduced the matrix. */

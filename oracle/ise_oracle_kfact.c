@@ -53,6 +53,27 @@ static long quickperm_table(int k, u8 *out)
     return cnt;
 }
 
+/* partner arrays part[r][x] of an nr x n matrix; 0 unless every round is a
+ * perfect matching of players 0..n-1 and no pair appears in two rounds (the
+ * checks the device path's host side makes before a launch) */
+static int kf_partners(const u8 *mat, int n, int nr, u8 part[][KF_MAX_N])
+{
+    u8 met[KF_MAX_N][KF_MAX_N];
+    memset(met, 0, sizeof met);
+    for (int r = 0; r < nr; r++) {
+        int seen = 0;
+        for (int q = 0; q < n; q += 2) {
+            const int x = mat[r * n + q], y = mat[r * n + q + 1];
+            if (x >= n || y >= n || x == y || (seen >> x & 1) || (seen >> y & 1) || met[x][y]) return 0;
+            seen |= (1 << x) | (1 << y);
+            met[x][y] = met[y][x] = 1;
+            part[r][x] = (u8)y;
+            part[r][y] = (u8)x;
+        }
+    }
+    return 1;
+}
+
 /* normal form of matrix mi (nr x n) relabelled by tr, into mo */
 static void normal_form(const u8 *mi, int nr, int n, const u8 *tr, u8 *mo)
 {
@@ -90,6 +111,8 @@ int orc_kf_check(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
                  uint64_t *n_auts, uint64_t *n_tried)
 {
     if (n < 4 || n > KF_MAX_N || (n & 1) || nrounds < 1 || nrounds > n - 1) return -1;
+    u8 part[KF_MAX_N][KF_MAX_N];
+    if (!kf_partners(mat, n, nrounds, part)) return -1;
     const int npairs = n / 2;
     long nperm = 1;
     for (int i = 2; i <= npairs; i++) nperm *= i;
@@ -134,7 +157,9 @@ int orc_kf_check(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
  * reproduce the matrix. */
 int orc_kf_bruteforce(const uint8_t *mat, int n, int nrounds, uint64_t *n_auts)
 {
-    if (n > 8) return -1;
+    u8 part[KF_MAX_N][KF_MAX_N];
+    if (n < 4 || n > 8 || (n & 1) || nrounds < 1 || nrounds > n - 1 || !kf_partners(mat, n, nrounds, part))
+        return -1;
     u8 tr[KF_MAX_N], mo[KF_MAX_N * KF_MAX_N];
     int c[KF_MAX_N], i = 0, canon = 1;
     uint64_t auts = 0;
@@ -190,12 +215,7 @@ int orc_kf_check_2p1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
 {
     if (n < 4 || n > KF_MAX_N || (n & 1) || nrounds < 1 || nrounds > n - 1) return -1;
     u8 part[KF_MAX_N][KF_MAX_N];
-    for (int r = 0; r < nrounds; r++)
-        for (int q = 0; q < n; q += 2) {
-            const u8 x = mat[r * n + q], y = mat[r * n + q + 1];
-            part[r][x] = y;
-            part[r][y] = x;
-        }
+    if (!kf_partners(mat, n, nrounds, part)) return -1;
     for (int a = 0; a < nrounds; a++)
         for (int b = a + 1; b < nrounds; b++)
             if (!kf_is_p1f_pair(part[a], part[b], n)) return -1;
@@ -291,6 +311,7 @@ static void kf_pair_cycles(const u8 *pa, const u8 *pb, int n, kf_cycles *c)
     c->n = 0;
     for (int k0 = 0; k0 < n; k0++) {
         if (used[k0]) continue;
+        if (c->n == KF_MAXCYC) { c->n = KF_MAXCYC + 1; return; }   /* more cycles than rounds 0, 1 can have */
         c->start[c->n] = ip / 2;
         int k = k0, len = 0;
         for (;;) {
@@ -330,12 +351,7 @@ int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
 {
     if (n < 4 || n > KF_MAX_N || (n & 1) || nrounds < 1 || nrounds > n - 1) return -1;
     u8 part[KF_MAX_N][KF_MAX_N];
-    for (int r = 0; r < nrounds; r++)
-        for (int q = 0; q < n; q += 2) {
-            const u8 x = mat[r * n + q], y = mat[r * n + q + 1];
-            part[r][x] = y;
-            part[r][y] = x;
-        }
+    if (!kf_partners(mat, n, nrounds, part)) return -1;
     uint64_t auts = 0, tried = 0, idx = 0;
     int verdict = 1;
     if (witness) *witness = ~0ull;

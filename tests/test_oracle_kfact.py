@@ -258,3 +258,21 @@ def test_u1f_relabel_generator(oracle):
         assert v == 1 or w != (1 << 64) - 1      # no pair of another cycle type
         verdicts.append(v)
     assert 0 < sum(verdicts) < len(verdicts)
+
+
+@pytest.mark.parametrize("fn", ["kf_check", "kf_check_2p1f", "kf_check_2u1f"])
+def test_canonizers_reject_what_is_not_a_factorization(oracle, fn):
+    """A round that is not a perfect matching of the players, or a pair met in
+    two rounds, is an argument error for every canonizer (the device path's
+    host side rejects the same records before a launch)."""
+    m = np.array(golden("kfact.json")["p10"]["matrices"][0]["rounds"], np.uint8)
+    check = getattr(oracle, fn)
+    check(m[:3])                                   # accepted as given
+    repeated = np.concatenate([m[:2], m[:1]])      # round 2 = round 0: five 2-cycles with round 0
+    bad_player = m[:3].copy()
+    bad_player[2, 3] = 10                          # no player 10 in K_10
+    twice = m[:3].copy()
+    twice[1, 0:2] = twice[1, 2:4]                  # a player twice in one round
+    for bad in (repeated, bad_player, twice):
+        with pytest.raises(ValueError):
+            check(bad)
