@@ -228,11 +228,16 @@ __device__ __forceinline__ int kf_compare_labels(const KfLds &s, const KfUni &u,
         }
         const uint32_t g = tget(u.glo, u.ghi, i);
         if (L != g) return L < g ? -1 : 1;
+        // The partner arrays can first differ only at the first element of a
+        // given pair (at a second element b of pair (a, b), a < b, position
+        // a already differs), so the walk visits the given pairs (a, gv) in
+        // order; pair 0 is (0, L), equal here.
         const uint32_t klo = s.plo[k], khi = s.phi[k];
-        const uint32_t qlo = s.plo[i], qhi = s.phi[i];
-        for (uint32_t v = 1; v < u.n; v++) {
-            const uint32_t gv = tget(qlo, qhi, v);                       // given: partner of v
-            const uint32_t z = tget(klo, khi, tget(vlo, vhi, v));        // player whose label is v's partner
+        const uint32_t qlo = s.rlo[i], qhi = s.rhi[i];
+        for (uint32_t j = 1; j < u.np; j++) {
+            const uint32_t pr = __builtin_amdgcn_ubfe(j < 4 ? qlo : qhi, (j & 3u) << 3, 8);
+            const uint32_t a = pr & 0xfu, gv = pr >> 4;                   // given pair (a, gv)
+            const uint32_t z = tget(klo, khi, tget(vlo, vhi, a));        // player whose label is a's partner
             if (z != tget(vlo, vhi, gv)) {
                 const uint32_t lz = tfind(vlo, vhi, z);
                 return lz < gv ? -1 : 1;
@@ -654,11 +659,13 @@ __device__ __forceinline__ int kf2_compare(const KfLds &s, const KfUni &u, uint3
         if (m == 15u) return 1;
         const uint32_t g = tget(u.glo, u.ghi, p);
         if (L != g) return L < g ? -1 : 1;
+        // given pairs (a, gv) after (0, L), as in kf_compare_labels
         const uint32_t klo = s.plo[m], khi = s.phi[m];
-        const uint32_t qlo = s.plo[p], qhi = s.phi[p];
-        for (uint32_t v = 1; v < n; v++) {
-            const uint32_t gv = tget(qlo, qhi, v);                  // given: partner of v
-            const uint32_t z = tget(klo, khi, tget(vlo, vhi, v));   // player whose label is v's partner
+        const uint32_t qlo = s.rlo[p], qhi = s.rhi[p];
+        for (uint32_t j = 1; j < u.np; j++) {
+            const uint32_t pr = __builtin_amdgcn_ubfe(j < 4 ? qlo : qhi, (j & 3u) << 3, 8);
+            const uint32_t a = pr & 0xfu, gv = pr >> 4;
+            const uint32_t z = tget(klo, khi, tget(vlo, vhi, a));   // player whose label is a's partner
             if (z != tget(vlo, vhi, gv)) return tfind(vlo, vhi, z) < gv ? -1 : 1;
         }
     }
@@ -749,14 +756,40 @@ __global__ __launch_bounds__(64) void kf2_check_kernel(uint32_t ncand, uint32_t 
 // two groups of n <= 16 both have two cycles or more, so ranks map one to one.)
 // A pair of another cycle type than rounds 0, 1 ends the test: not canonical,
 // no witness, the relabellings before it tried.  Rounds must be in normal form.
+//
+// Layout on the device.  Each ordered pair po (unordered pair po >> 1 in
+// order, then the swap bit) has its cycles staged once per candidate as a
+// path: the players of its sorted cycles, each from its start going r0-partner
+// first, 4 bits each (path[po]).  Swapping r0 and r1 keeps the cycles, their
+// order and their starts and reverses each walk: Q'[j] = Q[(L - j) mod L].
+// So "reversed from 2d + 1" is the swapped pair's path rotated by L - 2d - 1,
+// and "forward from 2d" the pair's own path rotated by 2d: the label -> player
+// map of a relabelling is a rotation of one segment per cycle, scattered by
+// the players of rounds 0, 1 in their path order.  Lanes step through the
+// relabellings in place (pair po, position t) and decode t with
+// multiply-high by per-candidate magic numbers (all digits are below 2^13).
 
 struct KfU1f {
     uint32_t ncyc, T, cut;            // cycles, relabellings per pair, relabellings before a mismatched pair
-    uint32_t start01, len01;          // rounds 0, 1: start players / half lengths, 4 bits per cycle
+    uint32_t len01, off01;            // rounds 0, 1: half length and first path position of each cycle (4 bits each)
+    uint32_t g0, gm;                  // the one group of equal cycles that permutes: first cycle, size (1: none)
+    uint32_t V, mV, mh[4];            // product of the half lengths; magic numbers for V and each half length
+    uint32_t plo01, phi01;            // rounds 0, 1: players in path order
 };
 
+struct KfU1fLds {
+    uint64_t path[KF_MAX_ROUNDS * (KF_MAX_ROUNDS - 1)];   // ordered pair: players of its cycles in path order
+    uint32_t rr[KF_MAX_ROUNDS * (KF_MAX_ROUNDS - 1)];     // ordered pair: r0 | r1 << 4
+    uint32_t perm[24];                                     // lexicographic orders of the permuting group
+    uint32_t bad;                                          // least unordered pair of another cycle type
+};
+
+// ceil(2^32 / d) for 2 <= d < 2^13: __umulhi(x, kf_magic(d)) = x / d for x < 2^19
+__device__ __forceinline__ uint32_t kf_magic(uint32_t d) { return 0xffffffffu / d + 1u; }
+
 // Cycles of rounds a, b (partner tables alo:ahi, blo:bhi): starts (4 bits per
-// cycle, sorted order) and half lengths likewise; returns the cycle count.
+// cycle, sorted order) and half lengths likewise; returns the cycle count
+// (at most four when the rounds share no pair).
 __device__ __forceinline__ uint32_t kf_cycles(uint32_t alo, uint32_t ahi, uint32_t blo, uint32_t bhi, uint32_t n,
                                               uint32_t &starts, uint32_t &halves) {
     uint32_t seen = 0, nc = 0;
@@ -781,76 +814,62 @@ __device__ __forceinline__ uint32_t kf_cycles(uint32_t alo, uint32_t ahi, uint32
     return nc;
 }
 
-__device__ __forceinline__ int kf_u1f_compare(const KfLds &s, const KfUni &u, const KfU1f &w, uint32_t r,
-                                              uint32_t z0lo, uint32_t z0hi, uint32_t z1lo, uint32_t z1hi) {
-    const uint32_t n = u.n, C = w.ncyc;
-    const uint32_t po = r / w.T, t = r - po * w.T;
-    const uint32_t pi = po >> 1;
-    uint32_t last = 1;
-    while ((last + 1) * last / 2 <= pi) last++;
-    const uint32_t ind = pi - last * (last - 1) / 2;
-    const uint32_t r0 = (po & 1) ? last : ind, r1 = (po & 1) ? ind : last;
-    const uint32_t alo = s.plo[r0], ahi = s.phi[r0], blo = s.plo[r1], bhi = s.phi[r1];
-    uint32_t starts, halves;
-    (void)kf_cycles(alo, ahi, blo, bhi, n, starts, halves);
-    // decode: directions (low bits), offsets, permutation ranks
-    const uint32_t dirs = t & ((1u << C) - 1u);
-    uint32_t rest = t >> C, V = 1;
-    for (uint32_t c = 0; c < C; c++) V *= (w.len01 >> (4 * c)) & 0xfu;
-    uint32_t var = rest % V, rank = rest / V;
-    uint32_t sout = starts;
-    for (uint32_t g0 = 0; g0 < C;) {
-        const uint32_t h = (w.len01 >> (4 * g0)) & 0xfu;
-        uint32_t g1 = g0 + 1;
-        while (g1 < C && ((w.len01 >> (4 * g1)) & 0xfu) == h) g1++;
-        const uint32_t m = g1 - g0;
-        uint32_t fact = 1;
-        for (uint32_t f = 2; f <= m; f++) fact *= f;
-        uint32_t rk = rank % fact;
-        rank /= fact;
-        // lexicographic unrank of rk over the group's cycles
-        uint32_t avail = (1u << m) - 1u;
-        for (uint32_t j = 0; j < m; j++) {
-            fact /= (m - j);
-            uint32_t pick = rk / fact;
-            rk %= fact;
-            uint32_t bits = avail;
-            while (pick--) bits &= bits - 1u;
-            const uint32_t e = (uint32_t)__builtin_ctz(bits);
-            avail &= ~(1u << e);
-            const uint32_t st = (starts >> (4 * (g0 + e))) & 0xfu;
-            sout = (sout & ~(0xfu << (4 * (g0 + j)))) | (st << (4 * (g0 + j)));
+// The players of the sorted cycles of rounds a, b in path order: each cycle
+// from its start, a-partner first.
+__device__ __forceinline__ uint64_t kf_path(uint32_t alo, uint32_t ahi, uint32_t blo, uint32_t bhi, uint32_t nc,
+                                            uint32_t starts, uint32_t halves) {
+    uint64_t path = 0;
+    uint32_t pos = 0;
+    for (uint32_t c = 0; c < nc; c++) {
+        uint32_t q = (starts >> (4 * c)) & 0xfu;
+        const uint32_t L = 2 * ((halves >> (4 * c)) & 0xfu);
+        for (uint32_t j = 0; j < L; j++, pos++) {
+            path |= (uint64_t)q << (4 * pos);
+            q = (j & 1u) ? tget(blo, bhi, q) : tget(alo, ahi, q);
         }
-        g0 = g1;
     }
-    // label -> player
-    uint32_t vlo = 0, vhi = 0;
-    for (uint32_t c = 0; c < C; c++) {
+    return path;
+}
+
+// Relabelling (po, t): -1 smaller, 0 equal, 1 larger.
+__device__ __forceinline__ int kf_u1f_compare(const KfLds &s, const KfU1fLds &x, const KfUni &u, const KfU1f &w,
+                                              uint32_t po, uint32_t t) {
+    const uint32_t C = w.ncyc;
+    const uint64_t qf = x.path[po], qb = x.path[po ^ 1u];
+    const uint32_t rr = x.rr[po];
+    const uint32_t dirs = t & ((1u << C) - 1u), rest = t >> C;
+    const uint32_t rank = __umulhi(rest, w.mV);
+    uint32_t var = rest - rank * w.V;
+    const uint32_t pw = w.gm > 1 ? x.perm[rank] : 0u;
+    uint64_t R = 0;   // players in the path order of rounds 0, 1
+#pragma unroll
+    for (uint32_t c = 0; c < 4; c++) {
+        if (c >= C) break;
         const uint32_t h = (w.len01 >> (4 * c)) & 0xfu, L = 2 * h;
-        const uint32_t d = var % h;
-        var /= h;
+        const uint32_t q = __umulhi(var, w.mh[c]);
+        const uint32_t d = var - q * h;
+        var = q;
         const bool back = (dirs >> (C - 1 - c)) & 1u;
-        // Q from its start to index 2d (+1 reversed).  Either way the walk
-        // then takes an r0 edge, an r1 edge, ...: forward from the even index
-        // 2d, and backward from 2d + 1, whose edge to 2d is an r0 edge.
-        uint32_t q = (sout >> (4 * c)) & 0xfu;
-        const uint32_t k0 = 2 * d + (back ? 1u : 0u);
-        for (uint32_t k = 0; k < k0; k++) q = (k & 1u) ? tget(blo, bhi, q) : tget(alo, ahi, q);
-        uint32_t p = (w.start01 >> (4 * c)) & 0xfu;
-        for (uint32_t m = 0; m < L; m++) {
-            tset(vlo, vhi, p, q);
-            p = (m & 1u) ? tget(z1lo, z1hi, p) : tget(z0lo, z0hi, p);
-            q = (m & 1u) ? tget(blo, bhi, q) : tget(alo, ahi, q);
-        }
+        const uint32_t e = c - w.g0 < w.gm ? w.g0 + ((pw >> (4 * (c - w.g0))) & 0xfu) : c;
+        const uint32_t offe = (w.off01 >> (4 * e)) & 0xfu, offc = (w.off01 >> (4 * c)) & 0xfu;
+        const uint64_t mask = L == 16 ? ~0ull : (1ull << (4 * L)) - 1ull;
+        const uint64_t seg = ((back ? qb : qf) >> (4 * offe)) & mask;
+        const uint32_t rot = back ? L - 2 * d - 1 : 2 * d;
+        const uint64_t segr = rot ? ((seg >> (4 * rot)) | (seg << (4 * (L - rot)))) & mask : seg;
+        R |= segr << (4 * offc);
     }
-    return kf_compare_labels(s, u, vlo, vhi, r0, r1);
+    const uint32_t Rlo = (uint32_t)R, Rhi = (uint32_t)(R >> 32);
+    uint32_t vlo = 0, vhi = 0;   // label -> player
+    for (uint32_t m = 0; m < u.n; m++)
+        tset(vlo, vhi, tget(w.plo01, w.phi01, m), tget(Rlo, Rhi, m));
+    return kf_compare_labels(s, u, vlo, vhi, rr & 0xfu, rr >> 4);
 }
 
 __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t stride,
                                                     const uint64_t *__restrict__ rounds,
                                                     const uint32_t *__restrict__ meta, KfOut o) {
     __shared__ KfLds s;
-    __shared__ uint32_t s_bad;
+    __shared__ KfU1fLds x;
     const uint32_t l = threadIdx.x;
     const uint32_t rl = stride < KF_MAX_ROUNDS ? stride : KF_MAX_ROUNDS;
     for (uint32_t ci = blockIdx.x; ci < ncand; ci += gridDim.x) {
@@ -866,47 +885,94 @@ __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t str
             __syncthreads();
             continue;
         }
-        KfU1f w{0u, 1u, 0u, 0u, 0u};
+        KfU1f w{};
+        w.T = 1u;
         uint32_t total = 0;
-        const uint32_t z0lo = kf_uni(s.plo[0]), z0hi = kf_uni(s.phi[0]);
-        const uint32_t z1lo = nr > 1 ? kf_uni(s.plo[1]) : 0u, z1hi = nr > 1 ? kf_uni(s.phi[1]) : 0u;
         if (nr > 1) {
+            const uint32_t z0lo = kf_uni(s.plo[0]), z0hi = kf_uni(s.phi[0]);
+            const uint32_t z1lo = kf_uni(s.plo[1]), z1hi = kf_uni(s.phi[1]);
             uint32_t st01, h01;
             w.ncyc = kf_uni(kf_cycles(z0lo, z0hi, z1lo, z1hi, n, st01, h01));
-            w.start01 = kf_uni(st01);
             w.len01 = kf_uni(h01);
-            uint32_t T = 1;
+            const uint64_t p01 = kf_path(z0lo, z0hi, z1lo, z1hi, w.ncyc, kf_uni(st01), w.len01);
+            w.plo01 = kf_uni((uint32_t)p01);
+            w.phi01 = kf_uni((uint32_t)(p01 >> 32));
+            uint32_t T = 1, V = 1, off = 0;
+            w.g0 = 0;
+            w.gm = 1;
             for (uint32_t c = 0, g = 0; c < w.ncyc; c = g) {
                 const uint32_t h = (w.len01 >> (4 * c)) & 0xfu;
-                for (g = c; g < w.ncyc && ((w.len01 >> (4 * g)) & 0xfu) == h; g++) T *= 2 * h;
+                for (g = c; g < w.ncyc && ((w.len01 >> (4 * g)) & 0xfu) == h; g++) {
+                    T *= 2 * h;
+                    V *= h;
+                    w.off01 |= off << (4 * g);
+                    off += 2 * h;
+                }
                 for (uint32_t f = 2; f <= g - c; f++) T *= f;
+                if (g - c > 1) {
+                    w.g0 = c;
+                    w.gm = g - c;
+                }
             }
             w.T = T;
-            // the first unordered pair (in order) of another cycle type
-            if (l == 0) s_bad = 0xffffffffu;
+            w.V = V;
+            w.mV = kf_magic(V);
+#pragma unroll
+            for (uint32_t c = 0; c < 4; c++) w.mh[c] = c < w.ncyc ? kf_magic((w.len01 >> (4 * c)) & 0xfu) : 0u;
+            // the orders of the permuting group, lexicographic
+            uint32_t fact = 1;
+            for (uint32_t f = 2; f <= w.gm; f++) fact *= f;
+            if (w.gm > 1 && l < fact) {
+                uint32_t rk = l, fr = fact, avail = (1u << w.gm) - 1u, word = 0;
+                for (uint32_t j = 0; j < w.gm; j++) {
+                    fr /= (w.gm - j);
+                    uint32_t pick = rk / fr;
+                    rk %= fr;
+                    uint32_t bits = avail;
+                    while (pick--) bits &= bits - 1u;
+                    const uint32_t e = (uint32_t)__builtin_ctz(bits);
+                    avail &= ~(1u << e);
+                    word |= e << (4 * j);
+                }
+                x.perm[l] = word;
+            }
+            // every ordered pair's path, and the first unordered pair (in
+            // order) of another cycle type
+            if (l == 0) x.bad = 0xffffffffu;
             __syncthreads();
-            const uint32_t npairs = nr * (nr - 1) / 2;
-            for (uint32_t b0 = 0; b0 < npairs; b0 += 64) {
-                const uint32_t pi = b0 + l;
-                if (pi < npairs) {
-                    uint32_t b = 1;
-                    while ((b + 1) * b / 2 <= pi) b++;
-                    const uint32_t a = pi - b * (b - 1) / 2;
+            const uint32_t nord = nr * (nr - 1);
+            for (uint32_t b0 = 0; b0 < nord; b0 += 64) {
+                const uint32_t po = b0 + l;
+                if (po < nord) {
+                    const uint32_t pi = po >> 1;
+                    uint32_t last = 1;
+                    while ((last + 1) * last / 2 <= pi) last++;
+                    const uint32_t ind = pi - last * (last - 1) / 2;
+                    const uint32_t r0 = (po & 1) ? last : ind, r1 = (po & 1) ? ind : last;
+                    const uint32_t alo = s.plo[r0], ahi = s.phi[r0], blo = s.plo[r1], bhi = s.phi[r1];
                     uint32_t st, h;
-                    const uint32_t nc = kf_cycles(s.plo[a], s.phi[a], s.plo[b], s.phi[b], n, st, h);
-                    if (nc != w.ncyc || h != w.len01) atomicMin(&s_bad, pi);
+                    const uint32_t nc = kf_cycles(alo, ahi, blo, bhi, n, st, h);
+                    const bool same = nc == w.ncyc && h == w.len01;
+                    if (!same) atomicMin(&x.bad, pi);
+                    x.path[po] = same ? kf_path(alo, ahi, blo, bhi, nc, st, h) : 0ull;
+                    x.rr[po] = r0 | (r1 << 4);
                 }
             }
             __syncthreads();
-            const uint32_t bad = kf_uni(s_bad);
+            const uint32_t bad = kf_uni(x.bad);
             total = nr * (nr - 1) * T;
             w.cut = bad == 0xffffffffu ? total : 2u * bad * T;
         }
         uint64_t auts = 0;
         uint32_t witness = KF_NONE;
+        // lane l starts at relabelling l: ordered pair l / T, position l % T
+        const uint32_t mT = w.T > 1 ? kf_magic(w.T) : 0u;
+        uint32_t po = w.T > 1 ? __umulhi(l, mT) : l;
+        uint32_t t = l - po * w.T;
+        const uint32_t sq = w.T > 1 ? __umulhi(64u, mT) : 64u, sr = 64u - sq * w.T;
         for (uint32_t base = 0; base < w.cut; base += 64) {
             const uint32_t r = base + l;
-            const int icmp = r < w.cut ? kf_u1f_compare(s, u, w, r, z0lo, z0hi, z1lo, z1hi) : 1;
+            const int icmp = r < w.cut ? kf_u1f_compare(s, x, u, w, po, t) : 1;
             const uint64_t smaller = __builtin_amdgcn_ballot_w64(icmp < 0);
             const uint64_t equal = __builtin_amdgcn_ballot_w64(r < w.cut && icmp == 0);
             if (smaller) {
@@ -916,6 +982,12 @@ __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t str
                 break;
             }
             auts += __popcll(equal);
+            t += sr;
+            po += sq;
+            if (t >= w.T) {
+                t -= w.T;
+                po++;
+            }
         }
         if (l == 0) {
             if (witness == KF_NONE && w.cut < total) {   // a pair of another cycle type
