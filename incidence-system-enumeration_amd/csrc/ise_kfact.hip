@@ -188,8 +188,11 @@ __device__ __forceinline__ KfUni kf_stage(KfLds &s, uint64_t row, uint32_t nr, u
     return u;
 }
 
+constexpr int KF_OPEN = 2;   // kf_compare_labels: the first `lim` compared rounds are equal
+
 __device__ __forceinline__ int kf_compare_labels(const KfLds &s, const KfUni &u, uint32_t vlo, uint32_t vhi,
-                                                 uint32_t same0 = 15u, uint32_t same1 = 15u);
+                                                 uint32_t same0 = 15u, uint32_t same1 = 15u,
+                                                 uint32_t lim = 0xffffffffu);
 
 // Label-order comparison of relabelling r (I = inverse slot arrangement).
 __device__ __forceinline__ int kf_compare_fast(const KfLds &s, const KfUni &u, uint64_t I, uint32_t r) {
@@ -211,11 +214,12 @@ __device__ __forceinline__ int kf_compare_fast(const KfLds &s, const KfUni &u, u
 }
 
 // The label-order walk for a relabelling given as its inverse map label ->
-// player (vlo:vhi): -1 smaller, 0 equal, 1 larger.  Rounds same0 / same1 are
-// known to become the given rounds 0 / 1 exactly (two-row relabellings): they
-// are not compared when they land in those positions.
+// player (vlo:vhi): -1 smaller, 0 equal, 1 larger, KF_OPEN once `lim` rounds
+// were compared and found equal.  Rounds same0 / same1 are known to become
+// the given rounds 0 / 1 exactly (two-row relabellings): they are not
+// compared when they land in those positions.
 __device__ __forceinline__ int kf_compare_labels(const KfLds &s, const KfUni &u, uint32_t vlo, uint32_t vhi,
-                                                 uint32_t same0, uint32_t same1) {
+                                                 uint32_t same0, uint32_t same1, uint32_t lim) {
     const uint64_t mt = s.meet[vlo & 0xfu];
     const uint32_t elo = (uint32_t)mt, ehi = (uint32_t)(mt >> 32);
     uint32_t i = 0;
@@ -244,6 +248,7 @@ __device__ __forceinline__ int kf_compare_labels(const KfLds &s, const KfUni &u,
             }
         }
         i++;
+        if (--lim == 0) return KF_OPEN;
     }
     return 0;
 }
@@ -777,11 +782,16 @@ struct KfU1f {
     uint32_t plo01, phi01;            // rounds 0, 1: players in path order
 };
 
+constexpr uint32_t KF_DEFER_SLOTS = 128;   // ring of deferred relabellings (two chunks)
+constexpr uint32_t KF_U1F_FIRST_ROUNDS = 1;  // rounds compared before a relabelling is deferred
+
 struct KfU1fLds {
     uint64_t path[KF_MAX_ROUNDS * (KF_MAX_ROUNDS - 1)];   // ordered pair: players of its cycles in path order
     uint32_t rr[KF_MAX_ROUNDS * (KF_MAX_ROUNDS - 1)];     // ordered pair: r0 | r1 << 4
     uint32_t perm[24];                                     // lexicographic orders of the permuting group
     uint32_t bad;                                          // least unordered pair of another cycle type
+    uint32_t dr[KF_DEFER_SLOTS];                           // deferred: relabelling index
+    uint32_t dpt[KF_DEFER_SLOTS];                          // deferred: po << 16 | t, bit 31 = known equal
 };
 
 // ceil(2^32 / d) for 2 <= d < 2^13: __umulhi(x, kf_magic(d)) = x / d for x < 2^19
@@ -831,9 +841,10 @@ __device__ __forceinline__ uint64_t kf_path(uint32_t alo, uint32_t ahi, uint32_t
     return path;
 }
 
-// Relabelling (po, t): -1 smaller, 0 equal, 1 larger.
+// Relabelling (po, t): -1 smaller, 0 equal, 1 larger (KF_OPEN: equal in the
+// first `lim` compared rounds).
 __device__ __forceinline__ int kf_u1f_compare(const KfLds &s, const KfU1fLds &x, const KfUni &u, const KfU1f &w,
-                                              uint32_t po, uint32_t t) {
+                                              uint32_t po, uint32_t t, uint32_t lim) {
     const uint32_t C = w.ncyc;
     const uint64_t qf = x.path[po], qb = x.path[po ^ 1u];
     const uint32_t rr = x.rr[po];
@@ -862,7 +873,7 @@ __device__ __forceinline__ int kf_u1f_compare(const KfLds &s, const KfU1fLds &x,
     uint32_t vlo = 0, vhi = 0;   // label -> player
     for (uint32_t m = 0; m < u.n; m++)
         tset(vlo, vhi, tget(w.plo01, w.phi01, m), tget(Rlo, Rhi, m));
-    return kf_compare_labels(s, u, vlo, vhi, rr & 0xfu, rr >> 4);
+    return kf_compare_labels(s, u, vlo, vhi, rr & 0xfu, rr >> 4, lim);
 }
 
 __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t stride,
@@ -965,23 +976,60 @@ __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t str
         }
         uint64_t auts = 0;
         uint32_t witness = KF_NONE;
+        // Deferred relabellings, in index order: [head, tail) of the ring.
+        // Full comparisons of up to 64 of them at a time; the first smaller
+        // one ends the scan, and the equal ones before it are automorphisms.
+        uint32_t head = 0, tail = 0;
+        auto drain = [&](bool all) {
+            __syncthreads();
+            while (tail - head >= 64u || (all && tail != head)) {
+                const uint32_t cnt = tail - head < 64u ? tail - head : 64u;
+                const uint32_t slot = (head + l) % KF_DEFER_SLOTS;
+                int c = 1;
+                if (l < cnt) {
+                    const uint32_t pt = x.dpt[slot];
+                    c = (pt >> 31) ? 0 : kf_u1f_compare(s, x, u, w, (pt >> 16) & 0x7fffu, pt & 0xffffu, 0xffffffffu);
+                }
+                const uint64_t smaller = __builtin_amdgcn_ballot_w64(c < 0);
+                const uint64_t equal = __builtin_amdgcn_ballot_w64(l < cnt && c == 0);
+                if (smaller) {
+                    const uint32_t first = __builtin_ctzll(smaller);
+                    auts += __popcll(equal & ((1ull << first) - 1ull));
+                    witness = kf_uni(x.dr[(head + first) % KF_DEFER_SLOTS]);
+                    return;
+                }
+                auts += __popcll(equal);
+                head += cnt;
+            }
+        };
         // lane l starts at relabelling l: ordered pair l / T, position l % T
         const uint32_t mT = w.T > 1 ? kf_magic(w.T) : 0u;
         uint32_t po = w.T > 1 ? __umulhi(l, mT) : l;
         uint32_t t = l - po * w.T;
         const uint32_t sq = w.T > 1 ? __umulhi(64u, mT) : 64u, sr = 64u - sq * w.T;
         for (uint32_t base = 0; base < w.cut; base += 64) {
+            // first pass: decided on the first compared round, or deferred
             const uint32_t r = base + l;
-            const int icmp = r < w.cut ? kf_u1f_compare(s, x, u, w, po, t) : 1;
+            const int icmp = r < w.cut ? kf_u1f_compare(s, x, u, w, po, t, KF_U1F_FIRST_ROUNDS) : 1;
             const uint64_t smaller = __builtin_amdgcn_ballot_w64(icmp < 0);
-            const uint64_t equal = __builtin_amdgcn_ballot_w64(r < w.cut && icmp == 0);
+            uint64_t defer = __builtin_amdgcn_ballot_w64(r < w.cut && icmp >= 0 && icmp != 1);
+            const uint32_t first = smaller ? (uint32_t)__builtin_ctzll(smaller) : 64u;
+            if (smaller) defer &= (1ull << first) - 1ull;
+            if ((defer >> l) & 1u) {
+                const uint32_t slot = (tail + __popcll(defer & ((1ull << l) - 1ull))) % KF_DEFER_SLOTS;
+                x.dr[slot] = r;
+                x.dpt[slot] = (icmp == 0 ? 0x80000000u : 0u) | (po << 16) | t;
+            }
+            tail += __popcll(defer);
             if (smaller) {
-                const uint32_t first = __builtin_ctzll(smaller);
-                auts += __popcll(equal & ((1ull << first) - 1ull));
-                witness = base + first;
+                drain(true);
+                if (witness == KF_NONE) witness = base + first;
                 break;
             }
-            auts += __popcll(equal);
+            if (tail - head >= 64u) {
+                drain(false);
+                if (witness != KF_NONE) break;
+            }
             t += sr;
             po += sq;
             if (t >= w.T) {
@@ -989,6 +1037,7 @@ __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t str
                 po++;
             }
         }
+        if (witness == KF_NONE) drain(true);
         if (l == 0) {
             if (witness == KF_NONE && w.cut < total) {   // a pair of another cycle type
                 o.verdict[ci] = ISE_VERDICT_NONCANONICAL;
