@@ -635,6 +635,89 @@ __global__ __launch_bounds__(64) void kf_combine_kernel(const uint32_t *__restri
 // one Hamiltonian cycle; otherwise the record is invalid.
 
 // Relabelling r against the matrix: -1 smaller, 0 equal, 1 not smaller.
+// ---- deferred full comparisons (the U1F canonizer) -------------------------
+//
+// In a canonical candidate with many automorphisms (a uniform factorization
+// of K_12 has 660 among 7920 two-row relabellings) nearly every chunk of 64
+// relabellings holds one, and that lane compares every round while the wave waits.  So
+// a first pass decides each relabelling on its first compared round only
+// (smaller or larger); those equal so far are queued in index order in an LDS
+// ring and compared in full 64 at a time.  Equal results come only from the
+// ring, in index order, so the first smaller relabelling and the
+// automorphisms before it are exactly those of the plain scan: a smaller one
+// in the first pass ends the scan after the queued relabellings before it,
+// and a smaller one in the ring ends it at once (everything queued after it
+// has a larger index).
+
+constexpr uint32_t KF_DEFER_SLOTS = 128;   // ring of deferred relabellings (two chunks)
+constexpr uint32_t KF_FIRST_ROUNDS = 1;    // rounds compared before a relabelling is deferred
+constexpr uint32_t KF_KNOWN_EQUAL = 0x80000000u;
+
+struct KfRing {
+    uint32_t r[KF_DEFER_SLOTS];     // relabelling index
+    uint32_t aux[KF_DEFER_SLOTS];   // the kernel's decode of it (< 2^31); KF_KNOWN_EQUAL: equal in full
+};
+
+struct KfDeferScan {   // wave-uniform state
+    uint32_t head = 0, tail = 0, witness = KF_NONE;
+    uint64_t auts = 0;
+
+    // Full comparisons of the queued relabellings while 64 are waiting
+    // (all: until none is).  full(aux) -> -1 / 0 / 1.
+    template <typename Full> __device__ __forceinline__ void drain(KfRing &q, uint32_t l, bool all, Full full) {
+        __syncthreads();
+        while (tail - head >= 64u || (all && tail != head)) {
+            const uint32_t cnt = tail - head < 64u ? tail - head : 64u;
+            const uint32_t slot = (head + l) % KF_DEFER_SLOTS;
+            int c = 1;
+            if (l < cnt) {
+                const uint32_t a = q.aux[slot];
+                c = (a & KF_KNOWN_EQUAL) ? 0 : full(a);
+            }
+            const uint64_t smaller = __builtin_amdgcn_ballot_w64(c < 0);
+            const uint64_t equal = __builtin_amdgcn_ballot_w64(l < cnt && c == 0);
+            if (smaller) {
+                const uint32_t first = __builtin_ctzll(smaller);
+                auts += __popcll(equal & ((1ull << first) - 1ull));
+                witness = kf_uni(q.r[(head + first) % KF_DEFER_SLOTS]);
+                return;
+            }
+            auts += __popcll(equal);
+            head += cnt;
+        }
+    }
+
+    // One chunk of the first pass: lane l holds relabelling base + l (live
+    // below the end), its first-pass result icmp (-1, 0, 1 or KF_OPEN) and
+    // decode aux.  Returns true when the scan has ended (witness set).
+    template <typename Full>
+    __device__ __forceinline__ bool chunk(KfRing &q, uint32_t l, uint32_t base, bool live, int icmp, uint32_t aux,
+                                          Full full) {
+        const uint64_t smaller = __builtin_amdgcn_ballot_w64(icmp < 0);
+        uint64_t defer = __builtin_amdgcn_ballot_w64(live && (icmp == 0 || icmp == KF_OPEN));
+        const uint32_t first = smaller ? (uint32_t)__builtin_ctzll(smaller) : 64u;
+        if (smaller) defer &= (1ull << first) - 1ull;
+        if ((defer >> l) & 1u) {
+            const uint32_t slot = (tail + __popcll(defer & ((1ull << l) - 1ull))) % KF_DEFER_SLOTS;
+            q.r[slot] = base + l;
+            q.aux[slot] = (icmp == 0 ? KF_KNOWN_EQUAL : 0u) | aux;
+        }
+        tail += __popcll(defer);
+        if (smaller) {
+            drain(q, l, true, full);
+            if (witness == KF_NONE) witness = base + first;
+            return true;
+        }
+        if (tail - head >= 64u) drain(q, l, false, full);
+        return witness != KF_NONE;
+    }
+
+    // After the last chunk: the queued rest.
+    template <typename Full> __device__ __forceinline__ void finish(KfRing &q, uint32_t l, Full full) {
+        if (witness == KF_NONE) drain(q, l, true, full);
+    }
+};
+
 __device__ __forceinline__ int kf2_compare(const KfLds &s, const KfUni &u, uint32_t r, uint32_t z0lo,
                                            uint32_t z0hi, uint32_t z1lo, uint32_t z1hi) {
     const uint32_t n = u.n, nr = u.nr;
@@ -719,6 +802,7 @@ __global__ __launch_bounds__(64) void kf2_check_kernel(uint32_t ncand, uint32_t 
         }
         const uint32_t z0lo = kf_uni(s.plo[0]), z0hi = kf_uni(s.phi[0]);
         const uint32_t z1lo = nr > 1 ? kf_uni(s.plo[1]) : 0u, z1hi = nr > 1 ? kf_uni(s.phi[1]) : 0u;
+        // (Few automorphisms here: the deferral of the U1F kernel does not pay.)
         const uint32_t total = nr * (nr - 1) * n;
         uint64_t auts = 0;
         uint32_t witness = KF_NONE;
@@ -782,16 +866,12 @@ struct KfU1f {
     uint32_t plo01, phi01;            // rounds 0, 1: players in path order
 };
 
-constexpr uint32_t KF_DEFER_SLOTS = 128;   // ring of deferred relabellings (two chunks)
-constexpr uint32_t KF_U1F_FIRST_ROUNDS = 1;  // rounds compared before a relabelling is deferred
-
 struct KfU1fLds {
     uint64_t path[KF_MAX_ROUNDS * (KF_MAX_ROUNDS - 1)];   // ordered pair: players of its cycles in path order
     uint32_t rr[KF_MAX_ROUNDS * (KF_MAX_ROUNDS - 1)];     // ordered pair: r0 | r1 << 4
     uint32_t perm[24];                                     // lexicographic orders of the permuting group
     uint32_t bad;                                          // least unordered pair of another cycle type
-    uint32_t dr[KF_DEFER_SLOTS];                           // deferred: relabelling index
-    uint32_t dpt[KF_DEFER_SLOTS];                          // deferred: po << 16 | t, bit 31 = known equal
+    KfRing q;                                              // deferred relabellings: aux = po << 16 | t
 };
 
 // ceil(2^32 / d) for 2 <= d < 2^13: __umulhi(x, kf_magic(d)) = x / d for x < 2^19
@@ -974,62 +1054,19 @@ __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t str
             total = nr * (nr - 1) * T;
             w.cut = bad == 0xffffffffu ? total : 2u * bad * T;
         }
-        uint64_t auts = 0;
-        uint32_t witness = KF_NONE;
-        // Deferred relabellings, in index order: [head, tail) of the ring.
-        // Full comparisons of up to 64 of them at a time; the first smaller
-        // one ends the scan, and the equal ones before it are automorphisms.
-        uint32_t head = 0, tail = 0;
-        auto drain = [&](bool all) {
-            __syncthreads();
-            while (tail - head >= 64u || (all && tail != head)) {
-                const uint32_t cnt = tail - head < 64u ? tail - head : 64u;
-                const uint32_t slot = (head + l) % KF_DEFER_SLOTS;
-                int c = 1;
-                if (l < cnt) {
-                    const uint32_t pt = x.dpt[slot];
-                    c = (pt >> 31) ? 0 : kf_u1f_compare(s, x, u, w, (pt >> 16) & 0x7fffu, pt & 0xffffu, 0xffffffffu);
-                }
-                const uint64_t smaller = __builtin_amdgcn_ballot_w64(c < 0);
-                const uint64_t equal = __builtin_amdgcn_ballot_w64(l < cnt && c == 0);
-                if (smaller) {
-                    const uint32_t first = __builtin_ctzll(smaller);
-                    auts += __popcll(equal & ((1ull << first) - 1ull));
-                    witness = kf_uni(x.dr[(head + first) % KF_DEFER_SLOTS]);
-                    return;
-                }
-                auts += __popcll(equal);
-                head += cnt;
-            }
+        const auto full = [&](uint32_t a) {
+            return kf_u1f_compare(s, x, u, w, a >> 16, a & 0xffffu, 0xffffffffu);
         };
+        KfDeferScan sc;
         // lane l starts at relabelling l: ordered pair l / T, position l % T
         const uint32_t mT = w.T > 1 ? kf_magic(w.T) : 0u;
         uint32_t po = w.T > 1 ? __umulhi(l, mT) : l;
         uint32_t t = l - po * w.T;
         const uint32_t sq = w.T > 1 ? __umulhi(64u, mT) : 64u, sr = 64u - sq * w.T;
         for (uint32_t base = 0; base < w.cut; base += 64) {
-            // first pass: decided on the first compared round, or deferred
             const uint32_t r = base + l;
-            const int icmp = r < w.cut ? kf_u1f_compare(s, x, u, w, po, t, KF_U1F_FIRST_ROUNDS) : 1;
-            const uint64_t smaller = __builtin_amdgcn_ballot_w64(icmp < 0);
-            uint64_t defer = __builtin_amdgcn_ballot_w64(r < w.cut && icmp >= 0 && icmp != 1);
-            const uint32_t first = smaller ? (uint32_t)__builtin_ctzll(smaller) : 64u;
-            if (smaller) defer &= (1ull << first) - 1ull;
-            if ((defer >> l) & 1u) {
-                const uint32_t slot = (tail + __popcll(defer & ((1ull << l) - 1ull))) % KF_DEFER_SLOTS;
-                x.dr[slot] = r;
-                x.dpt[slot] = (icmp == 0 ? 0x80000000u : 0u) | (po << 16) | t;
-            }
-            tail += __popcll(defer);
-            if (smaller) {
-                drain(true);
-                if (witness == KF_NONE) witness = base + first;
-                break;
-            }
-            if (tail - head >= 64u) {
-                drain(false);
-                if (witness != KF_NONE) break;
-            }
+            const int icmp = r < w.cut ? kf_u1f_compare(s, x, u, w, po, t, KF_FIRST_ROUNDS) : 1;
+            if (sc.chunk(x.q, l, base, r < w.cut, icmp, (po << 16) | t, full)) break;
             t += sr;
             po += sq;
             if (t >= w.T) {
@@ -1037,7 +1074,9 @@ __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t str
                 po++;
             }
         }
-        if (witness == KF_NONE) drain(true);
+        sc.finish(x.q, l, full);
+        const uint32_t witness = sc.witness;
+        const uint64_t auts = sc.auts;
         if (l == 0) {
             if (witness == KF_NONE && w.cut < total) {   // a pair of another cycle type
                 o.verdict[ci] = ISE_VERDICT_NONCANONICAL;
