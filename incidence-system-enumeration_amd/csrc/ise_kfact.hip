@@ -617,24 +617,6 @@ __global__ __launch_bounds__(64) void kf_combine_kernel(const uint32_t *__restri
     }
 }
 
-// ---- two-row canonizer of perfect 1-factorizations ------------------------
-//
-// alldata::cnvCheck2P1F (k-Factorization/TripleSys/Source/cnvSupport.cpp:3-95)
-// with create2P1FTr (p1fSupport.cpp:3-38) and kmProcessMatrix2p1f
-// (kmSupport.cpp:519-598): in a perfect 1-factorization every pair of rounds
-// is one Hamiltonian cycle, so a relabelling that could give a matrix no
-// larger than the given one must send some ordered pair of rounds (r0, r1)
-// onto rounds 0, 1; walking the two cycles side by side from a start player k
-// and from player 0 fixes it.  Relabelling index r = ((pair * 2 + swap) * n +
-// k) with pair = last * (last - 1) / 2 + ind over last = 1.., ind < last, and
-// (r0, r1) = (ind, last) or, swapped, (last, ind).  Rounds r0, r1 become rounds
-// 0, 1 and are not compared; the relabelled round with partner-of-0 label v
-// (v = 3..nr) is compared with round v - 1 as given, in order of v, a missing
-// v making the relabelling not smaller.  Fewer than 3 rounds: every
-// relabelling is equal.  Rounds must be in normal form and every pair of them
-// one Hamiltonian cycle; otherwise the record is invalid.
-
-// Relabelling r against the matrix: -1 smaller, 0 equal, 1 not smaller.
 // ---- deferred full comparisons (the U1F canonizer) -------------------------
 //
 // In a canonical candidate with many automorphisms (a uniform factorization
@@ -718,6 +700,24 @@ struct KfDeferScan {   // wave-uniform state
     }
 };
 
+// ---- two-row canonizer of perfect 1-factorizations ------------------------
+//
+// alldata::cnvCheck2P1F (k-Factorization/TripleSys/Source/cnvSupport.cpp:3-95)
+// with create2P1FTr (p1fSupport.cpp:3-38) and kmProcessMatrix2p1f
+// (kmSupport.cpp:519-598): in a perfect 1-factorization every pair of rounds
+// is one Hamiltonian cycle, so a relabelling that could give a matrix no
+// larger than the given one must send some ordered pair of rounds (r0, r1)
+// onto rounds 0, 1; walking the two cycles side by side from a start player k
+// and from player 0 fixes it.  Relabelling index r = ((pair * 2 + swap) * n +
+// k) with pair = last * (last - 1) / 2 + ind over last = 1.., ind < last, and
+// (r0, r1) = (ind, last) or, swapped, (last, ind).  Rounds r0, r1 become rounds
+// 0, 1 and are not compared; the relabelled round with partner-of-0 label v
+// (v = 3..nr) is compared with round v - 1 as given, in order of v, a missing
+// v making the relabelling not smaller.  Fewer than 3 rounds: every
+// relabelling is equal.  Rounds must be in normal form and every pair of them
+// one Hamiltonian cycle; otherwise the record is invalid.
+
+// Relabelling r against the matrix: -1 smaller, 0 equal, 1 not smaller.
 __device__ __forceinline__ int kf2_compare(const KfLds &s, const KfUni &u, uint32_t r, uint32_t z0lo,
                                            uint32_t z0hi, uint32_t z1lo, uint32_t z1hi) {
     const uint32_t n = u.n, nr = u.nr;
