@@ -37,8 +37,8 @@ ERR_STATE = -5
 DEFAULT_DEFER_PERMS = 0
 DEFAULT_DEFER_PERMS2 = 0
 DEFAULT_LEARNED_GATE = 16
-DEFAULT_KF_FIRST_CHUNKS = 8   # ISE_DEFAULT_KF_FIRST_CHUNKS
-DEFAULT_KF_MAX_RANGES = 256   # ISE_DEFAULT_KF_MAX_RANGES
+DEFAULT_KF_FIRST_CHUNKS = 4   # ISE_DEFAULT_KF_FIRST_CHUNKS
+DEFAULT_KF_MAX_RANGES = 1024   # ISE_DEFAULT_KF_MAX_RANGES
 
 ORDER_PLAIN = 0     # the reference's search alone
 ORDER_HUNT = 1      # budgeted witness hunt in increasing row order, then the plain search

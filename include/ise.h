@@ -440,8 +440,8 @@ int ise_kf_relabel_u1f(const uint64_t *src, const uint32_t *src_meta, uint32_t n
  * waves and combined in plain order (first witness, automorphisms before it).  Results
  * are identical to first_chunks = 0, one wave per candidate to its decision.
  * No counterpart in the reference, whose test is one sequential loop. */
-#define ISE_DEFAULT_KF_FIRST_CHUNKS 8u
-#define ISE_DEFAULT_KF_MAX_RANGES 256u
+#define ISE_DEFAULT_KF_FIRST_CHUNKS 4u
+#define ISE_DEFAULT_KF_MAX_RANGES 1024u
 int ise_kf_set_split(ise_ctx *ctx, uint32_t first_chunks, uint32_t max_ranges);
 
 /* Seeded random partial 1-factorizations of K_players with nrounds uniform in
