@@ -33,9 +33,13 @@ struct KfOut {
 // Split launch: after the first pass the relabellings of the candidates still
 // undecided are taken in windows [W_p, W_p+1), W_0 = the first-pass limit and
 // W_p+1 = KF_WINDOW_GROWTH * W_p, each window cut into ranges over many
-// waves; a candidate decided in a window leaves the list.
+// waves; a candidate decided in a window leaves the list.  The window that
+// starts at KF_LAST_WINDOW or later runs to the end of the index space, and
+// so does window KF_MAX_WINDOWS - 1 (from a limit of 64 the window from
+// 2^20 is window 4).
 constexpr uint32_t KF_WINDOW_GROWTH = 16;
-constexpr uint32_t KF_MAX_WINDOWS = 6;   // from a limit of 64: 64 * 16^6 > 2^28 > every index space
+constexpr uint32_t KF_MAX_WINDOWS = 6;
+constexpr uint64_t KF_LAST_WINDOW = 1ull << 20;
 
 // Queue words, each on its own 128-byte line: head[0] the candidate queue of
 // a one-pass launch, head[KF_HEAD_WORD * (1 + p)] the length of the list of

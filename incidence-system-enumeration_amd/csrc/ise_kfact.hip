@@ -1114,15 +1114,20 @@ hipError_t kf_launch(uint32_t blocks, uint32_t range_blocks, uint32_t ncand, uin
     }
     hipLaunchKernelGGL(kf_check_kernel, dim3(blocks), dim3(64), 0, stream, ncand, stride, rounds, meta, o,
                        head, qp, limit, sp, limit ? (const uint32_t *)sp.list[1] : nullptr);
-    // windows up to the largest index space (K_16, 15 rounds: 40320 * 256 * 15 < 2^28)
+    // Windows [wlo, 16 wlo) from the first-pass limit; the one that starts at
+    // KF_LAST_WINDOW or later (or is the last the head words allow) runs to
+    // the end of every candidate's index space, so every listed candidate is
+    // decided, and no launch is spent on windows past the sizes that need
+    // them (K_12: < 2^20 relabellings; K_16, 15 rounds: 40320 * 256 * 15).
     uint64_t wlo = limit;
-    for (uint32_t p = 0; limit && p < KF_MAX_WINDOWS && wlo < (1ull << 28); p++) {
-        const uint64_t whi = wlo * KF_WINDOW_GROWTH;
-        const uint32_t w1 = whi > 0xffffffffull ? 0xffffffffu : (uint32_t)whi;
+    for (uint32_t p = 0; limit && p < KF_MAX_WINDOWS; p++) {
+        const bool last = wlo >= KF_LAST_WINDOW || p + 1 == KF_MAX_WINDOWS;
+        const uint64_t whi = last ? 0xffffffffull : wlo * KF_WINDOW_GROWTH;
         hipLaunchKernelGGL(kf_range_kernel, dim3(range_blocks), dim3(64), 0, stream, stride, rounds, meta,
-                           (const uint32_t *)head, qp, sp, p, (uint32_t)wlo, w1);
+                           (const uint32_t *)head, qp, sp, p, (uint32_t)wlo, (uint32_t)whi);
         hipLaunchKernelGGL(kf_combine_kernel, dim3(1024), dim3(64), 0, stream, meta, head, qp, sp, o, p,
-                           (uint32_t)wlo, w1);
+                           (uint32_t)wlo, (uint32_t)whi);
+        if (last) break;
         wlo = whi;
     }
     return hipGetLastError();
