@@ -140,6 +140,8 @@ def test_split_launch_same_results(gpu_ctx, oracle, first, ranges):
     with one range per candidate, a few, and more ranges than chunks."""
     print(f"case kfact split first={first} ranges={ranges}", flush=True)
     rows, meta, o = split_case(oracle)
+    # candidates that reach the last window, which runs from 2^20 to the end
+    assert (o["n_tried"] > (1 << 20)).sum() >= 2
     gpu_ctx.kf_set_split(first, ranges)
     try:
         g = gpu_ctx.kf_check(rows, meta)
