@@ -372,6 +372,7 @@ __device__ __forceinline__ KfUni kf_stage_q(KfLds &s, uint64_t row, uint32_t nr,
                                             uint32_t q, bool ok) {
     const uint32_t rlo = (uint32_t)row, rhi = (uint32_t)(row >> 32);
     bool normal = true;
+    uint32_t g0 = 0;   // this lane's round: partner of player 0
     if (ok && gl < nr) {
         uint32_t plo = 0, phi = 0, prev = 0;
         for (uint32_t p = 0; p < n; p += 2) {
@@ -386,8 +387,18 @@ __device__ __forceinline__ KfUni kf_stage_q(KfLds &s, uint64_t row, uint32_t nr,
         s.phi[gl] = phi;
         s.rlo[gl] = rlo;
         s.rhi[gl] = rhi;
+        g0 = plo & 0xfu;
     }
+    // the group's table of g0 by round (an OR over its 16 lanes), and whether
+    // the rounds are in order of it
+    uint32_t glo = gl < 8 ? g0 << (4 * gl) : 0u, ghi = gl < 8 ? 0u : g0 << (4 * (gl - 8));
+    for (int d = 1; d < 16; d <<= 1) {
+        glo |= (uint32_t)__shfl_xor((int)glo, d, 16);
+        ghi |= (uint32_t)__shfl_xor((int)ghi, d, 16);
+    }
+    const uint32_t gprev = (uint32_t)__shfl_up((int)g0, 1, 16);
     const uint32_t sh = 16u * q;
+    const bool sorted = ((__builtin_amdgcn_ballot_w64(gl > 0 && gl < nr && g0 <= gprev) >> sh) & 0xffffull) == 0;
     const uint32_t fast = ((__builtin_amdgcn_ballot_w64(ok && gl < nr && !normal) >> sh) & 0xffffull) == 0;
     const uint64_t id = 0xfedcba9876543210ull & (n >= 16 ? ~0ull : ((1ull << (4 * n)) - 1ull));
     const uint32_t skip0 = (uint32_t)((__builtin_amdgcn_ballot_w64(ok && gl == 0 && row == id) >> sh) & 1ull);
@@ -398,14 +409,6 @@ __device__ __forceinline__ KfUni kf_stage_q(KfLds &s, uint64_t row, uint32_t nr,
             uint32_t mlo = 0, mhi = 0;
             for (uint32_t k = 0; k < nr; k++) tset(mlo, mhi, tget(s.plo[k], s.phi[k], gl), 15u - k);
             s.meet[gl] = ~(((uint64_t)mhi << 32) | mlo);
-        }
-        uint32_t glo = 0, ghi = 0, prev = 0;
-        bool sorted = true;
-        for (uint32_t k = 0; k < nr; k++) {
-            const uint32_t g = s.plo[k] & 0xfu;
-            tset(glo, ghi, k, g);
-            sorted = sorted && (k == 0 || g > prev);
-            prev = g;
         }
         u.glo = glo;
         u.ghi = ghi;
