@@ -432,15 +432,24 @@ __global__ __launch_bounds__(64) void kf_quad_kernel(uint32_t ncand, uint32_t st
     const uint32_t l = threadIdx.x, q = l >> 4, gl = l & 15u;
     const uint32_t rl = stride < KF_MAX_ROUNDS ? stride : KF_MAX_ROUNDS;
     KfLds &s = s4[q];
+    // the next candidate's meta and round words are loaded while this one
+    // runs (the words lie in the batch whatever the meta says)
+    uint32_t mc = blockIdx.x * 4u + q < ncand ? meta[blockIdx.x * 4u + q] : 0u;
+    uint64_t rc = blockIdx.x * 4u + q < ncand && gl < rl ? rounds[(uint64_t)(blockIdx.x * 4u + q) * stride + gl] : 0ull;
     for (uint32_t base = blockIdx.x * 4u; base < ncand; base += gridDim.x * 4u) {
         const uint32_t ci = base + q;
+        const uint32_t cn = base + gridDim.x * 4u + q;
+        const uint32_t mn = cn < ncand ? meta[cn] : 0u;
+        const uint64_t rn = cn < ncand && gl < rl ? rounds[(uint64_t)cn * stride + gl] : 0ull;
         const bool active = ci < ncand;
-        const uint32_t m = active ? meta[ci] : 0u;
+        const uint32_t m = mc;
         const bool ok = active && kf_meta_ok(m, stride);
         if (active && !ok && gl == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
         const uint32_t nr = ok ? (m & 0xff) : 0u, n = ok ? ((m >> 8) & 0xff) : 0u, np = n / 2;
         const uint32_t total = ok ? kf_total(qp, np, nr) : 0u;
-        const uint64_t row = (ok && gl < rl) ? rounds[(uint64_t)ci * stride + gl] : 0ull;
+        const uint64_t row = ok ? rc : 0ull;
+        mc = mn;
+        rc = rn;
         const uint64_t A = gl < total ? qp.inv[qp.offset[np] + (gl / nr >> np)] : 0ull;
         const KfUni u = kf_stage_q(s, row, nr, n, gl, q, ok);
         int icmp = 1;
