@@ -588,6 +588,8 @@ __device__ __forceinline__ uint64_t kf_wave_sum(uint64_t x) {
     return x;
 }
 
+constexpr uint32_t KF_COMBINE_PER_LANE = 8;
+
 // Window p: per listed candidate (one wave each), the ranges in index order.
 __global__ __launch_bounds__(64) void kf_combine_kernel(const uint32_t *__restrict__ meta, uint32_t *head,
                                                         KfTables qp, KfSplit sp, KfOut o, uint32_t p,
@@ -600,15 +602,32 @@ __global__ __launch_bounds__(64) void kf_combine_kernel(const uint32_t *__restri
     for (uint32_t j = blockIdx.x; j < listed; j += gridDim.x) {
         uint64_t auts = 0;
         uint32_t witness = KF_NONE;
-        for (uint32_t g0 = 0; g0 < G; g0 += 64) {
-            const uint64_t e = g0 + l < G ? sp.ranges[(uint64_t)j * G + g0 + l] : (uint64_t)KF_NONE;
-            const uint32_t w = (uint32_t)e;
-            const uint64_t hit = __builtin_amdgcn_ballot_w64(w != KF_NONE);
+        // blocks of 64 x KF_COMBINE_PER_LANE ranges, each lane taking
+        // consecutive ones (all loads of a block issued together)
+        for (uint32_t g0 = 0; g0 < G; g0 += 64u * KF_COMBINE_PER_LANE) {
+            uint64_t e[KF_COMBINE_PER_LANE];
+#pragma unroll
+            for (uint32_t i = 0; i < KF_COMBINE_PER_LANE; i++) {
+                const uint32_t g = g0 + l * KF_COMBINE_PER_LANE + i;
+                e[i] = g < G ? sp.ranges[(uint64_t)j * G + g] : (uint64_t)KF_NONE;
+            }
+            // this lane's automorphisms up to and including its first range
+            // with an index, and that index
+            uint64_t la = 0;
+            uint32_t lw = KF_NONE;
+#pragma unroll
+            for (uint32_t i = 0; i < KF_COMBINE_PER_LANE; i++) {
+                if (lw == KF_NONE) {
+                    la += e[i] >> 32;
+                    lw = (uint32_t)e[i];
+                }
+            }
+            const uint64_t hit = __builtin_amdgcn_ballot_w64(lw != KF_NONE);
             const uint32_t upto = hit ? (uint32_t)__builtin_ctzll(hit) : 63u;
-            auts += kf_wave_sum(l <= upto ? (e >> 32) : 0ull);
+            auts += kf_wave_sum(l <= upto ? la : 0ull);
             if (hit) {
                 // a real index: see the ordering argument at the top
-                witness = (uint32_t)__builtin_amdgcn_readlane((int)w, (int)upto);
+                witness = (uint32_t)__builtin_amdgcn_readlane((int)lw, (int)upto);
                 break;
             }
         }
