@@ -923,7 +923,7 @@ __device__ __forceinline__ uint32_t kf_cycles(uint32_t alo, uint32_t ahi, uint32
             seen |= (1u << x) | (1u << y);
             len += 2;
             x = tget(blo, bhi, y);
-        } while (x != k0);
+        } while (x != k0 && len < 2 * KF_MAX_PLAYERS);   // bounded also for tables that are not matchings
         // insert after every cycle no longer than this one (stable sort by length)
         uint32_t pos = nc;
         while (pos > 0 && ((halves >> (4 * (pos - 1))) & 0xfu) > len / 2) pos--;
@@ -1001,8 +1001,24 @@ __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t str
             continue;
         }
         const uint32_t nr = m & 0xff, n = (m >> 8) & 0xff;
-        const KfUni u = kf_stage(s, l < rl ? rounds[(uint64_t)ci * stride + l] : 0ull, nr, n, l);
-        if (!u.fast) {
+        const uint64_t row = l < rl ? rounds[(uint64_t)ci * stride + l] : 0ull;
+        const KfUni u = kf_stage(s, row, nr, n, l);
+        // Every round a perfect matching of players 0..n-1 and no pair in two
+        // rounds (what the host entry points check; device records come
+        // unchecked): only then are the cycles below at most four, of at
+        // least four players each.
+        bool valid = true;
+        if (l < nr) {
+            uint32_t pm = 0;
+            for (uint32_t p = 0; p < n; p++) pm |= 1u << nib(row, p);
+            valid = pm == (1u << n) - 1u;
+        }
+        if (u.fast && l < n) {
+            uint32_t pm = 0;   // partners of player l over the rounds
+            for (uint32_t k = 0; k < nr; k++) pm |= 1u << tget(s.plo[k], s.phi[k], l);
+            valid = valid && (uint32_t)__builtin_popcount(pm) == nr && !((pm >> l) & 1u);
+        }
+        if (!u.fast || __builtin_amdgcn_ballot_w64(!valid)) {
             if (l == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
             __syncthreads();
             continue;

@@ -84,7 +84,14 @@ def test_device_path_and_invalid_records(gpu_ctx, oracle):
     mats = u1f_candidates(golden("kfact.json")["u10"], 300, 0x30, rmin=1)
     flipped = mats[0].copy()
     flipped[0, [0, 1]] = flipped[0, [1, 0]]
-    rows, meta = ise_amd.kf_pack(mats + [flipped])
+    # in normal form but no factorization: round 2 repeats round 0's pairs;
+    # a player twice in round 0 (and one missing)
+    full = np.array(golden("kfact.json")["u10"]["matrices"][0]["rounds"], np.uint8)
+    repeated = np.concatenate([full[:2], full[:1]])
+    twice = full[:3].copy()
+    twice[0] = [0, 1, 2, 3, 4, 5, 6, 9, 8, 9]
+    bad = [flipped, repeated, twice]
+    rows, meta = ise_amd.kf_pack(mats + bad)
     n = len(meta)
     d_rows = gpu_ctx.device_array(rows.shape, np.uint64, rows)
     d_meta = gpu_ctx.device_array(meta.shape, np.uint32, meta)
@@ -92,10 +99,11 @@ def test_device_path_and_invalid_records(gpu_ctx, oracle):
     gpu_ctx.kf_check_2u1f_device(n, 16, d_rows, d_meta, res)
     gpu_ctx.sync()
     g = {k: dev[k].to_host() for k in ise_amd.KF_FIELDS}
-    assert g["verdict"][-1] == 3
-    assert_same({k: v[:-1] for k, v in g.items()}, oracle_2u1f(oracle, mats), "device")
+    assert (g["verdict"][-len(bad):] == 3).all()
+    assert_same({k: v[:-len(bad)] for k, v in g.items()}, oracle_2u1f(oracle, mats), "device")
     for b in list(dev.values()) + [d_rows, d_meta]:
         b.free()
-    r, m = ise_amd.kf_pack([flipped])
-    with pytest.raises(ise_amd.ISEError):
-        gpu_ctx.kf_check(r, m, two_row_u1f=True)
+    for mat in bad:
+        r, m = ise_amd.kf_pack([mat])
+        with pytest.raises(ise_amd.ISEError):
+            gpu_ctx.kf_check(r, m, two_row_u1f=True)
