@@ -805,9 +805,17 @@ __global__ __launch_bounds__(64) void kf2_check_kernel(uint32_t ncand, uint32_t 
             continue;
         }
         const uint32_t nr = m & 0xff, n = (m >> 8) & 0xff;
-        const KfUni u = kf_stage(s, l < rl ? rounds[(uint64_t)ci * stride + l] : 0ull, nr, n, l);
-        // every pair of rounds one Hamiltonian cycle
-        bool bad = !u.fast;
+        const uint64_t row = l < rl ? rounds[(uint64_t)ci * stride + l] : 0ull;
+        const KfUni u = kf_stage(s, row, nr, n, l);
+        // every round a perfect matching of players 0..n-1 (device records
+        // come unchecked) and every pair of rounds one Hamiltonian cycle
+        bool valid = true;
+        if (l < nr) {
+            uint32_t pm = 0;
+            for (uint32_t p = 0; p < n; p++) pm |= 1u << nib(row, p);
+            valid = pm == (1u << n) - 1u;
+        }
+        bool bad = !u.fast || __builtin_amdgcn_ballot_w64(!valid) != 0;
         const uint32_t npairs = nr * (nr - 1) / 2;
         for (uint32_t b0 = 0; b0 < npairs && !bad; b0 += 64) {
             bool ham = true;

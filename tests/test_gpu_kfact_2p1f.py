@@ -70,8 +70,9 @@ def test_agrees_with_general_canonizer(gpu_ctx):
 
 def test_device_path_and_invalid_records(gpu_ctx, oracle):
     """The device entry point gives the host path's results; a factorization
-    that is not perfect, and one with a pair written larger player first, get
-    ISE_VERDICT_INVALID there and ISE_ERR_ARG on the host path."""
+    that is not perfect, one with a pair written larger player first and one
+    with a player twice in a round get ISE_VERDICT_INVALID there and
+    ISE_ERR_ARG on the host path."""
     print("case 2p1f device + invalid", flush=True)
     kf = golden("kfact.json")
     mats = p1f_candidates(kf["p14"], 500, 0xDE, rmin=1)
@@ -79,7 +80,9 @@ def test_device_path_and_invalid_records(gpu_ctx, oracle):
     assert oracle.kf_check(not_p1f)[0] == 1
     flipped = mats[0].copy()
     flipped[0, [0, 1]] = flipped[0, [1, 0]]
-    rows, meta = ise_amd.kf_pack(mats + [not_p1f, flipped])
+    twice = np.array(kf["p14"]["matrices"][0]["rounds"], np.uint8)[:3].copy()
+    twice[0] = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 13, 12, 13]   # in normal form, player 13 twice
+    rows, meta = ise_amd.kf_pack(mats + [not_p1f, flipped, twice])
     n = len(meta)
     d_rows = gpu_ctx.device_array(rows.shape, np.uint64, rows)
     d_meta = gpu_ctx.device_array(meta.shape, np.uint32, meta)
@@ -87,13 +90,13 @@ def test_device_path_and_invalid_records(gpu_ctx, oracle):
     gpu_ctx.kf_check_2p1f_device(n, 16, d_rows, d_meta, res)
     gpu_ctx.sync()
     g = {k: dev[k].to_host() for k in ise_amd.KF_FIELDS}
-    assert (g["verdict"][-2:] == 3).all()
-    assert_same({k: v[:-2] for k, v in g.items()}, oracle_2p1f(oracle, mats), "device")
+    assert (g["verdict"][-3:] == 3).all()
+    assert_same({k: v[:-3] for k, v in g.items()}, oracle_2p1f(oracle, mats), "device")
     ms = gpu_ctx.kf_time_2p1f_device(n, 16, d_rows, d_meta, res, 2)
     assert (ms > 0).all()
     for b in list(dev.values()) + [d_rows, d_meta]:
         b.free()
-    for bad in (not_p1f, flipped):
+    for bad in (not_p1f, flipped, twice):
         r, m = ise_amd.kf_pack([bad])
         with pytest.raises(ise_amd.ISEError):
             gpu_ctx.kf_check(r, m, two_row_p1f=True)
