@@ -129,6 +129,7 @@ __device__ __forceinline__ uint32_t tfind(uint32_t lo, uint32_t hi, uint32_t z) 
 struct KfUni {   // per-candidate values held in scalar registers
     uint32_t nr, n, np;
     uint32_t fast, skip0, ident;
+    uint32_t valid;                  // every round a perfect matching, no pair in two rounds
     uint32_t glo, ghi;               // entry i: partner of player 0 in round i as given
 };
 
@@ -138,9 +139,9 @@ __device__ __forceinline__ uint32_t kf_uni(uint32_t x) { return (uint32_t)__buil
 // tables; returns the candidate's uniform values.  Ends with a barrier.
 __device__ __forceinline__ KfUni kf_stage(KfLds &s, uint64_t row, uint32_t nr, uint32_t n, uint32_t l) {
     const uint32_t rlo = (uint32_t)row, rhi = (uint32_t)(row >> 32);
-    bool normal = true;
+    bool normal = true, matching = true;
     if (l < nr) {
-        uint32_t plo = 0, phi = 0, prev = 0;
+        uint32_t plo = 0, phi = 0, prev = 0, seen = 0;
         for (uint32_t p = 0; p < n; p += 2) {
             const uint32_t w = p < 8 ? rlo : rhi;
             const uint32_t x = __builtin_amdgcn_ubfe(w, p << 2, 4), y = __builtin_amdgcn_ubfe(w, (p << 2) + 4, 4);
@@ -148,7 +149,9 @@ __device__ __forceinline__ KfUni kf_stage(KfLds &s, uint64_t row, uint32_t nr, u
             tset(plo, phi, y, x);
             normal = normal && x < y && (p == 0 || x > prev);
             prev = x;
+            seen |= (1u << x) | (1u << y);
         }
+        matching = seen == (1u << n) - 1u;
         s.plo[l] = plo;
         s.phi[l] = phi;
         s.rlo[l] = rlo;
@@ -164,13 +167,21 @@ __device__ __forceinline__ KfUni kf_stage(KfLds &s, uint64_t row, uint32_t nr, u
         s.key[l] = raw;
     }
     __syncthreads();
-    KfUni u{nr, n, n / 2, fast, skip0, 0u, 0u, 0u};
-    if (fast) {
-        if (l < n) {
-            uint32_t mlo = 0, mhi = 0;   // 15 - round, so that absent pairs read 15 after inverting
-            for (uint32_t k = 0; k < nr; k++) tset(mlo, mhi, tget(s.plo[k], s.phi[k], l), 15u - k);
-            s.meet[l] = ~(((uint64_t)mhi << 32) | mlo);
+    KfUni u{nr, n, n / 2, fast, skip0, 0u, 1u, 0u, 0u};
+    // player l's partners over the rounds: nr different players, not l
+    bool partners = true;
+    if (l < n) {
+        uint32_t mlo = 0, mhi = 0, pm = 0;   // 15 - round, so that absent pairs read 15 after inverting
+        for (uint32_t k = 0; k < nr; k++) {
+            const uint32_t y = tget(s.plo[k], s.phi[k], l);
+            pm |= 1u << y;
+            tset(mlo, mhi, y, 15u - k);
         }
+        partners = (uint32_t)__builtin_popcount(pm) == nr && !((pm >> l) & 1u);
+        if (fast) s.meet[l] = ~(((uint64_t)mhi << 32) | mlo);
+    }
+    u.valid = __builtin_amdgcn_ballot_w64(!matching || !partners) == 0;
+    if (fast) {
         uint32_t glo = 0, ghi = 0;
         bool sorted = true;   // rounds in order of player 0's partner
         uint32_t prev = 0;
@@ -371,10 +382,10 @@ __device__ __forceinline__ bool kf_meta_ok(uint32_t m, uint32_t stride) {
 __device__ __forceinline__ KfUni kf_stage_q(KfLds &s, uint64_t row, uint32_t nr, uint32_t n, uint32_t gl,
                                             uint32_t q, bool ok) {
     const uint32_t rlo = (uint32_t)row, rhi = (uint32_t)(row >> 32);
-    bool normal = true;
+    bool normal = true, matching = true;
     uint32_t g0 = 0;   // this lane's round: partner of player 0
     if (ok && gl < nr) {
-        uint32_t plo = 0, phi = 0, prev = 0;
+        uint32_t plo = 0, phi = 0, prev = 0, seen = 0;
         for (uint32_t p = 0; p < n; p += 2) {
             const uint32_t w = p < 8 ? rlo : rhi;
             const uint32_t x = __builtin_amdgcn_ubfe(w, p << 2, 4), y = __builtin_amdgcn_ubfe(w, (p << 2) + 4, 4);
@@ -382,7 +393,9 @@ __device__ __forceinline__ KfUni kf_stage_q(KfLds &s, uint64_t row, uint32_t nr,
             tset(plo, phi, y, x);
             normal = normal && x < y && (p == 0 || x > prev);
             prev = x;
+            seen |= (1u << x) | (1u << y);
         }
+        matching = seen == (1u << n) - 1u;
         s.plo[gl] = plo;
         s.phi[gl] = phi;
         s.rlo[gl] = rlo;
@@ -403,13 +416,21 @@ __device__ __forceinline__ KfUni kf_stage_q(KfLds &s, uint64_t row, uint32_t nr,
     const uint64_t id = 0xfedcba9876543210ull & (n >= 16 ? ~0ull : ((1ull << (4 * n)) - 1ull));
     const uint32_t skip0 = (uint32_t)((__builtin_amdgcn_ballot_w64(ok && gl == 0 && row == id) >> sh) & 1ull);
     __syncthreads();
-    KfUni u{nr, n, n / 2, fast, skip0, 0u, 0u, 0u};
-    if (ok && fast) {
-        if (gl < n) {
-            uint32_t mlo = 0, mhi = 0;
-            for (uint32_t k = 0; k < nr; k++) tset(mlo, mhi, tget(s.plo[k], s.phi[k], gl), 15u - k);
-            s.meet[gl] = ~(((uint64_t)mhi << 32) | mlo);
+    KfUni u{nr, n, n / 2, fast, skip0, 0u, 1u, 0u, 0u};
+    // player gl's partners over the rounds: nr different players, not gl
+    bool partners = true;
+    if (ok && gl < n) {
+        uint32_t mlo = 0, mhi = 0, pm = 0;
+        for (uint32_t k = 0; k < nr; k++) {
+            const uint32_t y = tget(s.plo[k], s.phi[k], gl);
+            pm |= 1u << y;
+            tset(mlo, mhi, y, 15u - k);
         }
+        partners = (uint32_t)__builtin_popcount(pm) == nr && !((pm >> gl) & 1u);
+        if (fast) s.meet[gl] = ~(((uint64_t)mhi << 32) | mlo);
+    }
+    u.valid = ((__builtin_amdgcn_ballot_w64(!matching || !partners) >> sh) & 0xffffull) == 0;
+    if (ok && fast) {
         u.glo = glo;
         u.ghi = ghi;
         u.ident = skip0 && sorted;
@@ -453,12 +474,14 @@ __global__ __launch_bounds__(64) void kf_quad_kernel(uint32_t ncand, uint32_t st
         const uint64_t A = gl < total ? qp.inv[qp.offset[np] + (gl / nr >> np)] : 0ull;
         const KfUni u = kf_stage_q(s, row, nr, n, gl, q, ok);
         int icmp = 1;
-        if (gl < total && u.fast) icmp = (gl == 0 && u.ident) ? 0 : kf_compare_fast(s, u, A, gl);
+        if (gl < total && u.fast && u.valid) icmp = (gl == 0 && u.ident) ? 0 : kf_compare_fast(s, u, A, gl);
         const uint32_t sh = 16u * q;
         const uint32_t sm = (uint32_t)((__builtin_amdgcn_ballot_w64(icmp < 0) >> sh) & 0xffffull);
         const uint32_t eq = (uint32_t)((__builtin_amdgcn_ballot_w64(gl < total && icmp == 0) >> sh) & 0xffffull);
         if (ok && gl == 0) {
-            if (u.fast && sm) {
+            if (!u.valid) {
+                o.verdict[ci] = ISE_VERDICT_INVALID;
+            } else if (u.fast && sm) {
                 const uint32_t first = (uint32_t)__builtin_ctz(sm);
                 kf_write(o, ci, first, (uint64_t)__builtin_popcount(eq & ((1u << first) - 1u)), total);
             } else if (u.fast && total <= 16u) {
@@ -510,9 +533,11 @@ __global__ __launch_bounds__(64) void kf_check_kernel(uint32_t ncand, uint32_t s
             const uint64_t A0 = l < hi ? qp.inv[qp.offset[np] + (l / nr >> np)] : 0;
             const KfUni u = kf_stage(s, row, nr, n, l);
             uint64_t auts = 0;
-            const uint32_t witness = kf_scan(s, u, qp, 0, hi, nullptr, auts, l, A0);
+            const uint32_t witness = u.valid ? kf_scan(s, u, qp, 0, hi, nullptr, auts, l, A0) : KF_NONE;
             if (l == 0) {
-                if (witness == KF_NONE && hi < total) {
+                if (!u.valid) {
+                    o.verdict[ci] = ISE_VERDICT_INVALID;
+                } else if (witness == KF_NONE && hi < total) {
                     const uint32_t j = atomicAdd(&head[KF_HEAD_WORD], 1u);
                     sp.list[0][j] = ci;
                     sp.part[0][j] = auts;
@@ -808,14 +833,9 @@ __global__ __launch_bounds__(64) void kf2_check_kernel(uint32_t ncand, uint32_t 
         const uint64_t row = l < rl ? rounds[(uint64_t)ci * stride + l] : 0ull;
         const KfUni u = kf_stage(s, row, nr, n, l);
         // every round a perfect matching of players 0..n-1 (device records
-        // come unchecked) and every pair of rounds one Hamiltonian cycle
-        bool valid = true;
-        if (l < nr) {
-            uint32_t pm = 0;
-            for (uint32_t p = 0; p < n; p++) pm |= 1u << nib(row, p);
-            valid = pm == (1u << n) - 1u;
-        }
-        bool bad = !u.fast || __builtin_amdgcn_ballot_w64(!valid) != 0;
+        // come unchecked; staging checks it) and every pair of rounds one
+        // Hamiltonian cycle
+        bool bad = !u.fast || !u.valid;
         const uint32_t npairs = nr * (nr - 1) / 2;
         for (uint32_t b0 = 0; b0 < npairs && !bad; b0 += 64) {
             bool ham = true;
@@ -1013,20 +1033,9 @@ __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t str
         const KfUni u = kf_stage(s, row, nr, n, l);
         // Every round a perfect matching of players 0..n-1 and no pair in two
         // rounds (what the host entry points check; device records come
-        // unchecked): only then are the cycles below at most four, of at
-        // least four players each.
-        bool valid = true;
-        if (l < nr) {
-            uint32_t pm = 0;
-            for (uint32_t p = 0; p < n; p++) pm |= 1u << nib(row, p);
-            valid = pm == (1u << n) - 1u;
-        }
-        if (u.fast && l < n) {
-            uint32_t pm = 0;   // partners of player l over the rounds
-            for (uint32_t k = 0; k < nr; k++) pm |= 1u << tget(s.plo[k], s.phi[k], l);
-            valid = valid && (uint32_t)__builtin_popcount(pm) == nr && !((pm >> l) & 1u);
-        }
-        if (!u.fast || __builtin_amdgcn_ballot_w64(!valid)) {
+        // unchecked, staging checks them): only then are the cycles below at
+        // most four, of at least four players each.
+        if (!u.fast || !u.valid) {
             if (l == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
             __syncthreads();
             continue;

@@ -335,6 +335,10 @@ int ise_mlp_witness_labels(uint32_t n, uint32_t od, const uint32_t *meta, const 
  * player 0), compared round by round -- alldata::canonizator / cnvCheckKm1 /
  * kmProcessMatrix (k-Factorization/TripleSys/Source/cnv.cpp:65-351,
  * kmSupport.cpp:275-313). */
+/* Records are checked on the device too (the device entry points get no host
+ * check): a meta outside the sizes above, a round that is not a perfect
+ * matching of players 0..n-1, or a pair in two rounds gives verdict 3; the
+ * host entry points reject such records (ISE_ERR_ARG) instead. */
 typedef struct {
     uint8_t  *verdict;  /* [n] 1 canonical, 0 not, 3 invalid record            */
     uint64_t *witness;  /* [n] plain-order index of the first smaller relabelling (~0 if none) */

@@ -106,6 +106,43 @@ def test_device_timed_path(gpu_ctx, oracle):
     assert_same(g, oracle_batch(oracle, rows, meta, 12), "device")
 
 
+@pytest.mark.parametrize("first,ranges", [(ise_amd.DEFAULT_KF_FIRST_CHUNKS, ise_amd.DEFAULT_KF_MAX_RANGES), (0, 1)])
+def test_device_records_that_are_no_factorization(gpu_ctx, oracle, first, ranges):
+    """Device records are not checked on the host: a round repeated (pairs in
+    two rounds) or a player twice in a round gives ISE_VERDICT_INVALID from
+    the kernels (the host entry point rejects them), the others their
+    results."""
+    print(f"case kfact device invalid records split={first}/{ranges}", flush=True)
+    rows, meta = ise_amd.kf_random(12, 3, 8, 512, 0x1D)
+    good = np.ones(len(meta), bool)
+    rows[5, 2] = rows[5, 0]                      # round 2 = round 0
+    twice = ise_amd.kf_unpack(rows[9, :int(meta[9]) & 0xff], 12)
+    twice[0] = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 9]   # player 9 twice, 11 missing
+    packed, pmeta = ise_amd.kf_pack([twice])
+    assert pmeta[0] == meta[9]
+    rows[9] = packed[0]
+    good[[5, 9]] = False
+    n = len(meta)
+    d_rows = gpu_ctx.device_array(rows.shape, np.uint64, rows)
+    d_meta = gpu_ctx.device_array(meta.shape, np.uint32, meta)
+    d_head = gpu_ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
+    dev, s = gpu_ctx.kf_device_results(n)
+    gpu_ctx.kf_set_split(first, ranges)
+    try:
+        gpu_ctx.kf_time_batch_device(n, 16, d_rows, d_meta, s, d_head, 1)
+    finally:
+        gpu_ctx.kf_set_split()
+    g = {k: dev[k].to_host() for k in ise_amd.KF_FIELDS}
+    for b in list(dev.values()) + [d_rows, d_meta, d_head]:
+        b.free()
+    assert (g["verdict"][~good] == 3).all()
+    o = oracle_batch(oracle, rows[good], meta[good], 12)
+    assert_same({k: v[good] for k, v in g.items()}, o, "device, valid records")
+    for i in (5, 9):
+        with pytest.raises(ise_amd.ISEError):
+            gpu_ctx.kf_check(rows[i:i + 1], meta[i:i + 1])
+
+
 _SPLIT_CASE = {}
 
 
