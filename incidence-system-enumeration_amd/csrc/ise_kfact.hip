@@ -53,7 +53,7 @@ __device__ __forceinline__ uint32_t nib(uint64_t w, uint32_t i) { return (uint32
 // chunks each (a range stages its candidate once, which costs about a chunk).
 // Both the range and the combine kernel derive it from the same values, so
 // they agree.
-constexpr uint32_t KF_MIN_RANGE_CHUNKS = 4;
+constexpr uint32_t KF_MIN_RANGE_CHUNKS = 3;
 __device__ __forceinline__ uint32_t kf_ranges(uint32_t listed, uint32_t range_words, uint32_t max_ranges,
                                               uint32_t span) {
     if (listed == 0) return 1u;
