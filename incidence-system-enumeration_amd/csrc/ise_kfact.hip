@@ -773,16 +773,12 @@ struct KfDeferScan {   // wave-uniform state
 // relabelling is equal.  Rounds must be in normal form and every pair of them
 // one Hamiltonian cycle; otherwise the record is invalid.
 
-// Relabelling r against the matrix: -1 smaller, 0 equal, 1 not smaller.
-__device__ __forceinline__ int kf2_compare(const KfLds &s, const KfUni &u, uint32_t r, uint32_t z0lo,
+// Relabelling (ordered pair po of rounds r0 | r1 << 4 = rr, start player k)
+// against the matrix: -1 smaller, 0 equal, 1 not smaller.
+__device__ __forceinline__ int kf2_compare(const KfLds &s, const KfUni &u, uint32_t rr, uint32_t k, uint32_t z0lo,
                                            uint32_t z0hi, uint32_t z1lo, uint32_t z1hi) {
     const uint32_t n = u.n, nr = u.nr;
-    const uint32_t k = r % n, q = r / n;
-    const uint32_t pi = q >> 1;
-    uint32_t last = 1;
-    while ((last + 1) * last / 2 <= pi) last++;
-    const uint32_t ind = pi - last * (last - 1) / 2;
-    const uint32_t r0 = (q & 1) ? last : ind, r1 = (q & 1) ? ind : last;
+    const uint32_t r0 = rr & 0xfu, r1 = rr >> 4;
     const uint32_t alo = s.plo[r0], ahi = s.phi[r0], blo = s.plo[r1], bhi = s.phi[r1];
     // label -> player: walk round r0 / r1 from k against round 0 / 1 from player 0
     uint32_t vlo = 0, vhi = 0, v1 = k, it = 0;
@@ -820,6 +816,7 @@ __global__ __launch_bounds__(64) void kf2_check_kernel(uint32_t ncand, uint32_t 
                                                        const uint64_t *__restrict__ rounds,
                                                        const uint32_t *__restrict__ meta, KfOut o) {
     __shared__ KfLds s;
+    __shared__ uint32_t s_rr[KF_MAX_ROUNDS * (KF_MAX_ROUNDS - 1)];   // ordered pair: r0 | r1 << 4
     const uint32_t l = threadIdx.x;
     const uint32_t rl = stride < KF_MAX_ROUNDS ? stride : KF_MAX_ROUNDS;
     // at most 15 * 14 * 16 relabellings per candidate: a fixed grid stride
@@ -844,6 +841,8 @@ __global__ __launch_bounds__(64) void kf2_check_kernel(uint32_t ncand, uint32_t 
                 uint32_t b = 1;
                 while ((b + 1) * b / 2 <= pi) b++;
                 const uint32_t a = pi - b * (b - 1) / 2;
+                s_rr[2 * pi] = a | (b << 4);
+                s_rr[2 * pi + 1] = b | (a << 4);
                 const uint32_t alo = s.plo[a], ahi = s.phi[a], blo = s.plo[b], bhi = s.phi[b];
                 uint32_t v = 0, len = 0;
                 do {
@@ -862,12 +861,16 @@ __global__ __launch_bounds__(64) void kf2_check_kernel(uint32_t ncand, uint32_t 
         const uint32_t z0lo = kf_uni(s.plo[0]), z0hi = kf_uni(s.phi[0]);
         const uint32_t z1lo = nr > 1 ? kf_uni(s.plo[1]) : 0u, z1hi = nr > 1 ? kf_uni(s.phi[1]) : 0u;
         // (Few automorphisms here: the deferral of the U1F kernel does not pay.)
+        __syncthreads();
         const uint32_t total = nr * (nr - 1) * n;
         uint64_t auts = 0;
         uint32_t witness = KF_NONE;
+        // lane l starts at relabelling l: ordered pair l / n, start player l % n
+        uint32_t po = l / n, k = l - po * n;
+        const uint32_t sq = 64u / n, sr = 64u - sq * n;
         for (uint32_t base = 0; base < total; base += 64) {
             const uint32_t r = base + l;
-            const int icmp = r < total ? kf2_compare(s, u, r, z0lo, z0hi, z1lo, z1hi) : 1;
+            const int icmp = r < total ? kf2_compare(s, u, s_rr[po], k, z0lo, z0hi, z1lo, z1hi) : 1;
             const uint64_t smaller = __builtin_amdgcn_ballot_w64(icmp < 0);
             const uint64_t equal = __builtin_amdgcn_ballot_w64(r < total && icmp == 0);
             if (smaller) {
@@ -877,6 +880,12 @@ __global__ __launch_bounds__(64) void kf2_check_kernel(uint32_t ncand, uint32_t 
                 break;
             }
             auts += __popcll(equal);
+            k += sr;
+            po += sq;
+            if (k >= n) {
+                k -= n;
+                po++;
+            }
         }
         if (l == 0) kf_write(o, ci, witness, auts, total);
         __syncthreads();
