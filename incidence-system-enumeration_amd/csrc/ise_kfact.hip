@@ -136,7 +136,10 @@ struct KfUni {   // per-candidate values held in scalar registers
 __device__ __forceinline__ uint32_t kf_uni(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
 
 // Lanes < nr decode their round (`row`, loaded by the caller) into the LDS
-// tables; returns the candidate's uniform values.  Ends with a barrier.
+// tables, lanes < n build their player's meet row; returns the candidate's
+// uniform values, among them `valid`: every round a perfect matching of
+// players 0..n-1 and each player's partners over the rounds all different
+// (no pair in two rounds).  Ends with a barrier.
 __device__ __forceinline__ KfUni kf_stage(KfLds &s, uint64_t row, uint32_t nr, uint32_t n, uint32_t l) {
     const uint32_t rlo = (uint32_t)row, rhi = (uint32_t)(row >> 32);
     bool normal = true, matching = true;
