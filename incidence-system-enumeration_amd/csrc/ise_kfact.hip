@@ -377,62 +377,80 @@ __device__ __forceinline__ bool kf_meta_ok(uint32_t m, uint32_t stride) {
            nr <= KF_MAX_ROUNDS;
 }
 
-// Quarter-wave staging for kf_quad_kernel: the 16 lanes of group q stage
-// candidate q's tables into s (lane gl of the group takes round gl and player
-// gl); the per-candidate values stay per lane (uniform within the group).
-// Every lane of the wave calls it (it holds the barriers); `ok` marks groups
-// with a valid candidate.
-__device__ __forceinline__ KfUni kf_stage_q(KfLds &s, uint64_t row, uint32_t nr, uint32_t n, uint32_t gl,
+// Lanes of a first-look group: each candidate gets W lanes (64 / W candidates
+// per wave), lane gl of a group taking rounds and players gl, gl + W, ...
+constexpr uint32_t KF_LOOK_LANES = 4;
+
+// Group staging for kf_quad_kernel: the W lanes of group q stage candidate
+// q's tables into s (lane gl takes rounds and players gl, gl + W, ...; rows[j]
+// is round gl + j W as loaded by the caller); the per-candidate values stay
+// per lane (uniform within the group).  Every lane of the wave calls it (it
+// holds the barriers); `ok` marks groups with a valid candidate.
+template <uint32_t W>
+__device__ __forceinline__ KfUni kf_stage_q(KfLds &s, const uint64_t *rows, uint32_t nr, uint32_t n, uint32_t gl,
                                             uint32_t q, bool ok) {
-    const uint32_t rlo = (uint32_t)row, rhi = (uint32_t)(row >> 32);
+    constexpr uint32_t KR = 16 / W;
+    constexpr uint64_t GM = W == 64 ? ~0ull : (1ull << W) - 1ull;
+    const uint32_t sh = W * q;
     bool normal = true, matching = true;
-    uint32_t g0 = 0;   // this lane's round: partner of player 0
-    if (ok && gl < nr) {
-        uint32_t plo = 0, phi = 0, prev = 0, seen = 0;
-        for (uint32_t p = 0; p < n; p += 2) {
-            const uint32_t w = p < 8 ? rlo : rhi;
-            const uint32_t x = __builtin_amdgcn_ubfe(w, p << 2, 4), y = __builtin_amdgcn_ubfe(w, (p << 2) + 4, 4);
-            tset(plo, phi, x, y);
-            tset(plo, phi, y, x);
-            normal = normal && x < y && (p == 0 || x > prev);
-            prev = x;
-            seen |= (1u << x) | (1u << y);
+    uint32_t glo = 0, ghi = 0;   // the group's table of player 0's partner by round
+#pragma unroll
+    for (uint32_t j = 0; j < KR; j++) {
+        const uint32_t r = gl + j * W;
+        if (ok && r < nr) {
+            const uint32_t rlo = (uint32_t)rows[j], rhi = (uint32_t)(rows[j] >> 32);
+            uint32_t plo = 0, phi = 0, prev = 0, seen = 0;
+            for (uint32_t p = 0; p < n; p += 2) {
+                const uint32_t w = p < 8 ? rlo : rhi;
+                const uint32_t x = __builtin_amdgcn_ubfe(w, p << 2, 4), y = __builtin_amdgcn_ubfe(w, (p << 2) + 4, 4);
+                tset(plo, phi, x, y);
+                tset(plo, phi, y, x);
+                normal = normal && x < y && (p == 0 || x > prev);
+                prev = x;
+                seen |= (1u << x) | (1u << y);
+            }
+            matching = matching && seen == (1u << n) - 1u;
+            s.plo[r] = plo;
+            s.phi[r] = phi;
+            s.rlo[r] = rlo;
+            s.rhi[r] = rhi;
+            tset(glo, ghi, r, plo & 0xfu);
         }
-        matching = seen == (1u << n) - 1u;
-        s.plo[gl] = plo;
-        s.phi[gl] = phi;
-        s.rlo[gl] = rlo;
-        s.rhi[gl] = rhi;
-        g0 = plo & 0xfu;
     }
-    // the group's table of g0 by round (an OR over its 16 lanes), and whether
-    // the rounds are in order of it
-    uint32_t glo = gl < 8 ? g0 << (4 * gl) : 0u, ghi = gl < 8 ? 0u : g0 << (4 * (gl - 8));
-    for (int d = 1; d < 16; d <<= 1) {
-        glo |= (uint32_t)__shfl_xor((int)glo, d, 16);
-        ghi |= (uint32_t)__shfl_xor((int)ghi, d, 16);
+    for (int d = 1; d < (int)W; d <<= 1) {   // OR over the group's lanes
+        glo |= (uint32_t)__shfl_xor((int)glo, d, W);
+        ghi |= (uint32_t)__shfl_xor((int)ghi, d, W);
     }
-    const uint32_t gprev = (uint32_t)__shfl_up((int)g0, 1, 16);
-    const uint32_t sh = 16u * q;
-    const bool sorted = ((__builtin_amdgcn_ballot_w64(gl > 0 && gl < nr && g0 <= gprev) >> sh) & 0xffffull) == 0;
-    const uint32_t fast = ((__builtin_amdgcn_ballot_w64(ok && gl < nr && !normal) >> sh) & 0xffffull) == 0;
+    // rounds in order of player 0's partner
+    bool unsorted = false;
+#pragma unroll
+    for (uint32_t j = 0; j < KR; j++) {
+        const uint32_t r = gl + j * W;
+        if (r > 0 && r < nr) unsorted = unsorted || tget(glo, ghi, r) <= tget(glo, ghi, r - 1);
+    }
+    const bool sorted = ((__builtin_amdgcn_ballot_w64(ok && unsorted) >> sh) & GM) == 0;
+    const uint32_t fast = ((__builtin_amdgcn_ballot_w64(ok && !normal) >> sh) & GM) == 0;
     const uint64_t id = 0xfedcba9876543210ull & (n >= 16 ? ~0ull : ((1ull << (4 * n)) - 1ull));
-    const uint32_t skip0 = (uint32_t)((__builtin_amdgcn_ballot_w64(ok && gl == 0 && row == id) >> sh) & 1ull);
+    const uint32_t skip0 = (uint32_t)((__builtin_amdgcn_ballot_w64(ok && gl == 0 && rows[0] == id) >> sh) & 1ull);
     __syncthreads();
     KfUni u{nr, n, n / 2, fast, skip0, 0u, 1u, 0u, 0u};
-    // player gl's partners over the rounds: nr different players, not gl
+    // each player's partners over the rounds: nr different players, not itself
     bool partners = true;
-    if (ok && gl < n) {
-        uint32_t mlo = 0, mhi = 0, pm = 0;
-        for (uint32_t k = 0; k < nr; k++) {
-            const uint32_t y = tget(s.plo[k], s.phi[k], gl);
-            pm |= 1u << y;
-            tset(mlo, mhi, y, 15u - k);
+#pragma unroll
+    for (uint32_t j = 0; j < KR; j++) {
+        const uint32_t x = gl + j * W;
+        if (ok && x < n) {
+            uint32_t mlo = 0, mhi = 0, pm = 0;
+            for (uint32_t k = 0; k < nr; k++) {
+                const uint32_t y = tget(s.plo[k], s.phi[k], x);
+                pm |= 1u << y;
+                tset(mlo, mhi, y, 15u - k);
+            }
+            partners = partners && (uint32_t)__builtin_popcount(pm) == nr && !((pm >> x) & 1u);
+            if (fast) s.meet[x] = ~(((uint64_t)mhi << 32) | mlo);
         }
-        partners = (uint32_t)__builtin_popcount(pm) == nr && !((pm >> gl) & 1u);
-        if (fast) s.meet[gl] = ~(((uint64_t)mhi << 32) | mlo);
     }
-    u.valid = ((__builtin_amdgcn_ballot_w64(!matching || !partners) >> sh) & 0xffffull) == 0;
+    u.valid = ((__builtin_amdgcn_ballot_w64(!matching || !partners) >> sh) & GM) == 0;
     if (ok && fast) {
         u.glo = glo;
         u.ghi = ghi;
@@ -442,53 +460,67 @@ __device__ __forceinline__ KfUni kf_stage_q(KfLds &s, uint64_t row, uint32_t nr,
     return u;
 }
 
-// First look, four candidates per wave: the 16 lanes of a group compare the
-// first 16 relabellings of their candidate (one in a hundred of the random K_12
+// First look, 64 / W candidates per wave: the W lanes of a group compare the
+// first W relabellings of their candidate (one in a hundred of the random K_12
 // partials needs more than two).  A candidate decided there (a smaller
 // relabelling, or all of them tried) gets its results; the others, and those
 // whose rounds are not in normal form, are listed for the first pass, which
 // runs them from relabelling 0 again.
+template <uint32_t W>
 __global__ __launch_bounds__(64) void kf_quad_kernel(uint32_t ncand, uint32_t stride,
                                                      const uint64_t *__restrict__ rounds,
                                                      const uint32_t *__restrict__ meta, KfOut o, uint32_t *head,
                                                      KfTables qp, KfSplit sp) {
-    __shared__ KfLds s4[4];
-    const uint32_t l = threadIdx.x, q = l >> 4, gl = l & 15u;
+    constexpr uint32_t G = 64 / W, KR = 16 / W;
+    constexpr uint64_t GM = W == 64 ? ~0ull : (1ull << W) - 1ull;
+    __shared__ KfLds sg[G];
+    const uint32_t l = threadIdx.x, q = l / W, gl = l % W;
     const uint32_t rl = stride < KF_MAX_ROUNDS ? stride : KF_MAX_ROUNDS;
-    KfLds &s = s4[q];
+    KfLds &s = sg[q];
     // the next candidate's meta and round words are loaded while this one
     // runs (the words lie in the batch whatever the meta says)
-    uint32_t mc = blockIdx.x * 4u + q < ncand ? meta[blockIdx.x * 4u + q] : 0u;
-    uint64_t rc = blockIdx.x * 4u + q < ncand && gl < rl ? rounds[(uint64_t)(blockIdx.x * 4u + q) * stride + gl] : 0ull;
-    for (uint32_t base = blockIdx.x * 4u; base < ncand; base += gridDim.x * 4u) {
+    const uint32_t c0 = blockIdx.x * G + q;
+    uint32_t mc = c0 < ncand ? meta[c0] : 0u;
+    uint64_t rc[KR];
+#pragma unroll
+    for (uint32_t j = 0; j < KR; j++)
+        rc[j] = c0 < ncand && gl + j * W < rl ? rounds[(uint64_t)c0 * stride + gl + j * W] : 0ull;
+    for (uint32_t base = blockIdx.x * G; base < ncand; base += gridDim.x * G) {
         const uint32_t ci = base + q;
-        const uint32_t cn = base + gridDim.x * 4u + q;
+        const uint32_t cn = base + gridDim.x * G + q;
         const uint32_t mn = cn < ncand ? meta[cn] : 0u;
-        const uint64_t rn = cn < ncand && gl < rl ? rounds[(uint64_t)cn * stride + gl] : 0ull;
+        uint64_t rn[KR];
+#pragma unroll
+        for (uint32_t j = 0; j < KR; j++)
+            rn[j] = cn < ncand && gl + j * W < rl ? rounds[(uint64_t)cn * stride + gl + j * W] : 0ull;
         const bool active = ci < ncand;
         const uint32_t m = mc;
         const bool ok = active && kf_meta_ok(m, stride);
         if (active && !ok && gl == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
         const uint32_t nr = ok ? (m & 0xff) : 0u, n = ok ? ((m >> 8) & 0xff) : 0u, np = n / 2;
         const uint32_t total = ok ? kf_total(qp, np, nr) : 0u;
-        const uint64_t row = ok ? rc : 0ull;
+        uint64_t row[KR];
+#pragma unroll
+        for (uint32_t j = 0; j < KR; j++) {
+            row[j] = ok ? rc[j] : 0ull;
+            rc[j] = rn[j];
+        }
         mc = mn;
-        rc = rn;
         const uint64_t A = gl < total ? qp.inv[qp.offset[np] + (gl / nr >> np)] : 0ull;
-        const KfUni u = kf_stage_q(s, row, nr, n, gl, q, ok);
+        const KfUni u = kf_stage_q<W>(s, row, nr, n, gl, q, ok);
         int icmp = 1;
         if (gl < total && u.fast && u.valid) icmp = (gl == 0 && u.ident) ? 0 : kf_compare_fast(s, u, A, gl);
-        const uint32_t sh = 16u * q;
-        const uint32_t sm = (uint32_t)((__builtin_amdgcn_ballot_w64(icmp < 0) >> sh) & 0xffffull);
-        const uint32_t eq = (uint32_t)((__builtin_amdgcn_ballot_w64(gl < total && icmp == 0) >> sh) & 0xffffull);
+        const uint32_t sh = W * q;
+        const uint64_t sm = (__builtin_amdgcn_ballot_w64(icmp < 0) >> sh) & GM;
+        const uint64_t eq = (__builtin_amdgcn_ballot_w64(gl < total && icmp == 0) >> sh) & GM;
         if (ok && gl == 0) {
             if (!u.valid) {
                 o.verdict[ci] = ISE_VERDICT_INVALID;
             } else if (u.fast && sm) {
-                const uint32_t first = (uint32_t)__builtin_ctz(sm);
-                kf_write(o, ci, first, (uint64_t)__builtin_popcount(eq & ((1u << first) - 1u)), total);
-            } else if (u.fast && total <= 16u) {
-                kf_write(o, ci, KF_NONE, (uint64_t)__builtin_popcount(eq), total);
+                const uint32_t first = (uint32_t)__builtin_ctzll(sm);
+                kf_write(o, ci, first, (uint64_t)__popcll(eq & ((1ull << first) - 1ull)), total);
+            } else if (u.fast && total <= W) {
+                kf_write(o, ci, KF_NONE, (uint64_t)__popcll(eq), total);
             } else {
                 const uint32_t j = atomicAdd(&head[KF_QUAD_WORD], 1u);
                 sp.list[1][j] = ci;
@@ -1184,9 +1216,11 @@ hipError_t kf_launch(uint32_t blocks, uint32_t range_blocks, uint32_t ncand, uin
                      uint32_t limit, const KfSplit &sp, hipStream_t stream) {
     hipLaunchKernelGGL(kf_queue_reset_kernel, dim3(1), dim3(64), 0, stream, head);
     if (limit) {
-        const uint32_t qblocks = (ncand + 3u) / 4u < range_blocks ? (ncand + 3u) / 4u : range_blocks;
-        hipLaunchKernelGGL(kf_quad_kernel, dim3(qblocks ? qblocks : 1u), dim3(64), 0, stream, ncand, stride, rounds,
-                           meta, o, head, qp, sp);
+        constexpr uint32_t G = 64 / KF_LOOK_LANES;
+        const uint32_t need = (ncand + G - 1u) / G;
+        const uint32_t qblocks = need < range_blocks ? need : range_blocks;
+        hipLaunchKernelGGL(kf_quad_kernel<KF_LOOK_LANES>, dim3(qblocks ? qblocks : 1u), dim3(64), 0, stream, ncand,
+                           stride, rounds, meta, o, head, qp, sp);
     }
     hipLaunchKernelGGL(kf_check_kernel, dim3(blocks), dim3(64), 0, stream, ncand, stride, rounds, meta, o,
                        head, qp, limit, sp, limit ? (const uint32_t *)sp.list[1] : nullptr);
