@@ -145,7 +145,7 @@ def profiled_launch_figures():
 def kfact_rate(ctx, steps, warmup):
     """1-factorizations of K_12: 2^18 seeded partial factorizations of 3-10
     rounds, plain relabelling order, one split launch per step (a first look
-    at 16 relabellings, a first pass of 256, then up to 1024 ranges per
+    at 4 relabellings, a first pass of 256, then up to 1024 ranges per
     undecided candidate); the
     one-wave-per-candidate launch is timed beside it."""
     import ise_amd

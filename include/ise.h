@@ -438,7 +438,7 @@ int ise_kf_relabel_u1f(const uint64_t *src, const uint32_t *src_meta, uint32_t n
 /* Split launch of the 1-factorization test.  A canonical candidate must try
  * every relabelling (46080 x rounds for K_12), which on one wave would be the
  * batch's tail.  With first_chunks > 0 every candidate first gets a look at
- * its first 16 relabellings (four candidates per wave), those still open run
+ * its first 4 relabellings (sixteen candidates per wave), those still open run
  * up to 64 x first_chunks relabellings, and the rest of each candidate still
  * undecided is cut into up to max_ranges contiguous ranges run by independent
  * waves and combined in plain order (first witness, automorphisms before it).  Results
