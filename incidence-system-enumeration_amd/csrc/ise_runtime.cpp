@@ -59,12 +59,22 @@ struct ise_ctx {
     uint64_t *d_kf_part = nullptr, *d_kf_ranges = nullptr;
     size_t kf_cap = 0, kf_range_words = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // executable graphs of ise_submit launches, keyed by batch shape
+    // executable graphs of ise_submit launches, keyed by batch shape without
+    // the batch size: a batch of another size updates the graph in place
+    // (hipGraphExecUpdate), so every level and tail chunk of an enumeration
+    // replays one graph.  The kernels' arguments hold the addresses of context
+    // buffers and the deferral/schedule settings, so every cached graph is
+    // dropped when one of those buffers is reallocated or a setting changes
+    // (drop_graphs).
     bool use_graphs = false;
-    using GraphKey = std::tuple<uint32_t, uint32_t, int, bool, uint32_t, uint32_t, uint32_t, uint32_t, int,
+    using GraphKey = std::tuple<uint32_t, int, bool, uint32_t, uint32_t, uint32_t, uint32_t, int,
                                 uint32_t, uintptr_t, uint32_t, bool>;
-    std::map<GraphKey, hipGraphExec_t> graphs;
-    uint64_t graph_replays = 0;
+    struct GraphEntry {
+        hipGraphExec_t exec = nullptr;
+        uint32_t n = 0;
+    };
+    std::map<GraphKey, GraphEntry> graphs;
+    uint64_t graph_replays = 0, graph_updates = 0;
     float last_ms = 0.f;
     std::string err;
 };
@@ -88,8 +98,20 @@ template <typename T> hipError_t dalloc(T **p, size_t count) {
     return hipMalloc((void **)p, count * sizeof(T) + 16);
 }
 
+// Destroy every cached graph (after the work already queued has finished):
+// called before a buffer whose address a graph holds is freed, and when a
+// launch setting that the graph key leaves out changes.
+int drop_graphs(ise_ctx *c) {
+    if (c->graphs.empty()) return ISE_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+    c->graphs.clear();
+    return ISE_OK;
+}
+
 int alloc_gens(ise_ctx *c, uint32_t gen_cap) {
     if (gen_cap <= c->gen_cap_alloc) return ISE_OK;
+    if (int rc = drop_graphs(c)) return rc;
     if (c->d.gens) HIPCHK(c, hipFree(c->d.gens));
     c->d.gens = nullptr;
     HIPCHK(c, dalloc(&c->d.gens, (size_t)c->capacity * gen_cap * ISE_MAX_COLS));
@@ -149,6 +171,7 @@ int launch_with(ise_ctx *c, uint32_t blocks, uint32_t n, uint32_t stride, const 
 // Schedule list for batches of up to n, grown outside any launch.
 int ensure_sched(ise_ctx *c, size_t n) {
     if (c->schedule != ISE_SCHEDULE_DEEPEST_FIRST || n <= c->sched_cap) return ISE_OK;
+    if (int rc = drop_graphs(c)) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->d_sched) HIPCHK(c, hipFree(c->d_sched));
     c->d_sched = nullptr;
@@ -162,6 +185,7 @@ int ensure_sched(ise_ctx *c, size_t n) {
 // n, grown outside any launch.
 int ensure_defer(ise_ctx *c, size_t n) {
     if (!c->defer_perms[0] || n <= c->defer_cap) return ISE_OK;
+    if (int rc = drop_graphs(c)) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->d_defer) HIPCHK(c, hipFree(c->d_defer));
     c->d_defer = nullptr;
@@ -333,7 +357,7 @@ int ise_close(ise_ctx *c) {
                     c->d_kf_list, c->d_kf_best, c->d_kf_part, c->d_kf_ranges};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
-    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -413,14 +437,13 @@ int ise_submit(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows,
         if ((uint32_t)group < md || (uint32_t)group < mb) return ISE_ERR_ARG;
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         if (c->use_graphs) {
-            const ise_ctx::GraphKey key{n, stride, group, cells != nullptr, gen_cap, c->cfg.waves_per_block,
+            const ise_ctx::GraphKey key{stride, group, cells != nullptr, gen_cap, c->cfg.waves_per_block,
                                         c->cfg.blocks, c->cfg.max_perms, c->order_mode, c->hunt_budget,
                                         (uintptr_t)c->d_order, c->order_dim, c->report_columns};
             auto it = c->graphs.find(key);
-            if (it == c->graphs.end()) {
-                // capture once: every pointer is a context buffer, so replays see new data
+            if (it == c->graphs.end() || it->second.n != n) {
+                // capture: every pointer is a context buffer, so replays see new data
                 hipGraph_t g = nullptr;
-                hipGraphExec_t ge = nullptr;
                 uint32_t blocks = 0;
                 int rc = grid_blocks(c, n, stride, group, o, &blocks);
                 if (rc) return rc;
@@ -428,14 +451,43 @@ int ise_submit(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows,
                 rc = launch_with(c, blocks, n, stride, c->d_rows, cells ? c->d_cells : nullptr, c->d_meta,
                                  o, c->d_head, c->stream, group);
                 hipError_t e = hipStreamEndCapture(c->stream, &g);
-                if (rc) return rc;
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
                 if (e != hipSuccess) return fail(c, e, "hipStreamEndCapture");
-                e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                bool updated = false;
+                if (it != c->graphs.end()) {
+                    // same kernels with other sizes: update the executable graph in
+                    // place, once its replays already queued have finished
+                    HIPCHK(c, hipStreamSynchronize(c->stream));
+                    hipGraphExecUpdateResult ur = hipGraphExecUpdateError;
+                    hipGraphNode_t err_node = nullptr;
+                    updated = hipGraphExecUpdate(it->second.exec, g, &err_node, &ur) == hipSuccess &&
+                              ur == hipGraphExecUpdateSuccess;
+                    if (updated) {
+                        it->second.n = n;
+                        c->graph_updates++;
+                    } else {
+                        (void)hipGetLastError();
+                        HIPCHK(c, hipStreamSynchronize(c->stream));
+                        (void)hipGraphExecDestroy(it->second.exec);
+                        c->graphs.erase(it);
+                        it = c->graphs.end();
+                    }
+                }
+                if (!updated) {
+                    hipGraphExec_t ge = nullptr;
+                    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                    if (e != hipSuccess) {
+                        (void)hipGraphDestroy(g);
+                        return fail(c, e, "hipGraphInstantiate");
+                    }
+                    it = c->graphs.emplace(key, ise_ctx::GraphEntry{ge, n}).first;
+                }
                 (void)hipGraphDestroy(g);
-                if (e != hipSuccess) return fail(c, e, "hipGraphInstantiate");
-                it = c->graphs.emplace(key, ge).first;
             }
-            HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+            HIPCHK(c, hipGraphLaunch(it->second.exec, c->stream));
             c->graph_replays++;
         } else {
             int rc = launch(c, n, stride, c->d_rows, cells ? c->d_cells : nullptr, c->d_meta, o,
@@ -821,6 +873,7 @@ int ise_set_order(ise_ctx *c, int mode, uint32_t hunt_budget, const uint8_t *d_o
 int ise_set_defer(ise_ctx *c, uint32_t first_pass_perms, uint32_t second_pass_perms) {
     if (!c) return ISE_ERR_ARG;
     if (second_pass_perms && second_pass_perms <= first_pass_perms) return ISE_ERR_ARG;
+    if (int rc = drop_graphs(c)) return rc;
     c->defer_perms[0] = first_pass_perms;
     c->defer_perms[1] = first_pass_perms ? second_pass_perms : 0u;
     return ensure_defer(c, c->capacity);
@@ -828,6 +881,7 @@ int ise_set_defer(ise_ctx *c, uint32_t first_pass_perms, uint32_t second_pass_pe
 
 int ise_set_schedule(ise_ctx *c, int schedule) {
     if (!c || (schedule != ISE_SCHEDULE_QUEUE && schedule != ISE_SCHEDULE_DEEPEST_FIRST)) return ISE_ERR_ARG;
+    if (int rc = drop_graphs(c)) return rc;
     c->schedule = schedule;
     return ensure_sched(c, c->capacity);
 }
@@ -856,6 +910,12 @@ int ise_graph_stats(const ise_ctx *c, uint32_t *graphs, uint64_t *replays) {
     if (!c) return ISE_ERR_ARG;
     if (graphs) *graphs = (uint32_t)c->graphs.size();
     if (replays) *replays = c->graph_replays;
+    return ISE_OK;
+}
+
+int ise_graph_updates(const ise_ctx *c, uint64_t *updates) {
+    if (!c || !updates) return ISE_ERR_ARG;
+    *updates = c->graph_updates;
     return ISE_OK;
 }
 

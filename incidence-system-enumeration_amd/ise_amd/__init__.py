@@ -87,6 +87,7 @@ def lib():
         L.ise_set_schedule.argtypes = [P, I]
         L.ise_kf_set_split.argtypes = [P, U32, U32]
         L.ise_graph_stats.argtypes = [P, ctypes.POINTER(U32), ctypes.POINTER(U64)]
+        L.ise_graph_updates.argtypes = [P, ctypes.POINTER(U64)]
         L.ise_autotune.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P, U32, U32, U32, P, P]
         L.ise_sync.argtypes = [P]
         L.ise_last_error.argtypes = [P]
@@ -339,6 +340,12 @@ class Context:
         g, r = ctypes.c_uint32(0), ctypes.c_uint64(0)
         self._check(self._L.ise_graph_stats(self.h, ctypes.byref(g), ctypes.byref(r)))
         return g.value, r.value
+
+    def graph_updates(self):
+        """In-place updates of cached graphs for batches of another size."""
+        u = ctypes.c_uint64(0)
+        self._check(self._L.ise_graph_updates(self.h, ctypes.byref(u)))
+        return u.value
 
     def autotune(self, n, stride, d_rows, d_meta, res_struct, d_head, max_depth, max_cols, repeats=2,
                  d_cells=None):

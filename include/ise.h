@@ -181,10 +181,14 @@ int ise_copy_to_device(ise_ctx *ctx, void *d_dst, const void *src, uint64_t byte
 int ise_copy_to_host(ise_ctx *ctx, void *dst, const void *d_src, uint64_t bytes);
 int ise_copy_on_device(ise_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes);
 
-/* Capture every launch of ise_submit as a hipGraph (queue reset + kernel),
- * one executable graph per batch shape (n, stride, group size, inputs and
- * outputs), instantiated once and replayed afterwards; the enumerator then
- * replays one graph per level and chunk size.  on = 0 launches directly. */
+/* Capture every launch of ise_submit as a hipGraph (queue reset, schedule
+ * and check kernels), one executable graph per batch shape (stride, group
+ * size, inputs and outputs, launch and order settings), instantiated once and
+ * replayed afterwards.  A batch of another size updates that graph in place
+ * (hipGraphExecUpdate), so an enumeration replays one graph for every level
+ * and tail chunk.  Cached graphs are dropped whenever a context buffer they
+ * address is reallocated (a larger gen_cap, a device batch above capacity)
+ * and by ise_set_defer / ise_set_schedule.  on = 0 launches directly. */
 int ise_use_graphs(ise_ctx *ctx, int on);
 
 /* Whether ise_submit reports the column permutation (perm_col; the
@@ -217,6 +221,8 @@ int ise_set_defer(ise_ctx *ctx, uint32_t first_pass_perms, uint32_t second_pass_
 int ise_set_schedule(ise_ctx *ctx, int schedule);
 /* Number of executable graphs instantiated so far and replays of them. */
 int ise_graph_stats(const ise_ctx *ctx, uint32_t *graphs, uint64_t *replays);
+/* In-place updates of cached graphs for batches of another size so far. */
+int ise_graph_updates(const ise_ctx *ctx, uint64_t *updates);
 
 /* Autotuner: times the device-resident batch (same arguments as
  * ise_check_batch_device) for every lane-group size that holds the batch

@@ -76,6 +76,16 @@ int orc_test_canonicity(const uint8_t *mat, int nrows, int ncols, int len_stab,
 /* Column-orbit start masks of a bit-packed matrix: out[i] = orbits of row i. */
 void orc_cell_masks(const uint64_t *rows, int nrows, int ncols, uint64_t *out);
 
+/* Test hook: one step of the row-permutation walk the test runs
+ * (CGroupOrder::next_permutation, GroupOrder.h:136-248) on caller-held state.
+ * perm[n], orbits[n] (orbit leader of each row); at = 255 for the plain
+ * successor, 254 for the first call after an automorphism, else the position
+ * whose prefix was refuted; fixed = rows held in place (lenStab).
+ * state[0] = stabiliser point (m_nStabLength), state[1] = m_nStabLengthAut.
+ * Returns the first position whose image changed, or 255 when the walk is
+ * exhausted. */
+int orc_walk_step(uint8_t *perm, const uint8_t *orbits, int n, int at, int fixed, uint16_t *state);
+
 /* Depth-first enumeration of 2-(v,k,lambda) designs (b, r derived).
  * Canonical full designs are written to designs[] as v packed row words each
  * (up to max_designs) with |Aut| in auts[].  stats[0..3] = candidates tested,

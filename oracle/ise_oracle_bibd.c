@@ -166,6 +166,18 @@ static u8 next_row_perm(grp_t *g, u8 *perm, const u8 *orb, u8 n, u8 at, u8 fixed
     return piv;
 }
 
+int orc_walk_step(uint8_t *perm, const uint8_t *orbits, int n, int at, int fixed, uint16_t *state)
+{
+    grp_t g;
+    g.stab = state[0];
+    g.stab_aut = state[1];
+    g.order = 1;
+    u8 r = next_row_perm(&g, perm, orbits, (u8)n, (u8)at, (u8)fixed);
+    state[0] = g.stab;
+    state[1] = g.stab_aut;
+    return r;
+}
+
 /* Column-orbit start list of row `row` from its start mask. */
 static int orbit_starts(uint64_t mask, int ncols, u8 *starts, u8 *lens)
 {

@@ -166,3 +166,34 @@ def u1f_candidates(block, count, seed, rmin=3):
         full = np.array(rows, np.uint8)
         out.append(full[:int(rng.integers(rmin, nr + 1))])
     return out
+
+
+def column_sorted(mats):
+    """Largest column order of each matrix (columns sorted decreasingly with
+    row 0 as the most significant bit); mats [m][d][b] -> same shape, int8."""
+    m, d, b = mats.shape
+    w = (1 << np.arange(d - 1, -1, -1)).astype(np.int64)
+    keys = np.einsum("mdb,d->mb", mats.astype(np.int64), w)
+    keys = -np.sort(-keys, axis=1)
+    return ((keys[:, None, :] >> np.arange(d - 1, -1, -1)[None, :, None]) & 1).astype(np.int8)
+
+
+def random_orbit_form_batch(seed, count, stride=8, max_rows=7, max_cols=10, stab_frac=0.35):
+    """Seeded random 0/1 matrices of 2..max_rows rows and 2..max_cols columns
+    in orbit form (columns in decreasing order: ones first inside every column
+    orbit, as MakeRow writes rows), a share of them with lenStab > 0.
+    Returns (list of matrices, rows [count][stride] uint64, meta [count])."""
+    rng = np.random.default_rng(seed)
+    mats = []
+    rows = np.zeros((count, stride), np.uint64)
+    meta = np.zeros(count, np.uint32)
+    for i in range(count):
+        d = int(rng.integers(2, max_rows + 1))
+        b = int(rng.integers(2, max_cols + 1))
+        density = float(rng.choice([0.3, 0.5, 0.7]))
+        m = column_sorted((rng.random((1, d, b)) < density).astype(np.int8))[0]
+        f = int(rng.integers(0, d)) if rng.random() < stab_frac else 0
+        mats.append(m)
+        rows[i, :d] = (m.astype(np.uint64) << np.arange(b, dtype=np.uint64)).sum(axis=1)
+        meta[i] = d | (b << 8) | (f << 16)
+    return mats, rows, meta

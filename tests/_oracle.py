@@ -32,6 +32,7 @@ def load():
         L.orc_check_batch.argtypes = [U32, U32, _P, _P, _P, U32, _P]
         L.orc_test_canonicity.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, U32]
         L.orc_cell_masks.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
+        L.orc_walk_step.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
         L.orc_enumerate_bibd.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_long,
                                          _P, _P, _P, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_enumerate_bibd.restype = ctypes.c_long
@@ -97,6 +98,13 @@ class Oracle:
         if rc != 0:
             raise ValueError("oracle rejected the batch")
         return res
+
+    def walk_step(self, perm, orbits, at, fixed, state):
+        """One step of the oracle's row-permutation walk; perm (uint8 [n]) and
+        state (uint16 [2]) are updated in place.  Returns the first changed
+        position or 255."""
+        return self.L.orc_walk_step(perm.ctypes.data, orbits.ctypes.data, len(perm), at, fixed,
+                                    state.ctypes.data)
 
     def cell_masks(self, rows, nrows, ncols):
         rows = np.ascontiguousarray(rows, dtype=np.uint64)

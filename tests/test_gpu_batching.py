@@ -56,13 +56,58 @@ def test_enumeration_with_graph_per_level(v, k, lam, oracle):
     print(f"case graph enumeration {(v, k, lam)}", flush=True)
     with ise_amd.Context(capacity=1 << 14, stride=32) as ctx:
         ctx.use_graphs(True)
-        des, auts, stats, _ = ctx.enumerate_bibd(v, k, lam)
+        des, auts, stats, levels = ctx.enumerate_bibd(v, k, lam)
         graphs, replays = ctx.graph_stats()
+        updates = ctx.graph_updates()
     ref = counts_for(v, k, lam)
     assert len(des) == ref["designs"] and hist(auts) == ref["aut_hist"]
     odes, oauts, _, _ = oracle.enumerate(v, k, lam)
     assert sorted(map(design_key, des)) == sorted(map(design_key, odes))
-    assert graphs > 0 and replays >= graphs
+    # one graph for the whole enumeration: every level (and every tail chunk)
+    # replays it, updated in place when the batch size changes
+    chunks = sum(-(-int(c) // (1 << 14)) for c in levels if c)
+    assert graphs == 1 and replays == chunks
+    assert 1 <= updates < replays
+
+
+def test_graph_cache_survives_buffer_growth(oracle):
+    """Captured graphs address context buffers: a larger gen_cap reallocates
+    the generator buffer and a device batch above the context capacity the
+    schedule list, so the cache must not replay a graph holding the old
+    addresses.  gen_cap 2 -> 4 -> 2, then a device batch above capacity, then
+    the first shape again; every result is checked against the oracle."""
+    print("case graph cache buffer growth", flush=True)
+    rows, meta = ise_amd.random_extensions(15, 7, 3, 1500, 5, 15, 0x77)
+    big_rows, big_meta = ise_amd.random_extensions(15, 7, 3, 3000, 5, 15, 0x78)
+    fields = ("verdict", "fail_row", "row_to_change", "group_order", "perm_row", "orbits", "n_gens", "gens")
+    with ise_amd.Context(capacity=2048, stride=32) as ctx:
+        ctx.use_graphs(True)
+        for step, cap in enumerate((2, 4, 2)):
+            g = ctx.check_batch(rows, meta, gen_cap=cap, fields=fields)
+            o = oracle.check_batch(rows, meta, gen_cap=cap)
+            for f in fields:
+                assert np.array_equal(g[f], o[f]), (step, cap, f)
+        n = len(big_meta)
+        d_rows = ctx.device_array(big_rows.shape, np.uint64, big_rows)
+        d_meta = ctx.device_array(big_meta.shape, np.uint32, big_meta)
+        d_head = ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
+        dev, s = ctx.device_results(n, ("verdict", "perm_row", "group_order"))
+        ctx.check_batch_device(n, 32, d_rows, d_meta, s, d_head)
+        ctx.sync()
+        ob = oracle.check_batch(big_rows, big_meta)
+        for f in ("verdict", "perm_row", "group_order"):
+            assert np.array_equal(dev[f].to_host(), ob[f]), ("device batch", f)
+        for step, cap in enumerate((2, 4)):
+            g = ctx.check_batch(rows, meta, gen_cap=cap, fields=fields)
+            o = oracle.check_batch(rows, meta, gen_cap=cap)
+            for f in fields:
+                assert np.array_equal(g[f], o[f]), ("after growth", step, cap, f)
+        ctx.set_defer(16, 0)
+        g = ctx.check_batch(rows, meta, gen_cap=2, fields=fields)
+        o = oracle.check_batch(rows, meta, gen_cap=2)
+        for f in fields:
+            assert np.array_equal(g[f], o[f]), ("after set_defer", f)
+        ctx.set_defer(0, 0)
 
 
 def test_graph_replays_between_eager_work(oracle):

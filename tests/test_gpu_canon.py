@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 import ise_amd
-from _helpers import check_witness, col_sorted, counts_for, design_key, golden, hist, unpack
+from _helpers import (check_witness, col_sorted, counts_for, design_key, golden, hist,
+                      random_orbit_form_batch, unpack)
 
 pytestmark = pytest.mark.gpu
 
@@ -72,6 +73,43 @@ def test_random_extensions_parity(gpu_ctx, oracle, with_cells, v, k, lam, group)
     for i in np.nonzero(g["verdict"] == 0)[0][:200]:
         d = int(meta[i]) & 0xff
         check_witness(unpack(rows[i, :d], b), g["perm_row"][i], int(g["fail_row"][i]))
+
+
+@pytest.mark.parametrize("columns", [True, False])
+def test_fixed_rows_parity(gpu_ctx, oracle, columns):
+    """lenStab > 0 (rows held in place, the reference's stabiliserLengthExt):
+    random orbit-form matrices, including those where the walk falls back past
+    the fixed rows after a refuted prefix exhausts its tail
+    (GroupOrder.h:183-186), and random extensions of 2-(15,3,1) and
+    2-(13,3,1) with a random number of fixed rows; every field against the
+    oracle, replayed search (columns=True) and column-set search."""
+    print(f"case fixed rows columns={columns}", flush=True)
+    _, r1, m1 = random_orbit_form_batch(0x51DE, 3600, stride=32)
+    rng = np.random.default_rng(0xF1)
+    parts = [(r1, m1)]
+    for v, k, lam in ((15, 3, 1), (13, 3, 1)):
+        r, m = ise_amd.random_extensions(v, k, lam, 4096, 4, v, 0xF15 + v)
+        d = (m & 0xff).astype(np.int64)
+        f = (rng.random(len(m)) * d).astype(np.uint32)
+        parts.append((r, m | (f << 16)))
+    rows = np.concatenate([p[0] for p in parts])
+    meta = np.concatenate([p[1] for p in parts])
+    assert ((meta >> 16) & 0xff).astype(bool).sum() > 3000
+    fields = None if columns else tuple(f for f in FIELDS_CMP if f != "perm_col") + ("gens",)
+    g = gpu_ctx.check_batch(rows, meta, gen_cap=4, fields=fields)
+    o = oracle.check_batch(rows, meta, gen_cap=4)
+    if columns:
+        assert_same(g, o, 4, "fixed rows")
+    else:
+        for f in fields:
+            if f == "gens":
+                continue
+            a, b = g[f], o[f]
+            eq = (a == b) if a.ndim == 1 else (a == b).all(axis=tuple(range(1, a.ndim)))
+            assert eq.all(), (f, int(np.nonzero(~eq)[0][0]))
+        for i in range(len(meta)):
+            ng = min(int(o["n_gens"][i]), 4)
+            assert np.array_equal(g["gens"][i, :ng], o["gens"][i, :ng]), ("gens", i)
 
 
 def test_group_too_small_is_rejected(gpu_ctx):

@@ -235,3 +235,65 @@ def test_split_buffers_grow_past_capacity(oracle):
         for b in list(dev.values()) + [d_rows, d_meta, d_head]:
             b.free()
     assert_same(g, oracle_batch(oracle, rows, meta, 12), "grown buffers")
+
+
+_LONG_CASE = {}
+
+
+def long_records_case(oracle):
+    """Records of 12-15 rounds: 200 relabelled prefixes (12-13 rounds) of the
+    printed perfect 1-factorizations of K_14 and 200 random partial
+    1-factorizations of K_16 with 12-15 rounds, with the oracle's results."""
+    if not _LONG_CASE:
+        kf = golden("kfact.json")
+        src, smeta = ise_amd.kf_pack([m["rounds"] for m in kf["p14"]["matrices"]])
+        r1, m1 = ise_amd.kf_relabel_p1f(src, smeta, 200, 12, 0x1412)
+        r2, m2 = ise_amd.kf_random(16, 12, 15, 200, 0x1612)
+        rows = np.concatenate([r1, r2])
+        meta = np.concatenate([m1, m2])
+        assert ((meta & 0xff) >= 12).all() and set(np.unique(m2 & 0xff)) == {12, 13, 14, 15}
+        o = {k: [] for k in ise_amd.KF_FIELDS}
+        for i in range(len(meta)):
+            nr, n = int(meta[i]) & 0xff, (int(meta[i]) >> 8) & 0xff
+            for k, x in zip(ise_amd.KF_FIELDS, oracle.kf_check(ise_amd.kf_unpack(rows[i, :nr], n))):
+                o[k].append(x)
+        o = {k: np.array(v, np.uint64 if k != "verdict" else np.uint8) for k, v in o.items()}
+        _LONG_CASE.update(rows=rows, meta=meta, o=o)
+    return _LONG_CASE["rows"], _LONG_CASE["meta"], _LONG_CASE["o"]
+
+
+@pytest.mark.parametrize("first,ranges", [(ise_amd.DEFAULT_KF_FIRST_CHUNKS, ise_amd.DEFAULT_KF_MAX_RANGES), (0, 1)])
+def test_full_k14_perfect_factorizations(gpu_ctx, first, ranges):
+    """The 23 printed perfect 1-factorizations of K_14 (13 rounds, the full
+    matrices) through the general canonizer: canonical, the printed |Aut(M)|,
+    and every relabelling tried (7! pair-slot arrangements x 2^7 orientations
+    x 13 rounds)."""
+    print(f"case kfact full K14 P1F general split={first}/{ranges}", flush=True)
+    kf = golden("kfact.json")
+    mats = kf["p14"]["matrices"]
+    rows, meta = ise_amd.kf_pack([m["rounds"] for m in mats])
+    assert ((meta & 0xff) == 13).all()
+    gpu_ctx.kf_set_split(first, ranges)
+    try:
+        g = gpu_ctx.kf_check(rows, meta)
+    finally:
+        gpu_ctx.kf_set_split()
+    assert (g["verdict"] == 1).all()
+    assert (g["n_auts"] == np.array([m["aut"] for m in mats], np.uint64)).all()
+    assert (g["n_tried"] == 5040 * 128 * 13).all()
+    assert (g["witness"] == np.uint64(2**64 - 1)).all()
+
+
+@pytest.mark.parametrize("first,ranges", [(ise_amd.DEFAULT_KF_FIRST_CHUNKS, ise_amd.DEFAULT_KF_MAX_RANGES), (0, 1)])
+def test_records_of_12_to_15_rounds(gpu_ctx, oracle, first, ranges):
+    """Rounds 12-14 are staged in the last slot of the first look's four-lane
+    groups; relabelled K_14 prefixes and K_16 partials of 12-15 rounds against
+    the oracle, split launch and one wave per candidate."""
+    print(f"case kfact 12-15 rounds split={first}/{ranges}", flush=True)
+    rows, meta, o = long_records_case(oracle)
+    gpu_ctx.kf_set_split(first, ranges)
+    try:
+        g = gpu_ctx.kf_check(rows, meta)
+    finally:
+        gpu_ctx.kf_set_split()
+    assert_same(g, o, f"12-15 rounds split {first}/{ranges}")
