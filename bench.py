@@ -7,7 +7,9 @@ every candidate a beam-sampled run of the row-by-row enumerator tests there
 (beam 64, seed 0x25041) followed by 2^20 seeded random valid extensions
 (seed 0x25041) -- rows bit-packed as uint64 words (50 columns).  Each
 candidate gets its verdict, witness row permutation, failing row,
-rowToChange and |Aut|.
+rowToChange, |Aut| and the automorphism generators found (up to 16, each the
+permutation it induces on the next row's column orbits).  The launch shape
+comes from the autotuner, run on the same batch before timing.
 
   * plain order: one launch of the canonicity kernel per step (column-set
     search, candidates started deepest first);
@@ -39,12 +41,28 @@ sys.path.insert(0, os.path.join(ROOT, "incidence-system-enumeration_amd"))
 
 METRIC = "candidates canon-checked/sec on 1 MI355X, learned-order vs plain; bit-exact"
 HBM_PEAK_GBS = 8000.0
+# LDS: 128 bytes per clock per CU x 256 CUs x 2.4 GHz
+LDS_PEAK_GBS = 128 * 256 * 2.4
+GEN_CAP = 16           # automorphism generators returned per candidate in the timed launch
 STRIDE = 16            # depth <= 16 rows per candidate
 SEED = 0x25041
 BEAM = 64
 DMIN, DMAX = 8, 16
 RESULT_BYTES = 1 + 1 + 1 + 8 + 32   # verdict, fail_row, row_to_change, |Aut|, witness row perm
 PROFILE = os.path.join(ROOT, "profiles", "check_kernel64_2-25-4-1_stream_traffic.txt")
+# the sources of the profiled kernel: PMC figures are reported only when the
+# profile was taken of these exact sources
+KERNEL_SOURCES = [os.path.join(ROOT, "incidence-system-enumeration_amd", "csrc", f)
+                  for f in ("ise_wave.h", "ise_kernels.hip", "ise_internal.h")]
+
+
+def kernel_sources_sha():
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def build_batch(n_random, path):
@@ -70,9 +88,11 @@ def load_batch(n_random):
     return z["rows"], z["meta"], int(z["n_enum"][0])
 
 
-def cpu_baseline(rows, meta, seconds=15.0):
+def cpu_baseline(rows, meta, gpu, seconds=15.0):
     """The oracle (a scalar port of the reference's checker), one thread, on a
-    bounded sample of the same batch in seeded random order."""
+    bounded sample of the same batch in seeded random order.  Its results on
+    the sample are also compared with the GPU's (`gpu`: host copies of the
+    timed launch's result fields)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle
     orc = _oracle.load()
@@ -81,12 +101,28 @@ def cpu_baseline(rows, meta, seconds=15.0):
     idx = np.random.default_rng(0).permutation(len(meta))
     while done < len(idx) and time.perf_counter() - t0 < seconds:
         sel = idx[done:done + step]
-        orc.check_batch(np.ascontiguousarray(rows[sel]), np.ascontiguousarray(meta[sel]))
+        o = orc.check_batch(np.ascontiguousarray(rows[sel]), np.ascontiguousarray(meta[sel]), gen_cap=GEN_CAP)
+        for f, a in gpu.items():
+            if f == "gens":
+                ng = np.minimum(o["n_gens"], GEN_CAP)
+                keep = np.arange(GEN_CAP)[None, :] < ng[:, None]
+                if not np.array_equal(a[sel][keep], o[f][keep]):
+                    raise RuntimeError("GPU generators differ from the oracle's on the CPU sample")
+            elif not np.array_equal(a[sel], o[f]):
+                raise RuntimeError(f"GPU field {f} differs from the oracle on the CPU sample")
         done += len(sel)
     dt = time.perf_counter() - t0
     return dict(value=done / dt, unit="candidates/s", cores=1, kind="port",
                 sample=f"{done} candidates of the same batch (seeded random order), oracle single "
-                       f"thread, {dt:.1f} s")
+                       f"thread, {dt:.1f} s; its results equal the GPU's on the sample")
+
+
+def host_cores():
+    """Cores this process may run on."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def cpu_all_cores(rows, meta, seconds=15.0):
@@ -96,7 +132,7 @@ def cpu_all_cores(rows, meta, seconds=15.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle
     orc = _oracle.load()
-    cores = min(os.cpu_count() or 1, 16)
+    cores = host_cores()
     step = 1024
     idx = np.random.default_rng(0).permutation(len(meta))
     chunks = [idx[i:i + step] for i in range(0, len(idx), step)]
@@ -132,13 +168,17 @@ def host_buffer_rate(rows, meta):
 
 def profiled_launch_figures():
     """HBM bytes and instruction counts of one plain launch of this workload,
-    as measured by rocprofv3 PMC runs and committed under profiles/."""
+    as measured by rocprofv3 PMC runs and committed under profiles/; empty
+    unless the profile was taken of the current kernel sources."""
     out = {}
     if os.path.exists(PROFILE):
         for line in open(PROFILE):
             if ":" in line and not line.startswith("#"):
                 k, v = line.split(":", 1)
-                out[k.strip()] = float(v)
+                k, v = k.strip(), v.strip()
+                out[k] = v if k in ("sources_sha", "commit") else float(v)
+    if out.get("sources_sha") != kernel_sources_sha():
+        return {}
     return out
 
 
@@ -154,25 +194,58 @@ def kfact_rate(ctx, steps, warmup):
     d_rows = ctx.device_array(rows.shape, np.uint64, rows)
     d_meta = ctx.device_array(meta.shape, np.uint32, meta)
     d_head = ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
-    dev, res = ctx.kf_device_results(n)
-    timed = {}
+    timed, results = {}, {}
     for name, split in (("one_pass", (0, 1)), ("split", (ise_amd.DEFAULT_KF_FIRST_CHUNKS,
                                                          ise_amd.DEFAULT_KF_MAX_RANGES))):
+        dev, res = ctx.kf_device_results(n)
         ctx.kf_set_split(*split)
         if warmup:
             ctx.kf_time_batch_device(n, 16, d_rows, d_meta, res, d_head, warmup)
         timed[name] = float(np.mean(ctx.kf_time_batch_device(n, 16, d_rows, d_meta, res, d_head, steps)))
+        results[name] = {k: dev[k].to_host() for k in ise_amd.KF_FIELDS}
+        for b in dev.values():
+            b.free()
     ctx.kf_set_split()
-    ms = timed["split"]
-    verdict = dev["verdict"].to_host()
-    tried = dev["n_tried"].to_host()
-    for b in list(dev.values()) + [d_rows, d_meta, d_head]:
+    for b in (d_rows, d_meta, d_head):
         b.free()
+    ms = timed["split"]
+    g = results["split"]
+    if (g["verdict"] > 1).any():
+        raise RuntimeError("invalid record in the K_12 batch")
+    for k in ise_amd.KF_FIELDS:
+        if not np.array_equal(g[k], results["one_pass"][k]):
+            raise RuntimeError(f"K_12: split launch and one-pass launch differ in {k}")
     return {"workload": "K_12 partial 1-factorizations, 3-10 rounds, 2^18 seeded (seed 0x12), plain order",
             "candidates": n, "ms_per_step": ms, "candidates_per_s": n / (ms / 1e3),
-            "canonical": int((verdict == 1).sum()), "relabellings_per_s": float(tried.sum()) / (ms / 1e3),
+            "canonical": int((g["verdict"] == 1).sum()),
+            "relabellings_per_s": float(g["n_tried"].sum()) / (ms / 1e3),
             "split": {"first_chunks": ise_amd.DEFAULT_KF_FIRST_CHUNKS, "max_ranges": ise_amd.DEFAULT_KF_MAX_RANGES},
-            "one_pass_ms_per_step": timed["one_pass"]}
+            "one_pass_ms_per_step": timed["one_pass"],
+            "one_pass_equal_to_split": True,
+            "_results": (rows, meta, g)}
+
+
+def kfact_cpu_baseline(rows, meta, g, seconds=10.0):
+    """The oracle's 1-factorization canonizer on one thread over a seeded
+    sample of the K_12 batch; its results are compared with the GPU's."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+    import ise_amd
+    orc = _oracle.load()
+    idx = np.random.default_rng(0x12).permutation(len(meta))
+    done, t0 = 0, time.perf_counter()
+    for i in idx:
+        if done >= 256 and time.perf_counter() - t0 > seconds:
+            break
+        nr = int(meta[i]) & 0xff
+        v, w, a, t = orc.kf_check(ise_amd.kf_unpack(rows[i, :nr], 12))
+        if (v, w, a, t) != (int(g["verdict"][i]), int(g["witness"][i]), int(g["n_auts"][i]), int(g["n_tried"][i])):
+            raise RuntimeError(f"K_12 candidate {i}: GPU and oracle differ")
+        done += 1
+    dt = time.perf_counter() - t0
+    return dict(value=done / dt, unit="candidates/s", cores=1, kind="port",
+                sample=f"{done} candidates of the K_12 batch (seeded random order), oracle single thread, "
+                       f"{dt:.1f} s; results equal to the GPU's")
 
 
 def kfact_p1f_rate(ctx, steps, warmup, uniform=False):
@@ -252,12 +325,19 @@ def stream_rates(ctx, steps, warmup):
     return out
 
 
-def enumeration_rate():
+def enumeration_rate(cpu=True):
     """Full enumeration of 2-(13,3,1) and 2-(15,3,1): host row solver + GPU
     canonicity per level, and the device-resident enumeration (GPU row
-    solver, canonicity and compaction); designs per second, wall clock."""
+    solver, canonicity and compaction); designs per second, wall clock.  With
+    cpu: the oracle's depth-first enumeration of the same parameter sets on
+    one host thread beside it (design lists compared)."""
     import ise_amd
     out = {}
+    orc = None
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle
+        orc = _oracle.load()
     with ise_amd.Context(capacity=1 << 16, stride=32) as ctx:
         for v, k, lam in ((13, 3, 1), (15, 3, 1)):
             ctx.enumerate_bibd(v, k, lam)          # warm-up (graphs, allocations)
@@ -270,10 +350,20 @@ def enumeration_rate():
             dt2 = time.perf_counter() - t0
             if not np.array_equal(des, des2):
                 raise RuntimeError("device enumeration differs from the host-solver enumeration")
-            out[f"2-({v},{k},{lam})"] = {"designs": len(des), "seconds": dt, "designs_per_s": len(des) / dt,
-                                         "candidates_per_s": st["candidates"] / dt,
-                                         "device_resident_seconds": dt2,
-                                         "device_resident_designs_per_s": len(des) / dt2}
+            rec = {"designs": len(des), "seconds": dt, "designs_per_s": len(des) / dt,
+                   "candidates_per_s": st["candidates"] / dt,
+                   "device_resident_seconds": dt2,
+                   "device_resident_designs_per_s": len(des) / dt2}
+            if orc is not None:
+                t0 = time.perf_counter()
+                odes, _, ost, _ = orc.enumerate(v, k, lam)
+                dtc = time.perf_counter() - t0
+                if sorted(map(tuple, odes.tolist())) != sorted(map(tuple, des.tolist())):
+                    raise RuntimeError(f"2-({v},{k},{lam}): oracle and GPU design lists differ")
+                rec["cpu_baseline"] = {"seconds": dtc, "designs_per_s": len(odes) / dtc,
+                                       "candidates": int(ost[0]), "cores": 1, "kind": "port",
+                                       "note": "oracle depth-first enumeration, one thread; same design list"}
+            out[f"2-({v},{k},{lam})"] = rec
     return out
 
 
@@ -288,6 +378,8 @@ def main():
     ap.add_argument("--hunt-budget", type=int, default=8)
     ap.add_argument("--learned-gate", type=int, default=16,
                     help="plain permutations after which a candidate is ordered by the network (0: all)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="use --group-lanes/--waves-per-block instead of the autotuner's choice")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the K_12 and enumeration figures")
@@ -328,11 +420,20 @@ def main():
     d_rows = ctx.device_array(rows.shape, np.uint64, rows)
     d_meta = ctx.device_array(meta.shape, np.uint32, meta)
     d_head = ctx.device_array((ise_amd.QUEUE_WORDS,), np.uint32)
-    fields = ("verdict", "fail_row", "row_to_change", "group_order", "perm_row")
-    out, res = ctx.device_results(n, fields)
-    ctx.set_launch(group_lanes=args.group_lanes, waves_per_block=args.waves_per_block)
+    fields = ("verdict", "fail_row", "row_to_change", "group_order", "perm_row", "n_gens", "gens")
+    out, res = ctx.device_results(n, fields, gen_cap=GEN_CAP)
+    tuned = None
+    if args.autotune:
+        # launch shape from the autotuner, on this batch (2-(25,4,1) needs 64 lanes: 50 columns)
+        best, table = ctx.autotune(n, STRIDE, d_rows, d_meta, res, d_head, max_depth=DMAX, max_cols=50,
+                                   repeats=2)
+        tuned = {"best": best, "ms_table_64_lanes_1_2_4_8_waves": [float(x) for x in table[2]]}
+        print(f"bench: autotuned {best}", file=sys.stderr, flush=True)
+    else:
+        ctx.set_launch(group_lanes=args.group_lanes, waves_per_block=args.waves_per_block)
 
-    # ---- plain order: the timed steps
+    # ---- plain order: the timed steps (every result field of the search,
+    # automorphism generators included)
     if args.warmup:
         ctx.time_batch_device(n, STRIDE, d_rows, d_meta, res, d_head, args.warmup)
     t0 = time.perf_counter()
@@ -341,7 +442,17 @@ def main():
     ms = float(np.mean(per_step_ms))
     value = n / (ms / 1e3)
     verdict = out["verdict"].to_host()
+    gpu_host = {f: out[f].to_host() for f in fields}
     print(f"bench: plain {ms:.2f} ms/step", file=sys.stderr, flush=True)
+    # the same without generators
+    nout, nres = ctx.device_results(n, fields[:5])
+    if args.warmup:
+        ctx.time_batch_device(n, STRIDE, d_rows, d_meta, nres, d_head, 1)
+    ms_nogens = float(np.mean(ctx.time_batch_device(n, STRIDE, d_rows, d_meta, nres, d_head,
+                                                    max(3, args.steps // 2))))
+    for f in fields[:5]:
+        if not np.array_equal(nout[f].to_host(), gpu_host[f]):
+            raise RuntimeError(f"{f} differs between the launches with and without generators")
 
     # statistics from one extra, untimed launch with the counters compiled in
     cout, cres = ctx.device_results(n, ("verdict", "counters"))
@@ -379,6 +490,7 @@ def main():
                                                         "row_order_weights.npy",
             "mean_perms_to_reject": float(lcount[rej, 0].mean()),
             "mean_rows_to_reject": float(lcount[rej, 1].mean()),
+            "rows_compared": float(lcount[:, 1].astype(np.float64).sum()),
             "verdicts_equal_to_plain": True,
         }
         print(f"bench: learned {lmean:.2f} ms/step (gated pass + scoring {np.mean(oms):.2f})", file=sys.stderr,
@@ -386,9 +498,21 @@ def main():
         d_order.free()
         model.close()
 
-    # algorithmic HBM bytes of one plain launch: candidate rows + meta in, result record out
-    hbm_bytes = float((depth * 8 + 4).sum() + n * RESULT_BYTES)
+    # algorithmic HBM bytes of one plain launch: candidate rows + meta in, result
+    # record out (generators: 64 bytes per stored automorphism)
+    ngens = gpu_host["n_gens"].astype(np.int64)
+    stored = np.minimum(ngens, GEN_CAP)
+    in_bytes = float((depth * 8 + 4).sum())
+    hbm_bytes = in_bytes + n * (RESULT_BYTES + 2) + 64.0 * float(stored.sum())
     achieved_gbs = hbm_bytes / (ms / 1e3) / 1e9
+    # algorithmic LDS bytes of the search: per row compared the 8-byte packed
+    # row, per matrix cell (row, column) examined up to the decision 8 bytes,
+    # plus the candidate stream
+    rows_cmp = float(counters[:, 1].astype(np.float64).sum())
+    cells_cmp = float(counters[:, 3].astype(np.float64).sum())
+    orbits_cmp = float(counters[:, 2].astype(np.float64).sum())
+    lds_bytes = 8.0 * (rows_cmp + cells_cmp) + in_bytes
+    lds_gbs = lds_bytes / (ms / 1e3) / 1e9
     prof = profiled_launch_figures()
 
     line = {
@@ -409,25 +533,44 @@ def main():
                         "(beam 64) + 2^20 seeded random valid extensions; value = plain order",
             "candidates": n, "enumerator_candidates": n_enum, "random_candidates": n - n_enum,
             "seed": SEED, "order": "plain", "group_lanes": ctx.get_launch()["group_lanes"] or 64,
-            "waves_per_block": args.waves_per_block,
+            "waves_per_block": ctx.get_launch()["waves_per_block"], "gen_cap": GEN_CAP,
         },
+        # the search is integer work on LDS and registers: its roofline is LDS
+        # bandwidth over the algorithmic LDS bytes; the HBM fraction is kept beside it
         "roofline": {
-            "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved_gbs / HBM_PEAK_GBS,
+            "bound": "lds", "achieved": lds_gbs, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+            "frac": lds_gbs / LDS_PEAK_GBS,
             "traffic": (prof["fetch_bytes"] + prof["write_bytes"]) if prof else None,
+            "lds": {"algorithmic_bytes": lds_bytes, "rows_compared": rows_cmp, "cells_examined": cells_cmp,
+                    "column_orbits_compared": orbits_cmp,
+                    "achieved_gbs": lds_gbs, "peak_gbs": LDS_PEAK_GBS, "frac": lds_gbs / LDS_PEAK_GBS,
+                    "peak_note": "128 B/clk/CU x 256 CUs x 2.4 GHz"},
+            "hbm": {"algorithmic_bytes": hbm_bytes, "achieved_gbs": achieved_gbs, "peak_gbs": HBM_PEAK_GBS,
+                    "frac": achieved_gbs / HBM_PEAK_GBS},
+            "kernel": "check_kernel<64,false,false,true> (column-set search)",
+            "binding": "instruction issue (scalar) and the dependency latency of each search step",
         },
-        # the binding resource: instruction issue (rocprofv3 SQ counters of one plain launch)
+        # what binds: instruction issue (rocprofv3 SQ counters of one plain launch of the
+        # same kernel sources; null when the committed profile is of other sources)
         "issue": ({"valu_frac": prof["valu_insts"] / prof["cu_cycles"],
                    "salu_frac": prof["salu_insts"] / prof["cu_cycles"],
                    "peak": "1 wave64 VALU + 1 SALU instruction per CU per cycle",
+                   "sources_sha": prof["sources_sha"],
                    "source": "profiles/check_kernel64_2-25-4-1_stream.md"} if prof else None),
+        "kernel_sources_sha": kernel_sources_sha(),
+        "generators": {"gen_cap": GEN_CAP, "candidates_with_generators": int((ngens > 0).sum()),
+                       "generators_found": int(ngens.sum()), "candidates_over_cap": int((ngens > GEN_CAP).sum()),
+                       "ms_per_step_without_generators": ms_nogens},
+        "autotune": tuned,
         "canonical": int((verdict == 1).sum()),
         "noncanonical": int(rej.sum()),
         "mean_perms_to_reject_plain": float(counters[rej, 0].mean()) if rej.any() else 0.0,
         "mean_rows_to_reject_plain": float(counters[rej, 1].mean()) if rej.any() else 0.0,
         "learned": learned_line,
         "learned_over_plain": (learned_line["value"] / value) if learned_line else None,
-        "rows_compared_per_s": float(counters[:, 1].sum()) / (ms / 1e3),
+        "rows_compared_per_s": rows_cmp / (ms / 1e3),
+        "rows_compared_plain": rows_cmp,
+        "rows_compared_learned": learned_line["rows_compared"] if learned_line else None,
         "wall_s_timed": wall,
         "ms_per_step_min": float(np.min(per_step_ms)),
     }
@@ -437,19 +580,23 @@ def main():
         ctx._check(ctx._L.ise_copy_on_device(ctx.h, dst.ptr, src.ptr, args.copy_bytes))
         src.free()
         dst.free()
-    for buf in list(out.values()) + list(cout.values()) + [d_rows, d_meta, d_head]:
+    for buf in list(out.values()) + list(nout.values()) + list(cout.values()) + [d_rows, d_meta, d_head]:
         buf.free()
+    kf_sample = None
     if not args.no_extra:
         line["kfact_k12"] = kfact_rate(ctx, max(3, args.steps // 4), 1)
+        kf_sample = line["kfact_k12"].pop("_results")
         line["kfact_p1f_k14"] = kfact_p1f_rate(ctx, max(3, args.steps // 4), 1)
         line["kfact_u1f_k12"] = kfact_p1f_rate(ctx, max(3, args.steps // 4), 1, uniform=True)
         line["streams"] = stream_rates(ctx, max(3, args.steps // 4), 1)
-        line["enumeration"] = enumeration_rate()
+        line["enumeration"] = enumeration_rate(cpu=not args.no_cpu)
         line["host_buffers"] = host_buffer_rate(rows, meta)
     ctx.close()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(rows, meta)
+        line["cpu_baseline"] = cpu_baseline(rows, meta, gpu_host)
         line["cpu_all_cores"] = cpu_all_cores(rows, meta)
+        if kf_sample is not None:
+            line["kfact_k12"]["cpu_baseline"] = kfact_cpu_baseline(*kf_sample)
     print(json.dumps(line), flush=True)
 
 
