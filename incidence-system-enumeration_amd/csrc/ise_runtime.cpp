@@ -15,6 +15,7 @@
 #include <vector>
 #include "ise_internal.h"
 #include "ise_kfact.h"
+#include "ise_noalloc.h"
 
 struct ise_ctx {
     int device = 0;
@@ -59,6 +60,9 @@ struct ise_ctx {
     uint64_t *d_kf_part = nullptr, *d_kf_ranges = nullptr;
     size_t kf_cap = 0, kf_range_words = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // the device's scratch array of ise_check_batch_device_noalloc (not owned)
+    uint32_t *noalloc_scratch = nullptr;
+    std::string noalloc_err;
     // executable graphs of ise_submit launches, keyed by batch shape without
     // the batch size: a batch of another size updates the graph in place
     // (hipGraphExecUpdate), so every level and tail chunk of an enumeration
@@ -334,6 +338,15 @@ int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride) {
             c->kf_tab.base = c->d_kf_tab;
             c->kf_tab.inv = c->d_kf_tab + words.size() / 2;
         }
+        if (e == hipSuccess) {
+            // allocation-free entry point: its scratch address on this device
+            const hipError_t se = ise::noalloc_scratch_init(device, &c->noalloc_scratch);
+            if (se != hipSuccess) {
+                c->noalloc_scratch = nullptr;
+                c->noalloc_err = std::string("scratch lookup: ") + hipGetErrorString(se);
+                (void)hipGetLastError();
+            }
+        }
         if (e == hipSuccess) e = hipEventCreate(&c->ev0);
         if (e == hipSuccess) e = hipEventCreate(&c->ev1);
         if (e != hipSuccess) { rc = fail(c, e, "ise_open"); break; }
@@ -572,6 +585,39 @@ int ise_check_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64
     if (rc == ISE_OK) rc = ensure_sched(c, n);
     if (rc) return rc;
     return launch(c, n, stride, d_rows, d_cells, d_meta, *d_out, d_counter, s, group);
+}
+
+int ise_check_batch_device_noalloc(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                                   const uint64_t *d_cells, const uint32_t *d_meta, const ise_results *d_out,
+                                   void *stream) {
+    if (!c || !d_out || !d_out->verdict || stride < 1 || stride > ISE_MAX_ROWS) return ISE_ERR_ARG;
+    if (n > ISE_NOALLOC_MAX_CANDIDATES) {
+        c->err = "ise_check_batch_device_noalloc: " + std::to_string(n) + " candidates exceed the maximum of " +
+                 std::to_string(ISE_NOALLOC_MAX_CANDIDATES) + " (ISE_NOALLOC_MAX_CANDIDATES)";
+        return ISE_ERR_CAPACITY;
+    }
+    if (n == 0) return ISE_OK;
+    if (!d_rows || !d_meta) return ISE_ERR_ARG;
+    if (!c->noalloc_scratch) {
+        c->err = "ise_check_batch_device_noalloc: no scratch on this device (" + c->noalloc_err + ")";
+        return ISE_ERR_HIP;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const int group = c->cfg.group_lanes ? (int)c->cfg.group_lanes : 64;
+    HIPCHK(c, hipSetDevice(c->device));
+    uint32_t blocks = 0;
+    int rc = grid_blocks(c, n, stride, group, *d_out, &blocks);
+    if (rc) return rc;
+    // one pass (no deferral); queue words and schedule list in the scratch array
+    uint32_t *head = c->noalloc_scratch;
+    uint32_t *sched = c->schedule == ISE_SCHEDULE_DEEPEST_FIRST ? c->noalloc_scratch + ISE_QUEUE_WORDS : nullptr;
+    const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
+    const uint32_t budget = c->order_mode == ISE_ORDER_PLAIN ? 0u : c->hunt_budget;
+    const uint8_t *order = c->order_mode == ISE_ORDER_LEARNED ? c->d_order : nullptr;
+    HIPCHK(c, ise::launch_check(group, blocks, wpb, n, stride, d_rows, d_cells, d_meta, *d_out, head,
+                                c->cfg.max_perms ? c->cfg.max_perms : (1u << 26), budget, order, c->order_dim,
+                                nullptr, nullptr, sched, s));
+    return ISE_OK;
 }
 
 int ise_check_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rows,

@@ -45,6 +45,7 @@ ORDER_HUNT = 1      # budgeted witness hunt in increasing row order, then the pl
 ORDER_LEARNED = 2   # budgeted witness hunt in a per-candidate order table, then the plain search
 
 QUEUE_WORDS = 256   # ISE_QUEUE_WORDS: device words behind a work-queue argument
+NOALLOC_MAX_CANDIDATES = 1 << 20   # ISE_NOALLOC_MAX_CANDIDATES
 
 _lib = None
 
@@ -75,6 +76,7 @@ def lib():
         L.ise_check_batch_device.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P, P]
         L.ise_check_batch_device_timed.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P,
                                                    U32, P]
+        L.ise_check_batch_device_noalloc.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P]
         L.ise_device_alloc.argtypes = [P, ctypes.POINTER(P), U64]
         L.ise_device_free.argtypes = [P, P]
         L.ise_copy_to_device.argtypes = [P, P, P, U64]
@@ -319,6 +321,13 @@ class Context:
         self._check(self._L.ise_check_batch_device(self.h, n, stride, d_rows.ptr,
                                                    d_cells.ptr if d_cells is not None else None,
                                                    d_meta.ptr, ctypes.byref(res_struct), d_head.ptr, None))
+
+    def check_batch_device_noalloc(self, n, stride, d_rows, d_meta, res_struct, d_cells=None):
+        """ise_check_batch_device_noalloc: the same batch with no allocation and
+        no queue buffer (the library's device scratch array holds them)."""
+        self._check(self._L.ise_check_batch_device_noalloc(self.h, n, stride, d_rows.ptr,
+                                                           d_cells.ptr if d_cells is not None else None,
+                                                           d_meta.ptr, ctypes.byref(res_struct), None))
 
     def time_batch_device(self, n, stride, d_rows, d_meta, res_struct, d_head, repeats, d_cells=None):
         """Launch `repeats` times; per-launch milliseconds from HIP events."""

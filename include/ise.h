@@ -164,6 +164,25 @@ int ise_check_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride,
                            const uint32_t *d_meta, const ise_results *d_out,
                            uint32_t *d_counter, void *stream);
 
+/* Allocation-free variant of ise_check_batch_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same inputs,
+ * results and launch settings as ise_check_batch_device, but the work-queue
+ * words and the schedule list live in a device array of the library
+ * (ISE_QUEUE_WORDS + ISE_NOALLOC_MAX_CANDIDATES + 128 uint32 words, about
+ * 4 MiB, held by every GPU the library runs on until the process ends), so no
+ * d_counter is passed, nothing is allocated or freed, and the launch runs in
+ * one pass whatever ise_set_defer says (results are the same either way).
+ * n above ISE_NOALLOC_MAX_CANDIDATES gives ISE_ERR_CAPACITY.  Every call on one
+ * GPU shares that array: calls that overlap in time on one GPU (other
+ * streams, other contexts or threads) race and must not be issued.
+ * Enqueued on `stream` (NULL = the context stream); asynchronous. */
+#define ISE_NOALLOC_MAX_CANDIDATES (1u << 20)
+int ise_check_batch_device_noalloc(ise_ctx *ctx, uint32_t n, uint32_t stride,
+                                   const uint64_t *d_rows, const uint64_t *d_cells,
+                                   const uint32_t *d_meta, const ise_results *d_out,
+                                   void *stream);
+
 /* Device-resident batch launched `repeats` times back to back on the context
  * stream; ms_out[i] receives the time of launch i from HIP events recorded on
  * that stream around it.  Same arguments and results as
