@@ -1,0 +1,106 @@
+// ise_canon.hip -- canonical form of 0/1 matrices on the GPU
+// (CMatrixCanonChecker::CanonizeMatrix with k = 0,
+// Source/MatrixCanonChecker.cpp:420-564).
+//
+// The reference alternates the canonicity test (row permutations saved) with
+// rearranging the rows by the witness it returns and putting the columns back
+// in orbit form (GenerateBinaryColumnOrbits, :186-263), until the test finds
+// no witness.  Every round makes the column-sorted matrix strictly larger, so
+// the loop ends at the unique largest form.  Here each round is one launch of
+// the canonicity kernel over the matrices still open plus one launch of
+// canon_step_kernel, which finishes the decided ones and writes the others,
+// rearranged, into the next round's compact batch.
+//
+// canon_step_kernel: one wave per matrix; lane c owns column c.  The rows
+// (d <= 32) are read once, rearranged by the witness, and column c's key is
+// its d bits with row 0 most significant; the columns in decreasing key order
+// are the orbit form (ones first inside every orbit of every row).  A column's
+// new position is the number of larger keys plus the equal keys before it;
+// ds_permute moves every key there and d ballots rebuild the rows.
+#include <hip/hip_runtime.h>
+#include "ise_canon.h"
+
+namespace ise {
+
+__global__ __launch_bounds__(256) void canon_step_kernel(CanonStep a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < a.n_cur; k += waves) {
+        const uint32_t orig = a.first ? k : a.idx_cur[k];
+        const uint32_t m = a.first ? a.meta_in[k] : a.meta_cur[k];
+        const uint32_t d = m & 0xffu, b = (m >> 8) & 0xffu, fixed = (m >> 16) & 0xffu;
+        const uint64_t *src = a.first ? a.rows_in + (size_t)k * a.stride_in : a.rows_cur + (size_t)k * a.stride;
+        const bool shape_ok = d >= 1 && d <= ISE_MAX_ROWS && d <= a.stride && d <= a.stride_in && b >= 1 &&
+                              b <= ISE_MAX_COLS && fixed < d;
+        uint32_t status = 0;            // 0: open, else the final status of this matrix
+        if (!shape_ok) {
+            status = ISE_VERDICT_INVALID;
+        } else if (!a.first) {
+            const uint32_t v = a.verdict[k];
+            if (v != ISE_VERDICT_NONCANONICAL) status = v;
+            else if (a.iter >= a.max_iters) status = ISE_VERDICT_LIMIT;
+        }
+        if (status) {
+            // decided: the matrix of this round (or nothing, for an invalid record)
+            if (lane < a.stride_out)
+                a.canon[(size_t)orig * a.stride_out + lane] =
+                    status == ISE_VERDICT_INVALID || lane >= d ? 0ull : src[lane];
+            if (lane == 0) {
+                a.status[orig] = (uint8_t)status;
+                if (a.aut) a.aut[orig] = status == ISE_VERDICT_CANONICAL ? a.group_order[k] : 0ull;
+                if (a.iters) a.iters[orig] = a.iter;
+            }
+            continue;
+        }
+        // rows rearranged by the witness: new row i = row perm[i] (identity in the first round)
+        uint64_t row = 0;
+        if (lane < d) {
+            const uint32_t from = a.first ? lane : a.perm_row[(size_t)k * ISE_MAX_ROWS + lane];
+            row = src[from < d ? from : lane];
+        }
+        // column keys: bit (d-1-i) of lane c's key is entry (i, c)
+        uint32_t key = 0;
+        for (uint32_t i = 0; i < d; i++) {
+            // both halves zero-extended (a signed 32-bit lane value would spill into the high word)
+            const uint64_t r = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)i) |
+                               ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row >> 32), (int)i)
+                                << 32);
+            key = (key << 1) | (uint32_t)((r >> lane) & 1ull);
+        }
+        // decreasing key order, ties by column: lanes past b keep their place
+        uint32_t pos = lane;
+        if (lane < b) {
+            pos = 0;
+            for (uint32_t c = 0; c < b; c++) {
+                const uint32_t kc = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)c);
+                pos += (kc > key) || (kc == key && c < lane);
+            }
+        }
+        const uint32_t sorted = (uint32_t)__builtin_amdgcn_ds_permute((int)(pos << 2), (int)key);
+        uint64_t out = 0;
+        for (uint32_t i = 0; i < d; i++) {
+            const uint64_t w = __builtin_amdgcn_ballot_w64(lane < b && ((sorted >> (d - 1 - i)) & 1u));
+            if (lane == i) out = w;
+        }
+        uint32_t slot = 0;
+        if (lane == 0) slot = atomicAdd(a.count_next, 1u);
+        slot = (uint32_t)__builtin_amdgcn_readfirstlane((int)slot);
+        if (lane < a.stride) a.rows_next[(size_t)slot * a.stride + lane] = lane < d ? out : 0ull;
+        if (lane == 0) {
+            a.meta_next[slot] = (m & 0x00ffffffu) | (ISE_SAVE_ROW_PERMUTATIONS << 24);
+            a.idx_next[slot] = orig;
+        }
+    }
+}
+
+__global__ void canon_count_reset_kernel(uint32_t *count) {
+    if (threadIdx.x == 0) *count = 0u;
+}
+
+hipError_t canon_step_launch(const CanonStep &a, uint32_t blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(canon_count_reset_kernel, dim3(1), dim3(64), 0, stream, a.count_next);
+    if (a.n_cur) hipLaunchKernelGGL(canon_step_kernel, dim3(blocks), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+} // namespace ise

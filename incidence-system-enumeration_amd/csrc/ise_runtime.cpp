@@ -16,6 +16,7 @@
 #include "ise_internal.h"
 #include "ise_kfact.h"
 #include "ise_noalloc.h"
+#include "ise_canon.h"
 
 struct ise_ctx {
     int device = 0;
@@ -60,6 +61,16 @@ struct ise_ctx {
     uint64_t *d_kf_part = nullptr, *d_kf_ranges = nullptr;
     size_t kf_cap = 0, kf_range_words = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // batched CanonizeMatrix (ise_canonize_*): two compact work batches of
+    // canon_cap matrices (rows, meta, original index), the open count, and
+    // result arrays for the host entry point
+    uint64_t *d_canon_rows[2] = {nullptr, nullptr};
+    uint32_t *d_canon_meta[2] = {nullptr, nullptr}, *d_canon_idx[2] = {nullptr, nullptr};
+    uint32_t *d_canon_count = nullptr;
+    uint64_t *d_canon_out = nullptr, *d_canon_aut = nullptr;
+    uint8_t *d_canon_status = nullptr;
+    uint32_t *d_canon_iters = nullptr;
+    size_t canon_cap = 0;
     // the device's scratch array of ise_check_batch_device_noalloc (not owned)
     uint32_t *noalloc_scratch = nullptr;
     std::string noalloc_err;
@@ -367,7 +378,10 @@ int ise_close(ise_ctx *c) {
                     c->d.fail_col, c->d.row_to_change, c->d.group_order, c->d.perm_row,
                     c->d.perm_col, c->d.orbits, c->d.n_gens, c->d.counters, c->d.gens,
                     c->d_kf_tab, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried, c->d_defer, c->d_sched,
-                    c->d_kf_list, c->d_kf_best, c->d_kf_part, c->d_kf_ranges};
+                    c->d_kf_list, c->d_kf_best, c->d_kf_part, c->d_kf_ranges,
+                    c->d_canon_rows[0], c->d_canon_rows[1], c->d_canon_meta[0], c->d_canon_meta[1],
+                    c->d_canon_idx[0], c->d_canon_idx[1], c->d_canon_count, c->d_canon_out, c->d_canon_aut,
+                    c->d_canon_status, c->d_canon_iters};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
@@ -585,6 +599,138 @@ int ise_check_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64
     if (rc == ISE_OK) rc = ensure_sched(c, n);
     if (rc) return rc;
     return launch(c, n, stride, d_rows, d_cells, d_meta, *d_out, d_counter, s, group);
+}
+
+// Work buffers of the batched CanonizeMatrix for up to the context capacity
+// (allocated on the first call).
+static int ensure_canon(ise_ctx *c) {
+    if (c->canon_cap) return ISE_OK;
+    const size_t n = c->capacity, words = n * c->stride;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && e == hipSuccess; i++) {
+        e = dalloc(&c->d_canon_rows[i], words);
+        if (e == hipSuccess) e = dalloc(&c->d_canon_meta[i], n);
+        if (e == hipSuccess) e = dalloc(&c->d_canon_idx[i], n);
+    }
+    if (e == hipSuccess) e = dalloc(&c->d_canon_count, 64);
+    if (e == hipSuccess) e = dalloc(&c->d_canon_out, words);
+    if (e == hipSuccess) e = dalloc(&c->d_canon_aut, n);
+    if (e == hipSuccess) e = dalloc(&c->d_canon_status, n);
+    if (e == hipSuccess) e = dalloc(&c->d_canon_iters, n);
+    if (e != hipSuccess) return fail(c, e, "ensure_canon");
+    c->canon_cap = n;
+    return ISE_OK;
+}
+
+int ise_canonize_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                              const uint32_t *d_meta, uint64_t *d_canon, uint8_t *d_status, uint64_t *d_aut,
+                              uint32_t *d_iters, uint32_t max_iters) {
+    if (!c || !d_canon || !d_status || stride < 1 || stride > c->stride || max_iters < 1) return ISE_ERR_ARG;
+    if (n > c->capacity) return ISE_ERR_CAPACITY;
+    if (n == 0) return ISE_OK;
+    if (!d_rows || !d_meta) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_canon(c);
+    if (rc == ISE_OK) rc = ensure_defer(c, n);
+    if (rc == ISE_OK) rc = ensure_sched(c, n);
+    if (rc) return rc;
+    hipStream_t s = c->stream;
+    const uint32_t step_blocks = (uint32_t)c->num_cus * 8u;
+    ise::CanonStep a{};
+    a.first = 1;
+    a.n_cur = n;
+    a.stride = stride;
+    a.stride_in = stride;
+    a.stride_out = stride;
+    a.iter = 0;
+    a.max_iters = max_iters;
+    a.rows_in = d_rows;
+    a.meta_in = d_meta;
+    a.canon = d_canon;
+    a.status = d_status;
+    a.aut = d_aut;
+    a.iters = d_iters;
+    a.count_next = c->d_canon_count;
+    // the test of every round: verdict, witness row permutation and |Aut| (row action)
+    ise_results out{};
+    out.verdict = c->d.verdict;
+    out.perm_row = c->d.perm_row;
+    out.group_order = c->d.group_order;
+    int cur = 0;
+    for (;;) {
+        a.rows_next = c->d_canon_rows[cur];
+        a.meta_next = c->d_canon_meta[cur];
+        a.idx_next = c->d_canon_idx[cur];
+        HIPCHK(c, ise::canon_step_launch(a, step_blocks, s));
+        uint32_t open = 0;
+        HIPCHK(c, hipMemcpyAsync(&open, c->d_canon_count, sizeof open, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        if (open == 0) break;
+        rc = launch(c, open, stride, c->d_canon_rows[cur], nullptr, c->d_canon_meta[cur], out, c->d_head, s, 64);
+        if (rc) return rc;
+        a.first = 0;
+        a.n_cur = open;
+        a.rows_cur = c->d_canon_rows[cur];
+        a.meta_cur = c->d_canon_meta[cur];
+        a.idx_cur = c->d_canon_idx[cur];
+        a.verdict = out.verdict;
+        a.perm_row = out.perm_row;
+        a.group_order = out.group_order;
+        a.iter++;
+        cur ^= 1;
+    }
+    return ISE_OK;
+}
+
+int ise_canonize_batch(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta,
+                       uint64_t *canon_out, uint8_t *status_out, uint64_t *aut_out, uint32_t *iters_out,
+                       uint32_t max_iters) {
+    if (!c || !canon_out || !status_out || (n && (!rows || !meta)) || stride < 1 || stride > c->stride)
+        return ISE_ERR_ARG;
+    if (n > c->capacity) return ISE_ERR_CAPACITY;
+    if (n == 0) return ISE_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_canon(c);
+    if (rc) return rc;
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipMemcpyAsync(c->d_rows, rows, (size_t)n * stride * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->d_meta, meta, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    rc = ise_canonize_batch_device(c, n, stride, c->d_rows, c->d_meta, c->d_canon_out, c->d_canon_status,
+                                   c->d_canon_aut, c->d_canon_iters, max_iters);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(canon_out, c->d_canon_out, (size_t)n * stride * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(status_out, c->d_canon_status, n, hipMemcpyDeviceToHost, s));
+    if (aut_out) HIPCHK(c, hipMemcpyAsync(aut_out, c->d_canon_aut, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    if (iters_out) HIPCHK(c, hipMemcpyAsync(iters_out, c->d_canon_iters, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return ISE_OK;
+}
+
+int ise_canonize_matrix(ise_ctx *c, const uint8_t *mat, uint32_t nrows, uint32_t ncols, uint32_t len_stab,
+                        uint8_t *out, uint64_t *aut, uint32_t *iters) {
+    if (!c || !mat || !out || nrows < 1 || nrows > c->stride || ncols < 1 || ncols > ISE_MAX_COLS ||
+        len_stab >= nrows)
+        return ISE_ERR_ARG;
+    std::vector<uint64_t> rows(c->stride, 0ull), canon(c->stride, 0ull);
+    for (uint32_t i = 0; i < nrows; i++)
+        for (uint32_t q = 0; q < ncols; q++)
+            if (mat[i * ncols + q]) rows[i] |= 1ull << q;
+    const uint32_t meta = nrows | (ncols << 8) | (len_stab << 16);
+    uint8_t status = 0;
+    uint64_t a = 0;
+    uint32_t it = 0;
+    int rc = ise_canonize_batch(c, 1, c->stride, rows.data(), &meta, canon.data(), &status, &a, &it,
+                                ISE_CANONIZE_MAX_ITERS);
+    if (rc) return rc;
+    if (status != ISE_VERDICT_CANONICAL) {
+        c->err = "ise_canonize_matrix: the matrix did not reach its canonical form";
+        return ISE_ERR_STATE;
+    }
+    for (uint32_t i = 0; i < nrows; i++)
+        for (uint32_t q = 0; q < ncols; q++) out[i * ncols + q] = (uint8_t)((canon[i] >> q) & 1ull);
+    if (aut) *aut = a;
+    if (iters) *iters = it;
+    return ISE_OK;
 }
 
 int ise_check_batch_device_noalloc(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rows,

@@ -46,6 +46,7 @@ ORDER_LEARNED = 2   # budgeted witness hunt in a per-candidate order table, then
 
 QUEUE_WORDS = 256   # ISE_QUEUE_WORDS: device words behind a work-queue argument
 NOALLOC_MAX_CANDIDATES = 1 << 20   # ISE_NOALLOC_MAX_CANDIDATES
+CANONIZE_MAX_ITERS = 4096   # ISE_CANONIZE_MAX_ITERS
 
 _lib = None
 
@@ -77,6 +78,10 @@ def lib():
         L.ise_check_batch_device_timed.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P,
                                                    U32, P]
         L.ise_check_batch_device_noalloc.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P]
+        L.ise_canonize_batch.argtypes = [P, U32, U32, P, P, P, P, P, P, U32]
+        L.ise_canonize_batch_device.argtypes = [P, U32, U32, P, P, P, P, P, P, U32]
+        L.ise_improved_solution.argtypes = [P, U32, U32, P, P, P, U32, P, ctypes.POINTER(U32)]
+        L.ise_canonize_matrix.argtypes = [P, P, U32, U32, U32, P, ctypes.POINTER(U64), ctypes.POINTER(U32)]
         L.ise_device_alloc.argtypes = [P, ctypes.POINTER(P), U64]
         L.ise_device_free.argtypes = [P, P]
         L.ise_copy_to_device.argtypes = [P, P, P, U64]
@@ -288,6 +293,36 @@ class Context:
             return alloc_results(0, gen_cap, fields)
         out = {k: np.concatenate([r[k] for r in res_all]) for k in res_all[0]}
         return out
+
+    def canonize_batch(self, rows, meta, max_iters=CANONIZE_MAX_ITERS):
+        """Canonical forms of a batch (CanonizeMatrix with k = 0): dict of
+        canon [n][stride] rows, status (1 reached, 2 iteration limit, 3 invalid),
+        aut (|Aut|, row action) and iters (tests run)."""
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32)
+        n, stride = rows.shape
+        out = {"canon": np.zeros_like(rows), "status": np.zeros(n, np.uint8), "aut": np.zeros(n, np.uint64),
+               "iters": np.zeros(n, np.uint32)}
+        for off in range(0, n, self.capacity):
+            cnt = min(self.capacity, n - off)
+            part = {k: v[off:off + cnt] for k, v in out.items()}
+            tmp = {k: np.ascontiguousarray(v) for k, v in part.items()}
+            self._check(self._L.ise_canonize_batch(self.h, cnt, stride, rows[off:off + cnt].ctypes.data,
+                                                   meta[off:off + cnt].ctypes.data, tmp["canon"].ctypes.data,
+                                                   tmp["status"].ctypes.data, tmp["aut"].ctypes.data,
+                                                   tmp["iters"].ctypes.data, max_iters))
+            for k in out:
+                out[k][off:off + cnt] = tmp[k]
+        return out
+
+    def canonize_matrix(self, mat, len_stab=0):
+        """CanonizeMatrix of one uint8 0/1 matrix: (canonical matrix, |Aut|, tests run)."""
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        out = np.zeros_like(mat)
+        aut, it = ctypes.c_uint64(0), ctypes.c_uint32(0)
+        self._check(self._L.ise_canonize_matrix(self.h, mat.ctypes.data, mat.shape[0], mat.shape[1], len_stab,
+                                                out.ctypes.data, ctypes.byref(aut), ctypes.byref(it)))
+        return out, aut.value, it.value
 
     def solution_orbits(self, group, sols):
         """Canonical (largest) image of every solution under the group, on the
@@ -706,6 +741,27 @@ def random_extensions(v, k, lam, n, dmin, dmax, seed, stride=32):
     if rc != OK:
         raise ISEError(rc, "ise_random_extensions")
     return rows, meta
+
+
+def improved_solution(rows, depth, ncols, perm_row, perm_col, fail_row, cells=None):
+    """reconstructSolution's improved solution from a witness on the last row
+    (host helper of the library); None when fail_row is not the last row."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint64)
+    perm_row = np.ascontiguousarray(perm_row, dtype=np.uint8)
+    perm_col = np.ascontiguousarray(perm_col, dtype=np.uint8)
+    cptr = None
+    if cells is not None:
+        cells = np.ascontiguousarray(cells, dtype=np.uint64)
+        cptr = cells.ctypes.data
+    out = np.zeros(64, np.uint8)
+    n = ctypes.c_uint32(0)
+    rc = lib().ise_improved_solution(rows.ctypes.data, depth, ncols, cptr, perm_row.ctypes.data,
+                                     perm_col.ctypes.data, int(fail_row), out.ctypes.data, ctypes.byref(n))
+    if rc == ERR_ARG:
+        return None
+    if rc != OK:
+        raise ISEError(rc, "ise_improved_solution")
+    return out[:n.value].copy()
 
 
 def kf_random(players, rmin, rmax, count, seed, stride=16):

@@ -117,12 +117,15 @@ int ise_set_order(ise_ctx *ctx, int mode, uint32_t hunt_budget, const uint8_t *d
 
 /* Open a context on HIP device `device` able to hold `capacity` candidates of
  * up to `stride` rows per submit.  Replaces the construction of
- * CGPU_CanonChecker (Source/MatrixCanonChecker.h:204-290) and the CDTools
- * createCanonizer (k-Factorization/Utils/Include/CDTools.h:9). */
+ * CGPU_CanonChecker (Source/MatrixCanonChecker.h:204-290) and of the
+ * per-thread CCanonicityChecker.  (The k-Sys C entry points createCanonizer /
+ * runCanonizer, k-Factorization/Utils/Include/CDTools.h:9-11, canonize a
+ * matrix whose first row holds day labels and may exceed 64 columns; they have
+ * no counterpart here: ise_canonize_* covers 0/1 matrices, ise_kf_check the
+ * k-Sys canonicity test.) */
 int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride);
 
-/* Release the context (CGPU_CanonChecker dtor; CDTools releaseCanonizer,
- * CDTools.h:10). */
+/* Release the context (CGPU_CanonChecker dtor). */
 int ise_close(ise_ctx *ctx);
 
 int ise_set_launch(ise_ctx *ctx, const ise_launch_cfg *cfg);
@@ -150,6 +153,41 @@ int ise_collect(ise_ctx *ctx, ise_results *out_host);
 int ise_test_canonicity(ise_ctx *ctx, const uint8_t *mat, uint32_t nrows, uint32_t ncols,
                         uint32_t len_stab, uint32_t out_info, uint32_t gen_cap,
                         ise_results *out_host);
+
+/* ---- canonical form (CMatrixCanonChecker::CanonizeMatrix with k = 0,
+ * Source/MatrixCanonChecker.cpp:420-564) ------------------------------------
+ * The test with row permutations saved is repeated on the matrix rearranged
+ * by each witness (rows by the witness, then the columns put back in orbit
+ * form: decreasing column order with row 0 most significant, the order
+ * GenerateBinaryColumnOrbits, :186-263, leaves) until the test finds none.
+ * The result is the largest column-sorted form over the row permutations that
+ * fix the first lenStab rows -- for lenStab = 0 the canonical form, the same
+ * for every row and column relabelling of the input.  Per matrix: status 1
+ * (canonical form reached), 2 (max_iters tests did not reach it; the last
+ * matrix is returned), 3 (invalid record); |Aut| of the canonical form (row
+ * action; 0 unless status 1); the tests run.  Matrices use the candidate wire
+ * format (meta = depth | ncols << 8 | lenStab << 16; outInfo ignored). */
+#define ISE_CANONIZE_MAX_ITERS 4096u
+
+/* Device-resident batch (every pointer device memory; d_aut and d_iters may be
+ * NULL), n up to the context capacity and stride up to the context's;
+ * synchronous on the context stream (it reads the number of open matrices
+ * after every round).  Uses context buffers (allocated on the first call). */
+int ise_canonize_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                              const uint32_t *d_meta, uint64_t *d_canon, uint8_t *d_status, uint64_t *d_aut,
+                              uint32_t *d_iters, uint32_t max_iters);
+
+/* The same on host arrays (canon_out [n][stride]; aut_out, iters_out optional). */
+int ise_canonize_batch(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta,
+                       uint64_t *canon_out, uint8_t *status_out, uint64_t *aut_out, uint32_t *iters_out,
+                       uint32_t max_iters);
+
+/* One row-major uint8 0/1 matrix (nrows x ncols), CanonizeMatrix's own
+ * interface: out receives the canonical matrix (same layout), *aut its |Aut|,
+ * *iters the tests run (both optional).  ISE_ERR_STATE if ISE_CANONIZE_MAX_ITERS
+ * tests did not reach it. */
+int ise_canonize_matrix(ise_ctx *ctx, const uint8_t *mat, uint32_t nrows, uint32_t ncols, uint32_t len_stab,
+                        uint8_t *out, uint64_t *aut, uint32_t *iters);
 
 /* Words of device memory behind the work-queue argument of the device entry
  * points: one queue head per XCD, each on its own 128-byte line. */
@@ -507,6 +545,21 @@ int ise_adjust_generators(const uint8_t *gens, uint32_t n_gens, uint32_t len, co
  * to that image, or 0xffffffff if none is listed.  len <= 64. */
 int ise_solution_orbits(ise_ctx *ctx, const uint8_t *group, uint64_t order, uint32_t len, const uint8_t *sols,
                         uint32_t n_sol, uint8_t *canon_out, uint32_t *rep_out);
+
+/* The improved solution of CCanonicityChecker::reconstructSolution
+ * (Source/CanonicityChecker.cpp:227-277), the input the reference passes to
+ * CRowSolution::moveNoncanonicalSolutions (Source/RowSolution.cpp:158-281)
+ * when the last row of a partial matrix makes it non-canonical (fail_row ==
+ * depth - 1): per column orbit of the last row (cells[depth-1], or derived
+ * from the rows when cells is NULL), the ones of row perm_row[depth-1] among
+ * the columns perm_col[] places in that orbit.  perm_row, perm_col: the
+ * witness of the test as reported with the column permutation
+ * (ise_report_columns on, or a non-null perm_col on the device paths).
+ * Writes *n_orbits entries to solution_out (<= 64).  Host only; ISE_ERR_ARG
+ * when fail_row is not the last row or the witness is not a permutation. */
+int ise_improved_solution(const uint64_t *rows, uint32_t depth, uint32_t ncols, const uint64_t *cells,
+                          const uint8_t *perm_row, const uint8_t *perm_col, uint32_t fail_row,
+                          uint8_t *solution_out, uint32_t *n_orbits);
 
 /* ---- text output of results (host) --------------------------------------
  * Both write a NUL-terminated string into buf (cap bytes); *len_out receives

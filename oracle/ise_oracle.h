@@ -76,6 +76,23 @@ int orc_test_canonicity(const uint8_t *mat, int nrows, int ncols, int len_stab,
 /* Column-orbit start masks of a bit-packed matrix: out[i] = orbits of row i. */
 void orc_cell_masks(const uint64_t *rows, int nrows, int ncols, uint64_t *out);
 
+/* CMatrixCanonChecker::CanonizeMatrix for a 0/1 matrix (k = 0,
+ * Source/MatrixCanonChecker.cpp:420-564): columns first put in orbit form
+ * (decreasing column order), then, while the test with row permutations saved
+ * finds a witness, the rows are rearranged by it and the columns put in orbit
+ * form again.  out receives the final (canonical) matrix, *aut its |Aut|,
+ * *iters the tests run.  Returns 1, or 0 when max_iters tests did not
+ * finish, -1 on bad sizes. */
+int orc_canonize(const uint8_t *mat, int nrows, int ncols, int len_stab, int max_iters, uint8_t *out,
+                 uint64_t *aut, uint32_t *iters);
+
+/* The improved solution CCanonicityChecker::reconstructSolution builds when
+ * the test of this matrix (out_info 0) fails on its last row
+ * (CanonicityChecker.cpp:227-277, called at CanonicityChecker.h:382): one
+ * count per column orbit of the last row.  Returns the number of orbits, or -1
+ * when the matrix is canonical or fails on an earlier row. */
+int orc_improved_solution(const uint8_t *mat, int nrows, int ncols, int len_stab, uint8_t *out);
+
 /* Test hook: one step of the row-permutation walk the test runs
  * (CGroupOrder::next_permutation, GroupOrder.h:136-248) on caller-held state.
  * perm[n], orbits[n] (orbit leader of each row); at = 255 for the plain

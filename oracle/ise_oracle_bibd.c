@@ -246,6 +246,32 @@ typedef struct {
 static int run_check_from(const cand_t *cd, orc_results *res, size_t idx, uint32_t gen_cap,
                           const uint32_t *init);
 
+/* The improved solution of the last test that failed on its last row
+ * (CCanonicityChecker::reconstructSolution, CanonicityChecker.cpp:227-277),
+ * captured at the failure: the current solution's entries for the orbits that
+ * compared equal, the deciding orbit's count of the permuted row, then fresh
+ * counts of the permuted row through the column permutation for the orbits
+ * after it. */
+static __thread u8 improved_sol[ORC_MAX_COLS];
+static __thread int improved_len = -1;
+
+static void reconstruct_solution(const cand_t *cd, int row, int deciding, const u8 *prow, const u8 *pcol)
+{
+    const u8 *orig = cd->mat + (size_t)row * cd->ncols;
+    int o = 0;
+    for (; o < deciding; o++) {          /* copied from the current solution */
+        int ones = 0;
+        for (int q = 0; q < cd->olen[row][o]; q++) ones += orig[cd->ostart[row][o] + q];
+        improved_sol[o] = (u8)ones;
+    }
+    for (; o < cd->norb[row]; o++) {     /* counters of the permuted row */
+        int ones = 0;
+        for (int q = 0; q < cd->olen[row][o]; q++) ones += prow[pcol[cd->ostart[row][o] + q]];
+        improved_sol[o] = (u8)ones;
+    }
+    improved_len = cd->norb[row];
+}
+
 static int run_check(const cand_t *cd, orc_results *res, size_t idx, uint32_t gen_cap)
 {
     return run_check_from(cd, res, idx, gen_cap, NULL);
@@ -298,6 +324,8 @@ static int run_check_from(const cand_t *cd, orc_results *res, size_t idx, uint32
                 frow = row;
                 fcol = cd->ostart[row][o];
                 rtc = row_to_change(perm, row);
+                if (row == d - 1)
+                    reconstruct_solution(cd, row, o, prow, pcol);
                 goto done;
             }
         }
@@ -399,6 +427,19 @@ int orc_check_batch(uint32_t n, uint32_t stride, const uint64_t *rows,
     return 0;
 }
 
+int orc_improved_solution(const uint8_t *mat, int nrows, int ncols, int len_stab, uint8_t *out)
+{
+    uint8_t verdict = 1;
+    orc_results res;
+    memset(&res, 0, sizeof res);
+    res.verdict = &verdict;
+    improved_len = -1;
+    if (orc_test_canonicity(mat, nrows, ncols, len_stab, 0, &res, 0) < 0) return -1;
+    if (verdict != 0 || improved_len < 0) return -1;
+    memcpy(out, improved_sol, (size_t)improved_len);
+    return improved_len;
+}
+
 int orc_test_canonicity(const uint8_t *mat, int nrows, int ncols, int len_stab,
                         unsigned out_info, orc_results *res, uint32_t gen_cap)
 {
@@ -417,6 +458,83 @@ int orc_test_canonicity(const uint8_t *mat, int nrows, int ncols, int len_stab,
     int v = run_check(cd, res, 0, gen_cap);
     free(cd);
     return v;
+}
+
+/* Column order of GenerateBinaryColumnOrbits (MatrixCanonChecker.cpp:186-263)
+ * applied to every row in turn: inside each orbit of row i the ones are moved
+ * first (the first zero swapped with the last one, the swap carried through
+ * the rows below), which leaves the columns in decreasing order. */
+static void orbit_form(u8 *mat, int nrows, int ncols)
+{
+    u8 start[ORC_MAX_COLS + 1];
+    int nst = 1;
+    start[0] = 0;
+    for (int i = 0; i < nrows; i++) {
+        u8 nstart[ORC_MAX_COLS + 1];
+        int nn = 0;
+        for (int o = 0; o < nst; o++) {
+            const int lo = start[o], hi = o + 1 < nst ? start[o + 1] : ncols;
+            int ones = 0;
+            for (int q = lo; q < hi; q++) ones += mat[i * ncols + q];
+            if (ones > 0 && ones < hi - lo) {
+                const int jlast = lo + ones;
+                int j1 = lo, j2 = hi;
+                while (j1 < jlast) {
+                    while (mat[i * ncols + j1]) j1++;       /* first zero */
+                    if (j1 == jlast) break;
+                    while (!mat[i * ncols + --j2]) {}       /* last one   */
+                    for (int r = i; r < nrows; r++) {       /* swap the two columns */
+                        u8 t = mat[r * ncols + j1];
+                        mat[r * ncols + j1] = mat[r * ncols + j2];
+                        mat[r * ncols + j2] = t;
+                    }
+                    j1++;
+                }
+            }
+            nstart[nn++] = (u8)lo;
+            if (ones > 0 && ones < hi - lo) nstart[nn++] = (u8)(lo + ones);
+        }
+        memcpy(start, nstart, (size_t)nn);
+        nst = nn;
+    }
+}
+
+int orc_canonize(const uint8_t *mat, int nrows, int ncols, int len_stab, int max_iters, uint8_t *out,
+                 uint64_t *aut, uint32_t *iters)
+{
+    if (nrows < 1 || nrows > ORC_MAX_ROWS || ncols < 1 || ncols > ORC_MAX_COLS || len_stab < 0 ||
+        len_stab >= nrows)
+        return -1;
+    u8 cur[ORC_MAX_ROWS * ORC_MAX_COLS], tmp[ORC_MAX_ROWS * ORC_MAX_COLS];
+    memcpy(cur, mat, (size_t)(nrows * ncols));
+    orbit_form(cur, nrows, ncols);
+    uint32_t it = 0;
+    int done = 0;
+    while (it < (uint32_t)max_iters) {
+        uint8_t verdict = 0, prow[ORC_MAX_ROWS];
+        uint64_t order = 0;
+        orc_results res;
+        memset(&res, 0, sizeof res);
+        res.verdict = &verdict;
+        res.group_order = &order;
+        res.perm_row = prow;
+        orc_test_canonicity(cur, nrows, ncols, len_stab, ORC_SAVE_ROW_PERMUTATIONS, &res, 0);
+        it++;
+        if (verdict == 1) {
+            if (aut) *aut = order;
+            done = 1;
+            break;
+        }
+        /* rows rearranged by the witness: row i of the new matrix is row prow[i] */
+        for (int i = 0; i < nrows; i++)
+            memcpy(tmp + i * ncols, cur + prow[i] * ncols, (size_t)ncols);
+        memcpy(cur, tmp, (size_t)(nrows * ncols));
+        orbit_form(cur, nrows, ncols);
+    }
+    memcpy(out, cur, (size_t)(nrows * ncols));
+    if (iters) *iters = it;
+    if (!done && aut) *aut = 0;
+    return done;
 }
 
 /* ------------------------------------------------------------------------ */

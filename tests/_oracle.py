@@ -33,6 +33,8 @@ def load():
         L.orc_test_canonicity.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, U32]
         L.orc_cell_masks.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
         L.orc_walk_step.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
+        L.orc_improved_solution.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
+        L.orc_canonize.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.orc_enumerate_bibd.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_long,
                                          _P, _P, _P, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_enumerate_bibd.restype = ctypes.c_long
@@ -105,6 +107,25 @@ class Oracle:
         position or 255."""
         return self.L.orc_walk_step(perm.ctypes.data, orbits.ctypes.data, len(perm), at, fixed,
                                     state.ctypes.data)
+
+    def improved_solution(self, mat, len_stab=0):
+        """reconstructSolution's vector when the test fails on the last row, else None."""
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        out = np.zeros(64, np.uint8)
+        n = self.L.orc_improved_solution(mat.ctypes.data, mat.shape[0], mat.shape[1], len_stab, out.ctypes.data)
+        return None if n < 0 else out[:n].copy()
+
+    def canonize(self, mat, len_stab=0, max_iters=1 << 16):
+        """CanonizeMatrix (k = 0): (canonical matrix, |Aut|, tests run)."""
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        out = np.zeros_like(mat)
+        aut = np.zeros(1, np.uint64)
+        it = np.zeros(1, np.uint32)
+        rc = self.L.orc_canonize(mat.ctypes.data, mat.shape[0], mat.shape[1], len_stab, max_iters,
+                                 out.ctypes.data, aut.ctypes.data, it.ctypes.data)
+        if rc != 1:
+            raise ValueError(f"oracle canonize failed ({rc})")
+        return out, int(aut[0]), int(it[0])
 
     def cell_masks(self, rows, nrows, ncols):
         rows = np.ascontiguousarray(rows, dtype=np.uint64)
