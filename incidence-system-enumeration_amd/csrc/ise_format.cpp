@@ -101,3 +101,72 @@ int ise_format_aut_table(int v, int b, int r, int k, int lambda, uint32_t n, con
 }
 
 } // extern "C"
+
+// ---- 1-factorizations as the k-Sys result files print them ----------------
+// (TableAut::name + Table::printTable + outMatrix,
+// k-Factorization/TripleSys/Include/Table.h:5-41, 93-115, 333-361, with the
+// cycle statistics of alldata::matrixStatOutput / collectCyclesAndPath,
+// TripleSys/Source/p1fCheck.cpp:77-118, 499-517).  Per matrix:
+//   "\n\n%5d: |Aut(M)| =  <aut>,  Cycles:<sets>\n"
+// where every pair of rounds contributes its cycle type (the lengths of the
+// cycles of the union of the two perfect matchings, ascending) and the sets are
+// "<count>(<l1>:<l2>...)" in increasing order of their length lists, then one
+// line per round, ' "' + for every pair " %3d%3d" + ' "', then an empty line.
+extern "C" int ise_kf_format_matrix(const uint64_t *rounds, int nrounds, int players, uint64_t number, uint64_t aut,
+                                    char *buf, uint64_t cap, uint64_t *len_out) {
+    if (!rounds || players < 4 || players > 16 || (players & 1) || nrounds < 1 || nrounds > players - 1)
+        return ISE_ERR_ARG;
+    const int n = players;
+    std::vector<std::array<uint8_t, 16>> part(nrounds);
+    for (int r = 0; r < nrounds; r++) {
+        uint32_t seen = 0;
+        for (int p = 0; p < n; p += 2) {
+            const int x = (int)(rounds[r] >> (4 * p)) & 0xf, y = (int)(rounds[r] >> (4 * p + 4)) & 0xf;
+            if (x >= n || y >= n || x == y || (seen >> x & 1) || (seen >> y & 1)) return ISE_ERR_ARG;
+            seen |= (1u << x) | (1u << y);
+            part[r][x] = (uint8_t)y;
+            part[r][y] = (uint8_t)x;
+        }
+    }
+    // cycle type of every pair of rounds, counted per type
+    std::map<std::vector<int>, uint64_t> types;
+    for (int m = 1; m < nrounds; m++)
+        for (int i = 0; i < m; i++) {
+            uint32_t used = 0;
+            std::vector<int> lens;
+            for (int s = 0; s < n; s++) {
+                if (used >> s & 1) continue;
+                int v = s, len = 0;
+                do {
+                    const int w = part[i][v];
+                    used |= (1u << v) | (1u << w);
+                    len += 2;
+                    v = part[m][w];
+                } while (v != s && len <= n);
+                lens.push_back(len);
+            }
+            std::sort(lens.begin(), lens.end());
+            types[lens]++;
+        }
+    std::string head = fmt("%5llu: |Aut(M)| = ", number) + fmt(" %llu", aut) + ",  Cycles:";
+    bool first = true;
+    for (const auto &t : types) {
+        if (!first) head += " ";
+        first = false;
+        head += fmt("%llu(", t.second);
+        for (size_t j = 0; j < t.first.size(); j++) head += (j ? ":" : "") + fmt("%llu", (unsigned long long)t.first[j]);
+        head += ")";
+    }
+    Out o{buf, cap};
+    o.put("\n\n" + head + "\n");
+    for (int r = 0; r < nrounds; r++) {
+        std::string line = " \"";
+        for (int p = 0; p < n; p++) {
+            if (!(p & 1)) line += " ";
+            line += fmt(" %2llu", (rounds[r] >> (4 * p)) & 0xf);
+        }
+        o.put(line + " \"\n");
+    }
+    o.put("\n");
+    return o.finish(len_out);
+}

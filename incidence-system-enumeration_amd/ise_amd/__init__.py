@@ -132,6 +132,7 @@ def lib():
         L.ise_enumerate_bibd_device.argtypes = [P, I, I, I, P, P, ctypes.c_long, P, P]
         L.ise_enumerate_bibd_device.restype = ctypes.c_long
         L.ise_format_matrix.argtypes = [P, I, I, U64, U64, I, U64, P, U64, ctypes.POINTER(U64)]
+        L.ise_kf_format_matrix.argtypes = [P, I, I, U64, U64, P, U64, ctypes.POINTER(U64)]
         L.ise_format_aut_table.argtypes = [I, I, I, I, I, U32, P, P, P, P, U64, ctypes.POINTER(U64)]
         L.ise_kf_check.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
         L.ise_kf_check_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P, P]
@@ -762,6 +763,19 @@ def improved_solution(rows, depth, ncols, perm_row, perm_col, fail_row, cells=No
     if rc != OK:
         raise ISEError(rc, "ise_improved_solution")
     return out[:n.value].copy()
+
+
+def kf_format_matrix(mat, number, aut):
+    """A 1-factorization (rounds x players) as the k-Sys result files print it."""
+    rows, meta = kf_pack([mat])
+    nr, n = int(meta[0]) & 0xff, (int(meta[0]) >> 8) & 0xff
+    need = ctypes.c_uint64(0)
+    lib().ise_kf_format_matrix(rows.ctypes.data, nr, n, number, aut, None, 0, ctypes.byref(need))
+    buf = ctypes.create_string_buffer(need.value + 1)
+    rc = lib().ise_kf_format_matrix(rows.ctypes.data, nr, n, number, aut, buf, need.value + 1, ctypes.byref(need))
+    if rc != OK:
+        raise ISEError(rc, "ise_kf_format_matrix")
+    return buf.value.decode()
 
 
 def kf_random(players, rmin, rmax, count, seed, stride=16):

@@ -582,6 +582,17 @@ int ise_format_matrix(const uint64_t *rows, int nrows, int ncols, uint64_t numbe
 int ise_format_aut_table(int v, int b, int r, int k, int lambda, uint32_t n, const uint64_t *designs,
                          const uint64_t *auts, const uint8_t *orbits, char *buf, uint64_t cap, uint64_t *len_out);
 
+/* One 1-factorization as the k-Sys result files print it (K_OrderedMatrices.txt,
+ * TableAut::name / Table::printTable / outMatrix, k-Factorization/TripleSys/
+ * Include/Table.h:5-41, 93-115, 333-361): "\n\n%5llu: |Aut(M)| =  <aut>,
+ * Cycles:<sets>" with the cycle types of every pair of rounds counted
+ * (matrixStatOutput / collectCyclesAndPath, TripleSys/Source/p1fCheck.cpp:
+ * 77-118, 499-517), then one quoted line per round and an empty line.  rounds
+ * in the ise_kf_check wire format (nibble p of word r = player at position p).
+ * Same buffer convention as the two formatters above. */
+int ise_kf_format_matrix(const uint64_t *rounds, int nrounds, int players, uint64_t number, uint64_t aut,
+                         char *buf, uint64_t cap, uint64_t *len_out);
+
 /* ---- host-side candidate producers (stay on the CPU) -------------------- */
 
 /* Level-synchronous enumeration of 2-(v,k,lambda) designs: every level's

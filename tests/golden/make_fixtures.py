@@ -188,7 +188,34 @@ def text_blocks():
                 i = j
             else:
                 i += 1
-    return dict(tables=tables, matrices=dict(source=os.path.relpath(mpath, REF), blocks=blocks))
+    return dict(tables=tables, matrices=dict(source=os.path.relpath(mpath, REF), blocks=blocks),
+                kfact=kfact_blocks())
+
+
+def kfact_blocks():
+    """Verbatim 1-factorization blocks ('<i>: |Aut(M)| = a,  Cycles:...' and the
+    quoted rows) of the k-Sys result files: the first 60 of K_10 and every
+    block of the perfect and uniform files."""
+    out = []
+    for n, rel, limit in ((10, "10/10x9x2/K_OrderedMatrices.txt", 60), (10, "10/10x9x2/P_OrderedMatrices.txt", None),
+                          (14, "14/14x13x2/P_OrderedMatrices.txt", None), (8, "8/8x7x2_44/U_OrderedMatrices.txt", None),
+                          (10, "10/10x9x2_46/U_OrderedMatrices.txt", None),
+                          (12, "12/12x11x2_66/U_OrderedMatrices.txt", None)):
+        path = os.path.join(REF, "k-Factorization/Results/Complete_graphs", rel)
+        if not os.path.exists(path):
+            continue
+        lines = read(path).splitlines()
+        blocks = []
+        for i, t in enumerate(lines):
+            if re.match(r"\s*\d+: \|Aut\(M\)\| =", t):
+                j = i + 1
+                while j < len(lines) and lines[j].startswith(' "'):
+                    j += 1
+                blocks.append(lines[i:j])
+                if limit and len(blocks) >= limit:
+                    break
+        out.append(dict(n=n, source=os.path.relpath(path, REF), blocks=blocks))
+    return out
 
 
 def main():

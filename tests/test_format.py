@@ -74,3 +74,20 @@ def test_aut_table_of_oracle_enumeration_matches_reference_file(oracle, name):
     res = oracle.check_batch(rows, meta)
     text = ise_amd.format_aut_table(v, k, lam, designs, auts, res["orbits"])
     assert text.splitlines() == want
+
+
+def test_kfact_blocks_verbatim():
+    """1-factorizations printed exactly as the k-Sys result files print them:
+    the first 60 of the 396 K_10 ones, the perfect ones of K_10 and K_14 and
+    the uniform ones of K_8, K_10 and K_12 -- header with |Aut(M)| and the
+    cycle-type counts, and the quoted rows."""
+    n_blocks = 0
+    for f in golden("format_golden.json")["kfact"]:
+        for blk in f["blocks"]:
+            head = re.match(r"\s*(\d+): \|Aut\(M\)\| =\s+(\d+)", blk[0])
+            number, aut = int(head.group(1)), int(head.group(2))
+            mat = [[int(x) for x in line.strip().strip('"').split()] for line in blk[1:]]
+            text = ise_amd.kf_format_matrix(np.array(mat, np.uint8), number, aut)
+            assert text == "\n\n" + "\n".join(blk) + "\n\n", (f["source"], number)
+            n_blocks += 1
+    assert n_blocks >= 80
