@@ -28,6 +28,7 @@ struct KfTables {
 struct KfOut {
     uint8_t *verdict;
     uint64_t *witness, *n_auts, *n_tried;
+    uint16_t *player_index = nullptr;
 };
 
 // Split launch: after the first pass the relabellings of the candidates still
@@ -84,5 +85,10 @@ hipError_t kf2_launch(uint32_t blocks, uint32_t ncand, uint32_t stride, const ui
 hipError_t kf_u1f_launch(uint32_t blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
                          const uint32_t *meta, const KfOut &o, hipStream_t stream);
 hipError_t kf_occupancy(int *blocks_per_cu);
+// The k-Sys backjump position of every candidate (o.player_index) from the
+// first witness the canonizer of `mode` (0 general, 1 two-row P1F, 2 two-row
+// U1F) wrote to o.witness; one lane per candidate.
+hipError_t kf_backjump_launch(int mode, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
+                              const uint32_t *meta, const KfOut &o, const KfTables &qp, hipStream_t stream);
 
 } // namespace ise

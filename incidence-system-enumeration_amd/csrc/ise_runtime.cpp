@@ -834,6 +834,7 @@ int ise_copy_on_device(ise_ctx *c, void *d_dst, const void *d_src, uint64_t byte
 static ise::KfOut kf_out(const ise_kf_results &r) {
     ise::KfOut o;
     o.verdict = r.verdict; o.witness = r.witness; o.n_auts = r.n_auts; o.n_tried = r.n_tried;
+    o.player_index = r.player_index;
     return o;
 }
 
@@ -859,6 +860,7 @@ static int kf_launch_on(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t 
                           c->d_kf_best, c->d_kf_ranges, (uint32_t)c->kf_range_words, c->kf_max_ranges};
     HIPCHK(c, ise::kf_launch(blocks, range_blocks, n, stride, d_rounds, d_meta, kf_out(d_out), d_queue,
                              c->kf_tab, limit, sp, s));
+    HIPCHK(c, ise::kf_backjump_launch(0, n, stride, d_rounds, d_meta, kf_out(d_out), c->kf_tab, s));
     return ISE_OK;
 }
 
@@ -867,6 +869,7 @@ int ise_kf_check_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uin
                               void *stream) {
     if (!c || !d_out || !d_out->verdict || !d_queue || stride < 1 || stride > ISE_MAX_ROWS)
         return ISE_ERR_ARG;
+    if (d_out->player_index && !d_out->witness) return ISE_ERR_ARG;
     if (n == 0) return ISE_OK;
     if (!d_rounds || !d_meta) return ISE_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
@@ -913,12 +916,15 @@ static int kf2_launch_on(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t
         HIPCHK(c, ise::kf_u1f_launch(blocks, n, stride, d_rounds, d_meta, kf_out(d_out), s));
     else
         HIPCHK(c, ise::kf2_launch(blocks, n, stride, d_rounds, d_meta, kf_out(d_out), s));
+    HIPCHK(c, ise::kf_backjump_launch(mode == KF_2U1F ? 2 : 1, n, stride, d_rounds, d_meta, kf_out(d_out),
+                                      c->kf_tab, s));
     return ISE_OK;
 }
 
 int ise_kf_check_2p1f_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
                                    const uint32_t *d_meta, const ise_kf_results *d_out, void *stream) {
     if (!c || !d_out || !d_out->verdict || stride < 1 || stride > ISE_MAX_ROWS) return ISE_ERR_ARG;
+    if (d_out->player_index && !d_out->witness) return ISE_ERR_ARG;
     if (n == 0) return ISE_OK;
     if (!d_rounds || !d_meta) return ISE_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
@@ -928,6 +934,7 @@ int ise_kf_check_2p1f_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, cons
 int ise_kf_check_2u1f_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
                                    const uint32_t *d_meta, const ise_kf_results *d_out, void *stream) {
     if (!c || !d_out || !d_out->verdict || stride < 1 || stride > ISE_MAX_ROWS) return ISE_ERR_ARG;
+    if (d_out->player_index && !d_out->witness) return ISE_ERR_ARG;
     if (n == 0) return ISE_OK;
     if (!d_rounds || !d_meta) return ISE_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
@@ -1024,7 +1031,10 @@ static int kf_host_run(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpyAsync(c->d_rows, rounds, (size_t)n * stride * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_meta, meta, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-    ise_kf_results d{c->d.verdict, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried};
+    // the context's result buffers; the backjump position goes to the fail_row
+    // array reinterpreted (one uint16 per candidate fits in perm_row's 32 bytes)
+    ise_kf_results d{c->d.verdict, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried,
+                     h->player_index ? (uint16_t *)c->d.perm_row : nullptr};
     const int rc = mode != KF_GENERAL ? kf2_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->stream, mode)
                                       : kf_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->d_head, c->stream);
     if (rc) return rc;
@@ -1032,6 +1042,8 @@ static int kf_host_run(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *
     if (h->witness) HIPCHK(c, hipMemcpyAsync(h->witness, d.witness, n * 8, hipMemcpyDeviceToHost, c->stream));
     if (h->n_auts) HIPCHK(c, hipMemcpyAsync(h->n_auts, d.n_auts, n * 8, hipMemcpyDeviceToHost, c->stream));
     if (h->n_tried) HIPCHK(c, hipMemcpyAsync(h->n_tried, d.n_tried, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (h->player_index)
+        HIPCHK(c, hipMemcpyAsync(h->player_index, d.player_index, n * 2, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ISE_OK;
 }

@@ -155,20 +155,23 @@ def lib():
 
 class KfResultsStruct(ctypes.Structure):
     _fields_ = [("verdict", ctypes.c_void_p), ("witness", ctypes.c_void_p), ("n_auts", ctypes.c_void_p),
-                ("n_tried", ctypes.c_void_p)]
+                ("n_tried", ctypes.c_void_p), ("player_index", ctypes.c_void_p)]
 
 
 KF_FIELDS = ("verdict", "witness", "n_auts", "n_tried")
 
 
-def kf_alloc(n):
-    return {"verdict": np.zeros(n, np.uint8), "witness": np.zeros(n, np.uint64),
-            "n_auts": np.zeros(n, np.uint64), "n_tried": np.zeros(n, np.uint64)}
+def kf_alloc(n, backjump=False):
+    out = {"verdict": np.zeros(n, np.uint8), "witness": np.zeros(n, np.uint64),
+           "n_auts": np.zeros(n, np.uint64), "n_tried": np.zeros(n, np.uint64)}
+    if backjump:
+        out["player_index"] = np.zeros(n, np.uint16)
+    return out
 
 
 def kf_struct(arrays, device=False):
     s = KfResultsStruct()
-    for k in KF_FIELDS:
+    for k in KF_FIELDS + ("player_index",):
         a = arrays.get(k)
         setattr(s, k, None if a is None else (a.ptr if device else a.ctypes.data))
     return s
@@ -413,10 +416,11 @@ class Context:
         (first_chunks=0: one wave per candidate to its decision)."""
         self._check(self._L.ise_kf_set_split(self.h, first_chunks, max_ranges))
 
-    def kf_check(self, rounds, meta, two_row_p1f=False, two_row_u1f=False):
+    def kf_check(self, rounds, meta, two_row_p1f=False, two_row_u1f=False, backjump=False):
         """Canonicity of partial 1-factorizations (host arrays in and out);
         two_row_p1f / two_row_u1f: the two-row canonizers of perfect / uniform
-        1-factorizations."""
+        1-factorizations; backjump: also the k-Sys backjump position
+        ("player_index")."""
         fn = (self._L.ise_kf_check_2p1f if two_row_p1f else
               self._L.ise_kf_check_2u1f if two_row_u1f else self._L.ise_kf_check)
         rounds = np.ascontiguousarray(rounds, dtype=np.uint64)
@@ -425,14 +429,14 @@ class Context:
         outs = []
         for off in range(0, n, self.capacity):
             cnt = min(self.capacity, n - off)
-            r = kf_alloc(cnt)
+            r = kf_alloc(cnt, backjump)
             s = kf_struct(r)
             self._check(fn(self.h, cnt, stride, rounds[off:off + cnt].ctypes.data,
                            meta[off:off + cnt].ctypes.data, ctypes.byref(s)))
             outs.append(r)
         if not outs:
-            return kf_alloc(0)
-        return {k: np.concatenate([o[k] for o in outs]) for k in KF_FIELDS}
+            return kf_alloc(0, backjump)
+        return {k: np.concatenate([o[k] for o in outs]) for k in outs[0]}
 
     def kf_device_results(self, n):
         dev = {"verdict": self.device_array((n,), np.uint8), "witness": self.device_array((n,), np.uint64),

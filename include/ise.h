@@ -408,6 +408,14 @@ typedef struct {
     uint64_t *n_auts;   /* [n] relabellings that reproduce the matrix, counted up to the
                            decision (|Aut| when canonical)                       */
     uint64_t *n_tried;  /* [n] relabellings evaluated up to the decision          */
+    uint16_t *player_index; /* [n] optional (needs witness): the k-Sys backjump position
+                           round * players + slot (alldata::setPlayerIndex,
+                           kmSupport.cpp:303-356): set at the first smaller row of the
+                           witness's normal form unless an earlier row of it came from
+                           a later round; nrounds * players - 3 (cnv.cpp:213) when
+                           canonical or kept; 0xffff for an invalid record.  Every
+                           matrix agreeing on the rounds before that round and on its
+                           slots up to that one is not canonical either.          */
 } ise_kf_results;
 
 /* Host arrays in and out, synchronous (cnvCheckNew of one matrix per record,

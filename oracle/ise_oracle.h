@@ -195,6 +195,13 @@ int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
  * each byte the base label 2*slot); returns k!. */
 long orc_kf_quickperm(int k, uint8_t *out);
 
+/* k-Sys backjump position (setPlayerIndex, kmSupport.cpp:314-356) of the first
+ * smaller relabelling found by canonizer `mode` (0 general, 1 two-row P1F,
+ * 2 two-row U1F); nrows * n - 3 when there is none.  info (optional, 3 ints):
+ * first smaller row of the normal form, the round it came from, 1 if the
+ * position was set there. */
+int orc_kf_player_index(const uint8_t *mat, int n, int nrounds, int mode, int *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,9 +107,21 @@ void orc_kf_normalize(uint8_t *mat, int n, int nrounds)
     memcpy(mat, out, (size_t)(n * nrounds));
 }
 
+/* The relabelling (player -> label) of the last witness found, for the
+ * player index below. */
+static __thread u8 witness_tr[KF_MAX_N];
+static __thread int witness_valid;
+
+static void keep_witness(const u8 *tr, int n)
+{
+    memcpy(witness_tr, tr, (size_t)n);
+    witness_valid = 1;
+}
+
 int orc_kf_check(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
                  uint64_t *n_auts, uint64_t *n_tried)
 {
+    witness_valid = 0;
     if (n < 4 || n > KF_MAX_N || (n & 1) || nrounds < 1 || nrounds > n - 1) return -1;
     u8 part[KF_MAX_N][KF_MAX_N];
     if (!kf_partners(mat, n, nrounds, part)) return -1;
@@ -141,6 +153,7 @@ int orc_kf_check(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
                 else if (cmp < 0) {
                     verdict = 0;
                     if (witness) *witness = idx;
+                    keep_witness(ttr, n);
                     break;
                 }
             }
@@ -219,6 +232,7 @@ int orc_kf_check_2p1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
     for (int a = 0; a < nrounds; a++)
         for (int b = a + 1; b < nrounds; b++)
             if (!kf_is_p1f_pair(part[a], part[b], n)) return -1;
+    witness_valid = 0;
     uint64_t auts = 0, tried = 0, idx = 0;
     int verdict = 1;
     if (witness) *witness = ~0ull;
@@ -268,6 +282,7 @@ int orc_kf_check_2p1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
                     else if (cmp < 0) {
                         verdict = 0;
                         if (witness) *witness = idx;
+                        keep_witness(tr, n);
                         break;
                     }
                 }
@@ -354,6 +369,7 @@ int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
     if (!kf_partners(mat, n, nrounds, part)) return -1;
     uint64_t auts = 0, tried = 0, idx = 0;
     int verdict = 1;
+    witness_valid = 0;
     if (witness) *witness = ~0ull;
     if (nrounds >= 2) {
         kf_cycles ref;
@@ -418,6 +434,7 @@ int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
                         else if (cmp < 0) {
                             verdict = 0;
                             if (witness) *witness = idx;
+                            keep_witness(tr, n);
                             break;
                         }
                     }
@@ -430,3 +447,67 @@ int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
 
 /* the QuickPerm visiting order (for tests) */
 long orc_kf_quickperm(int k, uint8_t *out) { return quickperm_table(k, out); }
+
+/*
+ * The k-Sys backjump position of a test that found a smaller relabelling:
+ * kmProcessMatrix's comparison (kmSupport.cpp:275-313) of the witness's normal
+ * form with the matrix, and setPlayerIndex / setPlayerIndexByPos
+ * (kmSupport.cpp:314-356) at the first smaller row; the position starts at
+ * nrows * n - 3 (cnvCheckNew, cnv.cpp:213: nrows * players - groupSize - 1)
+ * and only ever decreases.  mode 0: general canonizer, 1: two-row P1F,
+ * 2: two-row U1F (each run first, the witness relabelling taken from it).
+ * info (optional, 3 ints): the first smaller row of the normal form, the round
+ * it came from, and whether the position was set there (1) or kept (0).
+ */
+int orc_kf_player_index(const uint8_t *mat, int n, int nrounds, int mode, int *info)
+{
+    uint64_t w, a, t;
+    const int v = mode == 1 ? orc_kf_check_2p1f(mat, n, nrounds, &w, &a, &t)
+                : mode == 2 ? orc_kf_check_2u1f(mat, n, nrounds, &w, &a, &t)
+                            : orc_kf_check(mat, n, nrounds, &w, &a, &t);
+    const int start = nrounds * n - 2 - 1;
+    if (info) info[0] = info[1] = -1, info[2] = 0;
+    if (v < 0) return -1;
+    if (v == 1 || !witness_valid) return start;
+    /* the witness's rounds, relabelled and normalised, with their sources */
+    u8 rel[KF_MAX_N][KF_MAX_N];
+    int src_of_key[KF_MAX_N];
+    for (int q = 0; q < KF_MAX_N; q++) src_of_key[q] = -1;
+    for (int r = 0; r < nrounds; r++) {
+        u8 one[KF_MAX_N];
+        normal_form(mat + r * n, 1, n, witness_tr, one);
+        memcpy(rel[r], one, (size_t)n);
+        src_of_key[one[1]] = r;          /* the partner of label 0 orders the rounds */
+    }
+    int day_max = -1, row = 0;
+    for (int key = 0; key < n; key++) {
+        const int r = src_of_key[key];
+        if (r < 0) continue;
+        const int c = memcmp(rel[r], mat + row * n, (size_t)n);
+        if (day_max < 0) day_max = r;
+        if (c > 0) return start;         /* not smaller: cannot happen for a witness */
+        if (c == 0) {
+            if (day_max < r) day_max = r;
+            row++;
+            continue;
+        }
+        if (info) info[0] = row, info[1] = r;
+        if (day_max > r) return start;   /* an earlier row came from a later round */
+        /* first position where the relabelled row is smaller (at most n - 3) */
+        const u8 *co = rel[r], *ci = mat + row * n;
+        int ip = 0;
+        while (ip < n - 3 && !(co[ip] < ci[ip])) ip++;
+        u8 player_of[KF_MAX_N], pos_in_src[KF_MAX_N];
+        for (int q = 0; q < n; q++) {
+            player_of[witness_tr[q]] = (u8)q;
+            pos_in_src[mat[r * n + q]] = (u8)q;
+        }
+        int j = 0;
+        for (int q = 0; q <= ip; q++)
+            if (j < pos_in_src[player_of[co[q]]]) j = pos_in_src[player_of[co[q]]];
+        const int idx = r * n + j;
+        if (info) info[2] = 1;
+        return idx < start ? idx : start;
+    }
+    return start;
+}

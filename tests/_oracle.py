@@ -53,6 +53,7 @@ def load():
         L.orc_kf_check_2u1f.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.orc_kf_normalize.argtypes = [_P, ctypes.c_int, ctypes.c_int]
         L.orc_kf_bruteforce.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
+        L.orc_kf_player_index.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
         L.orc_kf_quickperm.argtypes = [ctypes.c_int, _P]
         L.orc_kf_quickperm.restype = ctypes.c_long
         L.orc_mlp_features.argtypes = [U32, U32, _P, _P, U32, _P]
@@ -243,6 +244,15 @@ class Oracle:
         if v < 0:
             raise ValueError("oracle rejected the factorization")
         return v, int(w[0]), int(a[0]), int(t[0])
+
+    def kf_player_index(self, mat, mode=0):
+        """(backjump position, first smaller row, its source round, set?)."""
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        info = np.zeros(3, np.int32)
+        pi = self.L.orc_kf_player_index(mat.ctypes.data, mat.shape[1], mat.shape[0], mode, info.ctypes.data)
+        if pi < 0:
+            raise ValueError("oracle rejected the factorization")
+        return pi, int(info[0]), int(info[1]), bool(info[2])
 
     def kf_bruteforce(self, mat):
         mat = np.ascontiguousarray(mat, dtype=np.uint8)
