@@ -21,7 +21,9 @@
 #include <stdint.h>
 #include <string.h>
 #include <algorithm>
+#include <string>
 #include <thread>
+#include <unordered_set>
 #include <vector>
 #include "ise_internal.h"
 
@@ -136,9 +138,60 @@ unsigned host_threads() {
     return t > 16 ? 16 : t;
 }
 
-// Children of every canonical parent (rows packed with `stride` words each).
+// Solution-orbit pruning (CRowSolution::sortSolutionsByGroup,
+// Source/RowSolution.h:325-383, with CPermutStorage::constructGroup /
+// findSolutionIndex, Source/PermutStorage.h:183-325): the automorphisms of a
+// canonical parent permute the column orbits of its next row, so a solution
+// (ones per orbit) and its images give isomorphic children with the same
+// first rows, and among those only the child whose solution is the
+// lexicographically largest image can be canonical.  The other solutions are
+// dropped before their children are built.  gens: n_gens permutations of the
+// next row's orbits (entry o = the orbit that orbit o goes to); the orbit of a
+// solution is walked breadth-first under them (image[o] = sol[g[o]]).
+constexpr uint32_t PRUNE_GEN_CAP = 32;
+
+struct ParentGroup {
+    uint32_t n_gens = 0;
+    const uint8_t *gens = nullptr;   // [n_gens][ISE_MAX_COLS]
+};
+
+bool largest_in_orbit(const std::string &sol, const ParentGroup &g) {
+    if (!g.n_gens) return true;
+    std::unordered_set<std::string> seen{sol};
+    std::vector<std::string> todo{sol};
+    std::string img(sol.size(), '\0');
+    while (!todo.empty()) {
+        const std::string x = todo.back();
+        todo.pop_back();
+        for (uint32_t q = 0; q < g.n_gens; q++) {
+            const uint8_t *perm = g.gens + (size_t)q * ISE_MAX_COLS;
+            for (size_t o = 0; o < x.size(); o++) img[o] = perm[o] < x.size() ? x[perm[o]] : x[o];
+            if (img > sol) return false;
+            if (seen.insert(img).second) todo.push_back(img);
+        }
+    }
+    return true;
+}
+
+// Ones of `row` per column orbit of row `depth` of the parent.
+std::string solution_counts(const uint64_t *parent, int depth, int b, uint64_t row) {
+    uint64_t starts = 1;
+    for (int i = 0; i < depth; i++) starts |= parent[i] ^ (parent[i] << 1);
+    starts &= low_bits(b);
+    std::string out;
+    for (uint64_t m = starts; m; m &= m - 1) {
+        const int s = __builtin_ctzll(m);
+        const uint64_t rest = m & (m - 1);
+        const int e = rest ? __builtin_ctzll(rest) : b;
+        out.push_back((char)__builtin_popcountll(row & (low_bits(e - s) << s)));
+    }
+    return out;
+}
+
+// Children of every canonical parent (rows packed with `stride` words each);
+// with groups (one per parent), only the children of orbit-largest solutions.
 void extend_level(const Design &ds, const std::vector<uint64_t> &parents, int depth, uint32_t stride,
-                  std::vector<uint64_t> &children) {
+                  std::vector<uint64_t> &children, const std::vector<ParentGroup> *groups = nullptr) {
     const size_t np = parents.size() / stride;
     const unsigned nt = np < 64 ? 1u : host_threads();
     std::vector<std::vector<uint64_t>> part(nt);
@@ -151,6 +204,8 @@ void extend_level(const Design &ds, const std::vector<uint64_t> &parents, int de
             const uint64_t *pr = &parents[p * stride];
             solver.solve(pr, depth, sol);
             for (uint64_t row : sol) {
+                if (groups && !largest_in_orbit(solution_counts(pr, depth, ds.b, row), (*groups)[p]))
+                    continue;
                 const size_t at = part[t].size();
                 part[t].resize(at + stride, 0ull);
                 memcpy(&part[t][at], pr, depth * sizeof(uint64_t));
@@ -201,7 +256,7 @@ void sample_beam(std::vector<uint64_t> &keep, uint32_t stride, uint32_t beam, ui
 
 long run_enumeration(ise_ctx *ctx, const Design &ds, EnumSink &sink, uint64_t *stats,
                      uint64_t *level_counts, uint32_t stride, uint32_t capacity, uint32_t beam = 0,
-                     uint64_t seed = 0) {
+                     uint64_t seed = 0, bool prune = false) {
     uint64_t rng = seed;
     uint64_t st[5] = {0, 0, 0, 0, 0};
     double kernel_ms = 0.0;
@@ -220,11 +275,17 @@ long run_enumeration(ise_ctx *ctx, const Design &ds, EnumSink &sink, uint64_t *s
     std::vector<uint32_t> meta;
     std::vector<uint8_t> verdict;
     std::vector<uint64_t> order;
+    // with pruning: the generators of every canonical candidate of the level
+    std::vector<uint16_t> ngen;
+    std::vector<uint8_t> gens, keep_gens;
+    std::vector<ParentGroup> groups;
     for (int depth = 1; depth <= ds.v && !level.empty(); depth++) {
         const size_t n = level.size() / stride;
         const uint32_t out_info = depth == ds.v ? (ISE_SAVE_ROW_PERMUTATIONS | ISE_SAVE_ROW_TO_CHANGE) : 0u;
         const uint32_t m = (uint32_t)depth | ((uint32_t)ds.b << 8) | (out_info << 24);
         keep.clear();
+        keep_gens.clear();
+        const bool want_gens = prune && depth < ds.v;
         if (level_counts) level_counts[depth] = n;
         st[0] += n;
         if (depth == ds.v) st[2] += n;
@@ -241,11 +302,19 @@ long run_enumeration(ise_ctx *ctx, const Design &ds, EnumSink &sink, uint64_t *s
             meta.assign(cnt, m);
             verdict.resize(cnt);
             order.resize(cnt);
-            int rc = ise_submit(ctx, cnt, stride, &level[off * stride], nullptr, meta.data(), 0);
+            const uint32_t gen_cap = want_gens ? PRUNE_GEN_CAP : 0u;
+            int rc = ise_submit(ctx, cnt, stride, &level[off * stride], nullptr, meta.data(), gen_cap);
             if (rc) return rc;
             ise_results h{};
             h.verdict = verdict.data();
             h.group_order = order.data();
+            if (want_gens) {
+                ngen.resize(cnt);
+                gens.resize((size_t)cnt * gen_cap * ISE_MAX_COLS);
+                h.n_gens = ngen.data();
+                h.gens = gens.data();
+                h.gen_cap = gen_cap;
+            }
             rc = ise_collect(ctx, &h);
             if (rc) return rc;
             kernel_ms += ise_last_kernel_ms(ctx);
@@ -261,6 +330,13 @@ long run_enumeration(ise_ctx *ctx, const Design &ds, EnumSink &sink, uint64_t *s
                         st[3]++;
                     } else {
                         keep.insert(keep.end(), src, src + stride);
+                        if (want_gens) {
+                            // this parent's generators, as many as were stored
+                            const uint32_t ng = std::min<uint32_t>(ngen[c], PRUNE_GEN_CAP);
+                            const uint8_t *gp = &gens[(size_t)c * PRUNE_GEN_CAP * ISE_MAX_COLS];
+                            keep_gens.push_back((uint8_t)ng);
+                            keep_gens.insert(keep_gens.end(), gp, gp + (size_t)PRUNE_GEN_CAP * ISE_MAX_COLS);
+                        }
                         st[1]++;
                     }
                 } else if (verdict[c] != ISE_VERDICT_NONCANONICAL) {
@@ -269,8 +345,19 @@ long run_enumeration(ise_ctx *ctx, const Design &ds, EnumSink &sink, uint64_t *s
             }
         }
         sample_beam(keep, stride, beam, rng);
-        if (depth < ds.v) extend_level(ds, keep, depth, stride, next);
-        else next.clear();
+        if (depth < ds.v) {
+            if (want_gens && beam == 0) {
+                const size_t rec = 1 + (size_t)PRUNE_GEN_CAP * ISE_MAX_COLS;
+                groups.assign(keep.size() / stride, ParentGroup{});
+                for (size_t p = 0; p < groups.size(); p++)
+                    groups[p] = ParentGroup{keep_gens[p * rec], &keep_gens[p * rec + 1]};
+                extend_level(ds, keep, depth, stride, next, &groups);
+            } else {
+                extend_level(ds, keep, depth, stride, next);
+            }
+        } else {
+            next.clear();
+        }
         level.swap(next);
     }
     st[4] = (uint64_t)(kernel_ms * 1e6);
@@ -377,6 +464,19 @@ long ise_enumerate_bibd(ise_ctx *ctx, int v, int k, int lambda, uint64_t *design
     const uint32_t stride = ise::ctx_stride(ctx);
     if ((int)stride < v) return ISE_ERR_ARG;
     return run_enumeration(ctx, ds, sink, stats, level_counts, stride, ise::ctx_capacity(ctx));
+}
+
+long ise_enumerate_bibd_pruned(ise_ctx *ctx, int v, int k, int lambda, uint64_t *designs, uint64_t *auts,
+                               long max_designs, uint64_t *stats, uint64_t *level_counts) {
+    Design ds;
+    if (!ctx || !derive(v, k, lambda, &ds)) return ISE_ERR_ARG;
+    EnumSink sink;
+    sink.designs = designs;
+    sink.auts = auts;
+    sink.max_designs = max_designs;
+    const uint32_t stride = ise::ctx_stride(ctx);
+    if ((int)stride < v) return ISE_ERR_ARG;
+    return run_enumeration(ctx, ds, sink, stats, level_counts, stride, ise::ctx_capacity(ctx), 0, 0, true);
 }
 
 long ise_enumerate_bibd_candidates(ise_ctx *ctx, int v, int k, int lambda, int dmin, int dmax,

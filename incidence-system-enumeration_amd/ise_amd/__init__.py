@@ -103,6 +103,8 @@ def lib():
         L.ise_last_kernel_ms.restype = ctypes.c_float
         L.ise_enumerate_bibd.argtypes = [P, I, I, I, P, P, ctypes.c_long, P, P]
         L.ise_enumerate_bibd.restype = ctypes.c_long
+        L.ise_enumerate_bibd_pruned.argtypes = [P, I, I, I, P, P, ctypes.c_long, P, P]
+        L.ise_enumerate_bibd_pruned.restype = ctypes.c_long
         L.ise_enumerate_bibd_candidates.argtypes = [P, I, I, I, I, I, U32, P, P, ctypes.c_long,
                                                     ctypes.POINTER(ctypes.c_long)]
         L.ise_enumerate_bibd_candidates.restype = ctypes.c_long
@@ -470,14 +472,16 @@ class Context:
     def last_kernel_ms(self):
         return float(self._L.ise_last_kernel_ms(self.h))
 
-    def enumerate_bibd(self, v, k, lam, max_designs=1 << 16):
-        """Full enumeration; returns (designs [m][v] uint64, auts [m], stats, level_counts)."""
+    def enumerate_bibd(self, v, k, lam, max_designs=1 << 16, pruned=False):
+        """Full enumeration; returns (designs [m][v] uint64, auts [m], stats, level_counts).
+        pruned: solutions pruned by each parent's automorphism group
+        (ise_enumerate_bibd_pruned)."""
         des = np.zeros((max_designs, v), np.uint64)
         auts = np.zeros(max_designs, np.uint64)
         stats = np.zeros(5, np.uint64)
         levels = np.zeros(v + 1, np.uint64)
-        n = self._L.ise_enumerate_bibd(self.h, v, k, lam, _ptr(des), _ptr(auts), max_designs,
-                                       _ptr(stats), _ptr(levels))
+        fn = self._L.ise_enumerate_bibd_pruned if pruned else self._L.ise_enumerate_bibd
+        n = fn(self.h, v, k, lam, _ptr(des), _ptr(auts), max_designs, _ptr(stats), _ptr(levels))
         if n < 0:
             self._check(int(n))
         m = min(n, max_designs)

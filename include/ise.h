@@ -616,6 +616,19 @@ long ise_enumerate_bibd(ise_ctx *ctx, int v, int k, int lambda, uint64_t *design
                         uint64_t *auts, long max_designs, uint64_t *stats,
                         uint64_t *level_counts);
 
+/* The same enumeration with the solutions of every canonical matrix pruned by
+ * its automorphism group (CRowSolution::sortSolutionsByGroup,
+ * Source/RowSolution.h:325-383, called from CBIBD_Enumerator::checkSolutions,
+ * Source/BIBD_Enumerator.h:90-114): the test of each level also returns the
+ * generators (up to 32 per matrix; fewer prune less, never wrongly), and of
+ * each orbit of next-row solutions under them only the lexicographically
+ * largest is extended -- the children of the others are isomorphic to it with
+ * the same first rows and cannot be canonical.  Same designs, |Aut| and
+ * outputs as ise_enumerate_bibd; fewer candidates tested. */
+long ise_enumerate_bibd_pruned(ise_ctx *ctx, int v, int k, int lambda, uint64_t *designs,
+                               uint64_t *auts, long max_designs, uint64_t *stats,
+                               uint64_t *level_counts);
+
 /* Same enumeration, but every candidate at depth in [dmin,dmax] is also
  * copied out (stride words each, up to cap; *n_out receives the count):
  * the "enumerator output" part of the candidate streams. */

@@ -197,3 +197,66 @@ def random_orbit_form_batch(seed, count, stride=8, max_rows=7, max_cols=10, stab
         rows[i, :d] = (m.astype(np.uint64) << np.arange(b, dtype=np.uint64)).sum(axis=1)
         meta[i] = d | (b << 8) | (f << 16)
     return mats, rows, meta
+
+
+def orbit_cells(rows, d, b):
+    """(start, length) of the column orbits of row d (defined by rows 0..d-1)."""
+    m, full = 1, (1 << b) - 1
+    for i in range(d):
+        r = int(rows[i])
+        m |= (r ^ (r << 1)) & full
+    st = [c for c in range(b) if (m >> c) & 1]
+    return [(s, (st[i + 1] if i + 1 < len(st) else b) - s) for i, s in enumerate(st)]
+
+
+def largest_in_orbit(sol, gens):
+    """Whether the solution (ones per orbit) is the largest of its orbit under
+    the group the permutations gens generate (image[i] = sol[g[i]])."""
+    seen, todo = {sol}, [sol]
+    while todo:
+        x = todo.pop()
+        for g in gens:
+            y = tuple(x[g[i]] for i in range(len(x)))
+            if y > sol:
+                return False
+            if y not in seen:
+                seen.add(y)
+                todo.append(y)
+    return True
+
+
+def oracle_pruned_levels(oracle, v, k, lam, gen_cap=32):
+    """Candidates per depth 1..v of a level enumeration built on the oracle
+    (its row solver, its test and the generators it returns) in which only
+    the orbit-largest solutions of each canonical matrix are extended."""
+    import ise_amd
+    b = ise_amd.design_params(v, k, lam)[1]
+    parents = [np.zeros(32, np.uint64)]
+    levels = []
+    for depth in range(v):
+        res = None
+        if depth:
+            res = oracle.check_batch(np.array(parents), np.full(len(parents), depth | (b << 8), np.uint32),
+                                     gen_cap=gen_cap)
+        kids = []
+        for pi, p in enumerate(parents):
+            cells = orbit_cells(p, depth, b) if depth else None
+            gens = []
+            if depth:
+                ng = min(int(res["n_gens"][pi]), gen_cap)
+                gens = [list(res["gens"][pi, g, :len(cells)]) for g in range(ng)]
+            for s in oracle.row_solutions(v, k, lam, p, depth):
+                if gens:
+                    sol = tuple(bin((int(s) >> a) & ((1 << L) - 1)).count("1") for a, L in cells)
+                    if not largest_in_orbit(sol, gens):
+                        continue
+                kid = p.copy()
+                kid[depth] = s
+                kids.append(kid)
+        levels.append(len(kids))
+        if not kids:
+            break
+        oi = (1 | 4) << 24 if depth + 1 == v else 0
+        r = oracle.check_batch(np.array(kids), np.full(len(kids), (depth + 1) | (b << 8) | oi, np.uint32))
+        parents = [kids[i] for i in range(len(kids)) if r["verdict"][i] == 1]
+    return levels
