@@ -141,9 +141,9 @@ __global__ __launch_bounds__(256) void sched_fill_kernel(uint32_t n, const uint3
 
 // 32-bit words of LDS scratch per lane group: orbit geometry (one word per
 // row and lane) or, for the column-set search, one 64-bit set per row and
-// lane plus one count byte per row and lane.
+// lane.
 template <int G, bool SETS> __host__ __device__ constexpr uint32_t group_scratch_words(uint32_t rows_cap) {
-    return SETS ? rows_cap * G * 2u + rows_cap * G / 4u : rows_cap * G;
+    return SETS ? rows_cap * G * 2u : rows_cap * G;
 }
 
 template <int G, bool COUNT, bool HUNT, bool SETS>

@@ -435,6 +435,16 @@ ISE_DEV void plain_search(const Grp<G> &g, const Staged &c, const uint32_t *geo,
     (void)b;
 }
 
+// A word of the set stack holds the set in its low b bits and, in its top
+// byte, the number of ones of original row i in the level-i orbit holding
+// position l (what every permuted row's count there is compared with):
+// written once by stage_sets and carried by every later store to the word,
+// so the stack is the group's only LDS.  Matrices wider than PACKED_COLS
+// columns run the replayed search.
+constexpr int PACKED_COLS = 56;
+constexpr uint64_t COUNT_BYTE = 0xff00000000000000ull;
+ISE_DEV int level_count(uint64_t w) { return (int)(uint32_t)(w >> 56); }
+
 // Column-set state of the search (used when the candidate is in orbit form,
 // see stage_sets).  The column permutation of the reference only matters
 // through the set of source columns it places in each column orbit: a row
@@ -455,7 +465,7 @@ ISE_DEV void plain_search(const Grp<G> &g, const Staged &c, const uint32_t *geo,
 // split after these ones.  Rows written as the reference's MakeRow writes them
 // (Source/MatrixCanonChecker.cpp:14-93) have it.
 template <int G>
-ISE_DEV bool stage_sets(const Grp<G> &g, const Staged &c, const uint64_t *cells, uint64_t *stk, uint8_t *cnt) {
+ISE_DEV bool stage_sets(const Grp<G> &g, const Staged &c, const uint64_t *cells, uint64_t *stk) {
     const int l = g.ln, d = c.d, b = c.b;
     const uint64_t le = (l == 63) ? ~0ull : ((2ull << l) - 1);
     bool ok = true;
@@ -467,8 +477,7 @@ ISE_DEV bool stage_sets(const Grp<G> &g, const Staged &c, const uint64_t *cells,
         uint64_t om = lowmask(e0) & ~lowmask(s0);
         int cR0 = popc64(R & om);
         if (l >= b) { om = 0; cR0 = 0; }
-        stk[r * G + l] = om;
-        cnt[r * G + l] = (uint8_t)cR0;
+        stk[r * G + l] = om | ((uint64_t)cR0 << 56);
         // ones of row r first inside the orbit
         ok = ok && (R & om) == (l < b ? (lowmask(s0 + cR0) & ~lowmask(s0)) : 0ull);
         // orbits of row r+1 = orbits of row r split after those ones (given cell masks only;
@@ -520,17 +529,81 @@ template <int G> ISE_DEV int fill_tail(const Grp<G> &g, int pr, int i, int d, ui
     return (l > i && l < d) ? select_bit((uint64_t)rest, l - i - 1) : pr;
 }
 
+// Rows of U that one of level i's first Q column orbits refutes, for all of
+// them at once: lane r*Q + q compares row r with orbit q (Q = G / RG orbits,
+// RG = 1 << rg_log = 16 or 32 rows), counting as the walk counts there.  A
+// row whose first difference among those orbits has the larger count in the
+// original row is refuted (sibling_run's `more` test) whatever the later
+// orbits hold, so the walk moves past it without comparing it again.
+// rrow: row (lane / Q) of the candidate; ost (lane i): the first columns of
+// level i's first four orbits, a byte each (0xff: no such orbit).  The
+// level's stack words are current.
+template <int G>
+ISE_DEV uint32_t level_refuted(const Grp<G> &g, const uint64_t *stk, int i, uint32_t U, uint64_t rrow,
+                               uint32_t ost, int rg_log) {
+    const int l = g.ln, qlog = __builtin_ctz(G) - rg_log, q = l & ((1 << qlog) - 1), r = l >> qlog;
+    const uint32_t s = ((uint32_t)g.read((int)ost, i) >> (8 * q)) & 0xffu;
+    const bool live = s != 0xffu && ((U >> r) & 1u);
+    const uint64_t w = stk[i * G + (live ? (int)s : 0)];
+    const int cP = popc64(rrow & w);             // rows hold no bit in the count byte
+    const int cR = level_count(w);
+    const uint64_t ne = g.ballot(live && cP != cR);
+    const uint64_t gt = g.ballot(live && cR > cP);
+    // lane j < RG: row j's slots, refuted when the first slot that differs is one of gt's
+    const bool row_lane = l < (1 << rg_log);
+    const int sh = (row_lane ? l : 0) << qlog;
+    const uint32_t qm = (1u << (1 << qlog)) - 1u;
+    const uint32_t n = row_lane ? (uint32_t)(ne >> sh) & qm : 0u;
+    const uint32_t t = (uint32_t)(gt >> sh) & qm;
+    return (uint32_t)g.ballot((n & (0u - n) & t) != 0u) & U;
+}
+
+// Byte k of lane i: the first column of the k-th orbit of level i (k < 4),
+// 0xff past the last orbit.
+template <int G> ISE_DEV uint32_t orbit_starts(const Staged &c) {
+    uint64_t C = c.cellv;
+    uint32_t o = 0;
+    for (int k = 0; k < 4; k++) {
+        o |= (C ? (uint32_t)ctz64(C) : 0xffu) << (8 * k);
+        C &= C - 1;
+    }
+    return o;
+}
+
 // Rows tried at one position of the depth-first phase: compares row `cur`
 // and, while it is refuted, moves on to the next larger unused row (the bits
 // of U above cur).  Stops on an equal row (bad = 0), a larger one (a witness)
 // or when no larger unused row is left; with CHECKED also when `left`
 // permutations have been used.  Returns the number of rows moved on to;
-// bad and more are the last comparison's ballots.
+// bad and more are the last comparison's ballots.  Rows of `ref` (known
+// refuted at this position, level_refuted) are moved past uncompared; when
+// the last row moved to is one of them, bad and more read "refuted".
 template <int G, bool COUNT, bool CHECKED>
 ISE_DEV uint32_t sibling_run(const Grp<G> &g, const Staged &c, uint64_t S, int cR, uint64_t C, uint32_t U,
-                             uint32_t left, int &cur, uint64_t &bad, uint64_t &more, uint64_t &Rp, Verdict &v) {
+                             uint32_t ref, uint32_t left, int &cur, uint64_t &bad, uint64_t &more, uint64_t &Rp,
+                             Verdict &v) {
     uint32_t k = 0;
     for (;;) {
+        if constexpr (!CHECKED && !COUNT) {
+            if ((ref >> cur) & 1u) {
+                const uint32_t abv = U & (0xfffffffeu << cur);
+                const uint32_t open = abv & ~ref;
+                if (!open) {
+                    // every larger unused row is refuted too: the walk ends on the last
+                    if (abv) {
+                        k += (uint32_t)__popc(abv);
+                        cur = 31 - __builtin_clz(abv);
+                    }
+                    bad = 1ull;
+                    more = 1ull;
+                    Rp = 0ull;
+                    return k;
+                }
+                const int nx = __builtin_ctz(open);
+                k += (uint32_t)__popc(abv & ((2u << nx) - 1u));
+                cur = nx;
+            }
+        }
         Rp = g.read64(c.rowv, cur);
         const int cP = popc64(Rp & S);
         bad = g.ballot(cP != cR) & C;
@@ -580,7 +653,7 @@ ISE_DEV uint32_t sibling_run(const Grp<G> &g, const Staged &c, uint64_t S, int c
 // when it stops.  From the first automorphism on, the replayed walk takes
 // over with the permutation, stabiliser point and orbits the walk would have.
 template <int G, bool COUNT>
-ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const uint8_t *cnt, uint32_t ci,
+ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, uint32_t ci,
                         const OutPtrs &o, uint32_t max_perms, int &pr, int &ob, Verdict &v) {
     const int l = g.ln, d = c.d;
     PermState st;
@@ -597,6 +670,20 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
         uint32_t U = 1u << (d - 1);            // rows not used before position i (perm[i] included)
         uint32_t left = max_perms > v.perms ? max_perms - v.perms : 0u;   // permutations still allowed
         bool first = true;
+        // rows refuted by a level's first orbits (level_refuted), found when the
+        // walk enters the level: lane i keeps level i's, refd marks the levels
+        // found for their current prefix.  Not with counters, which count every
+        // row the walk compares.
+        const int rg_log = (d <= 16 || G == 16) ? 4 : 5;   // a row past RG is never marked
+        uint64_t rrow = 0;
+        uint32_t refd = 0, ost = 0;
+        int refv = 0;
+        if constexpr (!COUNT) {
+            const int src = l >> (__builtin_ctz(G) - rg_log);
+            rrow = ((uint64_t)(uint32_t)g.gather((int)(uint32_t)(c.rowv >> 32), src) << 32) |
+                   (uint32_t)g.gather((int)(uint32_t)c.rowv, src);
+            ost = orbit_starts<G>(c);
+        }
         for (;;) {
             // back up to the deepest position with a larger unused row
             int p = g.uni(i), nxt = -1;
@@ -621,9 +708,20 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
             i = g.uni(p);
             int cur = g.uni(nxt);
             if (l == i) pr = cur;
-            S = stk[i * G + l];
-            int cR = cnt[i * G + l];
+            const uint64_t w = stk[i * G + l];
+            S = w & ~COUNT_BYTE;
             uint64_t C = g.uni64(g.read64(c.cellv, i));
+            int cR = level_count(w);
+            uint32_t ref = 0;
+            if constexpr (!COUNT) {
+                if ((refd >> i) & 1u) {
+                    ref = (uint32_t)g.read(refv, i);
+                } else {
+                    ref = level_refuted(g, stk, i, U, rrow, ost, rg_log);
+                    if (l == i) refv = (int)ref;
+                    refd |= 1u << i;
+                }
+            }
             bool fresh = true;                 // cur makes a new permutation (not a descent)
             int why = 0;                       // 1 witness, 2 automorphism, 3 limit
             for (;;) {
@@ -634,9 +732,9 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
                 // rows tried at position i until one is not refuted: only cur changes
                 uint64_t bad, more, Rp;
                 if (left >= 32u)               // at most 31 more rows at one position
-                    left -= sibling_run<G, COUNT, false>(g, c, S, cR, C, U, left, cur, bad, more, Rp, v);
+                    left -= sibling_run<G, COUNT, false>(g, c, S, cR, C, U, ref, left, cur, bad, more, Rp, v);
                 else
-                    left -= sibling_run<G, COUNT, true>(g, c, S, cR, C, U, left, cur, bad, more, Rp, v);
+                    left -= sibling_run<G, COUNT, true>(g, c, S, cR, C, U, 0u, left, cur, bad, more, Rp, v);
                 if (l == i) pr = cur;
                 if (!bad) {
                     // equal: descend to the next position (Rp: the row just compared)
@@ -646,12 +744,18 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
                     if (++i == d) { why = 2; break; }
                     cur = ctz64(U);
                     if (l == i) pr = cur;
-                    stk[i * G + l] = S;
-                    cR = cnt[i * G + l];
+                    const uint64_t wn = stk[i * G + l];
+                    stk[i * G + l] = S | (wn & COUNT_BYTE);
                     C = g.uni64(g.read64(c.cellv, i));
+                    cR = level_count(wn);
                     i = g.uni(i);
                     cur = g.uni(cur);
                     U = (uint32_t)g.uni((int)U);
+                    if constexpr (!COUNT) {
+                        ref = level_refuted(g, stk, i, U, rrow, ost, rg_log);
+                        if (l == i) refv = (int)ref;
+                        refd |= 1u << i;
+                    }
                     fresh = false;
                     continue;
                 }
@@ -703,9 +807,10 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
         if (row == EMAX)
             break;
         if (++v.perms > max_perms) { v.verdict = ISE_VERDICT_LIMIT; break; }
-        S = stk[row * G + l];
-        int cR = cnt[row * G + l];
+        const uint64_t w = stk[row * G + l];
+        S = w & ~COUNT_BYTE;
         uint64_t C = g.read64(c.cellv, row);
+        int cR = level_count(w);
         int outcome = 0, dec = 0;
         bool limit = false;
         for (;;) {
@@ -724,9 +829,10 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, const u
                 S &= ((Ro >> l) & 1ull) ? Rp : ~Rp;
                 if (++row == d)
                     break;                    // every row matched: automorphism
-                stk[row * G + l] = S;
-                cR = cnt[row * G + l];
+                const uint64_t wn = stk[row * G + l];
+                stk[row * G + l] = S | (wn & COUNT_BYTE);
                 C = g.read64(c.cellv, row);
+                cR = level_count(wn);
                 continue;
             }
             const int lim = ctz64(bad);
@@ -846,7 +952,7 @@ ISE_DEV bool hunt(const Grp<G> &g, const Staged &c, const uint32_t *geo, const u
 // set stack holds the identity sets on entry (as stage_sets leaves it) and
 // the sets of the hunt's prefixes on return.
 template <int G, bool COUNT>
-ISE_DEV bool hunt_sets(const Grp<G> &g, const Staged &c, uint64_t *stk, const uint8_t *cnt, const uint8_t *order,
+ISE_DEV bool hunt_sets(const Grp<G> &g, const Staged &c, uint64_t *stk, const uint8_t *order,
                        int od, uint32_t budget, int &pr, Verdict &v) {
     const int l = g.ln, d = c.d;
     int cursor = 0;            // lane i: next index into order row i
@@ -891,9 +997,10 @@ ISE_DEV bool hunt_sets(const Grp<G> &g, const Staged &c, uint64_t *stk, const ui
         if (nodes == budget) return false;
         nodes++;
         v.perms++;
-        const uint64_t S = stk[i * G + l];
-        const int cR = cnt[i * G + l];
+        const uint64_t w = stk[i * G + l];
+        const uint64_t S = w & ~COUNT_BYTE;
         const uint64_t C = g.uni64(g.read64(c.cellv, i));
+        const int cR = level_count(w);
         const uint64_t Rp = g.read64(c.rowv, j);
         const int cP = popc64(Rp & S);
         const uint64_t bad = g.ballot(cP != cR) & C;
@@ -920,7 +1027,8 @@ ISE_DEV bool hunt_sets(const Grp<G> &g, const Staged &c, uint64_t *stk, const ui
         if constexpr (COUNT) { v.orbs_cmp += popc64(C); v.cells_cmp += (uint32_t)c.b; }
         if (i + 1 < d) {
             const uint64_t Ro = g.read64(c.rowv, i);
-            stk[(i + 1) * G + l] = S & (((Ro >> l) & 1ull) ? Rp : ~Rp);
+            const uint64_t wn = stk[(i + 1) * G + l];
+            stk[(i + 1) * G + l] = (S & (((Ro >> l) & 1ull) ? Rp : ~Rp)) | (wn & COUNT_BYTE);
         }
         used |= 1u << j;
         i++;
@@ -931,10 +1039,10 @@ ISE_DEV bool hunt_sets(const Grp<G> &g, const Staged &c, uint64_t *stk, const ui
 // Full canonicity test of candidate `ci` by the group g.  With HUNT, a
 // budgeted witness hunt in the proposed order runs first; the plain search
 // decides whatever the hunt leaves open (canonical candidates, |Aut|).  With
-// SETS (no column permutation reported), a candidate in orbit form runs the
-// column-set search, any other the replayed one.  `lds` is the group's
-// scratch: d*G orbit-geometry words, or in SETS mode d*G set words followed by
-// d*G count bytes.
+// SETS (no column permutation reported), a candidate in orbit form with at
+// most PACKED_COLS columns runs the column-set search, any other the replayed
+// one.  `lds` is the group's
+// scratch: d*G orbit-geometry words, or in SETS mode d*G 64-bit set words.
 //
 // With `defer` set, a candidate whose search runs past max_perms is appended
 // to that list (count in *defer_count) and no record is written: it is
@@ -953,22 +1061,21 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const 
     uint32_t *geo = lds;
     if constexpr (SETS) {
         uint64_t *stk = reinterpret_cast<uint64_t *>(lds);
-        uint8_t *cnt = reinterpret_cast<uint8_t *>(stk + d * G);
-        if (stage_sets(g, c, cells, stk, cnt)) {
+        if (c.b <= PACKED_COLS && stage_sets(g, c, cells, stk)) {
             bool hfound = false;
             if constexpr (HUNT) {
                 // an order table narrower than the candidate cannot guide it: plain search only
                 if (!order || d <= od) {
-                    hfound = hunt_sets<G, COUNT>(g, c, stk, cnt, order ? order + (uint64_t)ci * od * od : nullptr,
+                    hfound = hunt_sets<G, COUNT>(g, c, stk, order ? order + (uint64_t)ci * od * od : nullptr,
                                                  od, hunt_budget, pr, v);
                     if (!hfound) {
                         pr = l;
-                        stage_sets(g, c, cells, stk, cnt);   // identity sets again
+                        stage_sets(g, c, cells, stk);   // identity sets again
                     }
                 }
             }
             if (!hfound)
-                set_search<G, COUNT>(g, c, stk, cnt, ci, o, max_perms, pr, ob, v);
+                set_search<G, COUNT>(g, c, stk, ci, o, max_perms, pr, ob, v);
             found = true;
         }
     }

@@ -34,14 +34,22 @@ def assert_same(gpu, orc, gen_cap, what):
 
 
 def run(gpu_ctx, oracle, rows, meta, what, cells=None, gen_cap=8, group=0):
+    """Counted launch (every row the walk compares is counted, so no row is
+    skipped) and the launch without counters, which moves past the rows a
+    level's first orbits refute without comparing them again: both must give
+    the oracle's results."""
     gpu_ctx.set_launch(group_lanes=group)
     try:
         g = gpu_ctx.check_batch(rows, meta, cells=cells, gen_cap=gen_cap, fields=FIELDS)
+        u = gpu_ctx.check_batch(rows, meta, cells=cells, gen_cap=gen_cap,
+                                fields=tuple(f for f in FIELDS if f != "counters"))
     finally:
         gpu_ctx.set_launch()
     o = oracle.check_batch(rows, meta, cells=cells, gen_cap=gen_cap)
     assert (g["verdict"] <= 1).all()
     assert_same(g, o, gen_cap, what)
+    u["counters"] = o["counters"]
+    assert_same(u, o, gen_cap, what + " (no counters)")
     return g
 
 
