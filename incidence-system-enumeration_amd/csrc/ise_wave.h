@@ -691,7 +691,19 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, uint32_
             int e = g.read(pr, p);
             for (;;) {
                 const uint32_t above = U & (0xfffffffeu << e);
-                if (above) { nxt = ctz64(above); break; }
+                if (above) {
+                    bool through = false;
+                    if constexpr (!COUNT) {
+                        // every larger row refuted by the level's first orbits: the walk
+                        // moves through them there and backs up again
+                        if (((refd >> p) & 1u) && left >= 32u && p >= (first ? c.fixed : 0))
+                            through = (above & ~(uint32_t)g.read(refv, p)) == 0u;
+                    }
+                    if (!through) { nxt = ctz64(above); break; }
+                    left -= (uint32_t)__popc(above);
+                    if (st.stab > p) st.stab = p;
+                    first = false;
+                }
                 if (p == 0) break;
                 p--;
                 e = g.read(pr, p);
