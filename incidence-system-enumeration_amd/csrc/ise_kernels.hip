@@ -203,6 +203,7 @@ static OutPtrs to_out(const ise_results &r) {
     OutPtrs o;
     o.verdict = r.verdict; o.fail_row = r.fail_row; o.fail_col = r.fail_col;
     o.row_to_change = r.row_to_change; o.group_order = r.group_order;
+    o.group_order_hi = r.group_order_hi;
     o.perm_row = r.perm_row; o.perm_col = r.perm_col; o.orbits = r.orbits;
     o.n_gens = r.n_gens; o.gens = r.gens; o.counters = r.counters;
     o.gen_cap = r.gens ? r.gen_cap : 0u;

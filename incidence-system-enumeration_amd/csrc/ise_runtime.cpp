@@ -324,6 +324,7 @@ int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride) {
         if (e == hipSuccess) e = dalloc(&c->d.fail_col, n);
         if (e == hipSuccess) e = dalloc(&c->d.row_to_change, n);
         if (e == hipSuccess) e = dalloc(&c->d.group_order, n);
+        if (e == hipSuccess) e = dalloc(&c->d.group_order_hi, n);
         if (e == hipSuccess) e = dalloc(&c->d.perm_row, n * ISE_MAX_ROWS);
         if (e == hipSuccess) e = dalloc(&c->d.perm_col, n * ISE_MAX_COLS);
         if (e == hipSuccess) e = dalloc(&c->d.orbits, n * ISE_MAX_ROWS);
@@ -375,7 +376,7 @@ int ise_close(ise_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_rows, c->d_cells, c->d_meta, c->d_head, c->d.verdict, c->d.fail_row,
-                    c->d.fail_col, c->d.row_to_change, c->d.group_order, c->d.perm_row,
+                    c->d.fail_col, c->d.row_to_change, c->d.group_order, c->d.group_order_hi, c->d.perm_row,
                     c->d.perm_col, c->d.orbits, c->d.n_gens, c->d.counters, c->d.gens,
                     c->d_kf_tab, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried, c->d_defer, c->d_sched,
                     c->d_kf_list, c->d_kf_best, c->d_kf_part, c->d_kf_ranges,
@@ -549,6 +550,8 @@ int ise_collect(ise_ctx *c, ise_results *h) {
             HIPCHK(c, hipMemcpyAsync(h->row_to_change, c->d.row_to_change, n, hipMemcpyDeviceToHost, s));
         if (h->group_order)
             HIPCHK(c, hipMemcpyAsync(h->group_order, c->d.group_order, n * 8, hipMemcpyDeviceToHost, s));
+        if (h->group_order_hi)
+            HIPCHK(c, hipMemcpyAsync(h->group_order_hi, c->d.group_order_hi, n * 8, hipMemcpyDeviceToHost, s));
         if (h->perm_row)
             HIPCHK(c, hipMemcpyAsync(h->perm_row, c->d.perm_row, n * ISE_MAX_ROWS, hipMemcpyDeviceToHost, s));
         if (h->perm_col)

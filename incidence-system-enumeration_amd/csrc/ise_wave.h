@@ -87,7 +87,8 @@ constexpr int IMAX = 254;   // IDX_MAX
 struct PermState {
     int stab;        // stabiliser length (m_nStabLength)
     int stab_aut;    // m_nStabLengthAut
-    uint64_t order;  // |Aut| accumulated along the stabiliser chain
+    uint64_t order;  // |Aut| accumulated along the stabiliser chain, bits 0..63
+    uint64_t order_hi;   // bits 64..127 (the reference's group order is a 128-bit UInt)
 };
 
 // perm[piv+1 .. n-1] reversed
@@ -196,7 +197,9 @@ ISE_DEV int next_row_perm(const Grp<G> &g, PermState &st, int &pr, int ob, int n
 template <int G> ISE_DEV void grow_order(const Grp<G> &g, PermState &st, int ob, int n) {
     if (st.stab_aut >= n) return;
     uint64_t m = g.ballot(g.ln > st.stab_aut && g.ln < n && ob == st.stab_aut);
-    st.order *= (uint64_t)(1 + popc64(m));
+    const uint64_t len = (uint64_t)(1 + popc64(m));
+    st.order_hi = st.order_hi * len + __umul64hi(st.order, len);
+    st.order *= len;
 }
 
 // New automorphism `pr` (GroupOrder.h:56-62, :76-102, Update_Orbits :30-49)
@@ -222,6 +225,7 @@ ISE_DEV void add_automorphism(const Grp<G> &g, PermState &st, int pr, int &ob, i
 struct OutPtrs {
     uint8_t *verdict, *fail_row, *fail_col, *row_to_change;
     uint64_t *group_order;
+    uint64_t *group_order_hi;   // bits 64..127 of |Aut|, may be null
     uint8_t *perm_row;   // [ISE_MAX_ROWS] per candidate
     uint8_t *perm_col;   // [ISE_MAX_COLS] per candidate
     uint8_t *orbits;     // [ISE_MAX_ROWS] per candidate
@@ -361,7 +365,7 @@ ISE_DEV int compare_row(const Grp<G> &g, const Staged &c, const uint32_t *geo, i
 // Result of one candidate before it is written.
 struct Verdict {
     int verdict, frow, fcol, rtc;
-    uint64_t order;
+    uint64_t order, order_hi;
     uint32_t perms, rows_cmp, orbs_cmp, cells_cmp, ngen;
 };
 
@@ -383,6 +387,7 @@ ISE_DEV void plain_search(const Grp<G> &g, const Staged &c, const uint32_t *geo,
     st.stab = d - 1;
     st.stab_aut = EMAX;
     st.order = 1;
+    st.order_hi = 0;
     int row = EMAX;
     for (;;) {
         row = next_row_perm(g, st, pr, ob, d, row, c.fixed);
@@ -432,6 +437,7 @@ ISE_DEV void plain_search(const Grp<G> &g, const Staged &c, const uint32_t *geo,
     if (v.verdict == 1 && c.row_permut)
         grow_order(g, st, ob, d);
     v.order = st.order;
+    v.order_hi = st.order_hi;
     (void)b;
 }
 
@@ -660,6 +666,7 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, uint32_
     st.stab = d - 1;
     st.stab_aut = EMAX;
     st.order = 1;
+    st.order_hi = 0;
     uint64_t S = 0;
     int row = EMAX;
     bool aut = false;
@@ -803,6 +810,7 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, uint32_
         if (v.verdict == 1 && c.row_permut)
             grow_order(g, st, ob, d);
         v.order = st.order;
+    v.order_hi = st.order_hi;
         return;
     }
 
@@ -884,6 +892,7 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, uint32_
     if (v.verdict == 1 && c.row_permut)
         grow_order(g, st, ob, d);
     v.order = st.order;
+    v.order_hi = st.order_hi;
 }
 
 // Witness hunt in a proposed order: depth-first over row prefixes, position i
@@ -1068,7 +1077,7 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const 
     const Staged c = load_candidate(g, ci, meta, rows, cells, stride);
     const int d = c.d, b = c.b;
     int pr = l, ob = l, pc = l;
-    Verdict v{1, EMAX, EMAX, EMAX, 1ull, 0u, 0u, 0u, 0u, 0u};
+    Verdict v{1, EMAX, EMAX, EMAX, 1ull, 0ull, 0u, 0u, 0u, 0u, 0u};
     bool found = false;
     uint32_t *geo = lds;
     if constexpr (SETS) {
@@ -1117,6 +1126,7 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const 
         if (o.fail_col) o.fail_col[ci] = (uint8_t)v.fcol;
         if (o.row_to_change) o.row_to_change[ci] = (uint8_t)v.rtc;
         if (o.group_order) o.group_order[ci] = v.verdict == 1 ? v.order : 0ull;
+        if (o.group_order_hi) o.group_order_hi[ci] = v.verdict == 1 ? v.order_hi : 0ull;
         if (o.n_gens) o.n_gens[ci] = (uint16_t)(v.ngen > 0xffffu ? 0xffffu : v.ngen);
         if (COUNT && o.counters) {
             uint32_t *cc = o.counters + (uint64_t)ci * 4;

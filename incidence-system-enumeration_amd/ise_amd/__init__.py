@@ -353,7 +353,7 @@ class Context:
         dev = {k: self.device_array(a.shape, a.dtype) for k, a in host_like.items()}
         s = ResultsStruct()
         for k in ("verdict", "fail_row", "fail_col", "row_to_change", "group_order", "perm_row",
-                  "perm_col", "orbits", "n_gens", "gens", "counters"):
+                  "perm_col", "orbits", "n_gens", "gens", "counters", "group_order_hi"):
             setattr(s, k, dev[k].ptr if k in dev else None)
         s.gen_cap = gen_cap if "gens" in dev else 0
         return dev, s
@@ -841,7 +841,8 @@ class CanonicityChecker:
         return self._res["orbits"][0, :self._nrow].copy()
 
     def groupOrder(self):
-        return int(self._res["group_order"][0])
+        """|Aut| as a Python int (the reference's 128-bit UInt)."""
+        return int(self._res["group_order"][0]) | (int(self._res["group_order_hi"][0]) << 64)
 
     def rowToChange(self):
         return int(self._res["row_to_change"][0])

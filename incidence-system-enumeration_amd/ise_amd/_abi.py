@@ -27,11 +27,12 @@ class ResultsStruct(ctypes.Structure):
         ("verdict", _P), ("fail_row", _P), ("fail_col", _P), ("row_to_change", _P),
         ("group_order", _P), ("perm_row", _P), ("perm_col", _P), ("orbits", _P),
         ("n_gens", _P), ("gens", _P), ("counters", _P), ("gen_cap", ctypes.c_uint32),
+        ("group_order_hi", _P),
     ]
 
 
 RESULT_FIELDS = ("verdict", "fail_row", "fail_col", "row_to_change", "group_order",
-                 "perm_row", "perm_col", "orbits", "n_gens", "gens", "counters")
+                 "perm_row", "perm_col", "orbits", "n_gens", "gens", "counters", "group_order_hi")
 
 
 def alloc_results(n, gen_cap=0, fields=None):
@@ -44,6 +45,7 @@ def alloc_results(n, gen_cap=0, fields=None):
         "group_order": ((n,), np.uint64), "perm_row": ((n, MAX_ROWS), np.uint8),
         "perm_col": ((n, MAX_COLS), np.uint8), "orbits": ((n, MAX_ROWS), np.uint8),
         "n_gens": ((n,), np.uint16), "counters": ((n, 4), np.uint32),
+        "group_order_hi": ((n,), np.uint64),
     }
     out = {}
     for name, (shape, dt) in shapes.items():

@@ -56,7 +56,7 @@ class CGPUCanonicityChecker {
         const uint32_t meta = (uint32_t)nRowMax | (nCol << 8) | ((uint32_t)stabExt_ << 16) |
                               ((outInfo & 0xffu) << 24);
         ise_results r = {&verdict_, &failRow_, &failCol_, &rowToChange_, &order_, permRow_, permCol_,
-                         orbits_, &nGens_, nullptr, nullptr, 0};
+                         orbits_, &nGens_, nullptr, nullptr, 0, &orderHi_};
         if (ise_submit(ctx_, 1, ctxRows(), rows, cells, &meta, 0) != ISE_OK || ise_collect(ctx_, &r) != ISE_OK)
             throw std::runtime_error(ise_last_error(ctx_));
         if (verdict_ == ISE_VERDICT_CANONICAL) return true;
@@ -86,6 +86,8 @@ class CGPUCanonicityChecker {
     const uint8_t *permCol() const { return permCol_; }
     const uint8_t *orbits() const { return orbits_; }
     uint64_t groupOrder() const { return order_; }
+    // bits 64..127 of |Aut| (the reference's group order is a 128-bit UInt, Global.h:197-211)
+    uint64_t groupOrderHi() const { return orderHi_; }
     uint8_t failRow() const { return failRow_; }
     uint8_t rowToChange() const { return rowToChange_; }
     uint16_t numAutomorphisms() const { return nGens_; }
@@ -98,5 +100,5 @@ class CGPUCanonicityChecker {
     uint8_t verdict_ = 0, failRow_ = 0, failCol_ = 0, rowToChange_ = 0;
     uint8_t permRow_[ISE_MAX_ROWS], permCol_[ISE_MAX_COLS], orbits_[ISE_MAX_ROWS], improved_[ISE_MAX_COLS];
     uint16_t nGens_ = 0;
-    uint64_t order_ = 0;
+    uint64_t order_ = 0, orderHi_ = 0;
 };

@@ -36,6 +36,12 @@
  *                              (ISE_SAVE_ROW_PERMUTATIONS); unused bytes 255
  *   counters       [n][4] uint32  permutations tried, rows compared, column
  *                              orbits compared, columns examined
+ *   group_order_hi [n] uint64  bits 64..127 of |Aut|: the reference keeps
+ *                              group orders in a 128-bit UInt
+ *                              (k-Factorization/TripleSys/Include/Global.h:197-211),
+ *                              and the row group of a 32-row matrix can
+ *                              exceed 2^64 (32! > 2^117); 0 whenever
+ *                              group_order is not |Aut|
  */
 #ifndef ISE_H
 #define ISE_H
@@ -81,6 +87,7 @@ typedef struct {
     uint8_t  *gens;
     uint32_t *counters;
     uint32_t  gen_cap;
+    uint64_t *group_order_hi;
 } ise_results;
 
 /* Permutation budgets of the first and second pass by default (ise_set_defer). */

@@ -58,6 +58,9 @@ typedef struct {
     uint16_t *n_gens;         /* [n]      automorphisms found (generators recorded) */
     uint8_t  *gens;           /* [n][gen_cap][64] or NULL */
     uint32_t *counters;       /* [n][4]   perms tried, rows compared, orbits compared, cells examined */
+    uint32_t gen_cap;         /* unused here (an argument of orc_check_batch); keeps the layout of the
+                                 library's ise_results so that one structure serves both */
+    uint64_t *group_order_hi; /* [n]      bits 64..127 of |Aut| (the reference's UInt is 128-bit) */
 } orc_results;
 
 /* Canonicity test of a batch of bit-packed candidates.

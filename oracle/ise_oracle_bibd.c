@@ -24,7 +24,8 @@ typedef unsigned char u8;
 typedef struct {
     unsigned short stab;      /* m_nStabLength    */
     unsigned short stab_aut;  /* m_nStabLengthAut */
-    uint64_t order;           /* m_nGroupOrder    */
+    uint64_t order;           /* m_nGroupOrder, bits 0..63 (a 128-bit UInt in the reference) */
+    uint64_t order_hi;        /* bits 64..127 */
 } grp_t;
 
 /* Reverse perm[lo+1 .. hi-1]  (GroupOrder.h:25-28, `revert`). */
@@ -55,7 +56,9 @@ static void grow_group_order(grp_t *g, unsigned short degree, const u8 *orb)
     unsigned short p = g->stab_aut;
     while (++p < degree)
         if (orb[p] == g->stab_aut) len++;
-    g->order *= len;
+    const unsigned __int128 o = (((unsigned __int128)g->order_hi << 64) | g->order) * len;
+    g->order = (uint64_t)o;
+    g->order_hi = (uint64_t)(o >> 64);
 }
 
 /* Record a new automorphism (GroupOrder.h:56-62, :76-79, :88-102). */
@@ -172,6 +175,7 @@ int orc_walk_step(uint8_t *perm, const uint8_t *orbits, int n, int at, int fixed
     g.stab = state[0];
     g.stab_aut = state[1];
     g.order = 1;
+    g.order_hi = 0;
     u8 r = next_row_perm(&g, perm, orbits, (u8)n, (u8)at, (u8)fixed);
     state[0] = g.stab;
     state[1] = g.stab_aut;
@@ -295,6 +299,7 @@ static int run_check_from(const cand_t *cd, orc_results *res, size_t idx, uint32
     g.stab = (unsigned short)(d - 1);
     g.stab_aut = EMAX;
     g.order = 1;
+    g.order_hi = 0;
 
     u8 row = EMAX;
     for (;;) {
@@ -361,6 +366,7 @@ done:
     if (res->fail_col) res->fail_col[idx] = fcol;
     if (res->row_to_change) res->row_to_change[idx] = rtc;
     if (res->group_order) res->group_order[idx] = verdict ? g.order : 0;
+    if (res->group_order_hi) res->group_order_hi[idx] = verdict ? g.order_hi : 0;
     if (res->perm_row) {
         memset(res->perm_row + idx * ORC_MAX_ROWS, 0xff, ORC_MAX_ROWS);
         memcpy(res->perm_row + idx * ORC_MAX_ROWS, perm, (size_t)d);
@@ -1048,6 +1054,7 @@ int orc_check_batch_hunt(uint32_t n, uint32_t stride, const uint64_t *rows, cons
             if (res->fail_col) res->fail_col[c] = fcol;
             if (res->row_to_change) res->row_to_change[c] = row_to_change(perm, frow);
             if (res->group_order) res->group_order[c] = 0;
+            if (res->group_order_hi) res->group_order_hi[c] = 0;
             if (res->perm_row) {
                 memset(res->perm_row + c * ORC_MAX_ROWS, 0xff, ORC_MAX_ROWS);
                 memcpy(res->perm_row + c * ORC_MAX_ROWS, perm, (size_t)d);

@@ -11,7 +11,7 @@ from _helpers import (check_witness, col_sorted, counts_for, design_key, golden,
 
 pytestmark = pytest.mark.gpu
 
-FIELDS_CMP = ("verdict", "fail_row", "fail_col", "row_to_change", "group_order", "perm_row",
+FIELDS_CMP = ("verdict", "fail_row", "fail_col", "row_to_change", "group_order", "group_order_hi", "perm_row",
               "perm_col", "orbits", "n_gens", "counters")
 
 

@@ -15,8 +15,8 @@ from _helpers import check_witness, col_sorted, golden, unpack
 
 pytestmark = pytest.mark.gpu
 
-FIELDS = ("verdict", "fail_row", "fail_col", "row_to_change", "group_order", "perm_row", "orbits", "n_gens",
-          "counters", "gens")
+FIELDS = ("verdict", "fail_row", "fail_col", "row_to_change", "group_order", "group_order_hi", "perm_row",
+          "orbits", "n_gens", "counters", "gens")
 
 
 def assert_same(gpu, orc, gen_cap, what):
@@ -208,3 +208,23 @@ def test_schedule_after_smaller_batches(gpu_ctx, oracle, deepest_first):
             assert_same(g, o, 4, f"schedule {(v, k, lam)}")
     finally:
         gpu_ctx.set_schedule(True)
+
+
+def test_group_orders_past_64_bits(gpu_ctx, oracle):
+    """|Aut| beyond 2^64 (rows of a few values: products of factorials up to
+    32!) in both words, on the column-set search, the uncounted launch and the
+    replayed search, equal to the closed form and to the oracle."""
+    print("case sets group orders past 2^64", flush=True)
+    from test_oracle import group_order_cases
+    cases = group_order_cases()
+    rows = np.zeros((len(cases), 32), np.uint64)
+    meta = np.zeros(len(cases), np.uint32)
+    for i, (mat, _) in enumerate(cases):
+        rows[i, :mat.shape[0]] = ise_amd.pack_rows(mat)
+        meta[i] = ise_amd.pack_meta(mat.shape[0], mat.shape[1], 0, 1)
+    want = [aut for _, aut in cases]
+    g = run(gpu_ctx, oracle, rows, meta, "group orders", gen_cap=4)
+    full = gpu_ctx.check_batch(rows, meta)            # column permutation reported: replayed search
+    for r in (g, full):
+        got = [int(lo) | (int(hi) << 64) for lo, hi in zip(r["group_order"], r["group_order_hi"])]
+        assert got == want

@@ -183,3 +183,42 @@ def test_hunt_never_changes_a_verdict(oracle, seed):
         for i in np.nonzero(h["verdict"] == 0)[0][:200]:
             d, b = int(meta[i]) & 0xff, (int(meta[i]) >> 8) & 0xff
             check_witness(unpack(rows[i, :d], b), h["perm_row"][i], int(h["fail_row"][i]))
+
+
+def _value_class_matrix(counts):
+    """Rows of a few values, orbit form: (11) x counts[0], (10) x counts[1],
+    (00) x counts[2] in two columns (one column when counts has two entries:
+    (1) then (0))."""
+    if len(counts) == 2:
+        return np.array([[1]] * counts[0] + [[0]] * counts[1], np.uint8)
+    return np.array([[1, 1]] * counts[0] + [[1, 0]] * counts[1] + [[0, 0]] * counts[2], np.uint8)
+
+
+GROUP_ORDER_CASES = [(21, 0), (25, 0), (32, 0), (32, 1), (32, 8), (32, 16), (24, 12)]
+GROUP_ORDER_CASES3 = [(12, 10, 10), (16, 8, 8), (1, 30, 1)]
+
+
+def group_order_cases():
+    """(matrix, |Aut|): rows of one value permute freely and neither column
+    order can move a row to another value, so |Aut| (row action) is the
+    product of the factorials of the value multiplicities, past 2^64 here."""
+    from math import factorial
+    out = []
+    for d, a in GROUP_ORDER_CASES:
+        counts = [d - a, a]
+        out.append((_value_class_matrix(counts), factorial(d - a) * factorial(a)))
+    for counts in GROUP_ORDER_CASES3:
+        out.append((_value_class_matrix(list(counts)),
+                    factorial(counts[0]) * factorial(counts[1]) * factorial(counts[2])))
+    return out
+
+
+def test_group_orders_past_64_bits(oracle):
+    """|Aut| as the reference's 128-bit UInt holds it (Global.h:197-211):
+    group_order the low word, group_order_hi the high one."""
+    for mat, aut in group_order_cases():
+        r = oracle.test_canonicity(mat, out_info=1)
+        assert r["verdict"][0] == 1
+        got = int(r["group_order"][0]) | (int(r["group_order_hi"][0]) << 64)
+        assert got == aut, (mat.shape, got, aut)
+    assert any(aut >= 1 << 64 for _, aut in group_order_cases())
