@@ -285,15 +285,21 @@ def kfact_p1f_rate(ctx, steps, warmup, uniform=False):
 def stream_rates(ctx, steps, warmup):
     """The other candidate-stream configurations, plain order, HBM-resident:
     2-(15,3,1) -- every candidate the enumerator tests at depth 4-15 plus 2^20
-    seeded random valid extensions (seed 0x15031); and the uneven batch -- a
+    seeded random valid extensions (seed 0x15031) whose depth is uniform in
+    [4, 15] (rejection sampling of the depth, generated on the GPU); and the
+    uneven batch -- a
     seeded 1:1:1 mix of 2-(15,7,3), 2-(16,6,2) and 2-(19,9,4) random valid
     extensions (2^18 each) at depth 5..v, one persistent launch."""
     import ise_amd
     out = {}
     fields = ("verdict", "fail_row", "row_to_change", "group_order", "perm_row")
     r1, m1 = ctx.enumerator_candidates(15, 3, 1, 4, 15, stride=16)
-    r2, m2 = ise_amd.random_extensions(15, 3, 1, 1 << 20, 4, 15, 0x15031, stride=16)
-    parts = [("2-(15,3,1) enumerator candidates (depth 4-15) + 2^20 random extensions",
+    t_gen = time.perf_counter()
+    r2, m2 = ise_amd.random_extensions(15, 3, 1, 1 << 20, 4, 15, 0x15031, stride=16,
+                                       attempts=ise_amd.RANDOM_MAX_ATTEMPTS, ctx=ctx)
+    t_gen = time.perf_counter() - t_gen
+    hist = np.bincount((m2 & 0xff).astype(np.int64), minlength=16)[4:16]
+    parts = [("2-(15,3,1) enumerator candidates (depth 4-15) + 2^20 random extensions, depth uniform in 4-15",
               np.concatenate([r1, r2]), np.concatenate([m1, m2]), 16)]
     mix_r, mix_m = [], []
     for i, (v, k, lam) in enumerate(((15, 7, 3), (16, 6, 2), (19, 9, 4))):
@@ -321,6 +327,9 @@ def stream_rates(ctx, steps, warmup):
                      "canonical": int((verdict == 1).sum())}
         for b in list(dev.values()) + [d_rows, d_meta, d_head]:
             b.free()
+    first = parts[0][0]
+    out[first]["random_depth_histogram_4_15"] = [int(x) for x in hist]
+    out[first]["random_generation_s"] = t_gen
     ctx.set_launch(group_lanes=64)
     return out
 

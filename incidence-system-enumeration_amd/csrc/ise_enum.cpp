@@ -26,6 +26,7 @@
 #include <unordered_set>
 #include <vector>
 #include "ise_internal.h"
+#include "ise_random.h"
 
 namespace {
 
@@ -365,89 +366,9 @@ long run_enumeration(ise_ctx *ctx, const Design &ds, EnumSink &sink, uint64_t *s
     return sink.ndesigns;
 }
 
-// ---- seeded random valid extensions ----------------------------------------
+// ---- seeded random valid extensions (the walk: ise_random.h) ---------------
 
-inline uint64_t splitmix(uint64_t &s) {
-    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    return z ^ (z >> 31);
-}
-
-inline uint32_t below(uint64_t &s, uint32_t n) { return (uint32_t)(splitmix(s) % n); }
-
-// Draw one random row for position t: r columns taken one at a time,
-// uniformly among the columns whose weight is below k and that keep every
-// intersection with rows[0..t-1] <= lambda.  Returns false when it gets stuck.
-bool draw_row(const Design &ds, const uint64_t *rows, const int *colw, int t, uint64_t &rng,
-              uint64_t *out) {
-    int inter[ISE_MAX_ROWS] = {0};
-    uint64_t row = 0;
-    for (int picked = 0; picked < ds.r; picked++) {
-        uint32_t elig[ISE_MAX_COLS];
-        uint32_t ne = 0;
-        for (int c = 0; c < ds.b; c++) {
-            if ((row >> c) & 1 || colw[c] >= ds.k) continue;
-            bool fits = true;
-            for (int u = 0; u < t && fits; u++)
-                if ((rows[u] >> c) & 1) fits = inter[u] + 1 <= ds.lambda;
-            if (fits) elig[ne++] = (uint32_t)c;
-        }
-        if (!ne) return false;
-        const int c = (int)elig[below(rng, ne)];
-        row |= 1ull << c;
-        for (int u = 0; u < t; u++)
-            if ((rows[u] >> c) & 1) inter[u]++;
-    }
-    *out = row;
-    return true;
-}
-
-// One random partial matrix of up to `depth` rows in orbit order; returns the
-// depth reached.  Rows are drawn one after the other (draw_row); a row that
-// cannot be drawn in 4 tries makes the previous row be drawn again
-// (randomized backtracking).  The walk stops after 256 draws and keeps the
-// deepest partial matrix met, so near-complete depths (where the rows are all
-// but forced) may come out shallower than asked.
-int random_candidate(const Design &ds, int depth, uint64_t &rng, uint64_t *rows_out) {
-    int colw[ISE_MAX_COLS] = {0};
-    int fails[ISE_MAX_ROWS + 1] = {0};
-    uint64_t rows[ISE_MAX_ROWS], best[ISE_MAX_ROWS];
-    int best_depth = 0, t = 0;
-    for (int budget = 256; budget > 0 && best_depth < depth; budget--) {
-        uint64_t row;
-        if (draw_row(ds, rows, colw, t, rng, &row)) {
-            rows[t] = row;
-            for (int c = 0; c < ds.b; c++)
-                if ((row >> c) & 1) colw[c]++;
-            fails[++t] = 0;
-            if (t > best_depth) {
-                best_depth = t;
-                memcpy(best, rows, sizeof(uint64_t) * t);
-            }
-        } else if (++fails[t] >= 4 && t > 0) {
-            --t;   // redraw the previous row
-            for (int c = 0; c < ds.b; c++)
-                if ((rows[t] >> c) & 1) colw[c]--;
-        }
-    }
-    const int d = best_depth;
-    // columns in decreasing lexicographic order (row 0 most significant)
-    uint32_t key[ISE_MAX_COLS];
-    for (int c = 0; c < ds.b; c++) {
-        uint32_t x = 0;
-        for (int i = 0; i < d; i++) x = (x << 1) | (uint32_t)((best[i] >> c) & 1);
-        key[c] = x;
-    }
-    std::sort(key, key + ds.b, [](uint32_t a, uint32_t b) { return a > b; });
-    for (int i = 0; i < d; i++) {
-        uint64_t w = 0;
-        for (int c = 0; c < ds.b; c++)
-            if ((key[c] >> (d - 1 - i)) & 1) w |= 1ull << c;
-        rows_out[i] = w;
-    }
-    return d;
-}
+inline uint64_t splitmix(uint64_t &s) { return ise_rw::splitmix(s); }
 
 } // namespace
 
@@ -521,30 +442,30 @@ long ise_enumerate_bibd_sampled(ise_ctx *ctx, int v, int k, int lambda, uint32_t
     return rc < 0 ? rc : sink.ncand;
 }
 
-int ise_random_extensions(int v, int k, int lambda, uint64_t n, int dmin, int dmax, uint64_t seed,
-                          uint32_t stride, uint64_t *rows_out, uint32_t *meta_out) {
+int ise_random_extensions_ex(ise_ctx *ctx, int v, int k, int lambda, uint64_t n, int dmin, int dmax,
+                             uint64_t seed, uint32_t attempts, uint32_t stride, uint64_t *rows_out,
+                             uint32_t *meta_out) {
     Design ds;
     if (!derive(v, k, lambda, &ds) || !rows_out || !meta_out || dmin < 1 || dmax < dmin ||
-        dmax > v || (uint32_t)dmax > stride)
+        dmax > v || (uint32_t)dmax > stride || attempts < 1 || attempts > ISE_RANDOM_MAX_ATTEMPTS)
         return ISE_ERR_ARG;
+    const ise_rw::Params p{ds.v, ds.b, ds.r, ds.k, ds.lambda};
+    if (ctx) return ise::random_extensions_device(ctx, p, n, dmin, dmax, seed, attempts, stride, rows_out, meta_out);
     const unsigned nt = n < 1024 ? 1u : host_threads();
     auto work = [&](unsigned t) {
-        for (uint64_t i = t; i < n; i += nt) {
-            uint64_t rng = seed ^ (0xd1342543de82ef95ull * (i + 1));
-            splitmix(rng);
-            const int target = dmin + (int)below(rng, (uint32_t)(dmax - dmin + 1));
-            uint64_t *dst = rows_out + i * stride;
-            memset(dst, 0, stride * sizeof(uint64_t));
-            const int depth = random_candidate(ds, target, rng, dst);
-            const uint32_t oi = depth == ds.v ? (ISE_SAVE_ROW_PERMUTATIONS | ISE_SAVE_ROW_TO_CHANGE) : 0u;
-            meta_out[i] = (uint32_t)depth | ((uint32_t)ds.b << 8) | (oi << 24);
-        }
+        for (uint64_t i = t; i < n; i += nt)
+            ise_rw::batch_candidate(p, i, dmin, dmax, seed, attempts, stride, rows_out, meta_out);
     };
     std::vector<std::thread> th;
     for (unsigned t = 1; t < nt; t++) th.emplace_back(work, t);
     work(0);
     for (auto &x : th) x.join();
     return ISE_OK;
+}
+
+int ise_random_extensions(int v, int k, int lambda, uint64_t n, int dmin, int dmax, uint64_t seed,
+                          uint32_t stride, uint64_t *rows_out, uint32_t *meta_out) {
+    return ise_random_extensions_ex(nullptr, v, k, lambda, n, dmin, dmax, seed, 1u, stride, rows_out, meta_out);
 }
 
 } // extern "C"

@@ -112,6 +112,7 @@ def lib():
                                                  ctypes.POINTER(ctypes.c_long), P]
         L.ise_enumerate_bibd_sampled.restype = ctypes.c_long
         L.ise_random_extensions.argtypes = [I, I, I, U64, I, I, U64, U32, P, P]
+        L.ise_random_extensions_ex.argtypes = [P, I, I, I, U64, I, I, U64, U32, U32, P, P]
         F = ctypes.c_float
         L.ise_mlp_create.argtypes = [P, ctypes.POINTER(P), U32, U64]
         L.ise_mlp_destroy.argtypes = [P]
@@ -742,11 +743,20 @@ class DeviceArray:
             pass
 
 
-def random_extensions(v, k, lam, n, dmin, dmax, seed, stride=32):
-    """Seeded random valid partial matrices (host code, no device needed)."""
+RANDOM_MAX_ATTEMPTS = 4096
+
+
+def random_extensions(v, k, lam, n, dmin, dmax, seed, stride=32, attempts=1, ctx=None):
+    """Seeded random valid partial matrices.  attempts > 1: a walk that falls
+    short of its depth starts again (rejection sampling of the depth).  With a
+    Context the batch is generated on its GPU (same candidates as the host)."""
     rows = np.zeros((n, stride), np.uint64)
     meta = np.zeros(n, np.uint32)
-    rc = lib().ise_random_extensions(v, k, lam, n, dmin, dmax, seed, stride, _ptr(rows), _ptr(meta))
+    if attempts == 1 and ctx is None:
+        rc = lib().ise_random_extensions(v, k, lam, n, dmin, dmax, seed, stride, _ptr(rows), _ptr(meta))
+    else:
+        rc = lib().ise_random_extensions_ex(ctx.h if ctx is not None else None, v, k, lam, n, dmin, dmax, seed,
+                                            attempts, stride, _ptr(rows), _ptr(meta))
     if rc != OK:
         raise ISEError(rc, "ise_random_extensions")
     return rows, meta

@@ -681,13 +681,27 @@ long ise_enumerate_bibd_sampled(ise_ctx *ctx, int v, int k, int lambda, uint32_t
 
 /* Seeded random valid extensions: each new row takes r columns uniformly
  * among those whose weight is below k and that keep every pairwise row
- * intersection <= lambda (restart when stuck); columns are then sorted in
+ * intersection <= lambda (randomized backtracking, 256 draws); columns are then sorted in
  * decreasing lexicographic order so that orbits are contiguous.  Depth is
  * uniform in [dmin, dmax].  Writes n candidates (stride words each) and
  * meta.  Deterministic for a given seed.  Returns ISE_OK or a status. */
 int ise_random_extensions(int v, int k, int lambda, uint64_t n, int dmin, int dmax,
                           uint64_t seed, uint32_t stride, uint64_t *rows_out,
                           uint32_t *meta_out);
+
+/* The same with rejection sampling of the depth: a walk that falls short of
+ * its candidate's depth starts again from the empty matrix, up to `attempts`
+ * walks (1 = ise_random_extensions; near-complete depths, where the rows are
+ * all but forced, are reached by few walks), and the deepest matrix met is
+ * kept.  A candidate whose first walk reaches its depth is the same for every
+ * `attempts`.  With ctx the batch is generated on ctx's device, one thread
+ * per candidate, identical to the host one (host output buffers); with NULL,
+ * on the host threads.  No counterpart in the reference (stress batches of
+ * the benchmark).  Returns ISE_OK or a status. */
+#define ISE_RANDOM_MAX_ATTEMPTS 4096u
+int ise_random_extensions_ex(ise_ctx *ctx, int v, int k, int lambda, uint64_t n, int dmin, int dmax,
+                             uint64_t seed, uint32_t attempts, uint32_t stride, uint64_t *rows_out,
+                             uint32_t *meta_out);
 
 #ifdef __cplusplus
 }

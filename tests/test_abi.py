@@ -65,3 +65,50 @@ def test_pack_roundtrip():
     mat = (rng.random((9, 50)) < 0.4).astype(np.uint8)
     assert np.array_equal(ise_amd.unpack_rows(ise_amd.pack_rows(mat), 50), mat)
     assert ise_amd.pack_meta(13, 26, 2, 5) == 13 | (26 << 8) | (2 << 16) | (5 << 24)
+
+
+def _targets(n, dmin, dmax, seed):
+    """Each candidate's depth as the generator draws it: its own splitmix64
+    state, seeded from (seed, index), first output skipped."""
+    M = (1 << 64) - 1
+
+    def splitmix(s):
+        s = (s + 0x9E3779B97F4A7C15) & M
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return s, z ^ (z >> 31)
+
+    out = []
+    for i in range(n):
+        s = seed ^ ((0xD1342543DE82EF95 * (i + 1)) & M)
+        s, _ = splitmix(s)
+        s, z = splitmix(s)
+        out.append(dmin + z % (dmax - dmin + 1))
+    return np.array(out)
+
+
+def test_rejection_sampling_reaches_every_depth():
+    """2-(15,3,1) stress batch, depth uniform in [4, 15]: with rejection
+    sampling every candidate has the depth drawn for it (depth 15 = a
+    complete Steiner triple system, every pair of rows meeting once), and a
+    candidate whose first walk reached its depth is the one-walk candidate."""
+    n, seed = 480, 0x15031
+    one, meta1 = ise_amd.random_extensions(15, 3, 1, n, 4, 15, seed, stride=16)
+    rows, meta = ise_amd.random_extensions(15, 3, 1, n, 4, 15, seed, stride=16, attempts=4096)
+    want = _targets(n, 4, 15, seed)
+    assert np.array_equal(meta & 0xff, want)
+    assert ((meta1 & 0xff) <= want).all() and ((meta1 & 0xff) < want).any()
+    same = (meta1 & 0xff) == want
+    assert np.array_equal(rows[same], one[same]) and np.array_equal(meta[same], meta1[same])
+    for i in range(n):
+        d = int(want[i])
+        m = unpack(rows[i, :d], 35).astype(int)
+        assert (m.sum(axis=1) == 7).all() and (m.sum(axis=0) <= 3).all()
+        g = m @ m.T
+        off = g[~np.eye(d, dtype=bool)]
+        assert (off <= 1).all()
+        if d == 15:
+            assert (off == 1).all() and (m.sum(axis=0) == 3).all()
+    with pytest.raises(ise_amd.ISEError):
+        ise_amd.random_extensions(15, 3, 1, 4, 4, 15, seed, attempts=ise_amd.RANDOM_MAX_ATTEMPTS + 1)
