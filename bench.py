@@ -557,7 +557,8 @@ def main():
             "hbm": {"algorithmic_bytes": hbm_bytes, "achieved_gbs": achieved_gbs, "peak_gbs": HBM_PEAK_GBS,
                     "frac": achieved_gbs / HBM_PEAK_GBS},
             "kernel": "check_kernel<64,false,false,true> (column-set search)",
-            "binding": "instruction issue (scalar) and the dependency latency of each search step",
+            "binding": "instruction issue (scalar and vector, near even) and the dependency latency of "
+                       "each search step",
         },
         # what binds: instruction issue (rocprofv3 SQ counters of one plain launch of the
         # same kernel sources; null when the committed profile is of other sources)
