@@ -80,10 +80,18 @@ hipError_t kf_launch(uint32_t blocks, uint32_t range_blocks, uint32_t ncand, uin
 // candidate, candidates by grid stride.
 hipError_t kf2_launch(uint32_t blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
                       const uint32_t *meta, const KfOut &o, hipStream_t stream);
-// Two-row canonizer of uniform 1-factorizations (cnvCheck2U1F): one wave per
-// candidate, candidates by grid stride.
+// Two-row canonizer of uniform 1-factorizations (cnvCheck2U1F, md.reps = 1)
+// or cnvCheck3U1F for groups of 2 (md.reps = 32, the cycle types the run
+// allows in md.types: (cycles << 16) | half lengths a nibble each, ascending;
+// none: one cycle through every player): one wave per candidate, candidates by
+// grid stride.
+constexpr uint32_t KF_U1F_MAX_TYPES = 4;
+struct KfU1fMode {
+    uint32_t reps, ntypes;
+    uint32_t types[KF_U1F_MAX_TYPES];
+};
 hipError_t kf_u1f_launch(uint32_t blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
-                         const uint32_t *meta, const KfOut &o, hipStream_t stream);
+                         const uint32_t *meta, const KfOut &o, const KfU1fMode &md, hipStream_t stream);
 hipError_t kf_occupancy(int *blocks_per_cu);
 // The k-Sys backjump position of every candidate (o.player_index) from the
 // first witness the canonizer of `mode` (0 general, 1 two-row P1F, 2 two-row

@@ -1061,7 +1061,7 @@ __device__ __forceinline__ int kf_u1f_compare(const KfLds &s, const KfU1fLds &x,
 
 __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t stride,
                                                     const uint64_t *__restrict__ rounds,
-                                                    const uint32_t *__restrict__ meta, KfOut o) {
+                                                    const uint32_t *__restrict__ meta, KfOut o, KfU1fMode md) {
     __shared__ KfLds s;
     __shared__ KfU1fLds x;
     const uint32_t l = threadIdx.x;
@@ -1162,6 +1162,23 @@ __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t str
             total = nr * (nr - 1) * T;
             w.cut = bad == 0xffffffffu ? total : 2u * bad * T;
         }
+        if (md.reps > 1 && nr > 1) {
+            // cnvCheck3U1F: rounds 0, 1 of a type the run does not allow -- not canonical
+            // before any relabelling (cyclesFor2Rows collects no cycle set)
+            bool allowed = md.ntypes == 0 && w.ncyc == 1 && 2u * (w.len01 & 0xfu) == n;
+            for (uint32_t k = 0; k < md.ntypes && k < KF_U1F_MAX_TYPES; k++)
+                allowed = allowed || md.types[k] == ((w.ncyc << 16) | w.len01);
+            if (!allowed) {
+                if (l == 0) {
+                    o.verdict[ci] = ISE_VERDICT_NONCANONICAL;
+                    if (o.witness) o.witness[ci] = ~0ull;
+                    if (o.n_auts) o.n_auts[ci] = 0ull;
+                    if (o.n_tried) o.n_tried[ci] = 0ull;
+                }
+                __syncthreads();
+                continue;
+            }
+        }
         const auto full = [&](uint32_t a) {
             return kf_u1f_compare(s, x, u, w, a >> 16, a & 0xffffu, 0xffffffffu);
         };
@@ -1186,13 +1203,27 @@ __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t str
         const uint32_t witness = sc.witness;
         const uint64_t auts = sc.auts;
         if (l == 0) {
+            // cnvCheck3U1F tries every pair's relabellings md.reps times over (its loop over
+            // MAX_CYCLE_SETS cycle sets, one set for groups of 2): index ((ordered pair * reps) +
+            // repetition) * T + position, a witness met in the first repetition
+            const uint64_t R = md.reps;
             if (witness == KF_NONE && w.cut < total) {   // a pair of another cycle type
                 o.verdict[ci] = ISE_VERDICT_NONCANONICAL;
                 if (o.witness) o.witness[ci] = ~0ull;
                 if (o.n_auts) o.n_auts[ci] = auts;
-                if (o.n_tried) o.n_tried[ci] = w.cut;
-            } else {
+                if (o.n_tried) o.n_tried[ci] = (uint64_t)w.cut * R;
+            } else if (R == 1) {
                 kf_write(o, ci, witness, auts, total);
+            } else if (witness == KF_NONE) {
+                kf_write(o, ci, witness, auts, 0u);
+                if (o.n_tried) o.n_tried[ci] = (uint64_t)total * R;
+            } else {
+                const uint32_t po = witness / w.T, t = witness - po * w.T;
+                const uint64_t r3 = (uint64_t)po * R * w.T + t;
+                o.verdict[ci] = ISE_VERDICT_NONCANONICAL;
+                if (o.witness) o.witness[ci] = r3;
+                if (o.n_auts) o.n_auts[ci] = auts;
+                if (o.n_tried) o.n_tried[ci] = r3 + 1;
             }
         }
         __syncthreads();
@@ -1200,8 +1231,8 @@ __global__ __launch_bounds__(64) void kf_u1f_kernel(uint32_t ncand, uint32_t str
 }
 
 hipError_t kf_u1f_launch(uint32_t blocks, uint32_t ncand, uint32_t stride, const uint64_t *rounds,
-                         const uint32_t *meta, const KfOut &o, hipStream_t stream) {
-    hipLaunchKernelGGL(kf_u1f_kernel, dim3(blocks), dim3(64), 0, stream, ncand, stride, rounds, meta, o);
+                         const uint32_t *meta, const KfOut &o, const KfU1fMode &md, hipStream_t stream) {
+    hipLaunchKernelGGL(kf_u1f_kernel, dim3(blocks), dim3(64), 0, stream, ncand, stride, rounds, meta, o, md);
     return hipGetLastError();
 }
 

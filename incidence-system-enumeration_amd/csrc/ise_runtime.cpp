@@ -904,10 +904,39 @@ int ise_kf_check_batch_device_timed(ise_ctx *c, uint32_t n, uint32_t stride, con
 }
 
 // The two-row canonizers: perfect (p1f) or uniform 1-factorizations.
-enum KfMode { KF_GENERAL = 0, KF_2P1F = 1, KF_2U1F = 2 };
+enum KfMode { KF_GENERAL = 0, KF_2P1F = 1, KF_2U1F = 2, KF_3U1F = 3 };
+
+// cnvCheck3U1F's repetitions of a pair's relabellings for groups of 2 (its loop
+// over MAX_CYCLE_SETS cycle sets, k-Factorization/TripleSys/Include/Global.h:12)
+constexpr uint32_t KF_3U1F_REPS = 32;
+
+// The cycle types a 3U1F run allows ([n_types][4] cycle lengths, ascending,
+// zero after the last) in the kernel's form; false for a malformed list.
+static bool kf_u1f_mode(const uint8_t *u1f_cycles, uint32_t n_types, ise::KfU1fMode *md) {
+    *md = ise::KfU1fMode{KF_3U1F_REPS, n_types, {0u, 0u, 0u, 0u}};
+    if (n_types > ISE_KF_MAX_U1F_TYPES || (n_types && !u1f_cycles)) return false;
+    for (uint32_t k = 0; k < n_types; k++) {
+        uint32_t nc = 0, halves = 0, prev = 0;
+        for (uint32_t c = 0; c < ISE_KF_MAX_U1F_CYCLES; c++) {
+            const uint32_t len = u1f_cycles[k * ISE_KF_MAX_U1F_CYCLES + c];
+            if (!len) {
+                for (uint32_t e = c; e < ISE_KF_MAX_U1F_CYCLES; e++)
+                    if (u1f_cycles[k * ISE_KF_MAX_U1F_CYCLES + e]) return false;
+                break;
+            }
+            if (len < 4 || (len & 1u) || len > 2u * 15u || len < prev) return false;
+            halves |= (len / 2u) << (4 * nc++);
+            prev = len;
+        }
+        if (!nc) return false;
+        md->types[k] = (nc << 16) | halves;
+    }
+    return true;
+}
 
 static int kf2_launch_on(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
-                         const uint32_t *d_meta, const ise_kf_results &d_out, hipStream_t s, int mode) {
+                         const uint32_t *d_meta, const ise_kf_results &d_out, hipStream_t s, int mode,
+                         const ise::KfU1fMode *md3 = nullptr) {
     uint32_t blocks = c->cfg.blocks;
     if (!blocks) {
         int per_cu = 0;
@@ -915,12 +944,16 @@ static int kf2_launch_on(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t
         blocks = (uint32_t)(per_cu < 1 ? 1 : per_cu) * (uint32_t)c->num_cus;
         if (blocks > n) blocks = n ? n : 1;
     }
-    if (mode == KF_2U1F)
-        HIPCHK(c, ise::kf_u1f_launch(blocks, n, stride, d_rounds, d_meta, kf_out(d_out), s));
-    else
+    if (mode == KF_2U1F || mode == KF_3U1F) {
+        const ise::KfU1fMode one{1u, 0u, {0u, 0u, 0u, 0u}};
+        HIPCHK(c, ise::kf_u1f_launch(blocks, n, stride, d_rounds, d_meta, kf_out(d_out),
+                                     mode == KF_3U1F ? *md3 : one, s));
+    } else {
         HIPCHK(c, ise::kf2_launch(blocks, n, stride, d_rounds, d_meta, kf_out(d_out), s));
-    HIPCHK(c, ise::kf_backjump_launch(mode == KF_2U1F ? 2 : 1, n, stride, d_rounds, d_meta, kf_out(d_out),
-                                      c->kf_tab, s));
+    }
+    if (mode != KF_3U1F)
+        HIPCHK(c, ise::kf_backjump_launch(mode == KF_2U1F ? 2 : 1, n, stride, d_rounds, d_meta, kf_out(d_out),
+                                          c->kf_tab, s));
     return ISE_OK;
 }
 
@@ -942,6 +975,20 @@ int ise_kf_check_2u1f_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, cons
     if (!d_rounds || !d_meta) return ISE_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     return kf2_launch_on(c, n, stride, d_rounds, d_meta, *d_out, stream ? (hipStream_t)stream : c->stream, KF_2U1F);
+}
+
+int ise_kf_check_3u1f_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                                   const uint32_t *d_meta, const uint8_t *u1f_cycles, uint32_t n_types,
+                                   const ise_kf_results *d_out, void *stream) {
+    if (!c || !d_out || !d_out->verdict || d_out->player_index || stride < 1 || stride > ISE_MAX_ROWS)
+        return ISE_ERR_ARG;
+    ise::KfU1fMode md;
+    if (!kf_u1f_mode(u1f_cycles, n_types, &md)) return ISE_ERR_ARG;
+    if (n == 0) return ISE_OK;
+    if (!d_rounds || !d_meta) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    return kf2_launch_on(c, n, stride, d_rounds, d_meta, *d_out, stream ? (hipStream_t)stream : c->stream, KF_3U1F,
+                         &md);
 }
 
 // `repeats` launches of `launch_one` on the context stream, each timed with HIP events.
@@ -1025,7 +1072,7 @@ static bool kf_records_ok(uint32_t n, uint32_t stride, const uint64_t *rounds, c
 }
 
 static int kf_host_run(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
-                       ise_kf_results *h, int mode) {
+                       ise_kf_results *h, int mode, const ise::KfU1fMode *md3 = nullptr) {
     if (!c || !h || !h->verdict || (n && (!rounds || !meta)) || stride < 1 || stride > c->stride)
         return ISE_ERR_ARG;
     if (n > c->capacity) return ISE_ERR_CAPACITY;
@@ -1038,7 +1085,7 @@ static int kf_host_run(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *
     // array reinterpreted (one uint16 per candidate fits in perm_row's 32 bytes)
     ise_kf_results d{c->d.verdict, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried,
                      h->player_index ? (uint16_t *)c->d.perm_row : nullptr};
-    const int rc = mode != KF_GENERAL ? kf2_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->stream, mode)
+    const int rc = mode != KF_GENERAL ? kf2_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->stream, mode, md3)
                                       : kf_launch_on(c, n, stride, c->d_rows, c->d_meta, d, c->d_head, c->stream);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(h->verdict, d.verdict, n, hipMemcpyDeviceToHost, c->stream));
@@ -1064,6 +1111,13 @@ int ise_kf_check_2p1f(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *r
 int ise_kf_check_2u1f(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
                       ise_kf_results *h) {
     return kf_host_run(c, n, stride, rounds, meta, h, KF_2U1F);
+}
+
+int ise_kf_check_3u1f(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rounds, const uint32_t *meta,
+                      const uint8_t *u1f_cycles, uint32_t n_types, ise_kf_results *h) {
+    ise::KfU1fMode md;
+    if (!h || h->player_index || !kf_u1f_mode(u1f_cycles, n_types, &md)) return ISE_ERR_ARG;
+    return kf_host_run(c, n, stride, rounds, meta, h, KF_3U1F, &md);
 }
 
 int ise_set_order(ise_ctx *c, int mode, uint32_t hunt_budget, const uint8_t *d_order, uint32_t order_dim) {

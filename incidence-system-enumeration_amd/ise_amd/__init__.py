@@ -146,6 +146,8 @@ def lib():
         L.ise_kf_relabel_p1f.argtypes = [P, P, U32, U32, U64, I, U64, U32, P, P]
         L.ise_kf_relabel_u1f.argtypes = [P, P, U32, U32, U64, I, U64, U32, P, P]
         L.ise_kf_check_2u1f.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct)]
+        L.ise_kf_check_3u1f.argtypes = [P, U32, U32, P, P, P, U32, ctypes.POINTER(KfResultsStruct)]
+        L.ise_kf_check_3u1f_batch_device.argtypes = [P, U32, U32, P, P, P, U32, ctypes.POINTER(KfResultsStruct), P]
         L.ise_kf_check_2u1f_batch_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct), P]
         L.ise_kf_check_2u1f_batch_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(KfResultsStruct),
                                                            U32, P]
@@ -453,6 +455,34 @@ class Context:
                                                             ms.ctypes.data))
         return ms
 
+    def kf_check_3u1f(self, rounds, meta, u1f_cycles=None):
+        """cnvCheck3U1F (groups of 2, complete graphs) on host arrays; u1f_cycles:
+        the run's cycle types, a list of ascending cycle-length lists (None: one
+        cycle through every player)."""
+        types = _u1f_types(u1f_cycles)
+        rounds = np.ascontiguousarray(rounds, dtype=np.uint64)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32)
+        n, stride = rounds.shape
+        outs = []
+        for off in range(0, n, self.capacity):
+            cnt = min(self.capacity, n - off)
+            r = kf_alloc(cnt)
+            s = kf_struct(r)
+            self._check(self._L.ise_kf_check_3u1f(self.h, cnt, stride, rounds[off:off + cnt].ctypes.data,
+                                                  meta[off:off + cnt].ctypes.data,
+                                                  types.ctypes.data if len(types) else None, len(types),
+                                                  ctypes.byref(s)))
+            outs.append(r)
+        if not outs:
+            return kf_alloc(0)
+        return {k: np.concatenate([o[k] for o in outs]) for k in outs[0]}
+
+    def kf_check_3u1f_device(self, n, stride, d_rounds, d_meta, res_struct, u1f_cycles=None):
+        types = _u1f_types(u1f_cycles)
+        self._check(self._L.ise_kf_check_3u1f_batch_device(self.h, n, stride, d_rounds.ptr, d_meta.ptr,
+                                                           types.ctypes.data if len(types) else None, len(types),
+                                                           ctypes.byref(res_struct), None))
+
     def kf_check_2p1f_device(self, n, stride, d_rounds, d_meta, res_struct):
         self._check(self._L.ise_kf_check_2p1f_batch_device(self.h, n, stride, d_rounds.ptr, d_meta.ptr,
                                                            ctypes.byref(res_struct), None))
@@ -744,6 +774,16 @@ class DeviceArray:
 
 
 RANDOM_MAX_ATTEMPTS = 4096
+KF_MAX_U1F_TYPES = 4
+KF_MAX_U1F_CYCLES = 4
+
+
+def _u1f_types(u1f_cycles):
+    """The run's cycle types as [n_types][4] uint8 (ascending lengths, zero after the last)."""
+    types = np.zeros((len(u1f_cycles or ()), KF_MAX_U1F_CYCLES), np.uint8)
+    for k, lens in enumerate(u1f_cycles or ()):
+        types[k, :len(lens)] = sorted(lens)
+    return types
 
 
 def random_extensions(v, k, lam, n, dmin, dmax, seed, stride=32, attempts=1, ctx=None):

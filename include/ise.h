@@ -494,6 +494,35 @@ int ise_kf_check_2u1f_batch_device_timed(ise_ctx *ctx, uint32_t n, uint32_t stri
                                          const uint64_t *d_rounds, const uint32_t *d_meta,
                                          const ise_kf_results *d_out, uint32_t repeats, float *ms_out);
 
+/* alldata::cnvCheck3U1F (k-Factorization/TripleSys/Source/cnvSupport.cpp:94-339)
+ * for groups of 2 on complete graphs, the 1-factorizations this library takes,
+ * with the reference's defaults there (AutSaveTestedTrs 0, undefined cycles not
+ * allowed, no second-row database): the check the reference selects for
+ * groups of 2 with UseFastCanonizerForG2, for canonized input and for the
+ * second-row generation (allData.cpp:123-143).  The run's cycle types are
+ * u1f_cycles[n_types][ISE_KF_MAX_U1F_CYCLES] (the u1fCycles parameter: cycle
+ * lengths in players, ascending, zero after the last); n_types 0 means one
+ * cycle through every player.  Rounds 0, 1 of a type not listed make the
+ * matrix not canonical before any relabelling (cyclesFor2Rows collects no
+ * cycle set: witness ~0, n_tried 0).  Otherwise the relabellings are those of
+ * ise_kf_check_2u1f, each ordered pair's tried 32 times over: the loop over
+ * MAX_CYCLE_SETS (Global.h:12) cycle sets (cnvSupport.cpp:208-216) recomputes
+ * the same set for groups of 2 (collectOneCyclesSet, :341-352), and the
+ * repeated relabellings add nothing to the group (n_auts counts distinct
+ * automorphisms).  Index: ((ordered pair * 32) + repetition) * T + position,
+ * T and position as for ise_kf_check_2u1f; n_tried counts every relabelling
+ * tried.  Groups of 3 and graphs that are not complete are outside the
+ * matrices this library takes.  player_index is not returned (ISE_ERR_ARG).
+ * Device path: no allocation. */
+#define ISE_KF_MAX_U1F_TYPES  4
+#define ISE_KF_MAX_U1F_CYCLES 4
+int ise_kf_check_3u1f(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *rounds,
+                      const uint32_t *meta, const uint8_t *u1f_cycles, uint32_t n_types,
+                      ise_kf_results *out_host);
+int ise_kf_check_3u1f_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rounds,
+                                   const uint32_t *d_meta, const uint8_t *u1f_cycles, uint32_t n_types,
+                                   const ise_kf_results *d_out, void *stream);
+
 /* Seeded relabelled prefixes of perfect 1-factorizations (a candidate stream
  * for the two-row canonizer): for candidate i a source matrix, an ordered pair
  * of its rounds and a start player are drawn, the pair is sent onto the

@@ -193,6 +193,12 @@ int orc_kf_check_2p1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
  * rounds of another cycle type than rounds 0, 1: not canonical, witness ~0). */
 int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
                       uint64_t *n_auts, uint64_t *n_tried);
+/* cnvCheck3U1F for groups of 2 on complete graphs: the run's cycle types
+ * types[ntypes][4] (ascending lengths, zero after the last; none: one cycle
+ * through every player); witness = ((ordered pair * 32) + repetition) * T +
+ * position; n_auts = distinct automorphisms found. */
+int orc_kf_check_3u1f(const uint8_t *mat, int n, int nrounds, const uint8_t *types, int ntypes,
+                      uint64_t *witness, uint64_t *n_auts, uint64_t *n_tried);
 
 /* Fuchs' countdown QuickPerm order of the pair slots (k! rows of k bytes,
  * each byte the base label 2*slot); returns k!. */

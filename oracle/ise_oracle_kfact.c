@@ -361,6 +361,68 @@ static int kf_next_perm(int *p, int m)
     return 1;
 }
 
+/* One mapping of a pair's cycles onto those of rounds 0, 1 (CycleMapping,
+ * cycles.cpp:111-211): direction and start offset per cycle, and the order of
+ * the cycles of one length. */
+typedef struct { int dir[KF_MAXCYC], off[KF_MAXCYC], perm[KF_MAXCYC], sout[KF_MAXCYC]; } u1f_map;
+
+static void u1f_map_first(u1f_map *m, const kf_cycles *cy)
+{
+    for (int c = 0; c < KF_MAXCYC; c++) { m->dir[c] = m->off[c] = 0; m->perm[c] = c; }
+    for (int c = 0; c < cy->n; c++) m->sout[c] = cy->start[c];
+}
+
+/* ProceedToNextMapping: directions fastest (last cycle first), then offsets
+ * (first cycle first), then the permutations of each group of equal lengths */
+static void u1f_map_next(u1f_map *m, const kf_cycles *cy)
+{
+    int i = cy->n;
+    while (i-- && m->dir[i]) m->dir[i] = 0;
+    if (i >= 0) { m->dir[i] = 1; return; }
+    int c = 0;
+    while (c < cy->n && m->off[c] + 1 == cy->len[c] / 2) m->off[c++] = 0;
+    if (c < cy->n) { m->off[c]++; return; }
+    for (int g0 = 0, g1; g0 < cy->n; g0 = g1) {
+        for (g1 = g0; g1 < cy->n && cy->len[g1] == cy->len[g0]; g1++) {}
+        if (kf_next_perm(m->perm + g0, g1 - g0)) {
+            for (int j = g0; j < g1; j++) m->sout[j] = cy->start[g0 + m->perm[j] - g0];
+            return;
+        }
+        for (int j = g0; j < g1; j++) m->perm[j] = j;
+    }
+}
+
+/* createU1FTr: the relabelling sending the pair's cycles, as mapped, onto
+ * those of rounds 0, 1 */
+static void u1f_map_tr(const u1f_map *m, const kf_cycles *cy, const kf_cycles *ref, u8 *tr)
+{
+    for (int c = 0; c < cy->n; c++) {
+        const int L2 = 2 * cy->len[c];
+        int ip = m->dir[c] ? 4 * m->off[c] + 1 : 4 * m->off[c];
+        const int step = m->dir[c] ? -1 : 1;
+        const u8 *fp = cy->path + 2 * m->sout[c];
+        const u8 *fp01 = ref->path + 2 * ref->start[c];
+        for (int j = 0; j < L2; j++, ip += step) {
+            if (ip >= L2) ip -= L2;
+            else if (ip < 0) ip += L2;
+            tr[fp[ip]] = fp01[j];
+        }
+    }
+}
+
+/* relabellings per pair: the cycle lengths' product (offsets x directions)
+ * times the orders of the cycles of equal lengths */
+static uint64_t u1f_per_pair(const kf_cycles *ref)
+{
+    uint64_t T = 1;
+    for (int c = 0; c < ref->n; c++) T *= (uint64_t)ref->len[c];
+    for (int c = 0, g = 0; c < ref->n; c = g) {
+        for (g = c; g < ref->n && ref->len[g] == ref->len[c]; g++) {}
+        for (int f = 2; f <= g - c; f++) T *= (uint64_t)f;
+    }
+    return T;
+}
+
 int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
                       uint64_t *n_auts, uint64_t *n_tried)
 {
@@ -375,12 +437,7 @@ int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
         kf_cycles ref;
         kf_pair_cycles(part[0], part[1], n, &ref);
         if (ref.n > KF_MAXCYC) return -1;
-        uint64_t T = 1;   /* relabellings per pair */
-        for (int c = 0; c < ref.n; c++) T *= (uint64_t)ref.len[c];
-        for (int c = 0, g = 0; c < ref.n; c = g) {
-            for (g = c; g < ref.n && ref.len[g] == ref.len[c]; g++) {}
-            for (int f = 2; f <= g - c; f++) T *= (uint64_t)f;
-        }
+        const uint64_t T = u1f_per_pair(&ref);
         for (int last = 1; last < nrounds && verdict; last++)
             for (int ind = 0; ind < last && verdict; ind++)
                 for (int swap = 0; swap < 2 && verdict; swap++) {
@@ -390,42 +447,12 @@ int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
                     int same = cy.n == ref.n;
                     for (int c = 0; c < ref.n && same; c++) same = cy.len[c] == ref.len[c];
                     if (!same) { verdict = 0; break; }
-                    int dir[KF_MAXCYC] = {0}, off[KF_MAXCYC] = {0}, perm[KF_MAXCYC], sout[KF_MAXCYC];
-                    for (int c = 0; c < cy.n; c++) { perm[c] = c; sout[c] = cy.start[c]; }
+                    u1f_map m;
+                    u1f_map_first(&m, &cy);
                     for (uint64_t t = 0; t < T; t++, idx++) {
-                        if (t) {   /* ProceedToNextMapping */
-                            int i = cy.n;
-                            while (i-- && dir[i]) dir[i] = 0;
-                            if (i >= 0) dir[i] = 1;
-                            else {
-                                int c = 0;
-                                while (c < cy.n && off[c] + 1 == cy.len[c] / 2) off[c++] = 0;
-                                if (c < cy.n) off[c]++;
-                                else {
-                                    for (int g0 = 0, g1; g0 < cy.n; g0 = g1) {
-                                        for (g1 = g0; g1 < cy.n && cy.len[g1] == cy.len[g0]; g1++) {}
-                                        if (kf_next_perm(perm + g0, g1 - g0)) {
-                                            for (int j = g0; j < g1; j++) sout[j] = cy.start[g0 + perm[j] - g0];
-                                            break;
-                                        }
-                                        for (int j = g0; j < g1; j++) perm[j] = j;
-                                    }
-                                }
-                            }
-                        }
-                        u8 tr[KF_MAX_N];   /* createU1FTr */
-                        for (int c = 0; c < cy.n; c++) {
-                            const int L2 = 2 * cy.len[c];
-                            int ip = dir[c] ? 4 * off[c] + 1 : 4 * off[c];
-                            const int step = dir[c] ? -1 : 1;
-                            const u8 *fp = cy.path + 2 * sout[c];
-                            const u8 *fp01 = ref.path + 2 * ref.start[c];
-                            for (int j = 0; j < L2; j++, ip += step) {
-                                if (ip >= L2) ip -= L2;
-                                else if (ip < 0) ip += L2;
-                                tr[fp[ip]] = fp01[j];
-                            }
-                        }
+                        if (t) u1f_map_next(&m, &cy);
+                        u8 tr[KF_MAX_N];
+                        u1f_map_tr(&m, &cy, &ref, tr);
                         u8 mo[KF_MAX_N * KF_MAX_N];
                         normal_form(mat, nrounds, n, tr, mo);
                         tried++;
@@ -441,6 +468,107 @@ int orc_kf_check_2u1f(const uint8_t *mat, int n, int nrounds, uint64_t *witness,
                 }
     }
     if (n_auts) *n_auts = auts;
+    if (n_tried) *n_tried = tried;
+    return verdict;
+}
+
+/*
+ * cnvCheck3U1F (k-Factorization/TripleSys/Source/cnvSupport.cpp:94-339) for
+ * groups of 2 on complete graphs, with the defaults of such runs: no
+ * second-row database (p1 = result(1), the current set, one pass of the outer
+ * loop), AutSaveTestedTrs 0, undefined cycles not allowed, early exit.
+ *  - cyclesFor2Rows (p1fCheck.cpp:769-795) collects the cycle set of rounds
+ *    0, 1 checked with eCheckErrors: every cycle length must be one the run
+ *    allows (cycleLengthOk, p1fCheck.cpp:540-558) and the set one of its types
+ *    (cyclesNotOk, :519-538); `types` ([ntypes][4] lengths, ascending, zero
+ *    after the last) is the u1fCycles parameter, none meaning one cycle
+ *    through every player.  When no set is collected, the two-row test
+ *    cyclesOfTwoRowsOk (cnvSupport4.cpp:2-55) fails at two rounds and no pair
+ *    selects a set at more: not canonical before any relabelling.
+ *  - Pairs of rounds in the order of cnvCheck2U1F; for each, the loop over
+ *    nv1 = MAX_CYCLE_SETS (32) cycle sets (cnvSupport.cpp:208-216), where
+ *    collectOneCyclesSet (:341-352) recomputes the pair's cycles for groups
+ *    of 2 whatever the index; a set whose lengths are not those of the
+ *    collected set is not selected, which makes the matrix not canonical
+ *    (:288-303).  Every mapping onto the selected set is tried
+ *    (createU1FTr, kmProcessMatrix): smaller -> not canonical; equal ->
+ *    updateGroup, which keeps each automorphism once.
+ * Relabelling index ((ordered pair * 32) + repetition) * T + position.
+ */
+#define KF_3U1F_REPS 32
+
+int orc_kf_check_3u1f(const uint8_t *mat, int n, int nrounds, const uint8_t *types, int ntypes,
+                      uint64_t *witness, uint64_t *n_auts, uint64_t *n_tried)
+{
+    if (n < 4 || n > KF_MAX_N || (n & 1) || nrounds < 1 || nrounds > n - 1 || ntypes < 0) return -1;
+    u8 part[KF_MAX_N][KF_MAX_N];
+    if (!kf_partners(mat, n, nrounds, part)) return -1;
+    uint64_t tried = 0;
+    size_t ngrp = 0, capgrp = 0;
+    u8 *grp = NULL;   /* the automorphisms found (updateGroup) */
+    int verdict = 1;
+    if (witness) *witness = ~0ull;
+    if (nrounds >= 2) {
+        kf_cycles ref;
+        kf_pair_cycles(part[0], part[1], n, &ref);
+        if (ref.n > KF_MAXCYC) return -1;
+        int allowed = 0;
+        if (ntypes == 0) allowed = ref.n == 1 && ref.len[0] == n;
+        for (int k = 0; k < ntypes && !allowed; k++) {
+            int same = 1, c = 0;
+            for (; c < KF_MAXCYC; c++) {
+                const int want = types[k * KF_MAXCYC + c];
+                if (c < ref.n ? want != ref.len[c] : want != 0) same = 0;
+            }
+            allowed = same;
+        }
+        if (!allowed) verdict = 0;
+        const uint64_t T = u1f_per_pair(&ref);
+        uint64_t po = 0;
+        for (int last = 1; last < nrounds && verdict; last++)
+            for (int ind = 0; ind < last && verdict; ind++)
+                for (int swap = 0; swap < 2 && verdict; swap++, po++) {
+                    const int r0 = swap ? last : ind, r1 = swap ? ind : last;
+                    for (int rep = 0; rep < KF_3U1F_REPS && verdict; rep++) {
+                        kf_cycles cy;
+                        kf_pair_cycles(part[r0], part[r1], n, &cy);
+                        int same = cy.n == ref.n;
+                        for (int c = 0; c < ref.n && same; c++) same = cy.len[c] == ref.len[c];
+                        if (!same) { verdict = 0; break; }
+                        u1f_map m;
+                        u1f_map_first(&m, &cy);
+                        for (uint64_t t = 0; t < T; t++) {
+                            if (t) u1f_map_next(&m, &cy);
+                            u8 tr[KF_MAX_N];
+                            u1f_map_tr(&m, &cy, &ref, tr);
+                            u8 mo[KF_MAX_N * KF_MAX_N];
+                            normal_form(mat, nrounds, n, tr, mo);
+                            tried++;
+                            const int cmp = memcmp(mo, mat, (size_t)(n * nrounds));
+                            if (cmp < 0) {
+                                verdict = 0;
+                                if (witness) *witness = (po * KF_3U1F_REPS + (uint64_t)rep) * T + t;
+                                break;
+                            }
+                            if (cmp == 0) {
+                                size_t g = 0;
+                                while (g < ngrp && memcmp(grp + g * KF_MAX_N, tr, (size_t)n)) g++;
+                                if (g == ngrp) {
+                                    if (ngrp == capgrp) {
+                                        capgrp = capgrp ? 2 * capgrp : 64;
+                                        u8 *ng = (u8 *)realloc(grp, capgrp * KF_MAX_N);
+                                        if (!ng) { free(grp); return -1; }
+                                        grp = ng;
+                                    }
+                                    memcpy(grp + ngrp++ * KF_MAX_N, tr, (size_t)n);
+                                }
+                            }
+                        }
+                    }
+                }
+    }
+    free(grp);
+    if (n_auts) *n_auts = (uint64_t)ngrp;
     if (n_tried) *n_tried = tried;
     return verdict;
 }

@@ -51,6 +51,7 @@ def load():
         L.orc_kf_check.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.orc_kf_check_2p1f.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.orc_kf_check_2u1f.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
+        L.orc_kf_check_3u1f.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, _P, _P, _P]
         L.orc_kf_normalize.argtypes = [_P, ctypes.c_int, ctypes.c_int]
         L.orc_kf_bruteforce.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
         L.orc_kf_player_index.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
@@ -241,6 +242,20 @@ class Oracle:
         w, a, t = (np.zeros(1, np.uint64) for _ in range(3))
         v = self.L.orc_kf_check_2u1f(mat.ctypes.data, mat.shape[1], mat.shape[0], w.ctypes.data, a.ctypes.data,
                                      t.ctypes.data)
+        if v < 0:
+            raise ValueError("oracle rejected the factorization")
+        return v, int(w[0]), int(a[0]), int(t[0])
+
+    def kf_check_3u1f(self, mat, u1f_cycles=None):
+        """cnvCheck3U1F for groups of 2: (verdict, witness index, automorphisms, tried)."""
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        types = np.zeros((len(u1f_cycles or ()), 4), np.uint8)
+        for k, lens in enumerate(u1f_cycles or ()):
+            types[k, :len(lens)] = sorted(lens)
+        w, a, t = (np.zeros(1, np.uint64) for _ in range(3))
+        v = self.L.orc_kf_check_3u1f(mat.ctypes.data, mat.shape[1], mat.shape[0],
+                                     types.ctypes.data if len(types) else None, len(types), w.ctypes.data,
+                                     a.ctypes.data, t.ctypes.data)
         if v < 0:
             raise ValueError("oracle rejected the factorization")
         return v, int(w[0]), int(a[0]), int(t[0])

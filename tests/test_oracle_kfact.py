@@ -276,3 +276,105 @@ def test_canonizers_reject_what_is_not_a_factorization(oracle, fn):
     for bad in (repeated, bad_player, twice):
         with pytest.raises(ValueError):
             check(bad)
+
+
+# ---- cnvCheck3U1F for groups of 2 -------------------------------------------
+
+@pytest.mark.parametrize("key,cycles,tried", [("u8", (4, 4), 7 * 6 * 2 * 4 * 4), ("u10", (4, 6), 9 * 8 * 4 * 6),
+                                              ("u12", (6, 6), 11 * 10 * 2 * 6 * 6), ("p10", (10,), 9 * 8 * 10)])
+def test_3u1f_printed_factorizations(oracle, key, cycles, tried):
+    """With their own cycle type allowed, the printed uniform and perfect
+    1-factorizations are canonical with their printed |Aut(M)| (each
+    automorphism kept once), every ordered pair's relabellings tried 32 times
+    over; with no type given (one cycle through every player) only the perfect
+    one passes, the others are not canonical before any relabelling."""
+    for m in golden("kfact.json")[key]["matrices"]:
+        v, w, auts, t = oracle.kf_check_3u1f(m["rounds"], [cycles])
+        assert v == 1 and auts == m["aut"] and t == 32 * tried and w == (1 << 64) - 1
+        v, w, auts, t = oracle.kf_check_3u1f(m["rounds"])
+        if key == "p10":
+            assert v == 1 and auts == m["aut"] and t == 32 * tried
+        else:
+            assert (v, w, auts, t) == (0, (1 << 64) - 1, 0, 0)
+        v, w, auts, t = oracle.kf_check_3u1f(m["rounds"], [(4, 4, 4), cycles])
+        assert v == 1 and t == 32 * tried
+
+
+def test_3u1f_matches_bruteforce_on_k8(oracle):
+    """Relabelled prefixes of the uniform 1-factorization of K_8 (every pair of
+    rounds of type 4:4, rounds 0, 1 those of the printed one): verdict and, for
+    the canonical ones, the number of automorphisms equal those of all 8!
+    player relabellings."""
+    cands = u1f_candidates(golden("kfact.json")["u8"], 150, 0x3A, rmin=2)
+    seen = set()
+    for mat in cands:
+        v, w, auts, t = oracle.kf_check_3u1f(mat, [(4, 4)])
+        bv, bauts = oracle.kf_bruteforce(mat)
+        assert v == bv
+        if v:
+            assert auts == bauts
+        seen.add(v)
+    assert seen == {0, 1} or seen == {1}
+
+
+def _per_pair(lengths):
+    """Relabellings per ordered pair: the product of the cycle lengths (start
+    offset x direction per cycle) times the orders of cycles of equal length."""
+    from collections import Counter
+    from math import factorial, prod
+    return prod(lengths) * prod(factorial(c) for c in Counter(lengths).values())
+
+
+def _assert_2u1f_tried_32_times(oracle, mat, types):
+    v2, w2, a2, t2 = oracle.kf_check_2u1f(mat)
+    v3, w3, a3, t3 = oracle.kf_check_3u1f(mat, types)
+    assert v3 == v2 and a3 == a2
+    if v2 == 1 or w2 == (1 << 64) - 1:
+        assert w3 == w2 and t3 == 32 * t2
+    else:
+        T = _per_pair(_cycle_lengths(mat[0], mat[1]))
+        p, q = divmod(w2, T)
+        assert w3 == p * 32 * T + q and t3 == w3 + 1
+    return v3
+
+
+def test_3u1f_is_2u1f_tried_32_times(oracle):
+    """Against the two-row U1F canonizer, rounds 0, 1's type allowed: the same
+    verdict and automorphisms; a witness of that canonizer at ordered pair p,
+    position q becomes (p * 32) * T + q, T the relabellings per pair, and the
+    relabellings tried are that index + 1 (canonical: 32 times all); a pair of
+    another cycle type ends both at once, after 32 times the relabellings of
+    the pairs before it."""
+    kf = golden("kfact.json")
+    verdicts = set()
+    for key, cyc, count in (("u10", (4, 6), 600), ("u12", (6, 6), 600), ("p10", (10,), 300)):
+        for mat in u1f_candidates(kf[key], count, 0x3B + count, rmin=1):
+            verdicts.add(_assert_2u1f_tried_32_times(oracle, mat, [cyc]))
+    rows, meta = ise_amd.kf_random(12, 2, 9, 300, 0x3C)
+    for i in range(len(meta)):
+        mat = ise_amd.kf_unpack(rows[i, :int(meta[i]) & 0xff], 12)
+        _assert_2u1f_tried_32_times(oracle, mat, [tuple(sorted(_cycle_lengths(mat[0], mat[1])))])
+    assert verdicts == {0, 1}
+
+
+def _cycle_lengths(r0, r1):
+    """Cycle lengths (players) of two perfect matchings given as pair lists."""
+    n = len(r0)
+    pa, pb = [0] * n, [0] * n
+    for r, p in ((r0, pa), (r1, pb)):
+        for j in range(0, n, 2):
+            p[r[j]], p[r[j + 1]] = r[j + 1], r[j]
+    seen, out = set(), []
+    for k0 in range(n):
+        if k0 in seen:
+            continue
+        k, length = k0, 0
+        while True:
+            k1 = pa[k]
+            seen.update((k, k1))
+            length += 2
+            k = pb[k1]
+            if k == k0:
+                break
+        out.append(length)
+    return out
