@@ -444,7 +444,7 @@ ISE_DEV void plain_search(const Grp<G> &g, const Staged &c, const uint32_t *geo,
 // A word of the set stack holds the set in its low b bits and, in its top
 // byte, the number of ones of original row i in the level-i orbit holding
 // position l (what every permuted row's count there is compared with):
-// written once by stage_sets and carried by every later store to the word,
+// written once by stage_levels and carried by every later store to the word,
 // so the stack is the group's only LDS.  Matrices wider than PACKED_COLS
 // columns run the replayed search.
 constexpr int PACKED_COLS = 56;
@@ -452,7 +452,7 @@ constexpr uint64_t COUNT_BYTE = 0xff00000000000000ull;
 ISE_DEV int level_count(uint64_t w) { return (int)(uint32_t)(w >> 56); }
 
 // Column-set state of the search (used when the candidate is in orbit form,
-// see stage_sets).  The column permutation of the reference only matters
+// see orbit_form).  The column permutation of the reference only matters
 // through the set of source columns it places in each column orbit: a row
 // compares orbit by orbit by the number of ones, which does not depend on the
 // order inside an orbit, and the ones-first partition of an orbit that
@@ -470,12 +470,31 @@ ISE_DEV int level_count(uint64_t w) { return (int)(uint32_t)(w >> 56); }
 // the ones of row i come first, and the orbits of row i+1 are those of row i
 // split after these ones.  Rows written as the reference's MakeRow writes them
 // (Source/MatrixCanonChecker.cpp:14-93) have it.
+// Orbit form, lane-parallel: lane r checks row r -- no zero followed by a
+// one inside one of its column orbits (the ones come first) and, for given
+// cell masks, the orbits of row r+1 are those of row r split after its ones.
 template <int G>
-ISE_DEV bool stage_sets(const Grp<G> &g, const Staged &c, const uint64_t *cells, uint64_t *stk) {
-    const int l = g.ln, d = c.d, b = c.b;
+ISE_DEV bool orbit_form(const Grp<G> &g, const Staged &c, const uint64_t *cells) {
+    const int l = g.ln;
+    const uint64_t R = c.rowv, C = c.cellv;
+    bool ok = ((R >> 1) & ~R & ~(C >> 1) & lowmask(c.b - 1)) == 0;
+    if (cells) {
+        const int nx = l + 1 < c.d ? l + 1 : l;
+        const uint64_t Cn = ((uint64_t)(uint32_t)g.gather((int)(uint32_t)(C >> 32), nx) << 32) |
+                            (uint32_t)g.gather((int)(uint32_t)C, nx);
+        if (l + 1 < c.d) ok = ok && Cn == ((C | (R ^ (R << 1))) & c.bmask);
+    }
+    return g.ballot(l < c.d && !ok) == 0;
+}
+
+// The identity sets of levels lo..hi-1 onto the stack, each word with its
+// count byte.  The search stages a level before it first reads it: a walk
+// that ends near the last rows never stages the others.
+template <int G>
+ISE_DEV void stage_levels(const Grp<G> &g, const Staged &c, uint64_t *stk, int lo, int hi) {
+    const int l = g.ln, b = c.b;
     const uint64_t le = (l == 63) ? ~0ull : ((2ull << l) - 1);
-    bool ok = true;
-    for (int r = 0; r < d; r++) {
+    for (int r = lo; r < hi; r++) {
         const uint64_t C = g.read64(c.cellv, r), R = g.read64(c.rowv, r);
         const int s0 = hib64(C & le);
         const uint64_t above = C & ~le & c.bmask;
@@ -484,14 +503,7 @@ ISE_DEV bool stage_sets(const Grp<G> &g, const Staged &c, const uint64_t *cells,
         int cR0 = popc64(R & om);
         if (l >= b) { om = 0; cR0 = 0; }
         stk[r * G + l] = om | ((uint64_t)cR0 << 56);
-        // ones of row r first inside the orbit
-        ok = ok && (R & om) == (l < b ? (lowmask(s0 + cR0) & ~lowmask(s0)) : 0ull);
-        // orbits of row r+1 = orbits of row r split after those ones (given cell masks only;
-        // derived masks are built that way)
-        if (cells && r + 1 < d)
-            ok = ok && g.read64(c.cellv, r + 1) == ((C | (R ^ (R << 1))) & c.bmask);
     }
-    return g.ballot(!ok) == 0;
 }
 
 // Generator of an automorphism found by the column-set search: the
@@ -578,16 +590,17 @@ template <int G> ISE_DEV uint32_t orbit_starts(const Staged &c) {
 
 // Rows tried at one position of the depth-first phase: compares row `cur`
 // and, while it is refuted, moves on to the next larger unused row (the bits
-// of U above cur).  Stops on an equal row (bad = 0), a larger one (a witness)
-// or when no larger unused row is left; with CHECKED also when `left`
-// permutations have been used.  Returns the number of rows moved on to;
-// bad and more are the last comparison's ballots.  Rows of `ref` (known
-// refuted at this position, level_refuted) are moved past uncompared; when
-// the last row moved to is one of them, bad and more read "refuted".
+// of U above cur).  Stops on an equal row (outcome SIB_EQUAL), a larger one
+// (SIB_LARGER, fcol = the first column of the deciding orbit) or a refuted
+// one with no larger unused row left (SIB_REFUTED); with CHECKED also when
+// `left` permutations have been used (SIB_REFUTED).  Returns the number of
+// rows moved on to.  Rows of `ref` (known refuted at this position,
+// level_refuted) are moved past uncompared.
+constexpr int SIB_EQUAL = 0, SIB_LARGER = 1, SIB_REFUTED = 2;
+
 template <int G, bool COUNT, bool CHECKED>
 ISE_DEV uint32_t sibling_run(const Grp<G> &g, const Staged &c, uint64_t S, int cR, uint64_t C, uint32_t U,
-                             uint32_t ref, uint32_t left, int &cur, uint64_t &bad, uint64_t &more, uint64_t &Rp,
-                             Verdict &v) {
+                             uint32_t ref, uint32_t left, int &cur, int &outcome, int &fcol, Verdict &v) {
     uint32_t k = 0;
     for (;;) {
         if constexpr (!CHECKED && !COUNT) {
@@ -600,9 +613,7 @@ ISE_DEV uint32_t sibling_run(const Grp<G> &g, const Staged &c, uint64_t S, int c
                         k += (uint32_t)__popc(abv);
                         cur = 31 - __builtin_clz(abv);
                     }
-                    bad = 1ull;
-                    more = 1ull;
-                    Rp = 0ull;
+                    outcome = SIB_REFUTED;
                     return k;
                 }
                 const int nx = __builtin_ctz(open);
@@ -610,10 +621,10 @@ ISE_DEV uint32_t sibling_run(const Grp<G> &g, const Staged &c, uint64_t S, int c
                 cur = nx;
             }
         }
-        Rp = g.read64(c.rowv, cur);
+        const uint64_t Rp = g.read64(c.rowv, cur);
         const int cP = popc64(Rp & S);
-        bad = g.ballot(cP != cR) & C;
-        more = g.ballot(cR > cP);
+        const uint64_t bad = g.ballot(cP != cR) & C;
+        const uint64_t more = g.ballot(cR > cP);
         if constexpr (COUNT) {
             v.rows_cmp++;
             if (bad) {
@@ -626,10 +637,17 @@ ISE_DEV uint32_t sibling_run(const Grp<G> &g, const Staged &c, uint64_t S, int c
                 v.cells_cmp += (uint32_t)c.b;
             }
         }
-        if (!bad)
-            break;                             // equal
-        if (!((more >> ctz64(bad)) & 1ull))
-            break;                             // larger: a witness
+        if (!bad) {
+            outcome = SIB_EQUAL;
+            break;
+        }
+        const int lim = ctz64(bad);
+        if (!((more >> lim) & 1ull)) {
+            outcome = SIB_LARGER;              // larger: a witness
+            fcol = lim;
+            break;
+        }
+        outcome = SIB_REFUTED;
         const uint32_t above = U & (0xfffffffeu << cur);
         if (!above)
             break;                             // no larger unused row
@@ -659,7 +677,7 @@ ISE_DEV uint32_t sibling_run(const Grp<G> &g, const Staged &c, uint64_t S, int c
 // when it stops.  From the first automorphism on, the replayed walk takes
 // over with the permutation, stabiliser point and orbits the walk would have.
 template <int G, bool COUNT>
-ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, uint32_t ci,
+ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, int floor, uint32_t ci,
                         const OutPtrs &o, uint32_t max_perms, int &pr, int &ob, Verdict &v) {
     const int l = g.ln, d = c.d;
     PermState st;
@@ -725,6 +743,11 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, uint32_
             first = false;
             if (st.stab > p) st.stab = p;
             i = g.uni(p);
+            if (i < floor) {
+                // levels i and above can be reached from here on
+                stage_levels(g, c, stk, i, floor);
+                floor = i;
+            }
             int cur = g.uni(nxt);
             if (l == i) pr = cur;
             const uint64_t w = stk[i * G + l];
@@ -749,14 +772,15 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, uint32_
                     left--;
                 }
                 // rows tried at position i until one is not refuted: only cur changes
-                uint64_t bad, more, Rp;
+                int outcome = SIB_REFUTED, fcol = 0;
                 if (left >= 32u)               // at most 31 more rows at one position
-                    left -= sibling_run<G, COUNT, false>(g, c, S, cR, C, U, ref, left, cur, bad, more, Rp, v);
+                    left -= sibling_run<G, COUNT, false>(g, c, S, cR, C, U, ref, left, cur, outcome, fcol, v);
                 else
-                    left -= sibling_run<G, COUNT, true>(g, c, S, cR, C, U, 0u, left, cur, bad, more, Rp, v);
+                    left -= sibling_run<G, COUNT, true>(g, c, S, cR, C, U, 0u, left, cur, outcome, fcol, v);
                 if (l == i) pr = cur;
-                if (!bad) {
+                if (outcome == SIB_EQUAL) {
                     // equal: descend to the next position (Rp: the row just compared)
+                    const uint64_t Rp = g.read64(c.rowv, cur);
                     const uint64_t Ro = g.read64(c.rowv, i);
                     S &= ((Ro >> l) & 1ull) ? Rp : ~Rp;
                     U &= ~(1u << cur);
@@ -778,12 +802,12 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, uint32_
                     fresh = false;
                     continue;
                 }
-                if (!((more >> ctz64(bad)) & 1ull)) {
+                if (outcome == SIB_LARGER) {
                     // the permuted matrix is larger: witness
                     pr = fill_tail(g, pr, i, d, U & ~(1u << cur));
                     v.verdict = 0;
                     v.frow = i;
-                    v.fcol = ctz64(bad);
+                    v.fcol = fcol;
                     v.rtc = row_to_change(g, pr, d, i);
                     why = 1;
                     break;
@@ -815,6 +839,8 @@ ISE_DEV void set_search(const Grp<G> &g, const Staged &c, uint64_t *stk, uint32_
     }
 
     // ---- replayed walk from the first automorphism on ------------------------
+    // (levels never reached hold the identity sets: the rows there never moved)
+    if (floor > 0) stage_levels(g, c, stk, 0, floor);
     for (;;) {
         if (aut) {
             set_generator(g, c, S, pr, ci, o, v.ngen);
@@ -970,7 +996,7 @@ ISE_DEV bool hunt(const Grp<G> &g, const Staged &c, const uint32_t *geo, const u
 
 // The witness hunt on the column-set state: the same depth-first order,
 // budget, counters and witness as `hunt`, comparisons as in set_search.  The
-// set stack holds the identity sets on entry (as stage_sets leaves it) and
+// set stack holds the identity sets on entry (as stage_levels leaves it) and
 // the sets of the hunt's prefixes on return.
 template <int G, bool COUNT>
 ISE_DEV bool hunt_sets(const Grp<G> &g, const Staged &c, uint64_t *stk, const uint8_t *order,
@@ -1082,21 +1108,24 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const 
     uint32_t *geo = lds;
     if constexpr (SETS) {
         uint64_t *stk = reinterpret_cast<uint64_t *>(lds);
-        if (c.b <= PACKED_COLS && stage_sets(g, c, cells, stk)) {
+        if (c.b <= PACKED_COLS && orbit_form(g, c, cells)) {
             bool hfound = false;
+            int floor = d;   // levels floor..d-1 staged
             if constexpr (HUNT) {
                 // an order table narrower than the candidate cannot guide it: plain search only
                 if (!order || d <= od) {
+                    stage_levels(g, c, stk, 0, d);   // the hunt starts at the first row
                     hfound = hunt_sets<G, COUNT>(g, c, stk, order ? order + (uint64_t)ci * od * od : nullptr,
                                                  od, hunt_budget, pr, v);
                     if (!hfound) {
                         pr = l;
-                        stage_sets(g, c, cells, stk);   // identity sets again
+                        stage_levels(g, c, stk, 0, d);   // identity sets again
+                        floor = 0;
                     }
                 }
             }
             if (!hfound)
-                set_search<G, COUNT>(g, c, stk, ci, o, max_perms, pr, ob, v);
+                set_search<G, COUNT>(g, c, stk, floor, ci, o, max_perms, pr, ob, v);
             found = true;
         }
     }
