@@ -13,17 +13,18 @@ comes from the autotuner, run on the same batch before timing.
 
   * plain order: one launch of the canonicity kernel per step (column-set
     search, candidates started deepest first);
-  * learned order: per step, the plain search on every candidate with a
-    budget of 16 permutations, the fused scoring kernel (pair features ->
-    16x128x128x1 bf16 MFMA network -> per-position row order) on the
-    candidates it leaves open, and on those the witness hunt in that order
-    followed by the plain search.  Verdicts are checked equal to the plain
-    order's.
+  * learned order: per step, one launch of the lead-order kernel: the plain
+    search on every candidate with a budget of 32 permutations; on the
+    candidates it leaves open, the 16x128x128x1 bf16 MFMA network scores
+    every row as the row of position 0 and a depth-first search tries them
+    in that order (meeting an automorphism sends a candidate back to the
+    plain search).  Verdicts, and every field of the canonical candidates,
+    are checked equal to the plain order's.
 
-`value` is the plain-order rate (the library's default order and, as
-measured, the faster one); the learned-order rate, both mean
-permutations-to-rejection, the 1-factorization kernel on K_12 and a full
-enumeration's designs/s are reported beside it.
+`value` is the plain-order rate (the library's default order); the
+learned-order rate and learned/plain ratio, both mean
+permutations-to-rejection and rows compared, the 1-factorization kernels and
+a full enumeration's designs/s are reported beside it.
 
 Prints one JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -384,9 +385,8 @@ def main():
     ap.add_argument("--random", type=int, default=1 << 20)
     ap.add_argument("--group-lanes", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=4)
-    ap.add_argument("--hunt-budget", type=int, default=8)
-    ap.add_argument("--learned-gate", type=int, default=16,
-                    help="plain permutations after which a candidate is ordered by the network (0: all)")
+    ap.add_argument("--learned-gate", type=int, default=32,
+                    help="plain permutations after which the network orders a candidate's first row (0: all)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="use --group-lanes/--waves-per-block instead of the autotuner's choice")
     ap.add_argument("--no-cpu", action="store_true")
@@ -473,38 +473,47 @@ def main():
         raise RuntimeError("search limit hit or invalid record in the bench batch")
     rej = verdict == 0
 
-    # ---- learned order: scoring kernel + hunt-then-plain kernel per step
+    # ---- learned order: one launch per step of the lead-order kernel (the
+    # plain search up to the gate, the network's order of the first row on
+    # the candidates it leaves open, every result field)
     learned_line = None
     if not args.no_learned:
-        model = learned.load(ctx)
+        model = learned.load(ctx, learned.LEAD_WEIGHTS)
         model.set_gate(args.learned_gate)
-        d_order = ctx.device_array((n, STRIDE, STRIDE), np.uint8)
         if args.warmup:
-            model.time_check(n, STRIDE, d_rows, d_meta, STRIDE, d_order, args.hunt_budget, res, d_head,
-                             args.warmup)
-        lms, oms = model.time_check(n, STRIDE, d_rows, d_meta, STRIDE, d_order, args.hunt_budget, res, d_head,
-                                    args.steps)
-        if not np.array_equal(out["verdict"].to_host(), verdict):
+            model.time_lead(n, STRIDE, d_rows, d_meta, res, d_head, args.warmup)
+        lms = model.time_lead(n, STRIDE, d_rows, d_meta, res, d_head, args.steps)
+        lhost = {f: out[f].to_host() for f in fields}
+        if not np.array_equal(lhost["verdict"], verdict):
             raise RuntimeError("the learned order changed a verdict")
-        model.time_check(n, STRIDE, d_rows, d_meta, STRIDE, d_order, args.hunt_budget, cres, d_head, 1)
+        canon = verdict == 1
+        for f in fields:
+            a, b = lhost[f], gpu_host[f]
+            if f == "gens":
+                keep = (np.arange(GEN_CAP)[None, :] < np.minimum(gpu_host["n_gens"], GEN_CAP)[:, None]) & \
+                    canon[:, None]
+                same = np.array_equal(a[keep], b[keep])
+            else:
+                same = np.array_equal(a[canon], b[canon])
+            if not same:
+                raise RuntimeError(f"the learned order changed {f} of a canonical candidate")
+        model.check_lead(n, STRIDE, d_rows, d_meta, cres, d_head)
         lcount = cout["counters"].to_host()
         if not np.array_equal(cout["verdict"].to_host(), verdict):
             raise RuntimeError("the learned order changed a verdict (counted launch)")
         lmean = float(np.mean(lms))
         learned_line = {
             "value": n / (lmean / 1e3), "ms_per_step": lmean,
-            "gated_pass_and_scoring_ms_per_step": float(np.mean(oms)),
+            "order": "learned order of the first row (lead order), one launch per step",
             "gate_perms": args.learned_gate,
-            "hunt_budget": args.hunt_budget, "weights": "incidence-system-enumeration_amd/ise_amd/"
-                                                        "row_order_weights.npy",
+            "weights": "incidence-system-enumeration_amd/ise_amd/lead_order_weights.npy",
             "mean_perms_to_reject": float(lcount[rej, 0].mean()),
             "mean_rows_to_reject": float(lcount[rej, 1].mean()),
             "rows_compared": float(lcount[:, 1].astype(np.float64).sum()),
             "verdicts_equal_to_plain": True,
+            "canonical_fields_equal_to_plain": True,
         }
-        print(f"bench: learned {lmean:.2f} ms/step (gated pass + scoring {np.mean(oms):.2f})", file=sys.stderr,
-              flush=True)
-        d_order.free()
+        print(f"bench: learned {lmean:.2f} ms/step", file=sys.stderr, flush=True)
         model.close()
 
     # algorithmic HBM bytes of one plain launch: candidate rows + meta in, result

@@ -24,6 +24,21 @@ int launch_gated_pass(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *r
                       const ise_results &out, uint32_t *head, uint32_t gate, int pass, uint32_t hunt_budget,
                       const uint8_t *order, uint32_t order_dim, const uint32_t **list);
 
+// The learned order of the first row (ise_lead.h) on a batch: queue reset,
+// schedule, then one wave per candidate.  frag/w: the network (A fragments,
+// packed fp32 parameters); lead_in (optional [n][16]) replaces its order;
+// lead_out / score_out (optional [n][16]) receive the order and scores used.
+hipError_t launch_lead(uint32_t blocks, uint32_t waves_per_block, uint32_t n, uint32_t stride, const uint64_t *rows,
+                       const uint32_t *meta, const ise_results &out, uint32_t *head, uint32_t max_perms, uint32_t gate,
+                       const uint16_t *frag, const float *w, const uint8_t *lead_in, uint8_t *lead_out,
+                       float *score_out, uint32_t *sched, hipStream_t stream);
+hipError_t lead_occupancy(uint32_t waves_per_block, uint32_t stride, int *blocks_per_cu);
+// The same with the context's launch shape, schedule and permutation limit
+// (on its stream).
+int launch_lead_ctx(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta,
+                    const ise_results &out, uint32_t *head, uint32_t gate, const uint16_t *frag, const float *w,
+                    const uint8_t *lead_in, uint8_t *lead_out, float *score_out);
+
 // Word of the queue buffer that counts the candidates pass `pass` deferred.
 uint32_t *defer_count_ptr(uint32_t *head, int pass);
 

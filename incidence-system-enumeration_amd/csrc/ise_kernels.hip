@@ -8,6 +8,7 @@
 // read past their bounds.
 #include <hip/hip_runtime.h>
 #include "ise_wave.h"
+#include "ise_lead.h"
 #include "ise_internal.h"
 
 namespace ise {
@@ -199,6 +200,49 @@ __global__ __launch_bounds__(1024) ISE_CHECK_OCCUPANCY void check_kernel(uint32_
     }
 }
 
+// Learned order of the first row (ise_lead.h): one candidate per wave from
+// the (scheduled) queue; candidates the lead order does not take run
+// check_candidate's column-set or replayed search.
+template <bool COUNT>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) void lead_kernel(
+    uint32_t n, uint32_t stride, const uint64_t *__restrict__ rows, const uint32_t *__restrict__ meta, OutPtrs o,
+    uint32_t *head, uint32_t max_perms, uint32_t rows_cap, uint32_t gate, const bf16x4 *__restrict__ frag,
+    const float *__restrict__ w, const uint8_t *__restrict__ lead_in, uint8_t *lead_out, float *score_out,
+    const uint32_t *__restrict__ sched) {
+    extern __shared__ uint32_t lds_geo[];
+    Grp<64> g;
+    uint32_t *lds = lds_geo + (threadIdx.x >> 6) * group_scratch_words<64, true>(rows_cap);
+    uint64_t *stk = reinterpret_cast<uint64_t *>(lds);
+    const int home = (int)(blockIdx.x % NUM_XCD);
+    int slice = home;
+    for (;;) {
+        uint32_t ci;
+        if (sched) {
+            const uint32_t p = next_scheduled(g, n, head, slice, home);
+            ci = p < n ? sched[p] : n;
+        } else {
+            ci = next_candidate(g, n, head, slice, home);
+        }
+        if (ci >= n)
+            break;
+        const uint32_t m = meta[ci];
+        const int d = m & 0xff, b = (m >> 8) & 0xff, fixed = (m >> 16) & 0xff;
+        const bool ok = d >= 1 && d <= ISE_MAX_ROWS && (uint32_t)d <= stride && (uint32_t)d <= rows_cap &&
+                        b >= 1 && b <= 64 && fixed < d;
+        if (!ok) {
+            if (g.ln == 0) o.verdict[ci] = ISE_VERDICT_INVALID;
+            continue;
+        }
+        const Staged c = load_candidate(g, ci, m, rows, nullptr, stride);
+        if (d >= 2 && d <= LEAD_MAX_ROWS && fixed == 0 && b <= PACKED_COLS && (frag || lead_in) &&
+            orbit_form(g, c, nullptr))
+            lead_candidate<COUNT>(g, ci, c, o, max_perms, gate, stk, frag, w, lead_in, lead_out, score_out);
+        else
+            check_candidate<64, COUNT, false, true>(g, ci, m, rows, nullptr, stride, o, max_perms, lds, nullptr, 0,
+                                                    0u, nullptr, nullptr);
+    }
+}
+
 static OutPtrs to_out(const ise_results &r) {
     OutPtrs o;
     o.verdict = r.verdict; o.fail_row = r.fail_row; o.fail_col = r.fail_col;
@@ -318,6 +362,38 @@ hipError_t launch_check(int group_lanes, uint32_t blocks, uint32_t waves_per_blo
 #undef ISE_LAUNCH_V
 #undef ISE_LAUNCH_P
     return hipGetLastError();
+}
+
+hipError_t launch_lead(uint32_t blocks, uint32_t waves_per_block, uint32_t n, uint32_t stride, const uint64_t *rows,
+                       const uint32_t *meta, const ise_results &out, uint32_t *head, uint32_t max_perms, uint32_t gate,
+                       const uint16_t *frag, const float *w, const uint8_t *lead_in, uint8_t *lead_out,
+                       float *score_out, uint32_t *sched, hipStream_t stream) {
+    if (out.perm_col) return hipErrorInvalidValue;   // the column-set form reports no column permutation
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, stream, head);
+    if (sched) {
+        uint32_t *aux = sched + n;
+        const uint32_t sb = (uint32_t)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+        hipLaunchKernelGGL(sched_reset_kernel, dim3(1), dim3(128), 0, stream, aux);
+        hipLaunchKernelGGL(sched_count_kernel, dim3(sb), dim3(256), 0, stream, n, meta, aux);
+        hipLaunchKernelGGL(sched_fill_kernel, dim3(sb), dim3(256), 0, stream, n, meta, aux, sched);
+    }
+    const OutPtrs o = to_out(out);
+    const uint32_t cap = rows_cap_for(64, stride);
+    const size_t lds = lds_bytes(64, waves_per_block, stride, true);
+    const bf16x4 *fr = reinterpret_cast<const bf16x4 *>(frag);
+    if (out.counters)
+        hipLaunchKernelGGL(lead_kernel<true>, dim3(blocks), dim3(64 * waves_per_block), lds, stream, n, stride, rows,
+                           meta, o, head, max_perms, cap, gate, fr, w, lead_in, lead_out, score_out, sched);
+    else
+        hipLaunchKernelGGL(lead_kernel<false>, dim3(blocks), dim3(64 * waves_per_block), lds, stream, n, stride, rows,
+                           meta, o, head, max_perms, cap, gate, fr, w, lead_in, lead_out, score_out, sched);
+    return hipGetLastError();
+}
+
+hipError_t lead_occupancy(uint32_t waves_per_block, uint32_t stride, int *blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void *)&lead_kernel<false>,
+                                                        (int)(64 * waves_per_block),
+                                                        lds_bytes(64, waves_per_block, stride, true));
 }
 
 hipError_t occupancy_blocks(int group_lanes, uint32_t waves_per_block, uint32_t stride, bool sets,

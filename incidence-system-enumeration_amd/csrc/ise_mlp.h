@@ -26,6 +26,9 @@ hipError_t adam(long count, float *w, const float *g, float *m1, float *m2, uint
 hipError_t to_bf16(long count, const float *x, uint16_t *y, hipStream_t s);
 hipError_t features(uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta, uint32_t od,
                     uint16_t *X, hipStream_t s);
+// Features of every row as the row of position 0: X [n][16][16] bf16.
+hipError_t lead_features(uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta, uint16_t *X,
+                         hipStream_t s);
 // Weights as MFMA A fragments (FRAG words of uint16) for learned_order.
 constexpr long MLP_FRAG_WORDS = 80L * 64 * 4;
 hipError_t frag_pack(const uint16_t *wb, uint16_t *frag, hipStream_t s);

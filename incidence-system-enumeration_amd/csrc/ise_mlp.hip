@@ -14,20 +14,10 @@
 #include <stdint.h>
 #include "ise_mlp.h"
 #include "ise_wave.h"
+#include "ise_mlp_dev.h"
+#include "ise_lead.h"
 
 namespace ise {
-
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ uint16_t f2bf(float x) {
-    // round to nearest even
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // C[m][n] = sum_k A(m,k) B(k,n) (+ bias[n]) (relu), for a 16-aligned M, N, K.
 // A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]; outputs optional.
@@ -289,6 +279,38 @@ hipError_t features(uint32_t n, uint32_t stride, const uint64_t *rows, const uin
     return hipGetLastError();
 }
 
+// Features of every row as the row of position 0 (the lead order's network
+// input): X[(c*16 + j)*16 + f], zero past the depth and for candidates the
+// lead order does not take (depth outside 2..16, more than 64 columns).
+__global__ __launch_bounds__(256) void lead_features_kernel(uint32_t n, uint32_t stride,
+                                                            const uint64_t *__restrict__ rows,
+                                                            const uint32_t *__restrict__ meta,
+                                                            uint16_t *__restrict__ X) {
+    Grp<64> g;
+    const uint32_t ci = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (ci >= n) return;
+    const uint32_t m = meta[ci];
+    const int d = m & 0xff, b = (m >> 8) & 0xff;
+    const int l = g.ln;
+    uint16_t *xc = X + (uint64_t)ci * LEAD_MAX_ROWS * MLP_IN;
+    if (d < 2 || d > LEAD_MAX_ROWS || (uint32_t)d > stride || b < 1 || b > 64) {
+        for (int t = l; t < LEAD_MAX_ROWS * MLP_IN; t += 64) xc[t] = 0;
+        return;
+    }
+    const Staged c = load_candidate(g, ci, m, rows, nullptr, stride);
+    const LeadStats s = lead_stats(g, c);
+    if (l < LEAD_MAX_ROWS)
+        for (int f = 0; f < MLP_IN; f++) xc[l * MLP_IN + f] = f2bf(lead_feature(f, s, l, d));
+}
+
+hipError_t lead_features(uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta, uint16_t *X,
+                         hipStream_t s) {
+    const uint32_t wpb = 4;
+    hipLaunchKernelGGL(lead_features_kernel, dim3((n + wpb - 1) / wpb), dim3(64 * wpb), 0, s, n, stride, rows, meta,
+                       X);
+    return hipGetLastError();
+}
+
 // ---- fused scoring: features -> MLP -> per-position row order -------------
 //
 // The products are computed transposed, H^T = W^T X^T, with the weights as
@@ -298,8 +320,6 @@ hipError_t features(uint32_t n, uint32_t stride, const uint64_t *rows, const uin
 // The weights are pre-arranged as A fragments (frag_pack_kernel) and staged
 // in LDS once per workgroup: tile (mt, kt) of a layer holds, for lane l,
 // W[kt*16 + 4*(l/16) + t][mt*16 + l % 16], t = 0..3.
-
-constexpr int FRAG_L1 = 0, FRAG_L2 = 8, FRAG_L3 = 8 + 64, FRAG_TILES = 8 + 64 + 8;
 
 __global__ void frag_pack_kernel(const uint16_t *__restrict__ wb, uint16_t *__restrict__ frag) {
     const int tile = blockIdx.x, l = threadIdx.x;   // 80 blocks of 64 lanes
@@ -322,39 +342,6 @@ __global__ void frag_pack_kernel(const uint16_t *__restrict__ wb, uint16_t *__re
 hipError_t frag_pack(const uint16_t *wb, uint16_t *frag, hipStream_t s) {
     hipLaunchKernelGGL(frag_pack_kernel, dim3(FRAG_TILES), dim3(64), 0, s, wb, frag);
     return hipGetLastError();
-}
-
-__device__ __forceinline__ bf16x4 relu_bias_bf16(f32x4 acc, const float *bias) {
-    bf16x4 h;
-    for (int t = 0; t < 4; t++) {
-        const float v = acc[t] + bias[t];
-        h[t] = (short)f2bf(v > 0.f ? v : 0.f);
-    }
-    return h;
-}
-
-// Scores of one tile of 16 pair rows given as a B fragment; lane l < 16
-// returns the score of row l of the tile.
-__device__ __forceinline__ float mlp_tile(bf16x4 xb, const bf16x4 *__restrict__ frag, const float *__restrict__ w,
-                                          int l) {
-    const int q = l >> 4;
-    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-    bf16x4 h1[8], h2[8];
-    for (int mt = 0; mt < 8; mt++) {
-        const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag[(FRAG_L1 + mt) * 64 + l], xb, zero, 0, 0, 0);
-        h1[mt] = relu_bias_bf16(acc, w + MLP_B1 + mt * 16 + 4 * q);
-    }
-    for (int mt = 0; mt < 8; mt++) {
-        f32x4 acc = zero;
-        for (int kt = 0; kt < 8; kt++)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag[(FRAG_L2 + mt * 8 + kt) * 64 + l], h1[kt], acc, 0, 0,
-                                                            0);
-        h2[mt] = relu_bias_bf16(acc, w + MLP_B2 + mt * 16 + 4 * q);
-    }
-    f32x4 acc = zero;
-    for (int kt = 0; kt < 8; kt++)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag[(FRAG_L3 + kt) * 64 + l], h2[kt], acc, 0, 0, 0);
-    return acc[0] + w[MLP_B3];
 }
 
 // One wave per candidate (grid-stride).  For every position i < d: features

@@ -339,6 +339,54 @@ int ise_mlp_check_batch_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, co
     return rc ? rc : rc2;
 }
 
+int ise_mlp_lead_features(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                          const uint32_t *d_meta, uint16_t *d_X) {
+    if (!ctx || !d_X || stride < 1 || stride > ISE_MAX_ROWS) return ISE_ERR_ARG;
+    if (n == 0) return ISE_OK;
+    if (!d_rows || !d_meta) return ISE_ERR_ARG;
+    CK(hipSetDevice(ctx_device(ctx)));
+    CK(lead_features(n, stride, d_rows, d_meta, d_X, ctx_stream(ctx)));
+    CK(hipStreamSynchronize(ctx_stream(ctx)));
+    return ISE_OK;
+}
+
+int ise_mlp_check_lead_device(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                              const uint32_t *d_meta, const ise_results *d_out, uint32_t *d_counter,
+                              const uint8_t *d_lead_in, uint8_t *d_lead_out, float *d_score_out) {
+    if (!m || !d_out || !d_counter || stride < 1 || stride > ISE_MAX_ROWS) return ISE_ERR_ARG;
+    if (n == 0) return ISE_OK;
+    if (!d_rows || !d_meta) return ISE_ERR_ARG;
+    int rc = launch_lead_ctx(m->ctx, n, stride, d_rows, d_meta, *d_out, d_counter, m->gate_perms, m->frag, m->w,
+                             d_lead_in, d_lead_out, d_score_out);
+    if (rc) return rc;
+    CK(hipStreamSynchronize(m->s));
+    return ISE_OK;
+}
+
+int ise_mlp_check_lead_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                                    const uint32_t *d_meta, const ise_results *d_out, uint32_t *d_counter,
+                                    uint32_t repeats, float *ms_out) {
+    if (!m || !d_out || !d_counter || !ms_out || repeats == 0 || stride < 1 || stride > ISE_MAX_ROWS)
+        return ISE_ERR_ARG;
+    if (!d_rows || !d_meta) return ISE_ERR_ARG;
+    std::vector<hipEvent_t> ev(repeats + 1, nullptr);
+    int rc = ISE_OK;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = ISE_ERR_HIP;
+    if (!rc && hipEventRecord(ev[0], m->s) != hipSuccess) rc = ISE_ERR_HIP;
+    for (uint32_t i = 0; i < repeats && !rc; i++) {
+        rc = launch_lead_ctx(m->ctx, n, stride, d_rows, d_meta, *d_out, d_counter, m->gate_perms, m->frag, m->w,
+                             nullptr, nullptr, nullptr);
+        if (!rc && hipEventRecord(ev[i + 1], m->s) != hipSuccess) rc = ISE_ERR_HIP;
+    }
+    if (!rc && hipStreamSynchronize(m->s) != hipSuccess) rc = ISE_ERR_HIP;
+    for (uint32_t i = 0; i < repeats && !rc; i++)
+        if (hipEventElapsedTime(&ms_out[i], ev[i], ev[i + 1]) != hipSuccess) rc = ISE_ERR_HIP;
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
+}
+
 int ise_mlp_set_gate(ise_mlp *m, uint32_t gate_perms) {
     if (!m) return ISE_ERR_ARG;
     m->gate_perms = gate_perms;

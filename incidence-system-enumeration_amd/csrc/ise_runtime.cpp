@@ -272,6 +272,31 @@ int launch_gated_pass(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *r
     return ISE_OK;
 }
 
+int launch_lead_ctx(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta,
+                    const ise_results &out, uint32_t *head, uint32_t gate, const uint16_t *frag, const float *w,
+                    const uint8_t *lead_in, uint8_t *lead_out, float *score_out) {
+    if (!c || !head || !out.verdict || out.perm_col || (!frag && !lead_in)) return ISE_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_sched(c, n);
+    if (rc) return rc;
+    // one candidate per wave; the context's waves per workgroup
+    const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
+    uint32_t blocks = c->cfg.blocks;
+    if (!blocks) {
+        int per_cu = 0;
+        HIPCHK(c, ise::lead_occupancy(wpb, stride, &per_cu));
+        if (per_cu < 1) per_cu = 1;
+        blocks = (uint32_t)per_cu * (uint32_t)c->num_cus;
+        const uint64_t need = ((uint64_t)n + wpb - 1) / wpb;
+        if (blocks > need) blocks = (uint32_t)(need ? need : 1);
+    }
+    uint32_t *sched = c->schedule == ISE_SCHEDULE_DEEPEST_FIRST ? c->d_sched : nullptr;
+    HIPCHK(c, ise::launch_lead(blocks, wpb, n, stride, rows, meta, out, head,
+                               c->cfg.max_perms ? c->cfg.max_perms : (1u << 26), gate, frag, w, lead_in, lead_out,
+                               score_out, sched, c->stream));
+    return ISE_OK;
+}
+
 uint32_t ctx_stride(const ise_ctx *c) { return c->stride; }
 uint32_t ctx_capacity(const ise_ctx *c) { return c->capacity; }
 hipStream_t ctx_stream(const ise_ctx *c) { return c->stream; }

@@ -1083,6 +1083,44 @@ ISE_DEV bool hunt_sets(const Grp<G> &g, const Staged &c, uint64_t *stk, const ui
     }
 }
 
+// Write the record of candidate `ci`: the scalar fields from lane 0, the
+// row permutation and orbits (and the column permutation when asked for)
+// from every lane of the group.
+template <int G, bool COUNT>
+ISE_DEV void write_record(const Grp<G> &g, uint32_t ci, const OutPtrs &o, const Verdict &v, int pr, int ob, int pc,
+                          int d, int b) {
+    const int l = g.ln;
+    if (l == 0) {
+        o.verdict[ci] = (uint8_t)v.verdict;
+        if (o.fail_row) o.fail_row[ci] = (uint8_t)v.frow;
+        if (o.fail_col) o.fail_col[ci] = (uint8_t)v.fcol;
+        if (o.row_to_change) o.row_to_change[ci] = (uint8_t)v.rtc;
+        if (o.group_order) o.group_order[ci] = v.verdict == 1 ? v.order : 0ull;
+        if (o.group_order_hi) o.group_order_hi[ci] = v.verdict == 1 ? v.order_hi : 0ull;
+        if (o.n_gens) o.n_gens[ci] = (uint16_t)(v.ngen > 0xffffu ? 0xffffu : v.ngen);
+        if (COUNT && o.counters) {
+            uint32_t *cc = o.counters + (uint64_t)ci * 4;
+            cc[0] = v.perms; cc[1] = v.rows_cmp; cc[2] = v.orbs_cmp; cc[3] = v.cells_cmp;
+        }
+    }
+    // every lane takes part in the gathers; lanes past the record skip the store
+    for (int x0 = 0; x0 < ISE_MAX_ROWS; x0 += G) {
+        const int x = x0 + l;
+        const int vp = g.gather(pr, x < G ? x : 0), vo = g.gather(ob, x < G ? x : 0);
+        if (x < ISE_MAX_ROWS) {
+            if (o.perm_row) o.perm_row[(uint64_t)ci * ISE_MAX_ROWS + x] = (uint8_t)(x < d ? vp : 0xff);
+            if (o.orbits) o.orbits[(uint64_t)ci * ISE_MAX_ROWS + x] = (uint8_t)(x < d ? vo : 0xff);
+        }
+    }
+    if (o.perm_col) {
+        for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
+            const int x = x0 + l;
+            const int vc = g.gather(pc, x < G ? x : 0);
+            o.perm_col[(uint64_t)ci * ISE_MAX_COLS + x] = (uint8_t)(x < b ? vc : 0xff);
+        }
+    }
+}
+
 // Full canonicity test of candidate `ci` by the group g.  With HUNT, a
 // budgeted witness hunt in the proposed order runs first; the plain search
 // decides whatever the hunt leaves open (canonical candidates, |Aut|).  With
@@ -1148,36 +1186,7 @@ ISE_DEV void check_candidate(const Grp<G> &g, uint32_t ci, uint32_t meta, const 
         if (l == 0) defer[atomicAdd(defer_count, 1u)] = ci;
         return;
     }
-    // write the record
-    if (l == 0) {
-        o.verdict[ci] = (uint8_t)v.verdict;
-        if (o.fail_row) o.fail_row[ci] = (uint8_t)v.frow;
-        if (o.fail_col) o.fail_col[ci] = (uint8_t)v.fcol;
-        if (o.row_to_change) o.row_to_change[ci] = (uint8_t)v.rtc;
-        if (o.group_order) o.group_order[ci] = v.verdict == 1 ? v.order : 0ull;
-        if (o.group_order_hi) o.group_order_hi[ci] = v.verdict == 1 ? v.order_hi : 0ull;
-        if (o.n_gens) o.n_gens[ci] = (uint16_t)(v.ngen > 0xffffu ? 0xffffu : v.ngen);
-        if (COUNT && o.counters) {
-            uint32_t *cc = o.counters + (uint64_t)ci * 4;
-            cc[0] = v.perms; cc[1] = v.rows_cmp; cc[2] = v.orbs_cmp; cc[3] = v.cells_cmp;
-        }
-    }
-    // every lane takes part in the gathers; lanes past the record skip the store
-    for (int x0 = 0; x0 < ISE_MAX_ROWS; x0 += G) {
-        const int x = x0 + l;
-        const int vp = g.gather(pr, x < G ? x : 0), vo = g.gather(ob, x < G ? x : 0);
-        if (x < ISE_MAX_ROWS) {
-            if (o.perm_row) o.perm_row[(uint64_t)ci * ISE_MAX_ROWS + x] = (uint8_t)(x < d ? vp : 0xff);
-            if (o.orbits) o.orbits[(uint64_t)ci * ISE_MAX_ROWS + x] = (uint8_t)(x < d ? vo : 0xff);
-        }
-    }
-    if (o.perm_col) {
-        for (int x0 = 0; x0 < ISE_MAX_COLS; x0 += G) {
-            const int x = x0 + l;
-            const int vc = g.gather(pc, x < G ? x : 0);
-            o.perm_col[(uint64_t)ci * ISE_MAX_COLS + x] = (uint8_t)(x < b ? vc : 0xff);
-        }
-    }
+    write_record<G, COUNT>(g, ci, o, v, pr, ob, pc, d, b);
 }
 
 } // namespace ise

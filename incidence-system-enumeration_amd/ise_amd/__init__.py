@@ -128,6 +128,9 @@ def lib():
                                                        ctypes.POINTER(ResultsStruct), P, U32, P, P]
         L.ise_mlp_witness_labels.argtypes = [U32, U32, P, P, P, P, P, P]
         L.ise_mlp_set_gate.argtypes = [P, U32]
+        L.ise_mlp_lead_features.argtypes = [P, U32, U32, P, P, P]
+        L.ise_mlp_check_lead_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(ResultsStruct), P, P, P, P]
+        L.ise_mlp_check_lead_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(ResultsStruct), P, U32, P]
         L.ise_group_closure.argtypes = [P, U32, U32, P, U64, ctypes.POINTER(U64)]
         L.ise_adjust_generators.argtypes = [P, U32, U32, P, U32, P]
         L.ise_solution_orbits.argtypes = [P, P, U64, U32, P, U32, P, P]
@@ -651,6 +654,26 @@ class RowOrderModel:
         """Order only the candidates the plain search leaves open after
         gate_perms permutations (0: order every candidate)."""
         self.ctx._check(self._L.ise_mlp_set_gate(self.h, gate_perms))
+
+    def lead_features(self, n, stride, d_rows, d_meta, d_X):
+        """Features of every row as the row of position 0: d_X [n][16][16] bf16 bits."""
+        self.ctx._check(self._L.ise_mlp_lead_features(self.ctx.h, n, stride, d_rows.ptr, d_meta.ptr, d_X.ptr))
+
+    def check_lead(self, n, stride, d_rows, d_meta, res_struct, d_head, d_lead_in=None, d_lead_out=None,
+                   d_score_out=None):
+        """Learned-order (lead order) test of a device batch, one launch."""
+        self.ctx._check(self._L.ise_mlp_check_lead_device(
+            self.h, n, stride, d_rows.ptr, d_meta.ptr, ctypes.byref(res_struct), d_head.ptr,
+            None if d_lead_in is None else d_lead_in.ptr, None if d_lead_out is None else d_lead_out.ptr,
+            None if d_score_out is None else d_score_out.ptr))
+
+    def time_lead(self, n, stride, d_rows, d_meta, res_struct, d_head, repeats):
+        """Milliseconds of `repeats` lead-order launches (HIP events on the context's stream)."""
+        ms = np.zeros(repeats, np.float32)
+        self.ctx._check(self._L.ise_mlp_check_lead_device_timed(
+            self.h, n, stride, d_rows.ptr, d_meta.ptr, ctypes.byref(res_struct), d_head.ptr, repeats,
+            ms.ctypes.data))
+        return ms
 
     def time_check(self, n, stride, d_rows, d_meta, od, d_order, hunt_budget, res_struct, d_head, repeats):
         """Per repeat: gated plain pass + ordering + learned-order check;

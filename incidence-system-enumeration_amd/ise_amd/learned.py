@@ -17,6 +17,12 @@ from . import (MAX_ROWS, SAVE_ROW_PERMUTATIONS, SAVE_ROW_TO_CHANGE, MLP_FEATURES
                RowOrderModel, random_extensions, witness_labels)
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "row_order_weights.npy")
+LEAD_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lead_order_weights.npy")
+# The lead order's training stream: a beam-sampled 2-(25,4,1) enumeration
+# with its own seed (the benchmark's stream uses 0x25041; candidates that
+# occur in both are dropped from training).
+LEAD_TRAIN_SEED = 0x7777
+LEAD_GATE = 32
 
 # Training streams: the plain enumerator's candidates and seeded random
 # extensions of the three enumeration parameter families.
@@ -88,6 +94,93 @@ def train(ctx, od=MAX_ROWS, per_set=4000, steps=3000, batch=4096, lr=2e-3, seed=
     return model, curve
 
 
+def lead_labels(meta, verdict, perm_row, counters, n_gens, gate):
+    """Labels of the rows of position 0 from plain witness logs: for a
+    non-canonical candidate the walk needed more than `gate` permutations for
+    (and met no automorphism on), the walk tried the rows of position 0 in
+    increasing order and exhausted every row before its witness's first row
+    without a witness, so row p0 = perm_row[0] is labelled 1, rows j < p0 are
+    labelled 0 and rows after p0 are unknown (weight 0).  Returns y, w
+    [n][16]."""
+    n = len(meta)
+    d = (meta & 0xff).astype(np.int64)
+    take = (verdict == 0) & (counters[:, 0] > gate) & (n_gens == 0) & (d >= 2) & (d <= 16)
+    p0 = perm_row[:, 0].astype(np.int64)
+    j = np.arange(16)[None, :]
+    w = (take[:, None] & (j <= p0[:, None])).astype(np.float32)
+    y = (w > 0) & (j == p0[:, None])
+    return y.astype(np.float32), w
+
+
+def lead_training_stream(ctx, exclude_rows=None, gate=LEAD_GATE, seed=LEAD_TRAIN_SEED, beam=64):
+    """Plain witness logs of a beam-sampled 2-(25,4,1) enumeration (depths
+    8-16): rows, meta and labels of the candidates lead_labels keeps."""
+    rows, meta, _ = ctx.enumerate_sampled(25, 4, 1, beam, seed, 8, 16, stride=16, cap=1 << 21)
+    if exclude_rows is not None:
+        seen = {r.tobytes() for r in exclude_rows}
+        keep = np.array([r.tobytes() not in seen for r in rows], bool)
+        rows, meta = rows[keep], meta[keep]
+    res = ctx.check_batch(rows, meta, fields=("verdict", "perm_row", "counters", "n_gens"))
+    y, w = lead_labels(meta, res["verdict"], res["perm_row"], res["counters"], res["n_gens"], gate)
+    sel = np.nonzero(w.sum(1) > 0)[0]
+    return rows[sel], meta[sel], y[sel], w[sel]
+
+
+def train_lead(ctx, rows, meta, y, w, steps=3000, batch=4096, lr=2e-3, seed=1):
+    """Train a fresh network on the labelled rows of position 0 with the
+    library's MFMA forward, backward and Adam kernels; returns (model, loss
+    curve, labelled rows)."""
+    model = RowOrderModel(ctx, max_rows=batch, seed=seed)
+    n, stride = rows.shape
+    d_rows = ctx.device_array(rows.shape, np.uint64, rows)
+    d_meta = ctx.device_array(meta.shape, np.uint32, meta)
+    d_X = ctx.device_array((n, 16, 16), np.uint16)
+    model.lead_features(n, stride, d_rows, d_meta, d_X)
+    X = d_X.to_host().reshape(n * 16, 16)
+    del d_X, d_rows, d_meta
+    keep = np.nonzero(w.ravel() > 0)[0]
+    keep = keep[np.random.default_rng(seed).permutation(len(keep))]
+    M = len(keep) // 16 * 16
+    keep = keep[:M]
+    Xk, yk, wk = X[keep], y.ravel()[keep], w.ravel()[keep]
+    d_X = ctx.device_array(Xk.shape, np.uint16, Xk)
+    d_y = ctx.device_array(yk.shape, np.float32, yk)
+    d_w = ctx.device_array(wk.shape, np.float32, wk)
+    curve = model.fit(d_X, d_y, d_w, M, batch, steps, lr)
+    return model, curve, M
+
+
+def compare_lead(ctx, model, rows, meta, gate=LEAD_GATE, repeats=3):
+    """Plain order against the lead order on one device batch: kernel times,
+    verdicts (must agree), rows compared and mean permutations before
+    rejection from counted launches."""
+    n, stride = rows.shape
+    d_rows = ctx.device_array(rows.shape, np.uint64, rows)
+    d_meta = ctx.device_array(meta.shape, np.uint32, meta)
+    d_head = ctx.device_array((256,), np.uint32)
+    tdev, tres = ctx.device_results(n, ("verdict",))
+    cdev, cres = ctx.device_results(n, ("verdict", "counters"))
+    ms_plain = ctx.time_batch_device(n, stride, d_rows, d_meta, tres, d_head, repeats)
+    ctx.time_batch_device(n, stride, d_rows, d_meta, cres, d_head, 1)
+    verdict, counters = cdev["verdict"].to_host(), cdev["counters"].to_host()
+    model.set_gate(gate)
+    ms_lead = model.time_lead(n, stride, d_rows, d_meta, tres, d_head, repeats)
+    lv = tdev["verdict"].to_host()
+    model.check_lead(n, stride, d_rows, d_meta, cres, d_head)
+    lc = cdev["counters"].to_host()
+    model.set_gate()
+    if not (np.array_equal(lv, verdict) and np.array_equal(cdev["verdict"].to_host(), verdict)):
+        raise AssertionError("the lead order changed a verdict")
+    rej = verdict == 0
+    return {"n": int(n), "noncanonical": int(rej.sum()), "gate": int(gate),
+            "plain_ms": float(np.min(ms_plain)), "lead_ms": float(np.min(ms_lead)),
+            "plain_rows": float(counters[:, 1].sum()), "lead_rows": float(lc[:, 1].sum()),
+            "plain_mean_perms_to_reject": float(counters[rej, 0].mean()),
+            "lead_mean_perms_to_reject": float(lc[rej, 0].mean()),
+            "plain_mean_rows_to_reject": float(counters[rej, 1].mean()),
+            "lead_mean_rows_to_reject": float(lc[rej, 1].mean())}
+
+
 def load(ctx, path=WEIGHTS):
     """A model with the stored trained parameters."""
     model = RowOrderModel(ctx, max_rows=4096)
@@ -142,6 +235,49 @@ def compare_orders(ctx, model, rows, meta, od, budgets=(8, 32, 128), repeats=3):
     return out
 
 
+def main_lead(argv=None):
+    """Train the lead-order network and compare it with the plain order on the
+    benchmark's 2-(25,4,1) stream."""
+    import argparse
+    import json
+    import time
+    from . import Context
+    ap = argparse.ArgumentParser(description="train the lead-order network on plain witness logs of a "
+                                             "2-(25,4,1) enumeration and compare it with the plain order")
+    ap.add_argument("--out", default=LEAD_WEIGHTS)
+    ap.add_argument("--report", default=None)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--gate", type=int, default=LEAD_GATE)
+    ap.add_argument("--eval-random", type=int, default=1 << 20)
+    args = ap.parse_args(argv)
+    ctx = Context(capacity=1 << 18, stride=32)
+    t0 = time.time()
+    rows, meta, n_enum = c5_stream(ctx, n_random=args.eval_random)
+    print(f"benchmark stream: {len(meta)} candidates ({n_enum} enumerator), {time.time() - t0:.1f}s", flush=True)
+    t0 = time.time()
+    tr = lead_training_stream(ctx, exclude_rows=rows[:n_enum], gate=args.gate)
+    print(f"training candidates {len(tr[1])}, {time.time() - t0:.1f}s", flush=True)
+    t1 = time.time()
+    model, curve, M = train_lead(ctx, *tr, steps=args.steps, batch=args.batch, lr=args.lr)
+    t2 = time.time()
+    np.save(args.out, model.params)
+    print(f"labelled rows {M}, loss {curve[:50].mean():.4f} -> {curve[-50:].mean():.4f}, "
+          f"training {t2 - t1:.1f}s", flush=True)
+    report = {"training": {"candidates": int(len(tr[1])), "labelled_rows": int(M), "steps": args.steps,
+                           "batch": args.batch, "lr": args.lr, "seed": LEAD_TRAIN_SEED, "gate": args.gate,
+                           "loss_first50": float(curve[:50].mean()), "loss_last50": float(curve[-50:].mean()),
+                           "seconds": t2 - t1}}
+    report["enumerator"] = compare_lead(ctx, model, rows[:n_enum], meta[:n_enum], args.gate)
+    print(json.dumps(report["enumerator"]), flush=True)
+    report["whole"] = compare_lead(ctx, model, rows, meta, args.gate)
+    print(json.dumps(report["whole"]), flush=True)
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(report, f, indent=1)
+
+
 def main(argv=None):
     import argparse
     import json
@@ -183,4 +319,8 @@ def main(argv=None):
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "lead":
+        main_lead(sys.argv[2:])
+    else:
+        main()

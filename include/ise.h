@@ -383,6 +383,41 @@ int ise_mlp_check_batch_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, co
                                      const ise_results *d_out, uint32_t *d_counter, uint32_t repeats, float *ms_out,
                                      float *order_ms_out);
 
+/* Features of every row of a device batch as the row of position 0 (the
+ * lead order's network input): d_X [n][16][16] bf16 bits, row j's 16
+ * features at [c][j], zero past the depth and for depths outside 2..16. */
+int ise_mlp_lead_features(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                          const uint32_t *d_meta, uint16_t *d_X);
+
+/* Learned-order test of a device batch in one launch (the lead order): each
+ * candidate in orbit form with depth 2..16, lenStab 0 and at most 56 columns
+ * runs the plain search with a budget of `gate` permutations
+ * (ise_mlp_set_gate); when that leaves it open without an automorphism, the
+ * network scores its rows as the row of position 0 and a depth-first search
+ * tries them in that order (every later position in increasing row order).
+ * It stops at a witness of that order (verdict 0, valid, not necessarily the
+ * plain order's first); when it ends without witness and automorphism the
+ * candidate is canonical with a trivial group and gets exactly the plain
+ * search's results; when it meets an automorphism the plain search decides
+ * from the start.  Verdicts and, for canonical candidates, every field equal
+ * the plain search's.  Other candidates run the plain search.  Counters (if
+ * d_out->counters): the gate's work, the lead search's, and the plain
+ * search's when it ran.  No column permutation (d_out->perm_col must be
+ * null).  d_lead_in (optional [n][16]) replaces the network's order of the
+ * rows of position 0 (rows >= depth and repeats skipped, unlisted rows after
+ * the listed ones in increasing order); d_lead_out / d_score_out (optional
+ * [n][16]) receive the order used (0xff past the depth) and the scores, for
+ * the candidates the lead search ran on.  Replaces, for the learned order,
+ * the permutation walk of CCanonicityChecker::TestCanonicity
+ * (Source/CanonicityChecker.h:255-586; GroupOrder.h:136-248). */
+int ise_mlp_check_lead_device(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                              const uint32_t *d_meta, const ise_results *d_out, uint32_t *d_counter,
+                              const uint8_t *d_lead_in, uint8_t *d_lead_out, float *d_score_out);
+/* The same, `repeats` launches on the context's stream, ms_out[r] each. */
+int ise_mlp_check_lead_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                                    const uint32_t *d_meta, const ise_results *d_out, uint32_t *d_counter,
+                                    uint32_t repeats, float *ms_out);
+
 /* Permutations of plain search after which a candidate of a learned-order
  * batch is ordered by the network (0: every candidate is). */
 #define ISE_DEFAULT_LEARNED_GATE 16u

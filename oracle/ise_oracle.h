@@ -154,6 +154,30 @@ int orc_check_batch_hunt(uint32_t n, uint32_t stride, const uint64_t *rows, cons
                          const uint32_t *meta, const uint8_t *order, uint32_t order_dim,
                          uint32_t budget, uint32_t gen_cap, orc_results *res);
 
+/* ---- lead order: the learned order of the rows of position 0 ----
+ * Features of every row j < d <= 16 as the row of position 0: X[c][j][16]
+ * (zero past d).  Defined in ise_oracle_bibd.c with the search below. */
+int orc_lead_features(uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta, float *X);
+/* Learned-order test of a batch: the plain search decides every candidate
+ * whose walk needs at most gate_perms permutations (and every candidate the
+ * lead order does not take: depth < 2 or > 16, lenStab > 0, more than 56
+ * columns or not in orbit form); the others run the depth-first search with
+ * the rows of position 0 in the order lead[c][0..15] (entries >= depth
+ * skipped): a witness of that order, the plain results when it ends without
+ * one (canonical, trivial group), the plain search when it meets an
+ * automorphism (as it does when the gate's permutations found one).  The
+ * rows of position 0: lead[c][0..15] (rows >= depth and repeats skipped), then
+ * the rows not listed, in increasing order.  res->counters is required (the
+ * gate reads it); for the candidates of the lead search it receives the
+ * gate's work (the walk's first gate_perms permutations), then the lead
+ * search's (permutations counted as the walk counts them: a new row after a
+ * back-up or a refutation), plus the whole plain search's when it met an
+ * automorphism.  mode[c] (optional): 0 plain only, 1 decided within the gate,
+ * 2 lead witness, 3 lead search exhausted, 4 automorphism met. */
+int orc_check_batch_lead(uint32_t n, uint32_t stride, const uint64_t *rows, const uint64_t *cells,
+                         const uint32_t *meta, const uint8_t *lead, uint32_t gate_perms, uint32_t gen_cap,
+                         orc_results *res, uint8_t *mode);
+
 /* ---- learned row-order scorer (ise_oracle_mlp.c) ---- */
 /* 16 features of every (position i, row j) pair: X[(c*od+i)*od+j][16],
  * zero outside depth x depth. */

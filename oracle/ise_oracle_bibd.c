@@ -276,6 +276,10 @@ static void reconstruct_solution(const cand_t *cd, int row, int deciding, const 
     improved_len = cd->norb[row];
 }
 
+/* Permutations the walk may try before it stops unfinished (the learned
+ * order's gate); UINT32_MAX: no limit. */
+static __thread uint32_t walk_limit = 0xffffffffu;
+
 static int run_check(const cand_t *cd, orc_results *res, size_t idx, uint32_t gen_cap)
 {
     return run_check_from(cd, res, idx, gen_cap, NULL);
@@ -306,6 +310,10 @@ static int run_check_from(const cand_t *cd, orc_results *res, size_t idx, uint32
         row = next_row_perm(&g, perm, orb, (u8)d, row, (u8)cd->len_stab);
         if (row == EMAX)
             break;
+        if (perms == walk_limit) {         /* unfinished: the counters so far */
+            verdict = 2;
+            goto done;
+        }
         perms++;
         int refuted = 0;
         for (; row < d; row++) {
@@ -1072,6 +1080,273 @@ int orc_check_batch_hunt(uint32_t n, uint32_t stride, const uint64_t *rows, cons
         } else {
             run_check_from(&cd, res, c, gen_cap, cnt);
         }
+    }
+    return 0;
+}
+
+/* ---- lead-order search (the learned order of the first row) --------------
+ *
+ * The project's learned order, not a reference algorithm.  Until the walk of
+ * next_row_perm meets its first automorphism it is a depth-first search over
+ * row prefixes in increasing row order (lexicographic order of the row
+ * permutations, GroupOrder.h:136-248 with trivial orbits), pruned where a
+ * prefix compares smaller.  The set of prefixes that compare equal does not
+ * depend on the order in which the rows of a position are tried, so a
+ * depth-first search that tries the rows of position 0 in another order and
+ * the rows of every later position in increasing order
+ *   - meets an automorphism iff the group is not trivial (the search then
+ *     stops and the candidate takes the plain search),
+ *   - ends without a witness and without an automorphism iff the matrix is
+ *     canonical with a trivial group (the plain search's results), and
+ *   - otherwise stops at a witness of its own order (a larger matrix, first
+ *     at *frow).
+ * The node "prefix 0..i-1 is the identity, row i at position i" is equal by
+ * construction and not counted (the plain walk starts on the identity and
+ * never compares it); it still partitions the columns as a comparison would.
+ * lead: the rows of position 0 in order (entries >= d skipped).  Returns 0
+ * (exhausted), 1 (witness: perm_out, *frow, *fcol) or 2 (automorphism). */
+static int lead_dfs(const cand_t *cd, const u8 *lead, int nlead, u8 *perm_out, u8 *frow, u8 *fcol,
+                    uint32_t *cnt)
+{
+    const int d = cd->nrows, b = cd->ncols;
+    const u8 *mat = cd->mat;
+    u8 perm[ORC_MAX_ROWS], pcol[ORC_MAX_COLS];
+    int cursor[ORC_MAX_ROWS + 1];
+    uint32_t used = 0, tried = 0;
+    for (int q = 0; q < b; q++) pcol[q] = (u8)q;
+    int i = 0, idlen = 0, desc = 0;               /* desc: the last step descended */
+    cursor[0] = 0;
+    for (;;) {
+        if (i == d) {
+            if (idlen < d) return 2;               /* a non-identity permutation matched */
+            i--;                                   /* the identity: back up */
+            used &= ~(1u << perm[i]);
+            idlen = i;
+            desc = 0;
+            continue;
+        }
+        int j = -1;
+        if (i == 0) {
+            /* the listed rows (each once, rows >= d skipped), then the others in increasing order */
+            while (j < 0 && cursor[0] < nlead + d) {
+                const int k = cursor[0]++;
+                const int r = k < nlead ? lead[k] : k - nlead;
+                if (r < d && !((tried >> r) & 1)) j = r;
+            }
+            if (j >= 0) tried |= 1u << j;
+        } else {
+            while (cursor[i] < d && ((used >> cursor[i]) & 1)) cursor[i]++;
+            if (cursor[i] < d) j = cursor[i]++;
+        }
+        if (j < 0) {
+            if (i == 0) return 0;
+            i--;
+            used &= ~(1u << perm[i]);
+            if (idlen > i) idlen = i;
+            desc = 0;
+            continue;
+        }
+        perm[i] = (u8)j;
+        const int ident = idlen == i && j == i;
+        /* the walk counts a permutation per new row after a back-up or a
+         * refutation; a descent continues the permutation */
+        if (!ident) { cnt[0] += !desc; cnt[1]++; }
+        desc = 0;
+        const u8 *orig = mat + (size_t)i * b, *prow = mat + (size_t)j * b;
+        int diff = 0, o;
+        for (o = 0; o < cd->norb[i]; o++) {
+            int s = cd->ostart[i][o], len = cd->olen[i][o];
+            if (!ident) { cnt[2]++; cnt[3] += (uint32_t)len; }
+            diff = len > 1 ? compare_orbit(s, len, orig, prow, pcol) : (int)orig[s] - (int)prow[pcol[s]];
+            if (diff) break;
+        }
+        if (diff > 0) continue;
+        if (diff < 0) {
+            uint32_t u = used | (1u << j);
+            int k = i + 1;
+            for (int r = 0; r < d; r++)
+                if (!((u >> r) & 1)) perm[k++] = (u8)r;
+            memcpy(perm_out, perm, (size_t)d);
+            *frow = (u8)i;
+            *fcol = cd->ostart[i][o];
+            return 1;
+        }
+        used |= 1u << j;
+        if (ident) idlen = i + 1;
+        i++;
+        cursor[i] = 0;
+        desc = 1;
+    }
+}
+
+/* Orbit form (what the column-set search needs): inside every column orbit
+ * of a row the ones come first.  With given cell masks, the orbits of row
+ * i+1 must also be those of row i split after its ones. */
+static int cand_orbit_form(const cand_t *cd, const uint64_t *cells_in, const uint64_t *rows)
+{
+    const int d = cd->nrows, b = cd->ncols;
+    for (int i = 0; i < d; i++)
+        for (int o = 0; o < cd->norb[i]; o++) {
+            const u8 *r = cd->mat + (size_t)i * b + cd->ostart[i][o];
+            for (int q = 1; q < cd->olen[i][o]; q++)
+                if (r[q] > r[q - 1]) return 0;
+        }
+    if (cells_in) {
+        const uint64_t full = b >= 64 ? ~0ull : ((1ull << b) - 1);
+        for (int i = 0; i + 1 < d; i++) {
+            uint64_t ci = (cells_in[i] | 1ull) & full, cn = (cells_in[i + 1] | 1ull) & full;
+            if (cn != ((ci | (rows[i] ^ (rows[i] << 1))) & full)) return 0;
+        }
+    }
+    return 1;
+}
+
+/* Features of the rows of one candidate as the row of position 0 (16 per row,
+ * small integers scaled by powers of two, exact in bf16):
+ * with w(c) = the number of rows holding a one in column c and m = max w,
+ * A_j[s] = ones of row j in columns of weight m - s (s = 0, 1, 2) and of
+ * weight <= m - 3 (s = 3):
+ *   0-3  A_j / 16            4-7  (A_j - A_0) / 16
+ *   8    sign of A_j against A_0 (lexicographic, +1 when larger)
+ *   9    j == 0              10   j == d - 1
+ *   11   j / 16              12   d / 16
+ *   13   largest weight of a column shared with row 0 (0 for row 0) / 16
+ *   14   largest weight of a column shared with row d-1 (0 for it) / 16
+ *   15   rows with the same A as row j / 16 */
+static void lead_features_one(const uint64_t *R, int d, int b, float *F)
+{
+    int w[ORC_MAX_COLS];
+    int m = 0;
+    for (int c = 0; c < b; c++) {
+        w[c] = 0;
+        for (int i = 0; i < d; i++) w[c] += (int)((R[i] >> c) & 1);
+        if (w[c] > m) m = w[c];
+    }
+    int A[ORC_MAX_ROWS][4];
+    for (int j = 0; j < d; j++) {
+        for (int s = 0; s < 4; s++) A[j][s] = 0;
+        for (int c = 0; c < b; c++) {
+            if (!((R[j] >> c) & 1)) continue;
+            int s = m - w[c];
+            A[j][s < 3 ? s : 3]++;
+        }
+    }
+    for (int j = 0; j < d; j++) {
+        float *f = F + (size_t)j * 16;
+        int sg = 0;
+        for (int s = 0; s < 4 && !sg; s++) sg = A[j][s] > A[0][s] ? 1 : (A[j][s] < A[0][s] ? -1 : 0);
+        int m0 = 0, mn = 0, ties = 0;
+        for (int c = 0; c < b; c++) {
+            if (j != 0 && ((R[j] & R[0]) >> c) & 1 && w[c] > m0) m0 = w[c];
+            if (j != d - 1 && ((R[j] & R[d - 1]) >> c) & 1 && w[c] > mn) mn = w[c];
+        }
+        for (int q = 0; q < d; q++)
+            ties += A[q][0] == A[j][0] && A[q][1] == A[j][1] && A[q][2] == A[j][2] && A[q][3] == A[j][3];
+        for (int s = 0; s < 4; s++) {
+            f[s] = (float)A[j][s] / 16.f;
+            f[4 + s] = (float)(A[j][s] - A[0][s]) / 16.f;
+        }
+        f[8] = (float)sg;
+        f[9] = j == 0 ? 1.f : 0.f;
+        f[10] = j == d - 1 ? 1.f : 0.f;
+        f[11] = (float)j / 16.f;
+        f[12] = (float)d / 16.f;
+        f[13] = (float)m0 / 16.f;
+        f[14] = (float)mn / 16.f;
+        f[15] = (float)ties / 16.f;
+    }
+}
+
+int orc_lead_features(uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta, float *X)
+{
+    memset(X, 0, sizeof(float) * (size_t)n * 16 * 16);
+    for (uint32_t c = 0; c < n; c++) {
+        const int d = meta[c] & 0xff, b = (meta[c] >> 8) & 0xff;
+        if (d < 1 || d > 16 || d > (int)stride || b < 1 || b > ORC_MAX_COLS) continue;
+        lead_features_one(rows + (size_t)c * stride, d, b, X + (size_t)c * 256);
+    }
+    return 0;
+}
+
+int orc_check_batch_lead(uint32_t n, uint32_t stride, const uint64_t *rows, const uint64_t *cells,
+                         const uint32_t *meta, const uint8_t *lead, uint32_t gate_perms, uint32_t gen_cap,
+                         orc_results *res, uint8_t *mode)
+{
+    static __thread cand_t cd;
+    static __thread u8 mat[ORC_MAX_ROWS * ORC_MAX_COLS];
+    for (uint32_t c = 0; c < n; c++) {
+        uint32_t m = meta[c];
+        int d = m & 0xff, b = (m >> 8) & 0xff;
+        if (d < 1 || d > ORC_MAX_ROWS || d > (int)stride || b < 1 || b > ORC_MAX_COLS)
+            return -1;
+        const uint64_t *rp = rows + (size_t)c * stride;
+        const uint64_t *cp = cells ? cells + (size_t)c * stride : NULL;
+        cd.nrows = d; cd.ncols = b;
+        cd.len_stab = (m >> 16) & 0xff;
+        cd.out_info = (m >> 24) & 0xff;
+        for (int i = 0; i < d; i++)
+            for (int q = 0; q < b; q++)
+                mat[i * b + q] = (u8)((rp[i] >> q) & 1);
+        cd.mat = mat;
+        build_orbits(&cd, cp, rp);
+        run_check(&cd, res, c, gen_cap);
+        uint8_t md = 0;
+        const int eligible = lead && d >= 2 && d <= 16 && cd.len_stab == 0 && b <= 56 &&
+                             cand_orbit_form(&cd, cp, rp);
+        if (eligible) {
+            if (!res->counters)            /* the gate reads the plain search's permutations */
+                return -2;
+            uint32_t plain_cnt[4];
+            memcpy(plain_cnt, res->counters + (size_t)c * 4, sizeof plain_cnt);
+            md = 1;
+            if (plain_cnt[0] > gate_perms) {
+                /* the gate: the walk's first gate_perms permutations */
+                uint32_t gate_cnt[4] = {0, 0, 0, 0};
+                uint16_t gate_ngen = 0;
+                uint8_t gv = 0;
+                orc_results gr;
+                memset(&gr, 0, sizeof gr);
+                gr.verdict = &gv;
+                gr.counters = gate_cnt;
+                gr.n_gens = &gate_ngen;
+                walk_limit = gate_perms;
+                run_check(&cd, &gr, 0, 0);
+                walk_limit = 0xffffffffu;
+                if (gate_ngen > 0) {
+                    md = 4;                        /* an automorphism within the gate: plain search */
+                } else {
+                    uint32_t cnt[4];
+                    memcpy(cnt, gate_cnt, sizeof cnt);
+                    u8 perm[ORC_MAX_ROWS], frow = 255, fcol = 255;
+                    int r = lead_dfs(&cd, lead + (size_t)c * 16, 16, perm, &frow, &fcol, cnt);
+                    if (r == 1) {
+                        md = 2;
+                        if (res->verdict) res->verdict[c] = 0;
+                        if (res->fail_row) res->fail_row[c] = frow;
+                        if (res->fail_col) res->fail_col[c] = fcol;
+                        if (res->row_to_change) res->row_to_change[c] = row_to_change(perm, frow);
+                        if (res->group_order) res->group_order[c] = 0;
+                        if (res->group_order_hi) res->group_order_hi[c] = 0;
+                        if (res->perm_row) {
+                            memset(res->perm_row + c * ORC_MAX_ROWS, 0xff, ORC_MAX_ROWS);
+                            memcpy(res->perm_row + c * ORC_MAX_ROWS, perm, (size_t)d);
+                        }
+                        if (res->orbits) {
+                            memset(res->orbits + c * ORC_MAX_ROWS, 0xff, ORC_MAX_ROWS);
+                            for (int i = 0; i < d; i++) res->orbits[c * ORC_MAX_ROWS + i] = (u8)i;
+                        }
+                        if (res->n_gens) res->n_gens[c] = 0;
+                    } else {
+                        md = r == 0 ? 3 : 4;
+                    }
+                    /* a candidate that met an automorphism also runs the whole plain search */
+                    if (r == 2)
+                        for (int q = 0; q < 4; q++) cnt[q] += plain_cnt[q];
+                    memcpy(res->counters + (size_t)c * 4, cnt, sizeof cnt);
+                }
+            }
+        }
+        if (mode) mode[c] = md;
     }
     return 0;
 }

@@ -47,6 +47,8 @@ def load():
         L.orc_row_solutions.restype = ctypes.c_long
         L.orc_bruteforce_canon.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P]
         L.orc_check_batch_hunt.argtypes = [U32, U32, _P, _P, _P, _P, U32, U32, U32, _P]
+        L.orc_lead_features.argtypes = [U32, U32, _P, _P, _P]
+        L.orc_check_batch_lead.argtypes = [U32, U32, _P, _P, _P, _P, U32, U32, _P, _P]
         L.orc_hunt.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, U32, _P, _P, _P]
         L.orc_kf_check.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.orc_kf_check_2p1f.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
@@ -102,6 +104,35 @@ class Oracle:
         if rc != 0:
             raise ValueError("oracle rejected the batch")
         return res
+
+    def lead_features(self, rows, meta):
+        """Features of every row as the row of position 0: [n][16][16] float32."""
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32)
+        X = np.zeros((len(meta), 16, 16), np.float32)
+        self.L.orc_lead_features(len(meta), rows.shape[1], rows.ctypes.data, meta.ctypes.data, X.ctypes.data)
+        return X
+
+    def check_batch_lead(self, rows, meta, lead, gate_perms, cells=None, gen_cap=0):
+        """Learned-order test with the rows of position 0 in the order lead[n][16]:
+        (results, mode[n]) -- see orc_check_batch_lead."""
+        import ise_amd._abi as abi
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32)
+        lead = np.ascontiguousarray(lead, dtype=np.uint8)
+        n, stride = rows.shape
+        res = abi.alloc_results(n, gen_cap)
+        s = abi.results_struct(res, gen_cap)
+        mode = np.zeros(n, np.uint8)
+        cells_p = None
+        if cells is not None:
+            cells = np.ascontiguousarray(cells, dtype=np.uint64)
+            cells_p = cells.ctypes.data
+        rc = self.L.orc_check_batch_lead(n, stride, rows.ctypes.data, cells_p, meta.ctypes.data, lead.ctypes.data,
+                                         gate_perms, gen_cap, ctypes.addressof(s), mode.ctypes.data)
+        if rc != 0:
+            raise ValueError(f"oracle rejected the batch ({rc})")
+        return res, mode
 
     def walk_step(self, perm, orbits, at, fixed, state):
         """One step of the oracle's row-permutation walk; perm (uint8 [n]) and
