@@ -120,7 +120,9 @@ def lead_training_stream(ctx, exclude_rows=None, gate=LEAD_GATE, seed=LEAD_TRAIN
         seen = {r.tobytes() for r in exclude_rows}
         keep = np.array([r.tobytes() not in seen for r in rows], bool)
         rows, meta = rows[keep], meta[keep]
-    res = ctx.check_batch(rows, meta, fields=("verdict", "perm_row", "counters", "n_gens"))
+    padded = np.zeros((len(meta), ctx.stride), np.uint64)
+    padded[:, :rows.shape[1]] = rows
+    res = ctx.check_batch(padded, meta, fields=("verdict", "perm_row", "counters", "n_gens"))
     y, w = lead_labels(meta, res["verdict"], res["perm_row"], res["counters"], res["n_gens"], gate)
     sel = np.nonzero(w.sum(1) > 0)[0]
     return rows[sel], meta[sel], y[sel], w[sel]

@@ -165,3 +165,28 @@ def test_uncounted_launch_and_plain_verdicts(gpu_ctx, oracle, model, cases):
         assert_same(name, {k: v for k, v in g2.items() if k != "counters"},
                     {k: v for k, v in plain.items() if k != "counters"}, canon)
     model.set_gate()
+
+
+def test_trained_lead_network_gain(gpu_ctx, oracle):
+    """On 2-(25,4,1) candidates of a seed used neither in training nor in the
+    benchmark, the trained network's first-row order compares fewer rows than
+    the increasing order (the plain walk's) and than an untrained network,
+    with the same verdicts."""
+    from ise_amd import learned
+    rows, meta, _ = oracle.enumerate_sampled(25, 4, 1, 32, 0x2929, 8, 16, stride=32, cap=1 << 20)
+    n = len(meta)
+    print(f"case lead gain: {n} candidates", flush=True)
+    plain = oracle.check_batch(rows, meta)
+    totals = {}
+    for name in ("trained", "untrained", "ascending"):
+        m = learned.load(gpu_ctx, learned.LEAD_WEIGHTS) if name == "trained" else \
+            ise_amd.RowOrderModel(gpu_ctx, max_rows=4096, seed=3)
+        m.set_gate(learned.LEAD_GATE)
+        lead = np.tile(np.arange(16, dtype=np.uint8), (n, 1)) if name == "ascending" else None
+        g, _, _ = run_lead(gpu_ctx, m, rows, meta, lead)
+        assert np.array_equal(g["verdict"], plain["verdict"]), name
+        totals[name] = int(g["counters"][:, 1].astype(np.int64).sum())
+        m.close()
+    print(f"rows compared: {totals}, plain {int(plain['counters'][:, 1].astype(np.int64).sum())}", flush=True)
+    assert totals["trained"] < 0.95 * totals["ascending"]
+    assert totals["trained"] < totals["untrained"]
