@@ -14,12 +14,13 @@ comes from the autotuner, run on the same batch before timing.
   * plain order: one launch of the canonicity kernel per step (column-set
     search, candidates started deepest first);
   * learned order: per step, one launch of the lead-order kernel: the plain
-    search on every candidate with a budget of 32 permutations; on the
+    search on every candidate with a budget of 16 permutations; on the
     candidates it leaves open, the 16x128x128x1 bf16 MFMA network scores
     every row as the row of position 0 and a depth-first search tries them
     in that order (meeting an automorphism sends a candidate back to the
-    plain search).  Verdicts, and every field of the canonical candidates,
-    are checked equal to the plain order's.
+    plain search).  Its launch shape comes from its own autotuning run on the
+    batch.  Verdicts, and every field of the canonical candidates, are
+    checked equal to the plain order's.
 
 `value` is the plain-order rate (the library's default order); the
 learned-order rate and learned/plain ratio, both mean
@@ -360,10 +361,24 @@ def enumeration_rate(cpu=True):
             dt2 = time.perf_counter() - t0
             if not np.array_equal(des, des2):
                 raise RuntimeError("device enumeration differs from the host-solver enumeration")
+            # the same host-solver enumeration with each level's launch replayed from a
+            # captured graph (updated in place across levels and tail chunks)
+            ctx.use_graphs(True)
+            ctx.enumerate_bibd(v, k, lam)
+            g0 = ctx.graph_stats()
+            t0 = time.perf_counter()
+            des3, _, _, _ = ctx.enumerate_bibd(v, k, lam)
+            dt3 = time.perf_counter() - t0
+            g1 = ctx.graph_stats()
+            ctx.use_graphs(False)
+            if not np.array_equal(des, des3):
+                raise RuntimeError("the enumeration with graphs differs")
             rec = {"designs": len(des), "seconds": dt, "designs_per_s": len(des) / dt,
                    "candidates_per_s": st["candidates"] / dt,
                    "device_resident_seconds": dt2,
-                   "device_resident_designs_per_s": len(des) / dt2}
+                   "device_resident_designs_per_s": len(des) / dt2,
+                   "graphs_seconds": dt3, "graphs_designs_per_s": len(des) / dt3,
+                   "graph_replays": int(g1[1] - g0[1]), "graphs_cached": int(g1[0])}
             if orc is not None:
                 t0 = time.perf_counter()
                 odes, _, ost, _ = orc.enumerate(v, k, lam)
@@ -385,7 +400,7 @@ def main():
     ap.add_argument("--random", type=int, default=1 << 20)
     ap.add_argument("--group-lanes", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=4)
-    ap.add_argument("--learned-gate", type=int, default=32,
+    ap.add_argument("--learned-gate", type=int, default=16,
                     help="plain permutations after which the network orders a candidate's first row (0: all)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="use --group-lanes/--waves-per-block instead of the autotuner's choice")
@@ -480,6 +495,13 @@ def main():
     if not args.no_learned:
         model = learned.load(ctx, learned.LEAD_WEIGHTS)
         model.set_gate(args.learned_gate)
+        plain_launch = ctx.get_launch()
+        lead_tuned = None
+        if args.autotune:
+            # the lead-order launch's own shape, on this batch (one candidate per wave)
+            best_wpb, ltable = learned.autotune_lead(ctx, model, n, STRIDE, d_rows, d_meta, res, d_head)
+            lead_tuned = {"best_waves_per_block": best_wpb, "ms_table_1_2_4_8_waves": ltable}
+            print(f"bench: lead order autotuned to {best_wpb} waves per workgroup", file=sys.stderr, flush=True)
         if args.warmup:
             model.time_lead(n, STRIDE, d_rows, d_meta, res, d_head, args.warmup)
         lms = model.time_lead(n, STRIDE, d_rows, d_meta, res, d_head, args.steps)
@@ -512,9 +534,11 @@ def main():
             "rows_compared": float(lcount[:, 1].astype(np.float64).sum()),
             "verdicts_equal_to_plain": True,
             "canonical_fields_equal_to_plain": True,
+            "autotune": lead_tuned,
         }
         print(f"bench: learned {lmean:.2f} ms/step", file=sys.stderr, flush=True)
         model.close()
+        ctx.set_launch(**plain_launch)
 
     # algorithmic HBM bytes of one plain launch: candidate rows + meta in, result
     # record out (generators: 64 bytes per stored automorphism)

@@ -204,7 +204,7 @@ __global__ __launch_bounds__(1024) ISE_CHECK_OCCUPANCY void check_kernel(uint32_
 // the (scheduled) queue; candidates the lead order does not take run
 // check_candidate's column-set or replayed search.
 template <bool COUNT>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) void lead_kernel(
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(ISE_LEAD_WAVES_PER_EU))) void lead_kernel(
     uint32_t n, uint32_t stride, const uint64_t *__restrict__ rows, const uint32_t *__restrict__ meta, OutPtrs o,
     uint32_t *head, uint32_t max_perms, uint32_t rows_cap, uint32_t gate, const bf16x4 *__restrict__ frag,
     const float *__restrict__ w, const uint8_t *__restrict__ lead_in, uint8_t *lead_out, float *score_out,

@@ -27,6 +27,13 @@
 namespace ise {
 
 constexpr int LEAD_MAX_ROWS = 16;   // one 16-row tile of the network
+
+// Waves per SIMD the lead kernel's registers must leave room for: 5 is what
+// its 8 KiB of LDS per wave (16 rows) allows; the compiler then keeps it to
+// 96 VGPRs.
+#ifndef ISE_LEAD_WAVES_PER_EU
+#define ISE_LEAD_WAVES_PER_EU 5
+#endif
 constexpr int LEAD_EXHAUSTED = 0, LEAD_WITNESS = 1, LEAD_AUT = 2, LEAD_LIMIT = 3;
 
 // Feature `id` of row j as the row of position 0 (the oracle's

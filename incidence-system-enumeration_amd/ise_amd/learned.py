@@ -22,7 +22,7 @@ LEAD_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lead_or
 # with its own seed (the benchmark's stream uses 0x25041; candidates that
 # occur in both are dropped from training).
 LEAD_TRAIN_SEED = 0x7777
-LEAD_GATE = 32
+LEAD_GATE = 16
 
 # Training streams: the plain enumerator's candidates and seeded random
 # extensions of the three enumeration parameter families.
@@ -128,7 +128,7 @@ def lead_training_stream(ctx, exclude_rows=None, gate=LEAD_GATE, seed=LEAD_TRAIN
     return rows[sel], meta[sel], y[sel], w[sel]
 
 
-def train_lead(ctx, rows, meta, y, w, steps=3000, batch=4096, lr=2e-3, seed=1):
+def train_lead(ctx, rows, meta, y, w, steps=1000, batch=4096, lr=1e-3, seed=1):
     """Train a fresh network on the labelled rows of position 0 with the
     library's MFMA forward, backward and Adam kernels; returns (model, loss
     curve, labelled rows)."""
@@ -150,6 +150,22 @@ def train_lead(ctx, rows, meta, y, w, steps=3000, batch=4096, lr=2e-3, seed=1):
     d_w = ctx.device_array(wk.shape, np.float32, wk)
     curve = model.fit(d_X, d_y, d_w, M, batch, steps, lr)
     return model, curve, M
+
+
+def autotune_lead(ctx, model, n, stride, d_rows, d_meta, res_struct, d_head, waves=(1, 2, 4, 8), repeats=2):
+    """Waves per workgroup of the lead-order launch: each of `waves` timed on
+    the batch (after one untimed launch), the fastest kept in the context's
+    launch settings.  Returns (best waves per workgroup, ms per setting)."""
+    saved = ctx.get_launch()
+    table = []
+    for wpb in waves:
+        ctx.set_launch(group_lanes=64, waves_per_block=wpb, blocks=saved["blocks"], max_perms=saved["max_perms"])
+        model.time_lead(n, stride, d_rows, d_meta, res_struct, d_head, 1)
+        table.append(float(np.mean(model.time_lead(n, stride, d_rows, d_meta, res_struct, d_head, repeats))))
+    best = waves[int(np.argmin(table))]
+    ctx.set_launch(group_lanes=saved["group_lanes"], waves_per_block=best, blocks=saved["blocks"],
+                   max_perms=saved["max_perms"])
+    return best, table
 
 
 def compare_lead(ctx, model, rows, meta, gate=LEAD_GATE, repeats=3):
@@ -248,9 +264,9 @@ def main_lead(argv=None):
                                              "2-(25,4,1) enumeration and compare it with the plain order")
     ap.add_argument("--out", default=LEAD_WEIGHTS)
     ap.add_argument("--report", default=None)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--gate", type=int, default=LEAD_GATE)
     ap.add_argument("--eval-random", type=int, default=1 << 20)
     args = ap.parse_args(argv)
