@@ -112,9 +112,11 @@ def lead_labels(meta, verdict, perm_row, counters, n_gens, gate):
     return y.astype(np.float32), w
 
 
-def lead_training_stream(ctx, exclude_rows=None, gate=LEAD_GATE, seed=LEAD_TRAIN_SEED, beam=64):
-    """Plain witness logs of a beam-sampled 2-(25,4,1) enumeration (depths
-    8-16): rows, meta and labels of the candidates lead_labels keeps."""
+def lead_training_stream(ctx, exclude_rows=None, gate=LEAD_GATE, seed=LEAD_TRAIN_SEED, beam=64, exact=True):
+    """Witness logs of a beam-sampled 2-(25,4,1) enumeration (depths 8-16):
+    rows and meta of the candidates lead_labels keeps (non-canonical, past
+    the gate, no automorphism met), with their exact labels
+    (lead_exact_labels) or, with exact=False, the witness-prefix labels."""
     rows, meta, _ = ctx.enumerate_sampled(25, 4, 1, beam, seed, 8, 16, stride=16, cap=1 << 21)
     if exclude_rows is not None:
         seen = {r.tobytes() for r in exclude_rows}
@@ -125,7 +127,44 @@ def lead_training_stream(ctx, exclude_rows=None, gate=LEAD_GATE, seed=LEAD_TRAIN
     res = ctx.check_batch(padded, meta, fields=("verdict", "perm_row", "counters", "n_gens"))
     y, w = lead_labels(meta, res["verdict"], res["perm_row"], res["counters"], res["n_gens"], gate)
     sel = np.nonzero(w.sum(1) > 0)[0]
-    return rows[sel], meta[sel], y[sel], w[sel]
+    rows, meta = rows[sel], meta[sel]
+    if exact:
+        y, w = lead_exact_labels(ctx, rows, meta, gate)
+    else:
+        y, w = y[sel], w[sel]
+    return rows, meta, y, w
+
+
+def lead_exact_labels(ctx, rows, meta, gate=LEAD_GATE):
+    """Labels of every row of position 0 from the checker's own witnesses: for
+    each row j the lead-order kernel runs with j first (the other rows after
+    it in increasing order), and row j is labelled 1 when the witness it
+    returns starts with j, i.e. when j's subtree holds a witness, else 0.
+    The witness-prefix labels of plain logs (lead_labels) see only the rows
+    up to the plain witness's first row, so they teach how likely a row is
+    right given that every lower row was wrong, and rank the last rows too
+    high; these see every row.  Returns y, w [n][16]."""
+    n, stride = rows.shape
+    d = (meta & 0xff).astype(np.int64)
+    model = RowOrderModel(ctx, max_rows=16, seed=1)
+    model.set_gate(gate)
+    d_rows = ctx.device_array(rows.shape, np.uint64, rows)
+    d_meta = ctx.device_array(meta.shape, np.uint32, meta)
+    d_head = ctx.device_array((256,), np.uint32)
+    dev, res = ctx.device_results(n, ("verdict", "perm_row"))
+    y = np.zeros((n, 16), np.float32)
+    for j in range(16):
+        lead = np.tile(np.arange(16, dtype=np.uint8), (n, 1))
+        lead[:, 0] = j
+        lead[:, 1:] = [[x for x in range(16) if x != j]] * n
+        d_in = ctx.device_array(lead.shape, np.uint8, lead)
+        model.check_lead(n, stride, d_rows, d_meta, res, d_head, d_in)
+        v, pr = dev["verdict"].to_host(), dev["perm_row"].to_host()
+        y[:, j] = ((v == 0) & (pr[:, 0] == j) & (j < d)).astype(np.float32)
+        d_in.free()
+    model.close()
+    w = (np.arange(16)[None, :] < d[:, None]).astype(np.float32)
+    return y, w
 
 
 def train_lead(ctx, rows, meta, y, w, steps=1000, batch=4096, lr=1e-3, seed=1):

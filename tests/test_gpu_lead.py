@@ -190,3 +190,27 @@ def test_trained_lead_network_gain(gpu_ctx, oracle):
     print(f"rows compared: {totals}, plain {int(plain['counters'][:, 1].astype(np.int64).sum())}", flush=True)
     assert totals["trained"] < 0.95 * totals["ascending"]
     assert totals["trained"] < totals["untrained"]
+
+
+def test_exact_labels_match_oracle(gpu_ctx, oracle):
+    """The training labels: row j is 1 exactly when the lead search with j
+    first returns a witness starting with j -- on the GPU as in the oracle."""
+    from ise_amd import learned
+    rows, meta, _ = oracle.enumerate_sampled(25, 4, 1, 8, 0x3131, 8, 16, stride=16, cap=1 << 16)
+    plain = oracle.check_batch(rows, meta)
+    d = (meta & 0xff).astype(int)
+    hard = np.nonzero((plain["verdict"] == 0) & (plain["counters"][:, 0] > learned.LEAD_GATE) &
+                      (plain["n_gens"] == 0) & (d >= 2) & (d <= 16))[0][:400]
+    assert len(hard) > 50
+    r, m = rows[hard], meta[hard]
+    print(f"case exact labels: {len(hard)} candidates", flush=True)
+    y, w = learned.lead_exact_labels(gpu_ctx, r, m)
+    for j in range(16):
+        lead = np.array([[j] + [x for x in range(16) if x != j]] * len(hard), np.uint8)
+        o, _ = oracle.check_batch_lead(r, m, lead, learned.LEAD_GATE)
+        ref = (o["verdict"] == 0) & (o["perm_row"][:, 0] == j) & (j < d[hard])
+        assert np.array_equal(y[:, j] > 0, ref), j
+    assert np.array_equal(w > 0, np.arange(16)[None, :] < d[hard][:, None])
+    # a non-canonical candidate has a right first row (unless every search that would show it
+    # meets an automorphism first and hands over to the plain search)
+    assert ((y * w).sum(1) >= 1).mean() > 0.9

@@ -116,3 +116,20 @@ def test_lead_features_identities(oracle, cases):
         assert np.array_equal(X[c, :d[c], 4:8], X[c, :d[c], 0:4] - X[c, 0, 0:4])
         assert (X[c, :d[c], 15] >= 1).all() and X[c, 0, 8] == 0
         assert not X[c, d[c]:].any()
+
+
+def test_witness_prefix_labels():
+    """Plain logs label the rows up to the witness's first row: that row 1,
+    the lower ones 0, the higher ones unlabelled; only non-canonical
+    candidates past the gate without automorphisms are labelled."""
+    from ise_amd import learned
+    meta = np.array([6 | (10 << 8)] * 4, np.uint32)
+    verdict = np.array([0, 0, 1, 0], np.uint8)
+    perm = np.zeros((4, 32), np.uint8)
+    perm[0, 0], perm[1, 0], perm[3, 0] = 3, 0, 2
+    counters = np.array([[40, 1, 1, 1], [40, 1, 1, 1], [40, 1, 1, 1], [8, 1, 1, 1]], np.uint32)
+    ngen = np.zeros(4, np.uint16)
+    y, w = learned.lead_labels(meta, verdict, perm, counters, ngen, 16)
+    assert list(w[0, :6]) == [1, 1, 1, 1, 0, 0] and list(y[0, :6]) == [0, 0, 0, 1, 0, 0]
+    assert list(w[1, :3]) == [1, 0, 0] and y[1, 0] == 1
+    assert not w[2].any() and not w[3].any()     # canonical; decided within the gate
