@@ -281,7 +281,10 @@ ISE_DEV int lead_search(const Grp<64> &g, const Staged &c, uint64_t *stk, int le
             r = lead_subtree<COUNT>(g, c, stk, d - 1, 1u << (d - 1), 0, false, staged ? 0 : d, left, pr, v,
                                     rrow, ost);
         } else {
-            if (left == 0) return LEAD_LIMIT;
+            if (left == 0) {
+                v.perms += max_perms;
+                return LEAD_LIMIT;
+            }
             left--;
             const uint64_t Rp = g.read64(c.rowv, j0);
             const int cP = popc64(Rp & S0);
@@ -308,6 +311,7 @@ ISE_DEV int lead_search(const Grp<64> &g, const Staged &c, uint64_t *stk, int le
                 v.frow = 0;
                 v.fcol = lim;
                 v.rtc = row_to_change(g, pr, d, 0);
+                v.perms += max_perms - left;
                 return LEAD_WITNESS;
             }
             const uint64_t S1 = S0 & (((R0 >> l) & 1ull) ? Rp : ~Rp);

@@ -81,25 +81,25 @@ __device__ __forceinline__ const bf16x4 *opaque_step(const bf16x4 *p) {
 __device__ __forceinline__ float mlp_tile_streamed(bf16x4 xb, const bf16x4 *frag_g, const float *w, int l) {
     const int q = l >> 4;
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-    bf16x4 h1[8], h2[8];
+    bf16x4 h1[8];
     const bf16x4 *frag = opaque_step(frag_g);
     for (int mt = 0; mt < 8; mt++) {
         const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag[(FRAG_L1 + mt) * 64 + l], xb, zero, 0, 0, 0);
         h1[mt] = relu_bias_bf16(acc, w + MLP_B1 + mt * 16 + 4 * q);
     }
+    // each tile of the second layer's output goes into the third layer at once,
+    // so only one of them is live
+    f32x4 out = zero;
     for (int mt = 0; mt < 8; mt++) {
         frag = opaque_step(frag_g);
         f32x4 acc = zero;
         for (int kt = 0; kt < 8; kt++)
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag[(FRAG_L2 + mt * 8 + kt) * 64 + l], h1[kt], acc, 0, 0,
                                                             0);
-        h2[mt] = relu_bias_bf16(acc, w + MLP_B2 + mt * 16 + 4 * q);
+        const bf16x4 h2 = relu_bias_bf16(acc, w + MLP_B2 + mt * 16 + 4 * q);
+        out = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag[(FRAG_L3 + mt) * 64 + l], h2, out, 0, 0, 0);
     }
-    frag = opaque_step(frag_g);
-    f32x4 acc = zero;
-    for (int kt = 0; kt < 8; kt++)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag[(FRAG_L3 + kt) * 64 + l], h2[kt], acc, 0, 0, 0);
-    return acc[0] + w[MLP_B3];
+    return out[0] + w[MLP_B3];
 }
 
 } // namespace ise

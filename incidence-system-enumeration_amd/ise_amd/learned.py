@@ -18,10 +18,10 @@ from . import (MAX_ROWS, SAVE_ROW_PERMUTATIONS, SAVE_ROW_TO_CHANGE, MLP_FEATURES
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "row_order_weights.npy")
 LEAD_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lead_order_weights.npy")
-# The lead order's training stream: a beam-sampled 2-(25,4,1) enumeration
-# with its own seed (the benchmark's stream uses 0x25041; candidates that
-# occur in both are dropped from training).
-LEAD_TRAIN_SEED = 0x7777
+# The lead order's training stream: beam-sampled 2-(25,4,1) enumerations
+# with seeds of their own (the benchmark's stream uses 0x25041; candidates
+# that occur in both are dropped from training).
+LEAD_TRAIN_SEED = (0x7777, 0x7778)
 LEAD_GATE = 16
 
 # Training streams: the plain enumerator's candidates and seeded random
@@ -113,11 +113,19 @@ def lead_labels(meta, verdict, perm_row, counters, n_gens, gate):
 
 
 def lead_training_stream(ctx, exclude_rows=None, gate=LEAD_GATE, seed=LEAD_TRAIN_SEED, beam=64, exact=True):
-    """Witness logs of a beam-sampled 2-(25,4,1) enumeration (depths 8-16):
+    """Witness logs of beam-sampled 2-(25,4,1) enumerations (depths 8-16; one
+    seed or several):
     rows and meta of the candidates lead_labels keeps (non-canonical, past
     the gate, no automorphism met), with their exact labels
     (lead_exact_labels) or, with exact=False, the witness-prefix labels."""
-    rows, meta, _ = ctx.enumerate_sampled(25, 4, 1, beam, seed, 8, 16, stride=16, cap=1 << 21)
+    seeds = seed if isinstance(seed, (list, tuple)) else (seed,)
+    parts = [ctx.enumerate_sampled(25, 4, 1, beam, sd, 8, 16, stride=16, cap=1 << 21)[:2] for sd in seeds]
+    rows = np.concatenate([p[0] for p in parts])
+    meta = np.concatenate([p[1] for p in parts])
+    # one copy of each candidate
+    _, first = np.unique(rows, axis=0, return_index=True)
+    first = np.sort(first)
+    rows, meta = rows[first], meta[first]
     if exclude_rows is not None:
         seen = {r.tobytes() for r in exclude_rows}
         keep = np.array([r.tobytes() not in seen for r in rows], bool)
@@ -167,7 +175,7 @@ def lead_exact_labels(ctx, rows, meta, gate=LEAD_GATE):
     return y, w
 
 
-def train_lead(ctx, rows, meta, y, w, steps=1000, batch=4096, lr=1e-3, seed=1):
+def train_lead(ctx, rows, meta, y, w, steps=3000, batch=4096, lr=1e-3, seed=1):
     """Train a fresh network on the labelled rows of position 0 with the
     library's MFMA forward, backward and Adam kernels; returns (model, loss
     curve, labelled rows)."""
@@ -303,7 +311,7 @@ def main_lead(argv=None):
                                              "2-(25,4,1) enumeration and compare it with the plain order")
     ap.add_argument("--out", default=LEAD_WEIGHTS)
     ap.add_argument("--report", default=None)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--gate", type=int, default=LEAD_GATE)
@@ -323,7 +331,7 @@ def main_lead(argv=None):
     print(f"labelled rows {M}, loss {curve[:50].mean():.4f} -> {curve[-50:].mean():.4f}, "
           f"training {t2 - t1:.1f}s", flush=True)
     report = {"training": {"candidates": int(len(tr[1])), "labelled_rows": int(M), "steps": args.steps,
-                           "batch": args.batch, "lr": args.lr, "seed": LEAD_TRAIN_SEED, "gate": args.gate,
+                           "batch": args.batch, "lr": args.lr, "seeds": list(LEAD_TRAIN_SEED), "gate": args.gate,
                            "loss_first50": float(curve[:50].mean()), "loss_last50": float(curve[-50:].mean()),
                            "seconds": t2 - t1}}
     report["enumerator"] = compare_lead(ctx, model, rows[:n_enum], meta[:n_enum], args.gate)

@@ -214,3 +214,50 @@ def test_exact_labels_match_oracle(gpu_ctx, oracle):
     # a non-canonical candidate has a right first row (unless every search that would show it
     # meets an automorphism first and hands over to the plain search)
     assert ((y * w).sum(1) >= 1).mean() > 0.9
+
+
+def test_lead_edge_shapes_and_fixed_rows(gpu_ctx, oracle, model):
+    """Shapes the lead order takes and those it hands to the plain search:
+    depths 1..32, 1..64 columns (more than 56: the replayed search), small
+    orbit-form matrices with lenStab > 0, row permutations saved (|Aut| and
+    generators as row permutations); given and network first-row orders;
+    every field and counter against the oracle."""
+    from _helpers import col_sorted, random_orbit_form_batch
+    rng = np.random.default_rng(17)
+    rows, meta = [], []
+    for d, b in [(1, 1), (1, 64), (2, 2), (2, 64), (16, 56), (16, 57), (17, 16), (32, 64), (20, 40), (8, 12),
+                 (16, 16), (5, 50)]:
+        for _ in range(12):
+            mat = col_sorted((rng.random((d, b)) < 0.5).astype(np.uint8))
+            w = np.zeros(32, np.uint64)
+            w[:d] = ise_amd.pack_rows(mat)
+            rows.append(w)
+            meta.append(ise_amd.pack_meta(d, b, 0, int(rng.integers(0, 2)) | 4))
+    _, r2, m2 = random_orbit_form_batch(0x1EAD, 3000, stride=32)
+    rows = np.concatenate([np.array(rows), r2])
+    meta = np.concatenate([np.array(meta, np.uint32), m2])
+    assert ((meta >> 16) & 0xff).astype(bool).sum() > 500
+    n = len(meta)
+    rng = np.random.default_rng(18)
+    model.params = (rng.standard_normal(ise_amd.MLP_PARAMS) * 0.2).astype(np.float32)
+    for gate in (0, 4):
+        model.set_gate(gate)
+        lead = np.stack([rng.permutation(16) for _ in range(n)]).astype(np.uint8)
+        for given in (lead, None):
+            print(f"case lead edges gate {gate} given {given is not None}", flush=True)
+            g, lo, _ = run_lead(gpu_ctx, model, rows, meta, given)
+            use = given if given is not None else lo
+            o, mode = oracle.check_batch_lead(rows, meta, use, gate, gen_cap=GEN_CAP)
+            assert_same("edges", g, o)
+    model.set_gate()
+
+
+def test_lead_invalid_device_records(gpu_ctx, model):
+    """Records that fail validation get ISE_VERDICT_INVALID from the lead
+    kernel too (the device path takes unchecked records)."""
+    rows = np.zeros((4, 16), np.uint64)
+    rows[:, 0] = 1
+    meta = np.array([ise_amd.pack_meta(0, 5), ise_amd.pack_meta(17, 5), ise_amd.pack_meta(3, 65),
+                     ise_amd.pack_meta(3, 5, 3)], np.uint32)
+    g, _, _ = run_lead(gpu_ctx, model, rows, meta)
+    assert (g["verdict"] == ise_amd.VERDICT_INVALID).all()
