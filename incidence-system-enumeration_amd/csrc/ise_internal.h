@@ -76,5 +76,22 @@ void ctx_set_error(ise_ctx *c, const char *msg);
 // Set whether ise_submit reports column permutations; returns the previous setting.
 bool ctx_report_columns(ise_ctx *c, bool on);
 void ctx_get_order(const ise_ctx *c, int *mode, uint32_t *budget, const uint8_t **order, uint32_t *dim);
+// The lead order on every launch of the context (frag null: back to the plain
+// order); cached graphs are dropped.  The network's buffers must outlive it.
+int ctx_set_lead(ise_ctx *c, const uint16_t *frag, const float *w, uint32_t gate);
+// The fragments the context's lead order uses (null when it is off).
+const uint16_t *ctx_lead_frag(const ise_ctx *c);
+// A context's search order, saved and restored around temporary changes.
+struct OrderState {
+    int mode;
+    uint32_t budget;
+    const uint8_t *order;
+    uint32_t dim;
+    const uint16_t *lead_frag;
+    const float *lead_w;
+    uint32_t lead_gate;
+};
+OrderState ctx_save_order(const ise_ctx *c);
+int ctx_restore_order(ise_ctx *c, const OrderState &st);
 
 } // namespace ise

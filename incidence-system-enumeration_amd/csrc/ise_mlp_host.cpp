@@ -191,6 +191,8 @@ int ise_mlp_create(ise_ctx *ctx, ise_mlp **out, uint32_t max_rows, uint64_t seed
 int ise_mlp_destroy(ise_mlp *m) {
     if (!m) return ISE_ERR_ARG;
     if (m->s) (void)hipStreamSynchronize(m->s);
+    // a context still ordering by this network goes back to the plain order
+    if (m->ctx && m->frag && ctx_lead_frag(m->ctx) == m->frag) (void)ctx_set_lead(m->ctx, nullptr, nullptr, 0);
     void *bufs[] = {m->w, m->g, m->m1, m->m2, m->wb, m->frag, m->h1f, m->h2f, m->dh1, m->dh2,
                     m->h1b, m->h2b, m->dz2, m->dz1, m->z3, m->dz3, m->lossv, m->scalar};
     for (void *b : bufs)
@@ -298,10 +300,7 @@ int ise_mlp_check_batch_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, co
         od > ISE_MAX_ROWS)
         return ISE_ERR_ARG;
     ise_ctx *c = m->ctx;
-    int pmode;
-    uint32_t pbudget, pdim;
-    const uint8_t *porder;
-    ctx_get_order(c, &pmode, &pbudget, &porder, &pdim);
+    const OrderState saved = ctx_save_order(c);
     int rc = ise_set_order(c, ISE_ORDER_LEARNED, hunt_budget, d_order, od);
     if (rc) return rc;
     std::vector<hipEvent_t> ev(2 * repeats + 1, nullptr);
@@ -335,7 +334,7 @@ int ise_mlp_check_batch_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, co
     }
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
-    const int rc2 = ise_set_order(c, pmode, pbudget, porder, pdim);
+    const int rc2 = ctx_restore_order(c, saved);
     return rc ? rc : rc2;
 }
 
@@ -387,9 +386,18 @@ int ise_mlp_check_lead_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, con
     return rc;
 }
 
+int ise_mlp_use_lead(ise_mlp *m, int on) {
+    if (!m) return ISE_ERR_ARG;
+    if (on) return ctx_set_lead(m->ctx, m->frag, m->w, m->gate_perms);
+    if (ctx_lead_frag(m->ctx) == m->frag) return ctx_set_lead(m->ctx, nullptr, nullptr, 0);
+    return ISE_OK;
+}
+
 int ise_mlp_set_gate(ise_mlp *m, uint32_t gate_perms) {
     if (!m) return ISE_ERR_ARG;
     m->gate_perms = gate_perms;
+    // a context ordering by this network takes the new gate
+    if (m->ctx && ctx_lead_frag(m->ctx) == m->frag) return ctx_set_lead(m->ctx, m->frag, m->w, gate_perms);
     return ISE_OK;
 }
 

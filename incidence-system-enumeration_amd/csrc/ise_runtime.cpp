@@ -46,10 +46,14 @@ struct ise_ctx {
     uint32_t *d_sched = nullptr;
     size_t sched_cap = 0;
     ise_launch_cfg cfg{0, 4, 0, 1u << 26};
-    // search order (ise_set_order)
+    // search order (ise_set_order; ISE_ORDER_LEAD by ise_mlp_use_lead: the
+    // network's fragments and packed parameters, owned by the ise_mlp)
     int order_mode = ISE_ORDER_PLAIN;
     uint32_t hunt_budget = 0, order_dim = 0;
     const uint8_t *d_order = nullptr;
+    const uint16_t *lead_frag = nullptr;
+    const float *lead_w = nullptr;
+    uint32_t lead_gate = 0;
     // 1-factorization path: QuickPerm tables and per-candidate results
     uint64_t *d_kf_tab = nullptr;
     ise::KfTables kf_tab{};
@@ -157,7 +161,13 @@ int grid_blocks(ise_ctx *c, uint32_t n, uint32_t stride, int group_lanes, const 
     uint32_t b = c->cfg.blocks;
     if (!b) {
         int per_cu = 0;
-        HIPCHK(c, ise::occupancy_blocks(group_lanes, wpb, stride, ise::uses_sets(out, budget), &per_cu));
+        if (c->order_mode == ISE_ORDER_LEAD) {
+            // one candidate per wave
+            group_lanes = 64;
+            HIPCHK(c, ise::lead_occupancy(wpb, stride, &per_cu));
+        } else {
+            HIPCHK(c, ise::occupancy_blocks(group_lanes, wpb, stride, ise::uses_sets(out, budget), &per_cu));
+        }
         if (per_cu < 1) per_cu = 1;
         b = (uint32_t)per_cu * (uint32_t)c->num_cus;
         const uint64_t groups_needed = ((uint64_t)n * group_lanes + 64ull * wpb - 1) / (64ull * wpb);
@@ -171,8 +181,16 @@ int grid_blocks(ise_ctx *c, uint32_t n, uint32_t stride, int group_lanes, const 
 // here, so the sequence can be captured into a graph).
 int launch_with(ise_ctx *c, uint32_t blocks, uint32_t n, uint32_t stride, const uint64_t *rows,
                 const uint64_t *cells, const uint32_t *meta, const ise_results &out, uint32_t *head,
-                hipStream_t s, int group_lanes) {
+                hipStream_t s, int group_lanes, bool allow_lead = true) {
     const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
+    if (c->order_mode == ISE_ORDER_LEAD && allow_lead && !cells && !out.perm_col) {
+        // the learned order of the first row: one launch, deepest-first schedule
+        uint32_t *sched = (c->schedule == ISE_SCHEDULE_DEEPEST_FIRST && n <= c->sched_cap) ? c->d_sched : nullptr;
+        HIPCHK(c, ise::launch_lead(blocks, wpb, n, stride, rows, meta, out, head,
+                                   c->cfg.max_perms ? c->cfg.max_perms : (1u << 26), c->lead_gate, c->lead_frag,
+                                   c->lead_w, nullptr, nullptr, nullptr, sched, s));
+        return ISE_OK;
+    }
     const uint32_t budget = c->order_mode == ISE_ORDER_PLAIN ? 0u : c->hunt_budget;
     const uint8_t *order = c->order_mode == ISE_ORDER_LEARNED ? c->d_order : nullptr;
     uint32_t *defer = (c->defer_perms[0] && n <= c->defer_cap) ? c->d_defer : nullptr;
@@ -235,11 +253,11 @@ int ensure_kf_split(ise_ctx *c, size_t n) {
 
 int launch(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const uint64_t *cells,
            const uint32_t *meta, const ise_results &out, uint32_t *head, hipStream_t s,
-           int group_lanes) {
+           int group_lanes, bool allow_lead = true) {
     uint32_t blocks = 0;
     int rc = grid_blocks(c, n, stride, group_lanes, out, &blocks);
     if (rc) return rc;
-    return launch_with(c, blocks, n, stride, rows, cells, meta, out, head, s, group_lanes);
+    return launch_with(c, blocks, n, stride, rows, cells, meta, out, head, s, group_lanes, allow_lead);
 }
 
 } // namespace
@@ -307,6 +325,29 @@ bool ctx_report_columns(ise_ctx *c, bool on) {
     const bool prev = c->report_columns;
     c->report_columns = on;
     return prev;
+}
+int ctx_set_lead(ise_ctx *c, const uint16_t *frag, const float *w, uint32_t gate) {
+    if (int rc = drop_graphs(c)) return rc;
+    c->order_mode = frag ? ISE_ORDER_LEAD : ISE_ORDER_PLAIN;
+    c->hunt_budget = 0;
+    c->d_order = nullptr;
+    c->order_dim = 0;
+    c->lead_frag = frag;
+    c->lead_w = frag ? w : nullptr;
+    c->lead_gate = frag ? gate : 0;
+    return ISE_OK;
+}
+const uint16_t *ctx_lead_frag(const ise_ctx *c) { return c->order_mode == ISE_ORDER_LEAD ? c->lead_frag : nullptr; }
+OrderState ctx_save_order(const ise_ctx *c) {
+    return OrderState{c->order_mode, c->hunt_budget, c->d_order, c->order_dim, c->lead_frag, c->lead_w,
+                      c->lead_gate};
+}
+int ctx_restore_order(ise_ctx *c, const OrderState &st) {
+    if (st.mode == ISE_ORDER_LEAD) return ctx_set_lead(c, st.lead_frag, st.lead_w, st.lead_gate);
+    c->lead_frag = nullptr;
+    c->lead_w = nullptr;
+    c->lead_gate = 0;
+    return ise_set_order(c, st.mode, st.budget, st.order, st.dim);
 }
 void ctx_get_order(const ise_ctx *c, int *mode, uint32_t *budget, const uint8_t **order, uint32_t *dim) {
     *mode = c->order_mode;
@@ -694,7 +735,8 @@ int ise_canonize_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uin
         HIPCHK(c, hipMemcpyAsync(&open, c->d_canon_count, sizeof open, hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));
         if (open == 0) break;
-        rc = launch(c, open, stride, c->d_canon_rows[cur], nullptr, c->d_canon_meta[cur], out, c->d_head, s, 64);
+        rc = launch(c, open, stride, c->d_canon_rows[cur], nullptr, c->d_canon_meta[cur], out, c->d_head, s, 64,
+                    false);   // the reference's witness order: CanonizeMatrix keeps the plain order
         if (rc) return rc;
         a.first = 0;
         a.n_cur = open;
@@ -786,6 +828,12 @@ int ise_check_batch_device_noalloc(ise_ctx *c, uint32_t n, uint32_t stride, cons
     uint32_t *head = c->noalloc_scratch;
     uint32_t *sched = c->schedule == ISE_SCHEDULE_DEEPEST_FIRST ? c->noalloc_scratch + ISE_QUEUE_WORDS : nullptr;
     const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
+    if (c->order_mode == ISE_ORDER_LEAD && !d_cells && !d_out->perm_col) {
+        HIPCHK(c, ise::launch_lead(blocks, wpb, n, stride, d_rows, d_meta, *d_out, head,
+                                   c->cfg.max_perms ? c->cfg.max_perms : (1u << 26), c->lead_gate, c->lead_frag,
+                                   c->lead_w, nullptr, nullptr, nullptr, sched, s));
+        return ISE_OK;
+    }
     const uint32_t budget = c->order_mode == ISE_ORDER_PLAIN ? 0u : c->hunt_budget;
     const uint8_t *order = c->order_mode == ISE_ORDER_LEARNED ? c->d_order : nullptr;
     HIPCHK(c, ise::launch_check(group, blocks, wpb, n, stride, d_rows, d_cells, d_meta, *d_out, head,

@@ -43,6 +43,7 @@ DEFAULT_KF_MAX_RANGES = 1024   # ISE_DEFAULT_KF_MAX_RANGES
 ORDER_PLAIN = 0     # the reference's search alone
 ORDER_HUNT = 1      # budgeted witness hunt in increasing row order, then the plain search
 ORDER_LEARNED = 2   # budgeted witness hunt in a per-candidate order table, then the plain search
+ORDER_LEAD = 3      # the learned order of the first row (RowOrderModel.use_lead)
 
 QUEUE_WORDS = 256   # ISE_QUEUE_WORDS: device words behind a work-queue argument
 NOALLOC_MAX_CANDIDATES = 1 << 20   # ISE_NOALLOC_MAX_CANDIDATES
@@ -129,6 +130,7 @@ def lib():
         L.ise_mlp_witness_labels.argtypes = [U32, U32, P, P, P, P, P, P]
         L.ise_mlp_set_gate.argtypes = [P, U32]
         L.ise_mlp_lead_features.argtypes = [P, U32, U32, P, P, P]
+        L.ise_mlp_use_lead.argtypes = [P, I]
         L.ise_mlp_check_lead_device.argtypes = [P, U32, U32, P, P, ctypes.POINTER(ResultsStruct), P, P, P, P]
         L.ise_mlp_check_lead_device_timed.argtypes = [P, U32, U32, P, P, ctypes.POINTER(ResultsStruct), P, U32, P]
         L.ise_group_closure.argtypes = [P, U32, U32, P, U64, ctypes.POINTER(U64)]
@@ -654,6 +656,12 @@ class RowOrderModel:
         """Order only the candidates the plain search leaves open after
         gate_perms permutations (0: order every candidate)."""
         self.ctx._check(self._L.ise_mlp_set_gate(self.h, gate_perms))
+
+    def use_lead(self, on=True):
+        """The context's launches (submit/collect, device batches, the
+        enumerators) run the lead order with this network, or, with on=False,
+        the plain order again."""
+        self.ctx._check(self._L.ise_mlp_use_lead(self.h, 1 if on else 0))
 
     def lead_features(self, n, stride, d_rows, d_meta, d_X):
         """Features of every row as the row of position 0: d_X [n][16][16] bf16 bits."""

@@ -115,10 +115,18 @@ typedef struct {
  *                     learned scorer (ise_mlp_order).
  * A hunt witness is reported in place of the plain one (verdict, fail_row,
  * rowToChange and the witness permutation refer to it); canonical verdicts,
- * |Aut| and generators always come from the plain search. */
+ * |Aut| and generators always come from the plain search. *  ISE_ORDER_LEAD     (ise_mlp_use_lead) the learned order of the first row:
+ *                     what ise_mlp_check_lead_device does, on every launch of
+ *                     the context that reports no column permutation and
+ *                     takes no cell masks (those run the plain search, as
+ *                     does the canonical-form loop, ise_canonize_*); the
+ *                     witness is the lead order's, every canonical
+ *                     candidate's results are the plain search's.
+ */
 #define ISE_ORDER_PLAIN   0
 #define ISE_ORDER_HUNT    1
 #define ISE_ORDER_LEARNED 2
+#define ISE_ORDER_LEAD    3   /* set by ise_mlp_use_lead, not by ise_set_order */
 int ise_set_order(ise_ctx *ctx, int mode, uint32_t hunt_budget, const uint8_t *d_order,
                   uint32_t order_dim);
 
@@ -417,6 +425,15 @@ int ise_mlp_check_lead_device(ise_mlp *m, uint32_t n, uint32_t stride, const uin
 int ise_mlp_check_lead_device_timed(ise_mlp *m, uint32_t n, uint32_t stride, const uint64_t *d_rows,
                                     const uint32_t *d_meta, const ise_results *d_out, uint32_t *d_counter,
                                     uint32_t repeats, float *ms_out);
+
+/* The lead order (ISE_ORDER_LEAD) with this network on every launch of its
+ * context (on != 0), or back to the plain order (on == 0, if the context uses
+ * this network).  The context keeps the network's device buffers: changed
+ * parameters and ise_mlp_set_gate apply to its next launches, and
+ * ise_mlp_destroy returns the context to the plain order.  Cached graphs of
+ * the context are dropped.  Lets the enumerators (ise_enumerate_bibd*) and
+ * ise_submit / ise_collect run the learned order. */
+int ise_mlp_use_lead(ise_mlp *m, int on);
 
 /* Permutations of plain search after which a candidate of a learned-order
  * batch is ordered by the network (0: every candidate is). */

@@ -261,3 +261,66 @@ def test_lead_invalid_device_records(gpu_ctx, model):
                      ise_amd.pack_meta(3, 5, 3)], np.uint32)
     g, _, _ = run_lead(gpu_ctx, model, rows, meta)
     assert (g["verdict"] == ise_amd.VERDICT_INVALID).all()
+
+
+@pytest.mark.parametrize("v,k,lam", [(13, 3, 1), (15, 7, 3), (16, 6, 2), (15, 3, 1)])
+def test_lead_mode_enumerations(oracle, v, k, lam):
+    """The enumerators with the context in the lead order (ise_mlp_use_lead):
+    host-solver, graph-replayed and device-resident enumerations give the
+    plain order's design lists, |Aut| and candidate counts."""
+    from ise_amd import learned
+    print(f"case lead-order enumeration {(v, k, lam)}", flush=True)
+    with ise_amd.Context(capacity=1 << 16, stride=32) as ctx:
+        des, auts, st, lv = ctx.enumerate_bibd(v, k, lam)
+        m = learned.load(ctx, learned.LEAD_WEIGHTS)
+        m.set_gate(learned.LEAD_GATE)
+        m.use_lead(True)
+        des2, auts2, st2, lv2 = ctx.enumerate_bibd(v, k, lam)
+        ctx.use_graphs(True)
+        des3, auts3, _, _ = ctx.enumerate_bibd(v, k, lam)
+        ctx.use_graphs(False)
+        des4, auts4, _, _ = ctx.enumerate_bibd_device(v, k, lam)
+        m.close()
+    key = lambda d, a: sorted(zip(map(tuple, d.tolist()), map(int, a)))
+    ref = key(des, auts)
+    assert key(des2, auts2) == ref and key(des3, auts3) == ref and key(des4, auts4) == ref
+    assert st2["candidates"] == st["candidates"] and np.array_equal(lv2, lv)
+
+
+def test_lead_mode_submit_collect_and_noalloc(gpu_ctx, oracle):
+    """ise_submit/ise_collect and the allocation-free device entry in the lead
+    order give the lead device entry point's results; with column
+    permutations asked for, the plain search's; after the network is
+    destroyed, the plain order again."""
+    from ise_amd import learned
+    rows, meta, _ = oracle.enumerate_sampled(25, 4, 1, 8, 0x4141, 8, 16, stride=32, cap=1 << 16)
+    r2, m2 = ise_amd.random_extensions(15, 3, 1, 2000, 4, 15, 81, stride=32)
+    rows, meta = np.concatenate([rows[:6000], r2]), np.concatenate([meta[:6000], m2])
+    n = len(meta)
+    print(f"case lead mode submit/collect {n}", flush=True)
+    fields = tuple(f for f in FIELDS if f != "counters")
+    m = learned.load(gpu_ctx, learned.LEAD_WEIGHTS)
+    m.set_gate(learned.LEAD_GATE)
+    ref, _, _ = run_lead(gpu_ctx, m, rows, meta, counted=False)
+    plain = gpu_ctx.check_batch(rows, meta, gen_cap=GEN_CAP)
+    try:
+        m.use_lead(True)
+        got = gpu_ctx.check_batch(rows, meta, gen_cap=GEN_CAP, fields=fields + ("gens",))
+        assert_same("submit/collect", got, {k: v for k, v in ref.items() if k != "counters"})
+        # the allocation-free entry point
+        dev, s = gpu_ctx.device_results(n, fields + ("gens",), gen_cap=GEN_CAP)
+        d_rows = gpu_ctx.device_array(rows.shape, np.uint64, rows)
+        d_meta = gpu_ctx.device_array(meta.shape, np.uint32, meta)
+        gpu_ctx.check_batch_device_noalloc(n, 32, d_rows, d_meta, s)
+        gpu_ctx.sync()
+        assert_same("noalloc", {k: v.to_host() for k, v in dev.items()}, ref)
+        # column permutations asked for: the plain search, every field
+        cols = gpu_ctx.check_batch(rows, meta, gen_cap=GEN_CAP)
+        for f in plain:
+            assert np.array_equal(cols[f], plain[f]), f
+    finally:
+        m.close()
+    after = gpu_ctx.check_batch(rows, meta, gen_cap=GEN_CAP, fields=fields + ("gens",))
+    for f in after:
+        assert np.array_equal(after[f], plain[f]), f
+    assert (ref["verdict"] == plain["verdict"]).all()
