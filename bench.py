@@ -373,11 +373,24 @@ def enumeration_rate(cpu=True):
             ctx.use_graphs(False)
             if not np.array_equal(des, des3):
                 raise RuntimeError("the enumeration with graphs differs")
+            # the same in the learned order (ise_mlp_use_lead): the same design list
+            from ise_amd import learned
+            lm = learned.load(ctx, learned.LEAD_WEIGHTS)
+            lm.set_gate(learned.LEAD_GATE)
+            lm.use_lead(True)
+            ctx.enumerate_bibd(v, k, lam)
+            t0 = time.perf_counter()
+            des5, _, _, _ = ctx.enumerate_bibd(v, k, lam)
+            dt5 = time.perf_counter() - t0
+            lm.close()
+            if sorted(map(tuple, des5.tolist())) != sorted(map(tuple, des.tolist())):
+                raise RuntimeError("the enumeration in the learned order differs")
             rec = {"designs": len(des), "seconds": dt, "designs_per_s": len(des) / dt,
                    "candidates_per_s": st["candidates"] / dt,
                    "device_resident_seconds": dt2,
                    "device_resident_designs_per_s": len(des) / dt2,
                    "graphs_seconds": dt3, "graphs_designs_per_s": len(des) / dt3,
+                   "learned_order_seconds": dt5, "learned_order_designs_per_s": len(des) / dt5,
                    "graph_replays": int(g1[1] - g0[1]), "graphs_cached": int(g1[0])}
             if orc is not None:
                 t0 = time.perf_counter()
