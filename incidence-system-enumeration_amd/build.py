@@ -18,7 +18,7 @@ SOURCES = ["ise_kernels.hip", "ise_kfact.hip", "ise_mlp.hip", "ise_runtime.cpp",
            "ise_kfact_host.cpp", "ise_mlp_host.cpp", "ise_group.hip",
            "ise_format.cpp", "ise_solver.hip", "ise_noalloc.hip",
            "ise_canon.hip", "ise_solution.cpp",
-           "ise_kf_backjump.hip", "ise_random.hip"]
+           "ise_kf_backjump.hip", "ise_random.hip", "ise_cdtools.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -14,8 +14,13 @@ namespace ise {
 // matrices go to canon / status / aut / iters at their original index, open
 // ones are rearranged by their witness and appended to the next compact batch
 // (rows_next / meta_next / idx_next, count_next).  `iter` = tests run so far.
+// part != 0: an initial column partition (bit c set: column c starts a block,
+// the orbits of a first row of labels); columns stay inside their block and
+// cells_next receives every row's orbit-start mask for the next test.
 struct CanonStep {
     uint32_t first, n_cur, stride, stride_in, stride_out, iter, max_iters;
+    uint64_t part;
+    uint64_t *cells_next;
     const uint64_t *rows_in;
     const uint32_t *meta_in;
     const uint64_t *rows_cur;

@@ -16,7 +16,12 @@
 // its d bits with row 0 most significant; the columns in decreasing key order
 // are the orbit form (ones first inside every orbit of every row).  A column's
 // new position is the number of larger keys plus the equal keys before it;
-// ds_permute moves every key there and d ballots rebuild the rows.
+// ds_permute moves every key there and d ballots rebuild the rows.  With an
+// initial column partition (the k-Sys form, whose first row of day labels
+// makes every day a column orbit, MatrixCanonChecker.cpp:437-449) a column is
+// ranked among the columns of its own block only, and lane i writes the
+// orbit-start mask row i is compared under: the block starts plus every place
+// an earlier row changes value.
 #include <hip/hip_runtime.h>
 #include "ise_canon.h"
 
@@ -67,13 +72,19 @@ __global__ __launch_bounds__(256) void canon_step_kernel(CanonStep a) {
                                 << 32);
             key = (key << 1) | (uint32_t)((r >> lane) & 1ull);
         }
-        // decreasing key order, ties by column: lanes past b keep their place
+        // decreasing key order inside each block, ties by column: lanes past b keep their place
+        const uint64_t bmask = b >= 64 ? ~0ull : ((1ull << b) - 1ull);
+        const uint64_t part = (a.part | 1ull) & bmask;
         uint32_t pos = lane;
         if (lane < b) {
-            pos = 0;
+            const uint64_t upto = lane >= 63 ? part : part & ((2ull << lane) - 1ull);
+            const uint32_t mine = 63u - (uint32_t)__builtin_clzll(upto);   // first column of my block
+            pos = mine;
+            uint32_t cs = 0;
             for (uint32_t c = 0; c < b; c++) {
+                if ((part >> c) & 1ull) cs = c;
                 const uint32_t kc = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)c);
-                pos += (kc > key) || (kc == key && c < lane);
+                pos += cs == mine && ((kc > key) || (kc == key && c < lane));
             }
         }
         const uint32_t sorted = (uint32_t)__builtin_amdgcn_ds_permute((int)(pos << 2), (int)key);
@@ -86,6 +97,16 @@ __global__ __launch_bounds__(256) void canon_step_kernel(CanonStep a) {
         if (lane == 0) slot = atomicAdd(a.count_next, 1u);
         slot = (uint32_t)__builtin_amdgcn_readfirstlane((int)slot);
         if (lane < a.stride) a.rows_next[(size_t)slot * a.stride + lane] = lane < d ? out : 0ull;
+        if (a.cells_next) {
+            uint64_t cm = part;
+            for (uint32_t j = 0; j < d; j++) {
+                const uint64_t r = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)out, (int)j) |
+                                   ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(out >> 32), (int)j)
+                                    << 32);
+                if (j < lane) cm |= r ^ (r << 1);
+            }
+            if (lane < a.stride) a.cells_next[(size_t)slot * a.stride + lane] = lane < d ? (cm & bmask) : 0ull;
+        }
         if (lane == 0) {
             a.meta_next[slot] = (m & 0x00ffffffu) | (ISE_SAVE_ROW_PERMUTATIONS << 24);
             a.idx_next[slot] = orig;

@@ -1,0 +1,407 @@
+// ise_cdtools.hip -- the k-Sys binary canonizer on the GPU (createCanonizer /
+// runCanonizer / releaseCanonizer, Source/CDTools.cpp:5-51).
+//
+// runCanonizer builds the binary matrix of a k-system (C_InSys::
+// convertToBinaryMatrix, Source/matrix.h:198-210) under a first row of day
+// labels and calls CMatrixCanonChecker::CanonizeMatrix(k, NULL, numClasses)
+// (MatrixCanonChecker.cpp:420-564).  The label row makes every day's columns
+// a column orbit; here it is the initial column partition of the batched
+// canonical-form loop (ise_canonize_batch_device_part), so the matrices hold
+// the v player rows only.  With every day present the reference also walks
+// the day permutations, canonizing each and keeping the largest matrix
+// (the walk starts again whenever the best matrix grows, so it ends on the
+// largest one over all day orders); here the days! day orders of a k-system
+// are canonized in one batch and a reduction keeps the largest.
+//
+// Per chunk of candidates (one candidate = a k-system in one day order):
+//   cd_expand_kernel        one wave per candidate: lane c builds column c
+//                           (day j = c / (v/k), group c % (v/k) of the day in
+//                           position j) as a mask of players, and v ballots
+//                           give the packed rows;
+//   canonical-form loop     ise_canonize_batch_device_part, day partition;
+//   cd_segment_best_kernel  one workgroup per run of <= CD_SEG candidates of
+//                           one k-system: the largest canonical matrix (rows
+//                           compared in order, each with column 0 first);
+//   cd_merge_kernel         one thread per k-system: the run winners against
+//                           the k-system's best so far (kept across chunks).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <algorithm>
+#include <string>
+#include <vector>
+#include "ise.h"
+#include "ise_internal.h"
+
+struct ise_cd_canonizer {
+    ise_ctx *ctx = nullptr;
+    int device = 0;
+    uint32_t v = 0, k = 0, days_max = 0, groups = 0, chunk = 0;
+    uint8_t *d_mats = nullptr;       // host batches staged here
+    size_t mats_cap = 0;             // bytes
+    uint64_t *d_rows = nullptr, *d_canon = nullptr;
+    uint32_t *d_meta = nullptr;
+    uint8_t *d_status = nullptr;
+    uint32_t *d_seg = nullptr;       // [nseg][3]: first candidate (in the chunk), length, winner
+    size_t seg_cap = 0;
+    uint32_t *d_mseg = nullptr;      // [nm][3]: k-system, first run, runs
+    size_t mseg_cap = 0;
+    uint64_t *d_best = nullptr;      // [n][v] best matrix of every k-system of a host batch
+    size_t best_cap = 0;             // k-systems
+    uint32_t *d_have = nullptr;      // [n] a best matrix is set
+    size_t have_cap = 0;
+    uint32_t *d_flags = nullptr;     // [0] a player id >= v, [1] a canonization did not end
+    std::vector<uint64_t> packed;
+    std::vector<uint8_t> out;
+    std::string err;
+};
+
+namespace {
+
+constexpr uint32_t CD_CHUNK = 1u << 16;   // candidates per canonical-form batch
+constexpr uint32_t CD_SEG = 1024u;        // candidates per reduction workgroup
+constexpr uint32_t CD_NONE = 0xffffffffu;
+
+// x > y for two v-row matrices (rows in order, each with column 0 first).
+__device__ inline bool cd_larger(const uint64_t *x, const uint64_t *y, uint32_t v) {
+    for (uint32_t i = 0; i < v; i++) {
+        const uint64_t a = x[i], b = y[i];
+        if (a != b) {
+            const uint64_t diff = a ^ b;
+            return (a & (diff & (0ull - diff))) != 0ull;
+        }
+    }
+    return false;
+}
+
+// Candidate c0 + w of the chunk: k-system c / P in day order c % P (the
+// factorial number system, most significant position first; P = 1: the
+// given order).  Day positions are packed 4 bits each (days <= 10 for a walk).
+__global__ __launch_bounds__(256) void cd_expand_kernel(const uint8_t *mats, uint32_t v, uint32_t k, uint32_t days,
+                                                        uint32_t walk, uint64_t P, uint64_t c0, uint32_t cnt,
+                                                        uint32_t stride, uint64_t *rows, uint32_t *meta,
+                                                        uint32_t *flags) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= cnt) return;
+    const uint64_t c = c0 + w;
+    const uint64_t mat = c / P;
+    uint32_t t = (uint32_t)(c - mat * P);
+    uint64_t order = 0;
+    if (walk) {
+        uint32_t used = 0, f = 1;
+        for (uint32_t i = 2; i < days; i++) f *= i;          // (days - 1)!
+        for (uint32_t i = 0; i < days; i++) {
+            const uint32_t q = t / f;
+            t -= q * f;
+            uint32_t pick = 0, seen = 0;
+            for (uint32_t dd = 0; dd < days; dd++) {
+                if ((used >> dd) & 1u) continue;
+                if (seen == q) { pick = dd; break; }
+                seen++;
+            }
+            used |= 1u << pick;
+            order |= (uint64_t)pick << (4 * i);
+            if (days - 1 - i) f /= days - 1 - i;
+        }
+    }
+    const uint32_t G = v / k, b = days * G;
+    uint32_t col = 0;
+    if (lane < b) {
+        const uint32_t j = lane / G, g = lane - j * G;
+        const uint32_t od = walk ? (uint32_t)((order >> (4 * j)) & 15u) : j;
+        const uint8_t *grp = mats + (mat * days + od) * v + (uint64_t)g * k;
+        for (uint32_t n = 0; n < k; n++) {
+            const uint32_t p = grp[n];
+            if (p < v) col |= 1u << p;
+            else atomicOr(&flags[0], 1u);
+        }
+    }
+    uint64_t word = 0;
+    for (uint32_t r = 0; r < v; r++) {
+        const uint64_t bw = __builtin_amdgcn_ballot_w64(((col >> r) & 1u) != 0u);
+        if (lane == r) word = bw;
+    }
+    if (lane < stride) rows[(uint64_t)w * stride + lane] = lane < v ? word : 0ull;
+    if (lane == 0) meta[w] = v | (b << 8);
+}
+
+__global__ __launch_bounds__(256) void cd_segment_best_kernel(const uint64_t *canon, const uint8_t *status,
+                                                              uint32_t stride, uint32_t v, uint32_t *seg,
+                                                              uint32_t *flags) {
+    __shared__ uint32_t best[256];
+    const uint32_t s = blockIdx.x, start = seg[3 * s], len = seg[3 * s + 1];
+    uint32_t mine = CD_NONE;
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+        const uint32_t c = start + i;
+        if (status[c] != ISE_VERDICT_CANONICAL) {
+            atomicOr(&flags[1], 1u);
+            continue;
+        }
+        if (mine == CD_NONE || cd_larger(canon + (uint64_t)c * stride, canon + (uint64_t)mine * stride, v)) mine = c;
+    }
+    best[threadIdx.x] = mine;
+    __syncthreads();
+    for (uint32_t h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            const uint32_t o = best[threadIdx.x + h], m = best[threadIdx.x];
+            if (o != CD_NONE &&
+                (m == CD_NONE || cd_larger(canon + (uint64_t)o * stride, canon + (uint64_t)m * stride, v)))
+                best[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) seg[3 * s + 2] = best[0];
+}
+
+__global__ void cd_merge_kernel(const uint64_t *canon, uint32_t stride, uint32_t v, const uint32_t *seg,
+                                const uint32_t *mseg, uint32_t nm, uint64_t *best, uint32_t *have) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nm) return;
+    const uint32_t mat = mseg[3 * e], s0 = mseg[3 * e + 1], ns = mseg[3 * e + 2];
+    uint64_t *bp = best + (uint64_t)mat * v;
+    bool h = have[mat] != 0u;
+    for (uint32_t s = s0; s < s0 + ns; s++) {
+        const uint32_t c = seg[3 * s + 2];
+        if (c == CD_NONE) continue;
+        const uint64_t *cp = canon + (uint64_t)c * stride;
+        if (!h || cd_larger(cp, bp, v)) {
+            for (uint32_t i = 0; i < v; i++) bp[i] = cp[i];
+            h = true;
+        }
+    }
+    have[mat] = h ? 1u : 0u;
+}
+
+// Without a walk every candidate is its own k-system's result.
+__global__ void cd_copy_kernel(const uint64_t *canon, const uint8_t *status, uint32_t stride, uint32_t v,
+                               uint64_t m0, uint32_t cnt, uint64_t *best, uint32_t *flags) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)cnt * v) return;
+    const uint64_t c = i / v, w = i - c * v;
+    best[(m0 + c) * v + w] = canon[c * stride + w];
+    if (w == 0 && status[c] != ISE_VERDICT_CANONICAL) atomicOr(&flags[1], 1u);
+}
+
+int fail(ise_cd_canonizer *h, int rc, const std::string &msg) {
+    h->err = msg;
+    return rc;
+}
+
+int hip_fail(ise_cd_canonizer *h, hipError_t e, const char *where) {
+    h->err = std::string(where) + ": " + hipGetErrorString(e);
+    return ISE_ERR_HIP;
+}
+
+#define CDCHK(h, call)                                      \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #call); \
+    } while (0)
+
+template <typename T> hipError_t grow(T **p, size_t *cap, size_t need) {
+    if (*cap >= need) return hipSuccess;
+    if (*p) {
+        hipError_t e = hipFree(*p);
+        if (e != hipSuccess) return e;
+        *p = nullptr;
+        *cap = 0;
+    }
+    hipError_t e = hipMalloc((void **)p, need * sizeof(T));
+    if (e == hipSuccess) *cap = need;
+    return e;
+}
+
+uint64_t factorial(uint32_t n) {
+    uint64_t f = 1;
+    for (uint32_t i = 2; i <= n; i++) f *= i;
+    return f;
+}
+
+// Days of a call: `day` <= 0 means all of them.
+int resolve_days(ise_cd_canonizer *h, int day, uint32_t *days, uint32_t *b, bool *walk) {
+    const uint32_t d = day <= 0 ? h->days_max : (uint32_t)day;
+    if (d > h->days_max) return fail(h, ISE_ERR_ARG, "runCanonizer: more days than the k-system has");
+    const uint32_t cols = d * h->groups;
+    if (cols > ISE_MAX_COLS) return fail(h, ISE_ERR_ARG, "runCanonizer: more than 64 columns");
+    *walk = d == h->days_max;
+    if (*walk && d > ISE_CD_MAX_WALK_DAYS)
+        return fail(h, ISE_ERR_ARG, "runCanonizer: too many days for the walk over day orders");
+    *days = d;
+    *b = cols;
+    return ISE_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int ise_cd_create_canonizer(ise_cd_canonizer **out, int device, uint32_t v, uint32_t k) {
+    if (!out) return ISE_ERR_ARG;
+    *out = nullptr;
+    if (k < 2 || v < k || v % k || (v - 1) % (k - 1) || v > ISE_MAX_ROWS) return ISE_ERR_ARG;
+    auto *h = new ise_cd_canonizer();
+    h->device = device;
+    h->v = v;
+    h->k = k;
+    h->days_max = (v - 1) / (k - 1);
+    h->groups = v / k;
+    h->chunk = CD_CHUNK;
+    int rc = ise_open(&h->ctx, device, h->chunk, v);
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    hipError_t e = hipSetDevice(device);
+    const size_t words = (size_t)h->chunk * v;
+    if (e == hipSuccess) e = hipMalloc((void **)&h->d_rows, words * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&h->d_canon, words * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&h->d_meta, (size_t)h->chunk * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&h->d_status, h->chunk);
+    if (e == hipSuccess) e = hipMalloc((void **)&h->d_flags, 64);
+    if (e != hipSuccess) {
+        ise_cd_release_canonizer(h);
+        return ISE_ERR_HIP;
+    }
+    *out = h;
+    return ISE_OK;
+}
+
+int ise_cd_release_canonizer(ise_cd_canonizer *h) {
+    if (!h) return ISE_ERR_ARG;
+    (void)hipSetDevice(h->device);
+    if (h->ctx) (void)hipStreamSynchronize(ise::ctx_stream(h->ctx));
+    void *bufs[] = {h->d_mats, h->d_rows, h->d_canon, h->d_meta, h->d_status,
+                    h->d_seg,  h->d_mseg, h->d_best,  h->d_have, h->d_flags};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (h->ctx) ise_close(h->ctx);
+    delete h;
+    return ISE_OK;
+}
+
+const char *ise_cd_error(const ise_cd_canonizer *h) {
+    return h ? h->err.c_str() : "null canonizer";
+}
+
+int ise_cd_run_canonizer_device(ise_cd_canonizer *h, uint32_t n, const uint8_t *d_matrices, int day,
+                                uint64_t *d_canon) {
+    if (!h) return ISE_ERR_ARG;
+    if (n == 0) return ISE_OK;
+    if (!d_matrices || !d_canon) return fail(h, ISE_ERR_ARG, "runCanonizer: null buffer");
+    uint32_t days = 0, b = 0;
+    bool walk = false;
+    int rc = resolve_days(h, day, &days, &b, &walk);
+    if (rc) return rc;
+    const uint32_t v = h->v, stride = v;
+    const uint64_t P = walk ? factorial(days) : 1ull, total = (uint64_t)n * P;
+    uint64_t part = 0;
+    for (uint32_t j = 0; j < days; j++) part |= 1ull << (j * h->groups);
+    CDCHK(h, hipSetDevice(h->device));
+    hipStream_t s = ise::ctx_stream(h->ctx);
+    CDCHK(h, grow(&h->d_have, &h->have_cap, (size_t)n));
+    CDCHK(h, hipMemsetAsync(h->d_have, 0, (size_t)n * 4, s));
+    CDCHK(h, hipMemsetAsync(h->d_flags, 0, 64, s));
+    std::vector<uint32_t> seg, mseg;
+    for (uint64_t c0 = 0; c0 < total; c0 += h->chunk) {
+        const uint32_t cnt = (uint32_t)std::min<uint64_t>(h->chunk, total - c0);
+        hipLaunchKernelGGL(cd_expand_kernel, dim3((cnt + 3) / 4), dim3(256), 0, s, d_matrices, v, h->k, days,
+                           walk ? 1u : 0u, P, c0, cnt, stride, h->d_rows, h->d_meta, h->d_flags);
+        CDCHK(h, hipGetLastError());
+        rc = ise_canonize_batch_device_part(h->ctx, cnt, stride, h->d_rows, h->d_meta, part, h->d_canon,
+                                            h->d_status, nullptr, nullptr, ISE_CANONIZE_MAX_ITERS);
+        if (rc) return fail(h, rc, std::string("runCanonizer: ") + ise_last_error(h->ctx));
+        if (!walk) {
+            const uint64_t items = (uint64_t)cnt * v;
+            hipLaunchKernelGGL(cd_copy_kernel, dim3((uint32_t)((items + 255) / 256)), dim3(256), 0, s, h->d_canon,
+                               h->d_status, stride, v, c0, cnt, d_canon, h->d_flags);
+            CDCHK(h, hipGetLastError());
+            continue;
+        }
+        // runs of <= CD_SEG candidates of one k-system, and every k-system's runs
+        seg.clear();
+        mseg.clear();
+        const uint64_t m_first = c0 / P, m_last = (c0 + cnt - 1) / P;
+        for (uint64_t m = m_first; m <= m_last; m++) {
+            const uint64_t lo = std::max<uint64_t>(c0, m * P), hi = std::min<uint64_t>(c0 + cnt, (m + 1) * P);
+            mseg.push_back((uint32_t)m);
+            mseg.push_back((uint32_t)(seg.size() / 3));
+            uint32_t runs = 0;
+            for (uint64_t a = lo; a < hi; a += CD_SEG, runs++) {
+                seg.push_back((uint32_t)(a - c0));
+                seg.push_back((uint32_t)std::min<uint64_t>(CD_SEG, hi - a));
+                seg.push_back(CD_NONE);
+            }
+            mseg.push_back(runs);
+        }
+        const uint32_t nseg = (uint32_t)(seg.size() / 3), nm = (uint32_t)(mseg.size() / 3);
+        CDCHK(h, grow(&h->d_seg, &h->seg_cap, seg.size()));
+        CDCHK(h, grow(&h->d_mseg, &h->mseg_cap, mseg.size()));
+        CDCHK(h, hipMemcpyAsync(h->d_seg, seg.data(), seg.size() * 4, hipMemcpyHostToDevice, s));
+        CDCHK(h, hipMemcpyAsync(h->d_mseg, mseg.data(), mseg.size() * 4, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(cd_segment_best_kernel, dim3(nseg), dim3(256), 0, s, h->d_canon, h->d_status, stride, v,
+                           h->d_seg, h->d_flags);
+        CDCHK(h, hipGetLastError());
+        hipLaunchKernelGGL(cd_merge_kernel, dim3((nm + 63) / 64), dim3(64), 0, s, h->d_canon, stride, v, h->d_seg,
+                           h->d_mseg, nm, d_canon, h->d_have);
+        CDCHK(h, hipGetLastError());
+        // the host run lists are rewritten next chunk: their copies must be done
+        CDCHK(h, hipStreamSynchronize(s));
+    }
+    uint32_t flags[2] = {0, 0};
+    CDCHK(h, hipMemcpyAsync(flags, h->d_flags, sizeof flags, hipMemcpyDeviceToHost, s));
+    CDCHK(h, hipStreamSynchronize(s));
+    if (flags[0]) return fail(h, ISE_ERR_ARG, "runCanonizer: a player number is not below v");
+    if (flags[1]) return fail(h, ISE_ERR_STATE, "runCanonizer: a canonization did not reach its canonical form");
+    return ISE_OK;
+}
+
+int ise_cd_run_canonizer_batch(ise_cd_canonizer *h, uint32_t n, const uint8_t *matrices, int day, uint8_t *out) {
+    if (!h) return ISE_ERR_ARG;
+    if (n == 0) return ISE_OK;
+    if (!matrices || !out) return fail(h, ISE_ERR_ARG, "runCanonizer: null buffer");
+    uint32_t days = 0, b = 0;
+    bool walk = false;
+    int rc = resolve_days(h, day, &days, &b, &walk);
+    if (rc) return rc;
+    const uint32_t v = h->v;
+    const size_t in_bytes = (size_t)n * days * v;
+    for (size_t i = 0; i < in_bytes; i++)
+        if (matrices[i] >= v) return fail(h, ISE_ERR_ARG, "runCanonizer: a player number is not below v");
+    CDCHK(h, hipSetDevice(h->device));
+    hipStream_t s = ise::ctx_stream(h->ctx);
+    CDCHK(h, grow(&h->d_mats, &h->mats_cap, in_bytes));
+    CDCHK(h, grow(&h->d_best, &h->best_cap, (size_t)n * v));
+    CDCHK(h, hipMemcpyAsync(h->d_mats, matrices, in_bytes, hipMemcpyHostToDevice, s));
+    rc = ise_cd_run_canonizer_device(h, n, h->d_mats, day, h->d_best);
+    if (rc) return rc;
+    h->packed.resize((size_t)n * v);
+    CDCHK(h, hipMemcpyAsync(h->packed.data(), h->d_best, h->packed.size() * 8, hipMemcpyDeviceToHost, s));
+    CDCHK(h, hipStreamSynchronize(s));
+    for (uint32_t m = 0; m < n; m++)
+        for (uint32_t i = 0; i < v; i++) {
+            const uint64_t r = h->packed[(size_t)m * v + i];
+            uint8_t *o = out + ((size_t)m * v + i) * b;
+            for (uint32_t q = 0; q < b; q++) o[q] = (uint8_t)((r >> q) & 1ull);
+        }
+    return ISE_OK;
+}
+
+int ise_cd_run_canonizer(ise_cd_canonizer *h, const uint8_t *matrix, uint32_t k, int day, const uint8_t **out,
+                         uint32_t *ncols) {
+    if (!h) return ISE_ERR_ARG;
+    if (!matrix || !out) return fail(h, ISE_ERR_ARG, "runCanonizer: null buffer");
+    if (k != h->k) return fail(h, ISE_ERR_ARG, "runCanonizer: group size differs from the canonizer's");
+    uint32_t days = 0, b = 0;
+    bool walk = false;
+    int rc = resolve_days(h, day, &days, &b, &walk);
+    if (rc) return rc;
+    h->out.resize((size_t)h->v * b);
+    rc = ise_cd_run_canonizer_batch(h, 1, matrix, day, h->out.data());
+    if (rc) return rc;
+    *out = h->out.data();
+    if (ncols) *ncols = b;
+    return ISE_OK;
+}
+
+} // extern "C"

@@ -69,6 +69,7 @@ struct ise_ctx {
     // canon_cap matrices (rows, meta, original index), the open count, and
     // result arrays for the host entry point
     uint64_t *d_canon_rows[2] = {nullptr, nullptr};
+    uint64_t *d_canon_cells[2] = {nullptr, nullptr};   // orbit-start masks under a column partition
     uint32_t *d_canon_meta[2] = {nullptr, nullptr}, *d_canon_idx[2] = {nullptr, nullptr};
     uint32_t *d_canon_count = nullptr;
     uint64_t *d_canon_out = nullptr, *d_canon_aut = nullptr;
@@ -446,7 +447,8 @@ int ise_close(ise_ctx *c) {
                     c->d.perm_col, c->d.orbits, c->d.n_gens, c->d.counters, c->d.gens,
                     c->d_kf_tab, c->d_kf_witness, c->d_kf_auts, c->d_kf_tried, c->d_defer, c->d_sched,
                     c->d_kf_list, c->d_kf_best, c->d_kf_part, c->d_kf_ranges,
-                    c->d_canon_rows[0], c->d_canon_rows[1], c->d_canon_meta[0], c->d_canon_meta[1],
+                    c->d_canon_rows[0], c->d_canon_rows[1], c->d_canon_cells[0], c->d_canon_cells[1],
+                    c->d_canon_meta[0], c->d_canon_meta[1],
                     c->d_canon_idx[0], c->d_canon_idx[1], c->d_canon_count, c->d_canon_out, c->d_canon_aut,
                     c->d_canon_status, c->d_canon_iters};
     for (void *p : bufs)
@@ -678,6 +680,7 @@ static int ensure_canon(ise_ctx *c) {
     hipError_t e = hipSuccess;
     for (int i = 0; i < 2 && e == hipSuccess; i++) {
         e = dalloc(&c->d_canon_rows[i], words);
+        if (e == hipSuccess) e = dalloc(&c->d_canon_cells[i], words);
         if (e == hipSuccess) e = dalloc(&c->d_canon_meta[i], n);
         if (e == hipSuccess) e = dalloc(&c->d_canon_idx[i], n);
     }
@@ -694,6 +697,13 @@ static int ensure_canon(ise_ctx *c) {
 int ise_canonize_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rows,
                               const uint32_t *d_meta, uint64_t *d_canon, uint8_t *d_status, uint64_t *d_aut,
                               uint32_t *d_iters, uint32_t max_iters) {
+    return ise_canonize_batch_device_part(c, n, stride, d_rows, d_meta, 0ull, d_canon, d_status, d_aut, d_iters,
+                                          max_iters);
+}
+
+int ise_canonize_batch_device_part(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                                   const uint32_t *d_meta, uint64_t part, uint64_t *d_canon, uint8_t *d_status,
+                                   uint64_t *d_aut, uint32_t *d_iters, uint32_t max_iters) {
     if (!c || !d_canon || !d_status || stride < 1 || stride > c->stride || max_iters < 1) return ISE_ERR_ARG;
     if (n > c->capacity) return ISE_ERR_CAPACITY;
     if (n == 0) return ISE_OK;
@@ -720,6 +730,7 @@ int ise_canonize_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uin
     a.aut = d_aut;
     a.iters = d_iters;
     a.count_next = c->d_canon_count;
+    a.part = part;
     // the test of every round: verdict, witness row permutation and |Aut| (row action)
     ise_results out{};
     out.verdict = c->d.verdict;
@@ -730,13 +741,14 @@ int ise_canonize_batch_device(ise_ctx *c, uint32_t n, uint32_t stride, const uin
         a.rows_next = c->d_canon_rows[cur];
         a.meta_next = c->d_canon_meta[cur];
         a.idx_next = c->d_canon_idx[cur];
+        a.cells_next = part ? c->d_canon_cells[cur] : nullptr;
         HIPCHK(c, ise::canon_step_launch(a, step_blocks, s));
         uint32_t open = 0;
         HIPCHK(c, hipMemcpyAsync(&open, c->d_canon_count, sizeof open, hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));
         if (open == 0) break;
-        rc = launch(c, open, stride, c->d_canon_rows[cur], nullptr, c->d_canon_meta[cur], out, c->d_head, s, 64,
-                    false);   // the reference's witness order: CanonizeMatrix keeps the plain order
+        rc = launch(c, open, stride, c->d_canon_rows[cur], a.cells_next, c->d_canon_meta[cur], out, c->d_head, s,
+                    64, false);   // the reference's witness order: CanonizeMatrix keeps the plain order
         if (rc) return rc;
         a.first = 0;
         a.n_cur = open;

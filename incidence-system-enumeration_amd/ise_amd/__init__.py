@@ -81,6 +81,14 @@ def lib():
         L.ise_check_batch_device_noalloc.argtypes = [P, U32, U32, P, P, P, ctypes.POINTER(ResultsStruct), P]
         L.ise_canonize_batch.argtypes = [P, U32, U32, P, P, P, P, P, P, U32]
         L.ise_canonize_batch_device.argtypes = [P, U32, U32, P, P, P, P, P, P, U32]
+        L.ise_canonize_batch_device_part.argtypes = [P, U32, U32, P, P, U64, P, P, P, P, U32]
+        L.ise_cd_create_canonizer.argtypes = [ctypes.POINTER(P), I, U32, U32]
+        L.ise_cd_release_canonizer.argtypes = [P]
+        L.ise_cd_run_canonizer.argtypes = [P, P, U32, I, ctypes.POINTER(P), ctypes.POINTER(U32)]
+        L.ise_cd_run_canonizer_batch.argtypes = [P, U32, P, I, P]
+        L.ise_cd_run_canonizer_device.argtypes = [P, U32, P, I, P]
+        L.ise_cd_error.argtypes = [P]
+        L.ise_cd_error.restype = ctypes.c_char_p
         L.ise_improved_solution.argtypes = [P, U32, U32, P, P, P, U32, P, ctypes.POINTER(U32)]
         L.ise_canonize_matrix.argtypes = [P, P, U32, U32, U32, P, ctypes.POINTER(U64), ctypes.POINTER(U32)]
         L.ise_device_alloc.argtypes = [P, ctypes.POINTER(P), U64]
@@ -328,6 +336,27 @@ class Context:
             for k in out:
                 out[k][off:off + cnt] = tmp[k]
         return out
+
+    def canonize_batch_part(self, rows, meta, part, max_iters=CANONIZE_MAX_ITERS):
+        """canonize_batch under an initial column partition `part` (bit c set:
+        column c starts a block; CanonizeMatrix with k > 0), through the
+        device entry point: dict of canon, status, aut, iters."""
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32)
+        n, stride = rows.shape
+        if n > self.capacity:
+            raise ValueError("more matrices than the context holds")
+        d_rows = DeviceArray(self, rows.shape, np.uint64, rows)
+        d_meta = DeviceArray(self, meta.shape, np.uint32, meta)
+        d_canon = DeviceArray(self, rows.shape, np.uint64)
+        d_status = DeviceArray(self, n, np.uint8)
+        d_aut = DeviceArray(self, n, np.uint64)
+        d_iters = DeviceArray(self, n, np.uint32)
+        self._check(self._L.ise_canonize_batch_device_part(self.h, n, stride, d_rows.ptr, d_meta.ptr, part,
+                                                           d_canon.ptr, d_status.ptr, d_aut.ptr, d_iters.ptr,
+                                                           max_iters))
+        return {"canon": d_canon.to_host(), "status": d_status.to_host(), "aut": d_aut.to_host(),
+                "iters": d_iters.to_host()}
 
     def canonize_matrix(self, mat, len_stab=0):
         """CanonizeMatrix of one uint8 0/1 matrix: (canonical matrix, |Aut|, tests run)."""
@@ -890,6 +919,76 @@ def kf_relabel_p1f(src_rows, src_meta, count, rmin, seed, stride=16, uniform=Fal
     if rc != OK:
         raise ISEError(rc, "ise_kf_relabel_u1f" if uniform else "ise_kf_relabel_p1f")
     return rows, meta
+
+
+CD_MAX_WALK_DAYS = 10
+
+
+class KSysCanonizer:
+    """The k-Sys binary canonizer: createCanonizer(v, k) / runCanonizer /
+    releaseCanonizer (k-Factorization/Utils/Include/CDTools.h:9-11,
+    Source/CDTools.cpp:5-51) on the GPU.  run(matrix, k, day) takes `day`
+    days (all when day <= 0) of v player numbers, the groups of k of each day
+    in order, and returns the v x b canonical binary matrix (uint8), b =
+    days * v / k; with every day present the day order is canonized too."""
+
+    def __init__(self, v, k, device=0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.ise_cd_create_canonizer(ctypes.byref(h), device, v, k)
+        if rc != OK:
+            raise ISEError(rc, "createCanonizer: invalid v, k or no gfx950 device")
+        self.h, self.v, self.k = h, v, k
+        self.days_max = (v - 1) // (k - 1)
+
+    def close(self):
+        if self.h:
+            self._L.ise_cd_release_canonizer(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise ISEError(rc, self._L.ise_cd_error(self.h).decode(errors="replace"))
+
+    def _days(self, day):
+        return self.days_max if day <= 0 else day
+
+    def run(self, matrix, k=None, day=0):
+        """runCanonizer(matrix, k, day): the canonical v x b player rows."""
+        matrix = np.ascontiguousarray(matrix, dtype=np.uint8)
+        out = ctypes.c_void_p()
+        nc = ctypes.c_uint32(0)
+        self._check(self._L.ise_cd_run_canonizer(self.h, matrix.ctypes.data, self.k if k is None else k, day,
+                                                 ctypes.byref(out), ctypes.byref(nc)))
+        b = nc.value
+        buf = (ctypes.c_uint8 * (self.v * b)).from_address(out.value)
+        return np.frombuffer(buf, np.uint8).reshape(self.v, b).copy()
+
+    def run_batch(self, matrices, day=0):
+        """runCanonizer on every k-system of matrices [n][days][v]: [n][v][b]."""
+        matrices = np.ascontiguousarray(matrices, dtype=np.uint8)
+        n = matrices.shape[0]
+        b = self._days(day) * (self.v // self.k)
+        out = np.zeros((n, self.v, b), np.uint8)
+        self._check(self._L.ise_cd_run_canonizer_batch(self.h, n, matrices.ctypes.data, day, out.ctypes.data))
+        return out
+
+    def run_device(self, n, d_matrices, day, d_canon):
+        """Device-resident batch: d_matrices [n][days][v] bytes, d_canon [n][v]
+        packed rows (pointers of device memory)."""
+        self._check(self._L.ise_cd_run_canonizer_device(self.h, n, d_matrices, day, d_canon))
 
 
 class CanonicityChecker:

@@ -134,10 +134,8 @@ int ise_set_order(ise_ctx *ctx, int mode, uint32_t hunt_budget, const uint8_t *d
  * up to `stride` rows per submit.  Replaces the construction of
  * CGPU_CanonChecker (Source/MatrixCanonChecker.h:204-290) and of the
  * per-thread CCanonicityChecker.  (The k-Sys C entry points createCanonizer /
- * runCanonizer, k-Factorization/Utils/Include/CDTools.h:9-11, canonize a
- * matrix whose first row holds day labels and may exceed 64 columns; they have
- * no counterpart here: ise_canonize_* covers 0/1 matrices, ise_kf_check the
- * k-Sys canonicity test.) */
+ * runCanonizer, k-Factorization/Utils/Include/CDTools.h:9-11, are
+ * ise_cd_create_canonizer / ise_cd_run_canonizer below.) */
 int ise_open(ise_ctx **out, int device, uint32_t capacity, uint32_t stride);
 
 /* Release the context (CGPU_CanonChecker dtor). */
@@ -191,6 +189,17 @@ int ise_test_canonicity(ise_ctx *ctx, const uint8_t *mat, uint32_t nrows, uint32
 int ise_canonize_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rows,
                               const uint32_t *d_meta, uint64_t *d_canon, uint8_t *d_status, uint64_t *d_aut,
                               uint32_t *d_iters, uint32_t max_iters);
+
+/* The same under an initial column partition `part` (bit c set: column c
+ * starts a block; 0 or 1: one block, the call above): CanonizeMatrix with
+ * k > 0, whose first row of labels makes every label's columns a column orbit
+ * (MatrixCanonChecker.cpp:437-449, the k-Sys matrix of runCanonizer); the
+ * label row itself is not part of the rows.  Columns stay inside their block;
+ * the result is the largest form over the row permutations and the column
+ * permutations inside every block. */
+int ise_canonize_batch_device_part(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rows,
+                                   const uint32_t *d_meta, uint64_t part, uint64_t *d_canon, uint8_t *d_status,
+                                   uint64_t *d_aut, uint32_t *d_iters, uint32_t max_iters);
 
 /* The same on host arrays (canon_out [n][stride]; aut_out, iters_out optional). */
 int ise_canonize_batch(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *rows, const uint32_t *meta,
@@ -614,6 +623,52 @@ int ise_kf_set_split(ise_ctx *ctx, uint32_t first_chunks, uint32_t max_ranges);
  * round 0 is (0 1)(2 3)... and put in normal form.  Deterministic per seed. */
 int ise_kf_random(int players, int rmin, int rmax, uint64_t count, uint64_t seed,
                   uint32_t stride, uint64_t *rounds_out, uint32_t *meta_out);
+
+/* ---- k-Sys binary canonizer (createCanonizer / runCanonizer /
+ * releaseCanonizer, k-Factorization/Utils/Include/CDTools.h:9-11,
+ * Source/CDTools.cpp:5-51) ---------------------------------------------------
+ * A k-system is `days` rows of v player bytes, each row the v/k groups of k
+ * players of one day (the k-Sys result matrix).  runCanonizer makes it the
+ * binary matrix of C_InSys::convertToBinaryMatrix (Source/matrix.h:198-210):
+ * player p's row has a 1 in column j * v/k + g when p is in group g of day j,
+ * under a first row of day labels that keeps every day's columns together
+ * (day 0 first).  CanonizeMatrix (MatrixCanonChecker.cpp:420-564) returns the
+ * largest such matrix over the player orders and the column orders inside
+ * each day -- and, when every one of the (v-1)/(k-1) days is present, over
+ * the day orders as well (its walk over the day permutations keeps the
+ * largest matrix and starts again whenever it grows).  The result is the v x b
+ * player rows, b = days * v/k: equal for two k-systems exactly when one is a
+ * relabelling of the other (for full days, with the days reordered too).
+ * Here every day order is canonized on the GPU at once (days! matrices, days
+ * <= ISE_CD_MAX_WALK_DAYS) and the largest kept.  Limits: v <= 32 players,
+ * b <= 64 columns. */
+#define ISE_CD_MAX_WALK_DAYS 10u
+typedef struct ise_cd_canonizer ise_cd_canonizer;
+
+/* createCanonizer(v, k) on HIP device `device`.  ISE_ERR_ARG unless k >= 2,
+ * k divides v, k - 1 divides v - 1 and v <= 32. */
+int ise_cd_create_canonizer(ise_cd_canonizer **out, int device, uint32_t v, uint32_t k);
+
+/* releaseCanonizer. */
+int ise_cd_release_canonizer(ise_cd_canonizer *h);
+
+/* runCanonizer(h, matrix, k, day): `day` days of `matrix` (day <= 0: all of
+ * them); *out receives the canonical v x b player rows (one byte per entry),
+ * valid until the next call on h, *ncols = b.  k must be the canonizer's. */
+int ise_cd_run_canonizer(ise_cd_canonizer *h, const uint8_t *matrix, uint32_t k, int day, const uint8_t **out,
+                         uint32_t *ncols);
+
+/* A batch of n k-systems of `day` days each (matrices [n][days][v]); out
+ * [n][v][b] bytes. */
+int ise_cd_run_canonizer_batch(ise_cd_canonizer *h, uint32_t n, const uint8_t *matrices, int day, uint8_t *out);
+
+/* The same on device memory: d_matrices [n][days][v] bytes, d_canon [n][v]
+ * packed rows (bit c = column c); synchronous. */
+int ise_cd_run_canonizer_device(ise_cd_canonizer *h, uint32_t n, const uint8_t *d_matrices, int day,
+                                uint64_t *d_canon);
+
+/* Last error message of h. */
+const char *ise_cd_error(const ise_cd_canonizer *h);
 
 /* ---- the automorphism group after the test ------------------------------
  * Generators returned by the test (ise_results.gens: column-orbit

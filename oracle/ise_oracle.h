@@ -235,6 +235,21 @@ long orc_kf_quickperm(int k, uint8_t *out);
  * position was set there. */
 int orc_kf_player_index(const uint8_t *mat, int n, int nrounds, int mode, int *info);
 
+/* runCanonizer (Source/CDTools.cpp:25-51): mat = `day` days (day <= 0: all
+ * (v - 1) / (k - 1) of them) of v players, groups of k in order; out receives
+ * the v x b player rows of the canonical binary matrix, *ncols = b.  Every day
+ * order is tried when all days are present.  0 ok, -1 invalid arguments, -2 a
+ * canonization did not end. */
+int orc_cd_run_canonizer(const uint8_t *mat, int v, int k, int day, uint8_t *out, int *ncols);
+/* Column-partition mask of `days` days of `groups` columns (bit j*groups). */
+uint64_t orc_cd_day_starts(int days, int groups);
+/* CanonizeMatrix with an initial column partition: part bit c set = column c
+ * starts a block (the orbits of a first row of labels; part = 1: one block,
+ * orc_canonize with lenStab 0).  Columns never leave their block.  Returns 1
+ * (canonical form in out), 0 (max_iters tests did not reach it), -1. */
+int orc_canonize_part(const uint8_t *mat, int nrows, int ncols, uint64_t part, int max_iters, uint8_t *out,
+                      uint64_t *aut, uint32_t *iters);
+
 #ifdef __cplusplus
 }
 #endif

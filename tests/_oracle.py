@@ -57,6 +57,10 @@ def load():
         L.orc_kf_normalize.argtypes = [_P, ctypes.c_int, ctypes.c_int]
         L.orc_kf_bruteforce.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
         L.orc_kf_player_index.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
+        L.orc_cd_run_canonizer.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
+        L.orc_cd_day_starts.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_cd_day_starts.restype = ctypes.c_uint64
+        L.orc_canonize_part.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _P, _P, _P]
         L.orc_kf_quickperm.argtypes = [ctypes.c_int, _P]
         L.orc_kf_quickperm.restype = ctypes.c_long
         L.orc_mlp_features.argtypes = [U32, U32, _P, _P, U32, _P]
@@ -299,6 +303,35 @@ class Oracle:
         if pi < 0:
             raise ValueError("oracle rejected the factorization")
         return pi, int(info[0]), int(info[1]), bool(info[2])
+
+    def cd_run_canonizer(self, mat, v, k, day=0):
+        """runCanonizer: the v x b player rows of the canonical binary matrix."""
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        days = day if day > 0 else (v - 1) // (k - 1)
+        out = np.zeros((v, 64), np.uint8)
+        nc = np.zeros(1, np.int32)
+        rc = self.L.orc_cd_run_canonizer(mat.ctypes.data, v, k, day, out.ctypes.data, nc.ctypes.data)
+        if rc != 0:
+            raise ValueError(f"oracle runCanonizer failed ({rc})")
+        b = int(nc[0])
+        assert b == days * (v // k)
+        return out.reshape(-1)[:v * b].reshape(v, b).copy()
+
+    def cd_day_starts(self, days, groups):
+        return int(self.L.orc_cd_day_starts(days, groups))
+
+    def canonize_part(self, mat, part, max_iters=1 << 16):
+        """CanonizeMatrix under an initial column partition (bit c: column c
+        starts a block): (canonical matrix, |Aut|, tests run)."""
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        out = np.zeros_like(mat)
+        aut = np.zeros(1, np.uint64)
+        it = np.zeros(1, np.uint32)
+        rc = self.L.orc_canonize_part(mat.ctypes.data, mat.shape[0], mat.shape[1], part, max_iters,
+                                      out.ctypes.data, aut.ctypes.data, it.ctypes.data)
+        if rc != 1:
+            raise ValueError(f"oracle canonize failed ({rc})")
+        return out, int(aut[0]), int(it[0])
 
     def kf_bruteforce(self, mat):
         mat = np.ascontiguousarray(mat, dtype=np.uint8)
