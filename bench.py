@@ -405,6 +405,66 @@ def enumeration_rate(cpu=True):
     return out
 
 
+def cd_canonizer_rate(cpu=True, reps=3):
+    """runCanonizer (ise_cd_*): the binary canonical form of k-systems, on the
+    396 canonical 1-factorizations of K_10 the reference prints
+    (tests/golden/kfact.json).  partial: their first 5 days (one canonization
+    each, day order kept -- the call the k-Sys result check makes for every
+    result); full: 8 of them with all 9 days (9! = 362880 day orders each,
+    the largest form kept).  Host entry points (batch in, canonical bytes
+    out); the oracle's partial forms are compared with the GPU's.  CPU: the
+    oracle on one thread -- partial on the whole batch, full as day orders
+    canonized per second over a timed sample of the first k-system's orders
+    (a whole 9-day walk takes about 40 s), divided by 9!."""
+    import ise_amd
+    with open(os.path.join(ROOT, "tests", "golden", "kfact.json")) as f:
+        kf = json.load(f)
+    mats = np.array([m["rounds"] for m in kf["k10"]["matrices"]], np.uint8)     # [396][9][10]
+    out = {}
+    with ise_amd.KSysCanonizer(10, 2) as cz:
+        part = np.ascontiguousarray(mats[:, :5])
+        cz.run_batch(part, day=5)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            got_p = cz.run_batch(part, day=5)
+        dt_p = (time.perf_counter() - t0) / reps
+        full = np.ascontiguousarray(mats[:8])
+        cz.run_batch(full[:1])
+        t0 = time.perf_counter()
+        got_f = cz.run_batch(full)
+        dt_f = time.perf_counter() - t0
+    out["partial"] = {"workload": "first 5 days of the 396 printed K_10 1-factorizations, day order kept",
+                      "k_systems": len(part), "seconds": dt_p, "k_systems_per_s": len(part) / dt_p}
+    out["full"] = {"workload": "8 printed K_10 1-factorizations, all 9 days (362880 day orders each)",
+                   "k_systems": len(full), "seconds": dt_f, "k_systems_per_s": len(full) / dt_f,
+                   "day_orders_per_s": len(full) * 362880 / dt_f}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle
+        orc = _oracle.load()
+        t0 = time.perf_counter()
+        for i in range(len(part)):
+            if not np.array_equal(orc.cd_run_canonizer(part[i], 10, 2, 5), got_p[i]):
+                raise RuntimeError(f"runCanonizer: GPU and oracle differ on K_10 factorization {i}")
+        dt = time.perf_counter() - t0
+        out["partial"]["cpu_baseline"] = dict(value=len(part) / dt, unit="k-systems/s", cores=1, kind="port",
+                                              sample=f"all {len(part)} k-systems, oracle single thread, "
+                                                     f"{dt:.2f} s; results equal to the GPU's")
+        rng = np.random.default_rng(0xcd)
+        orders = np.array([rng.permutation(9) for _ in range(20000)], np.uint8)
+        t0 = time.perf_counter()
+        sample_best = orc.cd_canonize_orders(full[0], 10, 2, orders)
+        dt = time.perf_counter() - t0
+        if tuple(sample_best.reshape(-1)) > tuple(got_f[0].reshape(-1)):
+            raise RuntimeError("runCanonizer: a day order's form exceeds the GPU's largest")
+        out["full"]["cpu_baseline"] = dict(value=len(orders) / dt / 362880, unit="k-systems/s", cores=1,
+                                           kind="port",
+                                           sample=f"{len(orders)} seeded random day orders of the first "
+                                                  f"k-system canonized by the oracle in {dt:.1f} s (the "
+                                                  f"largest at most the GPU's form), rate / 9!")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -419,8 +479,9 @@ def main():
                     help="use --group-lanes/--waves-per-block instead of the autotuner's choice")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the K_12 and enumeration figures")
+    ap.add_argument("--no-extra", action="store_true", help="skip the K_12, enumeration and runCanonizer figures")
     ap.add_argument("--only-kfact", action="store_true", help="run only the K_12 figure (profiling)")
+    ap.add_argument("--only-cd", action="store_true", help="run only the runCanonizer figure (profiling)")
     ap.add_argument("--build-batch", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run one device-to-device copy of this many bytes (counter calibration)")
@@ -440,6 +501,9 @@ def main():
                               "kfact_p1f_k14": kfact_p1f_rate(kctx, args.steps, args.warmup),
                               "kfact_u1f_k12": kfact_p1f_rate(kctx, args.steps, args.warmup, uniform=True)}),
                   flush=True)
+        return
+    if args.only_cd:
+        print(json.dumps({"cd_canonizer_k10": cd_canonizer_rate(cpu=not args.no_cpu)}), flush=True)
         return
 
     t_setup = time.time()
@@ -646,6 +710,7 @@ def main():
         line["kfact_u1f_k12"] = kfact_p1f_rate(ctx, max(3, args.steps // 4), 1, uniform=True)
         line["streams"] = stream_rates(ctx, max(3, args.steps // 4), 1)
         line["enumeration"] = enumeration_rate(cpu=not args.no_cpu)
+        line["cd_canonizer_k10"] = cd_canonizer_rate(cpu=not args.no_cpu)
         line["host_buffers"] = host_buffer_rate(rows, meta)
     ctx.close()
     if not args.no_cpu:

@@ -27,89 +27,121 @@
 
 namespace ise {
 
-__global__ __launch_bounds__(256) void canon_step_kernel(CanonStep a) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < a.n_cur; k += waves) {
-        const uint32_t orig = a.first ? k : a.idx_cur[k];
-        const uint32_t m = a.first ? a.meta_in[k] : a.meta_cur[k];
-        const uint32_t d = m & 0xffu, b = (m >> 8) & 0xffu, fixed = (m >> 16) & 0xffu;
-        const uint64_t *src = a.first ? a.rows_in + (size_t)k * a.stride_in : a.rows_cur + (size_t)k * a.stride;
-        const bool shape_ok = d >= 1 && d <= ISE_MAX_ROWS && d <= a.stride && d <= a.stride_in && b >= 1 &&
-                              b <= ISE_MAX_COLS && fixed < d;
-        uint32_t status = 0;            // 0: open, else the final status of this matrix
-        if (!shape_ok) {
-            status = ISE_VERDICT_INVALID;
-        } else if (!a.first) {
-            const uint32_t v = a.verdict[k];
-            if (v != ISE_VERDICT_NONCANONICAL) status = v;
-            else if (a.iter >= a.max_iters) status = ISE_VERDICT_LIMIT;
+// Slot reservation: per pass a workgroup takes CANON_TILE consecutive matrices
+// per wave, every wave ballots which of its matrices stay open, and one atomic
+// per workgroup reserves their slots in the next batch (one per matrix would
+// serialise on the counter's cache line); a wave's slots follow the earlier
+// waves' of its workgroup.
+constexpr uint32_t CANON_TILE = 8;
+constexpr uint32_t CANON_WAVES = 4;   // waves per workgroup (256 threads)
+
+// 0 while matrix k stays open, else its final status.
+__device__ inline uint32_t step_status(const CanonStep &a, uint32_t k) {
+    const uint32_t m = a.first ? a.meta_in[k] : a.meta_cur[k];
+    const uint32_t d = m & 0xffu, b = (m >> 8) & 0xffu, fixed = (m >> 16) & 0xffu;
+    const bool shape_ok = d >= 1 && d <= ISE_MAX_ROWS && d <= a.stride && d <= a.stride_in && b >= 1 &&
+                          b <= ISE_MAX_COLS && fixed < d;
+    if (!shape_ok) return ISE_VERDICT_INVALID;
+    if (!a.first) {
+        const uint32_t v = a.verdict[k];
+        if (v != ISE_VERDICT_NONCANONICAL) return v;
+        if (a.iter >= a.max_iters) return ISE_VERDICT_LIMIT;
+    }
+    return 0;
+}
+
+__global__ __launch_bounds__(64 * CANON_WAVES) void canon_step_kernel(CanonStep a) {
+    __shared__ uint32_t wave_open[CANON_WAVES];
+    __shared__ uint32_t group_base;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t per_pass = CANON_WAVES * CANON_TILE;
+    const uint32_t passes = (a.n_cur + per_pass - 1) / per_pass;
+    for (uint32_t p = blockIdx.x; p < passes; p += gridDim.x) {     // the same count for every wave
+        const uint32_t k0 = p * per_pass + wave * CANON_TILE;
+        const uint32_t kend = k0 >= a.n_cur ? k0 : (k0 + CANON_TILE < a.n_cur ? k0 + CANON_TILE : a.n_cur);
+        const bool open_l = lane < kend - k0 && step_status(a, k0 + lane) == 0u;
+        const uint64_t open_mask = __builtin_amdgcn_ballot_w64(open_l);
+        if (lane == 0) wave_open[wave] = (uint32_t)__popcll(open_mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t total = 0;
+            for (uint32_t w = 0; w < CANON_WAVES; w++) total += wave_open[w];
+            group_base = total ? atomicAdd(a.count_next, total) : 0u;
         }
-        if (status) {
-            // decided: the matrix of this round (or nothing, for an invalid record)
-            if (lane < a.stride_out)
-                a.canon[(size_t)orig * a.stride_out + lane] =
-                    status == ISE_VERDICT_INVALID || lane >= d ? 0ull : src[lane];
-            if (lane == 0) {
-                a.status[orig] = (uint8_t)status;
-                if (a.aut) a.aut[orig] = status == ISE_VERDICT_CANONICAL ? a.group_order[k] : 0ull;
-                if (a.iters) a.iters[orig] = a.iter;
+        __syncthreads();
+        uint32_t base = group_base;
+        for (uint32_t w = 0; w < wave; w++) base += wave_open[w];
+        __syncthreads();   // read before the next pass writes them
+        for (uint32_t k = k0; k < kend; k++) {
+            const uint32_t orig = a.first ? k : a.idx_cur[k];
+            const uint32_t m = a.first ? a.meta_in[k] : a.meta_cur[k];
+            const uint32_t d = m & 0xffu, b = (m >> 8) & 0xffu;
+            const uint64_t *src = a.first ? a.rows_in + (size_t)k * a.stride_in : a.rows_cur + (size_t)k * a.stride;
+            const uint32_t status = step_status(a, k);   // 0: open, else the final status of this matrix
+            if (status) {
+                // decided: the matrix of this round (or nothing, for an invalid record)
+                if (lane < a.stride_out)
+                    a.canon[(size_t)orig * a.stride_out + lane] =
+                        status == ISE_VERDICT_INVALID || lane >= d ? 0ull : src[lane];
+                if (lane == 0) {
+                    a.status[orig] = (uint8_t)status;
+                    if (a.aut) a.aut[orig] = status == ISE_VERDICT_CANONICAL ? a.group_order[k] : 0ull;
+                    if (a.iters) a.iters[orig] = a.iter;
+                }
+                continue;
             }
-            continue;
-        }
-        // rows rearranged by the witness: new row i = row perm[i] (identity in the first round)
-        uint64_t row = 0;
-        if (lane < d) {
-            const uint32_t from = a.first ? lane : a.perm_row[(size_t)k * ISE_MAX_ROWS + lane];
-            row = src[from < d ? from : lane];
-        }
-        // column keys: bit (d-1-i) of lane c's key is entry (i, c)
-        uint32_t key = 0;
-        for (uint32_t i = 0; i < d; i++) {
-            // both halves zero-extended (a signed 32-bit lane value would spill into the high word)
-            const uint64_t r = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)i) |
-                               ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row >> 32), (int)i)
-                                << 32);
-            key = (key << 1) | (uint32_t)((r >> lane) & 1ull);
-        }
-        // decreasing key order inside each block, ties by column: lanes past b keep their place
-        const uint64_t bmask = b >= 64 ? ~0ull : ((1ull << b) - 1ull);
-        const uint64_t part = (a.part | 1ull) & bmask;
-        uint32_t pos = lane;
-        if (lane < b) {
-            const uint64_t upto = lane >= 63 ? part : part & ((2ull << lane) - 1ull);
-            const uint32_t mine = 63u - (uint32_t)__builtin_clzll(upto);   // first column of my block
-            pos = mine;
-            uint32_t cs = 0;
-            for (uint32_t c = 0; c < b; c++) {
-                if ((part >> c) & 1ull) cs = c;
-                const uint32_t kc = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)c);
-                pos += cs == mine && ((kc > key) || (kc == key && c < lane));
+            // rows rearranged by the witness: new row i = row perm[i] (identity in the first round)
+            uint64_t row = 0;
+            if (lane < d) {
+                const uint32_t from = a.first ? lane : a.perm_row[(size_t)k * ISE_MAX_ROWS + lane];
+                row = src[from < d ? from : lane];
             }
-        }
-        const uint32_t sorted = (uint32_t)__builtin_amdgcn_ds_permute((int)(pos << 2), (int)key);
-        uint64_t out = 0;
-        for (uint32_t i = 0; i < d; i++) {
-            const uint64_t w = __builtin_amdgcn_ballot_w64(lane < b && ((sorted >> (d - 1 - i)) & 1u));
-            if (lane == i) out = w;
-        }
-        uint32_t slot = 0;
-        if (lane == 0) slot = atomicAdd(a.count_next, 1u);
-        slot = (uint32_t)__builtin_amdgcn_readfirstlane((int)slot);
-        if (lane < a.stride) a.rows_next[(size_t)slot * a.stride + lane] = lane < d ? out : 0ull;
-        if (a.cells_next) {
-            uint64_t cm = part;
-            for (uint32_t j = 0; j < d; j++) {
-                const uint64_t r = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)out, (int)j) |
-                                   ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(out >> 32), (int)j)
+            // column keys: bit (d-1-i) of lane c's key is entry (i, c)
+            uint32_t key = 0;
+            for (uint32_t i = 0; i < d; i++) {
+                // both halves zero-extended (a signed 32-bit lane value would spill into the high word)
+                const uint64_t r = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)i) |
+                                   ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row >> 32), (int)i)
                                     << 32);
-                if (j < lane) cm |= r ^ (r << 1);
+                key = (key << 1) | (uint32_t)((r >> lane) & 1ull);
             }
-            if (lane < a.stride) a.cells_next[(size_t)slot * a.stride + lane] = lane < d ? (cm & bmask) : 0ull;
-        }
-        if (lane == 0) {
-            a.meta_next[slot] = (m & 0x00ffffffu) | (ISE_SAVE_ROW_PERMUTATIONS << 24);
-            a.idx_next[slot] = orig;
+            // decreasing key order inside each block, ties by column: lanes past b keep their place
+            const uint64_t bmask = b >= 64 ? ~0ull : ((1ull << b) - 1ull);
+            const uint64_t part = (a.part | 1ull) & bmask;
+            uint32_t pos = lane;
+            if (lane < b) {
+                const uint64_t upto = lane >= 63 ? part : part & ((2ull << lane) - 1ull);
+                const uint32_t mine = 63u - (uint32_t)__builtin_clzll(upto);   // first column of my block
+                pos = mine;
+                uint32_t cs = 0;
+                for (uint32_t c = 0; c < b; c++) {
+                    if ((part >> c) & 1ull) cs = c;
+                    const uint32_t kc = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)c);
+                    pos += cs == mine && ((kc > key) || (kc == key && c < lane));
+                }
+            }
+            const uint32_t sorted = (uint32_t)__builtin_amdgcn_ds_permute((int)(pos << 2), (int)key);
+            uint64_t out = 0;
+            for (uint32_t i = 0; i < d; i++) {
+                const uint64_t w = __builtin_amdgcn_ballot_w64(lane < b && ((sorted >> (d - 1 - i)) & 1u));
+                if (lane == i) out = w;
+            }
+            const uint32_t slot = base + (uint32_t)__popcll(open_mask & ((1ull << (k - k0)) - 1ull));
+            if (lane < a.stride) a.rows_next[(size_t)slot * a.stride + lane] = lane < d ? out : 0ull;
+            if (a.cells_next) {
+                uint64_t cm = part;
+                for (uint32_t j = 0; j < d; j++) {
+                    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)out, (int)j);
+                    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(out >> 32), (int)j);
+                    const uint64_t r = ((uint64_t)hi << 32) | lo;
+                    if (j < lane) cm |= r ^ (r << 1);
+                }
+                if (lane < a.stride) a.cells_next[(size_t)slot * a.stride + lane] = lane < d ? (cm & bmask) : 0ull;
+            }
+            if (lane == 0) {
+                a.meta_next[slot] = (m & 0x00ffffffu) | (ISE_SAVE_ROW_PERMUTATIONS << 24);
+                a.idx_next[slot] = orig;
+            }
         }
     }
 }
@@ -120,7 +152,7 @@ __global__ void canon_count_reset_kernel(uint32_t *count) {
 
 hipError_t canon_step_launch(const CanonStep &a, uint32_t blocks, hipStream_t stream) {
     hipLaunchKernelGGL(canon_count_reset_kernel, dim3(1), dim3(64), 0, stream, a.count_next);
-    if (a.n_cur) hipLaunchKernelGGL(canon_step_kernel, dim3(blocks), dim3(256), 0, stream, a);
+    if (a.n_cur) hipLaunchKernelGGL(canon_step_kernel, dim3(blocks), dim3(64 * CANON_WAVES), 0, stream, a);
     return hipGetLastError();
 }
 

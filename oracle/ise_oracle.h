@@ -241,6 +241,11 @@ int orc_kf_player_index(const uint8_t *mat, int n, int nrounds, int mode, int *i
  * order is tried when all days are present.  0 ok, -1 invalid arguments, -2 a
  * canonization did not end. */
 int orc_cd_run_canonizer(const uint8_t *mat, int v, int k, int day, uint8_t *out, int *ncols);
+/* The largest canonical form (v x b bytes into best) over the given day
+ * orders (orders [n_orders][days]: position j takes day orders[t][j]): the
+ * inner step of the full-day walk, for timing samples of it. */
+int orc_cd_canonize_orders(const uint8_t *mat, int v, int k, int days, const uint8_t *orders, long n_orders,
+                           uint8_t *best);
 /* Column-partition mask of `days` days of `groups` columns (bit j*groups). */
 uint64_t orc_cd_day_starts(int days, int groups);
 /* CanonizeMatrix with an initial column partition: part bit c set = column c

@@ -145,6 +145,24 @@ uint64_t orc_cd_day_starts(int days, int groups)
     return p;
 }
 
+int orc_cd_canonize_orders(const uint8_t *mat, int v, int k, int days, const uint8_t *orders, long n_orders,
+                           uint8_t *best)
+{
+    if (k < 2 || v < k || v % k || days < 1) return -1;
+    const int G = v / k, b = days * G;
+    if (v > ORC_MAX_ROWS || b > ORC_MAX_COLS) return -1;
+    const uint64_t part = orc_cd_day_starts(days, G);
+    u8 M[ORC_MAX_ROWS * ORC_MAX_COLS], C[ORC_MAX_ROWS * ORC_MAX_COLS];
+    int order[ORC_MAX_COLS];
+    for (long t = 0; t < n_orders; t++) {
+        for (int j = 0; j < days; j++) order[j] = orders[t * days + j];
+        cd_binary(mat, v, k, days, order, M);
+        if (orc_canonize_part(M, v, b, part, 1 << 16, C, NULL, NULL) != 1) return -2;
+        if (!t || memcmp(C, best, (size_t)(v * b)) > 0) memcpy(best, C, (size_t)(v * b));
+    }
+    return 0;
+}
+
 int orc_cd_run_canonizer(const uint8_t *mat, int v, int k, int day, uint8_t *out, int *ncols)
 {
     if (k < 2 || v < k || v % k || (v - 1) % (k - 1)) return -1;

@@ -58,6 +58,7 @@ def load():
         L.orc_kf_bruteforce.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
         L.orc_kf_player_index.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
         L.orc_cd_run_canonizer.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
+        L.orc_cd_canonize_orders.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, ctypes.c_long, _P]
         L.orc_cd_day_starts.argtypes = [ctypes.c_int, ctypes.c_int]
         L.orc_cd_day_starts.restype = ctypes.c_uint64
         L.orc_canonize_part.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _P, _P, _P]
@@ -316,6 +317,18 @@ class Oracle:
         b = int(nc[0])
         assert b == days * (v // k)
         return out.reshape(-1)[:v * b].reshape(v, b).copy()
+
+    def cd_canonize_orders(self, mat, v, k, orders):
+        """Largest canonical form (v x b) over the given day orders [n][days]."""
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        orders = np.ascontiguousarray(orders, dtype=np.uint8)
+        days = orders.shape[1]
+        best = np.zeros((v, days * (v // k)), np.uint8)
+        rc = self.L.orc_cd_canonize_orders(mat.ctypes.data, v, k, days, orders.ctypes.data, len(orders),
+                                           best.ctypes.data)
+        if rc != 0:
+            raise ValueError(f"oracle canonize_orders failed ({rc})")
+        return best
 
     def cd_day_starts(self, days, groups):
         return int(self.L.orc_cd_day_starts(days, groups))

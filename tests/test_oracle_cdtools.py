@@ -206,3 +206,15 @@ def test_rejects_invalid_input(orc):
         orc.cd_run_canonizer(np.full((3, 4), 7, np.uint8), 4, 2)        # player 7 of 4
     with pytest.raises(ValueError):
         orc.cd_run_canonizer(round_robin(12), 12, 2)                      # 66 columns
+
+
+def test_canonize_orders_over_every_order_is_the_walk(orc):
+    """The timing helper: the largest form over all day orders given
+    explicitly equals the full-day run; over one order, that order's form."""
+    mat = kf_fixture("u8")
+    every = np.array(list(itertools.permutations(range(7))), np.uint8)
+    assert np.array_equal(orc.cd_canonize_orders(mat, 8, 2, every), orc.cd_run_canonizer(mat, 8, 2))
+    one = np.array([[3, 1, 4, 0, 6, 5, 2]], np.uint8)
+    identity = np.arange(7, dtype=np.uint8)[None]
+    reordered = np.ascontiguousarray(mat[one[0]])
+    assert np.array_equal(orc.cd_canonize_orders(mat, 8, 2, one), orc.cd_canonize_orders(reordered, 8, 2, identity))
