@@ -646,7 +646,8 @@ int ise_kf_random(int players, int rmin, int rmax, uint64_t count, uint64_t seed
 typedef struct ise_cd_canonizer ise_cd_canonizer;
 
 /* createCanonizer(v, k) on HIP device `device`.  ISE_ERR_ARG unless k >= 2,
- * k divides v, k - 1 divides v - 1 and v <= 32. */
+ * k divides v, k - 1 divides v - 1 and v <= 32.  A canonizer owns its own
+ * context and buffers; like the reference's, it serves one caller at a time. */
 int ise_cd_create_canonizer(ise_cd_canonizer **out, int device, uint32_t v, uint32_t k);
 
 /* releaseCanonizer. */
