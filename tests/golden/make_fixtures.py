@@ -18,7 +18,10 @@ JSON fixtures next to this script:
                      1-factorizations of K_10 and K_14 with |Aut(M)|,
                      .../10/10x9x2/P_OrderedMatrices.txt, .../14/14x13x2/P_OrderedMatrices.txt,
                      and the uniform ones of K_8 (cycles 4:4) and K_10 (4:6),
-                     .../8/8x7x2_44/U_OrderedMatrices.txt, .../10/10x9x2_46/U_OrderedMatrices.txt.
+                     .../8/8x7x2_44/U_OrderedMatrices.txt, .../10/10x9x2_46/U_OrderedMatrices.txt;
+                     k15x3 / p15x3: the 15-player k-systems in groups of three
+                     (7 days of five triples) of .../15/15x7x3/K_OrderedMatrices.txt and
+                     P_OrderedMatrices.txt (rows: the day's triples in order).
 
 Run: python tests/golden/make_fixtures.py  (needs the reference checkout).
 """
@@ -145,7 +148,8 @@ def kfact():
         res["k10"] = dict(n=10, source=os.path.relpath(p10, REF),
                           matrices=[dict(rounds=m, aut=a) for m, a in mats])
     for key, n, rel in (("p10", 10, "10/10x9x2/P_OrderedMatrices.txt"), ("p14", 14, "14/14x13x2/P_OrderedMatrices.txt"),
-                        ("u8", 8, "8/8x7x2_44/U_OrderedMatrices.txt"), ("u10", 10, "10/10x9x2_46/U_OrderedMatrices.txt")):
+                        ("u8", 8, "8/8x7x2_44/U_OrderedMatrices.txt"), ("u10", 10, "10/10x9x2_46/U_OrderedMatrices.txt"),
+                        ("k15x3", 15, "15/15x7x3/K_OrderedMatrices.txt"), ("p15x3", 15, "15/15x7x3/P_OrderedMatrices.txt")):
         path = os.path.join(REF, "k-Factorization/Results/Complete_graphs", rel)
         if os.path.exists(path):
             mats = parse_kfact(read(path), n)

@@ -206,3 +206,28 @@ def test_canonize_part_matches_oracle(oracle, gpu_ctx):
             assert got["status"][i] == 1
             assert np.array_equal(bits, want), (part_kind, i)
             assert int(got["aut"][i]) == aut and int(got["iters"][i]) == iters, (part_kind, i)
+
+
+def test_kirkman_solutions(oracle):
+    """The seven printed 15-player k-systems in groups of three (the Kirkman
+    schoolgirl solutions), all 7 days (5040 day orders each), in one batch:
+    the cached oracle forms, seven different matrices, the same forms from
+    relabelled copies, and the two printed in P_OrderedMatrices.txt land on
+    two of the seven."""
+    fx = json.load(open(os.path.join(HERE, "golden", "cd_canon.json")))
+    want = {c["index"]: np.array([[int(ch) for ch in r] for r in c["canon"]], np.uint8)
+            for c in fx["cases"] if c["fixture"] == "k15x3"}
+    assert sorted(want) == list(range(7))
+    kf = json.load(open(os.path.join(HERE, "golden", "kfact.json")))
+    mats = np.array([m["rounds"] for m in kf["k15x3"]["matrices"]], np.uint8)
+    rng = np.random.default_rng(153)
+    with canonizer(15, 3) as cz:
+        got = cz.run_batch(mats)
+        again = cz.run_batch(np.stack([relabel(m, 15, 3, rng, shuffle_days=True) for m in mats]))
+        p = cz.run_batch(np.array([m["rounds"] for m in kf["p15x3"]["matrices"]], np.uint8))
+    for i in range(7):
+        assert np.array_equal(got[i], want[i]), i
+        assert np.array_equal(again[i], want[i]), i
+    forms = {g.tobytes() for g in got}
+    assert len(forms) == 7
+    assert all(x.tobytes() in forms for x in p)

@@ -218,3 +218,45 @@ def test_canonize_orders_over_every_order_is_the_walk(orc):
     identity = np.arange(7, dtype=np.uint8)[None]
     reordered = np.ascontiguousarray(mat[one[0]])
     assert np.array_equal(orc.cd_canonize_orders(mat, 8, 2, one), orc.cd_canonize_orders(reordered, 8, 2, identity))
+
+
+def _aut_from_day_orders(orc, mat, v, k):
+    """|Aut(M)| of a full k-system from its canonical forms: the automorphisms
+    are a day permutation times a player permutation; the day orders whose
+    form is the largest are the day parts (one per coset of the automorphisms
+    that fix every day), and those are counted by the largest form's |Aut|
+    with the days fixed (the canonical-form loop's row-action order)."""
+    days = mat.shape[0]
+    best = orc.cd_run_canonizer(mat, v, k)
+    reach = sum(np.array_equal(orc.cd_canonize_orders(mat, v, k, np.array([o], np.uint8)), best)
+                for o in itertools.permutations(range(days)))
+    _, fixed_aut, _ = orc.canonize_part(best, orc.cd_day_starts(days, v // k))
+    return reach * fixed_aut, best
+
+
+@pytest.mark.parametrize("name,index", [("u8", 0), ("k15x3", 0), ("k15x3", 4), ("k15x3", 6)])
+def test_full_day_forms_give_the_printed_aut(orc, name, index):
+    """The printed |Aut(M)| of the reference's k-systems (K_8 uniform
+    1-factorization: 1344; Kirkman schoolgirl solutions: 24, 168, 21) equals
+    (day orders reaching the largest form) x (its |Aut| with days fixed)."""
+    m = KF[name]["matrices"][index]
+    mat = np.array(m["rounds"], np.uint8)
+    v = mat.shape[1]
+    k = 2 if name == "u8" else 3
+    aut, _ = _aut_from_day_orders(orc, mat, v, k)
+    assert aut == m["aut"]
+
+
+def test_kirkman_solutions_have_distinct_forms(orc):
+    """The seven printed 15-player k-systems in groups of three are the seven
+    non-isomorphic Kirkman schoolgirl solutions: seven different canonical
+    matrices, each also reached from a relabelled, regrouped, reordered copy."""
+    rng = np.random.default_rng(15)
+    forms = []
+    for m in KF["k15x3"]["matrices"]:
+        mat = np.array(m["rounds"], np.uint8)
+        f = orc.cd_run_canonizer(mat, 15, 3)
+        forms.append(f.tobytes())
+        if len(forms) <= 2:
+            assert np.array_equal(orc.cd_run_canonizer(relabel(mat, 15, 3, rng, shuffle_days=True), 15, 3), f)
+    assert len(set(forms)) == 7
