@@ -412,10 +412,11 @@ def cd_canonizer_rate(cpu=True, reps=3):
     each, day order kept -- the call the k-Sys result check makes for every
     result); full: 8 of them with all 9 days (9! = 362880 day orders each,
     the largest form kept).  Host entry points (batch in, canonical bytes
-    out); the oracle's partial forms are compared with the GPU's.  CPU: the
-    oracle on one thread -- partial on the whole batch, full as day orders
-    canonized per second over a timed sample of the first k-system's orders
-    (a whole 9-day walk takes about 40 s), divided by 9!."""
+    out); the oracle's partial forms are compared with the GPU's, and the full
+    runs' |Aut(M)| (day-group order x fixed-day |Aut|) with the printed
+    values.  CPU: the oracle on one thread -- partial on the whole batch, full
+    as day orders canonized per second over 20000 seeded random orders of the
+    first k-system (a whole 9-day walk takes about 40 s), divided by 9!."""
     import ise_amd
     with open(os.path.join(ROOT, "tests", "golden", "kfact.json")) as f:
         kf = json.load(f)
@@ -431,13 +432,18 @@ def cd_canonizer_rate(cpu=True, reps=3):
         full = np.ascontiguousarray(mats[:8])
         cz.run_batch(full[:1])
         t0 = time.perf_counter()
-        got_f = cz.run_batch(full)
+        got_f, day_ord, fixed_ord = cz.run_batch(full, orders=True)
         dt_f = time.perf_counter() - t0
+    aut = [int(d) * int(f) for d, f in zip(day_ord, fixed_ord)]
+    printed = [int(m["aut"]) for m in kf["k10"]["matrices"][:len(full)]]
+    if aut != printed:
+        raise RuntimeError(f"runCanonizer: |Aut(M)| {aut} differs from the printed {printed}")
     out["partial"] = {"workload": "first 5 days of the 396 printed K_10 1-factorizations, day order kept",
                       "k_systems": len(part), "seconds": dt_p, "k_systems_per_s": len(part) / dt_p}
     out["full"] = {"workload": "8 printed K_10 1-factorizations, all 9 days (362880 day orders each)",
                    "k_systems": len(full), "seconds": dt_f, "k_systems_per_s": len(full) / dt_f,
-                   "day_orders_per_s": len(full) * 362880 / dt_f}
+                   "day_orders_per_s": len(full) * 362880 / dt_f,
+                   "aut_equal_to_printed": True}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import _oracle

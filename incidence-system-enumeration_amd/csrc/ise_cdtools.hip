@@ -21,9 +21,14 @@
 //   canonical-form loop     ise_canonize_batch_device_part, day partition;
 //   cd_segment_best_kernel  one workgroup per run of <= CD_SEG candidates of
 //                           one k-system: the largest canonical matrix (rows
-//                           compared in order, each with column 0 first);
+//                           compared in order, each with column 0 first) and
+//                           how many candidates of the run equal it;
 //   cd_merge_kernel         one thread per k-system: the run winners against
-//                           the k-system's best so far (kept across chunks).
+//                           the k-system's best so far (kept across chunks),
+//                           with the count of day orders that reach it (the
+//                           order of the day-permutation group CanonizeMatrix
+//                           stores in extraGroupOrder) and its |Aut| with the
+//                           days fixed.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -42,7 +47,8 @@ struct ise_cd_canonizer {
     uint64_t *d_rows = nullptr, *d_canon = nullptr;
     uint32_t *d_meta = nullptr;
     uint8_t *d_status = nullptr;
-    uint32_t *d_seg = nullptr;       // [nseg][3]: first candidate (in the chunk), length, winner
+    uint64_t *d_aut = nullptr;       // [chunk] |Aut| of each canonical matrix, days fixed
+    uint32_t *d_seg = nullptr;       // [nseg][4]: first candidate (in the chunk), length, winner, ties
     size_t seg_cap = 0;
     uint32_t *d_mseg = nullptr;      // [nm][3]: k-system, first run, runs
     size_t mseg_cap = 0;
@@ -50,6 +56,9 @@ struct ise_cd_canonizer {
     size_t best_cap = 0;             // k-systems
     uint32_t *d_have = nullptr;      // [n] a best matrix is set
     size_t have_cap = 0;
+    uint64_t *d_orders = nullptr;    // [2][n] day orders reaching the best matrix, its |Aut| days fixed
+    size_t orders_cap = 0;           // k-systems
+    uint64_t last_day_order = 1, last_fixed_order = 0;
     uint32_t *d_flags = nullptr;     // [0] a player id >= v, [1] a canonization did not end
     std::vector<uint64_t> packed;
     std::vector<uint8_t> out;
@@ -61,6 +70,14 @@ namespace {
 constexpr uint32_t CD_CHUNK = 1u << 16;   // candidates per canonical-form batch
 constexpr uint32_t CD_SEG = 1024u;        // candidates per reduction workgroup
 constexpr uint32_t CD_NONE = 0xffffffffu;
+
+constexpr uint32_t SEG_WORDS = 4;
+
+__device__ inline bool cd_equal(const uint64_t *x, const uint64_t *y, uint32_t v) {
+    for (uint32_t i = 0; i < v; i++)
+        if (x[i] != y[i]) return false;
+    return true;
+}
 
 // x > y for two v-row matrices (rows in order, each with column 0 first).
 __device__ inline bool cd_larger(const uint64_t *x, const uint64_t *y, uint32_t v) {
@@ -130,8 +147,10 @@ __global__ __launch_bounds__(256) void cd_segment_best_kernel(const uint64_t *ca
                                                               uint32_t stride, uint32_t v, uint32_t *seg,
                                                               uint32_t *flags) {
     __shared__ uint32_t best[256];
-    const uint32_t s = blockIdx.x, start = seg[3 * s], len = seg[3 * s + 1];
+    __shared__ uint32_t ties;
+    const uint32_t s = blockIdx.x, start = seg[SEG_WORDS * s], len = seg[SEG_WORDS * s + 1];
     uint32_t mine = CD_NONE;
+    if (threadIdx.x == 0) ties = 0;
     for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
         const uint32_t c = start + i;
         if (status[c] != ISE_VERDICT_CANONICAL) {
@@ -151,36 +170,63 @@ __global__ __launch_bounds__(256) void cd_segment_best_kernel(const uint64_t *ca
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) seg[3 * s + 2] = best[0];
+    const uint32_t win = best[0];
+    // candidates of the run equal to the winner (day orders that reach it)
+    uint32_t eq = 0;
+    if (win != CD_NONE)
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+            const uint32_t c = start + i;
+            eq += status[c] == ISE_VERDICT_CANONICAL &&
+                  cd_equal(canon + (uint64_t)c * stride, canon + (uint64_t)win * stride, v);
+        }
+    if (eq) atomicAdd(&ties, eq);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        seg[SEG_WORDS * s + 2] = win;
+        seg[SEG_WORDS * s + 3] = ties;
+    }
 }
 
-__global__ void cd_merge_kernel(const uint64_t *canon, uint32_t stride, uint32_t v, const uint32_t *seg,
-                                const uint32_t *mseg, uint32_t nm, uint64_t *best, uint32_t *have) {
+__global__ void cd_merge_kernel(const uint64_t *canon, const uint64_t *aut, uint32_t stride, uint32_t v,
+                                const uint32_t *seg, const uint32_t *mseg, uint32_t nm, uint64_t *best,
+                                uint32_t *have, uint64_t *day_orders, uint64_t *fixed_orders) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nm) return;
     const uint32_t mat = mseg[3 * e], s0 = mseg[3 * e + 1], ns = mseg[3 * e + 2];
     uint64_t *bp = best + (uint64_t)mat * v;
     bool h = have[mat] != 0u;
+    uint64_t reach = h ? day_orders[mat] : 0ull, fixed = h ? fixed_orders[mat] : 0ull;
     for (uint32_t s = s0; s < s0 + ns; s++) {
-        const uint32_t c = seg[3 * s + 2];
+        const uint32_t c = seg[SEG_WORDS * s + 2];
         if (c == CD_NONE) continue;
         const uint64_t *cp = canon + (uint64_t)c * stride;
         if (!h || cd_larger(cp, bp, v)) {
             for (uint32_t i = 0; i < v; i++) bp[i] = cp[i];
             h = true;
+            reach = seg[SEG_WORDS * s + 3];
+            fixed = aut[c];
+        } else if (cd_equal(cp, bp, v)) {
+            reach += seg[SEG_WORDS * s + 3];
         }
     }
     have[mat] = h ? 1u : 0u;
+    day_orders[mat] = reach;
+    fixed_orders[mat] = fixed;
 }
 
 // Without a walk every candidate is its own k-system's result.
-__global__ void cd_copy_kernel(const uint64_t *canon, const uint8_t *status, uint32_t stride, uint32_t v,
-                               uint64_t m0, uint32_t cnt, uint64_t *best, uint32_t *flags) {
+__global__ void cd_copy_kernel(const uint64_t *canon, const uint8_t *status, const uint64_t *aut, uint32_t stride,
+                               uint32_t v, uint64_t m0, uint32_t cnt, uint64_t *best, uint64_t *day_orders,
+                               uint64_t *fixed_orders, uint32_t *flags) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint64_t)cnt * v) return;
     const uint64_t c = i / v, w = i - c * v;
     best[(m0 + c) * v + w] = canon[c * stride + w];
-    if (w == 0 && status[c] != ISE_VERDICT_CANONICAL) atomicOr(&flags[1], 1u);
+    if (w == 0) {
+        if (status[c] != ISE_VERDICT_CANONICAL) atomicOr(&flags[1], 1u);
+        day_orders[m0 + c] = 1ull;
+        fixed_orders[m0 + c] = aut[c];
+    }
 }
 
 int fail(ise_cd_canonizer *h, int rc, const std::string &msg) {
@@ -258,6 +304,7 @@ int ise_cd_create_canonizer(ise_cd_canonizer **out, int device, uint32_t v, uint
     if (e == hipSuccess) e = hipMalloc((void **)&h->d_canon, words * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&h->d_meta, (size_t)h->chunk * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&h->d_status, h->chunk);
+    if (e == hipSuccess) e = hipMalloc((void **)&h->d_aut, (size_t)h->chunk * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&h->d_flags, 64);
     if (e != hipSuccess) {
         ise_cd_release_canonizer(h);
@@ -271,8 +318,8 @@ int ise_cd_release_canonizer(ise_cd_canonizer *h) {
     if (!h) return ISE_ERR_ARG;
     (void)hipSetDevice(h->device);
     if (h->ctx) (void)hipStreamSynchronize(ise::ctx_stream(h->ctx));
-    void *bufs[] = {h->d_mats, h->d_rows, h->d_canon, h->d_meta, h->d_status,
-                    h->d_seg,  h->d_mseg, h->d_best,  h->d_have, h->d_flags};
+    void *bufs[] = {h->d_mats, h->d_rows, h->d_canon, h->d_meta, h->d_status, h->d_aut,
+                    h->d_seg,  h->d_mseg, h->d_best,  h->d_have, h->d_orders, h->d_flags};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (h->ctx) ise_close(h->ctx);
@@ -284,8 +331,15 @@ const char *ise_cd_error(const ise_cd_canonizer *h) {
     return h ? h->err.c_str() : "null canonizer";
 }
 
+int ise_cd_group_order(const ise_cd_canonizer *h, uint64_t *day_order, uint64_t *fixed_order) {
+    if (!h) return ISE_ERR_ARG;
+    if (day_order) *day_order = h->last_day_order;
+    if (fixed_order) *fixed_order = h->last_fixed_order;
+    return ISE_OK;
+}
+
 int ise_cd_run_canonizer_device(ise_cd_canonizer *h, uint32_t n, const uint8_t *d_matrices, int day,
-                                uint64_t *d_canon) {
+                                uint64_t *d_canon, uint64_t *d_day_orders, uint64_t *d_fixed_orders) {
     if (!h) return ISE_ERR_ARG;
     if (n == 0) return ISE_OK;
     if (!d_matrices || !d_canon) return fail(h, ISE_ERR_ARG, "runCanonizer: null buffer");
@@ -300,6 +354,11 @@ int ise_cd_run_canonizer_device(ise_cd_canonizer *h, uint32_t n, const uint8_t *
     CDCHK(h, hipSetDevice(h->device));
     hipStream_t s = ise::ctx_stream(h->ctx);
     CDCHK(h, grow(&h->d_have, &h->have_cap, (size_t)n));
+    if (!d_day_orders || !d_fixed_orders) {
+        CDCHK(h, grow(&h->d_orders, &h->orders_cap, 2 * (size_t)n));
+        if (!d_day_orders) d_day_orders = h->d_orders;
+        if (!d_fixed_orders) d_fixed_orders = h->d_orders + n;
+    }
     CDCHK(h, hipMemsetAsync(h->d_have, 0, (size_t)n * 4, s));
     CDCHK(h, hipMemsetAsync(h->d_flags, 0, 64, s));
     std::vector<uint32_t> seg, mseg;
@@ -309,12 +368,13 @@ int ise_cd_run_canonizer_device(ise_cd_canonizer *h, uint32_t n, const uint8_t *
                            walk ? 1u : 0u, P, c0, cnt, stride, h->d_rows, h->d_meta, h->d_flags);
         CDCHK(h, hipGetLastError());
         rc = ise_canonize_batch_device_part(h->ctx, cnt, stride, h->d_rows, h->d_meta, part, h->d_canon,
-                                            h->d_status, nullptr, nullptr, ISE_CANONIZE_MAX_ITERS);
+                                            h->d_status, h->d_aut, nullptr, ISE_CANONIZE_MAX_ITERS);
         if (rc) return fail(h, rc, std::string("runCanonizer: ") + ise_last_error(h->ctx));
         if (!walk) {
             const uint64_t items = (uint64_t)cnt * v;
             hipLaunchKernelGGL(cd_copy_kernel, dim3((uint32_t)((items + 255) / 256)), dim3(256), 0, s, h->d_canon,
-                               h->d_status, stride, v, c0, cnt, d_canon, h->d_flags);
+                               h->d_status, h->d_aut, stride, v, c0, cnt, d_canon, d_day_orders, d_fixed_orders,
+                               h->d_flags);
             CDCHK(h, hipGetLastError());
             continue;
         }
@@ -325,16 +385,17 @@ int ise_cd_run_canonizer_device(ise_cd_canonizer *h, uint32_t n, const uint8_t *
         for (uint64_t m = m_first; m <= m_last; m++) {
             const uint64_t lo = std::max<uint64_t>(c0, m * P), hi = std::min<uint64_t>(c0 + cnt, (m + 1) * P);
             mseg.push_back((uint32_t)m);
-            mseg.push_back((uint32_t)(seg.size() / 3));
+            mseg.push_back((uint32_t)(seg.size() / SEG_WORDS));
             uint32_t runs = 0;
             for (uint64_t a = lo; a < hi; a += CD_SEG, runs++) {
                 seg.push_back((uint32_t)(a - c0));
                 seg.push_back((uint32_t)std::min<uint64_t>(CD_SEG, hi - a));
                 seg.push_back(CD_NONE);
+                seg.push_back(0u);
             }
             mseg.push_back(runs);
         }
-        const uint32_t nseg = (uint32_t)(seg.size() / 3), nm = (uint32_t)(mseg.size() / 3);
+        const uint32_t nseg = (uint32_t)(seg.size() / SEG_WORDS), nm = (uint32_t)(mseg.size() / 3);
         CDCHK(h, grow(&h->d_seg, &h->seg_cap, seg.size()));
         CDCHK(h, grow(&h->d_mseg, &h->mseg_cap, mseg.size()));
         CDCHK(h, hipMemcpyAsync(h->d_seg, seg.data(), seg.size() * 4, hipMemcpyHostToDevice, s));
@@ -342,8 +403,8 @@ int ise_cd_run_canonizer_device(ise_cd_canonizer *h, uint32_t n, const uint8_t *
         hipLaunchKernelGGL(cd_segment_best_kernel, dim3(nseg), dim3(256), 0, s, h->d_canon, h->d_status, stride, v,
                            h->d_seg, h->d_flags);
         CDCHK(h, hipGetLastError());
-        hipLaunchKernelGGL(cd_merge_kernel, dim3((nm + 63) / 64), dim3(64), 0, s, h->d_canon, stride, v, h->d_seg,
-                           h->d_mseg, nm, d_canon, h->d_have);
+        hipLaunchKernelGGL(cd_merge_kernel, dim3((nm + 63) / 64), dim3(64), 0, s, h->d_canon, h->d_aut, stride, v,
+                           h->d_seg, h->d_mseg, nm, d_canon, h->d_have, d_day_orders, d_fixed_orders);
         CDCHK(h, hipGetLastError());
         // the host run lists are rewritten next chunk: their copies must be done
         CDCHK(h, hipStreamSynchronize(s));
@@ -356,7 +417,8 @@ int ise_cd_run_canonizer_device(ise_cd_canonizer *h, uint32_t n, const uint8_t *
     return ISE_OK;
 }
 
-int ise_cd_run_canonizer_batch(ise_cd_canonizer *h, uint32_t n, const uint8_t *matrices, int day, uint8_t *out) {
+int ise_cd_run_canonizer_batch(ise_cd_canonizer *h, uint32_t n, const uint8_t *matrices, int day, uint8_t *out,
+                               uint64_t *day_orders, uint64_t *fixed_orders) {
     if (!h) return ISE_ERR_ARG;
     if (n == 0) return ISE_OK;
     if (!matrices || !out) return fail(h, ISE_ERR_ARG, "runCanonizer: null buffer");
@@ -372,11 +434,14 @@ int ise_cd_run_canonizer_batch(ise_cd_canonizer *h, uint32_t n, const uint8_t *m
     hipStream_t s = ise::ctx_stream(h->ctx);
     CDCHK(h, grow(&h->d_mats, &h->mats_cap, in_bytes));
     CDCHK(h, grow(&h->d_best, &h->best_cap, (size_t)n * v));
+    CDCHK(h, grow(&h->d_orders, &h->orders_cap, 2 * (size_t)n));
     CDCHK(h, hipMemcpyAsync(h->d_mats, matrices, in_bytes, hipMemcpyHostToDevice, s));
-    rc = ise_cd_run_canonizer_device(h, n, h->d_mats, day, h->d_best);
+    rc = ise_cd_run_canonizer_device(h, n, h->d_mats, day, h->d_best, h->d_orders, h->d_orders + n);
     if (rc) return rc;
     h->packed.resize((size_t)n * v);
     CDCHK(h, hipMemcpyAsync(h->packed.data(), h->d_best, h->packed.size() * 8, hipMemcpyDeviceToHost, s));
+    if (day_orders) CDCHK(h, hipMemcpyAsync(day_orders, h->d_orders, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    if (fixed_orders) CDCHK(h, hipMemcpyAsync(fixed_orders, h->d_orders + n, (size_t)n * 8, hipMemcpyDeviceToHost, s));
     CDCHK(h, hipStreamSynchronize(s));
     for (uint32_t m = 0; m < n; m++)
         for (uint32_t i = 0; i < v; i++) {
@@ -397,8 +462,11 @@ int ise_cd_run_canonizer(ise_cd_canonizer *h, const uint8_t *matrix, uint32_t k,
     int rc = resolve_days(h, day, &days, &b, &walk);
     if (rc) return rc;
     h->out.resize((size_t)h->v * b);
-    rc = ise_cd_run_canonizer_batch(h, 1, matrix, day, h->out.data());
+    uint64_t dord = 1, ford = 0;
+    rc = ise_cd_run_canonizer_batch(h, 1, matrix, day, h->out.data(), &dord, &ford);
     if (rc) return rc;
+    h->last_day_order = dord;
+    h->last_fixed_order = ford;
     *out = h->out.data();
     if (ncols) *ncols = b;
     return ISE_OK;

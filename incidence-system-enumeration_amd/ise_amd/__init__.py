@@ -85,8 +85,9 @@ def lib():
         L.ise_cd_create_canonizer.argtypes = [ctypes.POINTER(P), I, U32, U32]
         L.ise_cd_release_canonizer.argtypes = [P]
         L.ise_cd_run_canonizer.argtypes = [P, P, U32, I, ctypes.POINTER(P), ctypes.POINTER(U32)]
-        L.ise_cd_run_canonizer_batch.argtypes = [P, U32, P, I, P]
-        L.ise_cd_run_canonizer_device.argtypes = [P, U32, P, I, P]
+        L.ise_cd_run_canonizer_batch.argtypes = [P, U32, P, I, P, P, P]
+        L.ise_cd_run_canonizer_device.argtypes = [P, U32, P, I, P, P, P]
+        L.ise_cd_group_order.argtypes = [P, ctypes.POINTER(U64), ctypes.POINTER(U64)]
         L.ise_cd_error.argtypes = [P]
         L.ise_cd_error.restype = ctypes.c_char_p
         L.ise_improved_solution.argtypes = [P, U32, U32, P, P, P, U32, P, ctypes.POINTER(U32)]
@@ -976,19 +977,33 @@ class KSysCanonizer:
         buf = (ctypes.c_uint8 * (self.v * b)).from_address(out.value)
         return np.frombuffer(buf, np.uint8).reshape(self.v, b).copy()
 
-    def run_batch(self, matrices, day=0):
-        """runCanonizer on every k-system of matrices [n][days][v]: [n][v][b]."""
+    def group_order(self):
+        """(day-permutation group order, |Aut| with the days fixed) of the last
+        run(): their product is |Aut(M)| of a full k-system."""
+        d, f = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        self._check(self._L.ise_cd_group_order(self.h, ctypes.byref(d), ctypes.byref(f)))
+        return d.value, f.value
+
+    def run_batch(self, matrices, day=0, orders=False):
+        """runCanonizer on every k-system of matrices [n][days][v]: [n][v][b]
+        (with orders=True also the day-permutation group orders and the
+        fixed-day |Aut| per k-system)."""
         matrices = np.ascontiguousarray(matrices, dtype=np.uint8)
         n = matrices.shape[0]
         b = self._days(day) * (self.v // self.k)
         out = np.zeros((n, self.v, b), np.uint8)
-        self._check(self._L.ise_cd_run_canonizer_batch(self.h, n, matrices.ctypes.data, day, out.ctypes.data))
-        return out
+        dord = np.zeros(n, np.uint64)
+        ford = np.zeros(n, np.uint64)
+        self._check(self._L.ise_cd_run_canonizer_batch(self.h, n, matrices.ctypes.data, day, out.ctypes.data,
+                                                       dord.ctypes.data, ford.ctypes.data))
+        return (out, dord, ford) if orders else out
 
-    def run_device(self, n, d_matrices, day, d_canon):
+    def run_device(self, n, d_matrices, day, d_canon, d_day_orders=None, d_fixed_orders=None):
         """Device-resident batch: d_matrices [n][days][v] bytes, d_canon [n][v]
-        packed rows (pointers of device memory)."""
-        self._check(self._L.ise_cd_run_canonizer_device(self.h, n, d_matrices, day, d_canon))
+        packed rows, d_day_orders / d_fixed_orders [n] words or None
+        (pointers of device memory)."""
+        self._check(self._L.ise_cd_run_canonizer_device(self.h, n, d_matrices, day, d_canon, d_day_orders,
+                                                        d_fixed_orders))
 
 
 class CanonicityChecker:

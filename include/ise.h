@@ -115,7 +115,8 @@ typedef struct {
  *                     learned scorer (ise_mlp_order).
  * A hunt witness is reported in place of the plain one (verdict, fail_row,
  * rowToChange and the witness permutation refer to it); canonical verdicts,
- * |Aut| and generators always come from the plain search. *  ISE_ORDER_LEAD     (ise_mlp_use_lead) the learned order of the first row:
+ * |Aut| and generators always come from the plain search.
+ *  ISE_ORDER_LEAD     (ise_mlp_use_lead) the learned order of the first row:
  *                     what ise_mlp_check_lead_device does, on every launch of
  *                     the context that reports no column permutation and
  *                     takes no cell masks (those run the plain search, as
@@ -659,14 +660,27 @@ int ise_cd_release_canonizer(ise_cd_canonizer *h);
 int ise_cd_run_canonizer(ise_cd_canonizer *h, const uint8_t *matrix, uint32_t k, int day, const uint8_t **out,
                          uint32_t *ncols);
 
+/* The group orders of the last ise_cd_run_canonizer call: day_order = the
+ * order of the day-permutation group of the result, i.e. the day orders whose
+ * canonical matrix is the result (the class group order CanonizeMatrix keeps
+ * in extraGroupOrder, MatrixCanonChecker.cpp:548-551; 1 when fewer than all
+ * days are given, where no day permutation is tried), fixed_order = |Aut| of
+ * the result with every day fixed (player permutations that, with column
+ * permutations inside the days, keep the matrix).  For a full k-system
+ * day_order x fixed_order = |Aut(M)|.  Either pointer may be NULL. */
+int ise_cd_group_order(const ise_cd_canonizer *h, uint64_t *day_order, uint64_t *fixed_order);
+
 /* A batch of n k-systems of `day` days each (matrices [n][days][v]); out
- * [n][v][b] bytes. */
-int ise_cd_run_canonizer_batch(ise_cd_canonizer *h, uint32_t n, const uint8_t *matrices, int day, uint8_t *out);
+ * [n][v][b] bytes; day_orders / fixed_orders [n] (optional, NULL: not
+ * written) as ise_cd_group_order per k-system. */
+int ise_cd_run_canonizer_batch(ise_cd_canonizer *h, uint32_t n, const uint8_t *matrices, int day, uint8_t *out,
+                               uint64_t *day_orders, uint64_t *fixed_orders);
 
 /* The same on device memory: d_matrices [n][days][v] bytes, d_canon [n][v]
- * packed rows (bit c = column c); synchronous. */
+ * packed rows (bit c = column c), d_day_orders / d_fixed_orders [n] device
+ * words or NULL; synchronous. */
 int ise_cd_run_canonizer_device(ise_cd_canonizer *h, uint32_t n, const uint8_t *d_matrices, int day,
-                                uint64_t *d_canon);
+                                uint64_t *d_canon, uint64_t *d_day_orders, uint64_t *d_fixed_orders);
 
 /* Last error message of h. */
 const char *ise_cd_error(const ise_cd_canonizer *h);

@@ -241,6 +241,11 @@ int orc_kf_player_index(const uint8_t *mat, int n, int nrounds, int mode, int *i
  * order is tried when all days are present.  0 ok, -1 invalid arguments, -2 a
  * canonization did not end. */
 int orc_cd_run_canonizer(const uint8_t *mat, int v, int k, int day, uint8_t *out, int *ncols);
+/* The same with the group orders of the result: *day_order = day orders whose
+ * canonical matrix is the result (1 without a walk), *fixed_order = its |Aut|
+ * with every day fixed. */
+int orc_cd_run_canonizer_ex(const uint8_t *mat, int v, int k, int day, uint8_t *out, int *ncols,
+                            uint64_t *day_order, uint64_t *fixed_order);
 /* The largest canonical form (v x b bytes into best) over the given day
  * orders (orders [n_orders][days]: position j takes day orders[t][j]): the
  * inner step of the full-day walk, for timing samples of it. */

@@ -163,7 +163,8 @@ int orc_cd_canonize_orders(const uint8_t *mat, int v, int k, int days, const uin
     return 0;
 }
 
-int orc_cd_run_canonizer(const uint8_t *mat, int v, int k, int day, uint8_t *out, int *ncols)
+int orc_cd_run_canonizer_ex(const uint8_t *mat, int v, int k, int day, uint8_t *out, int *ncols,
+                            uint64_t *day_order, uint64_t *fixed_order)
 {
     if (k < 2 || v < k || v % k || (v - 1) % (k - 1)) return -1;
     const int days_max = (v - 1) / (k - 1);
@@ -178,16 +179,30 @@ int orc_cd_run_canonizer(const uint8_t *mat, int v, int k, int day, uint8_t *out
     for (int j = 0; j < days; j++) order[j] = j;
     u8 M[ORC_MAX_ROWS * ORC_MAX_COLS], C[ORC_MAX_ROWS * ORC_MAX_COLS], best[ORC_MAX_ROWS * ORC_MAX_COLS];
     int have = 0;
+    uint64_t reach = 0, fixed = 0;
     const int walk = days == days_max;   /* numClasses = days only when every day is present */
     do {
         cd_binary(mat, v, k, days, order, M);
-        if (orc_canonize_part(M, v, b, part, 1 << 16, C, NULL, NULL) != 1) return -2;
-        if (!have || memcmp(C, best, (size_t)(v * b)) > 0) {
+        uint64_t aut = 0;
+        if (orc_canonize_part(M, v, b, part, 1 << 16, C, &aut, NULL) != 1) return -2;
+        const int c = have ? memcmp(C, best, (size_t)(v * b)) : 1;
+        if (c > 0) {
             memcpy(best, C, (size_t)(v * b));
             have = 1;
+            reach = 1;
+            fixed = aut;
+        } else if (c == 0) {
+            reach++;
         }
     } while (walk && next_order(order, days));
     memcpy(out, best, (size_t)(v * b));
     if (ncols) *ncols = b;
+    if (day_order) *day_order = reach;
+    if (fixed_order) *fixed_order = fixed;
     return 0;
+}
+
+int orc_cd_run_canonizer(const uint8_t *mat, int v, int k, int day, uint8_t *out, int *ncols)
+{
+    return orc_cd_run_canonizer_ex(mat, v, k, day, out, ncols, NULL, NULL);
 }

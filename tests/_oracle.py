@@ -58,6 +58,7 @@ def load():
         L.orc_kf_bruteforce.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
         L.orc_kf_player_index.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
         L.orc_cd_run_canonizer.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
+        L.orc_cd_run_canonizer_ex.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P]
         L.orc_cd_canonize_orders.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, ctypes.c_long, _P]
         L.orc_cd_day_starts.argtypes = [ctypes.c_int, ctypes.c_int]
         L.orc_cd_day_starts.restype = ctypes.c_uint64
@@ -317,6 +318,21 @@ class Oracle:
         b = int(nc[0])
         assert b == days * (v // k)
         return out.reshape(-1)[:v * b].reshape(v, b).copy()
+
+    def cd_run_canonizer_orders(self, mat, v, k, day=0):
+        """runCanonizer with the group orders: (v x b matrix, day orders that
+        reach it, its |Aut| with the days fixed)."""
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        out = np.zeros((v, 64), np.uint8)
+        nc = np.zeros(1, np.int32)
+        dord = np.zeros(1, np.uint64)
+        ford = np.zeros(1, np.uint64)
+        rc = self.L.orc_cd_run_canonizer_ex(mat.ctypes.data, v, k, day, out.ctypes.data, nc.ctypes.data,
+                                            dord.ctypes.data, ford.ctypes.data)
+        if rc != 0:
+            raise ValueError(f"oracle runCanonizer failed ({rc})")
+        b = int(nc[0])
+        return out.reshape(-1)[:v * b].reshape(v, b).copy(), int(dord[0]), int(ford[0])
 
     def cd_canonize_orders(self, mat, v, k, orders):
         """Largest canonical form (v x b) over the given day orders [n][days]."""

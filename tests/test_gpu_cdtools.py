@@ -231,3 +231,36 @@ def test_kirkman_solutions(oracle):
     forms = {g.tobytes() for g in got}
     assert len(forms) == 7
     assert all(x.tobytes() in forms for x in p)
+
+
+def test_group_orders_give_the_printed_aut(oracle):
+    """Day-permutation group order x fixed-day |Aut| of the GPU's result equals
+    the |Aut(M)| the reference prints, for the full k-systems of the fixtures
+    (24 of the 396 K_10 factorizations, the uniform and perfect K_10 ones, the
+    uniform K_8 one, the Kirkman solutions); the two factors equal the
+    oracle's walk on K_8 and two Kirkman solutions; partial days give one day
+    order and the oracle's fixed-day |Aut|; run() leaves them for
+    group_order()."""
+    kf = json.load(open(os.path.join(HERE, "golden", "kfact.json")))
+    cases = [("k10", list(range(0, 396, 17))), ("u10", [0]), ("p10", [0]), ("u8", [0]),
+             ("k15x3", list(range(7))), ("p15x3", [0, 1])]
+    for name, idx in cases:
+        k = 3 if name.endswith("x3") else 2
+        mats = np.array([kf[name]["matrices"][i]["rounds"] for i in idx], np.uint8)
+        v = mats.shape[2]
+        print(f"{name}: {len(idx)} k-systems", flush=True)
+        with canonizer(v, k) as cz:
+            _, dord, ford = cz.run_batch(mats, orders=True)
+            for j, i in enumerate(idx):
+                assert int(dord[j]) * int(ford[j]) == kf[name]["matrices"][i]["aut"], (name, i)
+            if name in ("u8", "k15x3"):
+                for j in range(min(2, len(idx))):
+                    _, od, of = oracle.cd_run_canonizer_orders(mats[j], v, k)
+                    assert (int(dord[j]), int(ford[j])) == (od, of), (name, j)
+                cz.run(mats[0])
+                assert cz.group_order() == (int(dord[0]), int(ford[0]))
+                part = np.ascontiguousarray(mats[:, :3])
+                _, pd, pf = cz.run_batch(part, day=3, orders=True)
+                for j in range(len(part)):
+                    _, od, of = oracle.cd_run_canonizer_orders(part[j], v, k, 3)
+                    assert od == 1 and (int(pd[j]), int(pf[j])) == (od, of), (name, j)

@@ -234,29 +234,51 @@ def _aut_from_day_orders(orc, mat, v, k):
     return reach * fixed_aut, best
 
 
-@pytest.mark.parametrize("name,index", [("u8", 0), ("k15x3", 0), ("k15x3", 4), ("k15x3", 6)])
+@pytest.mark.parametrize("name,index", [("u8", 0), ("k15x3", 6)])
 def test_full_day_forms_give_the_printed_aut(orc, name, index):
     """The printed |Aut(M)| of the reference's k-systems (K_8 uniform
-    1-factorization: 1344; Kirkman schoolgirl solutions: 24, 168, 21) equals
-    (day orders reaching the largest form) x (its |Aut| with days fixed)."""
+    1-factorization: 1344; a Kirkman schoolgirl solution: 21) equals (day
+    orders reaching the largest form, counted order by order) x (its |Aut|
+    with days fixed)."""
     m = KF[name]["matrices"][index]
     mat = np.array(m["rounds"], np.uint8)
     v = mat.shape[1]
     k = 2 if name == "u8" else 3
-    aut, _ = _aut_from_day_orders(orc, mat, v, k)
+    aut, best = _aut_from_day_orders(orc, mat, v, k)
     assert aut == m["aut"]
+    # the walk's own count gives the same orders
+    form, day_order, fixed_order = orc.cd_run_canonizer_orders(mat, v, k)
+    assert np.array_equal(form, best) and day_order * fixed_order == m["aut"]
 
 
-def test_kirkman_solutions_have_distinct_forms(orc):
+def test_kirkman_solutions(orc):
     """The seven printed 15-player k-systems in groups of three are the seven
     non-isomorphic Kirkman schoolgirl solutions: seven different canonical
-    matrices, each also reached from a relabelled, regrouped, reordered copy."""
-    rng = np.random.default_rng(15)
+    matrices; the walk's day-order count x fixed-day |Aut| is each printed
+    |Aut(M)| (24, 12, 24, 12, 168, 168, 21); a relabelled, regrouped,
+    reordered copy reaches the same matrix."""
     forms = []
     for m in KF["k15x3"]["matrices"]:
         mat = np.array(m["rounds"], np.uint8)
-        f = orc.cd_run_canonizer(mat, 15, 3)
-        forms.append(f.tobytes())
-        if len(forms) <= 2:
-            assert np.array_equal(orc.cd_run_canonizer(relabel(mat, 15, 3, rng, shuffle_days=True), 15, 3), f)
-    assert len(set(forms)) == 7
+        f, day_order, fixed_order = orc.cd_run_canonizer_orders(mat, 15, 3)
+        assert day_order * fixed_order == m["aut"]
+        forms.append(f)
+    assert len({f.tobytes() for f in forms}) == 7
+    rng = np.random.default_rng(15)
+    mat = np.array(KF["k15x3"]["matrices"][1]["rounds"], np.uint8)
+    assert np.array_equal(orc.cd_run_canonizer(relabel(mat, 15, 3, rng, shuffle_days=True), 15, 3), forms[1])
+
+
+@pytest.mark.parametrize("name", ["p15x3"])
+def test_walk_group_orders_give_the_printed_aut(orc, name):
+    """Every printed full k-system of the fixture: the walk's day-order count
+    times the fixed-day |Aut| is the printed |Aut(M)|; partial days report
+    one day order (the printed K_10 ones are checked on the GPU)."""
+    k = 3 if name.endswith("x3") else 2
+    for m in KF[name]["matrices"]:
+        mat = np.array(m["rounds"], np.uint8)
+        v = mat.shape[1]
+        _, day_order, fixed_order = orc.cd_run_canonizer_orders(mat, v, k)
+        assert day_order * fixed_order == m["aut"], (name, day_order, fixed_order)
+    _, day_order, _ = orc.cd_run_canonizer_orders(np.ascontiguousarray(mat[:3]), v, k, 3)
+    assert day_order == 1
