@@ -139,7 +139,18 @@ __global__ __launch_bounds__(256) void cd_expand_kernel(const uint8_t *mats, uin
         const uint64_t bw = __builtin_amdgcn_ballot_w64(((col >> r) & 1u) != 0u);
         if (lane == r) word = bw;
     }
-    if (lane < stride) rows[(uint64_t)w * stride + lane] = lane < v ? word : 0ull;
+    // rows in decreasing order, column 0 most significant (the reference sorts
+    // the rows before its loop, sortRowsUpdateColumnOrbits,
+    // MatrixCanonChecker.cpp:394-418): the same canonical form, fewer rounds
+    const uint64_t key = __builtin_bitreverse64(word);
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < v; j++) {
+        const uint64_t kj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(key >> 32), (int)j) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, (int)j);
+        rank += kj > key || (kj == key && j < lane);
+    }
+    if (lane < v) rows[(uint64_t)w * stride + rank] = word;
+    else if (lane < stride) rows[(uint64_t)w * stride + lane] = 0ull;
     if (lane == 0) meta[w] = v | (b << 8);
 }
 
