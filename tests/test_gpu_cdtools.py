@@ -177,14 +177,16 @@ def test_rejects_invalid_input():
 
 def test_canonize_part_matches_oracle(oracle, gpu_ctx):
     """The canonical-form loop under a column partition, field by field:
-    random 0/1 matrices and random partitions (one block = k = 0)."""
+    random 0/1 matrices and random partitions (one block = k = 0), and a
+    wider case past the column-set search's packed width (57-64 columns: the
+    replayed search under the partition)."""
     rng = np.random.default_rng(77)
     n, stride = 300, gpu_ctx.stride
-    for part_kind in range(3):
+    for part_kind in range(4):
         rows = np.zeros((n, stride), np.uint64)
         meta = np.zeros(n, np.uint32)
         mats, parts = [], []
-        b = int(rng.integers(8, 40))
+        b = int(rng.integers(8, 40)) if part_kind < 3 else int(rng.integers(57, 65))
         if part_kind == 0:
             part = 1
         elif part_kind == 1:
