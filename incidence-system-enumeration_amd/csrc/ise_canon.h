@@ -17,10 +17,16 @@ namespace ise {
 // part != 0: an initial column partition (bit c set: column c starts a block,
 // the orbits of a first row of labels); columns stay inside their block and
 // cells_next receives every row's orbit-start mask for the next test.
+// n_cur_dev (later rounds, may be null): the device word holding the number
+// of matrices of the current batch, written by the round before; n_cur is
+// then only an upper bound, and the slots of the next batch from its count up
+// to n_cur get an empty meta word (depth 0), so that a test launched over
+// n_cur slots marks them invalid at once and nothing reads them.
 struct CanonStep {
     uint32_t first, n_cur, stride, stride_in, stride_out, iter, max_iters;
     uint64_t part;
     uint64_t *cells_next;
+    const uint32_t *n_cur_dev;
     const uint64_t *rows_in;
     const uint32_t *meta_in;
     const uint64_t *rows_cur;

@@ -54,11 +54,12 @@ __global__ __launch_bounds__(64 * CANON_WAVES) void canon_step_kernel(CanonStep 
     __shared__ uint32_t wave_open[CANON_WAVES];
     __shared__ uint32_t group_base;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t n_cur = a.n_cur_dev ? *a.n_cur_dev : a.n_cur;   // every thread reads the same word
     const uint32_t per_pass = CANON_WAVES * CANON_TILE;
-    const uint32_t passes = (a.n_cur + per_pass - 1) / per_pass;
+    const uint32_t passes = (n_cur + per_pass - 1) / per_pass;
     for (uint32_t p = blockIdx.x; p < passes; p += gridDim.x) {     // the same count for every wave
         const uint32_t k0 = p * per_pass + wave * CANON_TILE;
-        const uint32_t kend = k0 >= a.n_cur ? k0 : (k0 + CANON_TILE < a.n_cur ? k0 + CANON_TILE : a.n_cur);
+        const uint32_t kend = k0 >= n_cur ? k0 : (k0 + CANON_TILE < n_cur ? k0 + CANON_TILE : n_cur);
         const bool open_l = lane < kend - k0 && step_status(a, k0 + lane) == 0u;
         const uint64_t open_mask = __builtin_amdgcn_ballot_w64(open_l);
         if (lane == 0) wave_open[wave] = (uint32_t)__popcll(open_mask);
@@ -150,9 +151,18 @@ __global__ void canon_count_reset_kernel(uint32_t *count) {
     if (threadIdx.x == 0) *count = 0u;
 }
 
+// Slots [*count, bound) of the next batch: empty meta word (depth 0).
+__global__ void canon_tail_kernel(uint32_t *meta_next, const uint32_t *count, uint32_t bound) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < bound && i >= *count) meta_next[i] = 0u;
+}
+
 hipError_t canon_step_launch(const CanonStep &a, uint32_t blocks, hipStream_t stream) {
     hipLaunchKernelGGL(canon_count_reset_kernel, dim3(1), dim3(64), 0, stream, a.count_next);
     if (a.n_cur) hipLaunchKernelGGL(canon_step_kernel, dim3(blocks), dim3(64 * CANON_WAVES), 0, stream, a);
+    if (a.n_cur_dev && a.n_cur)
+        hipLaunchKernelGGL(canon_tail_kernel, dim3((a.n_cur + 255) / 256), dim3(256), 0, stream, a.meta_next,
+                           a.count_next, a.n_cur);
     return hipGetLastError();
 }
 

@@ -180,9 +180,11 @@ int grid_blocks(ise_ctx *c, uint32_t n, uint32_t stride, int group_lanes, const 
 
 // Enqueue queue reset + kernel; `blocks` from grid_blocks (no runtime queries
 // here, so the sequence can be captured into a graph).
+// use_sched = false: candidates in index order, no deepest-first schedule
+// (it changes no result; batches of one depth gain nothing from it).
 int launch_with(ise_ctx *c, uint32_t blocks, uint32_t n, uint32_t stride, const uint64_t *rows,
                 const uint64_t *cells, const uint32_t *meta, const ise_results &out, uint32_t *head,
-                hipStream_t s, int group_lanes, bool allow_lead = true) {
+                hipStream_t s, int group_lanes, bool allow_lead = true, bool use_sched = true) {
     const uint32_t wpb = c->cfg.waves_per_block ? c->cfg.waves_per_block : 4;
     if (c->order_mode == ISE_ORDER_LEAD && allow_lead && !cells && !out.perm_col) {
         // the learned order of the first row: one launch, deepest-first schedule
@@ -195,7 +197,8 @@ int launch_with(ise_ctx *c, uint32_t blocks, uint32_t n, uint32_t stride, const 
     const uint32_t budget = c->order_mode == ISE_ORDER_PLAIN ? 0u : c->hunt_budget;
     const uint8_t *order = c->order_mode == ISE_ORDER_LEARNED ? c->d_order : nullptr;
     uint32_t *defer = (c->defer_perms[0] && n <= c->defer_cap) ? c->d_defer : nullptr;
-    uint32_t *sched = (c->schedule == ISE_SCHEDULE_DEEPEST_FIRST && n <= c->sched_cap) ? c->d_sched : nullptr;
+    uint32_t *sched =
+        (use_sched && c->schedule == ISE_SCHEDULE_DEEPEST_FIRST && n <= c->sched_cap) ? c->d_sched : nullptr;
     HIPCHK(c, ise::launch_check(group_lanes, blocks, wpb, n, stride, rows, cells, meta, out, head,
                                 c->cfg.max_perms ? c->cfg.max_perms : (1u << 26), budget, order,
                                 c->order_dim, defer, c->defer_perms, sched, s));
@@ -254,11 +257,11 @@ int ensure_kf_split(ise_ctx *c, size_t n) {
 
 int launch(ise_ctx *c, uint32_t n, uint32_t stride, const uint64_t *rows, const uint64_t *cells,
            const uint32_t *meta, const ise_results &out, uint32_t *head, hipStream_t s,
-           int group_lanes, bool allow_lead = true) {
+           int group_lanes, bool allow_lead = true, bool use_sched = true) {
     uint32_t blocks = 0;
     int rc = grid_blocks(c, n, stride, group_lanes, out, &blocks);
     if (rc) return rc;
-    return launch_with(c, blocks, n, stride, rows, cells, meta, out, head, s, group_lanes, allow_lead);
+    return launch_with(c, blocks, n, stride, rows, cells, meta, out, head, s, group_lanes, allow_lead, use_sched);
 }
 
 } // namespace
@@ -729,29 +732,47 @@ int ise_canonize_batch_device_part(ise_ctx *c, uint32_t n, uint32_t stride, cons
     a.status = d_status;
     a.aut = d_aut;
     a.iters = d_iters;
-    a.count_next = c->d_canon_count;
     a.part = part;
+    a.n_cur_dev = nullptr;
     // the test of every round: verdict, witness row permutation and |Aut| (row action)
     ise_results out{};
     out.verdict = c->d.verdict;
     out.perm_row = c->d.perm_row;
     out.group_order = c->d.group_order;
+    // The host reads the number of open matrices after the first round and
+    // every CANON_SYNC_ROUNDS rounds after that; in between each round's step
+    // reads the exact count on the device and the test runs over the last
+    // count read, an upper bound (open matrices only leave), whose slots past
+    // the true count hold an empty meta word.  The two batches' counts sit on
+    // their own cache lines.
+    constexpr int CANON_SYNC_ROUNDS = 4;
+    uint32_t bound = n;
+    int since_sync = 0;
     int cur = 0;
     for (;;) {
         a.rows_next = c->d_canon_rows[cur];
         a.meta_next = c->d_canon_meta[cur];
         a.idx_next = c->d_canon_idx[cur];
         a.cells_next = part ? c->d_canon_cells[cur] : nullptr;
+        a.count_next = c->d_canon_count + 32 * cur;
+        a.n_cur = bound;
         HIPCHK(c, ise::canon_step_launch(a, step_blocks, s));
-        uint32_t open = 0;
-        HIPCHK(c, hipMemcpyAsync(&open, c->d_canon_count, sizeof open, hipMemcpyDeviceToHost, s));
-        HIPCHK(c, hipStreamSynchronize(s));
-        if (open == 0) break;
-        rc = launch(c, open, stride, c->d_canon_rows[cur], a.cells_next, c->d_canon_meta[cur], out, c->d_head, s,
-                    64, false);   // the reference's witness order: CanonizeMatrix keeps the plain order
+        if (a.first || ++since_sync >= CANON_SYNC_ROUNDS) {
+            uint32_t open = 0;
+            HIPCHK(c, hipMemcpyAsync(&open, a.count_next, sizeof open, hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
+            since_sync = 0;
+            bound = open;
+            if (open == 0) break;
+        }
+        // the reference's witness order (CanonizeMatrix keeps the plain order); index
+        // order, no deepest-first schedule: the order changes no result, and the
+        // batches of the k-Sys canonizer hold one depth
+        rc = launch(c, bound, stride, c->d_canon_rows[cur], a.cells_next, c->d_canon_meta[cur], out, c->d_head, s,
+                    64, false, false);
         if (rc) return rc;
         a.first = 0;
-        a.n_cur = open;
+        a.n_cur_dev = a.count_next;
         a.rows_cur = c->d_canon_rows[cur];
         a.meta_cur = c->d_canon_meta[cur];
         a.idx_cur = c->d_canon_idx[cur];
