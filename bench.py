@@ -429,6 +429,11 @@ def cd_canonizer_rate(cpu=True, reps=3):
         for _ in range(reps):
             got_p = cz.run_batch(part, day=5)
         dt_p = (time.perf_counter() - t0) / reps
+        # one call per k-system, as the reference calls runCanonizer: launch latency
+        t0 = time.perf_counter()
+        for i in range(32):
+            cz.run(part[i], day=5)
+        dt_single = (time.perf_counter() - t0) / 32
         full = np.ascontiguousarray(mats[:8])
         cz.run_batch(full[:1])
         t0 = time.perf_counter()
@@ -439,7 +444,8 @@ def cd_canonizer_rate(cpu=True, reps=3):
     if aut != printed:
         raise RuntimeError(f"runCanonizer: |Aut(M)| {aut} differs from the printed {printed}")
     out["partial"] = {"workload": "first 5 days of the 396 printed K_10 1-factorizations, day order kept",
-                      "k_systems": len(part), "seconds": dt_p, "k_systems_per_s": len(part) / dt_p}
+                      "k_systems": len(part), "seconds": dt_p, "k_systems_per_s": len(part) / dt_p,
+                      "single_call_ms": dt_single * 1e3}
     out["full"] = {"workload": "8 printed K_10 1-factorizations, all 9 days (362880 day orders each)",
                    "k_systems": len(full), "seconds": dt_f, "k_systems_per_s": len(full) / dt_f,
                    "day_orders_per_s": len(full) * 362880 / dt_f,

@@ -35,6 +35,56 @@ namespace ise {
 constexpr uint32_t CANON_TILE = 8;
 constexpr uint32_t CANON_WAVES = 4;   // waves per workgroup (256 threads)
 
+// Lane i holds row i of a d x b matrix (i < d): the same matrix with its
+// columns in decreasing order inside each block of `part` (bit c set: column
+// c starts a block; part has bit 0 and no bit past b), row 0 most
+// significant, ties by column -- the orbit form.  Lane c ranks column c's key
+// among its block, ds_permute moves every key to its place, d ballots
+// rebuild the rows.
+__device__ inline uint64_t orbit_sort(uint32_t lane, uint32_t d, uint32_t b, uint64_t part, uint64_t row) {
+    // column keys: bit (d-1-i) of lane c's key is entry (i, c)
+    uint32_t key = 0;
+    for (uint32_t i = 0; i < d; i++) {
+        // both halves zero-extended (a signed 32-bit lane value would spill into the high word)
+        const uint64_t r = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)i) |
+                           ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row >> 32), (int)i) << 32);
+        key = (key << 1) | (uint32_t)((r >> lane) & 1ull);
+    }
+    // decreasing key order inside each block, ties by column: lanes past b keep their place
+    uint32_t pos = lane;
+    if (lane < b) {
+        const uint64_t upto = lane >= 63 ? part : part & ((2ull << lane) - 1ull);
+        const uint32_t mine = 63u - (uint32_t)__builtin_clzll(upto);   // first column of my block
+        pos = mine;
+        uint32_t cs = 0;
+        for (uint32_t c = 0; c < b; c++) {
+            if ((part >> c) & 1ull) cs = c;
+            const uint32_t kc = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)c);
+            pos += cs == mine && ((kc > key) || (kc == key && c < lane));
+        }
+    }
+    const uint32_t sorted = (uint32_t)__builtin_amdgcn_ds_permute((int)(pos << 2), (int)key);
+    uint64_t out = 0;
+    for (uint32_t i = 0; i < d; i++) {
+        const uint64_t w = __builtin_amdgcn_ballot_w64(lane < b && ((sorted >> (d - 1 - i)) & 1u));
+        if (lane == i) out = w;
+    }
+    return out;
+}
+
+// Lane i's orbit-start mask (the column orbits row i is compared under): the
+// block starts plus every place an earlier row changes value.
+__device__ inline uint64_t orbit_cells(uint32_t lane, uint32_t d, uint64_t bmask, uint64_t part, uint64_t row) {
+    uint64_t cm = part;
+    for (uint32_t j = 0; j < d; j++) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)j);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row >> 32), (int)j);
+        const uint64_t r = ((uint64_t)hi << 32) | lo;
+        if (j < lane) cm |= r ^ (r << 1);
+    }
+    return cm & bmask;
+}
+
 // 0 while matrix k stays open, else its final status.
 __device__ inline uint32_t step_status(const CanonStep &a, uint32_t k) {
     const uint32_t m = a.first ? a.meta_in[k] : a.meta_cur[k];
@@ -97,47 +147,14 @@ __global__ __launch_bounds__(64 * CANON_WAVES) void canon_step_kernel(CanonStep 
                 const uint32_t from = a.first ? lane : a.perm_row[(size_t)k * ISE_MAX_ROWS + lane];
                 row = src[from < d ? from : lane];
             }
-            // column keys: bit (d-1-i) of lane c's key is entry (i, c)
-            uint32_t key = 0;
-            for (uint32_t i = 0; i < d; i++) {
-                // both halves zero-extended (a signed 32-bit lane value would spill into the high word)
-                const uint64_t r = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)i) |
-                                   ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row >> 32), (int)i)
-                                    << 32);
-                key = (key << 1) | (uint32_t)((r >> lane) & 1ull);
-            }
-            // decreasing key order inside each block, ties by column: lanes past b keep their place
             const uint64_t bmask = b >= 64 ? ~0ull : ((1ull << b) - 1ull);
             const uint64_t part = (a.part | 1ull) & bmask;
-            uint32_t pos = lane;
-            if (lane < b) {
-                const uint64_t upto = lane >= 63 ? part : part & ((2ull << lane) - 1ull);
-                const uint32_t mine = 63u - (uint32_t)__builtin_clzll(upto);   // first column of my block
-                pos = mine;
-                uint32_t cs = 0;
-                for (uint32_t c = 0; c < b; c++) {
-                    if ((part >> c) & 1ull) cs = c;
-                    const uint32_t kc = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)c);
-                    pos += cs == mine && ((kc > key) || (kc == key && c < lane));
-                }
-            }
-            const uint32_t sorted = (uint32_t)__builtin_amdgcn_ds_permute((int)(pos << 2), (int)key);
-            uint64_t out = 0;
-            for (uint32_t i = 0; i < d; i++) {
-                const uint64_t w = __builtin_amdgcn_ballot_w64(lane < b && ((sorted >> (d - 1 - i)) & 1u));
-                if (lane == i) out = w;
-            }
+            const uint64_t out = orbit_sort(lane, d, b, part, row);
             const uint32_t slot = base + (uint32_t)__popcll(open_mask & ((1ull << (k - k0)) - 1ull));
             if (lane < a.stride) a.rows_next[(size_t)slot * a.stride + lane] = lane < d ? out : 0ull;
             if (a.cells_next) {
-                uint64_t cm = part;
-                for (uint32_t j = 0; j < d; j++) {
-                    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)out, (int)j);
-                    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(out >> 32), (int)j);
-                    const uint64_t r = ((uint64_t)hi << 32) | lo;
-                    if (j < lane) cm |= r ^ (r << 1);
-                }
-                if (lane < a.stride) a.cells_next[(size_t)slot * a.stride + lane] = lane < d ? (cm & bmask) : 0ull;
+                const uint64_t cm = orbit_cells(lane, d, bmask, part, out);   // every lane takes part
+                if (lane < a.stride) a.cells_next[(size_t)slot * a.stride + lane] = lane < d ? cm : 0ull;
             }
             if (lane == 0) {
                 a.meta_next[slot] = (m & 0x00ffffffu) | (ISE_SAVE_ROW_PERMUTATIONS << 24);
