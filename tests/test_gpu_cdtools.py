@@ -266,3 +266,25 @@ def test_group_orders_give_the_printed_aut(oracle):
                 for j in range(len(part)):
                     _, od, of = oracle.cd_run_canonizer_orders(part[j], v, k, 3)
                     assert od == 1 and (int(pd[j]), int(pf[j])) == (od, of), (name, j)
+
+
+def test_all_printed_k10_factorizations():
+    """All 396 printed 1-factorizations of K_10 (the complete list of their
+    isomorphism classes), all 9 days: 396 different canonical matrices, and
+    day-group order x fixed-day |Aut| equal to every printed |Aut(M)|."""
+    kf = json.load(open(os.path.join(HERE, "golden", "kfact.json")))
+    ms = kf["k10"]["matrices"]
+    mats = np.array([m["rounds"] for m in ms], np.uint8)
+    print(f"k10: {len(ms)} k-systems, 9! day orders each", flush=True)
+    with canonizer(10, 2) as cz:
+        forms, dord, ford = [], [], []
+        for i in range(0, len(mats), 33):       # a line of progress per part
+            f, d, o = cz.run_batch(mats[i:i + 33], orders=True)
+            forms.append(f)
+            dord.append(d)
+            ford.append(o)
+            print(f"k10: {i + len(f)} done", flush=True)
+    forms = np.concatenate(forms)
+    aut = np.concatenate(dord).astype(np.uint64) * np.concatenate(ford).astype(np.uint64)
+    assert len({f.tobytes() for f in forms}) == len(ms)
+    assert [int(a) for a in aut] == [int(m["aut"]) for m in ms]
