@@ -288,3 +288,16 @@ def test_all_printed_k10_factorizations():
     aut = np.concatenate(dord).astype(np.uint64) * np.concatenate(ford).astype(np.uint64)
     assert len({f.tobytes() for f in forms}) == len(ms)
     assert [int(a) for a in aut] == [int(m["aut"]) for m in ms]
+
+
+@pytest.mark.parametrize("v,k,day,mat", [(2, 2, 0, [[0, 1]]), (3, 3, 0, [[2, 0, 1]]), (4, 2, 1, [[0, 3, 1, 2]]),
+                                         (6, 2, 2, [[5, 0, 1, 4, 2, 3], [5, 1, 2, 0, 3, 4]])])
+def test_smallest_k_systems(oracle, v, k, day, mat):
+    """One group a day (v = k: one day, one column), single days, and a
+    two-day prefix of K_6: forms and both group orders against the oracle."""
+    m = np.array(mat, np.uint8)
+    want, d_order, f_order = oracle.cd_run_canonizer_orders(m, v, k, day)
+    with canonizer(v, k) as cz:
+        got = cz.run(m, day=day)
+        assert np.array_equal(got, want)
+        assert cz.group_order() == (d_order, f_order)
