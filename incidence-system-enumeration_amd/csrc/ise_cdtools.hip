@@ -67,7 +67,7 @@ struct ise_cd_canonizer {
 
 namespace {
 
-constexpr uint32_t CD_CHUNK = 1u << 16;   // candidates per canonical-form batch
+constexpr uint32_t CD_CHUNK = 1u << 20;   // candidates per canonical-form batch
 constexpr uint32_t CD_SEG = 1024u;        // candidates per reduction workgroup
 constexpr uint32_t CD_NONE = 0xffffffffu;
 
