@@ -435,7 +435,7 @@ def cd_canonizer_rate(cpu=True, reps=3):
             cz.run(part[i], day=5)
         dt_single = (time.perf_counter() - t0) / 32
         full = np.ascontiguousarray(mats[:8])
-        cz.run_batch(full[:1])
+        cz.run_batch(full[:3])          # warm-up at the full batch size (2^20 day orders)
         t0 = time.perf_counter()
         got_f, day_ord, fixed_ord = cz.run_batch(full, orders=True)
         dt_f = time.perf_counter() - t0

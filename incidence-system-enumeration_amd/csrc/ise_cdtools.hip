@@ -67,7 +67,8 @@ struct ise_cd_canonizer {
 
 namespace {
 
-constexpr uint32_t CD_CHUNK = 1u << 20;   // candidates per canonical-form batch
+constexpr uint32_t CD_CHUNK = 1u << 20;   // candidates per canonical-form batch, at most
+constexpr uint32_t CD_CHUNK_MIN = 1u << 12;   // ... and at first: the batch grows with the runs
 constexpr uint32_t CD_SEG = 1024u;        // candidates per reduction workgroup
 constexpr uint32_t CD_NONE = 0xffffffffu;
 
@@ -289,6 +290,47 @@ int resolve_days(ise_cd_canonizer *h, int day, uint32_t *days, uint32_t *b, bool
     return ISE_OK;
 }
 
+// The canonical-form batch size: a context for `chunk` candidates and the
+// staging buffers of that many.  The new ones are made first, so a failure
+// leaves the canonizer as it was.
+int set_chunk(ise_cd_canonizer *h, uint32_t chunk) {
+    ise_ctx *ctx = nullptr;
+    int rc = ise_open(&ctx, h->device, chunk, h->v);
+    if (rc) return fail(h, rc, "runCanonizer: cannot open a context for the batch");
+    const size_t words = (size_t)chunk * h->v;
+    uint64_t *rows = nullptr, *canon = nullptr, *aut = nullptr;
+    uint32_t *meta = nullptr;
+    uint8_t *status = nullptr;
+    hipError_t e = hipSetDevice(h->device);
+    if (e == hipSuccess) e = hipMalloc((void **)&rows, words * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&canon, words * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&meta, (size_t)chunk * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&status, chunk);
+    if (e == hipSuccess) e = hipMalloc((void **)&aut, (size_t)chunk * 8);
+    if (e != hipSuccess) {
+        void *made[] = {rows, canon, meta, status, aut};
+        for (void *p : made)
+            if (p) (void)hipFree(p);
+        ise_close(ctx);
+        return hip_fail(h, e, "runCanonizer: batch buffers");
+    }
+    if (h->ctx) {
+        (void)hipStreamSynchronize(ise::ctx_stream(h->ctx));
+        void *old[] = {h->d_rows, h->d_canon, h->d_meta, h->d_status, h->d_aut};
+        for (void *p : old)
+            if (p) (void)hipFree(p);
+        ise_close(h->ctx);
+    }
+    h->ctx = ctx;
+    h->chunk = chunk;
+    h->d_rows = rows;
+    h->d_canon = canon;
+    h->d_meta = meta;
+    h->d_status = status;
+    h->d_aut = aut;
+    return ISE_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -303,21 +345,12 @@ int ise_cd_create_canonizer(ise_cd_canonizer **out, int device, uint32_t v, uint
     h->k = k;
     h->days_max = (v - 1) / (k - 1);
     h->groups = v / k;
-    h->chunk = CD_CHUNK;
-    int rc = ise_open(&h->ctx, device, h->chunk, v);
+    int rc = set_chunk(h, CD_CHUNK_MIN);   // a small batch first (a missing device fails here)
     if (rc) {
         delete h;
         return rc;
     }
-    hipError_t e = hipSetDevice(device);
-    const size_t words = (size_t)h->chunk * v;
-    if (e == hipSuccess) e = hipMalloc((void **)&h->d_rows, words * 8);
-    if (e == hipSuccess) e = hipMalloc((void **)&h->d_canon, words * 8);
-    if (e == hipSuccess) e = hipMalloc((void **)&h->d_meta, (size_t)h->chunk * 4);
-    if (e == hipSuccess) e = hipMalloc((void **)&h->d_status, h->chunk);
-    if (e == hipSuccess) e = hipMalloc((void **)&h->d_aut, (size_t)h->chunk * 8);
-    if (e == hipSuccess) e = hipMalloc((void **)&h->d_flags, 64);
-    if (e != hipSuccess) {
+    if (hipMalloc((void **)&h->d_flags, 64) != hipSuccess) {
         ise_cd_release_canonizer(h);
         return ISE_ERR_HIP;
     }
@@ -363,6 +396,10 @@ int ise_cd_run_canonizer_device(ise_cd_canonizer *h, uint32_t n, const uint8_t *
     uint64_t part = 0;
     for (uint32_t j = 0; j < days; j++) part |= 1ull << (j * h->groups);
     CDCHK(h, hipSetDevice(h->device));
+    // the batch: the whole run if it fits in CD_CHUNK, in powers of two
+    uint32_t want = CD_CHUNK_MIN;
+    while (want < CD_CHUNK && want < total) want <<= 1;
+    if (want > h->chunk && (rc = set_chunk(h, want)) != ISE_OK) return rc;
     hipStream_t s = ise::ctx_stream(h->ctx);
     CDCHK(h, grow(&h->d_have, &h->have_cap, (size_t)n));
     if (!d_day_orders || !d_fixed_orders) {
@@ -449,6 +486,7 @@ int ise_cd_run_canonizer_batch(ise_cd_canonizer *h, uint32_t n, const uint8_t *m
     CDCHK(h, hipMemcpyAsync(h->d_mats, matrices, in_bytes, hipMemcpyHostToDevice, s));
     rc = ise_cd_run_canonizer_device(h, n, h->d_mats, day, h->d_best, h->d_orders, h->d_orders + n);
     if (rc) return rc;
+    s = ise::ctx_stream(h->ctx);   // a larger batch brings a new context (and stream)
     h->packed.resize((size_t)n * v);
     CDCHK(h, hipMemcpyAsync(h->packed.data(), h->d_best, h->packed.size() * 8, hipMemcpyDeviceToHost, s));
     if (day_orders) CDCHK(h, hipMemcpyAsync(day_orders, h->d_orders, (size_t)n * 8, hipMemcpyDeviceToHost, s));
