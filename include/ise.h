@@ -186,7 +186,8 @@ int ise_test_canonicity(ise_ctx *ctx, const uint8_t *mat, uint32_t nrows, uint32
 /* Device-resident batch (every pointer device memory; d_aut and d_iters may be
  * NULL), n up to the context capacity and stride up to the context's;
  * synchronous on the context stream (it reads the number of open matrices
- * after every round).  Uses context buffers (allocated on the first call). */
+ * every few rounds and returns when none is left).  Uses context buffers
+ * (allocated on the first call). */
 int ise_canonize_batch_device(ise_ctx *ctx, uint32_t n, uint32_t stride, const uint64_t *d_rows,
                               const uint32_t *d_meta, uint64_t *d_canon, uint8_t *d_status, uint64_t *d_aut,
                               uint32_t *d_iters, uint32_t max_iters);
