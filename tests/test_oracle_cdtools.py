@@ -282,3 +282,29 @@ def test_walk_group_orders_give_the_printed_aut(orc, name):
         assert day_order * fixed_order == m["aut"], (name, day_order, fixed_order)
     _, day_order, _ = orc.cd_run_canonizer_orders(np.ascontiguousarray(mat[:3]), v, k, 3)
     assert day_order == 1
+
+
+def test_partitioned_canonical_form_is_the_brute_force_maximum(orc):
+    """orc_canonize_part on random small matrices and random partitions: the
+    largest matrix over every row order, with the columns of each block in
+    decreasing order (row 0 most significant) -- which is the largest over
+    the column orders inside the blocks for that row order."""
+    rng = np.random.default_rng(31)
+    for _ in range(40):
+        d, b = int(rng.integers(2, 7)), int(rng.integers(2, 11))
+        mat = (rng.random((d, b)) < rng.uniform(0.25, 0.6)).astype(np.uint8)
+        starts = [0] + sorted(int(x) for x in rng.choice(np.arange(1, b), int(rng.integers(0, b)), replace=False))
+        part = sum(1 << s for s in starts)
+        bounds = starts + [b]
+        best = None
+        for perm in itertools.permutations(range(d)):
+            m = mat[list(perm)]
+            cols = []
+            for lo, hi in zip(bounds[:-1], bounds[1:]):
+                block = [tuple(m[:, q]) for q in range(lo, hi)]
+                cols.extend(sorted(block, reverse=True))
+            cand = tuple(tuple(c[i] for c in cols) for i in range(d))
+            if best is None or cand > best:
+                best = cand
+        got, _, _ = orc.canonize_part(mat, part)
+        assert [tuple(r) for r in got] == list(best), (mat.tolist(), starts)
